@@ -1,0 +1,190 @@
+// Internal definitions shared by the HIP translation units of libhipibs.
+//
+// Table layout in HBM: n_buckets buckets of `S` 64-bit slots (S = 8: one
+// 64 B bucket; S = 16: 128 B), 256 B aligned, EMPTY = all ones. A key
+// lives in bucket home(key) or, when that bucket was full at insert time,
+// in one of the next `max_disp` buckets (linear probing at bucket
+// granularity, wrapping). A probe therefore stops at the first of: key
+// found, an EMPTY slot in the bucket, or max_disp+1 buckets visited.
+// Canonical codes are never all ones (k < 32: < 2^62; k = 32:
+// min(x, ~x) < 2^63), so EMPTY can never collide with a key.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "hipibs.h"
+
+namespace hipibs
+{
+
+constexpr uint64_t EMPTY = ~0ull;
+
+// Bucket of a key: a multiply-xorshift mix, then the high 32 bits scaled
+// onto [0, n_buckets) (n_buckets < 2^32).
+__host__ __device__ inline uint64_t bucketOf(uint64_t key, uint64_t nBuckets)
+{
+    uint64_t h = (key ^ (key >> 31)) * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 29;
+    h *= 0xBF58476D1CE4E5B9ull;
+    return ((h >> 32) * nBuckets) >> 32;
+}
+
+// Device view of one table, passed by value to kernels.
+struct TableView
+{
+    const uint64_t *slots;
+    uint64_t nBuckets;
+    uint32_t maxDisp; // largest bucket displacement of any stored key
+    uint32_t pad;
+};
+
+// Run-length summary of a contiguous range of positions of one window
+// against one table (the monoid of the scan's reduction; see combine()).
+// With nHit == 0 the range is one run of misses: lead == trail == nValid.
+struct SegSummary
+{
+    uint32_t nValid;
+    uint32_t nHit;
+    uint32_t lead;  // valid misses before the first hit
+    uint32_t trail; // valid misses after the last hit
+    uint32_t innerVar;
+    uint32_t innerDist;
+};
+
+// Window::addVariationOriginal's distance term for one gap (gap >= 1):
+// d = gap - (k-1); if d <= 0 then d = |d + 1| (src/Window.hpp:29-36).
+__host__ __device__ inline int32_t gapDistance(int32_t gap, int32_t k)
+{
+    int32_t d = gap - (k - 1);
+    if (d <= 0)
+    {
+        d = d + 1;
+        d = d < 0 ? -d : d;
+    }
+    return d;
+}
+
+__host__ __device__ inline SegSummary emptySummary()
+{
+    return SegSummary{0, 0, 0, 0, 0, 0};
+}
+
+// Associative combine of the summaries of two adjacent ranges A then B.
+__host__ __device__ inline SegSummary combine(const SegSummary &A, const SegSummary &B, int32_t k)
+{
+    SegSummary r;
+    r.nValid = A.nValid + B.nValid;
+    r.nHit = A.nHit + B.nHit;
+    if (A.nHit == 0 && B.nHit == 0)
+    {
+        r.lead = r.trail = r.nValid;
+        r.innerVar = r.innerDist = 0;
+    }
+    else if (A.nHit == 0)
+    {
+        r.lead = A.nValid + B.lead;
+        r.trail = B.trail;
+        r.innerVar = B.innerVar;
+        r.innerDist = B.innerDist;
+    }
+    else if (B.nHit == 0)
+    {
+        r.lead = A.lead;
+        r.trail = A.trail + B.nValid;
+        r.innerVar = A.innerVar;
+        r.innerDist = A.innerDist;
+    }
+    else
+    {
+        const uint32_t g = A.trail + B.lead;
+        r.lead = A.lead;
+        r.trail = B.trail;
+        r.innerVar = A.innerVar + B.innerVar + (g > 0 ? 1u : 0u);
+        r.innerDist = A.innerDist + B.innerDist + (g > 0 ? (uint32_t)gapDistance((int32_t)g, k) : 0u);
+    }
+    return r;
+}
+
+// Window statistics of a whole window's summary: every maximal miss run,
+// whether closed by a hit or by the window end, adds one variation and its
+// distance term (src/KmerDatabase.hpp:596-672).
+__host__ __device__ inline hipibs_window_stats finalizeSummary(const SegSummary &S, int32_t k)
+{
+    hipibs_window_stats w;
+    w.total_kmers = (int32_t)S.nValid;
+    w.observed_kmers = (int32_t)S.nHit;
+    if (S.nHit == 0)
+    {
+        w.variations = S.nValid > 0 ? 1 : 0;
+        w.kmer_distance = S.nValid > 0 ? gapDistance((int32_t)S.nValid, k) : 0;
+        return w;
+    }
+    int32_t var = (int32_t)S.innerVar, dist = (int32_t)S.innerDist;
+    if (S.lead > 0)
+    {
+        var += 1;
+        dist += gapDistance((int32_t)S.lead, k);
+    }
+    if (S.trail > 0)
+    {
+        var += 1;
+        dist += gapDistance((int32_t)S.trail, k);
+    }
+    w.variations = var;
+    w.kmer_distance = dist;
+    return w;
+}
+
+// ---- error reporting (thread-local last error) ----------------------------
+int setError(int code, const std::string &msg);
+int hipFail(hipError_t e, const char *what);
+
+#define HIPIBS_CHECK(call)                                   \
+    do                                                       \
+    {                                                        \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess)                                \
+            return ::hipibs::hipFail(e_, #call);             \
+    } while (0)
+
+} // namespace hipibs
+
+// Host handle of a device table.
+struct hipibs_table
+{
+    int device = 0;
+    uint32_t k = 0;
+    uint32_t bucketSlots = 8;
+    uint64_t nBuckets = 0;
+    uint64_t nKeys = 0;
+    uint64_t bytes = 0;
+    uint32_t maxDisp = 0;
+    uint64_t *dSlots = nullptr;
+    uint64_t *dCounters = nullptr; // [0] inserted, [1] full flag, [2] max displacement
+    hipibs::TableView view() const
+    {
+        return hipibs::TableView{dSlots, nBuckets, maxDisp, 0};
+    }
+};
+
+namespace hipibs
+{
+// Launch helpers implemented in the .hip files (all asynchronous).
+int launchTableInsert(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStream_t s);
+int launchProbe(const hipibs_table *t, const uint64_t *dCodes, const uint8_t *dValid,
+                uint64_t n, uint8_t *dHits, uint64_t *dCount, hipStream_t s);
+int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *dSeq,
+               const uint64_t *dExc, uint64_t nBases, const hipibs_window *dWindows,
+               uint32_t nWindows, uint32_t maxWindowBases, hipibs_window_stats *dStats,
+               uint32_t segsPerWindow, void *dWork, uint64_t workBytes, hipStream_t s,
+               TableView *dViews);
+int uploadViews(const hipibs_table *const *tables, int nTables, hipStream_t s, TableView **out);
+int checkTables(const hipibs_table *const *tables, int nTables);
+uint32_t autoSegsPerWindow(uint32_t nWindows, int nTables, uint32_t maxPositions);
+uint64_t scanWorkBytes(uint32_t nWindows, int nTables, uint32_t segsPerWindow);
+int kmcReadKeys(const std::string &prefix, uint64_t *keys, uint64_t cap, uint64_t *nOut,
+                unsigned threads, hipibs_kmc_info *infoOut);
+} // namespace hipibs

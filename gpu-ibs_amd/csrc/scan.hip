@@ -1,0 +1,578 @@
+// The fused scan: encode + canonicalise + probe + per-window run-length
+// reduction, plus the mapper's hit-mask and coverage kernels.
+//
+// Work decomposition. A block (4 waves) owns one segment of one window
+// against one table (grid.y = table). The segment's positions are walked
+// in 64-position words aligned to the packed stream (word m = positions
+// 64m .. 64m+63); each wave owns a contiguous quarter of the words. For a
+// word, the wave reads 3 sequence words and 2 exception words with scalar
+// loads (every lane needs bases from the same <= 96-base span), each lane
+// cuts its k-mer out of them with a funnel shift, forms the reverse
+// complement as the complement of the low-first packing and the forward
+// code by reversing the 2-bit groups, and the 64 canonical codes are probed
+// cooperatively (wave_probe.hpp). The hit and valid masks are ballots, so
+// the run-length statistics of the word follow from bit arithmetic: each
+// hit lane measures the miss run that it closes (misses between the
+// previous hit of the word and itself, counted in valid positions only, so
+// a run continues across N bases exactly as the reference's compacted code
+// stream does: src/KmerCodec.hpp:38-61, src/KmerDatabase.hpp:596-672); the
+// run that crosses the word seam is carried in a wave-uniform counter. The
+// four wave summaries are folded in LDS with the associative combine of
+// hipibs_internal.hpp, and windows split over several blocks are folded by
+// a second, tiny kernel.
+#include "wave_probe.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace hipibs
+{
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_WAVES = SCAN_BLOCK / 64;
+
+// Reverses the order of the 32 2-bit groups of x.
+__device__ inline uint64_t reverse2(uint64_t x)
+{
+    const uint64_t v = __builtin_bitreverse64(x);
+    return ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+}
+
+__device__ inline uint32_t waveSumU32(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        v += (uint32_t)__shfl_xor((int)v, o, 64);
+    return v;
+}
+
+// Positions [g0, g1) of window w for segment s of segs: the window's
+// positions split into segs 64-aligned pieces.
+__device__ inline void segmentRange(const hipibs_window &win, uint64_t nBases, uint32_t k,
+                                    uint32_t s, uint32_t segs, uint64_t &g0, uint64_t &g1)
+{
+    uint64_t npos = win.n_bases >= k ? (uint64_t)win.n_bases - k + 1 : 0;
+    // defensive clamp to the stream (the host checks windows it can see)
+    if (win.base + (uint64_t)win.n_bases > nBases)
+        npos = (nBases >= win.base + k) ? nBases - win.base - k + 1 : 0;
+    const uint64_t per = ((npos + segs - 1) / segs + 63) & ~63ull;
+    g0 = win.base + std::min<uint64_t>(npos, (uint64_t)s * per);
+    g1 = win.base + std::min<uint64_t>(npos, (uint64_t)(s + 1) * per);
+}
+
+template <int S>
+__global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
+    const TableView *__restrict__ tables, const uint64_t *__restrict__ seq,
+    const uint64_t *__restrict__ exc, uint64_t nBases, const hipibs_window *__restrict__ windows,
+    uint32_t nWindows, uint32_t segs, uint32_t k, hipibs_window_stats *__restrict__ stats,
+    SegSummary *__restrict__ work)
+{
+    __shared__ SegSummary waveSum[SCAN_WAVES];
+    const uint32_t unit = blockIdx.x;
+    const uint32_t w = unit / segs, s = unit - w * segs;
+    const uint32_t t = blockIdx.y;
+    const TableView T = tables[t];
+    const hipibs_window win = windows[w];
+    uint64_t g0, g1;
+    segmentRange(win, nBases, k, s, segs, g0, g1);
+
+    const int lane = laneId();
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint64_t kmask1 = (1ull << k) - 1ull;
+    const uint64_t kmask2 = k == 32 ? ~0ull : (1ull << (2 * k)) - 1ull;
+    const uint32_t fwdShift = 64 - 2 * k;
+    const uint64_t lanesBelow = (1ull << lane) - 1ull;
+
+    uint32_t nV = 0, nH = 0, lead = 0, T0 = 0, uVar = 0, uDist = 0;
+    uint32_t lVar = 0, lDist = 0;
+    bool seen = false;
+    if (g0 < g1)
+    {
+        const uint64_t mFirst = g0 >> 6, mLast = (g1 - 1) >> 6;
+        const uint64_t nWords = mLast - mFirst + 1;
+        const uint64_t per = (nWords + SCAN_WAVES - 1) / SCAN_WAVES;
+        const uint64_t mBeg = mFirst + std::min<uint64_t>(nWords, (uint64_t)wv * per);
+        const uint64_t mEnd = mFirst + std::min<uint64_t>(nWords, (uint64_t)(wv + 1) * per);
+        const int j = lane & 31;
+        for (uint64_t m = mBeg; m < mEnd; ++m)
+        {
+            const uint64_t s0 = seq[2 * m], s1 = seq[2 * m + 1], s2 = seq[2 * m + 2];
+            const uint64_t e0 = exc[m], e1 = exc[m + 1];
+            const uint64_t a = lane < 32 ? s0 : s1;
+            const uint64_t b = lane < 32 ? s1 : s2;
+            uint64_t x = j ? ((a >> (2 * j)) | (b << (64 - 2 * j))) : a;
+            x &= kmask2;
+            const uint64_t ev = lane ? ((e0 >> lane) | (e1 << (64 - lane))) : e0;
+            const uint64_t p = (m << 6) + (uint64_t)lane;
+            const bool valid = ((ev & kmask1) == 0) && p >= g0 && p < g1;
+            const uint64_t rc = ~x & kmask2;
+            const uint64_t fwd = reverse2(x) >> fwdShift;
+            const uint64_t canon = fwd < rc ? fwd : rc;
+
+            const uint64_t H = waveProbe<S>(T, canon, valid);
+            const uint64_t V = ballot(valid);
+            const uint64_t Miss = V & ~H;
+            if (H)
+            {
+                const int f = __builtin_ctzll(H);
+                const int l = 63 - __builtin_clzll(H);
+                const uint32_t firstGap = T0 + (uint32_t)__popcll(Miss & ((1ull << f) - 1ull));
+                if (seen)
+                {
+                    if (firstGap)
+                    {
+                        uVar += 1;
+                        uDist += (uint32_t)gapDistance((int32_t)firstGap, (int32_t)k);
+                    }
+                }
+                else
+                {
+                    lead = firstGap;
+                    seen = true;
+                }
+                T0 = (uint32_t)__popcll(Miss >> l);
+                // runs closed inside the word, one per hit lane with an
+                // earlier hit in the word
+                if ((H >> lane) & 1ull)
+                {
+                    const uint64_t below = H & lanesBelow;
+                    if (below)
+                    {
+                        const int prev = 63 - __builtin_clzll(below);
+                        const uint32_t gap =
+                            (uint32_t)__popcll(Miss & lanesBelow & ~((2ull << prev) - 1ull));
+                        if (gap)
+                        {
+                            lVar += 1;
+                            lDist += (uint32_t)gapDistance((int32_t)gap, (int32_t)k);
+                        }
+                    }
+                }
+            }
+            else
+                T0 += (uint32_t)__popcll(Miss);
+            nV += (uint32_t)__popcll(V);
+            nH += (uint32_t)__popcll(H);
+        }
+    }
+    lVar = waveSumU32(lVar);
+    lDist = waveSumU32(lDist);
+    if (lane == 0)
+    {
+        SegSummary r;
+        r.nValid = nV;
+        r.nHit = nH;
+        r.lead = seen ? lead : nV;
+        r.trail = T0;
+        r.innerVar = uVar + lVar;
+        r.innerDist = uDist + lDist;
+        waveSum[wv] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        SegSummary acc = waveSum[0];
+#pragma unroll
+        for (int i = 1; i < SCAN_WAVES; ++i)
+            acc = combine(acc, waveSum[i], (int32_t)k);
+        if (segs == 1)
+            stats[(uint64_t)t * nWindows + w] = finalizeSummary(acc, (int32_t)k);
+        else
+            work[((uint64_t)t * nWindows + w) * segs + s] = acc;
+    }
+}
+
+// Folds the segs summaries of each (table, window) in order.
+__global__ void foldKernel(const SegSummary *__restrict__ work, uint32_t nItems, uint32_t segs,
+                           uint32_t k, hipibs_window_stats *__restrict__ stats)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nItems)
+        return;
+    const SegSummary *p = work + (uint64_t)i * segs;
+    SegSummary acc = p[0];
+    for (uint32_t s = 1; s < segs; ++s)
+        acc = combine(acc, p[s], (int32_t)k);
+    stats[i] = finalizeSummary(acc, (int32_t)k);
+}
+
+// Mapper: hit mask over positions pos0 .. pos0+nPos-1 (bit i = position
+// pos0+i), one 64-position word per wave step, each lane gathering its own
+// k-mer (positions need not be aligned to the packed words).
+template <int S>
+__global__ __launch_bounds__(256) void hitmaskKernel(const TableView T, const uint64_t *seq,
+                                                     const uint64_t *exc, uint64_t pos0,
+                                                     uint64_t nPos, uint64_t contigEnd, uint32_t k,
+                                                     uint32_t *hitWords)
+{
+    const int lane = laneId();
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nWaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t nChunks = (nPos + 63) >> 6;
+    const uint64_t kmask1 = (1ull << k) - 1ull;
+    const uint64_t kmask2 = k == 32 ? ~0ull : (1ull << (2 * k)) - 1ull;
+    const uint32_t fwdShift = 64 - 2 * k;
+    const uint64_t nOutWords = (nPos + 31) >> 5;
+    for (uint64_t c = wave; c < nChunks; c += nWaves)
+    {
+        const uint64_t i = (c << 6) + (uint64_t)lane;
+        const uint64_t p = pos0 + i;
+        bool valid = i < nPos && p + k <= contigEnd;
+        uint64_t canon = 0;
+        if (valid)
+        {
+            const uint64_t wi = p >> 5;
+            const uint32_t sh = (uint32_t)(p & 31) * 2;
+            const uint64_t a = seq[wi], b = seq[wi + 1];
+            uint64_t x = sh ? ((a >> sh) | (b << (64 - sh))) : a;
+            x &= kmask2;
+            const uint64_t ej = p >> 6;
+            const uint32_t es = (uint32_t)(p & 63);
+            const uint64_t ea = exc[ej], eb = exc[ej + 1];
+            const uint64_t ev = es ? ((ea >> es) | (eb << (64 - es))) : ea;
+            valid = (ev & kmask1) == 0;
+            const uint64_t rc = ~x & kmask2;
+            const uint64_t fwd = reverse2(x) >> fwdShift;
+            canon = fwd < rc ? fwd : rc;
+        }
+        const uint64_t H = waveProbe<S>(T, canon, valid);
+        if (lane < 2)
+        {
+            const uint64_t o = 2 * c + (uint64_t)lane;
+            if (o < nOutWords)
+                hitWords[o] = (uint32_t)(H >> (32 * lane));
+        }
+    }
+}
+
+// coverage[i] = popcount of hit bits p in [i-k+1, i] (clamped to the
+// positions), as a byte (k <= 32).
+__global__ void coverageKernel(const uint32_t *__restrict__ hitWords, uint64_t nPos,
+                               uint64_t nBases, uint32_t k, uint8_t *__restrict__ cov)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t nWords = (nPos + 31) >> 5;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nBases; i += stride)
+    {
+        uint32_t c = 0;
+        if (nPos)
+        {
+            const uint64_t lo = i + 1 >= k ? i + 1 - k : 0;
+            const uint64_t hi = std::min<uint64_t>(i, nPos - 1);
+            if (lo <= hi)
+            {
+                const uint64_t w0 = lo >> 5;
+                const uint32_t off = (uint32_t)(lo & 31);
+                const uint32_t len = (uint32_t)(hi - lo + 1);
+                uint64_t v = hitWords[w0];
+                if (w0 + 1 < nWords)
+                    v |= (uint64_t)hitWords[w0 + 1] << 32;
+                v >>= off;
+                v &= (len >= 64) ? ~0ull : ((1ull << len) - 1ull);
+                c = (uint32_t)__popcll(v);
+            }
+        }
+        cov[i] = (uint8_t)c;
+    }
+}
+
+// Canonical code per position (all ones when invalid).
+__global__ void encodeKernel(const uint64_t *__restrict__ seq, const uint64_t *__restrict__ exc,
+                             uint64_t pos0, uint64_t nPos, uint64_t contigEnd, uint32_t k,
+                             uint64_t *__restrict__ codes, uint8_t *__restrict__ valid)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t kmask1 = (1ull << k) - 1ull;
+    const uint64_t kmask2 = k == 32 ? ~0ull : (1ull << (2 * k)) - 1ull;
+    const uint32_t fwdShift = 64 - 2 * k;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nPos; i += stride)
+    {
+        const uint64_t p = pos0 + i;
+        bool ok = p + k <= contigEnd;
+        uint64_t canon = EMPTY;
+        if (ok)
+        {
+            const uint64_t wi = p >> 5;
+            const uint32_t sh = (uint32_t)(p & 31) * 2;
+            const uint64_t a = seq[wi], b = seq[wi + 1];
+            uint64_t x = sh ? ((a >> sh) | (b << (64 - sh))) : a;
+            x &= kmask2;
+            const uint64_t ej = p >> 6;
+            const uint32_t es = (uint32_t)(p & 63);
+            const uint64_t ea = exc[ej], eb = exc[ej + 1];
+            const uint64_t ev = es ? ((ea >> es) | (eb << (64 - es))) : ea;
+            ok = (ev & kmask1) == 0;
+            if (ok)
+            {
+                const uint64_t rc = ~x & kmask2;
+                const uint64_t fwd = reverse2(x) >> fwdShift;
+                canon = fwd < rc ? fwd : rc;
+            }
+        }
+        codes[i] = canon;
+        if (valid)
+            valid[i] = ok ? 1 : 0;
+    }
+}
+
+static int deviceCUs(int device)
+{
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    return cus;
+}
+
+uint32_t autoSegsPerWindow(uint32_t nWindows, int nTables, uint32_t maxPositions)
+{
+    // Aim for >= 8192 blocks (32 per CU on a 256-CU part) without cutting a
+    // segment below 4096 positions.
+    const uint64_t units = (uint64_t)std::max<uint32_t>(nWindows, 1) * std::max(nTables, 1);
+    uint64_t segs = (8192 + units - 1) / units;
+    const uint64_t maxSegs = std::max<uint64_t>(1, (uint64_t)maxPositions / 4096);
+    segs = std::min(segs, maxSegs);
+    return (uint32_t)std::max<uint64_t>(1, segs);
+}
+
+uint64_t scanWorkBytes(uint32_t nWindows, int nTables, uint32_t segs)
+{
+    if (segs <= 1)
+        return 0;
+    return (uint64_t)nWindows * nTables * segs * sizeof(SegSummary) + 64;
+}
+
+// Per-thread profiling of the scan kernel (HIP events on the launch stream).
+static thread_local bool tProfile = false;
+static thread_local hipEvent_t tEv[2] = {nullptr, nullptr};
+static thread_local bool tEvRecorded = false;
+
+int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *dSeq,
+               const uint64_t *dExc, uint64_t nBases, const hipibs_window *dWindows,
+               uint32_t nWindows, uint32_t maxWindowBases, hipibs_window_stats *dStats,
+               uint32_t segsPerWindow, void *dWork, uint64_t workBytes, hipStream_t s,
+               TableView *dViews)
+{
+    const uint32_t k = tables[0]->k;
+    const uint32_t maxPos = maxWindowBases >= k ? maxWindowBases - k + 1 : 0;
+    uint32_t segs = segsPerWindow ? segsPerWindow : autoSegsPerWindow(nWindows, nTables, maxPos);
+    if (segs > 1 && (!dWork || workBytes < scanWorkBytes(nWindows, nTables, segs)))
+        return setError(HIPIBS_E_ARG, "scan scratch too small for the segment split");
+    if ((uint64_t)nWindows * segs >= (1ull << 31) || nTables > 65535)
+        return setError(HIPIBS_E_ARG, "too many windows or tables for one launch");
+    if (nWindows == 0)
+        return 0;
+    const dim3 grid(nWindows * segs, (uint32_t)nTables);
+    auto *work = reinterpret_cast<SegSummary *>(dWork);
+    if (tProfile)
+    {
+        if (!tEv[0])
+        {
+            HIPIBS_CHECK(hipEventCreate(&tEv[0]));
+            HIPIBS_CHECK(hipEventCreate(&tEv[1]));
+        }
+        HIPIBS_CHECK(hipEventRecord(tEv[0], s));
+    }
+    if (tables[0]->bucketSlots == 8)
+        hipLaunchKernelGGL(scanKernel<8>, grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc, nBases,
+                           dWindows, nWindows, segs, k, dStats, work);
+    else
+        hipLaunchKernelGGL(scanKernel<16>, grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc, nBases,
+                           dWindows, nWindows, segs, k, dStats, work);
+    HIPIBS_CHECK(hipGetLastError());
+    if (tProfile)
+    {
+        HIPIBS_CHECK(hipEventRecord(tEv[1], s));
+        tEvRecorded = true;
+    }
+    if (segs > 1)
+    {
+        const uint32_t items = nWindows * (uint32_t)nTables;
+        hipLaunchKernelGGL(foldKernel, dim3((items + 255) / 256), dim3(256), 0, s, work, items,
+                           segs, k, dStats);
+        HIPIBS_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
+// Table views live in a small device array; one per thread, grown on demand
+// (the launch reads it asynchronously, so it is only replaced after a
+// device synchronisation).
+struct ViewCache
+{
+    TableView *d = nullptr;
+    int cap = 0;
+    int device = -1;
+    std::vector<TableView> last; // contents of d as last uploaded
+};
+static thread_local ViewCache tViews;
+
+static bool sameViews(const std::vector<TableView> &a, const std::vector<TableView> &b)
+{
+    if (a.size() != b.size())
+        return false;
+    for (size_t i = 0; i < a.size(); ++i)
+        if (a[i].slots != b[i].slots || a[i].nBuckets != b[i].nBuckets ||
+            a[i].maxDisp != b[i].maxDisp)
+            return false;
+    return true;
+}
+
+int uploadViews(const hipibs_table *const *tables, int nTables, hipStream_t s, TableView **out)
+{
+    std::vector<TableView> v(nTables);
+    for (int i = 0; i < nTables; ++i)
+        v[i] = tables[i]->view();
+    if (tViews.d && tViews.device == tables[0]->device && sameViews(v, tViews.last))
+    {
+        *out = tViews.d;
+        return 0;
+    }
+    // The previous contents may still be read by queued kernels.
+    HIPIBS_CHECK(hipDeviceSynchronize());
+    if (tViews.cap < nTables || tViews.device != tables[0]->device)
+    {
+        if (tViews.d)
+            (void)hipFree(tViews.d);
+        tViews.d = nullptr;
+        tViews.last.clear();
+        const int cap = std::max(nTables, 64);
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&tViews.d), cap * sizeof(TableView)));
+        tViews.cap = cap;
+        tViews.device = tables[0]->device;
+    }
+    HIPIBS_CHECK(hipMemcpy(tViews.d, v.data(), nTables * sizeof(TableView), hipMemcpyHostToDevice));
+    tViews.last = v;
+    (void)s;
+    *out = tViews.d;
+    return 0;
+}
+
+int checkTables(const hipibs_table *const *tables, int nTables)
+{
+    if (!tables || nTables <= 0)
+        return setError(HIPIBS_E_ARG, "no tables");
+    for (int i = 0; i < nTables; ++i)
+    {
+        if (!tables[i])
+            return setError(HIPIBS_E_ARG, "null table");
+        if (tables[i]->k != tables[0]->k || tables[i]->device != tables[0]->device ||
+            tables[i]->bucketSlots != tables[0]->bucketSlots)
+            return setError(HIPIBS_E_ARG, "tables differ in k, device or bucket width");
+    }
+    return 0;
+}
+
+} // namespace hipibs
+
+using namespace hipibs;
+
+extern "C" uint64_t hipibs_scan_work_bytes(uint32_t n_windows, int n_tables,
+                                           uint32_t max_window_bases, uint32_t k,
+                                           uint32_t segs_per_window, uint32_t *segs_resolved)
+{
+    const uint32_t maxPos = max_window_bases >= k ? max_window_bases - k + 1 : 0;
+    const uint32_t segs =
+        segs_per_window ? segs_per_window : autoSegsPerWindow(n_windows, n_tables, maxPos);
+    if (segs_resolved)
+        *segs_resolved = segs;
+    return scanWorkBytes(n_windows, n_tables, segs);
+}
+
+extern "C" int hipibs_scan_windows_device(const hipibs_table *const *tables, int n_tables,
+                                          const uint64_t *d_seq_words, const uint64_t *d_exc_words,
+                                          uint64_t n_bases, const hipibs_window *d_windows,
+                                          uint32_t n_windows, uint32_t max_window_bases,
+                                          hipibs_window_stats *d_stats, uint32_t segs_per_window,
+                                          void *d_work, uint64_t work_bytes, void *stream)
+{
+    int rc = checkTables(tables, n_tables);
+    if (rc)
+        return rc;
+    if (n_windows && (!d_seq_words || !d_exc_words || !d_windows || !d_stats))
+        return setError(HIPIBS_E_ARG, "null buffer");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    HIPIBS_CHECK(hipSetDevice(tables[0]->device));
+    TableView *views = nullptr;
+    rc = uploadViews(tables, n_tables, s, &views);
+    if (rc)
+        return rc;
+    return launchScan(tables, n_tables, d_seq_words, d_exc_words, n_bases, d_windows, n_windows,
+                      max_window_bases, d_stats, segs_per_window, d_work, work_bytes, s, views);
+}
+
+extern "C" int hipibs_hitmask_device(const hipibs_table *t, const uint64_t *d_seq_words,
+                                     const uint64_t *d_exc_words, uint64_t pos0, uint64_t n_pos,
+                                     uint64_t contig_end, uint32_t *d_hit_words, void *stream)
+{
+    if (!t || (n_pos && (!d_seq_words || !d_exc_words || !d_hit_words)))
+        return setError(HIPIBS_E_ARG, "null argument");
+    if (n_pos == 0)
+        return 0;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const uint64_t chunks = (n_pos + 63) / 64;
+    const uint64_t grid = std::min<uint64_t>((chunks + 3) / 4, (uint64_t)deviceCUs(t->device) * 16);
+    if (t->bucketSlots == 8)
+        hipLaunchKernelGGL(hitmaskKernel<8>, dim3((uint32_t)grid), dim3(256), 0, s, t->view(),
+                           d_seq_words, d_exc_words, pos0, n_pos, contig_end, t->k, d_hit_words);
+    else
+        hipLaunchKernelGGL(hitmaskKernel<16>, dim3((uint32_t)grid), dim3(256), 0, s, t->view(),
+                           d_seq_words, d_exc_words, pos0, n_pos, contig_end, t->k, d_hit_words);
+    HIPIBS_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int hipibs_coverage_device(const uint32_t *d_hit_words, uint64_t n_pos, uint64_t n_bases,
+                                      uint32_t k, uint8_t *d_cov, void *stream)
+{
+    if (k == 0 || k > 32 || (n_bases && !d_cov) || (n_pos && !d_hit_words))
+        return setError(HIPIBS_E_ARG, "bad argument");
+    if (n_pos > n_bases)
+        return setError(HIPIBS_E_ARG, "more positions than bases");
+    if (n_bases == 0)
+        return 0;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const uint64_t grid = std::min<uint64_t>((n_bases + 255) / 256, (uint64_t)deviceCUs(dev) * 16);
+    hipLaunchKernelGGL(coverageKernel, dim3((uint32_t)grid), dim3(256), 0, s, d_hit_words, n_pos,
+                       n_bases, k, d_cov);
+    HIPIBS_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int hipibs_set_profiling(int on)
+{
+    tProfile = on != 0;
+    tEvRecorded = false;
+    return 0;
+}
+
+extern "C" double hipibs_last_kernel_ms(void)
+{
+    if (!tProfile || !tEvRecorded)
+        return -1.0;
+    float ms = -1.0f;
+    if (hipEventSynchronize(tEv[1]) != hipSuccess)
+        return -1.0;
+    if (hipEventElapsedTime(&ms, tEv[0], tEv[1]) != hipSuccess)
+        return -1.0;
+    return (double)ms;
+}
+
+extern "C" int hipibs_encode_device(const uint64_t *d_seq_words, const uint64_t *d_exc_words,
+                                    uint64_t pos0, uint64_t n_pos, uint64_t contig_end, uint32_t k,
+                                    uint64_t *d_codes, uint8_t *d_valid, void *stream)
+{
+    if (k == 0 || k > 32 || (n_pos && (!d_seq_words || !d_exc_words || !d_codes)))
+        return setError(HIPIBS_E_ARG, "bad argument");
+    if (n_pos == 0)
+        return 0;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const uint64_t grid = std::min<uint64_t>((n_pos + 255) / 256, (uint64_t)deviceCUs(dev) * 16);
+    hipLaunchKernelGGL(encodeKernel, dim3((uint32_t)grid), dim3(256), 0, s, d_seq_words,
+                       d_exc_words, pos0, n_pos, contig_end, k, d_codes, d_valid);
+    HIPIBS_CHECK(hipGetLastError());
+    return 0;
+}

@@ -1,0 +1,518 @@
+// Device tables: construction (insert kernel + host API) and the probe /
+// probe-count kernels behind hipibs_table_probe{,_count{,_device}}.
+//
+// The table replaces the reference's two device structures -- the KMC
+// LUT + binary-search image (src/gpu/DeviceKmerIndex.cu:28-142, ~log2(bucket)+3
+// dependent loads per probe) and the double-hashed 8 B/key hash set
+// (src/gpu/DeviceHashIndex.cu:31-195) -- with one bucketised open-addressing
+// table whose probe costs one 64 B (or 128 B) HBM access in the common case.
+#include "wave_probe.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+namespace hipibs
+{
+
+// One key per lane: walk the buckets from home, claiming the first EMPTY
+// slot with a 64-bit CAS; a slot already holding the key ends the insert
+// (duplicates are stored once). Slots of a bucket are claimed in order, so
+// a concurrent insert of the same key always meets the first copy.
+// counters[0] += newly stored keys, counters[1] = 1 when the table ran out
+// of buckets, counters[2] = max displacement (buckets past home).
+template <int S>
+__global__ __launch_bounds__(256) void insertKernel(uint64_t *slots, uint64_t nBuckets,
+                                                    const uint64_t *keys, uint64_t n,
+                                                    unsigned long long *counters)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t localNew = 0, localDisp = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        const uint64_t key = keys[i];
+        if (key == EMPTY)
+            continue;
+        uint64_t b = bucketOf(key, nBuckets);
+        bool done = false;
+        for (uint64_t d = 0; d < nBuckets && !done; ++d)
+        {
+            unsigned long long *bucket = reinterpret_cast<unsigned long long *>(slots + b * S);
+            for (int s = 0; s < S; ++s)
+            {
+                unsigned long long cur = __hip_atomic_load(bucket + s, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+                if (cur == key)
+                {
+                    done = true;
+                    break;
+                }
+                if (cur == EMPTY)
+                {
+                    const unsigned long long prev = atomicCAS(bucket + s, EMPTY, key);
+                    if (prev == EMPTY)
+                    {
+                        localNew += 1;
+                        localDisp = max(localDisp, (uint32_t)min<uint64_t>(d, 0xffffffffu));
+                        done = true;
+                        break;
+                    }
+                    if (prev == key)
+                    {
+                        done = true;
+                        break;
+                    }
+                }
+            }
+            if (!done)
+            {
+                b += 1;
+                if (b == nBuckets)
+                    b = 0;
+            }
+        }
+        if (!done)
+            atomicExch(counters + 1, 1ull);
+    }
+    if (localNew)
+        atomicAdd(counters + 0, (unsigned long long)localNew);
+    if (localDisp)
+        atomicMax(counters + 2, (unsigned long long)localDisp);
+}
+
+__global__ void fillKernel(uint64_t *p, uint64_t n, uint64_t v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = v;
+}
+
+// 64 queries per wave per step; hits (nullable) gets one byte per query,
+// count (nullable) the number of hits. Grid-stride over 64-query chunks.
+template <int S>
+__global__ __launch_bounds__(256) void probeKernel(const TableView T, const uint64_t *codes,
+                                                   const uint8_t *valid, uint64_t n,
+                                                   uint8_t *hits, unsigned long long *count)
+{
+    const int lane = laneId();
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nWaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t nChunks = (n + 63) >> 6;
+    uint64_t local = 0;
+    for (uint64_t c = wave; c < nChunks; c += nWaves)
+    {
+        const uint64_t i = (c << 6) + (uint64_t)lane;
+        const bool in = i < n;
+        bool v = in && (valid == nullptr || valid[i] != 0);
+        const uint64_t code = v ? codes[i] : 0ull;
+        const uint64_t H = waveProbe<S>(T, code, v);
+        if (hits && in)
+            hits[i] = (uint8_t)((H >> lane) & 1ull);
+        local += (uint64_t)__popcll(H);
+    }
+    if (count && lane == 0 && local)
+        atomicAdd(count, (unsigned long long)local);
+}
+
+static int gridFor(uint64_t work, int block, int device)
+{
+    int cus = 256;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    const uint64_t want = (work + block - 1) / block;
+    const uint64_t cap = (uint64_t)cus * 16;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(want, cap));
+}
+
+int launchTableInsert(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStream_t s)
+{
+    if (n == 0)
+        return 0;
+    const int grid = gridFor(n, 256, t->device);
+    auto *ctr = reinterpret_cast<unsigned long long *>(t->dCounters);
+    if (t->bucketSlots == 8)
+        hipLaunchKernelGGL(insertKernel<8>, dim3(grid), dim3(256), 0, s, t->dSlots, t->nBuckets,
+                           dKeys, n, ctr);
+    else
+        hipLaunchKernelGGL(insertKernel<16>, dim3(grid), dim3(256), 0, s, t->dSlots, t->nBuckets,
+                           dKeys, n, ctr);
+    HIPIBS_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launchProbe(const hipibs_table *t, const uint64_t *dCodes, const uint8_t *dValid, uint64_t n,
+                uint8_t *dHits, uint64_t *dCount, hipStream_t s)
+{
+    if (n == 0)
+        return 0;
+    const uint64_t waves = (n + 63) / 64;
+    const int grid = gridFor(waves * 64, 256, t->device);
+    auto *cnt = reinterpret_cast<unsigned long long *>(dCount);
+    if (t->bucketSlots == 8)
+        hipLaunchKernelGGL(probeKernel<8>, dim3(grid), dim3(256), 0, s, t->view(), dCodes, dValid,
+                           n, dHits, cnt);
+    else
+        hipLaunchKernelGGL(probeKernel<16>, dim3(grid), dim3(256), 0, s, t->view(), dCodes, dValid,
+                           n, dHits, cnt);
+    HIPIBS_CHECK(hipGetLastError());
+    return 0;
+}
+
+// Allocates and clears a table sized for nKeys keys.
+static int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts,
+                      hipibs_table **out, hipStream_t s)
+{
+    hipibs_table_opts o{};
+    if (opts)
+        o = *opts;
+    if (o.load_factor <= 0)
+        o.load_factor = 0.6;
+    if (o.bucket_slots == 0)
+        o.bucket_slots = 8;
+    if (k == 0 || k > 32)
+        return setError(HIPIBS_E_ARG, "k must be in 1..32");
+    if (o.bucket_slots != 8 && o.bucket_slots != 16)
+        return setError(HIPIBS_E_ARG, "bucket_slots must be 8 or 16");
+    if (!(o.load_factor > 0 && o.load_factor < 0.98))
+        return setError(HIPIBS_E_ARG, "load_factor must be in (0, 0.98)");
+    const double slotsWanted = std::ceil((double)std::max<uint64_t>(nKeys, 1) / o.load_factor);
+    uint64_t nBuckets = (uint64_t)std::ceil(slotsWanted / o.bucket_slots);
+    nBuckets = std::max<uint64_t>(nBuckets, 2);
+    if (nBuckets >= (1ull << 32))
+        return setError(HIPIBS_E_ARG, "table too large: more than 2^32 buckets");
+    const uint64_t bytes = nBuckets * o.bucket_slots * 8ull;
+    if (o.hbm_budget && bytes > o.hbm_budget)
+        return setError(HIPIBS_E_NOMEM, "table of " + std::to_string(bytes) +
+                                            " bytes exceeds the HBM budget of " +
+                                            std::to_string(o.hbm_budget));
+    HIPIBS_CHECK(hipSetDevice(o.device));
+    auto *t = new hipibs_table();
+    t->device = o.device;
+    t->k = k;
+    t->bucketSlots = o.bucket_slots;
+    t->nBuckets = nBuckets;
+    t->bytes = bytes;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&t->dSlots), bytes);
+    if (e != hipSuccess)
+    {
+        delete t;
+        return setError(HIPIBS_E_NOMEM, "hipMalloc of " + std::to_string(bytes) +
+                                             " table bytes failed: " + hipGetErrorString(e));
+    }
+    e = hipMalloc(reinterpret_cast<void **>(&t->dCounters), 4 * sizeof(uint64_t));
+    if (e != hipSuccess)
+    {
+        hipFree(t->dSlots);
+        delete t;
+        return setError(HIPIBS_E_NOMEM, "hipMalloc of table counters failed");
+    }
+    const uint64_t nSlots = nBuckets * o.bucket_slots;
+    hipLaunchKernelGGL(fillKernel, dim3(gridFor(nSlots, 256, o.device)), dim3(256), 0, s,
+                       t->dSlots, nSlots, EMPTY);
+    hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, s, t->dCounters, (uint64_t)4, 0ull);
+    e = hipGetLastError();
+    if (e != hipSuccess)
+    {
+        hipFree(t->dSlots);
+        hipFree(t->dCounters);
+        delete t;
+        return hipFail(e, "table clear");
+    }
+    *out = t;
+    return 0;
+}
+
+// Reads back the build counters; fails when the table overflowed.
+static int finishTable(hipibs_table *t, hipStream_t s)
+{
+    uint64_t c[4] = {0, 0, 0, 0};
+    HIPIBS_CHECK(hipMemcpyAsync(c, t->dCounters, sizeof(c), hipMemcpyDeviceToHost, s));
+    HIPIBS_CHECK(hipStreamSynchronize(s));
+    if (c[1])
+        return setError(HIPIBS_E_FULL, "table overflow: no free slot for a key");
+    t->nKeys = c[0];
+    t->maxDisp = (uint32_t)c[2];
+    return 0;
+}
+
+} // namespace hipibs
+
+using namespace hipibs;
+
+extern "C" int hipibs_table_build_keys_device(const uint64_t *d_keys, uint64_t n, uint32_t k,
+                                              const hipibs_table_opts *opts, hipibs_table **out,
+                                              void *stream)
+{
+    if (!out || (n && !d_keys))
+        return setError(HIPIBS_E_ARG, "null argument");
+    *out = nullptr;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipibs_table *t = nullptr;
+    int rc = allocTable(n, k, opts, &t, s);
+    if (rc)
+        return rc;
+    rc = launchTableInsert(t, d_keys, n, s);
+    if (!rc)
+        rc = finishTable(t, s);
+    if (rc)
+    {
+        hipibs_table_free(t);
+        return rc;
+    }
+    *out = t;
+    return 0;
+}
+
+// Streams host keys through two pinned staging buffers (copy of chunk i+1
+// overlaps the insert of chunk i).
+int hipibsBuildFromHostChunks(hipibs_table *t, const uint64_t *hKeys, uint64_t n)
+{
+    const uint64_t chunk = std::min<uint64_t>(n, 32ull << 20);
+    hipStream_t s;
+    HIPIBS_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    uint64_t *pinned[2] = {nullptr, nullptr};
+    uint64_t *dev[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    int rc = 0;
+    for (int i = 0; i < 2 && !rc; ++i)
+    {
+        if (hipHostMalloc(reinterpret_cast<void **>(&pinned[i]), chunk * 8) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void **>(&dev[i]), chunk * 8) != hipSuccess ||
+            hipEventCreateWithFlags(&done[i], hipEventDisableTiming) != hipSuccess)
+            rc = setError(HIPIBS_E_NOMEM, "staging allocation failed");
+    }
+    bool used[2] = {false, false};
+    for (uint64_t off = 0, it = 0; off < n && !rc; off += chunk, ++it)
+    {
+        const int b = (int)(it & 1);
+        const uint64_t m = std::min(chunk, n - off);
+        if (used[b] && hipEventSynchronize(done[b]) != hipSuccess)
+        {
+            rc = setError(HIPIBS_E_HIP, "event wait failed");
+            break;
+        }
+        std::memcpy(pinned[b], hKeys + off, m * 8);
+        if (hipMemcpyAsync(dev[b], pinned[b], m * 8, hipMemcpyHostToDevice, s) != hipSuccess)
+        {
+            rc = setError(HIPIBS_E_HIP, "key upload failed");
+            break;
+        }
+        rc = launchTableInsert(t, dev[b], m, s);
+        if (!rc && hipEventRecord(done[b], s) != hipSuccess)
+            rc = setError(HIPIBS_E_HIP, "event record failed");
+        used[b] = true;
+    }
+    if (!rc)
+        rc = finishTable(t, s);
+    hipStreamSynchronize(s);
+    for (int i = 0; i < 2; ++i)
+    {
+        if (pinned[i])
+            hipHostFree(pinned[i]);
+        if (dev[i])
+            hipFree(dev[i]);
+        if (done[i])
+            hipEventDestroy(done[i]);
+    }
+    hipStreamDestroy(s);
+    return rc;
+}
+
+extern "C" int hipibs_table_build_keys(const uint64_t *h_keys, uint64_t n, uint32_t k,
+                                       const hipibs_table_opts *opts, hipibs_table **out)
+{
+    if (!out || (n && !h_keys))
+        return setError(HIPIBS_E_ARG, "null argument");
+    *out = nullptr;
+    hipibs_table *t = nullptr;
+    int rc = allocTable(n, k, opts, &t, nullptr);
+    if (rc)
+        return rc;
+    rc = hipDeviceSynchronize() == hipSuccess ? 0 : setError(HIPIBS_E_HIP, "table clear failed");
+    if (!rc)
+        rc = hipibsBuildFromHostChunks(t, h_keys, n);
+    if (rc)
+    {
+        hipibs_table_free(t);
+        return rc;
+    }
+    *out = t;
+    return 0;
+}
+
+extern "C" int hipibs_table_free(hipibs_table *t)
+{
+    if (!t)
+        return 0;
+    hipSetDevice(t->device);
+    if (t->dSlots)
+        hipFree(t->dSlots);
+    if (t->dCounters)
+        hipFree(t->dCounters);
+    delete t;
+    return 0;
+}
+
+extern "C" int hipibs_table_info(const hipibs_table *t, uint64_t *n_keys, uint64_t *bytes,
+                                 uint64_t *n_buckets, uint32_t *k, uint32_t *bucket_slots,
+                                 int *device)
+{
+    if (!t)
+        return setError(HIPIBS_E_ARG, "null table");
+    if (n_keys)
+        *n_keys = t->nKeys;
+    if (bytes)
+        *bytes = t->bytes;
+    if (n_buckets)
+        *n_buckets = t->nBuckets;
+    if (k)
+        *k = t->k;
+    if (bucket_slots)
+        *bucket_slots = t->bucketSlots;
+    if (device)
+        *device = t->device;
+    return 0;
+}
+
+extern "C" int hipibs_table_probe(const hipibs_table *t, const uint64_t *d_codes,
+                                  const uint8_t *d_valid, uint64_t n, uint8_t *d_hits,
+                                  void *stream)
+{
+    if (!t || (n && (!d_codes || !d_hits)))
+        return setError(HIPIBS_E_ARG, "null argument");
+    return launchProbe(t, d_codes, d_valid, n, d_hits, nullptr,
+                       reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int hipibs_table_probe_count_device(const hipibs_table *t, const uint64_t *d_keys,
+                                               uint64_t n, uint64_t *d_count, void *stream)
+{
+    if (!t || !d_count || (n && !d_keys))
+        return setError(HIPIBS_E_ARG, "null argument");
+    return launchProbe(t, d_keys, nullptr, n, nullptr, d_count,
+                       reinterpret_cast<hipStream_t>(stream));
+}
+
+// Host keys: pinned double-buffered chunks, one private stream per call, a
+// mutex per table so concurrent callers (the intersect's consumer threads)
+// serialise on the device as DeviceKmerIndex::probeCountHost does.
+extern "C" int hipibs_table_probe_count(const hipibs_table *t, const uint64_t *h_keys, uint64_t n,
+                                        uint64_t *out)
+{
+    if (!t || !out || (n && !h_keys))
+        return setError(HIPIBS_E_ARG, "null argument");
+    *out = 0;
+    if (n == 0)
+        return 0;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    HIPIBS_CHECK(hipSetDevice(t->device));
+    const uint64_t chunk = std::min<uint64_t>(n, 16ull << 20);
+    hipStream_t s;
+    HIPIBS_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    uint64_t *pinned[2] = {nullptr, nullptr};
+    uint64_t *dev[2] = {nullptr, nullptr};
+    uint64_t *dCount = nullptr;
+    hipEvent_t done[2] = {nullptr, nullptr};
+    int rc = 0;
+    for (int i = 0; i < 2 && !rc; ++i)
+        if (hipHostMalloc(reinterpret_cast<void **>(&pinned[i]), chunk * 8) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void **>(&dev[i]), chunk * 8) != hipSuccess ||
+            hipEventCreateWithFlags(&done[i], hipEventDisableTiming) != hipSuccess)
+            rc = setError(HIPIBS_E_NOMEM, "staging allocation failed");
+    if (!rc && hipMalloc(reinterpret_cast<void **>(&dCount), 8) != hipSuccess)
+        rc = setError(HIPIBS_E_NOMEM, "count allocation failed");
+    if (!rc)
+    {
+        hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, s, dCount, (uint64_t)1, 0ull);
+        if (hipGetLastError() != hipSuccess)
+            rc = setError(HIPIBS_E_HIP, "count clear failed");
+    }
+    bool used[2] = {false, false};
+    for (uint64_t off = 0, it = 0; off < n && !rc; off += chunk, ++it)
+    {
+        const int b = (int)(it & 1);
+        const uint64_t m = std::min(chunk, n - off);
+        if (used[b] && hipEventSynchronize(done[b]) != hipSuccess)
+        {
+            rc = setError(HIPIBS_E_HIP, "event wait failed");
+            break;
+        }
+        std::memcpy(pinned[b], h_keys + off, m * 8);
+        if (hipMemcpyAsync(dev[b], pinned[b], m * 8, hipMemcpyHostToDevice, s) != hipSuccess)
+        {
+            rc = setError(HIPIBS_E_HIP, "key upload failed");
+            break;
+        }
+        rc = launchProbe(t, dev[b], nullptr, m, nullptr, dCount, s);
+        if (!rc && hipEventRecord(done[b], s) != hipSuccess)
+            rc = setError(HIPIBS_E_HIP, "event record failed");
+        used[b] = true;
+    }
+    uint64_t c = 0;
+    if (!rc && hipMemcpyAsync(&c, dCount, 8, hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = setError(HIPIBS_E_HIP, "count download failed");
+    if (hipStreamSynchronize(s) != hipSuccess && !rc)
+        rc = setError(HIPIBS_E_HIP, "stream synchronize failed");
+    for (int i = 0; i < 2; ++i)
+    {
+        if (pinned[i])
+            hipHostFree(pinned[i]);
+        if (dev[i])
+            hipFree(dev[i]);
+        if (done[i])
+            hipEventDestroy(done[i]);
+    }
+    if (dCount)
+        hipFree(dCount);
+    hipStreamDestroy(s);
+    if (!rc)
+        *out = c;
+    return rc;
+}
+
+extern "C" int hipibs_table_build_kmc(const char *prefix, const hipibs_table_opts *opts,
+                                      hipibs_table **out)
+{
+    if (!prefix || !out)
+        return setError(HIPIBS_E_ARG, "null argument");
+    *out = nullptr;
+    hipibs_kmc_info info{};
+    int rc = hipibs_kmc_header(prefix, &info);
+    if (rc)
+        return rc;
+    if (info.kmer_length == 0 || info.kmer_length > 32)
+        return setError(HIPIBS_E_ARG, "KMC database k must be in 1..32");
+    std::vector<uint64_t> keys;
+    try
+    {
+        keys.resize(info.total_kmers);
+    }
+    catch (const std::exception &)
+    {
+        return setError(HIPIBS_E_NOMEM, "host allocation of the key array failed");
+    }
+    uint64_t n = 0;
+    rc = kmcReadKeys(prefix, keys.data(), keys.size(), &n, opts ? opts->load_threads : 0, nullptr);
+    if (rc)
+        return rc;
+    hipibs_table *t = nullptr;
+    rc = allocTable(n, info.kmer_length, opts, &t, nullptr);
+    if (rc)
+        return rc;
+    rc = hipDeviceSynchronize() == hipSuccess ? 0 : setError(HIPIBS_E_HIP, "table clear failed");
+    if (!rc)
+        rc = hipibsBuildFromHostChunks(t, keys.data(), n);
+    if (!rc && t->nKeys != n)
+        rc = setError(HIPIBS_E_IO, "KMC database holds duplicate k-mers (" +
+                                       std::to_string(n) + " records, " +
+                                       std::to_string(t->nKeys) + " distinct)");
+    if (rc)
+    {
+        hipibs_table_free(t);
+        return rc;
+    }
+    *out = t;
+    return 0;
+}

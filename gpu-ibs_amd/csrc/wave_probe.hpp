@@ -1,0 +1,188 @@
+// Wavefront-cooperative membership probe of 64 queries (one per lane) in a
+// bucketised table. Device-only.
+//
+// A bucket of S 8-byte slots (S = 8: 64 B, S = 16: 128 B) is read by
+// LPQ = S/2 adjacent lanes, 16 B each, so one load instruction of the wave
+// fetches QPR = 64/LPQ whole buckets as contiguous, aligned requests (one
+// 64 B or 128 B HBM access per query). The 64 queries take LPQ such rounds;
+// all of their loads are issued before any compare so they are in flight
+// together. The match / empty-slot flags come back as wave ballots, from
+// which each lane reads its own query's result. Queries whose bucket was
+// full without a match (rare) continue to the following buckets in compact
+// rounds until found, an empty slot is seen, or the table's maximum
+// displacement is exhausted.
+#pragma once
+
+#include "hipibs_internal.hpp"
+
+namespace hipibs
+{
+
+struct alignas(16) SlotPair
+{
+    uint64_t a, b;
+};
+
+__device__ inline int laneId()
+{
+    return (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of m below this lane.
+__device__ inline uint32_t maskedRank(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ inline uint64_t ballot(bool p)
+{
+    return __ballot(p);
+}
+
+__device__ inline uint32_t shuffle32(uint32_t v, int src)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
+}
+
+__device__ inline uint64_t shuffle64(uint64_t v, int src)
+{
+    const uint32_t lo = shuffle32((uint32_t)v, src);
+    const uint32_t hi = shuffle32((uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Position of the n-th (0-based) set bit of m; requires n < popcount(m).
+__device__ inline int selectBit(uint64_t m, uint32_t n)
+{
+    int pos = 0;
+    uint32_t c = __popc((uint32_t)m);
+    if (n >= c)
+    {
+        n -= c;
+        pos += 32;
+        m >>= 32;
+    }
+    c = __popc((uint32_t)m & 0xFFFFu);
+    if (n >= c)
+    {
+        n -= c;
+        pos += 16;
+        m >>= 16;
+    }
+    c = __popc((uint32_t)m & 0xFFu);
+    if (n >= c)
+    {
+        n -= c;
+        pos += 8;
+        m >>= 8;
+    }
+    c = __popc((uint32_t)m & 0xFu);
+    if (n >= c)
+    {
+        n -= c;
+        pos += 4;
+        m >>= 4;
+    }
+    c = __popc((uint32_t)m & 0x3u);
+    if (n >= c)
+    {
+        n -= c;
+        pos += 2;
+        m >>= 2;
+    }
+    c = (uint32_t)m & 1u;
+    if (n >= c)
+        pos += 1;
+    return pos;
+}
+
+// Returns the 64-bit hit mask of the wave: bit i set when lane i's query is
+// valid and its code is stored in the table. Must be called by all 64 lanes.
+template <int S>
+__device__ inline uint64_t waveProbe(const TableView T, uint64_t code, bool valid)
+{
+    constexpr int LPQ = S / 2;      // lanes per query
+    constexpr int QPR = 64 / LPQ;   // queries per round
+    constexpr int ROUNDS = 64 / QPR;
+    constexpr uint32_t GMASK = (1u << LPQ) - 1u;
+    const int lane = laneId();
+    const int g = lane / LPQ, sub = lane % LPQ;
+    const uint64_t V = ballot(valid);
+
+    SlotPair kv[ROUNDS];
+    uint64_t qc[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r)
+    {
+        const int q = r * QPR + g;
+        qc[r] = shuffle64(code, q);
+        kv[r] = SlotPair{EMPTY, EMPTY};
+        if ((V >> q) & 1ull)
+        {
+            const uint64_t b = bucketOf(qc[r], T.nBuckets);
+            kv[r] = *reinterpret_cast<const SlotPair *>(T.slots + b * S + (uint32_t)sub * 2u);
+        }
+    }
+    uint64_t M[ROUNDS], E[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r)
+    {
+        const bool vq = (V >> (r * QPR + g)) & 1ull;
+        M[r] = ballot(vq && (kv[r].a == qc[r] || kv[r].b == qc[r]));
+        E[r] = ballot(vq && (kv[r].a == EMPTY || kv[r].b == EMPTY));
+    }
+    const int rOwn = lane / QPR, gOwn = lane % QPR;
+    uint64_t Mo = M[0], Eo = E[0];
+#pragma unroll
+    for (int r = 1; r < ROUNDS; ++r)
+        if (rOwn == r)
+        {
+            Mo = M[r];
+            Eo = E[r];
+        }
+    const bool found = ((Mo >> (gOwn * LPQ)) & GMASK) != 0;
+    const bool hasEmpty = ((Eo >> (gOwn * LPQ)) & GMASK) != 0;
+    uint64_t H = ballot(valid && found);
+    uint64_t NC = ballot(valid && !found && !hasEmpty);
+    if (NC == 0 || T.maxDisp == 0)
+        return H;
+
+    // Overflow rounds: pending lane i continues at home(code_i) + disp_i.
+    const uint64_t home = bucketOf(code, T.nBuckets);
+    uint32_t disp = 1;
+    while (NC)
+    {
+        const bool pending = (NC >> lane) & 1ull;
+        const uint32_t nPend = (uint32_t)__popcll(NC);
+        // group g serves the pending lane of rank g
+        const bool serving = (uint32_t)g < nPend;
+        const int src = serving ? selectBit(NC, (uint32_t)g) : 0;
+        const uint64_t qcode = shuffle64(code, src);
+        const uint64_t qhome = shuffle64(home, src);
+        const uint32_t qdisp = shuffle32(disp, src);
+        SlotPair kv2{EMPTY, EMPTY};
+        if (serving)
+        {
+            uint64_t b = qhome + qdisp;
+            if (b >= T.nBuckets)
+                b -= T.nBuckets;
+            kv2 = *reinterpret_cast<const SlotPair *>(T.slots + b * S + (uint32_t)sub * 2u);
+        }
+        const uint64_t M2 = ballot(serving && (kv2.a == qcode || kv2.b == qcode));
+        const uint64_t E2 = ballot(serving && (kv2.a == EMPTY || kv2.b == EMPTY));
+        const uint32_t rank = maskedRank(NC);
+        const bool served = pending && rank < (uint32_t)QPR;
+        const uint32_t sh = (served ? rank : 0u) * (uint32_t)LPQ;
+        const bool f2 = served && ((M2 >> sh) & GMASK) != 0;
+        const bool e2 = served && ((E2 >> sh) & GMASK) != 0;
+        const bool done = served && (f2 || e2 || disp >= T.maxDisp);
+        H |= ballot(f2);
+        if (served && !done)
+            disp += 1;
+        NC &= ~ballot(done);
+    }
+    return H;
+}
+
+} // namespace hipibs

@@ -1,0 +1,496 @@
+"""Python bindings of libhipibs (the MI355X k-mer IBS scan) over its C ABI.
+
+The library is the product; this module is a thin ctypes layer used by the
+tests, the benchmark and scripting users. It never falls back to a CPU
+implementation: if libhipibs.so is missing, import fails loudly.
+
+Host-side helpers mirror the reference's host code so results line up with
+its tools: read_sequence_file (src/Utils.hpp:170-255), chunk_sequences
+(src/Utils.hpp:296-327), output naming and the TSV writer
+(src/FastIBS.cpp:14-25, src/KmerDatabase.hpp:935-948).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import gzip
+import os
+from dataclasses import dataclass
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhipibs.so")
+PACK_PAD_WORDS = 4
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"libhipibs.so not found at {LIB_PATH}: build it first "
+        "(python -c 'import __graft_entry__ as g; g.build()' or make -C gpu-ibs_amd)")
+
+_lib = C.CDLL(LIB_PATH)
+
+u64p = C.POINTER(C.c_uint64)
+u32p = C.POINTER(C.c_uint32)
+u8p = C.POINTER(C.c_uint8)
+vp = C.c_void_p
+
+
+class TableOpts(C.Structure):
+    _fields_ = [("load_factor", C.c_double), ("bucket_slots", C.c_uint32), ("device", C.c_int),
+                ("hbm_budget", C.c_uint64), ("load_threads", C.c_uint32)]
+
+
+class Window(C.Structure):
+    _fields_ = [("base", C.c_uint64), ("n_bases", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class WindowStats(C.Structure):
+    _fields_ = [("total_kmers", C.c_int32), ("observed_kmers", C.c_int32),
+                ("variations", C.c_int32), ("kmer_distance", C.c_int32)]
+
+
+class KmcInfo(C.Structure):
+    _fields_ = [("kmc_version", C.c_uint32), ("kmer_length", C.c_uint32), ("mode", C.c_uint32),
+                ("counter_size", C.c_uint32), ("lut_prefix_length", C.c_uint32),
+                ("signature_len", C.c_uint32), ("min_count", C.c_uint32),
+                ("max_count", C.c_uint32), ("total_kmers", C.c_uint64)]
+
+
+class HostSlot(C.Structure):
+    _fields_ = [("seq_words", u64p), ("exc_words", u64p), ("windows", C.POINTER(Window)),
+                ("stats", C.POINTER(WindowStats)), ("seq_cap", C.c_uint64),
+                ("exc_cap", C.c_uint64), ("window_cap", C.c_uint32), ("n_tables", C.c_uint32)]
+
+
+WINDOW_DTYPE = np.dtype([("base", "<u8"), ("n_bases", "<u4"), ("reserved", "<u4")])
+STATS_DTYPE = np.dtype([("total_kmers", "<i4"), ("observed_kmers", "<i4"),
+                        ("variations", "<i4"), ("kmer_distance", "<i4")])
+
+# (name, restype, argtypes): every symbol include/hipibs.h declares.
+_SIGS = [
+    ("hipibs_last_error", C.c_char_p, []),
+    ("hipibs_device_count", C.c_int, [C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+    ("hipibs_device_mem", C.c_int, [C.c_int, u64p, u64p]),
+    ("hipibs_kmc_header", C.c_int, [C.c_char_p, C.POINTER(KmcInfo)]),
+    ("hipibs_kmc_read_keys", C.c_int, [C.c_char_p, u64p, C.c_uint64, u64p, C.c_uint32]),
+    ("hipibs_kmc_write", C.c_int, [C.c_char_p, u64p, u32p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                   C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("hipibs_pack_bases", C.c_int, [C.c_char_p, C.c_uint64, C.c_uint64, u64p, u64p]),
+    ("hipibs_table_build_kmc", C.c_int, [C.c_char_p, C.POINTER(TableOpts), C.POINTER(vp)]),
+    ("hipibs_table_build_keys", C.c_int, [u64p, C.c_uint64, C.c_uint32, C.POINTER(TableOpts),
+                                          C.POINTER(vp)]),
+    ("hipibs_table_build_keys_device", C.c_int, [vp, C.c_uint64, C.c_uint32,
+                                                 C.POINTER(TableOpts), C.POINTER(vp), vp]),
+    ("hipibs_table_free", C.c_int, [vp]),
+    ("hipibs_table_info", C.c_int, [vp, u64p, u64p, u64p, u32p, u32p, C.POINTER(C.c_int)]),
+    ("hipibs_table_probe", C.c_int, [vp, vp, vp, C.c_uint64, vp, vp]),
+    ("hipibs_table_probe_count", C.c_int, [vp, u64p, C.c_uint64, u64p]),
+    ("hipibs_table_probe_count_device", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
+    ("hipibs_scan_windows_device", C.c_int, [C.POINTER(vp), C.c_int, vp, vp, C.c_uint64, vp,
+                                             C.c_uint32, C.c_uint32, vp, C.c_uint32, vp,
+                                             C.c_uint64, vp]),
+    ("hipibs_scan_work_bytes", C.c_uint64, [C.c_uint32, C.c_int, C.c_uint32, C.c_uint32,
+                                            C.c_uint32, u32p]),
+    ("hipibs_hitmask_device", C.c_int, [vp, vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, vp]),
+    ("hipibs_coverage_device", C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint32, vp, vp]),
+    ("hipibs_encode_device", C.c_int, [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32,
+                                       vp, vp, vp]),
+    ("hipibs_set_profiling", C.c_int, [C.c_int]),
+    ("hipibs_last_kernel_ms", C.c_double, []),
+    ("hipibs_scan_create", C.c_int, [C.POINTER(vp), C.c_int, C.c_uint64, C.c_uint32, C.c_int,
+                                     C.POINTER(vp)]),
+    ("hipibs_scan_slot", C.c_int, [vp, C.c_int, C.POINTER(HostSlot)]),
+    ("hipibs_scan_submit", C.c_int, [vp, C.c_int, C.c_uint32, C.c_uint64, C.c_uint64]),
+    ("hipibs_scan_wait", C.c_int, [vp, C.c_int]),
+    ("hipibs_scan_free", C.c_int, [vp]),
+]
+for _name, _res, _args in _SIGS:
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+EXPORTED = [s[0] for s in _SIGS]
+
+
+class HipibsError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        msg = _lib.hipibs_last_error().decode(errors="replace")
+        super().__init__(f"{where} failed ({code}): {msg}")
+        self.code = code
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != 0:
+        raise HipibsError(rc, where)
+
+
+def _ptr(a: np.ndarray, t=u64p):
+    return a.ctypes.data_as(t)
+
+
+# ---- devices ----------------------------------------------------------------
+
+def device_count() -> Tuple[int, str]:
+    n = C.c_int(0)
+    why = C.create_string_buffer(512)
+    _lib.hipibs_device_count(C.byref(n), why, 512)
+    return n.value, why.value.decode(errors="replace")
+
+
+def device_mem(device: int = 0) -> Tuple[int, int]:
+    f, t = C.c_uint64(0), C.c_uint64(0)
+    _check(_lib.hipibs_device_mem(device, C.byref(f), C.byref(t)), "hipibs_device_mem")
+    return f.value, t.value
+
+
+# ---- KMC databases ----------------------------------------------------------
+
+def kmc_header(prefix: str) -> KmcInfo:
+    info = KmcInfo()
+    _check(_lib.hipibs_kmc_header(prefix.encode(), C.byref(info)), "hipibs_kmc_header")
+    return info
+
+
+def kmc_read_keys(prefix: str, threads: int = 0) -> np.ndarray:
+    info = kmc_header(prefix)
+    keys = np.empty(info.total_kmers, dtype=np.uint64)
+    n = C.c_uint64(0)
+    _check(_lib.hipibs_kmc_read_keys(prefix.encode(), _ptr(keys), keys.size, C.byref(n), threads),
+           "hipibs_kmc_read_keys")
+    return keys
+
+
+def kmc_write(prefix: str, keys: np.ndarray, k: int, counts: Optional[np.ndarray] = None,
+              p: Optional[int] = None, sig_len: int = 9, counter_size: int = 1,
+              version: int = 0x200, min_count: int = 1, max_count: int = 255) -> None:
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    if p is None:
+        p = _default_lut_prefix(k)
+    cp = None
+    if counts is not None:
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        cp = _ptr(counts, u32p)
+    _check(_lib.hipibs_kmc_write(prefix.encode(), _ptr(keys), cp, keys.size, k, p, sig_len,
+                                 counter_size, version, min_count, max_count), "hipibs_kmc_write")
+
+
+def _default_lut_prefix(k: int) -> int:
+    """Largest p <= min(k, 12) with (k - p) % 4 == 0 and at least one suffix
+    byte when k allows it (KMC's own choice depends on the database size)."""
+    for p in range(min(k, 12), -1, -1):
+        if (k - p) % 4 == 0 and (k - p) >= 4:
+            if p <= 9:
+                return p
+    for p in range(min(k, 12), -1, -1):
+        if (k - p) % 4 == 0:
+            return p
+    raise ValueError(f"no LUT prefix length for k={k}")
+
+
+# ---- packing ------------------------------------------------------------------
+
+def pack_words(n_bases: int) -> Tuple[int, int]:
+    """Word counts (sequence, exception) of a packed stream of n_bases,
+    padding included."""
+    return (n_bases + 31) // 32 + PACK_PAD_WORDS, (n_bases + 63) // 64 + PACK_PAD_WORDS
+
+
+def pack_sequences(seqs: Sequence[bytes]) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Concatenates the contigs into one packed stream. Returns (seq_words,
+    exc_words, contig_offsets) with offsets[i] = first base of contig i."""
+    offsets = np.zeros(len(seqs) + 1, dtype=np.uint64)
+    for i, s in enumerate(seqs):
+        offsets[i + 1] = offsets[i] + len(s)
+    total = int(offsets[-1])
+    ns, ne = pack_words(total)
+    sw = np.zeros(ns, dtype=np.uint64)
+    ew = np.zeros(ne, dtype=np.uint64)
+    for i, s in enumerate(seqs):
+        if len(s):
+            _check(_lib.hipibs_pack_bases(bytes(s), len(s), int(offsets[i]), _ptr(sw), _ptr(ew)),
+                   "hipibs_pack_bases")
+    return sw, ew, offsets
+
+
+# ---- host input handling mirroring the reference ------------------------------
+
+@dataclass
+class ParseResult:
+    ids: List[bytes]
+    sequences: List[bytes]
+
+
+def read_sequence_file(path: str) -> ParseResult:
+    """readSequenceFile (src/Utils.hpp:170-255): '>' or '@' decides FASTA or
+    FASTQ; lines split on '\\n' only (a trailing '\\r' stays a base); empty
+    lines skipped; records with an empty sequence dropped."""
+    if ".gz" in path:
+        with gzip.open(path, "rb") as f:
+            data = f.read()
+    else:
+        with open(path, "rb") as f:
+            data = f.read()
+    if not data:
+        raise ValueError("Empty file")
+    if data[:1] == b">":
+        fastq = False
+    elif data[:1] == b"@":
+        fastq = True
+    else:
+        raise ValueError("Unknown file type")
+    id_char = b"@" if fastq else b">"
+    ids: List[bytes] = []
+    seqs: List[bytes] = []
+    cur_id = b""
+    cur: List[bytes] = []
+    quality = False
+    for line in data.split(b"\n"):
+        if not line:
+            continue
+        if fastq and line[:1] == b"+":
+            quality = True
+            continue
+        if quality:
+            quality = False
+            continue
+        if line[:1] == id_char:
+            if cur:
+                s = b"".join(cur)
+                if s:
+                    ids.append(cur_id)
+                    seqs.append(s)
+                cur = []
+            cur_id = line[1:]
+        else:
+            cur.append(line)
+    if cur:
+        s = b"".join(cur)
+        if s:
+            ids.append(cur_id)
+            seqs.append(s)
+    return ParseResult(ids, seqs)
+
+
+def chunk_sequences(lengths: Sequence[int], W: int, k: int) -> List[Tuple[int, int, int]]:
+    """chunkSequences (src/Utils.hpp:296-327): (contig, start, end) per
+    window, start = 0, W-k, 2(W-k), ... while start < len."""
+    if W <= k:
+        raise ValueError(f"window size {W} must be larger than k ({k})")
+    out = []
+    for c, n in enumerate(lengths):
+        j = 0
+        while j < n:
+            out.append((c, j, min(j + W, n)))
+            j += W - k
+    return out
+
+
+def format_tsv(rows: Sequence[Tuple[bytes, int, int, int, int, int, int]]) -> bytes:
+    """KmerDatabase::writeStats (src/KmerDatabase.hpp:935-948)."""
+    parts = [b"seqname\tstart\tend\ttotal_kmers\tobserved_kmers\tvariations\tkmer_distance\n"]
+    for r in rows:
+        parts.append(b"%s\t%d\t%d\t%d\t%d\t%d\t%d\n" % tuple(r))
+    return b"".join(parts)
+
+
+# ---- tables -------------------------------------------------------------------
+
+def _opts(load_factor=0.6, bucket_slots=8, device=0, hbm_budget=0, load_threads=0):
+    return TableOpts(load_factor, bucket_slots, device, hbm_budget, load_threads)
+
+
+class Table:
+    """A device-resident k-mer table (hipibs_table)."""
+
+    def __init__(self, handle: int):
+        self.handle = C.c_void_p(handle)
+
+    @classmethod
+    def from_keys(cls, keys: np.ndarray, k: int, **kw) -> "Table":
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        h = vp()
+        o = _opts(**kw)
+        _check(_lib.hipibs_table_build_keys(_ptr(keys), keys.size, k, C.byref(o), C.byref(h)),
+               "hipibs_table_build_keys")
+        return cls(h.value)
+
+    @classmethod
+    def from_kmc(cls, prefix: str, **kw) -> "Table":
+        h = vp()
+        o = _opts(**kw)
+        _check(_lib.hipibs_table_build_kmc(prefix.encode(), C.byref(o), C.byref(h)),
+               "hipibs_table_build_kmc")
+        return cls(h.value)
+
+    @classmethod
+    def from_device_keys(cls, d_keys: int, n: int, k: int, stream: int = 0, **kw) -> "Table":
+        h = vp()
+        o = _opts(**kw)
+        _check(_lib.hipibs_table_build_keys_device(vp(d_keys), n, k, C.byref(o), C.byref(h),
+                                                   vp(stream)), "hipibs_table_build_keys_device")
+        return cls(h.value)
+
+    def info(self) -> dict:
+        nk, nb, nbk = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        k, bs, dev = C.c_uint32(), C.c_uint32(), C.c_int()
+        _check(_lib.hipibs_table_info(self.handle, C.byref(nk), C.byref(nb), C.byref(nbk),
+                                      C.byref(k), C.byref(bs), C.byref(dev)), "hipibs_table_info")
+        return dict(n_keys=nk.value, bytes=nb.value, n_buckets=nbk.value, k=k.value,
+                    bucket_slots=bs.value, device=dev.value)
+
+    def probe_count(self, keys: np.ndarray) -> int:
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = C.c_uint64(0)
+        _check(_lib.hipibs_table_probe_count(self.handle, _ptr(keys), keys.size, C.byref(out)),
+               "hipibs_table_probe_count")
+        return out.value
+
+    def free(self) -> None:
+        if self.handle:
+            _lib.hipibs_table_free(self.handle)
+            self.handle = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _table_array(tables: Sequence[Table]):
+    arr = (vp * len(tables))(*[t.handle.value for t in tables])
+    return arr
+
+
+# ---- the streaming scan engine --------------------------------------------------
+
+class ScanEngine:
+    """hipibs_scan: pinned double-buffered slots (the EncodeProbeEngine role)."""
+
+    def __init__(self, tables: Sequence[Table], batch_bases: int, max_windows: int, depth: int = 2):
+        self.tables = list(tables)
+        h = vp()
+        _check(_lib.hipibs_scan_create(_table_array(self.tables), len(self.tables), batch_bases,
+                                       max_windows, depth, C.byref(h)), "hipibs_scan_create")
+        self.handle = h
+        self.depth = depth
+        self.slots = []
+        for i in range(depth):
+            s = HostSlot()
+            _check(_lib.hipibs_scan_slot(self.handle, i, C.byref(s)), "hipibs_scan_slot")
+            self.slots.append(s)
+
+    def slot_arrays(self, i: int):
+        s = self.slots[i]
+        seq = np.ctypeslib.as_array(s.seq_words, shape=(s.seq_cap,))
+        exc = np.ctypeslib.as_array(s.exc_words, shape=(s.exc_cap,))
+        win = np.ctypeslib.as_array(C.cast(s.windows, C.POINTER(C.c_uint8)),
+                                    shape=(s.window_cap * WINDOW_DTYPE.itemsize,)).view(WINDOW_DTYPE)
+        st = np.ctypeslib.as_array(C.cast(s.stats, C.POINTER(C.c_uint8)),
+                                   shape=(s.window_cap * s.n_tables * STATS_DTYPE.itemsize,)
+                                   ).view(STATS_DTYPE)
+        return seq, exc, win, st
+
+    def submit(self, i: int, n_windows: int, n_seq_words: int, n_exc_words: int) -> None:
+        _check(_lib.hipibs_scan_submit(self.handle, i, n_windows, n_seq_words, n_exc_words),
+               "hipibs_scan_submit")
+
+    def wait(self, i: int) -> None:
+        _check(_lib.hipibs_scan_wait(self.handle, i), "hipibs_scan_wait")
+
+    def free(self) -> None:
+        if self.handle:
+            _lib.hipibs_scan_free(self.handle)
+            self.handle = vp(0)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def scan_sequences(tables: Sequence[Table], ids: Sequence[bytes], seqs: Sequence[bytes], W: int,
+                   batch_bases: int = 1 << 26, depth: int = 2) -> List[List[tuple]]:
+    """Per-window statistics of every contig against every table through the
+    streaming engine; returns one list of TSV rows per table. Windows are
+    never split across batches (buildBatches, src/gpu/EncodeProbeCore.hpp:103-134)."""
+    k = tables[0].info()["k"]
+    wins = chunk_sequences([len(s) for s in seqs], W, k)
+    max_win = max([e - s for _, s, e in wins], default=1)
+    batch_bases = max(batch_bases, max_win + 64)
+    # windows per batch: each window packed on its own (contiguous copy)
+    eng = ScanEngine(tables, batch_bases, max_windows=max(1, min(len(wins), 1 << 16)), depth=depth)
+    try:
+        results = [[None] * len(wins) for _ in tables]
+        batches = []
+        cur, used = [], 0
+        cap_w = eng.slots[0].window_cap
+        for wi, (c, s, e) in enumerate(wins):
+            need = ((e - s) + 63) // 64 * 64
+            if cur and (used + need > batch_bases or len(cur) >= cap_w):
+                batches.append(cur)
+                cur, used = [], 0
+            cur.append(wi)
+            used += need
+        if cur:
+            batches.append(cur)
+        pending = {}
+        for bi, batch in enumerate(batches):
+            slot = bi % depth
+            if slot in pending:
+                eng.wait(slot)
+                _collect(eng, slot, pending.pop(slot), results)
+            seq, exc, win, _ = eng.slot_arrays(slot)
+            at = 0
+            seq[:] = 0
+            exc[:] = 0
+            for j, wi in enumerate(batch):
+                c, s, e = wins[wi]
+                text = seqs[c][s:e]
+                if text:
+                    _check(_lib.hipibs_pack_bases(bytes(text), len(text), at,
+                                                  seq.ctypes.data_as(u64p),
+                                                  exc.ctypes.data_as(u64p)), "hipibs_pack_bases")
+                win[j] = (at, e - s, 0)
+                at += ((e - s) + 63) // 64 * 64
+            nsw = (at + 31) // 32
+            new = (at + 63) // 64
+            eng.submit(slot, len(batch), nsw, new)
+            pending[slot] = batch
+        for slot in sorted(pending):
+            eng.wait(slot)
+            _collect(eng, slot, pending[slot], results)
+        rows = []
+        for t in range(len(tables)):
+            rr = []
+            for wi, (c, s, e) in enumerate(wins):
+                st = results[t][wi]
+                rr.append((ids[c], s, e) + st)
+            rows.append(rr)
+        return rows
+    finally:
+        eng.free()
+
+
+def _collect(eng: ScanEngine, slot: int, batch: List[int], results) -> None:
+    _, _, _, st = eng.slot_arrays(slot)
+    n = len(batch)
+    for t in range(len(results)):
+        part = st[t * n:(t + 1) * n]
+        for j, wi in enumerate(batch):
+            r = part[j]
+            results[t][wi] = (int(r["total_kmers"]), int(r["observed_kmers"]),
+                              int(r["variations"]), int(r["kmer_distance"]))
+
+
+def reference_name(filename: str) -> str:
+    """getReferenceName (src/FastIBS.cpp:14-25): cut at the first '_genome'."""
+    pos = filename.find("_genome")
+    return filename[:pos] if pos >= 0 else filename
+
+
+def output_name(db_folder: str, ref_file: str, W: int) -> str:
+    """<db folder name>_v_<reference name>_<W>.tsv (src/FastIBS.cpp:99)."""
+    return f"{os.path.basename(db_folder.rstrip('/'))}_v_{reference_name(ref_file)}_{W}.tsv"

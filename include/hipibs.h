@@ -1,0 +1,268 @@
+/*
+ * hipibs.h -- C ABI of the MI355X k-mer IBS scan library (libhipibs.so).
+ *
+ * Plain C types only: pointers, sizes, status codes. Every function returns
+ * 0 on success and a negative HIPIBS_E_* code on failure; the message of the
+ * last failure on the calling thread is hipibs_last_error(). Pointers named
+ * d_* are device pointers on the table's device; h_* are host pointers.
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *
+ * Data formats shared by every entry point:
+ *  - k-mer key: canonical 2-bit packed code, A=0 C=1 G=2 T=3, first base in
+ *    the most significant used bits, k <= 32 (src/KmerCodec.hpp:18-60).
+ *  - packed bases: base b of a stream lives at bits 2*(b%32) of 64-bit word
+ *    b/32 (code of a non-ACGT base is 0); its exception bit (set when the
+ *    byte is not one of ACGTacgt) is bit b%64 of word b/64 of a separate
+ *    array (src/gpu/EncodeProbeCore.hpp:40-50 uses the same two arrays).
+ *    Both arrays must carry HIPIBS_PACK_PAD_WORDS readable words past the
+ *    last base.
+ *  - window: a contiguous base range [base, base+n_bases) of the packed
+ *    stream; its k-mer positions are base .. base+n_bases-k (none when
+ *    n_bases < k), as cut by chunkSequences (src/Utils.hpp:296-327).
+ *  - window statistics: int32 total_kmers, observed_kmers, variations,
+ *    kmer_distance, with the run-length rule of
+ *    Window::addVariationOriginal (src/Window.hpp:29-36) applied to the
+ *    valid-position stream (src/KmerDatabase.hpp:546-673).
+ */
+#ifndef HIPIBS_H
+#define HIPIBS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HIPIBS_OK 0
+#define HIPIBS_E_ARG (-1)      /* invalid argument */
+#define HIPIBS_E_HIP (-2)      /* HIP runtime error */
+#define HIPIBS_E_IO (-3)       /* file missing, unreadable or malformed */
+#define HIPIBS_E_NOMEM (-4)    /* device or host allocation failed / budget */
+#define HIPIBS_E_FULL (-5)     /* table capacity exhausted during a build */
+#define HIPIBS_E_NODEV (-6)    /* no usable GPU */
+
+#define HIPIBS_PACK_PAD_WORDS 4
+
+typedef struct hipibs_table hipibs_table;
+typedef struct hipibs_scan hipibs_scan;
+
+/* Table construction options (zero-initialised fields take the defaults). */
+typedef struct hipibs_table_opts {
+    double load_factor;     /* keys / slots, default 0.6 */
+    uint32_t bucket_slots;  /* 8 (64 B bucket) or 16 (128 B bucket), default 8 */
+    int device;             /* HIP device ordinal, default 0 */
+    uint64_t hbm_budget;    /* max table bytes, 0 = no limit */
+    uint32_t load_threads;  /* host threads of a KMC read, 0 = all CPUs */
+} hipibs_table_opts;
+
+typedef struct hipibs_window {
+    uint64_t base;     /* first base of the window in the packed stream */
+    uint32_t n_bases;  /* window length in bases */
+    uint32_t reserved; /* must be 0 */
+} hipibs_window;
+
+typedef struct hipibs_window_stats {
+    int32_t total_kmers;
+    int32_t observed_kmers;
+    int32_t variations;
+    int32_t kmer_distance;
+} hipibs_window_stats;
+
+/* Header of a KMC database (<prefix>.kmc_pre), as KmcRaIndex::Header. */
+typedef struct hipibs_kmc_info {
+    uint32_t kmc_version;   /* 0 (KMC1) or 0x200 (KMC2) */
+    uint32_t kmer_length;
+    uint32_t mode;
+    uint32_t counter_size;
+    uint32_t lut_prefix_length;
+    uint32_t signature_len; /* 0 for KMC1 */
+    uint32_t min_count;
+    uint32_t max_count;
+    uint64_t total_kmers;
+} hipibs_kmc_info;
+
+/* Host slot of a streaming scan: pinned buffers the caller fills before
+ * hipibs_scan_submit and reads after hipibs_scan_wait. */
+typedef struct hipibs_host_slot {
+    uint64_t *seq_words;           /* packed bases (capacity seq_cap words) */
+    uint64_t *exc_words;           /* exception mask (capacity exc_cap words) */
+    hipibs_window *windows;        /* windows, bases relative to this slot */
+    hipibs_window_stats *stats;    /* n_tables x n_windows results, table-major */
+    uint64_t seq_cap, exc_cap;     /* capacities in words (pad included) */
+    uint32_t window_cap;
+    uint32_t n_tables;
+} hipibs_host_slot;
+
+/* ---- errors and devices ------------------------------------------------ */
+
+/* Message of the last failure on this thread ("" if none). */
+const char *hipibs_last_error(void);
+
+/* Replaces DeviceKmerIndex::deviceCount / deviceCountError
+ * (src/gpu/DeviceKmerIndex.cuh:30-40): number of usable GPUs in *n; on
+ * failure *n = 0 and the reason is copied to why (if non-NULL). */
+int hipibs_device_count(int *n, char *why, size_t why_len);
+
+/* Replaces DeviceKmerIndex::freeMemory (src/gpu/DeviceKmerIndex.cu:354-367). */
+int hipibs_device_mem(int device, uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* ---- KMC databases (host) ----------------------------------------------- */
+
+/* Replaces KmcRaIndex::readHeader (src/gpu/KmcRaIndex.hpp:104-110). */
+int hipibs_kmc_header(const char *prefix, hipibs_kmc_info *out);
+
+/* Replaces KmcRaIndex(prefix) + forEachKmer (src/gpu/KmcRaIndex.hpp:124-164):
+ * every k-mer of the database as its canonical code, in on-disk order.
+ * Writes at most cap keys; *n receives total_kmers. */
+int hipibs_kmc_read_keys(const char *prefix, uint64_t *h_keys, uint64_t cap,
+                         uint64_t *n, uint32_t threads);
+
+/* Writes a KMC database readable by CKMCFile (KMC/kmc_api/kmc_file.cpp:178-369)
+ * and KmcRaIndex: keys must be distinct canonical codes (any order),
+ * counts (nullable = all 1) are clamped to the counter width. version is 0
+ * (KMC1) or 0x200 (KMC2, sig_len 5..11); (k - p) % 4 == 0 is required.
+ * No counterpart in the reference (it reads databases written by KMC 3.2.4). */
+int hipibs_kmc_write(const char *prefix, const uint64_t *h_keys, const uint32_t *h_counts,
+                     uint64_t n, uint32_t k, uint32_t p, uint32_t sig_len,
+                     uint32_t counter_size, uint32_t version, uint32_t min_count,
+                     uint32_t max_count);
+
+/* ---- packing (host) ------------------------------------------------------ */
+
+/* Packs n bases of text into the 2-bit + exception layout, starting at base
+ * offset `at` of the destination arrays. Words that also hold bases before
+ * `at` are OR-ed into, so their bits for bases at..  must be zero (true for
+ * consecutive calls into a zeroed buffer); all other words are overwritten.
+ * The (tail) words past the last base keep unspecified bits for later bases. Same base table as KmerCodec::baseCode
+ * (src/KmerCodec.hpp:105-118) and packWindow (src/gpu/EncodeProbeCore.hpp:157-186). */
+int hipibs_pack_bases(const char *h_text, uint64_t n, uint64_t at, uint64_t *h_seq_words,
+                      uint64_t *h_exc_words);
+
+/* ---- device tables ------------------------------------------------------- */
+
+/* Replaces KmcRaIndex(prefix) + DeviceKmerIndex::buildShards
+ * (src/gpu/KmcRaIndex.hpp:124-135, src/gpu/DeviceKmerIndex.cu:144-183):
+ * reads the database and builds a device-resident bucketised open-addressing
+ * table of its canonical keys. */
+int hipibs_table_build_kmc(const char *prefix, const hipibs_table_opts *opts,
+                           hipibs_table **out);
+
+/* Builds a table from host keys (duplicates are stored once). The analogue
+ * of DeviceHashIndex's build from an enumeration (src/gpu/DeviceHashIndex.cu:121-153). */
+int hipibs_table_build_keys(const uint64_t *h_keys, uint64_t n, uint32_t k,
+                            const hipibs_table_opts *opts, hipibs_table **out);
+
+/* Same from device keys on `stream` (synchronises before returning). */
+int hipibs_table_build_keys_device(const uint64_t *d_keys, uint64_t n, uint32_t k,
+                                   const hipibs_table_opts *opts, hipibs_table **out,
+                                   void *stream);
+
+/* Replaces DeviceKmerIndex's destructor. NULL is accepted. */
+int hipibs_table_free(hipibs_table *t);
+
+/* Replaces DeviceKmerIndex::footprintBytes / view (src/gpu/DeviceKmerIndex.cuh:86-110). */
+int hipibs_table_info(const hipibs_table *t, uint64_t *n_keys, uint64_t *bytes,
+                      uint64_t *n_buckets, uint32_t *k, uint32_t *bucket_slots, int *device);
+
+/* Replaces DeviceKmerIndex::probe(d_q, d_valid, n, d_hits, stream)
+ * (src/gpu/DeviceKmerIndex.cuh:60-66): d_hits[i] = 1 if d_valid[i] (NULL =
+ * all valid) and d_codes[i] is in the table, else 0. Asynchronous. */
+int hipibs_table_probe(const hipibs_table *t, const uint64_t *d_codes, const uint8_t *d_valid,
+                       uint64_t n, uint8_t *d_hits, void *stream);
+
+/* Replaces DeviceKmerIndex::probeCountHost (src/gpu/DeviceKmerIndex.cu:264-306):
+ * number of host keys present in the table. Thread-safe; synchronous. */
+int hipibs_table_probe_count(const hipibs_table *t, const uint64_t *h_keys, uint64_t n,
+                             uint64_t *out);
+
+/* Device-key variant of hipibs_table_probe_count; the count is written to
+ * d_count (a device uint64, accumulated: the caller zeroes it). Asynchronous. */
+int hipibs_table_probe_count_device(const hipibs_table *t, const uint64_t *d_keys, uint64_t n,
+                                    uint64_t *d_count, void *stream);
+
+/* ---- the fused scan (device-resident inputs) ------------------------------ */
+
+/* Fused encode + canonicalise + probe + per-window run-length reduction over
+ * device-resident packed bases: for every table i and window j,
+ * d_stats[i*n_windows + j] receives the window's statistics against table i
+ * (the device half of EncodeProbeEngine::submit + BitStats::run +
+ * Window::addVariationOriginal: src/gpu/EncodeProbe.cu:152-193,
+ * src/gpu/EncodeProbeCore.hpp:338-370, src/Window.hpp:29-36). All tables
+ * must share k, bucket width and device. Every window must lie inside
+ * [0, n_bases) and be at most max_window_bases long. segs_per_window splits
+ * each window's positions into that many block work units (0 = chosen by
+ * the library); when it resolves to more than 1, d_work must hold
+ * hipibs_scan_work_bytes() bytes of scratch. Asynchronous on `stream`. */
+int hipibs_scan_windows_device(const hipibs_table *const *tables, int n_tables,
+                               const uint64_t *d_seq_words, const uint64_t *d_exc_words,
+                               uint64_t n_bases, const hipibs_window *d_windows,
+                               uint32_t n_windows, uint32_t max_window_bases,
+                               hipibs_window_stats *d_stats, uint32_t segs_per_window,
+                               void *d_work, uint64_t work_bytes, void *stream);
+
+/* Scratch bytes of hipibs_scan_windows_device for these arguments (0 when
+ * the resolved split is one segment per window); *segs_resolved (nullable)
+ * receives the split the library will use. */
+uint64_t hipibs_scan_work_bytes(uint32_t n_windows, int n_tables, uint32_t max_window_bases,
+                                uint32_t k, uint32_t segs_per_window, uint32_t *segs_resolved);
+
+/* Per-position hit mask over positions [pos0, pos0+n_pos) of a packed
+ * stream: bit (p-pos0)%32 of word (p-pos0)/32 of d_hit_words is set when
+ * position p is a valid k-mer of the stream's contig (all k bases ACGT and
+ * p+k <= contig_end) found in the table. The device half of the mapper
+ * (src/KmerDatabase.hpp:1544-1581). Asynchronous. */
+int hipibs_hitmask_device(const hipibs_table *t, const uint64_t *d_seq_words,
+                          const uint64_t *d_exc_words, uint64_t pos0, uint64_t n_pos,
+                          uint64_t contig_end, uint32_t *d_hit_words, void *stream);
+
+/* Canonical code of every position p in [pos0, pos0+n_pos) of a packed
+ * stream: d_codes[p-pos0] = canonical code when the k bases at p are all
+ * ACGT and p+k <= contig_end, else all ones; d_valid (nullable) gets 1/0.
+ * The encode half of the fused kernel on its own, as
+ * KmerCodec::encodePositions (src/KmerCodec.hpp:67-100). Asynchronous. */
+int hipibs_encode_device(const uint64_t *d_seq_words, const uint64_t *d_exc_words, uint64_t pos0,
+                         uint64_t n_pos, uint64_t contig_end, uint32_t k, uint64_t *d_codes,
+                         uint8_t *d_valid, void *stream);
+
+/* Per-base coverage of one contig from its hit mask (positions 0..n_pos-1,
+ * n_pos = n_bases-k+1, or 0): d_cov[i] = number of set positions p with
+ * p <= i <= p+k-1, i < n_bases (src/KmerDatabase.hpp:756-802). Asynchronous. */
+int hipibs_coverage_device(const uint32_t *d_hit_words, uint64_t n_pos, uint64_t n_bases,
+                           uint32_t k, uint8_t *d_cov, void *stream);
+
+/* Timing of the last scan kernel launched by this thread, in ms, measured
+ * with HIP events on the launch stream (valid after the stream completes;
+ * negative when profiling is off). hipibs_set_profiling(1) turns it on. */
+int hipibs_set_profiling(int on);
+double hipibs_last_kernel_ms(void);
+
+/* ---- streaming scan engine (host buffers, pinned, double-buffered) -------- */
+
+/* Replaces the EncodeProbeEngine constructors + caps + host(slot)
+ * (src/gpu/EncodeProbe.cuh:46-120): `depth` slots of pinned host and device
+ * buffers sized for batch_bases bases and max_windows windows against
+ * n_tables tables (all with the same k and device). */
+int hipibs_scan_create(const hipibs_table *const *tables, int n_tables, uint64_t batch_bases,
+                       uint32_t max_windows, int depth, hipibs_scan **out);
+
+/* The pinned host buffers of slot `slot` (valid until hipibs_scan_free). */
+int hipibs_scan_slot(hipibs_scan *s, int slot, hipibs_host_slot *out);
+
+/* Replaces EncodeProbeEngine::submit (src/gpu/EncodeProbe.cu:941-1022):
+ * uploads n_seq_words / n_exc_words words and n_windows windows of the
+ * slot, runs the fused scan, and downloads the statistics, all
+ * asynchronously on the slot's stream. The slot must not be in flight. */
+int hipibs_scan_submit(hipibs_scan *s, int slot, uint32_t n_windows, uint64_t n_seq_words,
+                       uint64_t n_exc_words);
+
+/* Replaces EncodeProbeEngine::wait (src/gpu/EncodeProbe.cu:1144-1154). */
+int hipibs_scan_wait(hipibs_scan *s, int slot);
+
+int hipibs_scan_free(hipibs_scan *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPIBS_H */
