@@ -1,0 +1,371 @@
+"""Benchmark of the fused k-mer IBS scan on one MI355X.
+
+Workload (single-sample drop-in, k=25): a synthetic 2 Gbp reference
+(uniform ACGT, 3% N runs, 40% of the sequence copied from 500 repeat
+families of 1-10 kbp at 85-99% identity) scanned in 50 kbp windows against
+one sample table of ~2e9 distinct canonical 25-mers (about 80% from the
+k-mers of a 1%-SNP haplotype of the reference, the rest random), 8 B per
+key at load 0.6. Everything is generated on the GPU from fixed seeds.
+
+A step is one fused scan of the whole reference (every window, every
+valid position encoded, canonicalised, probed and reduced to per-window
+statistics) with the packed reference and the table resident in HBM.
+
+Prints one JSON line (stdout). Progress goes to stderr.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "gpu-ibs_amd"), os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+K = 25
+W = 50000
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--gbp", type=float, default=2.0, help="reference length in Gbp")
+    ap.add_argument("--keys", type=float, default=2.0e9, help="target distinct keys")
+    ap.add_argument("--bucket-slots", type=int, default=8)
+    ap.add_argument("--load-factor", type=float, default=0.6)
+    ap.add_argument("--segs", type=int, default=0, help="segments per window (0 = auto)")
+    ap.add_argument("--cpu-mbp", type=float, default=256.0,
+                    help="reference slice (Mbp) of the CPU baseline sample; 0 = skip")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=0x1B50200)
+    return ap.parse_args()
+
+
+# ---- synthetic data on the GPU ---------------------------------------------------
+
+def gen_reference(torch, n, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    dev = "cuda"
+    codes = torch.randint(0, 4, (n,), dtype=torch.uint8, device=dev, generator=g)
+    # 500 repeat families, lengths U[1k, 10k], copied over ~40% of the sequence
+    fam_len = torch.randint(1000, 10001, (500,), device=dev, generator=g)
+    fam_off = torch.cumsum(fam_len, 0) - fam_len
+    fam = torch.randint(0, 4, (int(fam_len.sum()),), dtype=torch.uint8, device=dev, generator=g)
+    n_copies = int(0.40 * n / float(fam_len.float().mean()))
+    region = n // n_copies
+    f = torch.randint(0, 500, (n_copies,), device=dev, generator=g)
+    lens = fam_len[f]
+    slack = (region - lens).clamp(min=1)
+    starts = torch.arange(n_copies, device=dev) * region + (
+        torch.rand(n_copies, device=dev, generator=g) * slack).long()
+    ident = 0.85 + 0.14 * torch.rand(n_copies, device=dev, generator=g)
+    chunk = 20000
+    for c0 in range(0, n_copies, chunk):
+        sl = slice(c0, min(n_copies, c0 + chunk))
+        L = lens[sl]
+        rep = torch.repeat_interleave(torch.arange(L.numel(), device=dev), L)
+        first = torch.cumsum(L, 0) - L
+        within = torch.arange(int(L.sum()), device=dev) - first[rep]
+        pos = starts[sl][rep] + within
+        src = fam_off[f[sl]][rep] + within
+        vals = fam[src]
+        mut = torch.rand(pos.numel(), device=dev, generator=g) > ident[sl][rep]
+        rnd = torch.randint(0, 4, (pos.numel(),), dtype=torch.uint8, device=dev, generator=g)
+        codes[pos] = torch.where(mut, rnd, vals)
+    # N runs over ~3% of the sequence, lengths U[100, 5000]
+    n_runs = int(0.03 * n / 2550)
+    rl = torch.randint(100, 5001, (n_runs,), device=dev, generator=g)
+    rs = (torch.rand(n_runs, device=dev, generator=g) * (n - 5001)).long()
+    for c0 in range(0, n_runs, 20000):
+        sl = slice(c0, min(n_runs, c0 + 20000))
+        L = rl[sl]
+        rep = torch.repeat_interleave(torch.arange(L.numel(), device=dev), L)
+        first = torch.cumsum(L, 0) - L
+        within = torch.arange(int(L.sum()), device=dev) - first[rep]
+        codes[rs[sl][rep] + within] = 4
+    return codes
+
+
+def pack(torch, codes):
+    """2-bit words (32 bases / int64) + exception words (64 bases / int64),
+    with the library's pad words."""
+    import hipibs
+    n = codes.numel()
+    assert n % 64 == 0
+    ns, ne = hipibs.pack_words(n)
+    sw = torch.zeros(ns, dtype=torch.int64, device="cuda")
+    ew = torch.zeros(ne, dtype=torch.int64, device="cuda")
+    step = 1 << 28
+    for b0 in range(0, n, step):
+        c = codes[b0:b0 + step].long()
+        m = c.numel()
+        c32 = (c & 3).view(m // 32, 32)
+        w = torch.zeros(m // 32, dtype=torch.int64, device="cuda")
+        for j in range(32):
+            w |= c32[:, j] << (2 * j)
+        sw[b0 // 32:b0 // 32 + m // 32] = w
+        bad = (c > 3).long().view(m // 64, 64)
+        e = torch.zeros(m // 64, dtype=torch.int64, device="cuda")
+        for j in range(64):
+            e |= bad[:, j] << j
+        ew[b0 // 64:b0 // 64 + m // 64] = e
+        del c, c32, w, bad, e
+    return sw, ew
+
+
+def encode(torch, sw, ew, n, k):
+    """Canonical code per position of one contig (-1 where invalid)."""
+    import hipibs
+    npos = n - k + 1
+    out = torch.empty(npos, dtype=torch.int64, device="cuda")
+    hipibs._check(hipibs._lib.hipibs_encode_device(sw.data_ptr(), ew.data_ptr(), 0, npos, n, k,
+                                                   out.data_ptr(), None, None), "encode")
+    return out
+
+
+def random_canonical(torch, m, k, g):
+    x = torch.randint(0, 1 << (2 * k), (m,), dtype=torch.int64, device="cuda", generator=g)
+    rc = torch.zeros_like(x)
+    for j in range(k):
+        rc = (rc << 2) | (3 - ((x >> (2 * j)) & 3))
+    return torch.minimum(x, rc)
+
+
+def make_sample_keys(torch, hap_codes, keep_p, n_random, k, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed + 7)
+    keep = (hap_codes >= 0) & (torch.rand(hap_codes.numel(), device="cuda", generator=g) < keep_p)
+    keys = hap_codes[keep]
+    rnd = random_canonical(torch, n_random, k, g)
+    keys = torch.unique(torch.cat([keys, rnd]))
+    return keys
+
+
+# ---- CPU baseline (the reference's CPU-BATCHED scan, compiled from its sources) ----
+
+def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_check):
+    import numpy as np
+
+    import hipibs
+    lib_path = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+    if not os.path.exists(lib_path):
+        log("cpu baseline skipped: oracle/_ref/libref.so absent")
+        return None, None
+    ref = C.CDLL(lib_path)
+    ref.ref_index_open.argtypes = [C.c_char_p]
+    ref.ref_index_open.restype = C.c_void_p
+    ref.ref_index_close.argtypes = [C.c_void_p]
+    ref.ref_batched_scan.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_longlong,
+                                     C.c_int, C.c_int, C.c_void_p]
+    ref.ref_batched_scan.restype = C.c_double
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    frac = n_slice / ref_codes.numel()
+    keys = make_sample_keys(torch, hap_codes_slice, args.keep_p, int(0.2 * args.keys * frac),
+                            K, args.seed + 99)
+    hk = keys.cpu().numpy().view(np.uint64)
+    log(f"cpu sample: {n_slice/1e6:.0f} Mbp slice, {hk.size/1e6:.1f} M keys -> KMC2 database")
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
+        pre = os.path.join(td, "cpu_sample")
+        hipibs.kmc_write(pre, hk, K, p=9, sig_len=9, counter_size=1, version=0x200,
+                         min_count=1, max_count=255)
+        t0 = time.time()
+        h = ref.ref_index_open(pre.encode())
+        if not h:
+            raise RuntimeError("reference KmcRaIndex failed to load the sample database")
+        log(f"reference KmcRaIndex loaded in {time.time()-t0:.1f} s")
+        text = np.frombuffer(b"ACGTN", dtype=np.uint8)[ref_codes[:n_slice].cpu().numpy()].tobytes()
+        wins = hipibs.chunk_sequences([n_slice], W, K)
+        ws = np.array([w[1] for w in wins], dtype=np.int64)
+        wl = np.array([w[2] - w[1] for w in wins], dtype=np.int64)
+        stats = np.zeros((len(wins), 4), dtype=np.int32)
+        secs = ref.ref_batched_scan(h, text, ws.ctypes.data, wl.ctypes.data, len(wins), 16,
+                                    threads, stats.ctypes.data)
+        ref.ref_index_close(h)
+    probes = int(stats[:, 0].sum())
+    log(f"cpu baseline: {probes/1e6:.1f} M probes in {secs:.2f} s on {threads} threads")
+    # same sample on the GPU: must be identical
+    t = hipibs.Table.from_device_keys(keys.data_ptr(), keys.numel(), K)
+    gst = gpu_check(t, n_slice, wins)
+    t.free()
+    match = bool(np.array_equal(gst, stats))
+    log(f"cpu/gpu statistics identical on the sample: {match}")
+    lscpu = ""
+    try:
+        import subprocess
+        lscpu = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        lscpu = [l.split(":", 1)[1].strip() for l in lscpu.splitlines()
+                 if l.startswith("Model name")][0]
+    except Exception:
+        pass
+    return ({"value": probes / secs, "unit": "ref k-mers probed/s", "cores": threads,
+             "kind": "reference",
+             "sample": f"{n_slice/1e6:.0f} Mbp slice of the reference vs a {hk.size/1e6:.1f}M-key "
+                       f"KMC2 database of the same construction; reference CPU-BATCHED "
+                       f"(KmcRaIndex + probeBatched K=16, codec, run-length stats), "
+                       f"{threads} threads, {lscpu}".strip()},
+            match)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import hipibs
+    assert torch.cuda.is_available(), "no GPU visible"
+    n = int(args.gbp * 1e9) // 64 * 64
+    t0 = time.time()
+    log(f"generating {n/1e9:.2f} Gbp reference")
+    ref_codes = gen_reference(torch, n, args.seed)
+    sw, ew = pack(torch, ref_codes)
+    torch.cuda.synchronize()
+    log(f"reference ready ({time.time()-t0:.1f} s)")
+    # haplotype (1% SNPs) and its k-mers
+    g = torch.Generator(device="cuda")
+    g.manual_seed(args.seed + 1)
+    hap = ref_codes.clone()
+    snp = (torch.rand(n, device="cuda", generator=g) < 0.01) & (hap < 4)
+    hap[snp] = (hap[snp] + torch.randint(1, 4, (n,), dtype=torch.uint8, device="cuda",
+                                         generator=g)[snp]) % 4
+    del snp
+    hsw, hew = pack(torch, hap)
+    del hap
+    hap_codes = encode(torch, hsw, hew, n, K)
+    del hsw, hew
+    torch.cuda.synchronize()
+    n_valid_hap = int((hap_codes >= 0).sum())
+    keep_p = min(1.0, 0.8 * args.keys / max(1, n_valid_hap))
+    args.keep_p = keep_p
+    cpu_slice = int(args.cpu_mbp * 1e6) // 64 * 64 if args.cpu_mbp > 0 else 0
+    hap_slice = hap_codes[:cpu_slice - K + 1].clone() if cpu_slice else None
+    keys = make_sample_keys(torch, hap_codes, keep_p, int(0.2 * args.keys), K, args.seed)
+    del hap_codes
+    torch.cuda.synchronize()
+    log(f"sample keys: {keys.numel()/1e9:.3f} G distinct ({time.time()-t0:.1f} s)")
+    tb = time.time()
+    table = hipibs.Table.from_device_keys(keys.data_ptr(), keys.numel(), K,
+                                          bucket_slots=args.bucket_slots,
+                                          load_factor=args.load_factor)
+    build_s = time.time() - tb
+    info = table.info()
+    n_keys = keys.numel()
+    del keys
+    torch.cuda.empty_cache()
+    log(f"table: {info['n_keys']/1e9:.3f} G keys, {info['bytes']/1e9:.1f} GB, build {build_s:.2f} s")
+
+    wins = hipibs.chunk_sequences([n], W, K)
+    win = np.zeros(len(wins), dtype=hipibs.WINDOW_DTYPE)
+    win["base"] = np.array([w[1] for w in wins], dtype=np.uint64)
+    win["n_bases"] = np.array([w[2] - w[1] for w in wins], dtype=np.uint32)
+    d_win = torch.from_numpy(win.view(np.uint8)).cuda()
+    d_stats = torch.zeros(len(wins) * 4, dtype=torch.int32, device="cuda")
+    segs = C.c_uint32(0)
+    wb = hipibs._lib.hipibs_scan_work_bytes(len(wins), 1, W, K, args.segs, C.byref(segs))
+    d_work = torch.zeros(max(wb, 8), dtype=torch.uint8, device="cuda")
+    tabs = hipibs._table_array([table])
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        hipibs._check(hipibs._lib.hipibs_scan_windows_device(
+            tabs, 1, sw.data_ptr(), ew.data_ptr(), n, d_win.data_ptr(), len(wins), W,
+            d_stats.data_ptr(), args.segs, d_work.data_ptr(), wb, stream), "scan")
+
+    hipibs._lib.hipibs_set_profiling(1)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    first = d_stats.cpu().numpy().reshape(-1, 4).copy()
+    kms = []
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        kms.append(hipibs._lib.hipibs_last_kernel_ms())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - ts
+    st = d_stats.cpu().numpy().reshape(-1, 4)
+    stable = bool(np.array_equal(st, first))
+    probes = int(st[:, 0].astype(np.int64).sum())
+    observed = int(st[:, 1].astype(np.int64).sum())
+    ms = elapsed / args.steps * 1e3
+    k_ms = float(np.mean(kms))
+    alg_bytes = 64.0 * probes + 0.375 * n
+    log(f"scan: {ms:.2f} ms/step, kernel {k_ms:.2f} ms, {probes/1e9:.3f} G probes, "
+        f"hit rate {observed/max(1,probes):.3f}, repeat-stable {stable}")
+
+    cpu = None
+    cpu_match = None
+    if cpu_slice:
+        def gpu_check(t, n_slice, swins):
+            wv = np.zeros(len(swins), dtype=hipibs.WINDOW_DTYPE)
+            wv["base"] = np.array([w[1] for w in swins], dtype=np.uint64)
+            wv["n_bases"] = np.array([w[2] - w[1] for w in swins], dtype=np.uint32)
+            dw = torch.from_numpy(wv.view(np.uint8)).cuda()
+            ds = torch.zeros(len(swins) * 4, dtype=torch.int32, device="cuda")
+            s2 = C.c_uint32(0)
+            wb2 = hipibs._lib.hipibs_scan_work_bytes(len(swins), 1, W, K, 0, C.byref(s2))
+            dwk = torch.zeros(max(wb2, 8), dtype=torch.uint8, device="cuda")
+            hipibs._check(hipibs._lib.hipibs_scan_windows_device(
+                hipibs._table_array([t]), 1, sw.data_ptr(), ew.data_ptr(), n_slice,
+                dw.data_ptr(), len(swins), W, ds.data_ptr(), 0, dwk.data_ptr(), wb2, None),
+                "scan")
+            torch.cuda.synchronize()
+            return ds.cpu().numpy().reshape(-1, 4)
+        try:
+            cpu, cpu_match = cpu_baseline(torch, ref_codes, sw, ew, hap_slice, cpu_slice, args,
+                                          gpu_check)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline failed: {e}")
+    value = probes / (ms / 1e3)
+    out = {
+        "metric": "ref k-mers probed/s",
+        "value": value,
+        "unit": "ref k-mers probed/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (GPU-generated, seeded)",
+        "config": {"workload": "single-sample scan, k=25: 2 Gbp synthetic reference "
+                               "(3% N, 40% repeat-derived) vs ~2e9-key sample, W=50 kbp",
+                   "reference_bases": n, "windows": len(wins), "k": K, "window": W,
+                   "sample_keys": int(info["n_keys"]), "table_bytes": int(info["bytes"]),
+                   "bucket_slots": info["bucket_slots"], "load_factor": args.load_factor,
+                   "segs_per_window": segs.value, "scan_wall_s": ms / 1e3,
+                   "table_build_s": build_s, "probes_per_step": probes,
+                   "hit_rate": observed / max(1, probes), "repeat_stable": stable,
+                   "cpu_gpu_sample_identical": cpu_match},
+        "roofline": {"bound": "hbm", "achieved": alg_bytes / (k_ms / 1e3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg_bytes / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "scanKernel<8>", "kernel_ms": k_ms,
+                     "algorithmic_bytes": alg_bytes,
+                     "probe_rate_gps": probes / (k_ms / 1e3) / 1e9},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    table.free()
+
+
+if __name__ == "__main__":
+    main()

@@ -414,37 +414,108 @@ int kmcWrite(const std::string &prefix, const uint64_t *keys, const uint32_t *co
         for (size_t s = 0; s < sigMap.size(); ++s)
             sigMap[s] = (uint32_t)(s % nBins);
     }
-    // (bin, key) order == (bin, prefix, suffix) order
-    std::vector<std::pair<uint64_t, uint64_t>> order(n); // (bin << 32 | idx-free key), index
-    std::vector<uint32_t> binOf(n);
+    // Records sorted by (bin, key) == (bin, prefix, suffix): bins computed
+    // in parallel, a counting sort by bin, then each bin sorted on its own.
+    struct Rec
+    {
+        uint64_t key;
+        uint64_t count;
+    };
     const uint64_t kmask = k == 32 ? ~0ull : (1ull << (2 * k)) - 1;
-    for (uint64_t i = 0; i < n; ++i)
-    {
-        if (keys[i] & ~kmask)
-            return setError(HIPIBS_E_ARG, "kmc_write: key wider than 2k bits");
-        binOf[i] = version == 0x200 ? sigMap[signatureOf(keys[i], k, sigLen, norm)] : 0;
-        order[i] = {keys[i], i};
-    }
-    std::sort(order.begin(), order.end(),
-              [&](const std::pair<uint64_t, uint64_t> &a, const std::pair<uint64_t, uint64_t> &b)
-              {
-                  const uint32_t ba = binOf[a.second], bb = binOf[b.second];
-                  return ba != bb ? ba < bb : a.first < b.first;
-              });
-    for (uint64_t i = 1; i < n; ++i)
-        if (order[i].first == order[i - 1].first)
-            return setError(HIPIBS_E_ARG, "kmc_write: duplicate key");
     const uint32_t shift = 2 * (k - p);
-    std::vector<uint64_t> lut((uint64_t)nBins * single + 1, 0);
+    auto prefixOf = [shift](uint64_t key) { return shift >= 64 ? 0ull : key >> shift; };
+    const uint64_t sufMask = shift >= 64 ? ~0ull : ((1ull << shift) - 1);
+    unsigned T = std::max(1u, std::min(usableCpus(), 64u));
+    if (n < (1ull << 20))
+        T = 1;
+    std::vector<uint16_t> binOf(n);
+    std::vector<std::vector<uint64_t>> hist(T, std::vector<uint64_t>(nBins, 0));
+    std::atomic<bool> wide{false};
+    auto parallel = [T](auto &&fn)
     {
-        std::vector<uint64_t> cnt((uint64_t)nBins * single, 0);
-        for (auto &o : order)
-            cnt[(uint64_t)binOf[o.second] * single + (o.first >> shift)] += 1;
-        uint64_t acc = 0;
-        for (size_t j = 0; j < cnt.size(); ++j)
+        if (T == 1)
         {
+            fn(0u);
+            return;
+        }
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < T; ++t)
+            pool.emplace_back(fn, t);
+        for (auto &th : pool)
+            th.join();
+    };
+    const uint64_t per = (n + T - 1) / T;
+    parallel([&](unsigned t)
+             {
+                 const uint64_t lo = std::min<uint64_t>(n, (uint64_t)t * per);
+                 const uint64_t hi = std::min<uint64_t>(n, lo + per);
+                 for (uint64_t i = lo; i < hi; ++i)
+                 {
+                     if (keys[i] & ~kmask)
+                         wide = true;
+                     const uint32_t b =
+                         version == 0x200 ? sigMap[signatureOf(keys[i] & kmask, k, sigLen, norm)] : 0;
+                     binOf[i] = (uint16_t)b;
+                     hist[t][b] += 1;
+                 }
+             });
+    if (wide)
+        return setError(HIPIBS_E_ARG, "kmc_write: key wider than 2k bits");
+    std::vector<uint64_t> binStart(nBins + 1, 0);
+    for (uint32_t b = 0; b < nBins; ++b)
+    {
+        uint64_t c = 0;
+        for (unsigned t = 0; t < T; ++t)
+            c += hist[t][b];
+        binStart[b + 1] = binStart[b] + c;
+    }
+    std::vector<Rec> recs(n);
+    parallel([&](unsigned t)
+             {
+                 std::vector<uint64_t> pos(nBins);
+                 for (uint32_t b = 0; b < nBins; ++b)
+                 {
+                     uint64_t o = binStart[b];
+                     for (unsigned u = 0; u < t; ++u)
+                         o += hist[u][b];
+                     pos[b] = o;
+                 }
+                 const uint64_t lo = std::min<uint64_t>(n, (uint64_t)t * per);
+                 const uint64_t hi = std::min<uint64_t>(n, lo + per);
+                 for (uint64_t i = lo; i < hi; ++i)
+                     recs[pos[binOf[i]]++] = Rec{keys[i], counts ? counts[i] : 1ull};
+             });
+    std::vector<uint16_t>().swap(binOf);
+    std::vector<uint64_t> lut((uint64_t)nBins * single + 1, 0);
+    std::atomic<uint32_t> nextBin{0};
+    std::atomic<bool> dup{false};
+    parallel([&](unsigned)
+             {
+                 for (;;)
+                 {
+                     const uint32_t b = nextBin.fetch_add(1);
+                     if (b >= nBins)
+                         break;
+                     Rec *r0 = recs.data() + binStart[b], *r1 = recs.data() + binStart[b + 1];
+                     std::sort(r0, r1, [](const Rec &x, const Rec &y) { return x.key < y.key; });
+                     uint64_t *cnt = lut.data() + (uint64_t)b * single;
+                     for (Rec *r = r0; r < r1; ++r)
+                     {
+                         if (r > r0 && r->key == r[-1].key)
+                             dup = true;
+                         cnt[prefixOf(r->key)] += 1;
+                     }
+                 }
+             });
+    if (dup)
+        return setError(HIPIBS_E_ARG, "kmc_write: duplicate key");
+    {
+        uint64_t acc = 0;
+        for (size_t j = 0; j + 1 < lut.size(); ++j)
+        {
+            const uint64_t c = lut[j];
             lut[j] = acc;
-            acc += cnt[j];
+            acc += c;
         }
         lut.back() = acc;
     }
@@ -461,13 +532,12 @@ int kmcWrite(const std::string &prefix, const uint64_t *keys, const uint32_t *co
         std::vector<unsigned char> buf;
         buf.reserve(1 << 20);
         buf.insert(buf.end(), {'K', 'M', 'C', 'S'});
-        for (auto &o : order)
+        for (const Rec &o : recs)
         {
-            const uint64_t suf = o.first & ((shift == 64) ? ~0ull : ((1ull << shift) - 1));
+            const uint64_t suf = o.key & sufMask;
             for (int b = (int)sufBytes - 1; b >= 0; --b)
                 buf.push_back((unsigned char)(suf >> (8 * b)));
-            uint64_t c = counts ? counts[o.second] : 1;
-            c = std::min(c, cmax);
+            const uint64_t c = std::min<uint64_t>(o.count, cmax);
             for (uint32_t b = 0; b < counterSize; ++b)
                 buf.push_back((unsigned char)(c >> (8 * b)));
             if (buf.size() >= (1u << 20) - recBytes)

@@ -414,75 +414,97 @@ class ScanEngine:
 def scan_sequences(tables: Sequence[Table], ids: Sequence[bytes], seqs: Sequence[bytes], W: int,
                    batch_bases: int = 1 << 26, depth: int = 2) -> List[List[tuple]]:
     """Per-window statistics of every contig against every table through the
-    streaming engine; returns one list of TSV rows per table. Windows are
-    never split across batches (buildBatches, src/gpu/EncodeProbeCore.hpp:103-134)."""
+    streaming engine; returns one list of TSV rows per table.
+
+    A batch holds whole windows (never split, as buildBatches,
+    src/gpu/EncodeProbeCore.hpp:103-134); consecutive windows of a contig
+    share one packed region of the batch (they overlap by k bases), so a
+    batch costs 0.375 B per distinct base, not per window base."""
     k = tables[0].info()["k"]
     wins = chunk_sequences([len(s) for s in seqs], W, k)
-    max_win = max([e - s for _, s, e in wins], default=1)
-    batch_bases = max(batch_bases, max_win + 64)
-    # windows per batch: each window packed on its own (contiguous copy)
-    eng = ScanEngine(tables, batch_bases, max_windows=max(1, min(len(wins), 1 << 16)), depth=depth)
+    if not wins:
+        return [[] for _ in tables]
+    wc = np.array([w[0] for w in wins], dtype=np.int64)
+    ws = np.array([w[1] for w in wins], dtype=np.int64)
+    we = np.array([w[2] for w in wins], dtype=np.int64)
+    max_win = int((we - ws).max())
+    batch_bases = max(batch_bases, max_win + 128)
+    max_windows = int(min(len(wins), 1 << 18))
+    # group windows into batches of regions
+    batches = []  # list of (win_lo, win_hi, regions=[(contig, rs, re, first_win)])
+    lo = 0
+    regions = []
+    used = 0
+    for i in range(len(wins)):
+        c, s_, e_ = int(wc[i]), int(ws[i]), int(we[i])
+        if regions and regions[-1][0] == c and s_ >= regions[-1][1]:
+            grow = max(0, e_ - regions[-1][2])
+            extra = grow
+            new_region = False
+        else:
+            extra = (e_ - s_) + 64
+            new_region = True
+        if (used + extra + 64 > batch_bases or i - lo >= max_windows) and i > lo:
+            batches.append((lo, i, regions))
+            lo, regions, used = i, [], 0
+            new_region, extra = True, (e_ - s_) + 64
+        if new_region:
+            regions.append([c, s_, e_, i])
+        else:
+            regions[-1][2] = max(regions[-1][2], e_)
+        used += extra
+    batches.append((lo, len(wins), regions))
+
+    eng = ScanEngine(tables, batch_bases, max_windows=max_windows, depth=depth)
+    nt = len(tables)
+    stats = np.zeros((nt, len(wins), 4), dtype=np.int32)
     try:
-        results = [[None] * len(wins) for _ in tables]
-        batches = []
-        cur, used = [], 0
-        cap_w = eng.slots[0].window_cap
-        for wi, (c, s, e) in enumerate(wins):
-            need = ((e - s) + 63) // 64 * 64
-            if cur and (used + need > batch_bases or len(cur) >= cap_w):
-                batches.append(cur)
-                cur, used = [], 0
-            cur.append(wi)
-            used += need
-        if cur:
-            batches.append(cur)
         pending = {}
-        for bi, batch in enumerate(batches):
+        for bi, (blo, bhi, regs) in enumerate(batches):
             slot = bi % depth
             if slot in pending:
                 eng.wait(slot)
-                _collect(eng, slot, pending.pop(slot), results)
+                _collect(eng, slot, pending.pop(slot), stats)
             seq, exc, win, _ = eng.slot_arrays(slot)
             at = 0
-            seq[:] = 0
-            exc[:] = 0
-            for j, wi in enumerate(batch):
-                c, s, e = wins[wi]
-                text = seqs[c][s:e]
-                if text:
-                    _check(_lib.hipibs_pack_bases(bytes(text), len(text), at,
-                                                  seq.ctypes.data_as(u64p),
+            base_of = np.zeros(bhi - blo, dtype=np.int64)
+            for r, (c, rs, re, first) in enumerate(regs):
+                last = regs[r + 1][3] if r + 1 < len(regs) else bhi
+                n = re - rs
+                seq[at // 32:(at + n + 63) // 32 + 2] = 0
+                exc[at // 64:(at + n + 63) // 64 + 2] = 0
+                text = seqs[c][rs:re]
+                if n:
+                    _check(_lib.hipibs_pack_bases(bytes(text), n, at, seq.ctypes.data_as(u64p),
                                                   exc.ctypes.data_as(u64p)), "hipibs_pack_bases")
-                win[j] = (at, e - s, 0)
-                at += ((e - s) + 63) // 64 * 64
-            nsw = (at + 31) // 32
-            new = (at + 63) // 64
-            eng.submit(slot, len(batch), nsw, new)
-            pending[slot] = batch
+                base_of[first - blo:last - blo] = at - rs
+                at += (n + 63) // 64 * 64
+            nb = bhi - blo
+            wv = win[:nb]
+            wv["base"] = (ws[blo:bhi] + base_of).astype(np.uint64)
+            wv["n_bases"] = (we[blo:bhi] - ws[blo:bhi]).astype(np.uint32)
+            wv["reserved"] = 0
+            eng.submit(slot, nb, (at + 31) // 32, (at + 63) // 64)
+            pending[slot] = (blo, bhi)
         for slot in sorted(pending):
             eng.wait(slot)
-            _collect(eng, slot, pending[slot], results)
-        rows = []
-        for t in range(len(tables)):
-            rr = []
-            for wi, (c, s, e) in enumerate(wins):
-                st = results[t][wi]
-                rr.append((ids[c], s, e) + st)
-            rows.append(rr)
-        return rows
+            _collect(eng, slot, pending[slot], stats)
     finally:
         eng.free()
+    rows = []
+    for t in range(nt):
+        st = stats[t].tolist()
+        rows.append([(ids[int(wc[i])], int(ws[i]), int(we[i])) + tuple(st[i])
+                     for i in range(len(wins))])
+    return rows
 
 
-def _collect(eng: ScanEngine, slot: int, batch: List[int], results) -> None:
+def _collect(eng: ScanEngine, slot: int, rng: Tuple[int, int], stats: np.ndarray) -> None:
     _, _, _, st = eng.slot_arrays(slot)
-    n = len(batch)
-    for t in range(len(results)):
-        part = st[t * n:(t + 1) * n]
-        for j, wi in enumerate(batch):
-            r = part[j]
-            results[t][wi] = (int(r["total_kmers"]), int(r["observed_kmers"]),
-                              int(r["variations"]), int(r["kmer_distance"]))
+    lo, hi = rng
+    n = hi - lo
+    arr = st[:n * stats.shape[0]].view(np.int32).reshape(stats.shape[0], n, 4)
+    stats[:, lo:hi, :] = arr
 
 
 def reference_name(filename: str) -> str:

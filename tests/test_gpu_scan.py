@@ -1,0 +1,222 @@
+"""Parity of the HIP path with the reference (golden fixtures) and the
+oracle, through the C ABI. Bit-exact everywhere: the path is integer."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import corpus
+from util import ROOT, coverage_line, oracle, oracle_scan_rows, sha256, sorted_keys
+
+import hipibs
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def small_dbs(tmp_path_factory, small_corpus):
+    d = tmp_path_factory.mktemp("small")
+    _, samples = small_corpus
+    out = []
+    for i, (keys, counts) in enumerate(samples):
+        p = str(d / f"s{i}")
+        hipibs.kmc_write(p, keys, 25, counts=counts, p=9, sig_len=9, counter_size=1,
+                         version=0x200, min_count=2, max_count=255)
+        out.append(p)
+    return out
+
+
+def test_device_present():
+    n, why = hipibs.device_count()
+    assert n >= 1, why
+
+
+def test_small_scan_matches_golden(small_corpus, small_dbs, expected):
+    ref, samples = small_corpus
+    seq = corpus.to_text(ref)
+    for i, p in enumerate(small_dbs):
+        note(f"small scan s{i}")
+        t = hipibs.Table.from_kmc(p)
+        info = t.info()
+        assert info["n_keys"] == samples[i][0].size
+        rows = hipibs.scan_sequences([t], [b"chr1"], [seq], 50000)[0]
+        tsv = hipibs.format_tsv(rows)
+        with open(os.path.join(ROOT, "tests", "golden", f"small_s{i}_50000.tsv"), "rb") as f:
+            assert tsv == f.read()
+        t.free()
+
+
+def test_panel_scan_equals_single(small_corpus, small_dbs):
+    """One launch over three tables == three single-table scans."""
+    ref, _ = small_corpus
+    seq = corpus.to_text(ref)
+    tabs = [hipibs.Table.from_kmc(p) for p in small_dbs]
+    multi = hipibs.scan_sequences(tabs, [b"chr1"], [seq], 10000)
+    for i, t in enumerate(tabs):
+        single = hipibs.scan_sequences([t], [b"chr1"], [seq], 10000)[0]
+        assert multi[i] == single
+    for t in tabs:
+        t.free()
+
+
+@pytest.mark.parametrize("k", [25, 31, 32, 15])
+def test_edge_scans_match_golden(k, expected):
+    recs, keys = corpus.edge_config(k)
+    ids = [r[0] for r in recs]
+    seqs = [r[1] for r in recs]
+    t = hipibs.Table.from_keys(keys, k)
+    assert t.info()["n_keys"] == keys.size
+    for W, want in expected["edge"][str(k)]["tsv"].items():
+        note(f"edge k={k} W={W}")
+        rows = hipibs.scan_sequences([t], ids, seqs, int(W), batch_bases=1 << 22)[0]
+        tsv = hipibs.format_tsv(rows)
+        assert (sha256(tsv), len(tsv)) == (want["sha256"], want["bytes"]), (k, W)
+    t.free()
+
+
+@pytest.mark.parametrize("slots,lf", [(8, 0.6), (16, 0.6), (8, 0.9), (16, 0.95), (8, 0.3)])
+def test_table_geometry_does_not_change_results(small_corpus, slots, lf):
+    ref, samples = small_corpus
+    seq = corpus.to_text(ref)
+    keys = samples[1][0]
+    note(f"geometry slots={slots} lf={lf}")
+    t = hipibs.Table.from_keys(keys, 25, bucket_slots=slots, load_factor=lf)
+    rows = hipibs.scan_sequences([t], [b"chr1"], [seq], 5000)[0]
+    want = oracle_scan_rows([b"chr1"], [seq], 5000, 25, sorted_keys(keys))
+    assert rows == want
+    t.free()
+
+
+def test_probe_count_intersect(small_corpus, expected):
+    _, samples = small_corpus
+    tabs = [hipibs.Table.from_keys(s[0], 25) for s in samples]
+    for pair, want in expected["small"]["intersect"].items():
+        a, b = map(int, pair.split("-"))
+        small, big = (a, b) if samples[a][0].size <= samples[b][0].size else (b, a)
+        assert tabs[big].probe_count(samples[small][0]) == want
+        assert tabs[small].probe_count(samples[big][0]) == want
+    # self-intersection is the whole set; random codes mostly miss
+    assert tabs[0].probe_count(samples[0][0]) == samples[0][0].size
+    rnd = corpus.splitmix(3, 3, 200000) & np.uint64((1 << 50) - 1)
+    rnd = np.minimum(rnd, corpus.revcomp(rnd, 25))
+    want = int(np.isin(rnd, samples[0][0]).sum())
+    assert tabs[0].probe_count(rnd) == want
+    for t in tabs:
+        t.free()
+
+
+def test_duplicate_keys_stored_once():
+    keys = corpus.splitmix(5, 5, 100000) & np.uint64((1 << 50) - 1)
+    dup = np.concatenate([keys, keys[::3], keys[::7]])
+    t = hipibs.Table.from_keys(dup, 25, load_factor=0.9)
+    assert t.info()["n_keys"] == np.unique(keys).size
+    assert t.probe_count(keys) == keys.size
+    t.free()
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def test_coverage_matches_golden(small_corpus, expected):
+    torch = _torch()
+    ref, samples = small_corpus
+    seq = corpus.to_text(ref)
+    keys = samples[0][0]
+    t = hipibs.Table.from_keys(keys, 25)
+    sw, ew, off = hipibs.pack_sequences([seq])
+    d_sw = torch.from_numpy(sw.view(np.int64)).cuda()
+    d_ew = torch.from_numpy(ew.view(np.int64)).cuda()
+    n = len(seq)
+    npos = n - 25 + 1
+    d_hit = torch.zeros((npos + 31) // 32 + 1, dtype=torch.int32, device="cuda")
+    d_cov = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    hipibs._check(hipibs._lib.hipibs_hitmask_device(t.handle, d_sw.data_ptr(), d_ew.data_ptr(), 0,
+                                                    npos, n, d_hit.data_ptr(), None), "hitmask")
+    hipibs._check(hipibs._lib.hipibs_coverage_device(d_hit.data_ptr(), npos, n, 25,
+                                                     d_cov.data_ptr(), None), "coverage")
+    torch.cuda.synchronize()
+    cov = d_cov.cpu().numpy()
+    text = b"chr1\n" + coverage_line(cov) + b"\n"
+    assert sha256(text) == expected["small"]["s0"]["coverage_sha256"]
+    t.free()
+
+
+def test_encode_device_matches_oracle():
+    torch = _torch()
+    rng = np.random.default_rng(11)
+    for k in (15, 25, 31, 32):
+        c = rng.integers(0, 4, 20000).astype(np.uint8)
+        c[rng.random(20000) < 0.01] = 4
+        s = corpus.to_text(c, lower=bool(k % 2))
+        sw, ew, _ = hipibs.pack_sequences([s])
+        d_sw = torch.from_numpy(sw.view(np.int64)).cuda()
+        d_ew = torch.from_numpy(ew.view(np.int64)).cuda()
+        np_ = len(s) - k + 1
+        d_codes = torch.zeros(np_, dtype=torch.int64, device="cuda")
+        d_valid = torch.zeros(np_, dtype=torch.uint8, device="cuda")
+        hipibs._check(hipibs._lib.hipibs_encode_device(d_sw.data_ptr(), d_ew.data_ptr(), 0, np_,
+                                                       len(s), k, d_codes.data_ptr(),
+                                                       d_valid.data_ptr(), None), "encode")
+        torch.cuda.synchronize()
+        codes = d_codes.cpu().numpy().view(np.uint64)
+        valid = d_valid.cpu().numpy()
+        oc = np.zeros(np_, dtype=np.uint64)
+        ov = np.zeros(np_, dtype=np.uint8)
+        oracle().orc_encode_positions(s, len(s), k, oc.ctypes.data, ov.ctypes.data)
+        assert np.array_equal(valid, ov)
+        assert np.array_equal(codes[ov == 1], oc[ov == 1])
+        assert np.all(codes[ov == 0] == np.uint64(2 ** 64 - 1))
+
+
+@pytest.mark.parametrize("segs", [1, 2, 3, 7, 16])
+def test_segment_split_invariance(small_corpus, segs):
+    """Windows split over several blocks fold to the same statistics
+    (the run-length carry across block seams)."""
+    torch = _torch()
+    ref, samples = small_corpus
+    seq = corpus.to_text(ref)
+    keys = samples[2][0]
+    t = hipibs.Table.from_keys(keys, 25)
+    sw, ew, _ = hipibs.pack_sequences([seq])
+    wins = hipibs.chunk_sequences([len(seq)], 50000, 25)
+    win = np.zeros(len(wins), dtype=hipibs.WINDOW_DTYPE)
+    for i, (_, s, e) in enumerate(wins):
+        win[i] = (s, e - s, 0)
+    d_sw = torch.from_numpy(sw.view(np.int64)).cuda()
+    d_ew = torch.from_numpy(ew.view(np.int64)).cuda()
+    d_win = torch.from_numpy(win.view(np.uint8)).cuda()
+    d_st = torch.zeros(len(wins) * 4, dtype=torch.int32, device="cuda")
+    import ctypes as C
+    res = C.c_uint32(0)
+    wb = hipibs._lib.hipibs_scan_work_bytes(len(wins), 1, 50000, 25, segs, C.byref(res))
+    assert res.value == segs
+    d_work = torch.zeros(max(wb, 8), dtype=torch.uint8, device="cuda")
+    arr = hipibs._table_array([t])
+    hipibs._check(hipibs._lib.hipibs_scan_windows_device(
+        arr, 1, d_sw.data_ptr(), d_ew.data_ptr(), len(seq), d_win.data_ptr(), len(wins), 50000,
+        d_st.data_ptr(), segs, d_work.data_ptr(), wb, None), "scan")
+    torch.cuda.synchronize()
+    st = d_st.cpu().numpy().reshape(-1, 4)
+    want = oracle_scan_rows([b"chr1"], [seq], 50000, 25, sorted_keys(keys))
+    got = [(b"chr1", s, e) + tuple(int(x) for x in st[i]) for i, (_, s, e) in enumerate(wins)]
+    assert got == want
+    t.free()
+
+
+def test_build_from_kmc_equals_from_keys(small_dbs, small_corpus):
+    _, samples = small_corpus
+    a = hipibs.Table.from_kmc(small_dbs[1], load_threads=3)
+    keys = hipibs.kmc_read_keys(small_dbs[1])
+    assert np.array_equal(np.sort(keys), samples[1][0])
+    assert a.info()["n_keys"] == keys.size
+    assert a.probe_count(samples[0][0]) == int(np.isin(samples[0][0], samples[1][0]).sum())
+    a.free()
