@@ -309,6 +309,32 @@ def main():
     log(f"scan: {ms:.2f} ms/step, kernel {k_ms:.2f} ms, {probes/1e9:.3f} G probes, "
         f"hit rate {observed/max(1,probes):.3f}, repeat-stable {stable}")
 
+    # measured random-sector ceiling on a buffer of the table's size
+    gups = {}
+    try:
+        blib = C.CDLL(os.path.join(ROOT, "gpu-ibs_amd", "lib", "libhipibs_bench.so"))
+        blib.hipibs_bench_gups.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_int,
+                                           C.c_int, C.POINTER(C.c_double), C.c_void_p]
+        blib.hipibs_bench_gups_reads.argtypes = [C.c_uint32, C.c_uint64, C.c_int, C.c_int]
+        blib.hipibs_bench_gups_reads.restype = C.c_uint64
+        gbuf = torch.empty(info["bytes"], dtype=torch.uint8, device="cuda")
+        S = info["bucket_slots"]
+        for rounds in ((4, 8) if S == 8 else (4, 8)):
+            for wpc in (16, 32):
+                t_ms = C.c_double(0)
+                blib.hipibs_bench_gups(gbuf.data_ptr(), info["n_buckets"], S, probes, rounds, wpc,
+                                       C.byref(t_ms), None)
+                rc = blib.hipibs_bench_gups(gbuf.data_ptr(), info["n_buckets"], S, probes, rounds,
+                                            wpc, C.byref(t_ms), None)
+                reads = blib.hipibs_bench_gups_reads(S, probes, rounds, wpc)
+                if rc == 0 and t_ms.value > 0:
+                    gups[f"r{rounds}_w{wpc}"] = reads * S * 8 / (t_ms.value / 1e3) / 1e9
+        del gbuf
+        torch.cuda.empty_cache()
+        log(f"random {S*8} B-sector ceiling (GB/s): {gups}")
+    except Exception as e:
+        log(f"gups ceiling skipped: {e}")
+
     cpu = None
     cpu_match = None
     if cpu_slice:
@@ -360,7 +386,9 @@ def main():
                      "frac": alg_bytes / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "scanKernel<8>", "kernel_ms": k_ms,
                      "algorithmic_bytes": alg_bytes,
-                     "probe_rate_gps": probes / (k_ms / 1e3) / 1e9},
+                     "probe_rate_gps": probes / (k_ms / 1e3) / 1e9,
+                     "measured_random_sector_ceiling_gbs": max(gups.values()) if gups else None,
+                     "gups_sweep_gbs": gups},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

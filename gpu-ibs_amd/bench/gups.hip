@@ -1,0 +1,115 @@
+// Random-sector read ceiling of the GPU (measurement only, not part of the
+// product library): every wave reads random 64 B (or 128 B) buckets with
+// exactly the access pattern of the table probe -- 4 (or 8) adjacent lanes
+// load one bucket as 16 B each, ROUNDS loads in flight per wave -- over a
+// table-sized buffer. The analogue of the reference's GUPS kernel
+// (src/gpu/ProbeBench.cu:58-74), which reads 8 B words.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace
+{
+
+typedef unsigned long long U64x2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) U64x2 *GlobalU64x2;
+
+__device__ inline uint64_t mix(uint64_t x)
+{
+    x ^= x >> 31;
+    x *= 0x9E3779B97F4A7C15ull;
+    x ^= x >> 29;
+    x *= 0xBF58476D1CE4E5B9ull;
+    return x ^ (x >> 32);
+}
+
+template <int S, int ROUNDS>
+__global__ __launch_bounds__(256) void gupsKernel(const uint64_t *buf, uint64_t nBuckets,
+                                                  uint64_t iters, uint64_t seed, uint64_t *out)
+{
+    constexpr int LPQ = S / 2;
+    const int lane = threadIdx.x & 63;
+    const int g = lane / LPQ, sub = lane % LPQ;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint64_t acc = 0;
+    for (uint64_t it = 0; it < iters; ++it)
+    {
+        U64x2 v[ROUNDS];
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r)
+        {
+            const uint64_t h = mix(seed ^ (((wave * iters + it) * ROUNDS + r) * 64 + g));
+            const uint64_t b = ((h >> 32) * nBuckets) >> 32;
+            v[r] = *(GlobalU64x2)(reinterpret_cast<const U64x2 *>(buf + b * S + sub * 2));
+        }
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r)
+            acc ^= v[r].x ^ v[r].y;
+    }
+    if (acc == 0x0123456789ABCDEFull)
+        out[0] = acc;
+}
+
+} // namespace
+
+// Reads about n_reads random buckets of bucket_slots*8 bytes from d_buf
+// (n_buckets buckets) with `rounds` loads in flight per wave; returns the
+// kernel time in ms through *ms (HIP events on `stream`).
+extern "C" int hipibs_bench_gups(const void *d_buf, uint64_t n_buckets, uint32_t bucket_slots,
+                                 uint64_t n_reads, int rounds, int waves_per_cu, double *ms,
+                                 void *stream)
+{
+    if (!d_buf || !ms || n_buckets == 0 || n_buckets >= (1ull << 32))
+        return -1;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int blocks = cus * (waves_per_cu > 0 ? waves_per_cu : 32) / 4;
+    const uint64_t waves = (uint64_t)blocks * 4;
+    const int qpw = (bucket_slots == 16 ? 8 : 16) * rounds; // buckets per wave iteration
+    const uint64_t iters = (n_reads + waves * qpw - 1) / (waves * qpw);
+    uint64_t *out = nullptr;
+    if (hipMalloc(&out, 8) != hipSuccess)
+        return -2;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, s);
+    const uint64_t *buf = static_cast<const uint64_t *>(d_buf);
+#define GUPS_CASE(SS, RR)                                                                        \
+    if (bucket_slots == SS && rounds == RR)                                                      \
+        hipLaunchKernelGGL((gupsKernel<SS, RR>), dim3(blocks), dim3(256), 0, s, buf, n_buckets,  \
+                           iters, 0x5EEDull, out);
+    GUPS_CASE(8, 1)
+    GUPS_CASE(8, 2)
+    GUPS_CASE(8, 4)
+    GUPS_CASE(8, 8)
+    GUPS_CASE(16, 4)
+    GUPS_CASE(16, 8)
+#undef GUPS_CASE
+    (void)hipEventRecord(e1, s);
+    hipError_t err = hipEventSynchronize(e1);
+    float t = 0;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    if (err != hipSuccess)
+        return -3;
+    *ms = t;
+    return 0;
+}
+
+// Buckets actually read by hipibs_bench_gups for these arguments.
+extern "C" uint64_t hipibs_bench_gups_reads(uint32_t bucket_slots, uint64_t n_reads, int rounds,
+                                            int waves_per_cu)
+{
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint64_t waves = (uint64_t)(cus * (waves_per_cu > 0 ? waves_per_cu : 32) / 4) * 4;
+    const int qpw = (bucket_slots == 16 ? 8 : 16) * rounds;
+    const uint64_t iters = (n_reads + waves * qpw - 1) / (waves * qpw);
+    return iters * waves * qpw;
+}

@@ -23,6 +23,16 @@ struct alignas(16) SlotPair
     uint64_t a, b;
 };
 
+// 16-byte load through the global address space (a generic pointer would
+// compile to a flat load, which also counts against the LDS wait counter).
+typedef unsigned long long U64x2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) U64x2 *GlobalU64x2;
+__device__ inline SlotPair loadPair(const uint64_t *p)
+{
+    const U64x2 v = *(GlobalU64x2)(reinterpret_cast<const U64x2 *>(p));
+    return SlotPair{v.x, v.y};
+}
+
 __device__ inline int laneId()
 {
     return (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -110,18 +120,22 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, bool vali
     const int g = lane / LPQ, sub = lane % LPQ;
     const uint64_t V = ballot(valid);
 
-    SlotPair kv[ROUNDS];
+    // All shuffles first, then all bucket loads: the loads are global
+    // (vmcnt-only) so the LDS permutes never wait on them, and all ROUNDS
+    // loads of the wave are in flight together.
     uint64_t qc[ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r)
+        qc[r] = shuffle64(code, r * QPR + g);
+    SlotPair kv[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r)
     {
-        const int q = r * QPR + g;
-        qc[r] = shuffle64(code, q);
         kv[r] = SlotPair{EMPTY, EMPTY};
-        if ((V >> q) & 1ull)
+        if ((V >> (r * QPR + g)) & 1ull)
         {
             const uint64_t b = bucketOf(qc[r], T.nBuckets);
-            kv[r] = *reinterpret_cast<const SlotPair *>(T.slots + b * S + (uint32_t)sub * 2u);
+            kv[r] = loadPair(T.slots + b * S + (uint32_t)sub * 2u);
         }
     }
     uint64_t M[ROUNDS], E[ROUNDS];
@@ -167,7 +181,7 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, bool vali
             uint64_t b = qhome + qdisp;
             if (b >= T.nBuckets)
                 b -= T.nBuckets;
-            kv2 = *reinterpret_cast<const SlotPair *>(T.slots + b * S + (uint32_t)sub * 2u);
+            kv2 = loadPair(T.slots + b * S + (uint32_t)sub * 2u);
         }
         const uint64_t M2 = ballot(serving && (kv2.a == qcode || kv2.b == qcode));
         const uint64_t E2 = ballot(serving && (kv2.a == EMPTY || kv2.b == EMPTY));

@@ -104,6 +104,11 @@ _SIGS = [
     ("hipibs_scan_submit", C.c_int, [vp, C.c_int, C.c_uint32, C.c_uint64, C.c_uint64]),
     ("hipibs_scan_wait", C.c_int, [vp, C.c_int]),
     ("hipibs_scan_free", C.c_int, [vp]),
+    ("hipibs_dev_alloc", C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
+    ("hipibs_dev_free", C.c_int, [vp]),
+    ("hipibs_memcpy", C.c_int, [vp, vp, C.c_uint64, C.c_int]),
+    ("hipibs_dev_memset", C.c_int, [vp, C.c_int, C.c_uint64]),
+    ("hipibs_dev_synchronize", C.c_int, [C.c_int]),
 ]
 for _name, _res, _args in _SIGS:
     _f = getattr(_lib, _name)
@@ -292,6 +297,54 @@ def format_tsv(rows: Sequence[Tuple[bytes, int, int, int, int, int, int]]) -> by
     for r in rows:
         parts.append(b"%s\t%d\t%d\t%d\t%d\t%d\t%d\n" % tuple(r))
     return b"".join(parts)
+
+
+# ---- device memory ---------------------------------------------------------------
+
+class DeviceBuffer:
+    """A device allocation owned by libhipibs (no other HIP runtime needed)."""
+
+    def __init__(self, nbytes: int, device: int = 0):
+        p = vp()
+        _check(_lib.hipibs_dev_alloc(device, int(nbytes), C.byref(p)), "hipibs_dev_alloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_array(cls, a: np.ndarray, device: int = 0) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(max(a.nbytes, 1), device)
+        _check(_lib.hipibs_memcpy(vp(b.ptr), a.ctypes.data_as(vp), a.nbytes, 1), "hipibs_memcpy")
+        return b
+
+    @classmethod
+    def zeros(cls, nbytes: int, device: int = 0) -> "DeviceBuffer":
+        b = cls(max(int(nbytes), 1), device)
+        _check(_lib.hipibs_dev_memset(vp(b.ptr), 0, b.nbytes), "hipibs_dev_memset")
+        return b
+
+    def to_array(self, dtype, count: Optional[int] = None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else count
+        out = np.empty(n, dtype=dt)
+        _check(_lib.hipibs_memcpy(out.ctypes.data_as(vp), vp(self.ptr), out.nbytes, 2),
+               "hipibs_memcpy")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            _lib.hipibs_dev_free(vp(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize(device: int = 0) -> None:
+    _check(_lib.hipibs_dev_synchronize(device), "hipibs_dev_synchronize")
 
 
 # ---- tables -------------------------------------------------------------------

@@ -237,6 +237,15 @@ int hipibs_coverage_device(const uint32_t *d_hit_words, uint64_t n_pos, uint64_t
 int hipibs_set_profiling(int on);
 double hipibs_last_kernel_ms(void);
 
+/* ---- device memory for FFI callers without a HIP binding ------------------
+ * No counterpart in the reference (its callers are CUDA C++). Synchronous. */
+int hipibs_dev_alloc(int device, uint64_t bytes, void **out);
+int hipibs_dev_free(void *p);
+/* kind: 1 host->device, 2 device->host, 3 device->device. */
+int hipibs_memcpy(void *dst, const void *src, uint64_t bytes, int kind);
+int hipibs_dev_memset(void *dst, int value, uint64_t bytes);
+int hipibs_dev_synchronize(int device);
+
 /* ---- streaming scan engine (host buffers, pinned, double-buffered) -------- */
 
 /* Replaces the EncodeProbeEngine constructors + caps + host(slot)

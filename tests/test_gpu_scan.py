@@ -120,55 +120,47 @@ def test_duplicate_keys_stored_once():
     t.free()
 
 
-def _torch():
-    import torch
-    assert torch.cuda.is_available()
-    return torch
+D = hipibs.DeviceBuffer
 
 
 def test_coverage_matches_golden(small_corpus, expected):
-    torch = _torch()
     ref, samples = small_corpus
     seq = corpus.to_text(ref)
     keys = samples[0][0]
     t = hipibs.Table.from_keys(keys, 25)
     sw, ew, off = hipibs.pack_sequences([seq])
-    d_sw = torch.from_numpy(sw.view(np.int64)).cuda()
-    d_ew = torch.from_numpy(ew.view(np.int64)).cuda()
+    d_sw, d_ew = D.from_array(sw), D.from_array(ew)
     n = len(seq)
     npos = n - 25 + 1
-    d_hit = torch.zeros((npos + 31) // 32 + 1, dtype=torch.int32, device="cuda")
-    d_cov = torch.zeros(n, dtype=torch.uint8, device="cuda")
-    hipibs._check(hipibs._lib.hipibs_hitmask_device(t.handle, d_sw.data_ptr(), d_ew.data_ptr(), 0,
-                                                    npos, n, d_hit.data_ptr(), None), "hitmask")
-    hipibs._check(hipibs._lib.hipibs_coverage_device(d_hit.data_ptr(), npos, n, 25,
-                                                     d_cov.data_ptr(), None), "coverage")
-    torch.cuda.synchronize()
-    cov = d_cov.cpu().numpy()
+    d_hit = D.zeros(((npos + 31) // 32 + 1) * 4)
+    d_cov = D.zeros(n)
+    hipibs._check(hipibs._lib.hipibs_hitmask_device(t.handle, d_sw.ptr, d_ew.ptr, 0, npos, n,
+                                                    d_hit.ptr, None), "hitmask")
+    hipibs._check(hipibs._lib.hipibs_coverage_device(d_hit.ptr, npos, n, 25, d_cov.ptr, None),
+                  "coverage")
+    hipibs.synchronize()
+    cov = d_cov.to_array(np.uint8, n)
     text = b"chr1\n" + coverage_line(cov) + b"\n"
     assert sha256(text) == expected["small"]["s0"]["coverage_sha256"]
     t.free()
 
 
 def test_encode_device_matches_oracle():
-    torch = _torch()
     rng = np.random.default_rng(11)
     for k in (15, 25, 31, 32):
         c = rng.integers(0, 4, 20000).astype(np.uint8)
         c[rng.random(20000) < 0.01] = 4
         s = corpus.to_text(c, lower=bool(k % 2))
         sw, ew, _ = hipibs.pack_sequences([s])
-        d_sw = torch.from_numpy(sw.view(np.int64)).cuda()
-        d_ew = torch.from_numpy(ew.view(np.int64)).cuda()
+        d_sw, d_ew = D.from_array(sw), D.from_array(ew)
         np_ = len(s) - k + 1
-        d_codes = torch.zeros(np_, dtype=torch.int64, device="cuda")
-        d_valid = torch.zeros(np_, dtype=torch.uint8, device="cuda")
-        hipibs._check(hipibs._lib.hipibs_encode_device(d_sw.data_ptr(), d_ew.data_ptr(), 0, np_,
-                                                       len(s), k, d_codes.data_ptr(),
-                                                       d_valid.data_ptr(), None), "encode")
-        torch.cuda.synchronize()
-        codes = d_codes.cpu().numpy().view(np.uint64)
-        valid = d_valid.cpu().numpy()
+        d_codes = D.zeros(np_ * 8)
+        d_valid = D.zeros(np_)
+        hipibs._check(hipibs._lib.hipibs_encode_device(d_sw.ptr, d_ew.ptr, 0, np_, len(s), k,
+                                                       d_codes.ptr, d_valid.ptr, None), "encode")
+        hipibs.synchronize()
+        codes = d_codes.to_array(np.uint64, np_)
+        valid = d_valid.to_array(np.uint8, np_)
         oc = np.zeros(np_, dtype=np.uint64)
         ov = np.zeros(np_, dtype=np.uint8)
         oracle().orc_encode_positions(s, len(s), k, oc.ctypes.data, ov.ctypes.data)
@@ -177,11 +169,28 @@ def test_encode_device_matches_oracle():
         assert np.all(codes[ov == 0] == np.uint64(2 ** 64 - 1))
 
 
+def test_probe_bytes_matches_membership(small_corpus):
+    _, samples = small_corpus
+    keys = samples[0][0]
+    t = hipibs.Table.from_keys(keys, 25)
+    q = np.concatenate([keys[::5], np.minimum(corpus.splitmix(8, 8, 100000) & np.uint64((1 << 50) - 1),
+                                              corpus.revcomp(corpus.splitmix(8, 8, 100000) &
+                                                             np.uint64((1 << 50) - 1), 25))])
+    valid = (np.arange(q.size) % 7 != 3).astype(np.uint8)
+    d_q, d_v, d_h = D.from_array(q), D.from_array(valid), D.zeros(q.size)
+    hipibs._check(hipibs._lib.hipibs_table_probe(t.handle, d_q.ptr, d_v.ptr, q.size, d_h.ptr, None),
+                  "probe")
+    hipibs.synchronize()
+    hits = d_h.to_array(np.uint8, q.size)
+    want = (np.isin(q, keys) & (valid == 1)).astype(np.uint8)
+    assert np.array_equal(hits, want)
+    t.free()
+
+
 @pytest.mark.parametrize("segs", [1, 2, 3, 7, 16])
 def test_segment_split_invariance(small_corpus, segs):
     """Windows split over several blocks fold to the same statistics
     (the run-length carry across block seams)."""
-    torch = _torch()
     ref, samples = small_corpus
     seq = corpus.to_text(ref)
     keys = samples[2][0]
@@ -191,21 +200,19 @@ def test_segment_split_invariance(small_corpus, segs):
     win = np.zeros(len(wins), dtype=hipibs.WINDOW_DTYPE)
     for i, (_, s, e) in enumerate(wins):
         win[i] = (s, e - s, 0)
-    d_sw = torch.from_numpy(sw.view(np.int64)).cuda()
-    d_ew = torch.from_numpy(ew.view(np.int64)).cuda()
-    d_win = torch.from_numpy(win.view(np.uint8)).cuda()
-    d_st = torch.zeros(len(wins) * 4, dtype=torch.int32, device="cuda")
+    d_sw, d_ew, d_win = D.from_array(sw), D.from_array(ew), D.from_array(win)
+    d_st = D.zeros(len(wins) * 16)
     import ctypes as C
     res = C.c_uint32(0)
     wb = hipibs._lib.hipibs_scan_work_bytes(len(wins), 1, 50000, 25, segs, C.byref(res))
     assert res.value == segs
-    d_work = torch.zeros(max(wb, 8), dtype=torch.uint8, device="cuda")
+    d_work = D.zeros(max(wb, 8))
     arr = hipibs._table_array([t])
     hipibs._check(hipibs._lib.hipibs_scan_windows_device(
-        arr, 1, d_sw.data_ptr(), d_ew.data_ptr(), len(seq), d_win.data_ptr(), len(wins), 50000,
-        d_st.data_ptr(), segs, d_work.data_ptr(), wb, None), "scan")
-    torch.cuda.synchronize()
-    st = d_st.cpu().numpy().reshape(-1, 4)
+        arr, 1, d_sw.ptr, d_ew.ptr, len(seq), d_win.ptr, len(wins), 50000, d_st.ptr, segs,
+        d_work.ptr, wb, None), "scan")
+    hipibs.synchronize()
+    st = d_st.to_array(np.int32).reshape(-1, 4)
     want = oracle_scan_rows([b"chr1"], [seq], 50000, 25, sorted_keys(keys))
     got = [(b"chr1", s, e) + tuple(int(x) for x in st[i]) for i, (_, s, e) in enumerate(wins)]
     assert got == want
