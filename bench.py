@@ -143,14 +143,17 @@ def random_canonical(torch, m, k, g):
     return torch.minimum(x, rc)
 
 
-def make_sample_keys(torch, hap_codes, keep_p, n_random, k, seed):
+def make_sample_keys(torch, hap_codes, target, k, seed):
+    """~target distinct keys: 80% drawn from the distinct k-mers of the
+    haplotype, the rest random canonical codes (sequencing-error k-mers)."""
     g = torch.Generator(device="cuda")
     g.manual_seed(seed + 7)
-    keep = (hap_codes >= 0) & (torch.rand(hap_codes.numel(), device="cuda", generator=g) < keep_p)
-    keys = hap_codes[keep]
-    rnd = random_canonical(torch, n_random, k, g)
-    keys = torch.unique(torch.cat([keys, rnd]))
-    return keys
+    hk = torch.unique(hap_codes[hap_codes >= 0])
+    n_hap = int(0.8 * target)
+    if hk.numel() > n_hap:
+        hk = hk[torch.rand(hk.numel(), device="cuda", generator=g) < n_hap / hk.numel()]
+    rnd = random_canonical(torch, max(0, int(target) - hk.numel()), k, g)
+    return torch.unique(torch.cat([hk, rnd]))
 
 
 # ---- CPU baseline (the reference's CPU-BATCHED scan, compiled from its sources) ----
@@ -172,8 +175,7 @@ def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_c
     ref.ref_batched_scan.restype = C.c_double
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     frac = n_slice / ref_codes.numel()
-    keys = make_sample_keys(torch, hap_codes_slice, args.keep_p, int(0.2 * args.keys * frac),
-                            K, args.seed + 99)
+    keys = make_sample_keys(torch, hap_codes_slice, args.keys * frac, K, args.seed + 99)
     hk = keys.cpu().numpy().view(np.uint64)
     log(f"cpu sample: {n_slice/1e6:.0f} Mbp slice, {hk.size/1e6:.1f} M keys -> KMC2 database")
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
@@ -249,12 +251,9 @@ def main():
     hap_codes = encode(torch, hsw, hew, n, K)
     del hsw, hew
     torch.cuda.synchronize()
-    n_valid_hap = int((hap_codes >= 0).sum())
-    keep_p = min(1.0, 0.8 * args.keys / max(1, n_valid_hap))
-    args.keep_p = keep_p
     cpu_slice = int(args.cpu_mbp * 1e6) // 64 * 64 if args.cpu_mbp > 0 else 0
     hap_slice = hap_codes[:cpu_slice - K + 1].clone() if cpu_slice else None
-    keys = make_sample_keys(torch, hap_codes, keep_p, int(0.2 * args.keys), K, args.seed)
+    keys = make_sample_keys(torch, hap_codes, args.keys, K, args.seed)
     del hap_codes
     torch.cuda.synchronize()
     log(f"sample keys: {keys.numel()/1e9:.3f} G distinct ({time.time()-t0:.1f} s)")
