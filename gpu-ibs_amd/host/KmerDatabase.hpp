@@ -1,0 +1,346 @@
+// KmerDatabase: one KMC sample against references -- the scan (per-window
+// IBS table) and the mapper (per-base coverage) -- with the reference's
+// public interface (src/KmerDatabase.hpp:44-1598) over the MI355X library.
+// There is no CPU path: the device table is the only probe structure, and
+// a machine without a usable GPU gets an error, not a silent fallback.
+#pragma once
+
+#include "Hipibs.hpp"
+#include "KmerCodec.hpp"
+#include "ScanPipeline.hpp"
+#include "SeqIO.hpp"
+#include "Window.hpp"
+
+#include <cerrno>
+#include <charconv>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <thread>
+#include <vector>
+
+class KmerDatabase
+{
+public:
+    // Device and batch knobs (the reference's GpuOptions, set from the CLI).
+    struct GpuOptions
+    {
+        int device = 0;
+        uint64_t batchBases = 128ull << 20;
+        int depth = 3;
+        double loadFactor = 0.6;
+        uint32_t bucketSlots = 8;
+        uint64_t hbmBudget = 0;
+        unsigned loadThreads = 0;
+    };
+    static GpuOptions &gpuOptions()
+    {
+        static GpuOptions o;
+        return o;
+    }
+
+    // Reads the header only; the device table is built on first use.
+    explicit KmerDatabase(std::string sourcePath, bool listing = false)
+        : sourcePath_(std::move(sourcePath))
+    {
+        (void)listing;
+        hipibs::check(hipibs_kmc_header(sourcePath_.c_str(), &info_), "reading the KMC header");
+        kmerSize = info_.kmer_length;
+        if (kmerSize == 0 || kmerSize > 32)
+            throw std::runtime_error("k = " + std::to_string(kmerSize) +
+                                     ": this build supports k <= 32 only");
+    }
+
+    uint32_t getKmerSize() const { return kmerSize; }
+    uint64_t totalKmers() const { return info_.total_kmers; }
+    const std::string &sourcePath() const { return sourcePath_; }
+
+    void printKMCInfo() const
+    {
+        std::cout << "KMC version: " << (info_.kmc_version == 0x200 ? "KMC2" : "KMC1") << "\n"
+                  << "k: " << info_.kmer_length << "\n"
+                  << "Signature length: " << info_.signature_len << "\n"
+                  << "LUT prefix length: " << info_.lut_prefix_length << "\n"
+                  << "Counter size: " << info_.counter_size << "\n"
+                  << "Min count: " << info_.min_count << "\n"
+                  << "Max count: " << info_.max_count << "\n"
+                  << "Total k-mers: " << info_.total_kmers << std::endl;
+    }
+
+    bool gpuActive() const { return table_ != nullptr; }
+    const hipibs_table *table() const { return table_.get(); }
+
+    // Builds the device table (reads and repacks the database on the host
+    // threads, streams the keys to the device, inserts them).
+    void initGpuIndex()
+    {
+        if (table_)
+            return;
+        std::string why;
+        if (hipibs::deviceCount(why) < 1)
+            throw std::runtime_error("no usable GPU: " + why);
+        const GpuOptions &g = gpuOptions();
+        hipibs_table_opts o = hipibs::defaultOpts();
+        o.device = g.device;
+        o.load_factor = g.loadFactor;
+        o.bucket_slots = g.bucketSlots;
+        o.hbm_budget = g.hbmBudget;
+        o.load_threads = g.loadThreads;
+        const auto t0 = std::chrono::steady_clock::now();
+        table_ = hipibs::tableFromKmc(sourcePath_, o);
+        const auto ti = hipibs::info(table_.get());
+        std::cout << "Device table: " << ti.keys << " keys, " << ti.bytes / 1e9 << " GB, "
+                  << ti.slots * 8 << " B buckets, built in "
+                  << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count()
+                  << " s" << std::endl;
+    }
+
+    void releaseGpuIndex() { table_.reset(); }
+
+    // One window's statistics (the per-window call of the reference's
+    // CPU and GPU-sync paths).
+    Window getStatsFromSequence(const SeqWindow &window)
+    {
+        initGpuIndex();
+        std::vector<SeqWindow> one{window};
+        hipibs::ScanOptions so;
+        so.batchBases = std::max<uint64_t>(1 << 16, window.end - window.start + 128);
+        so.depth = 1;
+        return hipibs::scanWindowsStreamed({table_.get()}, one, so)[0][0];
+    }
+
+    struct ReferenceWindows
+    {
+        ParseResult parsed;
+        std::vector<SeqWindow> chunks;
+        double readSec = 0, windowSec = 0;
+    };
+
+    void readReferenceWindows(const std::string &refPath, int windowSize, ReferenceWindows &out)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        std::cout << "Reading sequences" << std::endl;
+        out.parsed = readSequenceFile(refPath, refPath.find(".gz") != std::string::npos);
+        const auto t1 = std::chrono::steady_clock::now();
+        out.readSec = std::chrono::duration<double>(t1 - t0).count();
+        chunkSequences(out.parsed.ids, out.parsed.sequences, (size_t)windowSize, kmerSize,
+                       out.chunks);
+        out.windowSec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+        std::cout << "Number of chunks: " << out.chunks.size() << std::endl;
+    }
+
+    static hipibs::ScanOptions scanOptions()
+    {
+        hipibs::ScanOptions so;
+        so.batchBases = gpuOptions().batchBases;
+        so.depth = gpuOptions().depth;
+        return so;
+    }
+
+    // "<out>.partial", then rename: a failed write leaves no file under
+    // the final name (src/KmerDatabase.hpp:899-931).
+    template <typename WriteFn>
+    static void writeOutputFile(const std::string &outPath, WriteFn write)
+    {
+        const std::string tmp = outPath + ".partial";
+        {
+            errno = 0;
+            std::ofstream out(tmp, std::ios::binary);
+            if (!out.is_open())
+                throw std::runtime_error("Unable to open " + tmp + " for writing" +
+                                         (errno ? std::string(": ") + std::strerror(errno) : ""));
+            write(out);
+            out.close();
+            if (!out)
+            {
+                std::error_code ec;
+                std::filesystem::remove(tmp, ec);
+                throw std::runtime_error("Writing " + outPath + " failed (disk full or I/O error)");
+            }
+        }
+        std::error_code ec;
+        std::filesystem::rename(tmp, outPath, ec);
+        if (ec)
+        {
+            std::error_code ig;
+            std::filesystem::remove(tmp, ig);
+            throw std::runtime_error("Unable to move " + tmp + " to " + outPath + ": " +
+                                     ec.message());
+        }
+    }
+
+    // The TSV (src/KmerDatabase.hpp:935-948 format).
+    static void writeStats(const std::string &outPath, const std::vector<Window> &rows)
+    {
+        writeOutputFile(outPath, [&rows](std::ofstream &f) {
+            std::string buf = "seqname\tstart\tend\ttotal_kmers\tobserved_kmers\tvariations\t"
+                              "kmer_distance\n";
+            char num[16];
+            auto put = [&](long long v, char sep) {
+                auto r = std::to_chars(num, num + sizeof(num), v);
+                buf.append(num, r.ptr);
+                buf.push_back(sep);
+            };
+            for (const Window &w : rows)
+            {
+                buf += w.id;
+                buf.push_back('\t');
+                put(w.start, '\t');
+                put(w.end, '\t');
+                put(w.totalKmers, '\t');
+                put(w.observedKmers, '\t');
+                put(w.variations, '\t');
+                put(w.kmerDistance, '\n');
+                if (buf.size() > (1u << 22))
+                {
+                    f.write(buf.data(), (std::streamsize)buf.size());
+                    buf.clear();
+                }
+            }
+            f.write(buf.data(), (std::streamsize)buf.size());
+        });
+    }
+
+    void processReference(std::string refPath, std::string outPath, int windowSize = 50000)
+    {
+        using clk = std::chrono::steady_clock;
+        auto t = clk::now();
+        auto lap = [&t]() {
+            const auto n = clk::now();
+            const double s = std::chrono::duration<double>(n - t).count();
+            t = n;
+            return s;
+        };
+        initGpuIndex();
+        const double indexSec = lap();
+        ReferenceWindows rw;
+        readReferenceWindows(refPath, windowSize, rw);
+        lap();
+        std::cout << "Kmer size: " << kmerSize << "\nK-mer path: fused device encode+probe+"
+                  << "window reduction (MI355X)" << std::endl;
+        hipibs::ScanTimes st;
+        auto res = hipibs::scanWindowsStreamed({table_.get()}, rw.chunks, scanOptions(), &st);
+        const double pipeSec = lap();
+        writeStats(outPath, res[0]);
+        const double writeSec = lap();
+        std::cout << "Scan phases: index " << indexSec << " s, read " << rw.readSec
+                  << " s, windows " << rw.windowSec << " s, pipeline " << pipeSec << " s ("
+                  << st.batches << " batches, pack " << st.packSec << " s, device wait "
+                  << st.waitSec << " s), write " << writeSec << " s" << std::endl;
+    }
+
+    // Coverage line: values joined by ',' (serializeCoverage,
+    // src/KmerDatabase.hpp:756-772), built on host threads.
+    static std::string serializeCoverage(const uint8_t *cov, size_t n)
+    {
+        if (n == 0)
+            return std::string();
+        const unsigned T = std::max(1u, std::min(std::thread::hardware_concurrency(), 64u));
+        const size_t per = (n + T - 1) / T;
+        std::vector<std::string> parts(T);
+        auto work = [&](unsigned i) {
+            const size_t lo = std::min(n, (size_t)i * per), hi = std::min(n, lo + per);
+            std::string &s = parts[i];
+            s.reserve((hi - lo) * 3);
+            for (size_t j = lo; j < hi; ++j)
+            {
+                const unsigned v = cov[j];
+                if (v >= 100)
+                    s.push_back((char)('0' + v / 100));
+                if (v >= 10)
+                    s.push_back((char)('0' + (v / 10) % 10));
+                s.push_back((char)('0' + v % 10));
+                s.push_back(',');
+            }
+        };
+        std::vector<std::thread> pool;
+        for (unsigned i = 0; i < T; ++i)
+            pool.emplace_back(work, i);
+        for (auto &th : pool)
+            th.join();
+        size_t total = 0;
+        for (auto &p : parts)
+            total += p.size();
+        std::string out;
+        out.reserve(total);
+        for (auto &p : parts)
+            out += p;
+        out.pop_back();
+        return out;
+    }
+
+    // Per-base coverage of every contig (fastibsmapper): device hit mask
+    // and k-window popcount, serialised on the host.
+    void produceMapping(std::string refPath, std::string outPath)
+    {
+        using clk = std::chrono::steady_clock;
+        initGpuIndex();
+        std::cout << "Kmer size: " << kmerSize << "\nK-mer path: device hit mask + coverage"
+                  << std::endl;
+        std::cout << "Reading sequences" << std::endl;
+        ParseResult r = readSequenceFile(refPath, refPath.find(".gz") != std::string::npos);
+        const auto t0 = clk::now();
+        const int dev = gpuOptions().device;
+        std::vector<std::pair<const std::string *, std::string>> lines(r.sequences.size());
+        std::vector<uint64_t> sw, ew;
+        std::vector<uint8_t> cov;
+        double probeSec = 0, serSec = 0;
+        for (size_t c = 0; c < r.sequences.size(); ++c)
+        {
+            const std::string &s = r.sequences[c];
+            const uint64_t n = s.size();
+            cov.assign(n, 0);
+            const auto tp = clk::now();
+            if (n >= kmerSize)
+            {
+                const uint64_t nsw = (n + 31) / 32 + HIPIBS_PACK_PAD_WORDS;
+                const uint64_t new_ = (n + 63) / 64 + HIPIBS_PACK_PAD_WORDS;
+                sw.assign(nsw, 0);
+                ew.assign(new_, 0);
+                hipibs::check(hipibs_pack_bases(s.data(), n, 0, sw.data(), ew.data()), "pack");
+                const uint64_t npos = n - kmerSize + 1;
+                hipibs::DeviceBuffer dsw(dev, nsw * 8), dew(dev, new_ * 8),
+                    dhit(dev, ((npos + 31) / 32 + 1) * 4), dcov(dev, n);
+                hipibs::check(hipibs_memcpy(dsw.get(), sw.data(), nsw * 8, 1), "upload");
+                hipibs::check(hipibs_memcpy(dew.get(), ew.data(), new_ * 8, 1), "upload");
+                hipibs::check(hipibs_hitmask_device(table_.get(), dsw.as<uint64_t>(),
+                                                    dew.as<uint64_t>(), 0, npos, n,
+                                                    dhit.as<uint32_t>(), nullptr),
+                              "hit mask");
+                hipibs::check(hipibs_coverage_device(dhit.as<uint32_t>(), npos, n, kmerSize,
+                                                     dcov.as<uint8_t>(), nullptr),
+                              "coverage");
+                hipibs::check(hipibs_memcpy(cov.data(), dcov.get(), n, 2), "download");
+            }
+            const auto ts = clk::now();
+            probeSec += std::chrono::duration<double>(ts - tp).count();
+            lines[c] = {&r.ids[c], serializeCoverage(cov.data(), n)};
+            serSec += std::chrono::duration<double>(clk::now() - ts).count();
+        }
+        std::cout << "Writing mapping to file" << std::endl;
+        const auto tw = clk::now();
+        writeOutputFile(outPath, [&lines](std::ofstream &f) {
+            for (auto &l : lines)
+            {
+                f << *l.first << '\n';
+                f.write(l.second.data(), (std::streamsize)l.second.size());
+                f << '\n';
+            }
+        });
+        std::cout << "Mapping phases: probe " << probeSec << " s, serialize " << serSec
+                  << " s, write " << std::chrono::duration<double>(clk::now() - tw).count()
+                  << " s (total " << std::chrono::duration<double>(clk::now() - t0).count()
+                  << " s)" << std::endl;
+    }
+
+    uint32_t kmerSize = 0;
+
+private:
+    std::string sourcePath_;
+    hipibs_kmc_info info_{};
+    hipibs::TablePtr table_;
+};

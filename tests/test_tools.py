@@ -1,0 +1,110 @@
+"""The drop-in commands (fastibs, fastibsmapper, KDBIntersect): CLI rules
+and error paths on the CPU, byte-identical outputs on the GPU."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import corpus
+from util import ROOT, sha256
+
+import hipibs
+
+BIN = os.path.join(ROOT, "gpu-ibs_amd", "bin")
+
+
+def tool(name, *args, cwd=None):
+    return subprocess.run([os.path.join(BIN, name), *map(str, args)], capture_output=True,
+                          text=True, cwd=cwd, timeout=600)
+
+
+@pytest.fixture(scope="module")
+def small_layout(tmp_path_factory, small_corpus):
+    """refs/small_genome.fasta and three dataset folders s0..s2."""
+    d = tmp_path_factory.mktemp("layout")
+    ref, samples = small_corpus
+    (d / "refs").mkdir()
+    (d / "refs" / "small_genome.fasta").write_bytes(
+        corpus.fasta([(b"chr1", corpus.to_text(ref))]))
+    (d / "refs" / "notes.txt").write_text("not a reference")
+    for i, (keys, counts) in enumerate(samples):
+        (d / f"s{i}").mkdir()
+        hipibs.kmc_write(str(d / f"s{i}" / "sample"), keys, 25, counts=counts, p=9, sig_len=9,
+                         counter_size=1, version=0x200, min_count=2, max_count=255)
+    return d
+
+
+def test_help_and_bad_arguments(tmp_path):
+    for name in ("fastibs", "fastibsmapper"):
+        r = tool(name, "--help")
+        assert r.returncode == 0 and "Usage" in r.stdout
+        assert tool(name).returncode == 1
+    r = tool("fastibs", tmp_path, tmp_path, tmp_path, "50000abc")
+    assert r.returncode == 1 and "Invalid <windowSize>" in r.stderr
+    r = tool("fastibs", tmp_path / "nodb", tmp_path, tmp_path, "50000")
+    assert r.returncode == 1 and "KMC database folder not found" in r.stderr
+    r = tool("fastibs", tmp_path, tmp_path, tmp_path, "50000", "--cpu")
+    assert r.returncode == 1 and "no CPU path" in r.stderr
+    r = tool("fastibs", tmp_path, tmp_path, tmp_path, "50000", "--bogus")
+    assert r.returncode == 1 and "unknown" in r.stderr
+
+
+def test_window_not_larger_than_k(small_layout, tmp_path):
+    r = tool("fastibs", small_layout / "s0", small_layout / "refs", tmp_path / "out", 25)
+    assert r.returncode == 1 and "must be larger than" in r.stderr
+
+
+def test_no_gpu_is_an_error_not_a_fallback(small_layout, tmp_path):
+    n, _ = hipibs.device_count()
+    if n > 0:
+        pytest.skip("a GPU is present")
+    r = tool("fastibs", small_layout / "s0", small_layout / "refs", tmp_path / "out", 50000,
+             cwd=tmp_path)
+    assert r.returncode == 1
+    assert "no usable GPU" in (tmp_path / "log.txt").read_text()
+    assert not any(p.suffix == ".tsv" for p in (tmp_path / "out").iterdir())
+
+
+@pytest.mark.gpu
+def test_fastibs_matches_golden_and_skips_existing(small_layout, tmp_path):
+    out = tmp_path / "out"
+    r = tool("fastibs", small_layout / "s1", small_layout / "refs", out, 50000, "--batch-mb=1")
+    assert r.returncode == 0, r.stderr
+    got = (out / "s1_v_small_50000.tsv").read_bytes()
+    assert got == (
+        open(os.path.join(ROOT, "tests", "golden", "small_s1_50000.tsv"), "rb").read())
+    r = tool("fastibs", small_layout / "s1", small_layout / "refs", out, 50000)
+    assert r.returncode == 0 and "already exists" in r.stdout
+
+
+@pytest.mark.gpu
+def test_fastibs_panel_equals_singles(small_layout, tmp_path):
+    out = tmp_path / "panel"
+    lst = tmp_path / "list.txt"
+    lst.write_text(f"# panel\n{small_layout / 's2'}\n")
+    r = tool("fastibs", small_layout / "s0", small_layout / "refs", out, 50000,
+             f"--db={small_layout / 's1'}", f"--db-list={lst}", "--resident=2")
+    assert r.returncode == 0, r.stderr
+    for i in range(3):
+        got = (out / f"s{i}_v_small_50000.tsv").read_bytes()
+        want = open(os.path.join(ROOT, "tests", "golden", f"small_s{i}_50000.tsv"), "rb").read()
+        assert got == want, i
+
+
+@pytest.mark.gpu
+def test_fastibsmapper_matches_golden(small_layout, tmp_path, expected):
+    out = tmp_path / "map"
+    r = tool("fastibsmapper", small_layout / "s0", small_layout / "refs", out)
+    assert r.returncode == 0, r.stderr
+    assert sha256((out / "s0_v_small_genome.txt").read_bytes()) == \
+        expected["small"]["s0"]["coverage_sha256"]
+
+
+@pytest.mark.gpu
+def test_kdbintersect_matches_golden(small_layout, expected):
+    for pair, want in expected["small"]["intersect"].items():
+        a, b = pair.split("-")
+        r = tool("KDBIntersect", small_layout / f"s{a}", small_layout / f"s{b}")
+        assert r.returncode == 0, r.stderr
+        assert r.stdout == f"{want}\n"
