@@ -108,3 +108,9 @@ def test_kdbintersect_matches_golden(small_layout, expected):
         r = tool("KDBIntersect", small_layout / f"s{a}", small_layout / f"s{b}")
         assert r.returncode == 0, r.stderr
         assert r.stdout == f"{want}\n"
+
+
+def test_cpp_host_layer_checks():
+    r = tool("test_host")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "host checks ok" in r.stdout
