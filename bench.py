@@ -42,7 +42,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--gbp", type=float, default=2.0, help="reference length in Gbp")
     ap.add_argument("--keys", type=float, default=2.0e9, help="target distinct keys")
-    ap.add_argument("--bucket-slots", type=int, default=8)
+    ap.add_argument("--bucket-slots", type=int, default=0, help="0 = library default")
+    ap.add_argument("--minimizer-len", type=int, default=0,
+                    help="0 = library default (minimizer buckets for k >= 21), 1 = off")
     ap.add_argument("--load-factor", type=float, default=0.6)
     ap.add_argument("--segs", type=int, default=0, help="segments per window (0 = auto)")
     ap.add_argument("--cpu-mbp", type=float, default=256.0,
@@ -198,7 +200,9 @@ def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_c
     probes = int(stats[:, 0].sum())
     log(f"cpu baseline: {probes/1e6:.1f} M probes in {secs:.2f} s on {threads} threads")
     # same sample on the GPU: must be identical
-    t = hipibs.Table.from_device_keys(keys.data_ptr(), keys.numel(), K)
+    t = hipibs.Table.from_device_keys(keys.data_ptr(), keys.numel(), K,
+                                      bucket_slots=args.bucket_slots,
+                                      minimizer_len=args.minimizer_len)
     gst = gpu_check(t, n_slice, wins)
     t.free()
     match = bool(np.array_equal(gst, stats))
@@ -260,7 +264,8 @@ def main():
     tb = time.time()
     table = hipibs.Table.from_device_keys(keys.data_ptr(), keys.numel(), K,
                                           bucket_slots=args.bucket_slots,
-                                          load_factor=args.load_factor)
+                                          load_factor=args.load_factor,
+                                          minimizer_len=args.minimizer_len)
     build_s = time.time() - tb
     info = table.info()
     n_keys = keys.numel()
@@ -361,10 +366,12 @@ def main():
     # workload and calibrated against a plain copy kernel
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "scan_k25_2gbp_traffic.json")
-    if (os.path.exists(tpath) and n == 2_000_000_000 and args.bucket_slots == 8 and
-            abs(args.keys - 2e9) < 1):
+    if os.path.exists(tpath) and n == 2_000_000_000 and abs(args.keys - 2e9) < 1:
         with open(tpath) as f:
-            traffic = json.load(f)["traffic_bytes_per_dispatch"]
+            tj = json.load(f)
+        if (tj.get("bucket_slots") == info["bucket_slots"] and
+                tj.get("minimizer_len") == info["minimizer_len"]):
+            traffic = tj["traffic_bytes_per_dispatch"]
     value = probes / (ms / 1e3)
     out = {
         "metric": "ref k-mers probed/s",
@@ -384,6 +391,7 @@ def main():
                    "reference_bases": n, "windows": len(wins), "k": K, "window": W,
                    "sample_keys": int(info["n_keys"]), "table_bytes": int(info["bytes"]),
                    "bucket_slots": info["bucket_slots"], "load_factor": args.load_factor,
+                   "minimizer_len": info["minimizer_len"],
                    "segs_per_window": segs.value, "scan_wall_s": ms / 1e3,
                    "table_build_s": build_s, "probes_per_step": probes,
                    "hit_rate": observed / max(1, probes), "repeat_stable": stable,
@@ -391,7 +399,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": alg_bytes / (k_ms / 1e3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg_bytes / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "scanKernel<8>", "kernel_ms": k_ms,
+                     "kernel": f"scanKernel<{info['bucket_slots']}>", "kernel_ms": k_ms,
                      "algorithmic_bytes": alg_bytes,
                      "probe_rate_gps": probes / (k_ms / 1e3) / 1e9,
                      "measured_random_sector_ceiling_gbs": max(gups.values()) if gups else None,

@@ -32,14 +32,71 @@ __host__ __device__ inline uint64_t bucketOf(uint64_t key, uint64_t nBuckets)
     return ((h >> 32) * nBuckets) >> 32;
 }
 
+// Reverses the order of the 32 2-bit groups of x.
+__host__ __device__ inline uint64_t reverse2(uint64_t x)
+{
+    const uint64_t v = __builtin_bitreverse64(x);
+    return ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+}
+
+// Minimizer bucketing. The home bucket of a k-mer may come from its
+// minimizer: the smallest 32-bit hash over its k-m+1 canonical m-mers
+// (canonical = min of the m-mer and its reverse complement, so a k-mer and
+// its reverse complement share it). Consecutive reference positions share
+// a minimizer for ~(k-m)/2 positions on average, so they probe the same
+// 128 B bucket and the wave's requests for them merge.
+__host__ __device__ inline uint32_t mmerHash(uint64_t canonMmer)
+{
+    return (uint32_t)((canonMmer * 0x9E3779B97F4A7C15ull) >> 32);
+}
+
+__host__ __device__ inline uint32_t minimizerOf(uint64_t key, uint32_t k, uint32_t m)
+{
+    const uint64_t maskm = (1ull << (2 * m)) - 1ull;
+    uint32_t best = 0xffffffffu;
+    for (uint32_t i = 0; i + m <= k; ++i)
+    {
+        const uint64_t mm = (key >> (2 * (k - m - i))) & maskm;
+        const uint64_t rc = reverse2(~mm & maskm) >> (64 - 2 * m);
+        const uint32_t h = mmerHash(mm < rc ? mm : rc);
+        best = h < best ? h : best;
+    }
+    return best;
+}
+
+// The minimum of hashes is skewed toward small values: re-mix it before
+// scaling onto the buckets.
+__host__ __device__ inline uint64_t bucketOfMinimizer(uint32_t minHash, uint64_t nBuckets)
+{
+    uint64_t x = (uint64_t)minHash + 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return ((x >> 32) * nBuckets) >> 32;
+}
+
+// Minimizer length a table of k-mers uses by default (0 = whole-k-mer hash).
+__host__ __device__ inline uint32_t autoMinimizerLen(uint32_t k)
+{
+    return k >= 21 ? (k - 8 < 17 ? k - 8 : 17) : 0;
+}
+
 // Device view of one table, passed by value to kernels.
 struct TableView
 {
     const uint64_t *slots;
     uint64_t nBuckets;
     uint32_t maxDisp; // largest bucket displacement of any stored key
+    uint32_t mlen;    // minimizer length of the home bucket, 0 = whole k-mer
+    uint32_t k;
     uint32_t pad;
 };
+
+__host__ __device__ inline uint64_t homeOf(uint64_t key, const TableView &T)
+{
+    return T.mlen ? bucketOfMinimizer(minimizerOf(key, T.k, T.mlen), T.nBuckets)
+                  : bucketOf(key, T.nBuckets);
+}
 
 // Run-length summary of a contiguous range of positions of one window
 // against one table (the monoid of the scan's reduction; see combine()).
@@ -162,11 +219,12 @@ struct hipibs_table
     uint64_t nKeys = 0;
     uint64_t bytes = 0;
     uint32_t maxDisp = 0;
+    uint32_t mlen = 0;
     uint64_t *dSlots = nullptr;
     uint64_t *dCounters = nullptr; // [0] inserted, [1] full flag, [2] max displacement
     hipibs::TableView view() const
     {
-        return hipibs::TableView{dSlots, nBuckets, maxDisp, 0};
+        return hipibs::TableView{dSlots, nBuckets, maxDisp, mlen, k, 0};
     }
 };
 

@@ -32,13 +32,6 @@ namespace hipibs
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_WAVES = SCAN_BLOCK / 64;
 
-// Reverses the order of the 32 2-bit groups of x.
-__device__ inline uint64_t reverse2(uint64_t x)
-{
-    const uint64_t v = __builtin_bitreverse64(x);
-    return ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
-}
-
 __device__ inline uint32_t waveSumU32(uint32_t v)
 {
 #pragma unroll
@@ -95,6 +88,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
         const uint64_t mBeg = mFirst + std::min<uint64_t>(nWords, (uint64_t)wv * per);
         const uint64_t mEnd = mFirst + std::min<uint64_t>(nWords, (uint64_t)(wv + 1) * per);
         const int j = lane & 31;
+        const uint32_t mlen = T.mlen;
+        const uint64_t maskm2 = mlen ? (1ull << (2 * mlen)) - 1ull : 0ull;
+        const uint32_t mShift = 64 - 2 * mlen;
+        const int win = (int)(k - mlen + 1); // m-mers per k-mer
         for (uint64_t m = mBeg; m < mEnd; ++m)
         {
             const uint64_t s0 = seq[2 * m], s1 = seq[2 * m + 1], s2 = seq[2 * m + 2];
@@ -102,6 +99,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
             const uint64_t a = lane < 32 ? s0 : s1;
             const uint64_t b = lane < 32 ? s1 : s2;
             uint64_t x = j ? ((a >> (2 * j)) | (b << (64 - 2 * j))) : a;
+            const uint64_t xm = x; // low-first window of bases p .. p+31
             x &= kmask2;
             const uint64_t ev = lane ? ((e0 >> lane) | (e1 << (64 - lane))) : e0;
             const uint64_t p = (m << 6) + (uint64_t)lane;
@@ -109,8 +107,42 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
             const uint64_t rc = ~x & kmask2;
             const uint64_t fwd = reverse2(x) >> fwdShift;
             const uint64_t canon = fwd < rc ? fwd : rc;
+            uint32_t home;
+            if (mlen)
+            {
+                // canonical m-mer hashes at positions p and p+64, then the
+                // minimum over each k-mer's window of `win` m-mers by a
+                // doubling sliding minimum across the wave
+                const uint64_t s3 = seq[2 * m + 3], s4 = seq[2 * m + 4];
+                const uint64_t a2 = lane < 32 ? s2 : s3;
+                const uint64_t b2 = lane < 32 ? s3 : s4;
+                uint64_t y = j ? ((a2 >> (2 * j)) | (b2 << (64 - 2 * j))) : a2;
+                uint64_t ylo = xm & maskm2;
+                y &= maskm2;
+                const uint64_t rlo = ~ylo & maskm2, rhi = ~y & maskm2;
+                const uint64_t flo = reverse2(ylo) >> mShift, fhi = reverse2(y) >> mShift;
+                uint32_t L = mmerHash(flo < rlo ? flo : rlo);
+                uint32_t Hh = mmerHash(fhi < rhi ? fhi : rhi);
+                int span = 1;
+                while (2 * span <= win)
+                {
+                    const int src = (lane + span) & 63;
+                    const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
+                    const bool inLo = lane + span < 64;
+                    L = min(L, inLo ? l2 : h2);
+                    Hh = min(Hh, inLo ? h2 : 0xffffffffu);
+                    span *= 2;
+                }
+                const int d = win - span;
+                const int src = (lane + d) & 63;
+                const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
+                const uint32_t minh = min(L, lane + d < 64 ? l2 : h2);
+                home = (uint32_t)bucketOfMinimizer(minh, T.nBuckets);
+            }
+            else
+                home = (uint32_t)bucketOf(canon, T.nBuckets);
 
-            const uint64_t H = waveProbe<S>(T, canon, valid);
+            const uint64_t H = waveProbe<S>(T, canon, home, valid);
             const uint64_t V = ballot(valid);
             const uint64_t Miss = V & ~H;
             if (H)
@@ -236,7 +268,8 @@ __global__ __launch_bounds__(256) void hitmaskKernel(const TableView T, const ui
             const uint64_t fwd = reverse2(x) >> fwdShift;
             canon = fwd < rc ? fwd : rc;
         }
-        const uint64_t H = waveProbe<S>(T, canon, valid);
+        const uint32_t home = valid ? (uint32_t)homeOf(canon, T) : 0u;
+        const uint64_t H = waveProbe<S>(T, canon, home, valid);
         if (lane < 2)
         {
             const uint64_t o = 2 * c + (uint64_t)lane;
@@ -456,8 +489,9 @@ int checkTables(const hipibs_table *const *tables, int nTables)
         if (!tables[i])
             return setError(HIPIBS_E_ARG, "null table");
         if (tables[i]->k != tables[0]->k || tables[i]->device != tables[0]->device ||
-            tables[i]->bucketSlots != tables[0]->bucketSlots)
-            return setError(HIPIBS_E_ARG, "tables differ in k, device or bucket width");
+            tables[i]->bucketSlots != tables[0]->bucketSlots || tables[i]->mlen != tables[0]->mlen)
+            return setError(HIPIBS_E_ARG,
+                            "tables differ in k, device, bucket width or minimizer length");
     }
     return 0;
 }

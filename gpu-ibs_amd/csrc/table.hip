@@ -24,10 +24,11 @@ namespace hipibs
 // counters[0] += newly stored keys, counters[1] = 1 when the table ran out
 // of buckets, counters[2] = max displacement (buckets past home).
 template <int S>
-__global__ __launch_bounds__(256) void insertKernel(uint64_t *slots, uint64_t nBuckets,
+__global__ __launch_bounds__(256) void insertKernel(uint64_t *slots, const TableView T,
                                                     const uint64_t *keys, uint64_t n,
                                                     unsigned long long *counters)
 {
+    const uint64_t nBuckets = T.nBuckets;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint32_t localNew = 0, localDisp = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -35,7 +36,7 @@ __global__ __launch_bounds__(256) void insertKernel(uint64_t *slots, uint64_t nB
         const uint64_t key = keys[i];
         if (key == EMPTY)
             continue;
-        uint64_t b = bucketOf(key, nBuckets);
+        uint64_t b = homeOf(key, T);
         bool done = false;
         for (uint64_t d = 0; d < nBuckets && !done; ++d)
         {
@@ -107,7 +108,8 @@ __global__ __launch_bounds__(256) void probeKernel(const TableView T, const uint
         const bool in = i < n;
         bool v = in && (valid == nullptr || valid[i] != 0);
         const uint64_t code = v ? codes[i] : 0ull;
-        const uint64_t H = waveProbe<S>(T, code, v);
+        const uint32_t home = v ? (uint32_t)homeOf(code, T) : 0u;
+        const uint64_t H = waveProbe<S>(T, code, home, v);
         if (hits && in)
             hits[i] = (uint8_t)((H >> lane) & 1ull);
         local += (uint64_t)__popcll(H);
@@ -132,10 +134,10 @@ int launchTableInsert(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStr
     const int grid = gridFor(n, 256, t->device);
     auto *ctr = reinterpret_cast<unsigned long long *>(t->dCounters);
     if (t->bucketSlots == 8)
-        hipLaunchKernelGGL(insertKernel<8>, dim3(grid), dim3(256), 0, s, t->dSlots, t->nBuckets,
+        hipLaunchKernelGGL(insertKernel<8>, dim3(grid), dim3(256), 0, s, t->dSlots, t->view(),
                            dKeys, n, ctr);
     else
-        hipLaunchKernelGGL(insertKernel<16>, dim3(grid), dim3(256), 0, s, t->dSlots, t->nBuckets,
+        hipLaunchKernelGGL(insertKernel<16>, dim3(grid), dim3(256), 0, s, t->dSlots, t->view(),
                            dKeys, n, ctr);
     HIPIBS_CHECK(hipGetLastError());
     return 0;
@@ -168,10 +170,13 @@ static int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts,
         o = *opts;
     if (o.load_factor <= 0)
         o.load_factor = 0.6;
-    if (o.bucket_slots == 0)
-        o.bucket_slots = 8;
     if (k == 0 || k > 32)
         return setError(HIPIBS_E_ARG, "k must be in 1..32");
+    uint32_t mlen = o.minimizer_len == 0 ? autoMinimizerLen(k) : (o.minimizer_len == 1 ? 0 : o.minimizer_len);
+    if (mlen && (mlen < 5 || mlen > 17 || mlen > k))
+        return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (auto), 1 (off) or 5..min(17, k)");
+    if (o.bucket_slots == 0)
+        o.bucket_slots = mlen ? 16 : 8;
     if (o.bucket_slots != 8 && o.bucket_slots != 16)
         return setError(HIPIBS_E_ARG, "bucket_slots must be 8 or 16");
     if (!(o.load_factor > 0 && o.load_factor < 0.98))
@@ -191,6 +196,7 @@ static int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts,
     t->device = o.device;
     t->k = k;
     t->bucketSlots = o.bucket_slots;
+    t->mlen = mlen;
     t->nBuckets = nBuckets;
     t->bytes = bytes;
     hipError_t e = hipMalloc(reinterpret_cast<void **>(&t->dSlots), bytes);
@@ -514,5 +520,13 @@ extern "C" int hipibs_table_build_kmc(const char *prefix, const hipibs_table_opt
         return rc;
     }
     *out = t;
+    return 0;
+}
+
+extern "C" int hipibs_table_minimizer_len(const hipibs_table *t, uint32_t *m)
+{
+    if (!t || !m)
+        return setError(HIPIBS_E_ARG, "null argument");
+    *m = t->mlen;
     return 0;
 }

@@ -108,9 +108,10 @@ __device__ inline int selectBit(uint64_t m, uint32_t n)
 }
 
 // Returns the 64-bit hit mask of the wave: bit i set when lane i's query is
-// valid and its code is stored in the table. Must be called by all 64 lanes.
+// valid and its code is stored in the table; `home` is the query's home
+// bucket (homeOf). Must be called by all 64 lanes.
 template <int S>
-__device__ inline uint64_t waveProbe(const TableView T, uint64_t code, bool valid)
+__device__ inline uint64_t waveProbe(const TableView T, uint64_t code, uint32_t home, bool valid)
 {
     constexpr int LPQ = S / 2;      // lanes per query
     constexpr int QPR = 64 / LPQ;   // queries per round
@@ -124,9 +125,13 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, bool vali
     // (vmcnt-only) so the LDS permutes never wait on them, and all ROUNDS
     // loads of the wave are in flight together.
     uint64_t qc[ROUNDS];
+    uint32_t qh[ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r)
+    {
         qc[r] = shuffle64(code, r * QPR + g);
+        qh[r] = shuffle32(home, r * QPR + g);
+    }
     SlotPair kv[ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r)
@@ -134,8 +139,7 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, bool vali
         kv[r] = SlotPair{EMPTY, EMPTY};
         if ((V >> (r * QPR + g)) & 1ull)
         {
-            const uint64_t b = bucketOf(qc[r], T.nBuckets);
-            kv[r] = loadPair(T.slots + b * S + (uint32_t)sub * 2u);
+            kv[r] = loadPair(T.slots + (uint64_t)qh[r] * S + (uint32_t)sub * 2u);
         }
     }
     uint64_t M[ROUNDS], E[ROUNDS];
@@ -162,8 +166,7 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, bool vali
     if (NC == 0 || T.maxDisp == 0)
         return H;
 
-    // Overflow rounds: pending lane i continues at home(code_i) + disp_i.
-    const uint64_t home = bucketOf(code, T.nBuckets);
+    // Overflow rounds: pending lane i continues at home_i + disp_i.
     uint32_t disp = 1;
     while (NC)
     {
@@ -173,7 +176,7 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, bool vali
         const bool serving = (uint32_t)g < nPend;
         const int src = serving ? selectBit(NC, (uint32_t)g) : 0;
         const uint64_t qcode = shuffle64(code, src);
-        const uint64_t qhome = shuffle64(home, src);
+        const uint64_t qhome = shuffle32(home, src);
         const uint32_t qdisp = shuffle32(disp, src);
         SlotPair kv2{EMPTY, EMPTY};
         if (serving)

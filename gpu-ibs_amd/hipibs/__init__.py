@@ -21,7 +21,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhipibs.so")
-PACK_PAD_WORDS = 4
+PACK_PAD_WORDS = 8
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
@@ -38,7 +38,8 @@ vp = C.c_void_p
 
 class TableOpts(C.Structure):
     _fields_ = [("load_factor", C.c_double), ("bucket_slots", C.c_uint32), ("device", C.c_int),
-                ("hbm_budget", C.c_uint64), ("load_threads", C.c_uint32)]
+                ("hbm_budget", C.c_uint64), ("load_threads", C.c_uint32),
+                ("minimizer_len", C.c_uint32)]
 
 
 class Window(C.Structure):
@@ -83,6 +84,7 @@ _SIGS = [
     ("hipibs_table_build_keys_device", C.c_int, [vp, C.c_uint64, C.c_uint32,
                                                  C.POINTER(TableOpts), C.POINTER(vp), vp]),
     ("hipibs_table_free", C.c_int, [vp]),
+    ("hipibs_table_minimizer_len", C.c_int, [vp, u32p]),
     ("hipibs_table_info", C.c_int, [vp, u64p, u64p, u64p, u32p, u32p, C.POINTER(C.c_int)]),
     ("hipibs_table_probe", C.c_int, [vp, vp, vp, C.c_uint64, vp, vp]),
     ("hipibs_table_probe_count", C.c_int, [vp, u64p, C.c_uint64, u64p]),
@@ -349,8 +351,9 @@ def synchronize(device: int = 0) -> None:
 
 # ---- tables -------------------------------------------------------------------
 
-def _opts(load_factor=0.6, bucket_slots=8, device=0, hbm_budget=0, load_threads=0):
-    return TableOpts(load_factor, bucket_slots, device, hbm_budget, load_threads)
+def _opts(load_factor=0.6, bucket_slots=0, device=0, hbm_budget=0, load_threads=0,
+          minimizer_len=0):
+    return TableOpts(load_factor, bucket_slots, device, hbm_budget, load_threads, minimizer_len)
 
 
 class Table:
@@ -389,8 +392,13 @@ class Table:
         k, bs, dev = C.c_uint32(), C.c_uint32(), C.c_int()
         _check(_lib.hipibs_table_info(self.handle, C.byref(nk), C.byref(nb), C.byref(nbk),
                                       C.byref(k), C.byref(bs), C.byref(dev)), "hipibs_table_info")
-        return dict(n_keys=nk.value, bytes=nb.value, n_buckets=nbk.value, k=k.value,
-                    bucket_slots=bs.value, device=dev.value)
+        d = dict(n_keys=nk.value, bytes=nb.value, n_buckets=nbk.value, k=k.value,
+                 bucket_slots=bs.value, device=dev.value)
+        ml = C.c_uint32(0)
+        _check(_lib.hipibs_table_minimizer_len(self.handle, C.byref(ml)),
+               "hipibs_table_minimizer_len")
+        d["minimizer_len"] = ml.value
+        return d
 
     def probe_count(self, keys: np.ndarray) -> int:
         keys = np.ascontiguousarray(keys, dtype=np.uint64)

@@ -42,18 +42,23 @@ extern "C" {
 #define HIPIBS_E_FULL (-5)     /* table capacity exhausted during a build */
 #define HIPIBS_E_NODEV (-6)    /* no usable GPU */
 
-#define HIPIBS_PACK_PAD_WORDS 4
+#define HIPIBS_PACK_PAD_WORDS 8
 
 typedef struct hipibs_table hipibs_table;
 typedef struct hipibs_scan hipibs_scan;
 
-/* Table construction options (zero-initialised fields take the defaults). */
+/* Table construction options (zero-initialised fields take the defaults).
+ * Tables scanned in one launch must share k, bucket width and minimizer
+ * length. */
 typedef struct hipibs_table_opts {
     double load_factor;     /* keys / slots, default 0.6 */
     uint32_t bucket_slots;  /* 8 (64 B bucket) or 16 (128 B bucket), default 8 */
     int device;             /* HIP device ordinal, default 0 */
     uint64_t hbm_budget;    /* max table bytes, 0 = no limit */
     uint32_t load_threads;  /* host threads of a KMC read, 0 = all CPUs */
+    uint32_t minimizer_len; /* home bucket from the k-mer's minimizer of this length
+                               (consecutive positions then share buckets); 0 = auto
+                               (on for k >= 21), 1 = off (hash of the whole k-mer) */
 } hipibs_table_opts;
 
 typedef struct hipibs_window {
@@ -164,6 +169,10 @@ int hipibs_table_free(hipibs_table *t);
 /* Replaces DeviceKmerIndex::footprintBytes / view (src/gpu/DeviceKmerIndex.cuh:86-110). */
 int hipibs_table_info(const hipibs_table *t, uint64_t *n_keys, uint64_t *bytes,
                       uint64_t *n_buckets, uint32_t *k, uint32_t *bucket_slots, int *device);
+
+/* Minimizer length the table's home buckets use (0 = whole-k-mer hash).
+ * No counterpart in the reference (its index has no such choice). */
+int hipibs_table_minimizer_len(const hipibs_table *t, uint32_t *m);
 
 /* Replaces DeviceKmerIndex::probe(d_q, d_valid, n, d_hits, stream)
  * (src/gpu/DeviceKmerIndex.cuh:60-66): d_hits[i] = 1 if d_valid[i] (NULL =
