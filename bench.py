@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--keys", type=float, default=2.0e9, help="target distinct keys")
     ap.add_argument("--bucket-slots", type=int, default=0, help="0 = library default")
     ap.add_argument("--minimizer-len", type=int, default=0,
-                    help="0 = library default (minimizer buckets for k >= 21), 1 = off")
+                    help="0 = whole-k-mer hash buckets (default), 5..17 = minimizer buckets")
     ap.add_argument("--load-factor", type=float, default=0.6)
     ap.add_argument("--segs", type=int, default=0, help="segments per window (0 = auto)")
     ap.add_argument("--cpu-mbp", type=float, default=256.0,

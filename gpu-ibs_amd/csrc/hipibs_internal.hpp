@@ -75,12 +75,6 @@ __host__ __device__ inline uint64_t bucketOfMinimizer(uint32_t minHash, uint64_t
     return ((x >> 32) * nBuckets) >> 32;
 }
 
-// Minimizer length a table of k-mers uses by default (0 = whole-k-mer hash).
-__host__ __device__ inline uint32_t autoMinimizerLen(uint32_t k)
-{
-    return k >= 21 ? (k - 8 < 17 ? k - 8 : 17) : 0;
-}
-
 // Device view of one table, passed by value to kernels.
 struct TableView
 {

@@ -172,9 +172,9 @@ static int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts,
         o.load_factor = 0.6;
     if (k == 0 || k > 32)
         return setError(HIPIBS_E_ARG, "k must be in 1..32");
-    uint32_t mlen = o.minimizer_len == 0 ? autoMinimizerLen(k) : (o.minimizer_len == 1 ? 0 : o.minimizer_len);
+    const uint32_t mlen = o.minimizer_len;
     if (mlen && (mlen < 5 || mlen > 17 || mlen > k))
-        return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (auto), 1 (off) or 5..min(17, k)");
+        return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (off) or 5..min(17, k)");
     if (o.bucket_slots == 0)
         o.bucket_slots = mlen ? 16 : 8;
     if (o.bucket_slots != 8 && o.bucket_slots != 16)

@@ -56,9 +56,11 @@ typedef struct hipibs_table_opts {
     int device;             /* HIP device ordinal, default 0 */
     uint64_t hbm_budget;    /* max table bytes, 0 = no limit */
     uint32_t load_threads;  /* host threads of a KMC read, 0 = all CPUs */
-    uint32_t minimizer_len; /* home bucket from the k-mer's minimizer of this length
-                               (consecutive positions then share buckets); 0 = auto
-                               (on for k >= 21), 1 = off (hash of the whole k-mer) */
+    uint32_t minimizer_len; /* 0 (default): home bucket from a hash of the whole k-mer;
+                               5..17: from the k-mer's minimizer of that length, so
+                               consecutive positions share buckets (fewer requests on
+                               non-repetitive sequence, long overflow chains on
+                               repeat-rich sequence) */
 } hipibs_table_opts;
 
 typedef struct hipibs_window {
