@@ -202,4 +202,26 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, uint32_t 
     return H;
 }
 
+__device__ inline uint32_t waveSumU32(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        v += (uint32_t)__shfl_xor((int)v, o, 64);
+    return v;
+}
+
+// Positions [g0, g1) of window w for segment s of segs: the window's
+// positions split into segs 64-aligned pieces.
+__device__ inline void segmentRange(const hipibs_window &win, uint64_t nBases, uint32_t k,
+                                    uint32_t s, uint32_t segs, uint64_t &g0, uint64_t &g1)
+{
+    uint64_t npos = win.n_bases >= k ? (uint64_t)win.n_bases - k + 1 : 0;
+    // defensive clamp to the stream (the host checks windows it can see)
+    if (win.base + (uint64_t)win.n_bases > nBases)
+        npos = (nBases >= win.base + k) ? nBases - win.base - k + 1 : 0;
+    const uint64_t per = ((npos + segs - 1) / segs + 63) & ~63ull;
+    g0 = win.base + std::min<uint64_t>(npos, (uint64_t)s * per);
+    g1 = win.base + std::min<uint64_t>(npos, (uint64_t)(s + 1) * per);
+}
+
 } // namespace hipibs

@@ -58,6 +58,11 @@ class KmcInfo(C.Structure):
                 ("max_count", C.c_uint32), ("total_kmers", C.c_uint64)]
 
 
+class ScanConfig(C.Structure):
+    _fields_ = [("segs_per_window", C.c_uint32), ("waves_per_cu", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+
 class HostSlot(C.Structure):
     _fields_ = [("seq_words", u64p), ("exc_words", u64p), ("windows", C.POINTER(Window)),
                 ("stats", C.POINTER(WindowStats)), ("seq_cap", C.c_uint64),
@@ -106,6 +111,18 @@ _SIGS = [
     ("hipibs_scan_submit", C.c_int, [vp, C.c_int, C.c_uint32, C.c_uint64, C.c_uint64]),
     ("hipibs_scan_wait", C.c_int, [vp, C.c_int]),
     ("hipibs_scan_free", C.c_int, [vp]),
+    ("hipibs_panel_work_bytes", C.c_uint64, [C.c_uint32, C.c_int, C.c_uint32]),
+    ("hipibs_panel_scan_device", C.c_int, [C.POINTER(vp), C.c_int, vp, vp, C.c_uint64, vp,
+                                           C.c_uint32, vp, C.POINTER(ScanConfig), vp, C.c_uint64,
+                                           vp]),
+    ("hipibs_panel_autotune", C.c_int, [C.POINTER(vp), C.c_int, vp, vp, C.c_uint64, vp,
+                                        C.c_uint32, C.c_uint32, vp, vp, C.c_uint64,
+                                        C.POINTER(ScanConfig), C.POINTER(C.c_double)]),
+    ("hipibs_panel_graph_create", C.c_int, [C.POINTER(vp), C.c_int, vp, vp, C.c_uint64, vp,
+                                            C.c_uint32, vp, C.POINTER(ScanConfig), C.c_uint32,
+                                            vp, C.c_uint64, C.POINTER(vp)]),
+    ("hipibs_graph_launch", C.c_int, [vp, vp]),
+    ("hipibs_graph_free", C.c_int, [vp]),
     ("hipibs_dev_alloc", C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
     ("hipibs_dev_free", C.c_int, [vp]),
     ("hipibs_memcpy", C.c_int, [vp, vp, C.c_uint64, C.c_int]),

@@ -218,6 +218,53 @@ int hipibs_scan_windows_device(const hipibs_table *const *tables, int n_tables,
 uint64_t hipibs_scan_work_bytes(uint32_t n_windows, int n_tables, uint32_t max_window_bases,
                                 uint32_t k, uint32_t segs_per_window, uint32_t *segs_resolved);
 
+/* ---- panel scans (many samples, one reference pass) ----------------------- */
+
+/* Launch knobs of the persistent panel scan. Zero fields take defaults. */
+typedef struct hipibs_scan_config {
+    uint32_t segs_per_window; /* block work units per window (default 1) */
+    uint32_t waves_per_cu;    /* persistent grid size (default 16) */
+    uint32_t reserved[2];
+} hipibs_scan_config;
+
+/* Scratch bytes of a panel scan of n_windows windows against n_tables. */
+uint64_t hipibs_panel_work_bytes(uint32_t n_windows, int n_tables, uint32_t segs_per_window);
+
+/* The panel scan (KmerPanel's device pass, src/KmerPanel.hpp:400-439, with
+ * encodeProbeWordMulti's one-encode-many-probes shape,
+ * src/gpu/EncodeProbeCore.hpp:291-328): one persistent launch per 16 tables,
+ * every position encoded once and probed against each table; results as
+ * hipibs_scan_windows_device (table-major). Tables need whole-k-mer
+ * buckets (minimizer_len 0). Asynchronous on `stream`. */
+int hipibs_panel_scan_device(const hipibs_table *const *tables, int n_tables,
+                             const uint64_t *d_seq_words, const uint64_t *d_exc_words,
+                             uint64_t n_bases, const hipibs_window *d_windows, uint32_t n_windows,
+                             hipibs_window_stats *d_stats, const hipibs_scan_config *cfg,
+                             void *d_work, uint64_t work_bytes, void *stream);
+
+/* Times (waves per CU, segments per window) candidates on the first
+ * sample_windows windows (0 = 4096) and returns the fastest in *best. No
+ * counterpart in the reference (its launch shape is fixed). Synchronous. */
+int hipibs_panel_autotune(const hipibs_table *const *tables, int n_tables,
+                          const uint64_t *d_seq_words, const uint64_t *d_exc_words,
+                          uint64_t n_bases, const hipibs_window *d_windows, uint32_t n_windows,
+                          uint32_t sample_windows, hipibs_window_stats *d_stats, void *d_work,
+                          uint64_t work_bytes, hipibs_scan_config *best, double *best_ms);
+
+typedef struct hipibs_graph hipibs_graph;
+
+/* Captures a panel scan split into n_chunks window ranges (per chunk: counter
+ * reset, panel kernel(s), fold) into a hipGraph; hipibs_graph_launch replays
+ * it. The captured pointers must stay valid. No counterpart in the reference. */
+int hipibs_panel_graph_create(const hipibs_table *const *tables, int n_tables,
+                              const uint64_t *d_seq_words, const uint64_t *d_exc_words,
+                              uint64_t n_bases, const hipibs_window *d_windows, uint32_t n_windows,
+                              hipibs_window_stats *d_stats, const hipibs_scan_config *cfg,
+                              uint32_t n_chunks, void *d_work, uint64_t work_bytes,
+                              hipibs_graph **out);
+int hipibs_graph_launch(hipibs_graph *g, void *stream);
+int hipibs_graph_free(hipibs_graph *g);
+
 /* Per-position hit mask over positions [pos0, pos0+n_pos) of a packed
  * stream: bit (p-pos0)%32 of word (p-pos0)/32 of d_hit_words is set when
  * position p is a valid k-mer of the stream's contig (all k bases ACGT and

@@ -232,3 +232,91 @@ def test_build_from_kmc_equals_from_keys(small_dbs, small_corpus):
     assert a.info()["n_keys"] == keys.size
     assert a.probe_count(samples[0][0]) == int(np.isin(samples[0][0], samples[1][0]).sum())
     a.free()
+
+
+def _packed_windows(seq, W, k):
+    sw, ew, _ = hipibs.pack_sequences([seq])
+    wins = hipibs.chunk_sequences([len(seq)], W, k)
+    win = np.zeros(len(wins), dtype=hipibs.WINDOW_DTYPE)
+    for i, (_, s, e) in enumerate(wins):
+        win[i] = (s, e - s, 0)
+    return D.from_array(sw), D.from_array(ew), D.from_array(win), wins
+
+
+@pytest.mark.parametrize("n_tables", [1, 3, 17])
+def test_panel_kernel_equals_single_scans(small_corpus, n_tables):
+    """The persistent one-encode-many-probes panel kernel (16 tables per
+    launch, so 17 spans two) gives each table's single-scan statistics, for
+    several launch configurations."""
+    import ctypes as C
+    ref, samples = small_corpus
+    seq = corpus.to_text(ref)
+    keysets = [samples[i % 3][0][: (samples[i % 3][0].size * (i + 5)) // (n_tables + 5)]
+               for i in range(n_tables)]
+    tabs = [hipibs.Table.from_keys(ks, 25, bucket_slots=(8 if i % 2 == 0 else 8))
+            for i, ks in enumerate(keysets)]
+    d_sw, d_ew, d_win, wins = _packed_windows(seq, 20000, 25)
+    nw = len(wins)
+    arr = hipibs._table_array(tabs)
+    want = [oracle_scan_rows([b"chr1"], [seq], 20000, 25, sorted_keys(ks)) for ks in keysets]
+    for segs, wpc in ((1, 16), (3, 8), (2, 32)):
+        note(f"panel n={n_tables} segs={segs} wpc={wpc}")
+        wb = hipibs._lib.hipibs_panel_work_bytes(nw, n_tables, segs)
+        d_work = D.zeros(wb)
+        d_st = D.zeros(nw * n_tables * 16)
+        cfg = hipibs.ScanConfig(segs, wpc)
+        hipibs._check(hipibs._lib.hipibs_panel_scan_device(
+            arr, n_tables, d_sw.ptr, d_ew.ptr, len(seq), d_win.ptr, nw, d_st.ptr, C.byref(cfg),
+            d_work.ptr, wb, None), "panel")
+        hipibs.synchronize()
+        st = d_st.to_array(np.int32).reshape(n_tables, nw, 4)
+        for t in range(n_tables):
+            got = [(b"chr1", s, e) + tuple(int(x) for x in st[t][i])
+                   for i, (_, s, e) in enumerate(wins)]
+            assert got == want[t], (t, segs, wpc)
+    for t in tabs:
+        t.free()
+
+
+def test_panel_graph_replays_and_autotune(small_corpus):
+    """A captured panel graph (4 chunks) replayed three times, with eager
+    work between the replays, matches the eager result every time; the
+    autotuner returns a configuration that runs."""
+    import ctypes as C
+    ref, samples = small_corpus
+    seq = corpus.to_text(ref)
+    tabs = [hipibs.Table.from_keys(samples[i][0], 25) for i in range(3)]
+    d_sw, d_ew, d_win, wins = _packed_windows(seq, 10000, 25)
+    nw = len(wins)
+    arr = hipibs._table_array(tabs)
+    best = hipibs.ScanConfig()
+    ms = C.c_double(0)
+    wb = hipibs._lib.hipibs_panel_work_bytes(nw, 3, 4)
+    d_work = D.zeros(wb)
+    d_eager = D.zeros(nw * 3 * 16)
+    hipibs._check(hipibs._lib.hipibs_panel_autotune(
+        arr, 3, d_sw.ptr, d_ew.ptr, len(seq), d_win.ptr, nw, 64, d_eager.ptr, d_work.ptr, wb,
+        C.byref(best), C.byref(ms)), "autotune")
+    assert best.segs_per_window in (1, 2, 4) and best.waves_per_cu in (8, 12, 16, 24, 32)
+    assert ms.value > 0
+    hipibs._check(hipibs._lib.hipibs_panel_scan_device(
+        arr, 3, d_sw.ptr, d_ew.ptr, len(seq), d_win.ptr, nw, d_eager.ptr, C.byref(best),
+        d_work.ptr, wb, None), "panel")
+    hipibs.synchronize()
+    eager = d_eager.to_array(np.int32)
+    d_g = D.zeros(nw * 3 * 16)
+    g = C.c_void_p()
+    hipibs._check(hipibs._lib.hipibs_panel_graph_create(
+        arr, 3, d_sw.ptr, d_ew.ptr, len(seq), d_win.ptr, nw, d_g.ptr, C.byref(best), 4,
+        d_work.ptr, wb, C.byref(g)), "graph")
+    for rep in range(3):
+        hipibs._check(hipibs._lib.hipibs_dev_memset(C.c_void_p(d_g.ptr), 0xFF, nw * 3 * 16),
+                      "memset")
+        hipibs._check(hipibs._lib.hipibs_graph_launch(g, None), "graph launch")
+        hipibs.synchronize()
+        assert np.array_equal(d_g.to_array(np.int32), eager), rep
+        # eager work between replays
+        assert tabs[0].probe_count(samples[1][0][:1000]) >= 0
+    hipibs._lib.hipibs_graph_free(g)
+    for t in tabs:
+        t.free()
