@@ -51,6 +51,13 @@ def parse():
                     help="reference slice (Mbp) of the CPU baseline sample; 0 = skip")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0x1B50200)
+    ap.add_argument("--workload", default="scan", choices=["scan", "panel"],
+                    help="scan: the single-sample device scan (default line); panel: "
+                         "end-to-end fastibs runs, N single-sample runs vs one panel run")
+    ap.add_argument("--panel-samples", type=int, default=8)
+    ap.add_argument("--panel-min-keys", type=float, default=0.1e9)
+    ap.add_argument("--panel-max-keys", type=float, default=0.5e9)
+    ap.add_argument("--workdir", default=os.environ.get("TMPDIR", "/tmp"))
     return ap.parse_args()
 
 
@@ -224,8 +231,123 @@ def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_c
             match)
 
 
+def write_fasta(path, codes_cpu, name=b"chr1", width=80):
+    import numpy as np
+    lut = np.frombuffer(b"ACGTN", dtype=np.uint8)
+    n = codes_cpu.size
+    full = n // width
+    with open(path, "wb") as f:
+        f.write(b">" + name + b"\n")
+        step = (1 << 24) // width * width
+        for b0 in range(0, full * width, step):
+            b1 = min(full * width, b0 + step)
+            block = lut[codes_cpu[b0:b1]].reshape(-1, width)
+            out = np.empty((block.shape[0], width + 1), dtype=np.uint8)
+            out[:, :width] = block
+            out[:, width] = ord("\n")
+            f.write(out.tobytes())
+        if full * width < n:
+            f.write(lut[codes_cpu[full * width:]].tobytes() + b"\n")
+
+
+def run_panel(args):
+    """Panel samples/hr, end to end: N plain fastibs runs (one sample each)
+    vs one fastibs run with the other N-1 samples in --db-list; the TSVs
+    must be byte-identical."""
+    import shutil
+    import subprocess
+
+    import numpy as np
+    import torch
+
+    import hipibs
+    n = int(args.gbp * 1e9) // 64 * 64
+    N = args.panel_samples
+    root = os.path.join(args.workdir, "hipibs_panel")
+    shutil.rmtree(root, ignore_errors=True)
+    os.makedirs(os.path.join(root, "refs"))
+    free = shutil.disk_usage(root).free
+    log(f"panel workdir {root}: {free/1e9:.0f} GB free")
+    ref_codes = gen_reference(torch, n, args.seed)
+    write_fasta(os.path.join(root, "refs", "panel_genome.fasta"), ref_codes.cpu().numpy())
+    log("reference FASTA written")
+    rng = np.random.default_rng(args.seed)
+    sizes = np.exp(rng.uniform(np.log(args.panel_min_keys), np.log(args.panel_max_keys), N))
+    rates = rng.uniform(0.001, 0.05, N)
+    need = float(sizes.sum()) * 6 + 3e9
+    if need > free * 0.8:
+        raise RuntimeError(f"panel needs ~{need/1e9:.0f} GB of disk, {free/1e9:.0f} GB free")
+    dbs = []
+    for i in range(N):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(args.seed + 1000 + i)
+        hap = ref_codes.clone()
+        snp = (torch.rand(n, device="cuda", generator=g) < float(rates[i])) & (hap < 4)
+        hap[snp] = (hap[snp] + torch.randint(1, 4, (n,), dtype=torch.uint8, device="cuda",
+                                             generator=g)[snp]) % 4
+        hsw, hew = pack(torch, hap)
+        del hap, snp
+        codes = encode(torch, hsw, hew, n, K)
+        del hsw, hew
+        keys = make_sample_keys(torch, codes, float(sizes[i]), K, args.seed + 2000 + i)
+        del codes
+        hk = keys.cpu().numpy().view(np.uint64)
+        del keys
+        torch.cuda.empty_cache()
+        d = os.path.join(root, f"s{i:02d}")
+        os.makedirs(d)
+        hipibs.kmc_write(os.path.join(d, "sample"), hk, K, p=5, sig_len=9, counter_size=1,
+                         version=0x200, min_count=1, max_count=255)
+        dbs.append(d)
+        log(f"sample {i}: {hk.size/1e9:.3f} G keys (SNP rate {rates[i]*100:.2f}%)")
+    del ref_codes
+    torch.cuda.empty_cache()
+    fastibs = os.path.join(ROOT, "gpu-ibs_amd", "bin", "fastibs")
+    refs = os.path.join(root, "refs")
+
+    def run(cmd):
+        t0 = time.time()
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{cmd[0]} failed: {r.stderr[-2000:]}")
+        return time.time() - t0, r.stdout
+
+    plain_s = []
+    for d in dbs:
+        t, out = run([fastibs, d, refs, os.path.join(root, "out_plain"), str(W)])
+        plain_s.append(t)
+        log(f"plain run {os.path.basename(d)}: {t:.2f} s")
+    with open(os.path.join(root, "list.txt"), "w") as f:
+        f.write("\n".join(dbs[1:]) + "\n")
+    t_panel, out = run([fastibs, dbs[0], refs, os.path.join(root, "out_panel"), str(W),
+                        f"--db-list={os.path.join(root, 'list.txt')}"])
+    phases = [l for l in out.splitlines() if l.startswith(("Panel", "Autotuned"))]
+    log(f"panel run: {t_panel:.2f} s; " + " | ".join(phases))
+    same = True
+    for d in dbs:
+        name = f"{os.path.basename(d)}_v_panel_{W}.tsv"
+        a = open(os.path.join(root, "out_plain", name), "rb").read()
+        b = open(os.path.join(root, "out_panel", name), "rb").read()
+        same &= a == b
+    plain_rate = N / sum(plain_s) * 3600
+    panel_rate = N / t_panel * 3600
+    res = {"metric": "panel samples/hr", "value": panel_rate, "unit": "samples/hr",
+           "plain_value": plain_rate, "ratio_vs_plain": panel_rate / plain_rate,
+           "outputs_identical": same, "n_gpus": 1, "higher_is_better": True,
+           "config": {"workload": f"panel, k=25: {n/1e9:.1f} Gbp synthetic reference vs {N} "
+                                  f"samples, keys log-uniform {args.panel_min_keys/1e9:.2f}-"
+                                  f"{args.panel_max_keys/1e9:.2f} G, SNP 0.1-5%, W=50 kbp; "
+                                  f"end-to-end fastibs runs from FASTA + KMC files",
+                      "sample_keys": [int(x) for x in sizes], "plain_run_s": plain_s,
+                      "panel_run_s": t_panel, "panel_phases": phases}}
+    print(json.dumps(res), flush=True)
+    shutil.rmtree(root, ignore_errors=True)
+
+
 def main():
     args = parse()
+    if args.workload == "panel":
+        return run_panel(args)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)

@@ -90,11 +90,27 @@ static int run(int argc, char **argv)
         return 1;
     if (!cli::prepareResultsFolder(resultsFolder))
         return 1;
-    KmerPanel panel(prefixes, opt.resident);
-    if (!cli::windowFitsK(W, panel.getKmerSize()))
+    // One sample: the streamed drop-in path (pinned, double-buffered batches
+    // of the reference). Several: the panel path (reference resident on the
+    // device, memory-budgeted rounds of tables, one graph replay per round).
+    unique_ptr<KmerDatabase> single;
+    unique_ptr<KmerPanel> panel;
+    uint32_t k = 0;
+    if (prefixes.size() == 1)
     {
-        cerr << "Error: <windowSize> " << W << " must be larger than the database's k ("
-             << panel.getKmerSize() << ")" << endl;
+        single = make_unique<KmerDatabase>(prefixes[0]);
+        single->printKMCInfo();
+        k = single->getKmerSize();
+    }
+    else
+    {
+        panel = make_unique<KmerPanel>(prefixes, opt.resident);
+        k = panel->getKmerSize();
+    }
+    if (!cli::windowFitsK(W, k))
+    {
+        cerr << "Error: <windowSize> " << W << " must be larger than the database's k (" << k
+             << ")" << endl;
         return 1;
     }
     int failed = 0;
@@ -121,7 +137,10 @@ static int run(int argc, char **argv)
             continue;
         try
         {
-            panel.processReference(ref.string(), outs, (int)W);
+            if (single)
+                single->processReference(ref.string(), outs[0], (int)W);
+            else
+                panel->processReference(ref.string(), outs, (int)W);
         }
         catch (const exception &e)
         {

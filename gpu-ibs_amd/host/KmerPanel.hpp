@@ -9,6 +9,7 @@
 #include "KmerDatabase.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <memory>
 
@@ -44,7 +45,7 @@ public:
         const auto &g = KmerDatabase::gpuOptions();
         uint64_t freeB = 0, totalB = 0;
         hipibs::check(hipibs_device_mem(g.device, &freeB, &totalB), "device memory query");
-        const uint64_t reserve = g.batchBases / 2 * (uint64_t)g.depth + (4ull << 30);
+        const uint64_t reserve = reserveBytes_ + (4ull << 30);
         uint64_t budget = freeB > reserve ? freeB - reserve : 0;
         if (g.hbmBudget)
             budget = std::min(budget, g.hbmBudget);
@@ -71,18 +72,92 @@ public:
         return rounds;
     }
 
+    // The whole reference packed once into device memory: contigs
+    // concatenated, every window a (base, length) range of it.
+    struct DeviceReference
+    {
+        hipibs::DeviceBuffer seq, exc, windows;
+        uint64_t nBases = 0, bytes = 0;
+        uint32_t nWindows = 0;
+    };
+
+    static void packReference(const KmerDatabase::ReferenceWindows &rw, int device,
+                              DeviceReference &out)
+    {
+        const auto &seqs = rw.parsed.sequences;
+        std::vector<uint64_t> off(seqs.size() + 1, 0);
+        for (size_t c = 0; c < seqs.size(); ++c)
+            off[c + 1] = off[c] + (seqs[c].size() + 63) / 64 * 64; // contigs 64-aligned
+        const uint64_t n = off.back();
+        const uint64_t nsw = n / 32 + HIPIBS_PACK_PAD_WORDS, new_ = n / 64 + HIPIBS_PACK_PAD_WORDS;
+        std::vector<uint64_t> sw(nsw, 0), ew(new_, 0);
+        // pieces of <= 4 Mbases, 64-aligned in the stream, packed on threads
+        struct Piece
+        {
+            const char *text;
+            uint64_t n, at;
+        };
+        std::vector<Piece> pieces;
+        for (size_t c = 0; c < seqs.size(); ++c)
+            for (uint64_t o = 0; o < seqs[c].size(); o += (4ull << 20))
+                pieces.push_back(Piece{seqs[c].data() + o,
+                                       std::min<uint64_t>(4ull << 20, seqs[c].size() - o), off[c] + o});
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t i; (i = next.fetch_add(1)) < pieces.size();)
+                hipibs_pack_bases(pieces[i].text, pieces[i].n, pieces[i].at, sw.data(), ew.data());
+        };
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < std::max(1u, std::thread::hardware_concurrency()); ++t)
+            pool.emplace_back(work);
+        for (auto &th : pool)
+            th.join();
+        std::vector<hipibs_window> win(rw.chunks.size());
+        std::vector<const std::string *> contigId(seqs.size());
+        for (size_t i = 0; i < rw.chunks.size(); ++i)
+        {
+            const SeqWindow &w = rw.chunks[i];
+            const size_t c = (size_t)(w.id - rw.parsed.ids.data());
+            win[i] = hipibs_window{off[c] + w.start, (uint32_t)(w.end - w.start), 0};
+        }
+        out.seq = hipibs::DeviceBuffer(device, nsw * 8);
+        out.exc = hipibs::DeviceBuffer(device, new_ * 8);
+        out.windows = hipibs::DeviceBuffer(device, std::max<size_t>(1, win.size()) * sizeof(hipibs_window));
+        hipibs::check(hipibs_memcpy(out.seq.get(), sw.data(), nsw * 8, 1), "reference upload");
+        hipibs::check(hipibs_memcpy(out.exc.get(), ew.data(), new_ * 8, 1), "reference upload");
+        hipibs::check(hipibs_memcpy(out.windows.get(), win.data(), win.size() * sizeof(hipibs_window), 1),
+                      "window upload");
+        out.nBases = n;
+        out.nWindows = (uint32_t)win.size();
+        out.bytes = nsw * 8 + new_ * 8 + win.size() * sizeof(hipibs_window);
+    }
+
     // outPaths[i] is database i's TSV; an empty path skips the database.
+    // Device-resident path: the reference is packed and uploaded once;
+    // each memory-budgeted round builds its tables and replays one captured
+    // graph of chunked panel-kernel steps (launch knobs autotuned once).
     void processReference(const std::string &refPath, const std::vector<std::string> &outPaths,
                           int windowSize)
     {
+        using clk = std::chrono::steady_clock;
         std::string why;
         if (hipibs::deviceCount(why) < 1)
             throw std::runtime_error("no usable GPU: " + why);
+        const auto t0 = clk::now();
         KmerDatabase::ReferenceWindows rw;
         dbs_[0]->readReferenceWindows(refPath, windowSize, rw);
+        const auto &g = KmerDatabase::gpuOptions();
+        DeviceReference dref;
+        packReference(rw, g.device, dref);
+        const double ingestSec = std::chrono::duration<double>(clk::now() - t0).count();
+        reserveBytes_ = dref.bytes;
         const auto rounds = planRounds();
         std::cout << "Panel: " << dbs_.size() << " databases in " << rounds.size()
-                  << " device round(s)" << std::endl;
+                  << " device round(s); reference " << dref.nBases << " bases, "
+                  << dref.nWindows << " windows, ingest " << ingestSec << " s" << std::endl;
+        hipibs_scan_config cfg{0, 0, {0, 0}};
+        bool tuned = false;
+        double buildSec = 0, scanSec = 0, writeSec = 0;
         for (const auto &round : rounds)
         {
             std::vector<size_t> todo;
@@ -91,22 +166,80 @@ public:
                     todo.push_back(i);
             if (todo.empty())
                 continue;
+            auto tb = clk::now();
             std::vector<const hipibs_table *> tabs;
             for (size_t i : todo)
             {
                 dbs_[i]->initGpuIndex();
                 tabs.push_back(dbs_[i]->table());
             }
-            auto res = hipibs::scanWindowsStreamed(tabs, rw.chunks, KmerDatabase::scanOptions());
+            auto ts = clk::now();
+            buildSec += std::chrono::duration<double>(ts - tb).count();
+            const int nt = (int)tabs.size();
+            hipibs::DeviceBuffer stats(g.device, std::max<uint64_t>(1, (uint64_t)dref.nWindows * nt) *
+                                                     sizeof(hipibs_window_stats));
+            const uint64_t wb = hipibs_panel_work_bytes(dref.nWindows, nt, 4);
+            hipibs::DeviceBuffer work(g.device, wb);
+            if (!tuned && dref.nWindows)
+            {
+                double ms = 0;
+                hipibs::check(hipibs_panel_autotune(tabs.data(), nt, dref.seq.as<uint64_t>(),
+                                                    dref.exc.as<uint64_t>(), dref.nBases,
+                                                    dref.windows.as<hipibs_window>(), dref.nWindows,
+                                                    2048, stats.as<hipibs_window_stats>(),
+                                                    work.get(), wb, &cfg, &ms),
+                              "panel autotune");
+                std::cout << "Autotuned panel launch: " << cfg.waves_per_cu << " waves/CU, "
+                          << cfg.segs_per_window << " segment(s)/window" << std::endl;
+                tuned = true;
+            }
+            std::vector<hipibs_window_stats> host((uint64_t)dref.nWindows * nt);
+            if (dref.nWindows)
+            {
+                hipibs_graph *graph = nullptr;
+                hipibs::check(hipibs_panel_graph_create(tabs.data(), nt, dref.seq.as<uint64_t>(),
+                                                        dref.exc.as<uint64_t>(), dref.nBases,
+                                                        dref.windows.as<hipibs_window>(),
+                                                        dref.nWindows,
+                                                        stats.as<hipibs_window_stats>(), &cfg, 8,
+                                                        work.get(), wb, &graph),
+                              "panel graph");
+                const int rc = hipibs_graph_launch(graph, nullptr);
+                const int rc2 = rc ? rc : hipibs_dev_synchronize(g.device);
+                hipibs_graph_free(graph);
+                hipibs::check(rc2, "panel graph launch");
+                hipibs::check(hipibs_memcpy(host.data(), stats.get(),
+                                            host.size() * sizeof(hipibs_window_stats), 2),
+                              "statistics download");
+            }
+            auto tw = clk::now();
+            scanSec += std::chrono::duration<double>(tw - ts).count();
             for (size_t j = 0; j < todo.size(); ++j)
             {
-                KmerDatabase::writeStats(outPaths[todo[j]], res[j]);
+                std::vector<Window> rows(rw.chunks.size());
+                for (size_t i = 0; i < rw.chunks.size(); ++i)
+                {
+                    const SeqWindow &w = rw.chunks[i];
+                    const hipibs_window_stats &x = host[(uint64_t)j * dref.nWindows + i];
+                    rows[i].id = *w.id;
+                    rows[i].start = (int)w.start;
+                    rows[i].end = (int)w.end;
+                    rows[i].totalKmers = x.total_kmers;
+                    rows[i].observedKmers = x.observed_kmers;
+                    rows[i].variations = x.variations;
+                    rows[i].kmerDistance = x.kmer_distance;
+                }
+                KmerDatabase::writeStats(outPaths[todo[j]], rows);
                 dbs_[todo[j]]->releaseGpuIndex();
             }
+            writeSec += std::chrono::duration<double>(clk::now() - tw).count();
         }
+        std::cout << "Panel phases: ingest " << ingestSec << " s, table builds " << buildSec
+                  << " s, scans " << scanSec << " s, write " << writeSec << " s" << std::endl;
     }
 
 private:
     std::vector<std::unique_ptr<KmerDatabase>> dbs_;
     int resident_;
+    uint64_t reserveBytes_ = 0; // device bytes held outside the tables
 };
