@@ -32,7 +32,10 @@ namespace hipibs
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_WAVES = SCAN_BLOCK / 64;
 
-template <int S>
+// MIXED: every window names its own table (hipibs_window.reserved) and its
+// result goes to row w; otherwise grid.y is the table and results are
+// table-major.
+template <int S, bool MIXED>
 __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
     const TableView *__restrict__ tables, const uint64_t *__restrict__ seq,
     const uint64_t *__restrict__ exc, uint64_t nBases, const hipibs_window *__restrict__ windows,
@@ -42,9 +45,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
     __shared__ SegSummary waveSum[SCAN_WAVES];
     const uint32_t unit = blockIdx.x;
     const uint32_t w = unit / segs, s = unit - w * segs;
-    const uint32_t t = blockIdx.y;
-    const TableView T = tables[t];
     const hipibs_window win = windows[w];
+    const uint32_t t = MIXED ? win.reserved : blockIdx.y;
+    const TableView T = tables[t];
+    const uint64_t row = MIXED ? (uint64_t)w : (uint64_t)t * nWindows + w;
     uint64_t g0, g1;
     segmentRange(win, nBases, k, s, segs, g0, g1);
 
@@ -187,9 +191,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
         for (int i = 1; i < SCAN_WAVES; ++i)
             acc = combine(acc, waveSum[i], (int32_t)k);
         if (segs == 1)
-            stats[(uint64_t)t * nWindows + w] = finalizeSummary(acc, (int32_t)k);
+            stats[row] = finalizeSummary(acc, (int32_t)k);
         else
-            work[((uint64_t)t * nWindows + w) * segs + s] = acc;
+            work[row * segs + s] = acc;
     }
 }
 
@@ -384,11 +388,11 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
         HIPIBS_CHECK(hipEventRecord(tEv[0], s));
     }
     if (tables[0]->bucketSlots == 8)
-        hipLaunchKernelGGL(scanKernel<8>, grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc, nBases,
-                           dWindows, nWindows, segs, k, dStats, work);
+        hipLaunchKernelGGL((scanKernel<8, false>), grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc,
+                           nBases, dWindows, nWindows, segs, k, dStats, work);
     else
-        hipLaunchKernelGGL(scanKernel<16>, grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc, nBases,
-                           dWindows, nWindows, segs, k, dStats, work);
+        hipLaunchKernelGGL((scanKernel<16, false>), grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc,
+                           nBases, dWindows, nWindows, segs, k, dStats, work);
     HIPIBS_CHECK(hipGetLastError());
     if (tProfile)
     {
@@ -402,6 +406,25 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
                            segs, k, dStats);
         HIPIBS_CHECK(hipGetLastError());
     }
+    return 0;
+}
+
+// One launch over windows that each name their table (windows[i].reserved
+// indexes dViews); results row i per window; one segment per window.
+int launchScanMixed(int bucketSlots, uint32_t k, const TableView *dViews, const uint64_t *dSeq,
+                    const uint64_t *dExc, uint64_t nBases, const hipibs_window *dWindows,
+                    uint32_t nWindows, hipibs_window_stats *dStats, hipStream_t s)
+{
+    if (nWindows == 0)
+        return 0;
+    const dim3 grid(nWindows, 1);
+    if (bucketSlots == 8)
+        hipLaunchKernelGGL((scanKernel<8, true>), grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc,
+                           nBases, dWindows, nWindows, 1u, k, dStats, (SegSummary *)nullptr);
+    else
+        hipLaunchKernelGGL((scanKernel<16, true>), grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc,
+                           nBases, dWindows, nWindows, 1u, k, dStats, (SegSummary *)nullptr);
+    HIPIBS_CHECK(hipGetLastError());
     return 0;
 }
 

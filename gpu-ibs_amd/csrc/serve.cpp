@@ -1,0 +1,553 @@
+// Serving engine: scan queries (sample, reference range, window size)
+// against sample tables kept resident in HBM, with continuous batching.
+//
+// One worker thread owns the device work. Each turn it (1) retires
+// finished launches, handing their statistics to requests and unpinning
+// their tables; (2) if a launch slot is free, fills it from the pending
+// requests in arrival order -- a request whose sample is not resident is
+// admitted by building its table from the registered host keys, after
+// evicting least-recently-used tables that no launch holds, as far as the
+// HBM budget requires -- and launches one mixed scan in which every
+// window names its own table, so requests for different samples, ranges
+// and window sizes share a launch; (3) sleeps until a submission or a
+// completion if nothing moved. Up to max_inflight launches are in flight
+// on their own streams, so requests that arrive while others run are
+// admitted into the next launch and overlap on the device. A request's
+// windows may span launches; its statistics do not depend on how it was
+// batched (the per-window scan is independent of the launch).
+#include "hipibs_internal.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+using namespace hipibs;
+
+namespace
+{
+
+using Clock = std::chrono::steady_clock;
+
+struct Request
+{
+    uint64_t id = 0;
+    uint32_t sample = 0;
+    std::vector<hipibs_window> wins;
+    std::vector<hipibs_window_stats> stats;
+    uint32_t nextWin = 0, doneWins = 0;
+    Clock::time_point tSubmit, tFirstLaunch, tDone;
+    bool launched = false;
+};
+
+struct Sample
+{
+    const uint64_t *keys = nullptr;
+    uint64_t n = 0;
+    hipibs_table *table = nullptr;
+    uint64_t bytes = 0;
+    int pins = 0;
+    uint64_t lastUse = 0;
+};
+
+struct Part
+{
+    std::shared_ptr<Request> req;
+    uint32_t first, count, offset;
+};
+
+struct Slot
+{
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    hipibs_window *hWin = nullptr, *dWin = nullptr;
+    hipibs_window_stats *hStats = nullptr, *dStats = nullptr;
+    TableView *hViews = nullptr, *dViews = nullptr;
+    bool busy = false;
+    std::vector<Part> parts;
+    std::vector<uint32_t> pinned;
+};
+
+} // namespace
+
+struct hipibs_serve
+{
+    int device = 0;
+    uint32_t k = 0;
+    uint64_t budget = 0;
+    hipibs_table_opts topts{};
+    uint32_t maxWindows = 0;
+    uint64_t *dSeq = nullptr, *dExc = nullptr;
+    uint64_t nBases = 0;
+    std::unordered_map<uint32_t, Sample> samples;
+    uint64_t resident = 0, residentTables = 0, useClock = 0; // worker-owned
+    std::vector<Slot> slots;
+    std::deque<std::shared_ptr<Request>> incoming, pending;
+    std::unordered_map<uint64_t, std::shared_ptr<Request>> completed;
+    std::deque<uint64_t> completedOrder;
+    std::mutex mu;
+    std::condition_variable cvWork, cvDone;
+    std::thread worker;
+    std::atomic<bool> stop{false};
+    std::string workerError;
+    hipibs_serve_counters counters{};
+};
+
+namespace
+{
+
+double ms(Clock::time_point a, Clock::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// Evicts unpinned tables, least recently used first, until `need` more
+// bytes fit the budget; false if they cannot.
+bool makeRoom(hipibs_serve *s, uint64_t need)
+{
+    while (s->resident + need > s->budget)
+    {
+        Sample *victim = nullptr;
+        for (auto &kv : s->samples)
+            if (kv.second.table && kv.second.pins == 0 &&
+                (!victim || kv.second.lastUse < victim->lastUse))
+                victim = &kv.second;
+        if (!victim)
+            return false;
+        hipibs_table_free(victim->table);
+        victim->table = nullptr;
+        s->resident -= victim->bytes;
+        s->residentTables -= 1;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->counters.evictions += 1;
+    }
+    return true;
+}
+
+uint64_t tableBytesFor(const hipibs_serve *s, uint64_t n)
+{
+    const double lf = s->topts.load_factor > 0 ? s->topts.load_factor : 0.6;
+    const uint32_t slots = s->topts.bucket_slots ? s->topts.bucket_slots : 8;
+    const uint64_t buckets = std::max<uint64_t>(2, (uint64_t)std::ceil(std::ceil((double)std::max<uint64_t>(n, 1) / lf) / slots));
+    return buckets * slots * 8;
+}
+
+// 0 = resident, 1 = cannot fit now, <0 = error.
+int ensureResident(hipibs_serve *s, uint32_t id)
+{
+    Sample &sm = s->samples.at(id);
+    sm.lastUse = ++s->useClock;
+    if (sm.table)
+        return 0;
+    const uint64_t need = tableBytesFor(s, sm.n);
+    if (need > s->budget)
+        return setError(HIPIBS_E_NOMEM, "a sample table alone exceeds the serving HBM budget");
+    if (!makeRoom(s, need))
+        return 1;
+    hipibs_table_opts o = s->topts;
+    o.device = s->device;
+    int rc = hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &sm.table);
+    if (rc)
+        return rc;
+    uint64_t bytes = 0;
+    hipibs_table_info(sm.table, nullptr, &bytes, nullptr, nullptr, nullptr, nullptr);
+    sm.bytes = bytes;
+    s->resident += bytes;
+    s->residentTables += 1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->counters.admissions += 1;
+    s->counters.resident_bytes = s->resident;
+    s->counters.resident_tables = s->residentTables;
+    return 0;
+}
+
+void retire(hipibs_serve *s, Slot &sl)
+{
+    const auto now = Clock::now();
+    std::vector<std::shared_ptr<Request>> finished;
+    for (auto &p : sl.parts)
+    {
+        std::copy(sl.hStats + p.offset, sl.hStats + p.offset + p.count,
+                  p.req->stats.begin() + p.first);
+        p.req->doneWins += p.count;
+        if (p.req->doneWins == p.req->wins.size())
+        {
+            p.req->tDone = now;
+            finished.push_back(p.req);
+        }
+    }
+    for (uint32_t id : sl.pinned)
+        s->samples.at(id).pins -= 1;
+    sl.parts.clear();
+    sl.pinned.clear();
+    sl.busy = false;
+    if (!finished.empty())
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (auto &r : finished)
+        {
+            s->completed[r->id] = r;
+            s->completedOrder.push_back(r->id);
+        }
+        s->counters.completed += finished.size();
+        s->cvDone.notify_all();
+    }
+}
+
+// Fills a free slot from the pending queue and launches it. Returns true
+// if a launch was made.
+int fillAndLaunch(hipibs_serve *s, Slot &sl, bool &launched)
+{
+    launched = false;
+    uint32_t used = 0;
+    std::vector<uint32_t> viewOf; // sample id per view index
+    std::unordered_map<uint32_t, uint32_t> viewIdx;
+    for (auto it = s->pending.begin(); it != s->pending.end() && used < s->maxWindows;)
+    {
+        const std::shared_ptr<Request> &r = *it;
+        const int rc = ensureResident(s, r->sample);
+        if (rc < 0)
+            return rc;
+        if (rc == 1)
+        {
+            ++it; // memory is held by in-flight launches: try later
+            continue;
+        }
+        uint32_t vi;
+        auto f = viewIdx.find(r->sample);
+        if (f == viewIdx.end())
+        {
+            vi = (uint32_t)viewOf.size();
+            viewIdx[r->sample] = vi;
+            viewOf.push_back(r->sample);
+            s->samples.at(r->sample).pins += 1;
+            sl.pinned.push_back(r->sample);
+        }
+        else
+            vi = f->second;
+        const uint32_t cnt = std::min<uint32_t>((uint32_t)r->wins.size() - r->nextWin,
+                                                s->maxWindows - used);
+        for (uint32_t j = 0; j < cnt; ++j)
+        {
+            hipibs_window w = r->wins[r->nextWin + j];
+            w.reserved = vi;
+            sl.hWin[used + j] = w;
+        }
+        if (!r->launched)
+        {
+            r->launched = true;
+            r->tFirstLaunch = Clock::now();
+        }
+        sl.parts.push_back(Part{r, r->nextWin, cnt, used});
+        r->nextWin += cnt;
+        used += cnt;
+        if (r->nextWin == r->wins.size())
+            it = s->pending.erase(it);
+        else
+            ++it;
+    }
+    if (used == 0)
+    {
+        for (uint32_t id : sl.pinned)
+            s->samples.at(id).pins -= 1;
+        sl.pinned.clear();
+        return 0;
+    }
+    for (size_t v = 0; v < viewOf.size(); ++v)
+        sl.hViews[v] = s->samples.at(viewOf[v]).table->view();
+    const int slots = s->samples.at(viewOf[0]).table->bucketSlots;
+    HIPIBS_CHECK(hipMemcpyAsync(sl.dViews, sl.hViews, viewOf.size() * sizeof(TableView),
+                                hipMemcpyHostToDevice, sl.stream));
+    HIPIBS_CHECK(hipMemcpyAsync(sl.dWin, sl.hWin, used * sizeof(hipibs_window),
+                                hipMemcpyHostToDevice, sl.stream));
+    int rc = launchScanMixed(slots, s->k, sl.dViews, s->dSeq, s->dExc, s->nBases, sl.dWin, used,
+                             sl.dStats, sl.stream);
+    if (rc)
+        return rc;
+    HIPIBS_CHECK(hipMemcpyAsync(sl.hStats, sl.dStats, used * sizeof(hipibs_window_stats),
+                                hipMemcpyDeviceToHost, sl.stream));
+    HIPIBS_CHECK(hipEventRecord(sl.done, sl.stream));
+    sl.busy = true;
+    launched = true;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->counters.launches += 1;
+    s->counters.windows += used;
+    s->counters.resident_bytes = s->resident;
+    s->counters.resident_tables = s->residentTables;
+    return 0;
+}
+
+void workerLoop(hipibs_serve *s)
+{
+    if (hipSetDevice(s->device) != hipSuccess)
+    {
+        s->workerError = "hipSetDevice failed in the serving worker";
+        return;
+    }
+    while (!s->stop)
+    {
+        bool progress = false;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            while (!s->incoming.empty())
+            {
+                s->pending.push_back(std::move(s->incoming.front()));
+                s->incoming.pop_front();
+                progress = true;
+            }
+        }
+        for (auto &sl : s->slots)
+            if (sl.busy && hipEventQuery(sl.done) == hipSuccess)
+            {
+                retire(s, sl);
+                progress = true;
+            }
+        for (auto &sl : s->slots)
+        {
+            if (sl.busy || s->pending.empty())
+                continue;
+            bool launched = false;
+            const int rc = fillAndLaunch(s, sl, launched);
+            if (rc)
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->workerError = hipibs_last_error();
+                s->stop = true;
+                s->cvDone.notify_all();
+                return;
+            }
+            progress |= launched;
+        }
+        if (!progress)
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cvWork.wait_for(lk, std::chrono::microseconds(200));
+        }
+    }
+}
+
+void destroy(hipibs_serve *s)
+{
+    s->stop = true;
+    s->cvWork.notify_all();
+    if (s->worker.joinable())
+        s->worker.join();
+    (void)hipSetDevice(s->device);
+    for (auto &sl : s->slots)
+    {
+        if (sl.stream)
+            (void)hipStreamSynchronize(sl.stream);
+        (void)hipHostFree(sl.hWin);
+        (void)hipHostFree(sl.hStats);
+        (void)hipHostFree(sl.hViews);
+        (void)hipFree(sl.dWin);
+        (void)hipFree(sl.dStats);
+        (void)hipFree(sl.dViews);
+        if (sl.done)
+            (void)hipEventDestroy(sl.done);
+        if (sl.stream)
+            (void)hipStreamDestroy(sl.stream);
+    }
+    for (auto &kv : s->samples)
+        hipibs_table_free(kv.second.table);
+    (void)hipFree(s->dSeq);
+    (void)hipFree(s->dExc);
+    delete s;
+}
+
+} // namespace
+
+extern "C" int hipibs_serve_create(int device, uint32_t k, uint64_t hbm_budget,
+                                   const hipibs_table_opts *topts, int max_inflight,
+                                   uint32_t max_windows_per_launch, hipibs_serve **out)
+{
+    if (!out || k == 0 || k > 32 || max_inflight < 1 || max_inflight > 16 ||
+        max_windows_per_launch == 0 || hbm_budget == 0)
+        return setError(HIPIBS_E_ARG, "bad serving engine arguments");
+    *out = nullptr;
+    HIPIBS_CHECK(hipSetDevice(device));
+    auto *s = new hipibs_serve();
+    s->device = device;
+    s->k = k;
+    s->budget = hbm_budget;
+    if (topts)
+        s->topts = *topts;
+    s->maxWindows = max_windows_per_launch;
+    s->slots.resize(max_inflight);
+    bool ok = true;
+    for (auto &sl : s->slots)
+    {
+        const uint32_t W = max_windows_per_launch;
+        ok = ok && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
+             hipHostMalloc(reinterpret_cast<void **>(&sl.hWin), W * sizeof(hipibs_window)) == hipSuccess &&
+             hipHostMalloc(reinterpret_cast<void **>(&sl.hStats), W * sizeof(hipibs_window_stats)) == hipSuccess &&
+             hipHostMalloc(reinterpret_cast<void **>(&sl.hViews), W * sizeof(TableView)) == hipSuccess &&
+             hipMalloc(reinterpret_cast<void **>(&sl.dWin), W * sizeof(hipibs_window)) == hipSuccess &&
+             hipMalloc(reinterpret_cast<void **>(&sl.dStats), W * sizeof(hipibs_window_stats)) == hipSuccess &&
+             hipMalloc(reinterpret_cast<void **>(&sl.dViews), W * sizeof(TableView)) == hipSuccess;
+    }
+    if (!ok)
+    {
+        destroy(s);
+        return setError(HIPIBS_E_NOMEM, "serving engine allocation failed");
+    }
+    *out = s;
+    return 0;
+}
+
+extern "C" int hipibs_serve_set_reference(hipibs_serve *s, const uint64_t *h_seq_words,
+                                          const uint64_t *h_exc_words, uint64_t n_bases)
+{
+    if (!s || !h_seq_words || !h_exc_words)
+        return setError(HIPIBS_E_ARG, "null argument");
+    if (s->worker.joinable())
+        return setError(HIPIBS_E_ARG, "the reference is set before the first sample");
+    HIPIBS_CHECK(hipSetDevice(s->device));
+    const uint64_t nsw = (n_bases + 31) / 32 + HIPIBS_PACK_PAD_WORDS;
+    const uint64_t new_ = (n_bases + 63) / 64 + HIPIBS_PACK_PAD_WORDS;
+    HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&s->dSeq), nsw * 8));
+    HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&s->dExc), new_ * 8));
+    HIPIBS_CHECK(hipMemcpy(s->dSeq, h_seq_words, nsw * 8, hipMemcpyHostToDevice));
+    HIPIBS_CHECK(hipMemcpy(s->dExc, h_exc_words, new_ * 8, hipMemcpyHostToDevice));
+    s->nBases = n_bases;
+    return 0;
+}
+
+extern "C" int hipibs_serve_add_sample(hipibs_serve *s, uint32_t sample_id, const uint64_t *h_keys,
+                                       uint64_t n_keys, int preload)
+{
+    if (!s || (n_keys && !h_keys))
+        return setError(HIPIBS_E_ARG, "null argument");
+    if (!s->dSeq)
+        return setError(HIPIBS_E_ARG, "set the reference first");
+    if (s->worker.joinable())
+        return setError(HIPIBS_E_ARG, "samples are registered before the first request");
+    if (s->samples.count(sample_id))
+        return setError(HIPIBS_E_ARG, "sample id already registered");
+    Sample sm;
+    sm.keys = h_keys;
+    sm.n = n_keys;
+    s->samples[sample_id] = sm;
+    if (preload)
+    {
+        const int rc = ensureResident(s, sample_id);
+        if (rc < 0)
+            return rc;
+        if (rc == 1)
+            return setError(HIPIBS_E_NOMEM, "preloaded samples exceed the HBM budget");
+    }
+    return 0;
+}
+
+extern "C" int hipibs_serve_submit(hipibs_serve *s, uint64_t request_id, uint32_t sample_id,
+                                   uint64_t base, uint64_t n_bases, uint32_t W)
+{
+    if (!s)
+        return setError(HIPIBS_E_ARG, "null engine");
+    if (!s->samples.count(sample_id))
+        return setError(HIPIBS_E_ARG, "unknown sample id");
+    if (W <= s->k)
+        return setError(HIPIBS_E_ARG, "window size must be larger than k");
+    if (base + n_bases > s->nBases)
+        return setError(HIPIBS_E_ARG, "range outside the reference");
+    auto r = std::make_shared<Request>();
+    r->id = request_id;
+    r->sample = sample_id;
+    // chunkSequences geometry over the range (src/Utils.hpp:296-327)
+    for (uint64_t j = 0; j < n_bases; j += W - s->k)
+        r->wins.push_back(hipibs_window{base + j, (uint32_t)std::min<uint64_t>(W, n_bases - j), 0});
+    r->stats.resize(r->wins.size());
+    r->tSubmit = Clock::now();
+    if (r->wins.empty())
+    {
+        r->tFirstLaunch = r->tDone = r->tSubmit;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->completed[request_id] = r;
+        s->completedOrder.push_back(request_id);
+        s->counters.completed += 1;
+        s->cvDone.notify_all();
+        return 0;
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!s->workerError.empty())
+            return setError(HIPIBS_E_HIP, "serving worker stopped: " + s->workerError);
+        s->incoming.push_back(std::move(r));
+        s->counters.submitted += 1;
+        if (!s->worker.joinable())
+            s->worker = std::thread(workerLoop, s);
+    }
+    s->cvWork.notify_one();
+    return 0;
+}
+
+extern "C" int hipibs_serve_poll(hipibs_serve *s, uint64_t *ids, uint32_t max, int timeout_ms,
+                                 uint32_t *n_out)
+{
+    if (!s || !n_out || (max && !ids))
+        return setError(HIPIBS_E_ARG, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->completedOrder.empty() && timeout_ms > 0)
+        s->cvDone.wait_for(lk, std::chrono::milliseconds(timeout_ms),
+                           [s] { return !s->completedOrder.empty() || !s->workerError.empty(); });
+    if (!s->workerError.empty() && s->completedOrder.empty())
+        return setError(HIPIBS_E_HIP, "serving worker stopped: " + s->workerError);
+    uint32_t n = 0;
+    while (n < max && !s->completedOrder.empty())
+    {
+        ids[n++] = s->completedOrder.front();
+        s->completedOrder.pop_front();
+    }
+    *n_out = n;
+    return 0;
+}
+
+extern "C" int hipibs_serve_fetch(hipibs_serve *s, uint64_t request_id, hipibs_window_stats *out,
+                                  uint32_t cap, uint32_t *n_windows, double *latency_ms,
+                                  double *queue_ms)
+{
+    if (!s)
+        return setError(HIPIBS_E_ARG, "null engine");
+    std::shared_ptr<Request> r;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->completed.find(request_id);
+        if (it == s->completed.end())
+            return setError(HIPIBS_E_ARG, "request not completed (or already fetched)");
+        r = it->second;
+        if (out) // a call without a buffer only reports the sizes
+            s->completed.erase(it);
+    }
+    if (n_windows)
+        *n_windows = (uint32_t)r->wins.size();
+    if (out)
+        std::copy(r->stats.begin(), r->stats.begin() + std::min<size_t>(cap, r->stats.size()), out);
+    if (latency_ms)
+        *latency_ms = ms(r->tSubmit, r->tDone);
+    if (queue_ms)
+        *queue_ms = ms(r->tSubmit, r->tFirstLaunch);
+    return 0;
+}
+
+extern "C" int hipibs_serve_get_counters(hipibs_serve *s, hipibs_serve_counters *out)
+{
+    if (!s || !out)
+        return setError(HIPIBS_E_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->counters;
+    return 0;
+}
+
+extern "C" int hipibs_serve_free(hipibs_serve *s)
+{
+    if (s)
+        destroy(s);
+    return 0;
+}

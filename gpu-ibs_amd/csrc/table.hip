@@ -335,7 +335,7 @@ extern "C" int hipibs_table_build_keys(const uint64_t *h_keys, uint64_t n, uint3
     int rc = allocTable(n, k, opts, &t, nullptr);
     if (rc)
         return rc;
-    rc = hipDeviceSynchronize() == hipSuccess ? 0 : setError(HIPIBS_E_HIP, "table clear failed");
+    rc = hipStreamSynchronize(nullptr) == hipSuccess ? 0 : setError(HIPIBS_E_HIP, "table clear failed");
     if (!rc)
         rc = hipibsBuildFromHostChunks(t, h_keys, n);
     if (rc)
@@ -507,7 +507,7 @@ extern "C" int hipibs_table_build_kmc(const char *prefix, const hipibs_table_opt
     rc = allocTable(n, info.kmer_length, opts, &t, nullptr);
     if (rc)
         return rc;
-    rc = hipDeviceSynchronize() == hipSuccess ? 0 : setError(HIPIBS_E_HIP, "table clear failed");
+    rc = hipStreamSynchronize(nullptr) == hipSuccess ? 0 : setError(HIPIBS_E_HIP, "table clear failed");
     if (!rc)
         rc = hipibsBuildFromHostChunks(t, keys.data(), n);
     if (!rc && t->nKeys != n)

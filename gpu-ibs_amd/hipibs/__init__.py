@@ -63,6 +63,12 @@ class ScanConfig(C.Structure):
                 ("reserved", C.c_uint32 * 2)]
 
 
+class ServeCounters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "launches", "windows",
+                                         "admissions", "evictions", "resident_bytes",
+                                         "resident_tables")]
+
+
 class HostSlot(C.Structure):
     _fields_ = [("seq_words", u64p), ("exc_words", u64p), ("windows", C.POINTER(Window)),
                 ("stats", C.POINTER(WindowStats)), ("seq_cap", C.c_uint64),
@@ -123,6 +129,17 @@ _SIGS = [
                                             vp, C.c_uint64, C.POINTER(vp)]),
     ("hipibs_graph_launch", C.c_int, [vp, vp]),
     ("hipibs_graph_free", C.c_int, [vp]),
+    ("hipibs_serve_create", C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.POINTER(TableOpts),
+                                      C.c_int, C.c_uint32, C.POINTER(vp)]),
+    ("hipibs_serve_set_reference", C.c_int, [vp, u64p, u64p, C.c_uint64]),
+    ("hipibs_serve_add_sample", C.c_int, [vp, C.c_uint32, u64p, C.c_uint64, C.c_int]),
+    ("hipibs_serve_submit", C.c_int, [vp, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64,
+                                      C.c_uint32]),
+    ("hipibs_serve_poll", C.c_int, [vp, u64p, C.c_uint32, C.c_int, u32p]),
+    ("hipibs_serve_fetch", C.c_int, [vp, C.c_uint64, vp, C.c_uint32, u32p,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("hipibs_serve_get_counters", C.c_int, [vp, C.POINTER(ServeCounters)]),
+    ("hipibs_serve_free", C.c_int, [vp]),
     ("hipibs_dev_alloc", C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
     ("hipibs_dev_free", C.c_int, [vp]),
     ("hipibs_memcpy", C.c_int, [vp, vp, C.c_uint64, C.c_int]),
@@ -583,6 +600,72 @@ def _collect(eng: ScanEngine, slot: int, rng: Tuple[int, int], stats: np.ndarray
     n = hi - lo
     arr = st[:n * stats.shape[0]].view(np.int32).reshape(stats.shape[0], n, 4)
     stats[:, lo:hi, :] = arr
+
+
+class ServeEngine:
+    """hipibs_serve: resident sample tables, continuous batching of scan
+    queries, per-request latency."""
+
+    def __init__(self, k: int, hbm_budget: int, max_inflight: int = 3,
+                 max_windows_per_launch: int = 65536, device: int = 0, **table_opts):
+        h = vp()
+        o = _opts(device=device, **table_opts)
+        _check(_lib.hipibs_serve_create(device, k, int(hbm_budget), C.byref(o), max_inflight,
+                                        max_windows_per_launch, C.byref(h)), "hipibs_serve_create")
+        self.handle = h
+        self._keep = []
+
+    def set_reference(self, seq_words: np.ndarray, exc_words: np.ndarray, n_bases: int) -> None:
+        self._keep += [seq_words, exc_words]
+        _check(_lib.hipibs_serve_set_reference(self.handle, _ptr(seq_words), _ptr(exc_words),
+                                               n_bases), "hipibs_serve_set_reference")
+
+    def add_sample(self, sample_id: int, keys: np.ndarray, preload: bool = False) -> None:
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        self._keep.append(keys)
+        _check(_lib.hipibs_serve_add_sample(self.handle, sample_id, _ptr(keys), keys.size,
+                                            int(preload)), "hipibs_serve_add_sample")
+
+    def submit(self, request_id: int, sample_id: int, base: int, n_bases: int, W: int) -> None:
+        _check(_lib.hipibs_serve_submit(self.handle, request_id, sample_id, base, n_bases, W),
+               "hipibs_serve_submit")
+
+    def poll(self, timeout_ms: int = 0, max_n: int = 4096) -> List[int]:
+        ids = np.zeros(max_n, dtype=np.uint64)
+        n = C.c_uint32(0)
+        _check(_lib.hipibs_serve_poll(self.handle, _ptr(ids), max_n, timeout_ms, C.byref(n)),
+               "hipibs_serve_poll")
+        return [int(x) for x in ids[:n.value]]
+
+    def fetch(self, request_id: int):
+        """(stats array (n, 4) int32, latency ms, queue ms)."""
+        n = C.c_uint32(0)
+        lat, q = C.c_double(0), C.c_double(0)
+        _check(_lib.hipibs_serve_fetch(self.handle, request_id, None, 0, C.byref(n), None, None),
+               "hipibs_serve_fetch")
+        out = np.zeros((max(1, n.value), 4), dtype=np.int32)
+        _check(_lib.hipibs_serve_fetch(self.handle, request_id, out.ctypes.data_as(vp),
+                                       out.shape[0], C.byref(n), C.byref(lat), C.byref(q)),
+               "hipibs_serve_fetch")
+        if n.value > out.shape[0]:
+            raise RuntimeError("request has more windows than the fetch buffer")
+        return out[:n.value].copy(), lat.value, q.value
+
+    def counters(self) -> dict:
+        c = ServeCounters()
+        _check(_lib.hipibs_serve_get_counters(self.handle, C.byref(c)), "hipibs_serve_get_counters")
+        return {f: getattr(c, f) for f, _ in ServeCounters._fields_}
+
+    def free(self) -> None:
+        if self.handle:
+            _lib.hipibs_serve_free(self.handle)
+            self.handle = vp(0)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def reference_name(filename: str) -> str:

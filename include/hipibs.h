@@ -295,6 +295,60 @@ int hipibs_coverage_device(const uint32_t *d_hit_words, uint64_t n_pos, uint64_t
 int hipibs_set_profiling(int on);
 double hipibs_last_kernel_ms(void);
 
+/* ---- serving: resident tables, continuous batching ------------------------- */
+
+typedef struct hipibs_serve hipibs_serve;
+
+typedef struct hipibs_serve_counters {
+    uint64_t submitted, completed, launches, windows;
+    uint64_t admissions, evictions;   /* tables built / freed for memory */
+    uint64_t resident_bytes, resident_tables;
+} hipibs_serve_counters;
+
+/* A serving engine on `device` for k-mers of length k: sample tables are
+ * built (with topts) on demand from registered host keys and kept resident
+ * within hbm_budget bytes, least recently used unpinned tables evicted
+ * first; up to max_inflight launches of at most max_windows_per_launch
+ * windows run at once, each batching windows of any pending requests.
+ * No counterpart in the reference (it has no serving mode). */
+int hipibs_serve_create(int device, uint32_t k, uint64_t hbm_budget,
+                        const hipibs_table_opts *topts, int max_inflight,
+                        uint32_t max_windows_per_launch, hipibs_serve **out);
+
+/* The packed reference (words as hipibs_pack_bases, pad included), copied
+ * to the device; set once, before samples are added. */
+int hipibs_serve_set_reference(hipibs_serve *s, const uint64_t *h_seq_words,
+                               const uint64_t *h_exc_words, uint64_t n_bases);
+
+/* Registers a sample's canonical keys (host memory the caller keeps valid
+ * for the engine's lifetime); preload != 0 builds its table now. All
+ * samples are registered before the first request. */
+int hipibs_serve_add_sample(hipibs_serve *s, uint32_t sample_id, const uint64_t *h_keys,
+                            uint64_t n_keys, int preload);
+
+/* Queues a scan of bases [base, base+n_bases) of the reference (one
+ * contig's range) in windows of W (chunkSequences geometry within the
+ * range) against one sample. Returns at once. */
+int hipibs_serve_submit(hipibs_serve *s, uint64_t request_id, uint32_t sample_id, uint64_t base,
+                        uint64_t n_bases, uint32_t W);
+
+/* Ids of completed, not yet fetched requests (waits up to timeout_ms for
+ * the first one). */
+int hipibs_serve_poll(hipibs_serve *s, uint64_t *ids, uint32_t max, int timeout_ms,
+                      uint32_t *n_out);
+
+/* A completed request's per-window statistics (up to cap), window count,
+ * submit-to-completion latency and submit-to-first-launch queueing time in
+ * ms; the request is forgotten afterwards. With out == NULL only the sizes
+ * and times are reported and the request is kept. */
+int hipibs_serve_fetch(hipibs_serve *s, uint64_t request_id, hipibs_window_stats *out,
+                       uint32_t cap, uint32_t *n_windows, double *latency_ms, double *queue_ms);
+
+int hipibs_serve_get_counters(hipibs_serve *s, hipibs_serve_counters *out);
+
+/* Stops the worker (in-flight launches finish), frees every table. */
+int hipibs_serve_free(hipibs_serve *s);
+
 /* ---- device memory for FFI callers without a HIP binding ------------------
  * No counterpart in the reference (its callers are CUDA C++). Synchronous. */
 int hipibs_dev_alloc(int device, uint64_t bytes, void **out);

@@ -320,3 +320,61 @@ def test_panel_graph_replays_and_autotune(small_corpus):
     hipibs._lib.hipibs_graph_free(g)
     for t in tabs:
         t.free()
+
+
+def test_serving_results_same_alone_and_in_the_mix(small_corpus):
+    """Continuous batching: every request's statistics are identical when
+    served alone and when served mixed with other samples, ranges and
+    window sizes, with an HBM budget that forces evictions and
+    re-admissions; and equal to the oracle."""
+    import time
+    ref, samples = small_corpus
+    seq = corpus.to_text(ref)
+    sw, ew, _ = hipibs.pack_sequences([seq])
+    n = len(seq)
+    keys = [s[0] for s in samples]
+    one_table = hipibs.Table.from_keys(keys[0], 25).info()["bytes"]
+    rng = np.random.default_rng(42)
+    reqs = []
+    for i in range(40):
+        sid = int(rng.integers(0, 3))
+        W = int(rng.choice([1000, 5000, 10000]))
+        ln = int(rng.integers(30, n // 3))
+        base = int(rng.integers(0, n - ln))
+        reqs.append((i, sid, base, ln, W))
+
+    def serve(batch, budget, burst):
+        eng = hipibs.ServeEngine(25, budget, max_inflight=3, max_windows_per_launch=512)
+        eng.set_reference(sw, ew, n)
+        for sid in range(3):
+            eng.add_sample(sid, keys[sid])
+        out = {}
+        pending = set()
+        for j, (rid, sid, base, ln, W) in enumerate(batch):
+            eng.submit(rid, sid, base, ln, W)
+            pending.add(rid)
+            if burst and j % 7 == 6:
+                time.sleep(0.002)  # later arrivals while earlier ones run
+        while pending:
+            for rid in eng.poll(timeout_ms=2000):
+                st, lat, q = eng.fetch(rid)
+                assert lat >= q >= 0
+                out[rid] = st
+                pending.discard(rid)
+        c = eng.counters()
+        eng.free()
+        return out, c
+
+    alone = {}
+    for r in reqs:
+        o, _ = serve([r], 3 * one_table, False)
+        alone.update(o)
+    mixed, c = serve(reqs, int(1.5 * one_table), True)
+    assert c["evictions"] > 0 and c["admissions"] > 3
+    assert c["launches"] < len(reqs)  # requests shared launches
+    for rid, sid, base, ln, W in reqs:
+        assert np.array_equal(alone[rid], mixed[rid]), rid
+    for rid, sid, base, ln, W in reqs[:4]:
+        want = oracle_scan_rows([b"c"], [seq[base:base + ln]], W, 25, sorted_keys(keys[sid]))
+        got = [tuple(int(x) for x in row) for row in mixed[rid]]
+        assert got == [w[3:] for w in want], rid
