@@ -51,7 +51,13 @@ def parse():
                     help="reference slice (Mbp) of the CPU baseline sample; 0 = skip")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0x1B50200)
-    ap.add_argument("--workload", default="scan", choices=["scan", "panel"],
+    ap.add_argument("--serve-samples", type=int, default=32)
+    ap.add_argument("--serve-min-keys", type=float, default=0.1e9)
+    ap.add_argument("--serve-max-keys", type=float, default=0.5e9)
+    ap.add_argument("--serve-budget-gb", type=float, default=100.0)
+    ap.add_argument("--serve-rates", default="20,100")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--workload", default="scan", choices=["scan", "panel", "serve"],
                     help="scan: the single-sample device scan (default line); panel: "
                          "end-to-end fastibs runs, N single-sample runs vs one panel run")
     ap.add_argument("--panel-samples", type=int, default=8)
@@ -344,10 +350,125 @@ def run_panel(args):
     shutil.rmtree(root, ignore_errors=True)
 
 
+def run_serve(args):
+    """Served queries: Poisson arrivals of (sample, chromosome, range,
+    window) scans against sample tables kept resident within an HBM budget
+    smaller than the panel (admission builds a table from host keys,
+    eviction frees the least recently used idle one); p50/p99 latency and
+    probe throughput per arrival rate."""
+    import numpy as np
+    import torch
+
+    import hipibs
+    n = int(args.gbp * 1e9) // 64 * 64
+    ref_codes = gen_reference(torch, n, args.seed)
+    sw, ew = pack(torch, ref_codes)
+    h_sw = sw.cpu().numpy().view(np.uint64)
+    h_ew = ew.cpu().numpy().view(np.uint64)
+    nchrom = 7
+    cuts = np.linspace(0, n, nchrom + 1).astype(np.int64)
+    rng = np.random.default_rng(args.seed)
+    S = args.serve_samples
+    sizes = np.exp(rng.uniform(np.log(args.serve_min_keys), np.log(args.serve_max_keys), S))
+    # four haplotype pools (SNP 0.1 / 1 / 2.5 / 5 %), each sample a subset
+    pools = []
+    for pi, rate in enumerate((0.001, 0.01, 0.025, 0.05)):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(args.seed + 50 + pi)
+        hap = ref_codes.clone()
+        snp = (torch.rand(n, device="cuda", generator=g) < rate) & (hap < 4)
+        hap[snp] = (hap[snp] + torch.randint(1, 4, (n,), dtype=torch.uint8, device="cuda",
+                                             generator=g)[snp]) % 4
+        hsw, hew = pack(torch, hap)
+        del hap, snp
+        codes = encode(torch, hsw, hew, n, K)
+        del hsw, hew
+        pools.append(torch.unique(codes[codes >= 0]))
+        del codes
+        torch.cuda.empty_cache()
+    keys = []
+    for i in range(S):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(args.seed + 500 + i)
+        pool = pools[i % 4]
+        want = int(sizes[i])
+        sel = pool[torch.rand(pool.numel(), device="cuda", generator=g) < 0.8 * want / pool.numel()]
+        rnd = random_canonical(torch, want - sel.numel(), K, g)
+        keys.append(torch.cat([sel, rnd]).cpu().numpy().view(np.uint64))
+    del pools, ref_codes, sw, ew
+    torch.cuda.empty_cache()
+    total_keys = int(sum(k.size for k in keys))
+    log(f"serve: {S} samples, {total_keys/1e9:.2f} G keys on host "
+        f"(~{total_keys*13.3/1e9:.0f} GB of tables at load 0.6), budget {args.serve_budget_gb} GB")
+    results = []
+    for rate in [float(x) for x in args.serve_rates.split(",")]:
+        eng = hipibs.ServeEngine(K, int(args.serve_budget_gb * 1e9), max_inflight=3,
+                                 max_windows_per_launch=65536)
+        eng.set_reference(h_sw, h_ew, n)
+        budget_used = 0
+        for i in range(S):
+            tb = int(np.ceil(np.ceil(keys[i].size / 0.6) / 8) * 64)
+            pre = budget_used + tb <= 0.7 * args.serve_budget_gb * 1e9
+            budget_used += tb if pre else 0
+            eng.add_sample(i, keys[i], preload=pre)
+        rr = np.random.default_rng(args.seed + int(rate))
+        m = int(rate * args.serve_seconds)
+        arrivals = np.cumsum(rr.exponential(1.0 / rate, m))
+        reqs = []
+        for j in range(m):
+            c = int(rr.integers(0, nchrom))
+            clen = int(cuts[c + 1] - cuts[c])
+            ln = int(np.exp(rr.uniform(np.log(1e6), np.log(clen))))
+            st = int(cuts[c] + rr.integers(0, clen - ln + 1))
+            W = int(rr.choice([10000, 50000, 100000]))
+            reqs.append((j, int(rr.integers(0, S)), st, ln, W))
+        lat, probes = {}, 0
+        t0 = time.perf_counter()
+        nxt = 0
+        while len(lat) < m:
+            now = time.perf_counter() - t0
+            while nxt < m and arrivals[nxt] <= now:
+                j, sid, st, ln, W = reqs[nxt]
+                eng.submit(j, sid, st, ln, W)
+                nxt += 1
+            for rid in eng.poll(timeout_ms=0):
+                stt, l, q = eng.fetch(rid)
+                lat[rid] = l
+                probes += int(stt[:, 0].astype(np.int64).sum())
+            if nxt < m:
+                time.sleep(max(0.0, min(0.002, arrivals[nxt] - (time.perf_counter() - t0))))
+            else:
+                time.sleep(0.0005)
+        wall = time.perf_counter() - t0
+        c = eng.counters()
+        eng.free()
+        L = np.array(list(lat.values()))
+        res = {"rate_req_s": rate, "requests": m, "p50_ms": float(np.percentile(L, 50)),
+               "p99_ms": float(np.percentile(L, 99)), "mean_ms": float(L.mean()),
+               "probes_per_s": probes / wall, "wall_s": wall, "launches": c["launches"],
+               "admissions": c["admissions"], "evictions": c["evictions"],
+               "resident_tables_end": c["resident_tables"]}
+        log(f"rate {rate}/s: p50 {res['p50_ms']:.1f} ms p99 {res['p99_ms']:.1f} ms, "
+            f"{res['probes_per_s']/1e9:.2f} G probes/s, {c['launches']} launches, "
+            f"{c['admissions']} admissions, {c['evictions']} evictions")
+        results.append(res)
+    out = {"metric": "served p50/p99 query latency", "unit": "ms", "n_gpus": 1,
+           "higher_is_better": False, "value": results[-1]["p99_ms"],
+           "config": {"workload": f"served queries, k=25: {n/1e9:.1f} Gbp reference in {nchrom} "
+                                  f"chromosomes, {S} samples of {args.serve_min_keys/1e9:.2f}-"
+                                  f"{args.serve_max_keys/1e9:.2f} G keys, HBM budget "
+                                  f"{args.serve_budget_gb} GB, Poisson arrivals, ranges "
+                                  f"log-uniform 1 Mbp-chromosome, W in 10/50/100 kbp",
+                      "total_keys": total_keys, "per_rate": results}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.workload == "panel":
         return run_panel(args)
+    if args.workload == "serve":
+        return run_serve(args)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
