@@ -51,10 +51,22 @@ struct Sample
     const uint64_t *keys = nullptr;
     uint64_t n = 0;
     hipibs_table *table = nullptr;
-    uint64_t bytes = 0;
+    uint64_t bytes = 0;    // bytes counted against the budget (reserved while building)
+    bool building = false; // queued on or running in the builder thread
     int pins = 0;
     uint64_t lastUse = 0;
 };
+
+struct Built
+{
+    uint32_t id;
+    hipibs_table *table;
+    std::string error;
+};
+
+// Bases of window positions one launch may take (keeps launches short so a
+// small request never waits behind a whole chromosome of others).
+constexpr uint64_t LAUNCH_BASES = 256ull << 20;
 
 struct Part
 {
@@ -97,6 +109,13 @@ struct hipibs_serve
     std::atomic<bool> stop{false};
     std::string workerError;
     hipibs_serve_counters counters{};
+    // asynchronous admission: the builder thread turns queued sample ids
+    // into tables while the worker keeps launching resident requests
+    std::thread builder;
+    std::deque<uint32_t> buildQueue;
+    std::vector<Built> built;
+    std::mutex bmu;
+    std::condition_variable bcv;
 };
 
 namespace
@@ -138,32 +157,100 @@ uint64_t tableBytesFor(const hipibs_serve *s, uint64_t n)
     return buckets * slots * 8;
 }
 
-// 0 = resident, 1 = cannot fit now, <0 = error.
-int ensureResident(hipibs_serve *s, uint32_t id)
+// Builds sample id's table now (preload) -- or, async, queues it on the
+// builder thread -- after making room. 0 = resident, 1 = not (yet), <0 =
+// error. The bytes of a queued build are reserved against the budget.
+int ensureResident(hipibs_serve *s, uint32_t id, bool async)
 {
     Sample &sm = s->samples.at(id);
     sm.lastUse = ++s->useClock;
     if (sm.table)
         return 0;
+    if (sm.building)
+        return 1;
     const uint64_t need = tableBytesFor(s, sm.n);
     if (need > s->budget)
         return setError(HIPIBS_E_NOMEM, "a sample table alone exceeds the serving HBM budget");
     if (!makeRoom(s, need))
         return 1;
+    sm.bytes = need;
+    s->resident += need;
+    if (async)
+    {
+        sm.building = true;
+        {
+            std::lock_guard<std::mutex> lk(s->bmu);
+            s->buildQueue.push_back(id);
+        }
+        s->bcv.notify_one();
+        return 1;
+    }
     hipibs_table_opts o = s->topts;
     o.device = s->device;
     int rc = hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &sm.table);
     if (rc)
+    {
+        s->resident -= need;
+        sm.bytes = 0;
         return rc;
-    uint64_t bytes = 0;
-    hipibs_table_info(sm.table, nullptr, &bytes, nullptr, nullptr, nullptr, nullptr);
-    sm.bytes = bytes;
-    s->resident += bytes;
+    }
     s->residentTables += 1;
     std::lock_guard<std::mutex> lk(s->mu);
     s->counters.admissions += 1;
     s->counters.resident_bytes = s->resident;
     s->counters.resident_tables = s->residentTables;
+    return 0;
+}
+
+void builderLoop(hipibs_serve *s)
+{
+    (void)hipSetDevice(s->device);
+    for (;;)
+    {
+        uint32_t id;
+        {
+            std::unique_lock<std::mutex> lk(s->bmu);
+            s->bcv.wait(lk, [s] { return s->stop || !s->buildQueue.empty(); });
+            if (s->stop)
+                return;
+            id = s->buildQueue.front();
+            s->buildQueue.pop_front();
+        }
+        const Sample &sm = s->samples.at(id); // keys and n never change
+        hipibs_table_opts o = s->topts;
+        o.device = s->device;
+        hipibs_table *t = nullptr;
+        const int rc = hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &t);
+        {
+            std::lock_guard<std::mutex> lk(s->bmu);
+            s->built.push_back(Built{id, rc ? nullptr : t, rc ? hipibs_last_error() : ""});
+        }
+        s->cvWork.notify_one();
+    }
+}
+
+// Worker side of a finished build: the table becomes resident.
+int collectBuilds(hipibs_serve *s, bool &progress)
+{
+    std::vector<Built> done;
+    {
+        std::lock_guard<std::mutex> lk(s->bmu);
+        done.swap(s->built);
+    }
+    for (auto &b : done)
+    {
+        Sample &sm = s->samples.at(b.id);
+        sm.building = false;
+        if (!b.table)
+            return setError(HIPIBS_E_HIP, "table build failed: " + b.error);
+        sm.table = b.table;
+        s->residentTables += 1;
+        progress = true;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->counters.admissions += 1;
+        s->counters.resident_bytes = s->resident;
+        s->counters.resident_tables = s->residentTables;
+    }
     return 0;
 }
 
@@ -206,17 +293,19 @@ int fillAndLaunch(hipibs_serve *s, Slot &sl, bool &launched)
 {
     launched = false;
     uint32_t used = 0;
+    uint64_t bases = 0;
     std::vector<uint32_t> viewOf; // sample id per view index
     std::unordered_map<uint32_t, uint32_t> viewIdx;
-    for (auto it = s->pending.begin(); it != s->pending.end() && used < s->maxWindows;)
+    for (auto it = s->pending.begin();
+         it != s->pending.end() && used < s->maxWindows && bases < LAUNCH_BASES;)
     {
         const std::shared_ptr<Request> &r = *it;
-        const int rc = ensureResident(s, r->sample);
+        const int rc = ensureResident(s, r->sample, true);
         if (rc < 0)
             return rc;
         if (rc == 1)
         {
-            ++it; // memory is held by in-flight launches: try later
+            ++it; // being built, or memory held by in-flight launches: later
             continue;
         }
         uint32_t vi;
@@ -231,8 +320,10 @@ int fillAndLaunch(hipibs_serve *s, Slot &sl, bool &launched)
         }
         else
             vi = f->second;
-        const uint32_t cnt = std::min<uint32_t>((uint32_t)r->wins.size() - r->nextWin,
-                                                s->maxWindows - used);
+        uint32_t cnt = 0;
+        while (r->nextWin + cnt < r->wins.size() && used + cnt < s->maxWindows &&
+               (bases < LAUNCH_BASES || cnt == 0))
+            bases += r->wins[r->nextWin + cnt++].n_bases;
         for (uint32_t j = 0; j < cnt; ++j)
         {
             hipibs_window w = r->wins[r->nextWin + j];
@@ -302,6 +393,14 @@ void workerLoop(hipibs_serve *s)
                 progress = true;
             }
         }
+        if (collectBuilds(s, progress))
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->workerError = hipibs_last_error();
+            s->stop = true;
+            s->cvDone.notify_all();
+            return;
+        }
         for (auto &sl : s->slots)
             if (sl.busy && hipEventQuery(sl.done) == hipSuccess)
             {
@@ -336,8 +435,16 @@ void destroy(hipibs_serve *s)
 {
     s->stop = true;
     s->cvWork.notify_all();
+    {
+        std::lock_guard<std::mutex> lk(s->bmu);
+    }
+    s->bcv.notify_all();
     if (s->worker.joinable())
         s->worker.join();
+    if (s->builder.joinable())
+        s->builder.join();
+    for (auto &b : s->built)
+        hipibs_table_free(b.table);
     (void)hipSetDevice(s->device);
     for (auto &sl : s->slots)
     {
@@ -437,7 +544,7 @@ extern "C" int hipibs_serve_add_sample(hipibs_serve *s, uint32_t sample_id, cons
     s->samples[sample_id] = sm;
     if (preload)
     {
-        const int rc = ensureResident(s, sample_id);
+        const int rc = ensureResident(s, sample_id, false);
         if (rc < 0)
             return rc;
         if (rc == 1)
@@ -482,7 +589,10 @@ extern "C" int hipibs_serve_submit(hipibs_serve *s, uint64_t request_id, uint32_
         s->incoming.push_back(std::move(r));
         s->counters.submitted += 1;
         if (!s->worker.joinable())
+        {
+            s->builder = std::thread(builderLoop, s);
             s->worker = std::thread(workerLoop, s);
+        }
     }
     s->cvWork.notify_one();
     return 0;
