@@ -54,7 +54,9 @@ def parse():
     ap.add_argument("--serve-samples", type=int, default=48)
     ap.add_argument("--serve-min-keys", type=float, default=0.05e9)
     ap.add_argument("--serve-max-keys", type=float, default=0.3e9)
-    ap.add_argument("--serve-budget-gb", type=float, default=50.0)
+    ap.add_argument("--serve-budget-gb", default="150,50",
+                    help="HBM budgets to run (GB); the first is used with every rate, "
+                         "the others with the lowest rate")
     ap.add_argument("--serve-zipf", type=float, default=1.1,
                     help="sample popularity ~ 1/rank^s (0 = uniform)")
     ap.add_argument("--serve-rates", default="20,100,200")
@@ -400,17 +402,20 @@ def run_serve(args):
     del pools, ref_codes, sw, ew
     torch.cuda.empty_cache()
     total_keys = int(sum(k.size for k in keys))
+    budgets = [float(x) for x in args.serve_budget_gb.split(",")]
+    rates = [float(x) for x in args.serve_rates.split(",")]
     log(f"serve: {S} samples, {total_keys/1e9:.2f} G keys on host "
-        f"(~{total_keys*13.3/1e9:.0f} GB of tables at load 0.6), budget {args.serve_budget_gb} GB")
+        f"(~{total_keys*13.3/1e9:.0f} GB of tables at load 0.6), budgets {budgets} GB")
+    runs = [(budgets[0], r) for r in rates] + [(b, min(rates)) for b in budgets[1:]]
     results = []
-    for rate in [float(x) for x in args.serve_rates.split(",")]:
-        eng = hipibs.ServeEngine(K, int(args.serve_budget_gb * 1e9), max_inflight=3,
+    for budget_gb, rate in runs:
+        eng = hipibs.ServeEngine(K, int(budget_gb * 1e9), max_inflight=3,
                                  max_windows_per_launch=65536)
         eng.set_reference(h_sw, h_ew, n)
         budget_used = 0
         for i in range(S):
             tb = int(np.ceil(np.ceil(keys[i].size / 0.6) / 8) * 64)
-            pre = budget_used + tb <= 0.7 * args.serve_budget_gb * 1e9
+            pre = budget_used + tb <= 0.7 * budget_gb * 1e9
             budget_used += tb if pre else 0
             eng.add_sample(i, keys[i], preload=pre)
         rr = np.random.default_rng(args.seed + int(rate))
@@ -447,22 +452,23 @@ def run_serve(args):
         c = eng.counters()
         eng.free()
         L = np.array(list(lat.values()))
-        res = {"rate_req_s": rate, "requests": m, "p50_ms": float(np.percentile(L, 50)),
+        res = {"budget_gb": budget_gb, "rate_req_s": rate, "requests": m, "p50_ms": float(np.percentile(L, 50)),
                "p99_ms": float(np.percentile(L, 99)), "mean_ms": float(L.mean()),
                "probes_per_s": probes / wall, "wall_s": wall, "launches": c["launches"],
                "admissions": c["admissions"], "evictions": c["evictions"],
                "resident_tables_end": c["resident_tables"]}
-        log(f"rate {rate}/s: p50 {res['p50_ms']:.1f} ms p99 {res['p99_ms']:.1f} ms, "
+        log(f"budget {budget_gb} GB, rate {rate}/s: p50 {res['p50_ms']:.1f} ms p99 {res['p99_ms']:.1f} ms, "
             f"{res['probes_per_s']/1e9:.2f} G probes/s, {c['launches']} launches, "
             f"{c['admissions']} admissions, {c['evictions']} evictions")
         results.append(res)
     out = {"metric": "served p50/p99 query latency", "unit": "ms", "n_gpus": 1,
            "zipf": args.serve_zipf,
-           "higher_is_better": False, "value": results[-1]["p99_ms"],
+           "higher_is_better": False,
+           "value": [r for r in results if r["budget_gb"] == budgets[0]][-1]["p99_ms"],
            "config": {"workload": f"served queries, k=25: {n/1e9:.1f} Gbp reference in {nchrom} "
                                   f"chromosomes, {S} samples of {args.serve_min_keys/1e9:.2f}-"
-                                  f"{args.serve_max_keys/1e9:.2f} G keys, HBM budget "
-                                  f"{args.serve_budget_gb} GB, Poisson arrivals, ranges "
+                                  f"{args.serve_max_keys/1e9:.2f} G keys, HBM budgets "
+                                  f"{budgets} GB, Poisson arrivals, ranges "
                                   f"log-uniform 1 Mbp-chromosome, W in 10/50/100 kbp",
                       "total_keys": total_keys, "per_rate": results}}
     print(json.dumps(out), flush=True)

@@ -111,7 +111,7 @@ struct hipibs_serve
     hipibs_serve_counters counters{};
     // asynchronous admission: the builder thread turns queued sample ids
     // into tables while the worker keeps launching resident requests
-    std::thread builder;
+    std::vector<std::thread> builders;
     std::deque<uint32_t> buildQueue;
     std::vector<Built> built;
     std::mutex bmu;
@@ -441,8 +441,9 @@ void destroy(hipibs_serve *s)
     s->bcv.notify_all();
     if (s->worker.joinable())
         s->worker.join();
-    if (s->builder.joinable())
-        s->builder.join();
+    for (auto &b : s->builders)
+        if (b.joinable())
+            b.join();
     for (auto &b : s->built)
         hipibs_table_free(b.table);
     (void)hipSetDevice(s->device);
@@ -590,7 +591,10 @@ extern "C" int hipibs_serve_submit(hipibs_serve *s, uint64_t request_id, uint32_
         s->counters.submitted += 1;
         if (!s->worker.joinable())
         {
-            s->builder = std::thread(builderLoop, s);
+            // concurrent builds overlap one sample's host-side staging with
+            // another's upload and insert
+            for (int i = 0; i < 3; ++i)
+                s->builders.emplace_back(builderLoop, s);
             s->worker = std::thread(workerLoop, s);
         }
     }
