@@ -61,9 +61,14 @@ def parse():
                     help="sample popularity ~ 1/rank^s (0 = uniform)")
     ap.add_argument("--serve-rates", default="20,100,200")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
-    ap.add_argument("--workload", default="scan", choices=["scan", "panel", "serve"],
+    ap.add_argument("--workload", default="scan", choices=["scan", "panel", "serve", "large"],
                     help="scan: the single-sample device scan (default line); panel: "
-                         "end-to-end fastibs runs, N single-sample runs vs one panel run")
+                         "end-to-end fastibs runs, N single-sample runs vs one panel run; "
+                         "serve: served queries; large: the k=31 4.2 Gbp scan + coverage")
+    ap.add_argument("--cpu-mapper-mbp", type=float, default=8.0,
+                    help="large: bases (Mbp) of the reference mapper's CPU sample")
+    ap.add_argument("--e2e", type=int, default=1,
+                    help="large: also run fastibs / fastibsmapper end to end from files")
     ap.add_argument("--panel-samples", type=int, default=8)
     ap.add_argument("--panel-min-keys", type=float, default=0.1e9)
     ap.add_argument("--panel-max-keys", type=float, default=0.5e9)
@@ -73,23 +78,26 @@ def parse():
 
 # ---- synthetic data on the GPU ---------------------------------------------------
 
-def gen_reference(torch, n, seed):
+def gen_reference(torch, n, seed, rep_frac=0.40, ident_lo=0.85, ident_hi=0.99, n_frac=0.03):
+    """Uniform ACGT; rep_frac of the sequence copied from 500 repeat
+    families of 1-10 kbp at per-copy identity U[ident_lo, ident_hi]; N runs
+    of 100-5000 bp, n_frac * n / 2550 of them (codes: 0..3 = ACGT, 4 = N)."""
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
     dev = "cuda"
     codes = torch.randint(0, 4, (n,), dtype=torch.uint8, device=dev, generator=g)
-    # 500 repeat families, lengths U[1k, 10k], copied over ~40% of the sequence
+    # 500 repeat families, lengths U[1k, 10k], copied over rep_frac of the sequence
     fam_len = torch.randint(1000, 10001, (500,), device=dev, generator=g)
     fam_off = torch.cumsum(fam_len, 0) - fam_len
     fam = torch.randint(0, 4, (int(fam_len.sum()),), dtype=torch.uint8, device=dev, generator=g)
-    n_copies = int(0.40 * n / float(fam_len.float().mean()))
+    n_copies = int(rep_frac * n / float(fam_len.float().mean()))
     region = n // n_copies
     f = torch.randint(0, 500, (n_copies,), device=dev, generator=g)
     lens = fam_len[f]
     slack = (region - lens).clamp(min=1)
     starts = torch.arange(n_copies, device=dev) * region + (
         torch.rand(n_copies, device=dev, generator=g) * slack).long()
-    ident = 0.85 + 0.14 * torch.rand(n_copies, device=dev, generator=g)
+    ident = ident_lo + (ident_hi - ident_lo) * torch.rand(n_copies, device=dev, generator=g)
     chunk = 20000
     for c0 in range(0, n_copies, chunk):
         sl = slice(c0, min(n_copies, c0 + chunk))
@@ -103,8 +111,8 @@ def gen_reference(torch, n, seed):
         mut = torch.rand(pos.numel(), device=dev, generator=g) > ident[sl][rep]
         rnd = torch.randint(0, 4, (pos.numel(),), dtype=torch.uint8, device=dev, generator=g)
         codes[pos] = torch.where(mut, rnd, vals)
-    # N runs over ~3% of the sequence, lengths U[100, 5000]
-    n_runs = int(0.03 * n / 2550)
+    # N runs, lengths U[100, 5000]
+    n_runs = int(n_frac * n / 2550)
     rl = torch.randint(100, 5001, (n_runs,), device=dev, generator=g)
     rs = (torch.rand(n_runs, device=dev, generator=g) * (n - 5001)).long()
     for c0 in range(0, n_runs, 20000):
@@ -177,7 +185,13 @@ def make_sample_keys(torch, hap_codes, target, k, seed):
 
 # ---- CPU baseline (the reference's CPU-BATCHED scan, compiled from its sources) ----
 
-def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_check):
+def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_check, k=K,
+                 keys_target=None, mapper_bp=0, gpu_coverage=None):
+    """The reference's CPU-BATCHED scan on a slice of the reference, and
+    the same sample scanned on the GPU (must be identical). With mapper_bp,
+    also the reference's mapper (CKMCFile::IsKmer per position + coverage
+    text) on the first mapper_bp bases of the slice against the GPU's
+    coverage of the same bases."""
     import numpy as np
 
     import hipibs
@@ -192,22 +206,34 @@ def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_c
     ref.ref_batched_scan.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_longlong,
                                      C.c_int, C.c_int, C.c_void_p]
     ref.ref_batched_scan.restype = C.c_double
+    ref.ref_coverage.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     frac = n_slice / ref_codes.numel()
-    keys = make_sample_keys(torch, hap_codes_slice, args.keys * frac, K, args.seed + 99)
+    keys = make_sample_keys(torch, hap_codes_slice, (keys_target or args.keys) * frac, k,
+                            args.seed + 99)
     hk = keys.cpu().numpy().view(np.uint64)
     log(f"cpu sample: {n_slice/1e6:.0f} Mbp slice, {hk.size/1e6:.1f} M keys -> KMC2 database")
+    mapper = None
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
         pre = os.path.join(td, "cpu_sample")
-        hipibs.kmc_write(pre, hk, K, p=9, sig_len=9, counter_size=1, version=0x200,
-                         min_count=1, max_count=255)
+        hipibs.kmc_write(pre, hk, k, p=9 if k == 25 else None, sig_len=9, counter_size=1,
+                         version=0x200, min_count=1, max_count=255)
+        if mapper_bp:
+            fa = os.path.join(td, "slice.fasta")
+            write_fasta(fa, ref_codes[:mapper_bp].cpu().numpy())
+            ct = os.path.join(td, "slice_cov.txt")
+            t0 = time.time()
+            if ref.ref_coverage(pre.encode(), fa.encode(), ct.encode()) != 1:
+                raise RuntimeError("reference mapper failed on the slice")
+            mapper_s = time.time() - t0
+            ref_text = open(ct, "rb").read()
         t0 = time.time()
         h = ref.ref_index_open(pre.encode())
         if not h:
             raise RuntimeError("reference KmcRaIndex failed to load the sample database")
         log(f"reference KmcRaIndex loaded in {time.time()-t0:.1f} s")
         text = np.frombuffer(b"ACGTN", dtype=np.uint8)[ref_codes[:n_slice].cpu().numpy()].tobytes()
-        wins = hipibs.chunk_sequences([n_slice], W, K)
+        wins = hipibs.chunk_sequences([n_slice], W, k)
         ws = np.array([w[1] for w in wins], dtype=np.int64)
         wl = np.array([w[2] - w[1] for w in wins], dtype=np.int64)
         stats = np.zeros((len(wins), 4), dtype=np.int32)
@@ -217,13 +243,26 @@ def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_c
     probes = int(stats[:, 0].sum())
     log(f"cpu baseline: {probes/1e6:.1f} M probes in {secs:.2f} s on {threads} threads")
     # same sample on the GPU: must be identical
-    t = hipibs.Table.from_device_keys(keys.data_ptr(), keys.numel(), K,
+    t = hipibs.Table.from_device_keys(keys.data_ptr(), keys.numel(), k,
                                       bucket_slots=args.bucket_slots,
                                       minimizer_len=args.minimizer_len)
     gst = gpu_check(t, n_slice, wins)
-    t.free()
     match = bool(np.array_equal(gst, stats))
     log(f"cpu/gpu statistics identical on the sample: {match}")
+    if mapper_bp:
+        cov = gpu_coverage(t, 0, mapper_bp)
+        lut = [b"%d" % v for v in range(256)]
+        gpu_text = b"chr1\n" + b",".join([lut[v] for v in cov.tolist()]) + b"\n"
+        same = gpu_text == ref_text
+        log(f"reference mapper: {mapper_bp/1e6:.0f} Mbp in {mapper_s:.1f} s (1 thread); "
+            f"coverage text identical to the GPU's: {same}")
+        mapper = {"value": mapper_bp / mapper_s, "unit": "reference bases mapped/s", "cores": 1,
+                  "kind": "reference",
+                  "sample": f"first {mapper_bp/1e6:.0f} Mbp of the slice vs the same KMC2 "
+                            f"database; the reference's mapper loop (CKMCFile::IsKmer per "
+                            f"position, coverage, to_chars text)",
+                  "text_identical_to_gpu": same}
+    t.free()
     lscpu = ""
     try:
         import subprocess
@@ -237,27 +276,38 @@ def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_c
              "sample": f"{n_slice/1e6:.0f} Mbp slice of the reference vs a {hk.size/1e6:.1f}M-key "
                        f"KMC2 database of the same construction; reference CPU-BATCHED "
                        f"(KmcRaIndex + probeBatched K=16, codec, run-length stats), "
-                       f"{threads} threads, {lscpu}".strip()},
+                       f"{threads} threads, {lscpu}".strip(),
+             **({"mapper": mapper} if mapper else {})},
             match)
 
 
 def write_fasta(path, codes_cpu, name=b"chr1", width=80):
+    write_fasta_records(path, [(name, codes_cpu)], width)
+
+
+def write_fasta_records(path, records, width=80):
+    """records: (name, uint8 codes 0..4 as numpy or a torch tensor on any device)."""
     import numpy as np
     lut = np.frombuffer(b"ACGTN", dtype=np.uint8)
-    n = codes_cpu.size
-    full = n // width
+    step = (1 << 24) // width * width
     with open(path, "wb") as f:
-        f.write(b">" + name + b"\n")
-        step = (1 << 24) // width * width
-        for b0 in range(0, full * width, step):
-            b1 = min(full * width, b0 + step)
-            block = lut[codes_cpu[b0:b1]].reshape(-1, width)
-            out = np.empty((block.shape[0], width + 1), dtype=np.uint8)
-            out[:, :width] = block
-            out[:, width] = ord("\n")
-            f.write(out.tobytes())
-        if full * width < n:
-            f.write(lut[codes_cpu[full * width:]].tobytes() + b"\n")
+        for name, codes in records:
+            n = int(codes.numel() if hasattr(codes, "numel") else codes.size)
+            full = n // width
+            f.write(b">" + name + b"\n")
+            for b0 in range(0, full * width, step):
+                b1 = min(full * width, b0 + step)
+                c = codes[b0:b1]
+                c = c.cpu().numpy() if hasattr(c, "cpu") else c
+                block = lut[c].reshape(-1, width)
+                out = np.empty((block.shape[0], width + 1), dtype=np.uint8)
+                out[:, :width] = block
+                out[:, width] = ord("\n")
+                f.write(out.tobytes())
+            if full * width < n:
+                c = codes[full * width:]
+                c = c.cpu().numpy() if hasattr(c, "cpu") else c
+                f.write(lut[c].tobytes() + b"\n")
 
 
 def run_panel(args):
@@ -474,12 +524,433 @@ def run_serve(args):
     print(json.dumps(out), flush=True)
 
 
+LARGE_K = 31
+LARGE_CHROM_MBP = (800, 700, 620, 560, 520, 500, 500)
+
+
+def unique_partitioned(torch, parts_of, n_parts=4):
+    """Distinct values of int64 codes too many for one sort: codes are split
+    by a hash of their value into n_parts disjoint classes, each made
+    unique on its own. parts_of(p) yields the code tensors of class p."""
+    out = []
+    for p in range(n_parts):
+        chunks = list(parts_of(p))
+        x = torch.cat(chunks) if chunks else torch.empty(0, dtype=torch.int64, device="cuda")
+        del chunks
+        out.append(torch.unique(x))
+        del x
+        torch.cuda.empty_cache()
+    return out
+
+
+def code_class(torch, x, n_parts):
+    """Class of each code in [0, n_parts) (a power of two <= 16): the top
+    bits of a multiplicative hash (int64 arithmetic wraps)."""
+    return ((x * -7046029254386353131) >> 60) & (n_parts - 1)
+
+
+def encode_range(torch, sw, ew, pos0, npos, contig_end, k):
+    import hipibs
+    out = torch.empty(npos, dtype=torch.int64, device="cuda")
+    hipibs._check(hipibs._lib.hipibs_encode_device(sw.data_ptr(), ew.data_ptr(), pos0, npos,
+                                                   contig_end, k, out.data_ptr(), None, None),
+                  "encode")
+    return out
+
+
+def mapper_text_checks(path, names, lens, chunk=1 << 26):
+    """Size-independent checks of a coverage text file, streamed: per
+    record the id line, the number of commas (n_bases - 1) and the sum of
+    the comma-separated values; values are 0..k <= 99."""
+    import numpy as np
+    out = []
+    with open(path, "rb") as f:
+        for name, L in zip(names, lens):
+            head = f.readline()
+            if head != name + b"\n":
+                return None
+            total, commas, prev_digit, prev_val, done = 0, 0, False, 0, False
+            while not done:
+                buf = f.read(chunk)
+                if not buf:
+                    return None
+                b = np.frombuffer(buf, dtype=np.uint8)
+                nl = np.flatnonzero(b == 10)
+                if nl.size:
+                    # the rest of this chunk belongs to the next record
+                    f.seek(-(b.size - nl[0] - 1), os.SEEK_CUR)
+                    b = b[:nl[0]]
+                    done = True
+                dig = (b >= 48) & (b <= 57)
+                val = b.astype(np.int64) - 48
+                # a digit followed by a digit is a tens digit
+                nxt = np.zeros_like(dig)
+                nxt[:-1] = dig[1:]
+                total += int((val * np.where(nxt, 10, 1))[dig].sum())
+                if prev_digit and dig.size and dig[0]:
+                    # the previous chunk's last digit was a tens digit
+                    total += 9 * prev_val
+                commas += int((b == 44).sum())
+                prev_digit = bool(dig[-1]) if dig.size else False
+                prev_val = int(val[-1]) if dig.size else 0
+            out.append((total, commas))
+    return out
+
+
+def run_large(args):
+    """Large-genome scan, k=31: 4.2 Gbp in 7 chromosomes (15% N, 60%
+    repeat-derived at 80-99% identity) vs a ~2.1e9-key sample (a 0.5%-SNP
+    haplotype with 30 introgressed 5-20 Mbp blocks at ~3% divergence, 80%
+    of the keys from its k-mers, the rest random), W = 50 kbp, plus
+    per-base coverage. Device-resident scan steps and coverage on the
+    device; the reference's CPU scan and mapper on slices; end to end, the
+    fastibs and fastibsmapper commands from FASTA + KMC files."""
+    import shutil
+    import subprocess
+
+    import numpy as np
+    import torch
+
+    import hipibs
+    k = LARGE_K
+    lens = [int(m * 1e6) // 64 * 64 for m in LARGE_CHROM_MBP]
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    n = int(offs[-1])
+    names = [b"chr%d" % (i + 1) for i in range(len(lens))]
+    keys_target = 2.1e9 if abs(args.keys - 2.0e9) < 1 else args.keys
+    t0 = time.time()
+    log(f"generating {n/1e9:.2f} Gbp reference in {len(lens)} chromosomes")
+    # N runs: -ln(1 - 0.15) of the length in expected run starts, so ~15% of
+    # the bases end up N after overlaps
+    ref_codes = gen_reference(torch, n, args.seed + 3, rep_frac=0.60, ident_lo=0.80,
+                              ident_hi=0.99, n_frac=0.1625)
+    sw, ew = pack(torch, ref_codes)
+    n_frac = float((ref_codes == 4).sum()) / n
+    log(f"reference ready: {n_frac*100:.1f}% N ({time.time()-t0:.1f} s)")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(args.seed + 4)
+    hap = ref_codes.clone()
+    step = 1 << 28
+
+    def mutate(lo, hi, rate):
+        for b0 in range(lo, hi, step):
+            b1 = min(hi, b0 + step)
+            seg = hap[b0:b1]
+            m = (torch.rand(b1 - b0, device="cuda", generator=g) < rate) & (seg < 4)
+            d = torch.randint(1, 4, (b1 - b0,), dtype=torch.uint8, device="cuda", generator=g)
+            seg[m] = (seg[m] + d[m]) % 4
+    mutate(0, n, 0.005)
+    rng = np.random.default_rng(args.seed + 5)
+    blocks = []
+    for _ in range(30):
+        c = int(rng.integers(0, len(lens)))
+        L = int(rng.uniform(5e6, 20e6))
+        s0 = int(offs[c] + rng.integers(0, lens[c] - L))
+        mutate(s0, s0 + L, 0.0251)  # 1 - (1 - 0.005)(1 - 0.0251) = 3%
+        blocks.append((c, s0 - int(offs[c]), L))
+    hsw, hew = pack(torch, hap)
+    del hap
+    hap_codes = [encode_range(torch, hsw, hew, int(offs[c]), lens[c] - k + 1,
+                              int(offs[c + 1]), k) for c in range(len(lens))]
+    del hsw, hew
+    cpu_slice = int(args.cpu_mbp * 1e6) // 64 * 64 if args.cpu_mbp > 0 else 0
+    cpu_slice = min(cpu_slice, lens[0])
+    hap_slice = hap_codes[0][:cpu_slice - k + 1].clone() if cpu_slice else None
+    n_parts = 4
+
+    def hap_part(p):
+        for hc in hap_codes:
+            v = hc[hc >= 0]
+            yield v[code_class(torch, v, n_parts) == p]
+    hk = unique_partitioned(torch, hap_part, n_parts)
+    del hap_codes
+    torch.cuda.empty_cache()
+    n_hap = sum(x.numel() for x in hk)
+    gk = torch.Generator(device="cuda")
+    gk.manual_seed(args.seed + 6)
+    want_hap = int(0.8 * keys_target)
+    hk = [x[torch.rand(x.numel(), device="cuda", generator=gk) < want_hap / n_hap] for x in hk]
+    n_sel = sum(x.numel() for x in hk)
+    rnd = random_canonical(torch, max(0, int(keys_target) - n_sel), k, gk)
+    parts = []
+    for p in range(n_parts):
+        r = torch.unique(rnd[code_class(torch, rnd, n_parts) == p])
+        r = r[~torch.isin(r, hk[p])]
+        parts.append(torch.cat([hk[p], r]))
+    del hk, rnd
+    keys = torch.cat(parts)
+    del parts
+    torch.cuda.empty_cache()
+    torch.cuda.synchronize()
+    log(f"sample keys: {keys.numel()/1e9:.3f} G distinct ({n_hap/1e9:.2f} G distinct haplotype "
+        f"k-mers) ({time.time()-t0:.1f} s)")
+    tb = time.time()
+    table = hipibs.Table.from_device_keys(keys.data_ptr(), keys.numel(), k,
+                                          bucket_slots=args.bucket_slots,
+                                          load_factor=args.load_factor)
+    build_s = time.time() - tb
+    info = table.info()
+    log(f"table: {info['n_keys']/1e9:.3f} G keys, {info['bytes']/1e9:.1f} GB, build {build_s:.2f} s")
+
+    wins = hipibs.chunk_sequences(lens, W, k)
+    win = np.zeros(len(wins), dtype=hipibs.WINDOW_DTYPE)
+    win["base"] = np.array([offs[w[0]] + w[1] for w in wins], dtype=np.uint64)
+    win["n_bases"] = np.array([w[2] - w[1] for w in wins], dtype=np.uint32)
+    d_win = torch.from_numpy(win.view(np.uint8)).cuda()
+    d_stats = torch.zeros(len(wins) * 4, dtype=torch.int32, device="cuda")
+    segs = C.c_uint32(0)
+    wb = hipibs._lib.hipibs_scan_work_bytes(len(wins), 1, W, k, args.segs, C.byref(segs))
+    d_work = torch.zeros(max(wb, 8), dtype=torch.uint8, device="cuda")
+    tabs = hipibs._table_array([table])
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def scan_step():
+        hipibs._check(hipibs._lib.hipibs_scan_windows_device(
+            tabs, 1, sw.data_ptr(), ew.data_ptr(), n, d_win.data_ptr(), len(wins), W,
+            d_stats.data_ptr(), args.segs, d_work.data_ptr(), wb, stream), "scan")
+
+    hipibs._lib.hipibs_set_profiling(1)
+    for _ in range(args.warmup):
+        scan_step()
+    torch.cuda.synchronize()
+    first = d_stats.cpu().numpy().reshape(-1, 4).copy()
+    kms = []
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        scan_step()
+        kms.append(hipibs._lib.hipibs_last_kernel_ms())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - ts
+    st = d_stats.cpu().numpy().reshape(-1, 4)
+    stable = bool(np.array_equal(st, first))
+    probes = int(st[:, 0].astype(np.int64).sum())
+    observed = int(st[:, 1].astype(np.int64).sum())
+    ms = elapsed / args.steps * 1e3
+    k_ms = float(np.mean(kms))
+    alg_bytes = 64.0 * probes + 0.375 * n
+    log(f"scan: {ms:.2f} ms/step, kernel {k_ms:.2f} ms, {probes/1e9:.3f} G probes, "
+        f"hit rate {observed/max(1,probes):.3f}, repeat-stable {stable}")
+
+    # per-base coverage of every chromosome on the device
+    hits_buf = [torch.empty((lens[c] - k + 1 + 31) // 32, dtype=torch.int32, device="cuda")
+                for c in range(len(lens))]
+    cov_buf = [torch.empty(lens[c], dtype=torch.uint8, device="cuda") for c in range(len(lens))]
+
+    def coverage_pass(t, c_list, pos_base=None):
+        for c in c_list:
+            npos = lens[c] - k + 1
+            hipibs._check(hipibs._lib.hipibs_hitmask_device(
+                t.handle, sw.data_ptr(), ew.data_ptr(), int(offs[c]), npos, int(offs[c + 1]),
+                hits_buf[c].data_ptr(), stream), "hitmask")
+            hipibs._check(hipibs._lib.hipibs_coverage_device(
+                hits_buf[c].data_ptr(), npos, lens[c], k, cov_buf[c].data_ptr(), stream),
+                "coverage")
+    coverage_pass(table, range(len(lens)))
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    cov_ms, hm_ms = [], []
+    for _ in range(max(1, args.steps // 2)):
+        ev[0].record()
+        for c in range(len(lens)):
+            npos = lens[c] - k + 1
+            hipibs._check(hipibs._lib.hipibs_hitmask_device(
+                table.handle, sw.data_ptr(), ew.data_ptr(), int(offs[c]), npos, int(offs[c + 1]),
+                hits_buf[c].data_ptr(), stream), "hitmask")
+        ev[1].record()
+        for c in range(len(lens)):
+            hipibs._check(hipibs._lib.hipibs_coverage_device(
+                hits_buf[c].data_ptr(), lens[c] - k + 1, lens[c], k, cov_buf[c].data_ptr(),
+                stream), "coverage")
+        ev[2].record()
+        torch.cuda.synchronize()
+        hm_ms.append(ev[0].elapsed_time(ev[1]))
+        cov_ms.append(ev[1].elapsed_time(ev[2]))
+    hm = float(np.mean(hm_ms))
+    cv = float(np.mean(cov_ms))
+    # full-size properties: sum(coverage) = k * hits; each window's observed
+    # count = the hit positions of its range (two different kernels)
+    shifts = torch.arange(32, device="cuda", dtype=torch.int32)
+    prop_sum, prop_obs = True, True
+    cov_sums, cov_ge10 = [], []
+    win_arr = np.array(wins, dtype=np.int64)
+    for c in range(len(lens)):
+        npos = lens[c] - k + 1
+        bits = ((hits_buf[c].view(-1, 1) >> shifts) & 1).view(-1)[:npos]
+        hits = int(bits.sum())
+        csum = int(cov_buf[c].sum(dtype=torch.int64))
+        cov_sums.append(csum)
+        cov_ge10.append(int((cov_buf[c] >= 10).sum()))
+        prop_sum &= csum == k * hits
+        cs = torch.zeros(npos + 1, dtype=torch.int64, device="cuda")
+        cs[1:] = torch.cumsum(bits, 0)
+        del bits
+        sel = np.flatnonzero(win_arr[:, 0] == c)
+        s_ = win_arr[sel, 1]
+        e_ = win_arr[sel, 2]
+        ok = e_ - s_ >= k
+        lo = torch.from_numpy(s_[ok]).cuda()
+        hi = torch.from_numpy(e_[ok] - k + 1).cuda()
+        obs = (cs[hi] - cs[lo]).cpu().numpy()
+        prop_obs &= bool(np.array_equal(obs, st[sel[ok], 1])) and bool((st[sel[~ok], 1] == 0).all())
+        del cs
+    torch.cuda.empty_cache()
+    log(f"coverage: hitmask {hm:.1f} ms + coverage {cv:.1f} ms for {n/1e9:.2f} Gbp; "
+        f"sum(cov) = k*hits: {prop_sum}; window observed = hit-mask range counts: {prop_obs}")
+
+    cpu, cpu_match = None, None
+    if cpu_slice:
+        def gpu_check(t, n_slice, swins):
+            wv = np.zeros(len(swins), dtype=hipibs.WINDOW_DTYPE)
+            wv["base"] = np.array([w[1] for w in swins], dtype=np.uint64)
+            wv["n_bases"] = np.array([w[2] - w[1] for w in swins], dtype=np.uint32)
+            dw = torch.from_numpy(wv.view(np.uint8)).cuda()
+            ds = torch.zeros(len(swins) * 4, dtype=torch.int32, device="cuda")
+            s2 = C.c_uint32(0)
+            wb2 = hipibs._lib.hipibs_scan_work_bytes(len(swins), 1, W, k, 0, C.byref(s2))
+            dwk = torch.zeros(max(wb2, 8), dtype=torch.uint8, device="cuda")
+            hipibs._check(hipibs._lib.hipibs_scan_windows_device(
+                hipibs._table_array([t]), 1, sw.data_ptr(), ew.data_ptr(), n_slice,
+                dw.data_ptr(), len(swins), W, ds.data_ptr(), 0, dwk.data_ptr(), wb2, None),
+                "scan")
+            torch.cuda.synchronize()
+            return ds.cpu().numpy().reshape(-1, 4)
+
+        def gpu_coverage(t, base, nb):
+            npos = nb - k + 1
+            hw = torch.empty((npos + 31) // 32, dtype=torch.int32, device="cuda")
+            cvb = torch.empty(nb, dtype=torch.uint8, device="cuda")
+            hipibs._check(hipibs._lib.hipibs_hitmask_device(
+                t.handle, sw.data_ptr(), ew.data_ptr(), base, npos, base + nb, hw.data_ptr(),
+                None), "hitmask")
+            hipibs._check(hipibs._lib.hipibs_coverage_device(hw.data_ptr(), npos, nb, k,
+                                                             cvb.data_ptr(), None), "coverage")
+            torch.cuda.synchronize()
+            return cvb.cpu().numpy()
+        try:
+            cpu, cpu_match = cpu_baseline(torch, ref_codes, sw, ew, hap_slice, cpu_slice, args,
+                                          gpu_check, k=k, keys_target=keys_target,
+                                          mapper_bp=int(args.cpu_mapper_mbp * 1e6) // 64 * 64,
+                                          gpu_coverage=gpu_coverage)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline failed: {e}")
+
+    e2e = None
+    if args.e2e:
+        root = os.path.join(args.workdir, "hipibs_large")
+        shutil.rmtree(root, ignore_errors=True)
+        os.makedirs(os.path.join(root, "refs"))
+        os.makedirs(os.path.join(root, "sample"))
+        free = shutil.disk_usage(root).free
+        text_bytes = sum(len(nm) + 1 + 2 * L + cov_ge10[c] for c, (nm, L) in
+                         enumerate(zip(names, lens)))
+        need = n * 1.03 + keys.numel() * 4.2 + text_bytes
+        if need > 0.85 * free:
+            log(f"end-to-end runs skipped: need ~{need/1e9:.0f} GB, {free/1e9:.0f} GB free")
+        else:
+            t1 = time.time()
+            write_fasta_records(os.path.join(root, "refs", "large_genome.fasta"),
+                                [(names[c], ref_codes[int(offs[c]):int(offs[c + 1])])
+                                 for c in range(len(lens))])
+            fasta_s = time.time() - t1
+            t1 = time.time()
+            hkeys = keys.cpu().numpy().view(np.uint64)
+            hipibs.kmc_write(os.path.join(root, "sample", "sample"), hkeys, k, sig_len=9,
+                             counter_size=1, version=0x200, min_count=1, max_count=255)
+            del hkeys
+            kmc_s = time.time() - t1
+            log(f"files: FASTA {fasta_s:.1f} s, KMC2 database {kmc_s:.1f} s")
+            del keys
+            table.free()
+            hits_buf, cov_buf = None, None
+            torch.cuda.empty_cache()
+            bindir = os.path.join(ROOT, "gpu-ibs_amd", "bin")
+
+            def run(cmd):
+                t2 = time.time()
+                r = subprocess.run(cmd, capture_output=True, text=True)
+                if r.returncode != 0:
+                    raise RuntimeError(f"{cmd[0]} failed: {r.stderr[-2000:]}")
+                return time.time() - t2, r.stdout
+            db = os.path.join(root, "sample")
+            refs = os.path.join(root, "refs")
+            scan_s, out = run([os.path.join(bindir, "fastibs"), db, refs,
+                               os.path.join(root, "out"), str(W)])
+            tsv = os.path.join(root, "out", f"sample_v_large_{W}.tsv")
+            rows = open(tsv, "rb").read().splitlines()[1:]
+            tv = np.array([[int(x) for x in r.split(b"\t")[3:7]] for r in rows], dtype=np.int64)
+            tsv_same = tv.shape == st.shape and bool(np.array_equal(tv, st))
+            log(f"fastibs end to end: {scan_s:.1f} s; TSV equals the device statistics: {tsv_same}")
+            map_s, mout = run([os.path.join(bindir, "fastibsmapper"), db, refs,
+                               os.path.join(root, "map")])
+            mfiles = [f for f in os.listdir(os.path.join(root, "map")) if f.endswith(".txt")]
+            mpath = os.path.join(root, "map", mfiles[0])
+            msize = os.path.getsize(mpath)
+            chk = mapper_text_checks(mpath, names, lens)
+            map_ok = (msize == text_bytes and chk is not None and
+                      all(s_ == cov_sums[c] and cm == lens[c] - 1
+                          for c, (s_, cm) in enumerate(chk)))
+            log(f"fastibsmapper end to end: {map_s:.1f} s, {msize/1e9:.2f} GB of coverage text; "
+                f"size, comma counts and value sums match the device coverage: {map_ok}")
+            e2e = {"fastibs_s": scan_s, "fastibsmapper_s": map_s, "fasta_write_s": fasta_s,
+                   "kmc_write_s": kmc_s, "tsv_equals_device_stats": tsv_same,
+                   "coverage_text_bytes": msize, "coverage_text_checks_pass": map_ok,
+                   "fastibs_log": [l for l in out.splitlines() if l.strip()][-6:],
+                   "fastibsmapper_log": [l for l in mout.splitlines() if l.strip()][-6:]}
+            shutil.rmtree(root, ignore_errors=True)
+    table.free()
+    value = probes / (ms / 1e3)
+    hm_bytes = 64.0 * probes + 0.375 * n + n / 8
+    out = {
+        "metric": "ref k-mers probed/s",
+        "value": value,
+        "unit": "ref k-mers probed/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (GPU-generated, seeded)",
+        "config": {"workload": "large-genome scan, k=31: 4.2 Gbp synthetic reference in 7 "
+                               "chromosomes (15% N, 60% repeat-derived) vs ~2.1e9-key sample, "
+                               "W=50 kbp, plus per-base coverage",
+                   "reference_bases": n, "n_fraction": n_frac, "chromosome_bases": lens,
+                   "windows": len(wins), "k": k, "window": W,
+                   "sample_keys": int(info["n_keys"]), "table_bytes": int(info["bytes"]),
+                   "bucket_slots": info["bucket_slots"], "load_factor": args.load_factor,
+                   "segs_per_window": segs.value, "scan_wall_s": ms / 1e3,
+                   "table_build_s": build_s, "probes_per_step": probes,
+                   "hit_rate": observed / max(1, probes), "repeat_stable": stable,
+                   "introgressed_blocks": blocks,
+                   "coverage": {"hitmask_ms": hm, "coverage_ms": cv,
+                                "bases_per_s": n / ((hm + cv) / 1e3),
+                                "hitmask_achieved_gbs": hm_bytes / (hm / 1e3) / 1e9,
+                                "coverage_achieved_gbs": (n / 8 + n) / (cv / 1e3) / 1e9,
+                                "sum_equals_k_times_hits": prop_sum,
+                                "window_observed_equals_hitmask": prop_obs},
+                   "cpu_gpu_sample_identical": cpu_match,
+                   "end_to_end": e2e},
+        "roofline": {"bound": "hbm", "achieved": alg_bytes / (k_ms / 1e3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg_bytes / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"scanKernel<{info['bucket_slots']}>", "kernel_ms": k_ms,
+                     "algorithmic_bytes": alg_bytes,
+                     "probe_rate_gps": probes / (k_ms / 1e3) / 1e9},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.workload == "panel":
         return run_panel(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload == "large":
+        return run_large(args)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
