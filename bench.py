@@ -712,17 +712,19 @@ def run_large(args):
     hipibs._lib.hipibs_set_profiling(1)
     for _ in range(args.warmup):
         scan_step()
-    torch.cuda.synchronize()
-    first = d_stats.cpu().numpy().reshape(-1, 4).copy()
     kms = []
+    torch.cuda.synchronize()
     ts = time.perf_counter()
     for _ in range(args.steps):
         scan_step()
         kms.append(hipibs._lib.hipibs_last_kernel_ms())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - ts
-    st = d_stats.cpu().numpy().reshape(-1, 4)
-    stable = bool(np.array_equal(st, first))
+    st = d_stats.cpu().numpy().reshape(-1, 4).copy()
+    d_stats.zero_()
+    scan_step()
+    torch.cuda.synchronize()
+    stable = bool(np.array_equal(d_stats.cpu().numpy().reshape(-1, 4), st))
     probes = int(st[:, 0].astype(np.int64).sum())
     observed = int(st[:, 1].astype(np.int64).sum())
     ms = elapsed / args.steps * 1e3
@@ -1018,8 +1020,6 @@ def main():
     hipibs._lib.hipibs_set_profiling(1)
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    first = d_stats.cpu().numpy().reshape(-1, 4).copy()
     kms = []
     torch.cuda.synchronize()
     ts = time.perf_counter()
@@ -1028,8 +1028,12 @@ def main():
         kms.append(hipibs._lib.hipibs_last_kernel_ms())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - ts
-    st = d_stats.cpu().numpy().reshape(-1, 4)
-    stable = bool(np.array_equal(st, first))
+    st = d_stats.cpu().numpy().reshape(-1, 4).copy()
+    # one more (untimed) step must give the same statistics
+    d_stats.zero_()
+    step()
+    torch.cuda.synchronize()
+    stable = bool(np.array_equal(d_stats.cpu().numpy().reshape(-1, 4), st))
     probes = int(st[:, 0].astype(np.int64).sum())
     observed = int(st[:, 1].astype(np.int64).sum())
     ms = elapsed / args.steps * 1e3
@@ -1042,22 +1046,27 @@ def main():
     gups = {}
     try:
         blib = C.CDLL(os.path.join(ROOT, "gpu-ibs_amd", "lib", "libhipibs_bench.so"))
-        blib.hipibs_bench_gups.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_int,
-                                           C.c_int, C.POINTER(C.c_double), C.c_void_p]
+        blib.hipibs_bench_gups_policy.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64,
+                                                  C.c_int, C.c_int, C.c_int,
+                                                  C.POINTER(C.c_double), C.c_void_p]
         blib.hipibs_bench_gups_reads.argtypes = [C.c_uint32, C.c_uint64, C.c_int, C.c_int]
         blib.hipibs_bench_gups_reads.restype = C.c_uint64
         gbuf = torch.empty(info["bytes"], dtype=torch.uint8, device="cuda")
         S = info["bucket_slots"]
-        for rounds in ((4, 8) if S == 8 else (4, 8)):
-            for wpc in (16, 32):
-                t_ms = C.c_double(0)
-                blib.hipibs_bench_gups(gbuf.data_ptr(), info["n_buckets"], S, probes, rounds, wpc,
-                                       C.byref(t_ms), None)
-                rc = blib.hipibs_bench_gups(gbuf.data_ptr(), info["n_buckets"], S, probes, rounds,
-                                            wpc, C.byref(t_ms), None)
-                reads = blib.hipibs_bench_gups_reads(S, probes, rounds, wpc)
-                if rc == 0 and t_ms.value > 0:
-                    gups[f"r{rounds}_w{wpc}"] = reads * S * 8 / (t_ms.value / 1e3) / 1e9
+        # policy 0: plain loads; 1: non-temporal loads, as the probe issues them
+        for policy in ((0, 1) if S == 8 else (0,)):
+            for rounds in (4, 8):
+                for wpc in (16, 32):
+                    t_ms = C.c_double(0)
+                    blib.hipibs_bench_gups_policy(gbuf.data_ptr(), info["n_buckets"], S, probes,
+                                                  rounds, wpc, policy, C.byref(t_ms), None)
+                    rc = blib.hipibs_bench_gups_policy(gbuf.data_ptr(), info["n_buckets"], S,
+                                                       probes, rounds, wpc, policy,
+                                                       C.byref(t_ms), None)
+                    reads = blib.hipibs_bench_gups_reads(S, probes, rounds, wpc)
+                    if rc == 0 and t_ms.value > 0:
+                        gups[f"{'nt_' if policy else ''}r{rounds}_w{wpc}"] = \
+                            reads * S * 8 / (t_ms.value / 1e3) / 1e9
         del gbuf
         torch.cuda.empty_cache()
         log(f"random {S*8} B-sector ceiling (GB/s): {gups}")

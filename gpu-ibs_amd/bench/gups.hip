@@ -23,7 +23,33 @@ __device__ inline uint64_t mix(uint64_t x)
     return x ^ (x >> 32);
 }
 
-template <int S, int ROUNDS>
+// Load policy of the bucket reads: 0 = plain 16 B loads; 1 = non-temporal
+// 16 B loads (nt); 2 = two relaxed 8 B atomic loads at system scope
+// (sc0 sc1); 3 = the same at device scope (sc1).
+template <int POLICY>
+__device__ inline U64x2 loadBucketPart(const uint64_t *p)
+{
+    if constexpr (POLICY == 0)
+        return *(GlobalU64x2)(reinterpret_cast<const U64x2 *>(p));
+    else if constexpr (POLICY == 1)
+        return __builtin_nontemporal_load((GlobalU64x2)(reinterpret_cast<const U64x2 *>(p)));
+    else if constexpr (POLICY == 2)
+    {
+        U64x2 v;
+        v.x = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        v.y = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return v;
+    }
+    else
+    {
+        U64x2 v;
+        v.x = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v.y = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return v;
+    }
+}
+
+template <int S, int ROUNDS, int POLICY = 0>
 __global__ __launch_bounds__(256) void gupsKernel(const uint64_t *buf, uint64_t nBuckets,
                                                   uint64_t iters, uint64_t seed, uint64_t *out)
 {
@@ -40,7 +66,7 @@ __global__ __launch_bounds__(256) void gupsKernel(const uint64_t *buf, uint64_t 
         {
             const uint64_t h = mix(seed ^ (((wave * iters + it) * ROUNDS + r) * 64 + g));
             const uint64_t b = ((h >> 32) * nBuckets) >> 32;
-            v[r] = *(GlobalU64x2)(reinterpret_cast<const U64x2 *>(buf + b * S + sub * 2));
+            v[r] = loadBucketPart<POLICY>(buf + b * S + sub * 2);
         }
 #pragma unroll
         for (int r = 0; r < ROUNDS; ++r)
@@ -55,10 +81,26 @@ __global__ __launch_bounds__(256) void gupsKernel(const uint64_t *buf, uint64_t 
 // Reads about n_reads random buckets of bucket_slots*8 bytes from d_buf
 // (n_buckets buckets) with `rounds` loads in flight per wave; returns the
 // kernel time in ms through *ms (HIP events on `stream`).
+extern "C" int hipibs_bench_gups_policy(const void *d_buf, uint64_t n_buckets,
+                                        uint32_t bucket_slots, uint64_t n_reads, int rounds,
+                                        int waves_per_cu, int policy, double *ms, void *stream);
+
 extern "C" int hipibs_bench_gups(const void *d_buf, uint64_t n_buckets, uint32_t bucket_slots,
                                  uint64_t n_reads, int rounds, int waves_per_cu, double *ms,
                                  void *stream)
 {
+    return hipibs_bench_gups_policy(d_buf, n_buckets, bucket_slots, n_reads, rounds, waves_per_cu,
+                                    0, ms, stream);
+}
+
+// As hipibs_bench_gups with the load policy of gupsKernel (0..3; policies
+// 1..3 with 64 B buckets and 4 or 8 rounds only).
+extern "C" int hipibs_bench_gups_policy(const void *d_buf, uint64_t n_buckets,
+                                        uint32_t bucket_slots, uint64_t n_reads, int rounds,
+                                        int waves_per_cu, int policy, double *ms, void *stream)
+{
+    if (policy < 0 || policy > 3 || (policy && (bucket_slots != 8 || (rounds != 4 && rounds != 8))))
+        return -1;
     if (!d_buf || !ms || n_buckets == 0 || n_buckets >= (1ull << 32))
         return -1;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -77,16 +119,22 @@ extern "C" int hipibs_bench_gups(const void *d_buf, uint64_t n_buckets, uint32_t
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, s);
     const uint64_t *buf = static_cast<const uint64_t *>(d_buf);
-#define GUPS_CASE(SS, RR)                                                                        \
-    if (bucket_slots == SS && rounds == RR)                                                      \
-        hipLaunchKernelGGL((gupsKernel<SS, RR>), dim3(blocks), dim3(256), 0, s, buf, n_buckets,  \
-                           iters, 0x5EEDull, out);
-    GUPS_CASE(8, 1)
-    GUPS_CASE(8, 2)
-    GUPS_CASE(8, 4)
-    GUPS_CASE(8, 8)
-    GUPS_CASE(16, 4)
-    GUPS_CASE(16, 8)
+#define GUPS_CASE(SS, RR, PP)                                                                    \
+    if (bucket_slots == SS && rounds == RR && policy == PP)                                      \
+        hipLaunchKernelGGL((gupsKernel<SS, RR, PP>), dim3(blocks), dim3(256), 0, s, buf,         \
+                           n_buckets, iters, 0x5EEDull, out);
+    GUPS_CASE(8, 1, 0)
+    GUPS_CASE(8, 2, 0)
+    GUPS_CASE(8, 4, 0)
+    GUPS_CASE(8, 8, 0)
+    GUPS_CASE(16, 4, 0)
+    GUPS_CASE(16, 8, 0)
+    GUPS_CASE(8, 4, 1)
+    GUPS_CASE(8, 8, 1)
+    GUPS_CASE(8, 4, 2)
+    GUPS_CASE(8, 8, 2)
+    GUPS_CASE(8, 4, 3)
+    GUPS_CASE(8, 8, 3)
 #undef GUPS_CASE
     (void)hipEventRecord(e1, s);
     hipError_t err = hipEventSynchronize(e1);

@@ -14,6 +14,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "hipibs.h"
 
@@ -242,4 +243,25 @@ uint32_t autoSegsPerWindow(uint32_t nWindows, int nTables, uint32_t maxPositions
 uint64_t scanWorkBytes(uint32_t nWindows, int nTables, uint32_t segsPerWindow);
 int kmcReadKeys(const std::string &prefix, uint64_t *keys, uint64_t cap, uint64_t *nOut,
                 unsigned threads, hipibs_kmc_info *infoOut);
+
+// The raw records of a KMC suffix file, for readers that decode on the
+// device: record r of bucket j (lut[j] <= r < lut[j+1]) holds the key
+// ((j mod 4^p) << 2(k-p)) | (its first sufBytes bytes, big-endian).
+struct KmcRecords
+{
+    hipibs_kmc_info info{};
+    std::vector<uint64_t> lut; // one entry per bucket plus a guard > total
+    std::string path;
+    int fd = -1;
+    uint32_t sufBytes = 0, recBytes = 0;
+    uint64_t total = 0;
+    KmcRecords() = default;
+    KmcRecords(const KmcRecords &) = delete;
+    KmcRecords &operator=(const KmcRecords &) = delete;
+    ~KmcRecords();
+};
+int kmcOpenRecords(const std::string &prefix, KmcRecords &out);
+// Records [r0, r0+m) as stored (m * recBytes bytes) into dst, read by up to
+// `threads` parallel preads (0 = every usable CPU).
+int kmcReadRecords(const KmcRecords &src, uint64_t r0, uint64_t m, void *dst, unsigned threads);
 } // namespace hipibs

@@ -241,6 +241,71 @@ int loadPre(const std::string &prefix, PreFile &pf, bool withLut)
 // Reads every record's suffix and rebuilds the key, multithreaded over
 // contiguous record slices (each thread locates its first bucket by binary
 // search over the monotonic LUT and walks forward).
+KmcRecords::~KmcRecords()
+{
+    if (fd >= 0)
+        ::close(fd);
+}
+
+int kmcOpenRecords(const std::string &prefix, KmcRecords &out)
+{
+    PreFile pf;
+    int rc = loadPre(prefix, pf, true);
+    if (rc)
+        return rc;
+    const hipibs_kmc_info &I = pf.info;
+    out.path = prefix + ".kmc_suf";
+    out.fd = ::open(out.path.c_str(), O_RDONLY);
+    if (out.fd < 0)
+        return setError(HIPIBS_E_IO, "cannot open " + out.path);
+    const uint64_t fsize = fileSize(out.fd);
+    char head[4], tail[4];
+    if (fsize < 8 || !preadAll(out.fd, head, 4, 0) || !preadAll(out.fd, tail, 4, fsize - 4) ||
+        std::memcmp(head, "KMCS", 4) || std::memcmp(tail, "KMCS", 4))
+        return setError(HIPIBS_E_IO, "bad KMC suffix file markers: " + out.path);
+    out.info = I;
+    out.total = I.total_kmers;
+    out.sufBytes = (I.kmer_length - I.lut_prefix_length) / 4;
+    out.recBytes = out.sufBytes + I.counter_size;
+    if (out.recBytes == 0 && out.total > 0)
+        return setError(HIPIBS_E_IO, "KMC: zero-length records");
+    if (fsize - 8 < out.total * (uint64_t)out.recBytes)
+        return setError(HIPIBS_E_IO, "KMC: suffix file smaller than its k-mer count");
+    out.lut = std::move(pf.lut);
+    return 0;
+}
+
+int kmcReadRecords(const KmcRecords &src, uint64_t r0, uint64_t m, void *dst, unsigned threads)
+{
+    const uint64_t bytes = m * src.recBytes;
+    const uint64_t off0 = 4 + r0 * src.recBytes;
+    if (threads == 0)
+        threads = usableCpus();
+    const uint64_t minPiece = 4ull << 20;
+    threads = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>({threads, 64, bytes / minPiece}));
+    const uint64_t per = (bytes + threads - 1) / threads;
+    std::atomic<bool> bad{false};
+    auto work = [&](unsigned t)
+    {
+        const uint64_t a = std::min(bytes, (uint64_t)t * per), b = std::min(bytes, a + per);
+        if (a < b && !preadAll(src.fd, static_cast<char *>(dst) + a, b - a, off0 + a))
+            bad = true;
+    };
+    if (threads == 1)
+        work(0);
+    else
+    {
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < threads; ++t)
+            pool.emplace_back(work, t);
+        for (auto &th : pool)
+            th.join();
+    }
+    if (bad)
+        return setError(HIPIBS_E_IO, "short read of KMC suffix records: " + src.path);
+    return 0;
+}
+
 int kmcReadKeys(const std::string &prefix, uint64_t *keys, uint64_t cap, uint64_t *nOut,
                 unsigned threads, hipibs_kmc_info *infoOut)
 {

@@ -4,14 +4,22 @@
 // partitioning chunks.
 #pragma once
 
+#include <fcntl.h>
+#include <sched.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <zlib.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <string_view>
+#include <thread>
 #include <vector>
 
 struct ParseResult
@@ -66,38 +74,112 @@ inline std::string decompressGzip(const std::string &path)
     return out;
 }
 
-// FASTA or FASTQ, decided by the first byte ('>' or '@'). Lines are split
-// on '\n' only (a '\r' stays in the sequence and becomes an invalid base),
-// empty lines are ignored, a record with no sequence is dropped.
-inline ParseResult readSequenceFile(const std::string &path, bool compressed = false)
+namespace seqio_detail
 {
-    std::string data;
-    if (compressed)
-        data = decompressGzip(path);
-    else
+
+// CPUs this process may run on.
+inline unsigned hostThreads()
+{
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0)
+        return std::max(1, CPU_COUNT(&set));
+    return std::max(1u, std::thread::hardware_concurrency());
+}
+
+// fn(i) for every i in [0, n), spread over up to `threads` threads; the
+// first exception thrown by fn is rethrown.
+template <class F>
+void parallelFor(size_t n, unsigned threads, F fn)
+{
+    threads = (unsigned)std::min<size_t>(std::max(1u, threads), n);
+    if (threads <= 1)
     {
-        std::ifstream f(path, std::ios::binary | std::ios::ate);
-        if (!f)
-            throw std::runtime_error("Failed to open file " + path);
-        data.resize((size_t)f.tellg());
-        f.seekg(0);
-        f.read(data.data(), (std::streamsize)data.size());
+        for (size_t i = 0; i < n; ++i)
+            fn(i);
+        return;
     }
-    if (data.empty())
-        throw std::runtime_error("Empty file");
-    const bool fastq = data[0] == '@';
-    if (!fastq && data[0] != '>')
-        throw std::runtime_error("Unknown file type");
-    std::cout << "Parsing " << (fastq ? "fastq" : "fasta") << " data\n";
+    std::atomic<size_t> next{0};
+    std::exception_ptr err;
+    std::atomic<bool> failed{false};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < threads; ++t)
+        pool.emplace_back([&]
+                          {
+                              for (size_t i; !failed && (i = next++) < n;)
+                              {
+                                  try
+                                  {
+                                      fn(i);
+                                  }
+                                  catch (...)
+                                  {
+                                      if (!failed.exchange(true))
+                                          err = std::current_exception();
+                                  }
+                              }
+                          });
+    for (auto &th : pool)
+        th.join();
+    if (err)
+        std::rethrow_exception(err);
+}
+
+// A whole file in memory, read by parallel pread calls (no zero fill).
+struct FileBytes
+{
+    std::unique_ptr<char[]> data;
+    size_t size = 0;
+};
+
+inline FileBytes readWholeFile(const std::string &path, unsigned threads)
+{
+    const int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0)
+        throw std::runtime_error("Failed to open file " + path);
+    struct stat st;
+    if (fstat(fd, &st) != 0)
+    {
+        ::close(fd);
+        throw std::runtime_error("Failed to open file " + path);
+    }
+    FileBytes fb;
+    fb.size = (size_t)st.st_size;
+    fb.data.reset(new char[fb.size ? fb.size : 1]);
+    const size_t piece = 64ull << 20;
+    std::atomic<bool> bad{false};
+    parallelFor((fb.size + piece - 1) / piece, threads, [&](size_t i)
+                {
+                    size_t off = i * piece;
+                    const size_t end = std::min(fb.size, off + piece);
+                    while (off < end)
+                    {
+                        const ssize_t r = ::pread(fd, fb.data.get() + off, end - off, (off_t)off);
+                        if (r <= 0)
+                        {
+                            bad = true;
+                            return;
+                        }
+                        off += (size_t)r;
+                    }
+                });
+    ::close(fd);
+    if (bad)
+        throw std::runtime_error("Failed to read file " + path);
+    return fb;
+}
+
+// FASTQ: '@' id lines, sequence lines up to a '+' line, whose next line is
+// quality and skipped.
+inline ParseResult parseSerial(const char *data, size_t n, bool fastq)
+{
     const char head = fastq ? '@' : '>';
     ParseResult r;
     std::string id, seq;
     bool skipQuality = false;
     size_t pos = 0;
-    const size_t n = data.size();
     while (pos < n)
     {
-        const char *p = data.data() + pos;
+        const char *p = data + pos;
         const void *nl = std::memchr(p, '\n', n - pos);
         const size_t len = nl ? (size_t)((const char *)nl - p) : n - pos;
         pos += len + 1;
@@ -132,6 +214,139 @@ inline ParseResult readSequenceFile(const std::string &path, bool compressed = f
         r.sequences.push_back(std::move(seq));
     }
     return r;
+}
+
+// FASTA with the serial parser's line rules, in parallel: a record starts
+// at every '>' that begins a line; its id is the rest of that line, its
+// sequence the concatenation of the following lines (up to the next
+// record) without their '\n'; records with no sequence are dropped.
+// `piece` is the largest span of input one task handles.
+inline ParseResult parseFastaParallel(const char *d, size_t n, unsigned threads,
+                                      size_t piece = 32ull << 20)
+{
+    piece = std::max<size_t>(piece, 1);
+    const size_t nPieces = (n + piece - 1) / piece;
+    std::vector<std::vector<size_t>> found(nPieces);
+    parallelFor(nPieces, threads, [&](size_t i)
+                {
+                    const size_t lo = i * piece, hi = std::min(n, lo + piece);
+                    for (size_t q = lo; q < hi;)
+                    {
+                        const void *g = std::memchr(d + q, '>', hi - q);
+                        if (!g)
+                            break;
+                        const size_t at = (size_t)((const char *)g - d);
+                        if (at == 0 || d[at - 1] == '\n')
+                            found[i].push_back(at);
+                        q = at + 1;
+                    }
+                });
+    std::vector<size_t> heads;
+    for (auto &f : found)
+        heads.insert(heads.end(), f.begin(), f.end());
+    found.clear();
+    struct Rec
+    {
+        size_t idBegin, idEnd, bodyBegin, bodyEnd;
+    };
+    std::vector<Rec> recs(heads.size());
+    for (size_t j = 0; j < heads.size(); ++j)
+    {
+        const size_t h = heads[j];
+        const size_t next = j + 1 < heads.size() ? heads[j + 1] : n;
+        const void *nl = std::memchr(d + h, '\n', next - h);
+        const size_t e = nl ? (size_t)((const char *)nl - d) : next;
+        recs[j] = Rec{h + 1, e, std::min(e + 1, next), next};
+    }
+    // tasks: spans of record bodies of at most `piece` bytes
+    struct Task
+    {
+        size_t rec, begin, end, out;
+    };
+    std::vector<Task> tasks;
+    for (size_t j = 0; j < recs.size(); ++j)
+        for (size_t b = recs[j].bodyBegin; b < recs[j].bodyEnd; b += piece)
+            tasks.push_back(Task{j, b, std::min(recs[j].bodyEnd, b + piece), 0});
+    std::vector<size_t> bases(tasks.size());
+    parallelFor(tasks.size(), threads, [&](size_t i)
+                {
+                    const Task &t = tasks[i];
+                    bases[i] = (t.end - t.begin) - (size_t)std::count(d + t.begin, d + t.end, '\n');
+                });
+    std::vector<size_t> recLen(recs.size(), 0);
+    for (size_t i = 0; i < tasks.size(); ++i)
+    {
+        tasks[i].out = recLen[tasks[i].rec];
+        recLen[tasks[i].rec] += bases[i];
+    }
+    std::vector<size_t> keep;
+    for (size_t j = 0; j < recs.size(); ++j)
+        if (recLen[j])
+            keep.push_back(j);
+    std::vector<size_t> slot(recs.size(), 0);
+    ParseResult r;
+    r.ids.resize(keep.size());
+    r.sequences.resize(keep.size());
+    for (size_t q = 0; q < keep.size(); ++q)
+    {
+        slot[keep[q]] = q;
+        r.ids[q].assign(d + recs[keep[q]].idBegin, recs[keep[q]].idEnd - recs[keep[q]].idBegin);
+    }
+    parallelFor(keep.size(), threads, [&](size_t q) { r.sequences[q].resize(recLen[keep[q]]); });
+    parallelFor(tasks.size(), threads, [&](size_t i)
+                {
+                    const Task &t = tasks[i];
+                    if (!recLen[t.rec])
+                        return;
+                    char *out = r.sequences[slot[t.rec]].data() + t.out;
+                    for (size_t q = t.begin; q < t.end;)
+                    {
+                        const void *nl = std::memchr(d + q, '\n', t.end - q);
+                        const size_t stop = nl ? (size_t)((const char *)nl - d) : t.end;
+                        std::memcpy(out, d + q, stop - q);
+                        out += stop - q;
+                        q = stop + 1;
+                    }
+                });
+    return r;
+}
+
+} // namespace seqio_detail
+
+// FASTA or FASTQ, decided by the first byte ('>' or '@'). Lines are split
+// on '\n' only (a '\r' stays in the sequence and becomes an invalid base),
+// empty lines are ignored, a record with no sequence is dropped. FASTA is
+// read and parsed on all usable CPUs (`threads` = 0) or on `threads`.
+inline ParseResult readSequenceFile(const std::string &path, bool compressed = false,
+                                    unsigned threads = 0)
+{
+    if (threads == 0)
+        threads = std::min(64u, seqio_detail::hostThreads());
+    std::string gz;
+    seqio_detail::FileBytes fb;
+    const char *data;
+    size_t n;
+    if (compressed)
+    {
+        gz = decompressGzip(path);
+        data = gz.data();
+        n = gz.size();
+    }
+    else
+    {
+        fb = seqio_detail::readWholeFile(path, threads);
+        data = fb.data.get();
+        n = fb.size;
+    }
+    if (n == 0)
+        throw std::runtime_error("Empty file");
+    const bool fastq = data[0] == '@';
+    if (!fastq && data[0] != '>')
+        throw std::runtime_error("Unknown file type");
+    std::cout << "Parsing " << (fastq ? "fastq" : "fasta") << " data\n";
+    if (fastq)
+        return seqio_detail::parseSerial(data, n, true);
+    return seqio_detail::parseFastaParallel(data, n, threads);
 }
 
 // Scan windows: [j, min(j+W, len)) for j = 0, W-k, 2(W-k), ... while

@@ -141,6 +141,25 @@ int main()
     win.addVariationOriginal(24, 25);
     win.addVariationOriginal(30, 25);
     CHECK(win.variations == 4 && win.kmerDistance == 22 + 0 + 1 + 6);
+
+    // the parallel FASTA parser against the serial line rules, on inputs
+    // with '>' inside lines, empty lines, '\r', records without sequence
+    // and no final newline, cut into tasks of 1..13 bytes
+    const char fa[] = "ACGTN\r\n>x y\n>";
+    for (int trial = 0; trial < 400; ++trial)
+    {
+        std::string in = ">";
+        const size_t len = rng() % 200;
+        for (size_t i = 0; i < len; ++i)
+            in.push_back(fa[rng() % (sizeof(fa) - 1)]);
+        if (rng() % 2)
+            in.push_back('\n');
+        const ParseResult a = seqio_detail::parseSerial(in.data(), in.size(), false);
+        const size_t piece = 1 + rng() % 13;
+        const ParseResult b = seqio_detail::parseFastaParallel(in.data(), in.size(),
+                                                                1 + (unsigned)(rng() % 4), piece);
+        CHECK(a.ids == b.ids && a.sequences == b.sequences);
+    }
     std::printf("%s\n", fails ? "host checks FAILED" : "host checks ok");
     return fails ? 1 : 0;
 }
