@@ -497,9 +497,221 @@ int checkTables(const hipibs_table *const *tables, int nTables)
     return 0;
 }
 
+// ---- coverage text: comma-separated decimal values, as the mapper writes ----
+
+constexpr uint32_t kTextPerThread = 16;
+constexpr uint32_t kTextBlock = 256 * kTextPerThread; // bases per block
+
+__device__ inline uint32_t decimalDigits(uint32_t v)
+{
+    return v >= 100 ? 3u : v >= 10 ? 2u : 1u;
+}
+
+// Text bytes of each block's bases, one separator counted per value.
+__global__ __launch_bounds__(256) void coverageTextLenKernel(const uint8_t *__restrict__ cov,
+                                                             uint64_t n, uint64_t *blockLen)
+{
+    __shared__ uint32_t part[4];
+    const uint64_t base = (uint64_t)blockIdx.x * kTextBlock + threadIdx.x * kTextPerThread;
+    uint32_t len = 0;
+    for (uint32_t i = 0; i < kTextPerThread; ++i)
+        if (base + i < n)
+            len += decimalDigits(cov[base + i]) + 1u;
+    for (int off = 32; off > 0; off >>= 1)
+        len += __shfl_down(len, off, 64);
+    if ((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = len;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        blockLen[blockIdx.x] = (uint64_t)part[0] + part[1] + part[2] + part[3];
+}
+
+// Writes every block's values at blockOff[block]: decimal digits, ',' after
+// every value but the contig's last.
+__global__ __launch_bounds__(256) void coverageTextKernel(const uint8_t *__restrict__ cov,
+                                                          uint64_t n,
+                                                          const uint64_t *__restrict__ blockOff,
+                                                          char *__restrict__ text)
+{
+    __shared__ uint32_t scan[256];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * kTextBlock + tid * kTextPerThread;
+    uint8_t v[kTextPerThread];
+    uint32_t len = 0;
+    for (uint32_t i = 0; i < kTextPerThread; ++i)
+    {
+        v[i] = base + i < n ? cov[base + i] : 0;
+        if (base + i < n)
+            len += decimalDigits(v[i]) + 1u;
+    }
+    scan[tid] = len;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1)
+    {
+        const uint32_t x = tid >= off ? scan[tid - off] : 0u;
+        __syncthreads();
+        scan[tid] += x;
+        __syncthreads();
+    }
+    uint64_t o = blockOff[blockIdx.x] + scan[tid] - len;
+    for (uint32_t i = 0; i < kTextPerThread && base + i < n; ++i)
+    {
+        const uint32_t x = v[i];
+        if (x >= 100)
+            text[o++] = (char)('0' + x / 100);
+        if (x >= 10)
+            text[o++] = (char)('0' + (x / 10) % 10);
+        text[o++] = (char)('0' + x % 10);
+        if (base + i + 1 < n)
+            text[o++] = ',';
+    }
+}
+
+// Lays out the text of d_cov[0..n): exclusive block offsets (device) and
+// the total length. Synchronises s.
+static int coverageTextLayout(const uint8_t *dCov, uint64_t n, hipStream_t s, uint64_t **dOff,
+                              uint64_t *total)
+{
+    const uint64_t nBlocks = (n + kTextBlock - 1) / kTextBlock;
+    std::vector<uint64_t> h(nBlocks);
+    HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(dOff), nBlocks * 8));
+    hipLaunchKernelGGL(coverageTextLenKernel, dim3((uint32_t)nBlocks), dim3(256), 0, s, dCov, n,
+                       *dOff);
+    int rc = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(h.data(), *dOff, nBlocks * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        rc = setError(HIPIBS_E_HIP, "coverage text layout failed");
+    uint64_t acc = 0;
+    for (uint64_t b = 0; b < nBlocks; ++b)
+    {
+        const uint64_t l = h[b];
+        h[b] = acc;
+        acc += l;
+    }
+    if (!rc && (hipMemcpyAsync(*dOff, h.data(), nBlocks * 8, hipMemcpyHostToDevice, s) !=
+                    hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+        rc = setError(HIPIBS_E_HIP, "coverage text layout failed");
+    if (rc)
+    {
+        hipFree(*dOff);
+        *dOff = nullptr;
+        return rc;
+    }
+    *total = acc - 1; // no separator after the last value
+    return 0;
+}
+
+static int launchCoverageText(const uint8_t *dCov, uint64_t n, const uint64_t *dOff, char *dText,
+                              hipStream_t s)
+{
+    const uint64_t nBlocks = (n + kTextBlock - 1) / kTextBlock;
+    hipLaunchKernelGGL(coverageTextKernel, dim3((uint32_t)nBlocks), dim3(256), 0, s, dCov, n, dOff,
+                       dText);
+    HIPIBS_CHECK(hipGetLastError());
+    return 0;
+}
+
 } // namespace hipibs
 
 using namespace hipibs;
+
+extern "C" int hipibs_coverage_text_device(const uint8_t *d_cov, uint64_t n_bases, char *d_text,
+                                           uint64_t text_cap, uint64_t *text_len, void *stream)
+{
+    if (!text_len || (n_bases && !d_cov))
+        return setError(HIPIBS_E_ARG, "null argument");
+    *text_len = 0;
+    if (n_bases == 0)
+        return 0;
+    if (n_bases > (uint64_t)UINT32_MAX * kTextBlock)
+        return setError(HIPIBS_E_ARG, "contig too long for the coverage text kernels");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    uint64_t *dOff = nullptr, total = 0;
+    int rc = coverageTextLayout(d_cov, n_bases, s, &dOff, &total);
+    if (rc)
+        return rc;
+    *text_len = total;
+    if (d_text)
+    {
+        if (total > text_cap)
+            rc = setError(HIPIBS_E_ARG, "coverage text buffer too small (" + std::to_string(total) +
+                                            " bytes needed)");
+        else
+            rc = launchCoverageText(d_cov, n_bases, dOff, d_text, s);
+        if (!rc && hipStreamSynchronize(s) != hipSuccess)
+            rc = setError(HIPIBS_E_HIP, "coverage text failed");
+    }
+    hipFree(dOff);
+    return rc;
+}
+
+extern "C" int hipibs_coverage_text_stream(const uint8_t *d_cov, uint64_t n_bases,
+                                           hipibs_sink_fn sink, void *user, void *stream)
+{
+    if (!sink || (n_bases && !d_cov))
+        return setError(HIPIBS_E_ARG, "null argument");
+    if (n_bases == 0)
+        return 0;
+    if (n_bases > (uint64_t)UINT32_MAX * kTextBlock)
+        return setError(HIPIBS_E_ARG, "contig too long for the coverage text kernels");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    uint64_t *dOff = nullptr, total = 0;
+    int rc = coverageTextLayout(d_cov, n_bases, s, &dOff, &total);
+    if (rc)
+        return rc;
+    char *dText = nullptr, *pinned[2] = {nullptr, nullptr};
+    hipEvent_t ready[2] = {nullptr, nullptr};
+    const uint64_t chunk = std::min<uint64_t>(total, 64ull << 20);
+    if (hipMalloc(reinterpret_cast<void **>(&dText), total) != hipSuccess)
+        rc = setError(HIPIBS_E_NOMEM, "coverage text allocation failed");
+    for (int i = 0; i < 2 && !rc; ++i)
+        if (hipHostMalloc(reinterpret_cast<void **>(&pinned[i]), chunk) != hipSuccess ||
+            hipEventCreateWithFlags(&ready[i], hipEventDisableTiming) != hipSuccess)
+            rc = setError(HIPIBS_E_NOMEM, "staging allocation failed");
+    if (!rc)
+        rc = launchCoverageText(d_cov, n_bases, dOff, dText, s);
+    // chunk i is copied while the sink consumes chunk i-1
+    const uint64_t nChunks = rc ? 0 : (total + chunk - 1) / chunk;
+    for (uint64_t c = 0; c <= nChunks && !rc; ++c)
+    {
+        if (c < nChunks)
+        {
+            const uint64_t o = c * chunk, m = std::min(chunk, total - o);
+            if (hipMemcpyAsync(pinned[c & 1], dText + o, m, hipMemcpyDeviceToHost, s) !=
+                    hipSuccess ||
+                hipEventRecord(ready[c & 1], s) != hipSuccess)
+            {
+                rc = setError(HIPIBS_E_HIP, "coverage text download failed");
+                break;
+            }
+        }
+        if (c > 0)
+        {
+            const uint64_t p = c - 1, o = p * chunk, m = std::min(chunk, total - o);
+            if (hipEventSynchronize(ready[p & 1]) != hipSuccess)
+            {
+                rc = setError(HIPIBS_E_HIP, "coverage text download failed");
+                break;
+            }
+            if (sink(user, pinned[p & 1], m) != 0)
+                rc = setError(HIPIBS_E_IO, "coverage text sink failed");
+        }
+    }
+    hipStreamSynchronize(s);
+    for (int i = 0; i < 2; ++i)
+    {
+        if (pinned[i])
+            hipHostFree(pinned[i]);
+        if (ready[i])
+            hipEventDestroy(ready[i]);
+    }
+    if (dText)
+        hipFree(dText);
+    hipFree(dOff);
+    return rc;
+}
 
 extern "C" uint64_t hipibs_scan_work_bytes(uint32_t n_windows, int n_tables,
                                            uint32_t max_window_bases, uint32_t k,

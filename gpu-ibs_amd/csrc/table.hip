@@ -90,6 +90,55 @@ __global__ void fillKernel(uint64_t *p, uint64_t n, uint64_t v)
         p[i] = v;
 }
 
+// Last bucket j in [lo, hi] with lut[j] <= r (lut is non-decreasing and
+// lut[lo] <= r).
+__device__ inline uint64_t lutBucketOf(const uint64_t *lut, uint64_t lo, uint64_t hi, uint64_t r)
+{
+    while (lo < hi)
+    {
+        const uint64_t mid = lo + (hi - lo + 1) / 2;
+        if (lut[mid] <= r)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Keys of the KMC records [r0, r0+m) from their stored bytes (raw[0] is
+// record r0): record r of bucket j (lut[j] <= r < lut[j+1]) holds the key
+// ((j & prefixMask) << shift) | (its first sufBytes bytes, big-endian);
+// the counter bytes that follow are skipped. j0..j1 are the buckets the
+// records span. A thread decodes 8 consecutive records.
+constexpr uint32_t kDecodePerThread = 8;
+__global__ __launch_bounds__(256) void kmcDecodeKernel(const uint8_t *__restrict__ raw,
+                                                       uint64_t r0, uint64_t m, uint32_t recBytes,
+                                                       uint32_t sufBytes,
+                                                       const uint64_t *__restrict__ lut,
+                                                       uint64_t j0, uint64_t j1,
+                                                       uint64_t prefixMask, uint32_t shift,
+                                                       uint64_t *__restrict__ keys)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * kDecodePerThread;
+    for (uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kDecodePerThread;
+         i0 < m; i0 += stride)
+    {
+        uint64_t j = lutBucketOf(lut, j0, j1, r0 + i0);
+        const uint64_t iEnd = i0 + kDecodePerThread < m ? i0 + kDecodePerThread : m;
+        for (uint64_t i = i0; i < iEnd; ++i)
+        {
+            const uint64_t r = r0 + i;
+            if (j < j1 && lut[j + 1] <= r)
+                j = lutBucketOf(lut, j + 1, j1, r);
+            const uint8_t *q = raw + i * recBytes;
+            uint64_t v = 0;
+            for (uint32_t b = 0; b < sufBytes; ++b)
+                v = (v << 8) | q[b];
+            keys[i] = (shift < 64 ? (j & prefixMask) << shift : 0ull) | v;
+        }
+    }
+}
+
 // 64 queries per wave per step; hits (nullable) gets one byte per query,
 // count (nullable) the number of hits. Grid-stride over 64-query chunks.
 template <int S>
@@ -325,6 +374,110 @@ int hipibsBuildFromHostChunks(hipibs_table *t, const uint64_t *hKeys, uint64_t n
     return rc;
 }
 
+// Streams the records of a KMC database through two pinned staging buffers:
+// the host only reads raw records (parallel preads, 4-9 B per key instead of
+// 8 B keys over PCIe); the device decodes them with the LUT and hands each
+// chunk of keys to `consume` (enqueued on s). The read of chunk i+1 overlaps
+// the upload, decode and consumption of chunk i. Does not synchronise s at
+// the end.
+template <class Consume>
+static int streamKmcKeys(const KmcRecords &src, int device, unsigned threads, hipStream_t s,
+                         Consume consume)
+{
+    const uint64_t total = src.total;
+    if (total == 0)
+        return 0;
+    const uint64_t nBuckets = src.lut.size() - 1;
+    const uint64_t chunk = std::min<uint64_t>(total, 32ull << 20);
+    const uint64_t rawBytes = chunk * src.recBytes;
+    const uint32_t p = src.info.lut_prefix_length, k = src.info.kmer_length;
+    const uint64_t prefixMask = (1ull << (2 * p)) - 1ull;
+    const uint32_t shift = 2 * (k - p);
+    uint8_t *pinned[2] = {nullptr, nullptr}, *raw[2] = {nullptr, nullptr};
+    uint64_t *dKeys = nullptr, *dLut = nullptr;
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    int rc = 0;
+    for (int i = 0; i < 2 && !rc; ++i)
+        if (hipHostMalloc(reinterpret_cast<void **>(&pinned[i]), rawBytes) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void **>(&raw[i]), rawBytes) != hipSuccess ||
+            hipEventCreateWithFlags(&copied[i], hipEventDisableTiming) != hipSuccess)
+            rc = setError(HIPIBS_E_NOMEM, "staging allocation failed");
+    if (!rc && (hipMalloc(reinterpret_cast<void **>(&dKeys), chunk * 8) != hipSuccess ||
+                hipMalloc(reinterpret_cast<void **>(&dLut), src.lut.size() * 8) != hipSuccess))
+        rc = setError(HIPIBS_E_NOMEM, "device allocation for the KMC decode failed");
+    if (!rc && hipMemcpyAsync(dLut, src.lut.data(), src.lut.size() * 8, hipMemcpyHostToDevice,
+                              s) != hipSuccess)
+        rc = setError(HIPIBS_E_HIP, "LUT upload failed");
+    auto bucketOfRecord = [&](uint64_t r)
+    {
+        const uint64_t u = (uint64_t)(std::upper_bound(src.lut.begin(), src.lut.begin() + nBuckets,
+                                                       r) -
+                                      src.lut.begin());
+        return u ? u - 1 : 0;
+    };
+    bool used[2] = {false, false};
+    for (uint64_t r0 = 0, it = 0; r0 < total && !rc; r0 += chunk, ++it)
+    {
+        const int b = (int)(it & 1);
+        const uint64_t m = std::min(chunk, total - r0);
+        if (used[b] && hipEventSynchronize(copied[b]) != hipSuccess)
+        {
+            rc = setError(HIPIBS_E_HIP, "event wait failed");
+            break;
+        }
+        rc = kmcReadRecords(src, r0, m, pinned[b], threads);
+        if (rc)
+            break;
+        if (hipMemcpyAsync(raw[b], pinned[b], m * src.recBytes, hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipEventRecord(copied[b], s) != hipSuccess)
+        {
+            rc = setError(HIPIBS_E_HIP, "record upload failed");
+            break;
+        }
+        used[b] = true;
+        const uint64_t j0 = bucketOfRecord(r0), j1 = bucketOfRecord(r0 + m - 1);
+        const int grid = gridFor((m + kDecodePerThread - 1) / kDecodePerThread, 256, device);
+        hipLaunchKernelGGL(kmcDecodeKernel, dim3(grid), dim3(256), 0, s, raw[b], r0, m,
+                           src.recBytes, src.sufBytes, dLut, j0, j1, prefixMask, shift, dKeys);
+        if (hipGetLastError() != hipSuccess)
+        {
+            rc = setError(HIPIBS_E_HIP, "KMC decode launch failed");
+            break;
+        }
+        rc = consume(dKeys, m);
+    }
+    // the staging buffers may only go once the stream has used them
+    if (hipStreamSynchronize(s) != hipSuccess && !rc)
+        rc = setError(HIPIBS_E_HIP, "KMC streaming failed on the device");
+    for (int i = 0; i < 2; ++i)
+    {
+        if (pinned[i])
+            hipHostFree(pinned[i]);
+        if (raw[i])
+            hipFree(raw[i]);
+        if (copied[i])
+            hipEventDestroy(copied[i]);
+    }
+    if (dKeys)
+        hipFree(dKeys);
+    if (dLut)
+        hipFree(dLut);
+    return rc;
+}
+
+static int hipibsBuildFromKmc(hipibs_table *t, const KmcRecords &src, unsigned threads)
+{
+    hipStream_t s;
+    HIPIBS_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int rc = streamKmcKeys(src, t->device, threads, s, [&](const uint64_t *dKeys, uint64_t m)
+                           { return launchTableInsert(t, dKeys, m, s); });
+    if (!rc)
+        rc = finishTable(t, s);
+    hipStreamDestroy(s);
+    return rc;
+}
+
 extern "C" int hipibs_table_build_keys(const uint64_t *h_keys, uint64_t n, uint32_t k,
                                        const hipibs_table_opts *opts, hipibs_table **out)
 {
@@ -484,32 +637,18 @@ extern "C" int hipibs_table_build_kmc(const char *prefix, const hipibs_table_opt
     if (!prefix || !out)
         return setError(HIPIBS_E_ARG, "null argument");
     *out = nullptr;
-    hipibs_kmc_info info{};
-    int rc = hipibs_kmc_header(prefix, &info);
+    KmcRecords src;
+    int rc = kmcOpenRecords(prefix, src);
     if (rc)
         return rc;
-    if (info.kmer_length == 0 || info.kmer_length > 32)
-        return setError(HIPIBS_E_ARG, "KMC database k must be in 1..32");
-    std::vector<uint64_t> keys;
-    try
-    {
-        keys.resize(info.total_kmers);
-    }
-    catch (const std::exception &)
-    {
-        return setError(HIPIBS_E_NOMEM, "host allocation of the key array failed");
-    }
-    uint64_t n = 0;
-    rc = kmcReadKeys(prefix, keys.data(), keys.size(), &n, opts ? opts->load_threads : 0, nullptr);
-    if (rc)
-        return rc;
+    const uint64_t n = src.total;
     hipibs_table *t = nullptr;
-    rc = allocTable(n, info.kmer_length, opts, &t, nullptr);
+    rc = allocTable(n, src.info.kmer_length, opts, &t, nullptr);
     if (rc)
         return rc;
     rc = hipStreamSynchronize(nullptr) == hipSuccess ? 0 : setError(HIPIBS_E_HIP, "table clear failed");
     if (!rc)
-        rc = hipibsBuildFromHostChunks(t, keys.data(), n);
+        rc = hipibsBuildFromKmc(t, src, opts ? opts->load_threads : 0);
     if (!rc && t->nKeys != n)
         rc = setError(HIPIBS_E_IO, "KMC database holds duplicate k-mers (" +
                                        std::to_string(n) + " records, " +
@@ -521,6 +660,48 @@ extern "C" int hipibs_table_build_kmc(const char *prefix, const hipibs_table_opt
     }
     *out = t;
     return 0;
+}
+
+// Keys of a KMC database found in the table, the records streamed and
+// decoded on the device (no host key array): the counting half of
+// KDBIntersect.
+extern "C" int hipibs_table_probe_count_kmc(const hipibs_table *t, const char *prefix,
+                                            unsigned load_threads, uint64_t *out)
+{
+    if (!t || !prefix || !out)
+        return setError(HIPIBS_E_ARG, "null argument");
+    *out = 0;
+    KmcRecords src;
+    int rc = kmcOpenRecords(prefix, src);
+    if (rc)
+        return rc;
+    if (src.info.kmer_length != t->k)
+        return setError(HIPIBS_E_ARG, "the KMC database and the table have different k");
+    HIPIBS_CHECK(hipSetDevice(t->device));
+    hipStream_t s;
+    HIPIBS_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    uint64_t *dCount = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&dCount), 8) != hipSuccess)
+        rc = setError(HIPIBS_E_NOMEM, "count allocation failed");
+    if (!rc)
+    {
+        hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, s, dCount, (uint64_t)1, 0ull);
+        if (hipGetLastError() != hipSuccess)
+            rc = setError(HIPIBS_E_HIP, "count clear failed");
+    }
+    if (!rc)
+        rc = streamKmcKeys(src, t->device, load_threads, s, [&](const uint64_t *dKeys, uint64_t m)
+                           { return launchProbe(t, dKeys, nullptr, m, nullptr, dCount, s); });
+    uint64_t c = 0;
+    if (!rc && (hipMemcpyAsync(&c, dCount, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+        rc = setError(HIPIBS_E_HIP, "count readback failed");
+    if (dCount)
+        hipFree(dCount);
+    hipStreamDestroy(s);
+    if (!rc)
+        *out = c;
+    return rc;
 }
 
 extern "C" int hipibs_table_minimizer_len(const hipibs_table *t, uint32_t *m)

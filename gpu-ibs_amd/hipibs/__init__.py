@@ -100,6 +100,9 @@ _SIGS = [
     ("hipibs_table_probe", C.c_int, [vp, vp, vp, C.c_uint64, vp, vp]),
     ("hipibs_table_probe_count", C.c_int, [vp, u64p, C.c_uint64, u64p]),
     ("hipibs_table_probe_count_device", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
+    ("hipibs_table_probe_count_kmc", C.c_int, [vp, C.c_char_p, C.c_uint, u64p]),
+    ("hipibs_coverage_text_device", C.c_int, [vp, C.c_uint64, vp, C.c_uint64, u64p, vp]),
+    ("hipibs_coverage_text_stream", C.c_int, [vp, C.c_uint64, vp, vp, vp]),
     ("hipibs_scan_windows_device", C.c_int, [C.POINTER(vp), C.c_int, vp, vp, C.c_uint64, vp,
                                              C.c_uint32, C.c_uint32, vp, C.c_uint32, vp,
                                              C.c_uint64, vp]),
@@ -439,6 +442,14 @@ class Table:
         out = C.c_uint64(0)
         _check(_lib.hipibs_table_probe_count(self.handle, _ptr(keys), keys.size, C.byref(out)),
                "hipibs_table_probe_count")
+        return out.value
+
+    def probe_count_kmc(self, prefix: str, threads: int = 0) -> int:
+        """Keys of a KMC database present in the table (records decoded on
+        the device)."""
+        out = C.c_uint64(0)
+        _check(_lib.hipibs_table_probe_count_kmc(self.handle, prefix.encode(), threads,
+                                                 C.byref(out)), "hipibs_table_probe_count_kmc")
         return out.value
 
     def free(self) -> None:

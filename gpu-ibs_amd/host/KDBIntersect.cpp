@@ -1,8 +1,8 @@
 // KDBIntersect: number of k-mers shared by two KMC databases (the
 // reference's src/KDBIntersect.cpp). The database with more k-mers (ties:
 // the larger .kmc_pre, then the first argument) becomes the device table;
-// the other is read on host threads and probed in pinned double-buffered
-// chunks. stdout carries the count only.
+// the other's records are read raw on host threads and decoded + probed on
+// the device in pinned double-buffered chunks. stdout carries the count only.
 #include "CliInput.hpp"
 
 #include <filesystem>
@@ -51,15 +51,15 @@ static int run(int argc, char **argv)
     KmerDatabase db(p[big]);
     const auto t0 = chrono::steady_clock::now();
     db.initGpuIndex();
-    vector<uint64_t> keys(h[small].total_kmers);
-    uint64_t n = 0;
-    hipibs::check(hipibs_kmc_read_keys(p[small].c_str(), keys.data(), keys.size(), &n,
-                                       KmerDatabase::gpuOptions().loadThreads),
-                  "reading the smaller database");
+    const auto t1 = chrono::steady_clock::now();
     uint64_t count = 0;
-    hipibs::check(hipibs_table_probe_count(db.table(), keys.data(), n, &count), "probe count");
-    cerr << "[gpu] intersect: " << n << " queries in "
-         << chrono::duration<double>(chrono::steady_clock::now() - t0).count() << " s" << endl;
+    hipibs::check(hipibs_table_probe_count_kmc(db.table(), p[small].c_str(),
+                                               KmerDatabase::gpuOptions().loadThreads, &count),
+                  "counting the smaller database's k-mers");
+    const auto t2 = chrono::steady_clock::now();
+    cerr << "[gpu] intersect: table " << chrono::duration<double>(t1 - t0).count() << " s, "
+         << h[small].total_kmers << " queries streamed in "
+         << chrono::duration<double>(t2 - t1).count() << " s" << endl;
     fprintf(stdout, "%llu\n", (unsigned long long)count);
     fflush(stdout);
     return 0;

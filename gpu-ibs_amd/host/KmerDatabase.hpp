@@ -275,6 +275,10 @@ public:
 
     // Per-base coverage of every contig (fastibsmapper): device hit mask
     // and k-window popcount, serialised on the host.
+    // Per-base coverage of every contig (produceMappingGpu,
+    // src/KmerDatabase.hpp:1544-1581): the contig is packed on host threads,
+    // its hit mask, coverage and coverage text are made on the device, and
+    // the text streams from pinned buffers into the output file.
     void produceMapping(std::string refPath, std::string outPath)
     {
         using clk = std::chrono::steady_clock;
@@ -285,56 +289,66 @@ public:
         ParseResult r = readSequenceFile(refPath, refPath.find(".gz") != std::string::npos);
         const auto t0 = clk::now();
         const int dev = gpuOptions().device;
-        std::vector<std::pair<const std::string *, std::string>> lines(r.sequences.size());
+        const unsigned threads = std::min(64u, seqio_detail::hostThreads());
         std::vector<uint64_t> sw, ew;
-        std::vector<uint8_t> cov;
-        double probeSec = 0, serSec = 0;
-        for (size_t c = 0; c < r.sequences.size(); ++c)
-        {
-            const std::string &s = r.sequences[c];
-            const uint64_t n = s.size();
-            cov.assign(n, 0);
-            const auto tp = clk::now();
-            if (n >= kmerSize)
+        double packSec = 0, devSec = 0, textSec = 0;
+        std::cout << "Writing mapping to file" << std::endl;
+        writeOutputFile(outPath, [&](std::ofstream &f) {
+            for (size_t c = 0; c < r.sequences.size(); ++c)
             {
+                const std::string &s = r.sequences[c];
+                const uint64_t n = s.size();
+                const auto tp = clk::now();
                 const uint64_t nsw = (n + 31) / 32 + HIPIBS_PACK_PAD_WORDS;
                 const uint64_t new_ = (n + 63) / 64 + HIPIBS_PACK_PAD_WORDS;
                 sw.assign(nsw, 0);
                 ew.assign(new_, 0);
-                hipibs::check(hipibs_pack_bases(s.data(), n, 0, sw.data(), ew.data()), "pack");
-                const uint64_t npos = n - kmerSize + 1;
-                hipibs::DeviceBuffer dsw(dev, nsw * 8), dew(dev, new_ * 8),
-                    dhit(dev, ((npos + 31) / 32 + 1) * 4), dcov(dev, n);
-                hipibs::check(hipibs_memcpy(dsw.get(), sw.data(), nsw * 8, 1), "upload");
-                hipibs::check(hipibs_memcpy(dew.get(), ew.data(), new_ * 8, 1), "upload");
-                hipibs::check(hipibs_hitmask_device(table_.get(), dsw.as<uint64_t>(),
-                                                    dew.as<uint64_t>(), 0, npos, n,
-                                                    dhit.as<uint32_t>(), nullptr),
-                              "hit mask");
-                hipibs::check(hipibs_coverage_device(dhit.as<uint32_t>(), npos, n, kmerSize,
-                                                     dcov.as<uint8_t>(), nullptr),
-                              "coverage");
-                hipibs::check(hipibs_memcpy(cov.data(), dcov.get(), n, 2), "download");
-            }
-            const auto ts = clk::now();
-            probeSec += std::chrono::duration<double>(ts - tp).count();
-            lines[c] = {&r.ids[c], serializeCoverage(cov.data(), n)};
-            serSec += std::chrono::duration<double>(clk::now() - ts).count();
-        }
-        std::cout << "Writing mapping to file" << std::endl;
-        const auto tw = clk::now();
-        writeOutputFile(outPath, [&lines](std::ofstream &f) {
-            for (auto &l : lines)
-            {
-                f << *l.first << '\n';
-                f.write(l.second.data(), (std::streamsize)l.second.size());
+                // 64-base-aligned pieces write disjoint words
+                const uint64_t piece = 1ull << 22;
+                seqio_detail::parallelFor((n + piece - 1) / piece, threads, [&](size_t i) {
+                    const uint64_t lo = i * piece, hi = std::min<uint64_t>(n, lo + piece);
+                    hipibs::check(hipibs_pack_bases(s.data() + lo, hi - lo, lo, sw.data(),
+                                                    ew.data()),
+                                  "pack");
+                });
+                const auto td = clk::now();
+                const uint64_t npos = n >= kmerSize ? n - kmerSize + 1 : 0;
+                hipibs::DeviceBuffer dcov(dev, n);
+                {
+                    hipibs::DeviceBuffer dsw(dev, nsw * 8), dew(dev, new_ * 8),
+                        dhit(dev, ((npos + 31) / 32 + 1) * 4);
+                    hipibs::check(hipibs_memcpy(dsw.get(), sw.data(), nsw * 8, 1), "upload");
+                    hipibs::check(hipibs_memcpy(dew.get(), ew.data(), new_ * 8, 1), "upload");
+                    if (npos)
+                        hipibs::check(hipibs_hitmask_device(table_.get(), dsw.as<uint64_t>(),
+                                                            dew.as<uint64_t>(), 0, npos, n,
+                                                            dhit.as<uint32_t>(), nullptr),
+                                      "hit mask");
+                    hipibs::check(hipibs_coverage_device(dhit.as<uint32_t>(), npos, n, kmerSize,
+                                                         dcov.as<uint8_t>(), nullptr),
+                                  "coverage");
+                    hipibs::check(hipibs_dev_synchronize(dev), "coverage");
+                }
+                const auto tt = clk::now();
+                f << r.ids[c] << '\n';
+                hipibs::check(hipibs_coverage_text_stream(
+                                  dcov.as<uint8_t>(), n,
+                                  [](void *u, const char *d, uint64_t m) -> int {
+                                      auto &o = *static_cast<std::ofstream *>(u);
+                                      o.write(d, (std::streamsize)m);
+                                      return o ? 0 : 1;
+                                  },
+                                  &f, nullptr),
+                              "coverage text");
                 f << '\n';
+                packSec += std::chrono::duration<double>(td - tp).count();
+                devSec += std::chrono::duration<double>(tt - td).count();
+                textSec += std::chrono::duration<double>(clk::now() - tt).count();
             }
         });
-        std::cout << "Mapping phases: probe " << probeSec << " s, serialize " << serSec
-                  << " s, write " << std::chrono::duration<double>(clk::now() - tw).count()
-                  << " s (total " << std::chrono::duration<double>(clk::now() - t0).count()
-                  << " s)" << std::endl;
+        std::cout << "Mapping phases: pack " << packSec << " s, device " << devSec
+                  << " s, text + write " << textSec << " s (total "
+                  << std::chrono::duration<double>(clk::now() - t0).count() << " s)" << std::endl;
     }
 
     uint32_t kmerSize = 0;

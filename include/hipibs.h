@@ -192,6 +192,15 @@ int hipibs_table_probe_count(const hipibs_table *t, const uint64_t *h_keys, uint
 int hipibs_table_probe_count_device(const hipibs_table *t, const uint64_t *d_keys, uint64_t n,
                                     uint64_t *d_count, void *stream);
 
+/* Replaces the counting half of KDBIntersect::gpuIntersect, enumerateAndCount
+ * + probeCountHost (src/KDBIntersect.cpp:133-238, src/gpu/ParallelEnum.hpp:181-362):
+ * number of the KMC database's k-mers present in the table. The records are
+ * read raw on up to load_threads host threads (0 = all usable CPUs) and
+ * decoded on the device; no host key array. The database's k must equal the
+ * table's. Synchronous. */
+int hipibs_table_probe_count_kmc(const hipibs_table *t, const char *kmc_prefix,
+                                 unsigned load_threads, uint64_t *out);
+
 /* ---- the fused scan (device-resident inputs) ------------------------------ */
 
 /* Fused encode + canonicalise + probe + per-window run-length reduction over
@@ -288,6 +297,27 @@ int hipibs_encode_device(const uint64_t *d_seq_words, const uint64_t *d_exc_word
  * p <= i <= p+k-1, i < n_bases (src/KmerDatabase.hpp:756-802). Asynchronous. */
 int hipibs_coverage_device(const uint32_t *d_hit_words, uint64_t n_pos, uint64_t n_bases,
                            uint32_t k, uint8_t *d_cov, void *stream);
+
+/* The mapper's coverage line of one contig, written on the device: the
+ * decimal values of d_cov[0..n_bases) joined by ',' (no newline), as
+ * KmerDatabase::serializeCoverage formats them with to_chars
+ * (src/KmerDatabase.hpp:756-772). *text_len receives the length; with
+ * d_text NULL only the length is computed, else the text is written when it
+ * fits text_cap (at most 4 * n_bases bytes). Synchronous on stream. */
+int hipibs_coverage_text_device(const uint8_t *d_cov, uint64_t n_bases, char *d_text,
+                                uint64_t text_cap, uint64_t *text_len, void *stream);
+
+/* Receives consecutive pieces of a text; returns 0 to continue, non-zero to
+ * stop with an error. */
+typedef int (*hipibs_sink_fn)(void *user, const char *data, uint64_t n);
+
+/* The same text handed to `sink` in order, in pieces of at most 64 MiB
+ * copied through pinned double buffers (the copy of piece i+1 overlaps the
+ * sink's work on piece i): the device half of the mapper's serialise +
+ * write (src/KmerDatabase.hpp:756-772, src/FastIBSMapper.cpp:20-171).
+ * Synchronous on stream. */
+int hipibs_coverage_text_stream(const uint8_t *d_cov, uint64_t n_bases, hipibs_sink_fn sink,
+                                void *user, void *stream);
 
 /* Timing of the last scan kernel launched by this thread, in ms, measured
  * with HIP events on the launch stream (valid after the stream completes;

@@ -150,6 +150,36 @@ def test_coverage_matches_golden(small_corpus, expected):
     t.free()
 
 
+@pytest.mark.parametrize("n", [1, 2, 15, 16, 17, 4095, 4096, 4097, 12288, 100003])
+def test_coverage_text_matches_the_serialisation(n):
+    """Device text of a coverage line == the values joined by ',' (1-3
+    digits, no separator after the last), through both text entry points."""
+    import ctypes as C
+    rng = np.random.default_rng(n)
+    cov = rng.integers(0, 33, n).astype(np.uint8)
+    big = rng.random(n) < 0.05
+    cov[big] = rng.integers(100, 256, int(big.sum()))
+    want = coverage_line(cov)
+    d_cov = D.from_array(cov)
+    ln = C.c_uint64(0)
+    hipibs._check(hipibs._lib.hipibs_coverage_text_device(d_cov.ptr, n, None, 0, C.byref(ln), None),
+                  "text length")
+    assert ln.value == len(want)
+    d_txt = D.zeros(4 * n)
+    hipibs._check(hipibs._lib.hipibs_coverage_text_device(d_cov.ptr, n, d_txt.ptr, 4 * n,
+                                                          C.byref(ln), None), "text")
+    assert d_txt.to_array(np.uint8, ln.value).tobytes() == want
+    small = D.zeros(max(1, len(want) - 1))
+    assert hipibs._lib.hipibs_coverage_text_device(d_cov.ptr, n, small.ptr, len(want) - 1,
+                                                   C.byref(ln), None) != 0
+    parts = []
+    SINK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_char), C.c_uint64)
+    sink = SINK(lambda u, d, m: parts.append(C.string_at(d, m)) or 0)
+    hipibs._check(hipibs._lib.hipibs_coverage_text_stream(d_cov.ptr, n, sink, None, None),
+                  "text stream")
+    assert b"".join(parts) == want
+
+
 def test_encode_device_matches_oracle():
     rng = np.random.default_rng(11)
     for k in (15, 25, 31, 32):
@@ -232,6 +262,57 @@ def test_build_from_kmc_equals_from_keys(small_dbs, small_corpus):
     assert a.info()["n_keys"] == keys.size
     assert a.probe_count(samples[0][0]) == int(np.isin(samples[0][0], samples[1][0]).sum())
     a.free()
+
+
+@pytest.mark.parametrize("variant", [(25, 9, 9, 1, 0x200), (25, 5, 5, 2, 0x200),
+                                     (25, 9, 11, 4, 0x200), (25, 9, 0, 1, 0), (25, 1, 7, 0, 0x200),
+                                     (32, 8, 9, 1, 0x200), (32, 0, 9, 1, 0x200), (31, 7, 9, 3, 0),
+                                     (15, 3, 5, 1, 0x200)])
+def test_build_from_kmc_decodes_every_layout(tmp_path, variant):
+    """Table builds decode KMC records on the device (LUT bucket search +
+    big-endian suffix): every layout gives exactly the written key set."""
+    k, p, s, c, v = variant
+    note(f"kmc layout k={k} p={p} sig={s} counter={c} version={v:#x}")
+    _, keys = corpus.edge_config(k)
+    counts = (np.arange(keys.size) % 300 + 1).astype(np.uint32)
+    pre = str(tmp_path / "db")
+    hipibs.kmc_write(pre, keys, k, counts=counts, p=p, sig_len=max(s, 5), counter_size=c,
+                     version=v, min_count=1, max_count=2 ** 31)
+    t = hipibs.Table.from_kmc(pre, load_threads=3)
+    assert t.info()["n_keys"] == keys.size
+    assert t.probe_count(keys) == keys.size
+    other = np.unique(corpus.splitmix(k, 77, 20000) & np.uint64((1 << (2 * k)) - 1 if k < 32
+                                                              else 0xFFFFFFFFFFFFFFFF))
+    other = np.unique(np.minimum(other, corpus.revcomp(other, k)))
+    other = np.unique(np.concatenate([other, keys[::5]]))
+    want = int(np.isin(other, keys).sum())
+    assert t.probe_count(other) == want
+    # the intersect's streamed count: records decoded on the device
+    assert t.probe_count_kmc(pre, threads=2) == keys.size
+    pre2 = str(tmp_path / "other")
+    hipibs.kmc_write(pre2, other, k, p=p, sig_len=max(s, 5), counter_size=c, version=v,
+                     min_count=1, max_count=2 ** 31)
+    assert t.probe_count_kmc(pre2) == want
+    t.free()
+
+
+def test_build_from_kmc_across_chunks(tmp_path):
+    """More records than one staging chunk (32M): the decode of every chunk
+    starts at its own record and bucket range."""
+    k = 31
+    keys = np.unique(corpus.splitmix(31, 5, 40_000_000) & np.uint64((1 << 62) - 1))
+    keys = np.unique(np.minimum(keys, corpus.revcomp(keys, k)))
+    note(f"kmc across chunks: {keys.size} keys")
+    pre = str(tmp_path / "big")
+    hipibs.kmc_write(pre, keys, k, p=7, sig_len=9, counter_size=1, version=0x200,
+                     min_count=1, max_count=255)
+    t = hipibs.Table.from_kmc(pre)
+    assert t.info()["n_keys"] == keys.size
+    assert t.probe_count(keys) == keys.size
+    probe = keys[::97] ^ np.uint64(1)
+    assert t.probe_count(probe) == int(np.isin(probe, keys).sum())
+    assert t.probe_count_kmc(pre) == keys.size
+    t.free()
 
 
 def _packed_windows(seq, W, k):
