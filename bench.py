@@ -69,9 +69,12 @@ def parse():
                     help="large: bases (Mbp) of the reference mapper's CPU sample")
     ap.add_argument("--e2e", type=int, default=1,
                     help="large: also run fastibs / fastibsmapper end to end from files")
-    ap.add_argument("--panel-samples", type=int, default=8)
-    ap.add_argument("--panel-min-keys", type=float, default=0.1e9)
-    ap.add_argument("--panel-max-keys", type=float, default=0.5e9)
+    ap.add_argument("--panel-samples", type=int, default=48)
+    ap.add_argument("--panel-min-keys", type=float, default=0.02e9)
+    ap.add_argument("--panel-max-keys", type=float, default=0.25e9)
+    ap.add_argument("--panel-groups", type=int, default=3,
+                    help="panel: HBM budget = the panel's table bytes / this, so the panel "
+                         "runs in at least this many memory-budgeted groups")
     ap.add_argument("--workdir", default=os.environ.get("TMPDIR", "/tmp"))
     return ap.parse_args()
 
@@ -379,8 +382,11 @@ def run_panel(args):
         log(f"plain run {os.path.basename(d)}: {t:.2f} s")
     with open(os.path.join(root, "list.txt"), "w") as f:
         f.write("\n".join(dbs[1:]) + "\n")
+    table_bytes = sum(int(np.ceil(np.ceil(x / 0.6) / 8)) * 64 for x in sizes)
+    budget_gb = max(1, int(table_bytes / args.panel_groups / (1 << 30)))
     t_panel, out = run([fastibs, dbs[0], refs, os.path.join(root, "out_panel"), str(W),
-                        f"--db-list={os.path.join(root, 'list.txt')}"])
+                        f"--db-list={os.path.join(root, 'list.txt')}",
+                        f"--hbm-budget-gb={budget_gb}"])
     phases = [l for l in out.splitlines() if l.startswith(("Panel", "Autotuned"))]
     log(f"panel run: {t_panel:.2f} s; " + " | ".join(phases))
     same = True
@@ -397,7 +403,9 @@ def run_panel(args):
            "config": {"workload": f"panel, k=25: {n/1e9:.1f} Gbp synthetic reference vs {N} "
                                   f"samples, keys log-uniform {args.panel_min_keys/1e9:.2f}-"
                                   f"{args.panel_max_keys/1e9:.2f} G, SNP 0.1-5%, W=50 kbp; "
-                                  f"end-to-end fastibs runs from FASTA + KMC files",
+                                  f"end-to-end fastibs runs from FASTA + KMC files; panel "
+                                  f"HBM budget {budget_gb} GiB for {table_bytes/2**30:.1f} GiB "
+                                  f"of tables",
                       "sample_keys": [int(x) for x in sizes], "plain_run_s": plain_s,
                       "panel_run_s": t_panel, "panel_phases": phases}}
     print(json.dumps(res), flush=True)
