@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--bucket-slots", type=int, default=0, help="0 = library default")
     ap.add_argument("--minimizer-len", type=int, default=0,
                     help="0 = whole-k-mer hash buckets (default), 5..17 = minimizer buckets")
+    ap.add_argument("--minimizer-cap", type=int, default=0,
+                    help="minimizer tables: keys per minimizer group before it moves to the "
+                         "whole-k-mer part (0 = library default)")
     ap.add_argument("--load-factor", type=float, default=0.6)
     ap.add_argument("--segs", type=int, default=0, help="segments per window (0 = auto)")
     ap.add_argument("--cpu-mbp", type=float, default=256.0,
@@ -248,7 +251,8 @@ def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_c
     # same sample on the GPU: must be identical
     t = hipibs.Table.from_device_keys(keys.data_ptr(), keys.numel(), k,
                                       bucket_slots=args.bucket_slots,
-                                      minimizer_len=args.minimizer_len)
+                                      minimizer_len=args.minimizer_len,
+                                      minimizer_cap=args.minimizer_cap)
     gst = gpu_check(t, n_slice, wins)
     match = bool(np.array_equal(gst, stats))
     log(f"cpu/gpu statistics identical on the sample: {match}")
@@ -695,7 +699,9 @@ def run_large(args):
     tb = time.time()
     table = hipibs.Table.from_device_keys(keys.data_ptr(), keys.numel(), k,
                                           bucket_slots=args.bucket_slots,
-                                          load_factor=args.load_factor)
+                                          load_factor=args.load_factor,
+                                          minimizer_len=args.minimizer_len,
+                                          minimizer_cap=args.minimizer_cap)
     build_s = time.time() - tb
     info = table.info()
     log(f"table: {info['n_keys']/1e9:.3f} G keys, {info['bytes']/1e9:.1f} GB, build {build_s:.2f} s")
@@ -1000,7 +1006,8 @@ def main():
     table = hipibs.Table.from_device_keys(keys.data_ptr(), keys.numel(), K,
                                           bucket_slots=args.bucket_slots,
                                           load_factor=args.load_factor,
-                                          minimizer_len=args.minimizer_len)
+                                          minimizer_len=args.minimizer_len,
+                                          minimizer_cap=args.minimizer_cap)
     build_s = time.time() - tb
     info = table.info()
     n_keys = keys.numel()

@@ -76,7 +76,11 @@ __host__ __device__ inline uint64_t bucketOfMinimizer(uint32_t minHash, uint64_t
     return ((x >> 32) * nBuckets) >> 32;
 }
 
-// Device view of one table, passed by value to kernels.
+// Device view of one table, passed by value to kernels. A minimizer table
+// (mlen > 0) has two parts: minimizer-ordered buckets holding the keys of
+// every minimizer group of at most `cap` keys plus one marker per larger
+// group, and a whole-k-mer table (8-slot buckets) holding the larger
+// groups' keys.
 struct TableView
 {
     const uint64_t *slots;
@@ -84,8 +88,19 @@ struct TableView
     uint32_t maxDisp; // largest bucket displacement of any stored key
     uint32_t mlen;    // minimizer length of the home bucket, 0 = whole k-mer
     uint32_t k;
-    uint32_t pad;
+    uint32_t maxDisp2;
+    const uint64_t *slots2; // the whole-k-mer part of a minimizer table
+    uint64_t nBuckets2;
 };
+
+// The marker of a minimizer group (by its 32-bit minimizer hash) that keeps
+// its keys in the whole-k-mer part: first and last base T, so it is never a
+// canonical code for k = 32 (its reverse complement starts with A) and above
+// every code for k < 32; never EMPTY.
+__host__ __device__ inline uint64_t markerOf(uint32_t minHash)
+{
+    return 0xC000000000000003ull | ((uint64_t)minHash << 2);
+}
 
 __host__ __device__ inline uint64_t homeOf(uint64_t key, const TableView &T)
 {
@@ -216,10 +231,18 @@ struct hipibs_table
     uint32_t maxDisp = 0;
     uint32_t mlen = 0;
     uint64_t *dSlots = nullptr;
-    uint64_t *dCounters = nullptr; // [0] inserted, [1] full flag, [2] max displacement
+    uint64_t *dCounters = nullptr; // [0] inserted, [1] full flag, [2] max displacement,
+                                   // minimizer tables: [3] keys in the whole-k-mer part,
+                                   // [4] its max displacement, [5] markers
+    // minimizer tables only: the whole-k-mer part and the group cap
+    uint64_t *dSlots2 = nullptr;
+    uint64_t nBuckets2 = 0;
+    uint32_t maxDisp2 = 0;
+    uint32_t mcap = 0;
+    uint64_t nKeys2 = 0, nMarkers = 0;
     hipibs::TableView view() const
     {
-        return hipibs::TableView{dSlots, nBuckets, maxDisp, mlen, k, 0};
+        return hipibs::TableView{dSlots, nBuckets, maxDisp, mlen, k, maxDisp2, dSlots2, nBuckets2};
     }
 };
 

@@ -451,8 +451,9 @@ extern "C" int hipibs_panel_autotune(const hipibs_table *const *tables, int n_ta
     if (!best)
         return setError(HIPIBS_E_ARG, "null argument");
     const uint32_t n = std::max<uint32_t>(1, std::min(n_windows, sample_windows ? sample_windows : 4096));
-    const uint32_t wpcs[] = {8, 12, 16, 24, 32};
-    const uint32_t segss[] = {1, 2, 4};
+    // occupancy from 8 to 32 waves per CU; one segment per window or four
+    const uint32_t wpcs[] = {8, 16, 24, 32};
+    const uint32_t segss[] = {1, 4};
     hipEvent_t e0, e1;
     HIPIBS_CHECK(hipSetDevice(tables && n_tables > 0 && tables[0] ? tables[0]->device : 0));
     HIPIBS_CHECK(hipEventCreate(&e0));
@@ -469,7 +470,7 @@ extern "C" int hipibs_panel_autotune(const hipibs_table *const *tables, int n_ta
                 continue;
             hipibs_scan_config c{segs, wpc, {0, 0}};
             double t = 0;
-            for (int rep = 0; rep < 3 && !rc; ++rep)
+            for (int rep = 0; rep < 2 && !rc; ++rep)
             {
                 (void)hipEventRecord(e0, s);
                 rc = hipibs_panel_scan_device(tables, n_tables, d_seq_words, d_exc_words, n_bases,
@@ -497,6 +498,6 @@ extern "C" int hipibs_panel_autotune(const hipibs_table *const *tables, int n_ta
         return rc;
     *best = bestC;
     if (best_ms)
-        *best_ms = bestT / 2;
+        *best_ms = bestT;
     return 0;
 }

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
             const uint64_t rc = ~x & kmask2;
             const uint64_t fwd = reverse2(x) >> fwdShift;
             const uint64_t canon = fwd < rc ? fwd : rc;
-            uint32_t home;
+            uint64_t H;
             if (mlen)
             {
                 // canonical m-mer hashes at positions p and p+64, then the
@@ -119,12 +119,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
                 const int src = (lane + d) & 63;
                 const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
                 const uint32_t minh = min(L, lane + d < 64 ? l2 : h2);
-                home = (uint32_t)bucketOfMinimizer(minh, T.nBuckets);
+                H = waveProbeMinimizer<S>(T, canon, minh, valid);
             }
             else
-                home = (uint32_t)bucketOf(canon, T.nBuckets);
-
-            const uint64_t H = waveProbe<S>(T, canon, home, valid);
+                H = waveProbe<S>(T, canon, (uint32_t)bucketOf(canon, T.nBuckets), valid);
             const uint64_t V = ballot(valid);
             const uint64_t Miss = V & ~H;
             if (H)
@@ -250,8 +248,7 @@ __global__ __launch_bounds__(256) void hitmaskKernel(const TableView T, const ui
             const uint64_t fwd = reverse2(x) >> fwdShift;
             canon = fwd < rc ? fwd : rc;
         }
-        const uint32_t home = valid ? (uint32_t)homeOf(canon, T) : 0u;
-        const uint64_t H = waveProbe<S>(T, canon, home, valid);
+        const uint64_t H = waveLookup<S>(T, canon, valid);
         if (lane < 2)
         {
             const uint64_t o = 2 * c + (uint64_t)lane;

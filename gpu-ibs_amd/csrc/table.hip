@@ -17,18 +17,55 @@
 namespace hipibs
 {
 
-// One key per lane: walk the buckets from home, claiming the first EMPTY
-// slot with a 64-bit CAS; a slot already holding the key ends the insert
-// (duplicates are stored once). Slots of a bucket are claimed in order, so
-// a concurrent insert of the same key always meets the first copy.
-// counters[0] += newly stored keys, counters[1] = 1 when the table ran out
-// of buckets, counters[2] = max displacement (buckets past home).
+// Stores `key` from bucket `home` on: walks the buckets, claiming the first
+// EMPTY slot with a 64-bit CAS; a slot already holding the key ends the
+// insert (duplicates are stored once). Slots of a bucket are claimed in
+// order, so a concurrent insert of the same key always meets the first copy.
+// Returns false when every bucket is full; *isNew / *disp report a new slot
+// and its displacement (buckets past home).
+template <int S>
+__device__ inline bool insertOne(uint64_t *slots, uint64_t nBuckets, uint64_t key, uint64_t home,
+                                 bool *isNew, uint32_t *disp)
+{
+    uint64_t b = home;
+    *isNew = false;
+    for (uint64_t d = 0; d < nBuckets; ++d)
+    {
+        unsigned long long *bucket = reinterpret_cast<unsigned long long *>(slots + b * S);
+        for (int s = 0; s < S; ++s)
+        {
+            unsigned long long cur = __hip_atomic_load(bucket + s, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == key)
+                return true;
+            if (cur == EMPTY)
+            {
+                const unsigned long long prev = atomicCAS(bucket + s, EMPTY, key);
+                if (prev == EMPTY)
+                {
+                    *isNew = true;
+                    *disp = (uint32_t)min<uint64_t>(d, 0xffffffffu);
+                    return true;
+                }
+                if (prev == key)
+                    return true;
+            }
+        }
+        b += 1;
+        if (b == nBuckets)
+            b = 0;
+    }
+    return false;
+}
+
+// One key per thread into a whole-k-mer table. counters[0] += newly stored
+// keys, counters[1] = 1 when the table ran out of buckets, counters[2] =
+// max displacement.
 template <int S>
 __global__ __launch_bounds__(256) void insertKernel(uint64_t *slots, const TableView T,
                                                     const uint64_t *keys, uint64_t n,
                                                     unsigned long long *counters)
 {
-    const uint64_t nBuckets = T.nBuckets;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint32_t localNew = 0, localDisp = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -36,51 +73,187 @@ __global__ __launch_bounds__(256) void insertKernel(uint64_t *slots, const Table
         const uint64_t key = keys[i];
         if (key == EMPTY)
             continue;
-        uint64_t b = homeOf(key, T);
-        bool done = false;
-        for (uint64_t d = 0; d < nBuckets && !done; ++d)
-        {
-            unsigned long long *bucket = reinterpret_cast<unsigned long long *>(slots + b * S);
-            for (int s = 0; s < S; ++s)
-            {
-                unsigned long long cur = __hip_atomic_load(bucket + s, __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_AGENT);
-                if (cur == key)
-                {
-                    done = true;
-                    break;
-                }
-                if (cur == EMPTY)
-                {
-                    const unsigned long long prev = atomicCAS(bucket + s, EMPTY, key);
-                    if (prev == EMPTY)
-                    {
-                        localNew += 1;
-                        localDisp = max(localDisp, (uint32_t)min<uint64_t>(d, 0xffffffffu));
-                        done = true;
-                        break;
-                    }
-                    if (prev == key)
-                    {
-                        done = true;
-                        break;
-                    }
-                }
-            }
-            if (!done)
-            {
-                b += 1;
-                if (b == nBuckets)
-                    b = 0;
-            }
-        }
-        if (!done)
+        bool isNew = false;
+        uint32_t d = 0;
+        if (!insertOne<S>(slots, T.nBuckets, key, homeOf(key, T), &isNew, &d))
             atomicExch(counters + 1, 1ull);
+        else if (isNew)
+        {
+            localNew += 1;
+            localDisp = max(localDisp, d);
+        }
     }
     if (localNew)
         atomicAdd(counters + 0, (unsigned long long)localNew);
     if (localDisp)
         atomicMax(counters + 2, (unsigned long long)localDisp);
+}
+
+// ---- minimizer tables ------------------------------------------------------
+// A group = the keys sharing a 32-bit minimizer hash. The group map holds
+// one 64-bit entry per group: hash in the low 32 bits, key count in bits
+// 32..62, bit 63 set once the group's marker is stored. EMPTY = all ones.
+
+__device__ inline uint64_t groupSlot(uint32_t minHash, uint64_t mask)
+{
+    uint64_t x = (uint64_t)minHash * 0xD6E8FEB86659FD93ull;
+    x ^= x >> 32;
+    return x & mask;
+}
+
+// Counts the keys of every group; *overflow = 1 if the map filled up (the
+// walk of a key is bounded by the map size, so the kernel always ends).
+__global__ __launch_bounds__(256) void groupCountKernel(const uint64_t *keys, uint64_t n,
+                                                        uint32_t k, uint32_t mlen,
+                                                        unsigned long long *map, uint64_t mask,
+                                                        unsigned long long *overflow)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        const uint64_t key = keys[i];
+        if (key == EMPTY)
+            continue;
+        const uint32_t mh = minimizerOf(key, k, mlen);
+        bool counted = false;
+        uint64_t h = groupSlot(mh, mask);
+        for (uint64_t step = 0; step <= mask && !counted; ++step, h = (h + 1) & mask)
+        {
+            unsigned long long cur = __hip_atomic_load(map + h, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == EMPTY)
+            {
+                cur = atomicCAS(map + h, EMPTY, (unsigned long long)mh | (1ull << 32));
+                if (cur == EMPTY)
+                {
+                    counted = true;
+                    break;
+                }
+            }
+            // cur is a claimed entry here
+            if ((uint32_t)cur == mh)
+            {
+                atomicAdd(map + h, 1ull << 32);
+                counted = true;
+            }
+        }
+        if (!counted)
+            atomicExch(overflow, 1ull);
+    }
+}
+
+// tally[0] += keys of groups of at most cap keys, tally[1] += larger
+// groups (one marker each), tally[2] += their keys.
+__global__ __launch_bounds__(256) void groupTallyKernel(const unsigned long long *map,
+                                                        uint64_t size, uint32_t cap,
+                                                        unsigned long long *tally)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long a = 0, g = 0, b = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride)
+    {
+        const unsigned long long e = map[i];
+        if (e == EMPTY)
+            continue;
+        const uint64_t c = (e >> 32) & 0x7fffffffull;
+        if (c <= cap)
+            a += c;
+        else
+        {
+            g += 1;
+            b += c;
+        }
+    }
+    if (a)
+        atomicAdd(tally + 0, a);
+    if (g)
+        atomicAdd(tally + 1, g);
+    if (b)
+        atomicAdd(tally + 2, b);
+}
+
+// Places every key: groups of at most cap keys into the minimizer-ordered
+// part at their group's home bucket; larger groups' keys into the
+// whole-k-mer part, and the first of them stores the group's marker at the
+// group's home. counters: [0] keys in the minimizer part, [1] full flag,
+// [2] its max displacement, [3] keys in the whole-k-mer part, [4] its max
+// displacement, [5] markers.
+template <int S>
+__global__ __launch_bounds__(256) void placeKernel(const uint64_t *keys, uint64_t n, uint32_t k,
+                                                   uint32_t mlen, unsigned long long *map,
+                                                   uint64_t mask, uint32_t cap, uint64_t *slots,
+                                                   uint64_t nBuckets, uint64_t *slots2,
+                                                   uint64_t nBuckets2,
+                                                   unsigned long long *counters)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t nA = 0, dA = 0, nB = 0, dB = 0, nM = 0;
+    bool full = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        const uint64_t key = keys[i];
+        if (key == EMPTY)
+            continue;
+        const uint32_t mh = minimizerOf(key, k, mlen);
+        uint64_t h = groupSlot(mh, mask);
+        unsigned long long e = EMPTY;
+        // every key's group was counted, and entries are never removed: the
+        // walk meets the group before any EMPTY entry
+        for (uint64_t step = 0; step <= mask; ++step, h = (h + 1) & mask)
+        {
+            e = map[h];
+            if (e == EMPTY || (uint32_t)e == mh)
+                break;
+        }
+        if (e == EMPTY || (uint32_t)e != mh)
+        {
+            full = true;
+            continue;
+        }
+        const uint64_t c = (e >> 32) & 0x7fffffffull;
+        bool isNew = false;
+        uint32_t d = 0;
+        if (c <= cap)
+        {
+            full |= !insertOne<S>(slots, nBuckets, key, bucketOfMinimizer(mh, nBuckets), &isNew,
+                                  &d);
+            if (isNew)
+            {
+                nA += 1;
+                dA = max(dA, d);
+            }
+            continue;
+        }
+        full |= !insertOne<8>(slots2, nBuckets2, key, bucketOf(key, nBuckets2), &isNew, &d);
+        if (isNew)
+        {
+            nB += 1;
+            dB = max(dB, d);
+        }
+        const unsigned long long old = atomicOr(map + h, 1ull << 63);
+        if (!(old >> 63))
+        {
+            full |= !insertOne<S>(slots, nBuckets, markerOf(mh), bucketOfMinimizer(mh, nBuckets),
+                                  &isNew, &d);
+            if (isNew)
+            {
+                nM += 1;
+                dA = max(dA, d);
+            }
+        }
+    }
+    if (full)
+        atomicExch(counters + 1, 1ull);
+    if (nA)
+        atomicAdd(counters + 0, (unsigned long long)nA);
+    if (dA)
+        atomicMax(counters + 2, (unsigned long long)dA);
+    if (nB)
+        atomicAdd(counters + 3, (unsigned long long)nB);
+    if (dB)
+        atomicMax(counters + 4, (unsigned long long)dB);
+    if (nM)
+        atomicAdd(counters + 5, (unsigned long long)nM);
 }
 
 __global__ void fillKernel(uint64_t *p, uint64_t n, uint64_t v)
@@ -157,8 +330,7 @@ __global__ __launch_bounds__(256) void probeKernel(const TableView T, const uint
         const bool in = i < n;
         bool v = in && (valid == nullptr || valid[i] != 0);
         const uint64_t code = v ? codes[i] : 0ull;
-        const uint32_t home = v ? (uint32_t)homeOf(code, T) : 0u;
-        const uint64_t H = waveProbe<S>(T, code, home, v);
+        const uint64_t H = waveLookup<S>(T, code, v);
         if (hits && in)
             hits[i] = (uint8_t)((H >> lane) & 1ull);
         local += (uint64_t)__popcll(H);
@@ -291,6 +463,154 @@ static int finishTable(hipibs_table *t, hipStream_t s)
     return 0;
 }
 
+// Default keys per minimizer group kept in the minimizer-ordered part.
+constexpr uint32_t kDefaultMinimizerCap = 32;
+
+static uint64_t pow2AtLeast(uint64_t x)
+{
+    uint64_t p = 1;
+    while (p < x)
+        p <<= 1;
+    return p;
+}
+
+static int allocSlots(uint64_t items, uint32_t slotsPerBucket, double loadFactor, int device,
+                      uint64_t **out, uint64_t *nBuckets, hipStream_t s)
+{
+    const double slotsWanted = std::ceil((double)std::max<uint64_t>(items, 1) / loadFactor);
+    uint64_t nb = std::max<uint64_t>(2, (uint64_t)std::ceil(slotsWanted / slotsPerBucket));
+    if (nb >= (1ull << 32))
+        return setError(HIPIBS_E_ARG, "table too large: more than 2^32 buckets");
+    const uint64_t bytes = nb * slotsPerBucket * 8ull;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(out), bytes);
+    if (e != hipSuccess)
+        return setError(HIPIBS_E_NOMEM, "hipMalloc of " + std::to_string(bytes) +
+                                            " table bytes failed: " + hipGetErrorString(e));
+    hipLaunchKernelGGL(fillKernel, dim3(gridFor(nb * slotsPerBucket, 256, device)), dim3(256), 0,
+                       s, *out, nb * slotsPerBucket, EMPTY);
+    *nBuckets = nb;
+    return 0;
+}
+
+// A minimizer table from n device-resident keys: count the keys of every
+// minimizer group, size both parts from the counts, then place the keys.
+static int buildMinimizerTable(const uint64_t *dKeys, uint64_t n, uint32_t k,
+                               const hipibs_table_opts *opts, hipStream_t s, hipibs_table **out)
+{
+    *out = nullptr;
+    hipibs_table_opts o{};
+    if (opts)
+        o = *opts;
+    if (o.load_factor <= 0)
+        o.load_factor = 0.6;
+    const uint32_t mlen = o.minimizer_len;
+    if (k == 0 || k > 32)
+        return setError(HIPIBS_E_ARG, "k must be in 1..32");
+    if (mlen < 5 || mlen > 17 || mlen > k)
+        return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (off) or 5..min(17, k)");
+    if (o.bucket_slots == 0)
+        o.bucket_slots = 16;
+    if (o.bucket_slots != 8 && o.bucket_slots != 16)
+        return setError(HIPIBS_E_ARG, "bucket_slots must be 8 or 16");
+    if (!(o.load_factor > 0 && o.load_factor < 0.98))
+        return setError(HIPIBS_E_ARG, "load_factor must be in (0, 0.98)");
+    const uint32_t cap = o.minimizer_cap ? o.minimizer_cap : kDefaultMinimizerCap;
+    if (cap > 4096)
+        return setError(HIPIBS_E_ARG, "minimizer_cap must be at most 4096");
+    if (n >= (1ull << 31))
+        return setError(HIPIBS_E_ARG, "minimizer tables hold fewer than 2^31 keys");
+    HIPIBS_CHECK(hipSetDevice(o.device));
+    // group map: at most one group per key, so a map of >= n entries always
+    // has room
+    const uint64_t mapSize = pow2AtLeast(std::max<uint64_t>(1024, n));
+    unsigned long long *map = nullptr, *tally = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&map), mapSize * 8) != hipSuccess)
+        return setError(HIPIBS_E_NOMEM, "hipMalloc of the minimizer group map failed");
+    if (hipMalloc(reinterpret_cast<void **>(&tally), 4 * 8) != hipSuccess)
+    {
+        hipFree(map);
+        return setError(HIPIBS_E_NOMEM, "hipMalloc of the group tally failed");
+    }
+    int dev = o.device;
+    hipLaunchKernelGGL(fillKernel, dim3(gridFor(mapSize, 256, dev)), dim3(256), 0, s,
+                       reinterpret_cast<uint64_t *>(map), mapSize, EMPTY);
+    hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, s, reinterpret_cast<uint64_t *>(tally),
+                       (uint64_t)4, 0ull);
+    if (n)
+        hipLaunchKernelGGL(groupCountKernel, dim3(gridFor(n, 256, dev)), dim3(256), 0, s, dKeys, n,
+                           k, mlen, map, mapSize - 1, tally + 3);
+    hipLaunchKernelGGL(groupTallyKernel, dim3(gridFor(mapSize, 256, dev)), dim3(256), 0, s, map,
+                       mapSize, cap, tally);
+    uint64_t tl[4] = {0, 0, 0, 0};
+    int rc = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(tl, tally, sizeof(tl), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        rc = setError(HIPIBS_E_HIP, "minimizer group count failed");
+    if (!rc && tl[3])
+        rc = setError(HIPIBS_E_HIP, "minimizer group map overflow");
+    hipibs_table *t = nullptr;
+    if (!rc)
+    {
+        t = new hipibs_table();
+        t->device = o.device;
+        t->k = k;
+        t->bucketSlots = o.bucket_slots;
+        t->mlen = mlen;
+        t->mcap = cap;
+        rc = allocSlots(tl[0] + tl[1], o.bucket_slots, o.load_factor, dev, &t->dSlots,
+                        &t->nBuckets, s);
+        if (!rc)
+            rc = allocSlots(tl[2], 8, o.load_factor, dev, &t->dSlots2, &t->nBuckets2, s);
+        t->bytes = t->nBuckets * o.bucket_slots * 8ull + t->nBuckets2 * 64ull;
+        if (!rc && o.hbm_budget && t->bytes > o.hbm_budget)
+            rc = setError(HIPIBS_E_NOMEM, "table of " + std::to_string(t->bytes) +
+                                              " bytes exceeds the HBM budget of " +
+                                              std::to_string(o.hbm_budget));
+        if (!rc && hipMalloc(reinterpret_cast<void **>(&t->dCounters), 8 * sizeof(uint64_t)) !=
+                       hipSuccess)
+            rc = setError(HIPIBS_E_NOMEM, "hipMalloc of table counters failed");
+        if (!rc)
+            hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, s, t->dCounters, (uint64_t)8, 0ull);
+    }
+    if (!rc && n)
+    {
+        auto *ctr = reinterpret_cast<unsigned long long *>(t->dCounters);
+        if (t->bucketSlots == 8)
+            hipLaunchKernelGGL(placeKernel<8>, dim3(gridFor(n, 256, dev)), dim3(256), 0, s, dKeys,
+                               n, k, mlen, map, mapSize - 1, cap, t->dSlots, t->nBuckets,
+                               t->dSlots2, t->nBuckets2, ctr);
+        else
+            hipLaunchKernelGGL(placeKernel<16>, dim3(gridFor(n, 256, dev)), dim3(256), 0, s, dKeys,
+                               n, k, mlen, map, mapSize - 1, cap, t->dSlots, t->nBuckets,
+                               t->dSlots2, t->nBuckets2, ctr);
+        if (hipGetLastError() != hipSuccess)
+            rc = setError(HIPIBS_E_HIP, "minimizer table placement launch failed");
+    }
+    uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (!rc && (hipMemcpyAsync(c, t->dCounters, sizeof(c), hipMemcpyDeviceToHost, s) !=
+                    hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+        rc = setError(HIPIBS_E_HIP, "minimizer table placement failed");
+    if (!rc && c[1])
+        rc = setError(HIPIBS_E_FULL, "table overflow: no free slot for a key");
+    (void)hipStreamSynchronize(s);
+    hipFree(map);
+    hipFree(tally);
+    if (rc)
+    {
+        hipibs_table_free(t);
+        return rc;
+    }
+    t->nKeys = c[0] + c[3];
+    t->maxDisp = (uint32_t)c[2];
+    t->nKeys2 = c[3];
+    t->maxDisp2 = (uint32_t)c[4];
+    t->nMarkers = c[5];
+    *out = t;
+    return 0;
+}
+
 } // namespace hipibs
 
 using namespace hipibs;
@@ -303,6 +623,8 @@ extern "C" int hipibs_table_build_keys_device(const uint64_t *d_keys, uint64_t n
         return setError(HIPIBS_E_ARG, "null argument");
     *out = nullptr;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (opts && opts->minimizer_len)
+        return buildMinimizerTable(d_keys, n, k, opts, s, out);
     hipibs_table *t = nullptr;
     int rc = allocTable(n, k, opts, &t, s);
     if (rc)
@@ -484,6 +806,21 @@ extern "C" int hipibs_table_build_keys(const uint64_t *h_keys, uint64_t n, uint3
     if (!out || (n && !h_keys))
         return setError(HIPIBS_E_ARG, "null argument");
     *out = nullptr;
+    if (opts && opts->minimizer_len)
+    {
+        // the group counts need every key: upload them all first
+        HIPIBS_CHECK(hipSetDevice(opts->device));
+        uint64_t *dAll = nullptr;
+        if (hipMalloc(reinterpret_cast<void **>(&dAll), std::max<uint64_t>(n, 1) * 8) != hipSuccess)
+            return setError(HIPIBS_E_NOMEM, "device key buffer allocation failed");
+        int rc = hipMemcpy(dAll, h_keys, n * 8, hipMemcpyHostToDevice) == hipSuccess
+                     ? 0
+                     : setError(HIPIBS_E_HIP, "key upload failed");
+        if (!rc)
+            rc = buildMinimizerTable(dAll, n, k, opts, nullptr, out);
+        hipFree(dAll);
+        return rc;
+    }
     hipibs_table *t = nullptr;
     int rc = allocTable(n, k, opts, &t, nullptr);
     if (rc)
@@ -507,6 +844,8 @@ extern "C" int hipibs_table_free(hipibs_table *t)
     hipSetDevice(t->device);
     if (t->dSlots)
         hipFree(t->dSlots);
+    if (t->dSlots2)
+        hipFree(t->dSlots2);
     if (t->dCounters)
         hipFree(t->dCounters);
     delete t;
@@ -643,12 +982,45 @@ extern "C" int hipibs_table_build_kmc(const char *prefix, const hipibs_table_opt
         return rc;
     const uint64_t n = src.total;
     hipibs_table *t = nullptr;
-    rc = allocTable(n, src.info.kmer_length, opts, &t, nullptr);
-    if (rc)
-        return rc;
-    rc = hipStreamSynchronize(nullptr) == hipSuccess ? 0 : setError(HIPIBS_E_HIP, "table clear failed");
-    if (!rc)
-        rc = hipibsBuildFromKmc(t, src, opts ? opts->load_threads : 0);
+    if (opts && opts->minimizer_len)
+    {
+        // the group counts need every key: decode them all into one buffer
+        HIPIBS_CHECK(hipSetDevice(opts->device));
+        uint64_t *dAll = nullptr;
+        if (hipMalloc(reinterpret_cast<void **>(&dAll), std::max<uint64_t>(n, 1) * 8) != hipSuccess)
+            return setError(HIPIBS_E_NOMEM, "device key buffer allocation failed");
+        hipStream_t s;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+        {
+            hipFree(dAll);
+            return setError(HIPIBS_E_HIP, "stream creation failed");
+        }
+        uint64_t off = 0;
+        rc = streamKmcKeys(src, opts->device, opts->load_threads, s,
+                           [&](const uint64_t *dKeys, uint64_t m)
+                           {
+                               if (hipMemcpyAsync(dAll + off, dKeys, m * 8,
+                                                  hipMemcpyDeviceToDevice, s) != hipSuccess)
+                                   return setError(HIPIBS_E_HIP, "key copy failed");
+                               off += m;
+                               return 0;
+                           });
+        if (!rc)
+            rc = buildMinimizerTable(dAll, n, src.info.kmer_length, opts, s, &t);
+        (void)hipStreamSynchronize(s);
+        hipStreamDestroy(s);
+        hipFree(dAll);
+    }
+    else
+    {
+        rc = allocTable(n, src.info.kmer_length, opts, &t, nullptr);
+        if (rc)
+            return rc;
+        rc = hipStreamSynchronize(nullptr) == hipSuccess ? 0
+                                                         : setError(HIPIBS_E_HIP, "table clear failed");
+        if (!rc)
+            rc = hipibsBuildFromKmc(t, src, opts ? opts->load_threads : 0);
+    }
     if (!rc && t->nKeys != n)
         rc = setError(HIPIBS_E_IO, "KMC database holds duplicate k-mers (" +
                                        std::to_string(n) + " records, " +

@@ -112,9 +112,12 @@ __device__ inline int selectBit(uint64_t m, uint32_t n)
 
 // Returns the 64-bit hit mask of the wave: bit i set when lane i's query is
 // valid and its code is stored in the table; `home` is the query's home
-// bucket (homeOf). Must be called by all 64 lanes.
-template <int S>
-__device__ inline uint64_t waveProbe(const TableView T, uint64_t code, uint32_t home, bool valid)
+// bucket (homeOf). Must be called by all 64 lanes. With ALT, a query also
+// stops at the slot value `alt` (its minimizer group's marker) and *altHit
+// gets the mask of the queries that stopped there.
+template <int S, bool ALT = false>
+__device__ inline uint64_t waveProbe(const TableView T, uint64_t code, uint32_t home, bool valid,
+                                     uint64_t alt = 0, uint64_t *altHit = nullptr)
 {
     constexpr int LPQ = S / 2;      // lanes per query
     constexpr int QPR = 64 / LPQ;   // queries per round
@@ -127,13 +130,15 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, uint32_t 
     // All shuffles first, then all bucket loads: the loads are global
     // (vmcnt-only) so the LDS permutes never wait on them, and all ROUNDS
     // loads of the wave are in flight together.
-    uint64_t qc[ROUNDS];
+    uint64_t qc[ROUNDS], qa[ROUNDS];
     uint32_t qh[ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r)
     {
         qc[r] = shuffle64(code, r * QPR + g);
         qh[r] = shuffle32(home, r * QPR + g);
+        if constexpr (ALT)
+            qa[r] = shuffle64(alt, r * QPR + g);
     }
     SlotPair kv[ROUNDS];
 #pragma unroll
@@ -145,29 +150,43 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, uint32_t 
             kv[r] = loadPair(T.slots + (uint64_t)qh[r] * S + (uint32_t)sub * 2u);
         }
     }
-    uint64_t M[ROUNDS], E[ROUNDS];
+    uint64_t M[ROUNDS], E[ROUNDS], AM[ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r)
     {
         const bool vq = (V >> (r * QPR + g)) & 1ull;
         M[r] = ballot(vq && (kv[r].a == qc[r] || kv[r].b == qc[r]));
         E[r] = ballot(vq && (kv[r].a == EMPTY || kv[r].b == EMPTY));
+        if constexpr (ALT)
+            AM[r] = ballot(vq && (kv[r].a == qa[r] || kv[r].b == qa[r]));
     }
     const int rOwn = lane / QPR, gOwn = lane % QPR;
-    uint64_t Mo = M[0], Eo = E[0];
+    uint64_t Mo = M[0], Eo = E[0], Ao = 0;
+    if constexpr (ALT)
+        Ao = AM[0];
 #pragma unroll
     for (int r = 1; r < ROUNDS; ++r)
         if (rOwn == r)
         {
             Mo = M[r];
             Eo = E[r];
+            if constexpr (ALT)
+                Ao = AM[r];
         }
     const bool found = ((Mo >> (gOwn * LPQ)) & GMASK) != 0;
     const bool hasEmpty = ((Eo >> (gOwn * LPQ)) & GMASK) != 0;
+    const bool marked = ALT && ((Ao >> (gOwn * LPQ)) & GMASK) != 0;
     uint64_t H = ballot(valid && found);
-    uint64_t NC = ballot(valid && !found && !hasEmpty);
+    uint64_t A = 0;
+    if constexpr (ALT)
+        A = ballot(valid && !found && marked);
+    uint64_t NC = ballot(valid && !found && !marked && !hasEmpty);
     if (NC == 0 || T.maxDisp == 0)
+    {
+        if constexpr (ALT)
+            *altHit = A;
         return H;
+    }
 
     // Overflow rounds: pending lane i continues at home_i + disp_i.
     uint32_t disp = 1;
@@ -181,6 +200,9 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, uint32_t 
         const uint64_t qcode = shuffle64(code, src);
         const uint64_t qhome = shuffle32(home, src);
         const uint32_t qdisp = shuffle32(disp, src);
+        uint64_t qalt = 0;
+        if constexpr (ALT)
+            qalt = shuffle64(alt, src);
         SlotPair kv2{EMPTY, EMPTY};
         if (serving)
         {
@@ -191,18 +213,54 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, uint32_t 
         }
         const uint64_t M2 = ballot(serving && (kv2.a == qcode || kv2.b == qcode));
         const uint64_t E2 = ballot(serving && (kv2.a == EMPTY || kv2.b == EMPTY));
+        uint64_t A2 = 0;
+        if constexpr (ALT)
+            A2 = ballot(serving && (kv2.a == qalt || kv2.b == qalt));
         const uint32_t rank = maskedRank(NC);
         const bool served = pending && rank < (uint32_t)QPR;
         const uint32_t sh = (served ? rank : 0u) * (uint32_t)LPQ;
         const bool f2 = served && ((M2 >> sh) & GMASK) != 0;
         const bool e2 = served && ((E2 >> sh) & GMASK) != 0;
-        const bool done = served && (f2 || e2 || disp >= T.maxDisp);
+        const bool a2 = ALT && served && !f2 && ((A2 >> sh) & GMASK) != 0;
+        const bool done = served && (f2 || a2 || e2 || disp >= T.maxDisp);
         H |= ballot(f2);
+        if constexpr (ALT)
+            A |= ballot(a2);
         if (served && !done)
             disp += 1;
         NC &= ~ballot(done);
     }
+    if constexpr (ALT)
+        *altHit = A;
     return H;
+}
+
+// Membership in a minimizer table of the codes whose minimizer hash is
+// minHash: the minimizer-ordered part first; queries that meet their
+// group's marker there continue in the whole-k-mer part.
+template <int S>
+__device__ inline uint64_t waveProbeMinimizer(const TableView &T, uint64_t code, uint32_t minHash,
+                                              bool valid)
+{
+    const uint32_t home = valid ? (uint32_t)bucketOfMinimizer(minHash, T.nBuckets) : 0u;
+    uint64_t A = 0;
+    uint64_t H = waveProbe<S, true>(T, code, home, valid, markerOf(minHash), &A);
+    if (A)
+    {
+        const TableView T2{T.slots2, T.nBuckets2, T.maxDisp2, 0u, T.k, 0u, nullptr, 0ull};
+        const bool v2 = (A >> laneId()) & 1ull;
+        H |= waveProbe<8>(T2, code, v2 ? (uint32_t)bucketOf(code, T.nBuckets2) : 0u, v2);
+    }
+    return H;
+}
+
+// Membership of each lane's code in T, whole-k-mer or minimizer table.
+template <int S>
+__device__ inline uint64_t waveLookup(const TableView &T, uint64_t code, bool valid)
+{
+    if (T.mlen == 0)
+        return waveProbe<S>(T, code, valid ? (uint32_t)bucketOf(code, T.nBuckets) : 0u, valid);
+    return waveProbeMinimizer<S>(T, code, valid ? minimizerOf(code, T.k, T.mlen) : 0u, valid);
 }
 
 __device__ inline uint32_t waveSumU32(uint32_t v)

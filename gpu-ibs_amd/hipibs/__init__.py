@@ -39,7 +39,7 @@ vp = C.c_void_p
 class TableOpts(C.Structure):
     _fields_ = [("load_factor", C.c_double), ("bucket_slots", C.c_uint32), ("device", C.c_int),
                 ("hbm_budget", C.c_uint64), ("load_threads", C.c_uint32),
-                ("minimizer_len", C.c_uint32)]
+                ("minimizer_len", C.c_uint32), ("minimizer_cap", C.c_uint32)]
 
 
 class Window(C.Structure):
@@ -389,8 +389,9 @@ def synchronize(device: int = 0) -> None:
 # ---- tables -------------------------------------------------------------------
 
 def _opts(load_factor=0.6, bucket_slots=0, device=0, hbm_budget=0, load_threads=0,
-          minimizer_len=0):
-    return TableOpts(load_factor, bucket_slots, device, hbm_budget, load_threads, minimizer_len)
+          minimizer_len=0, minimizer_cap=0):
+    return TableOpts(load_factor, bucket_slots, device, hbm_budget, load_threads, minimizer_len,
+                     minimizer_cap)
 
 
 class Table:

@@ -165,7 +165,8 @@ inline const char *optionsUsage()
            "  --batch-mb=N          bases per streamed batch, in MiB (default 128)\n"
            "  --streams=N           batches in flight (default 3)\n"
            "  --load-factor=F       device table load factor (default 0.6)\n"
-           "  --bucket-bytes=64|128 device table bucket width (default 64)\n"
+           "  --bucket-bytes=64|128 device table bucket width (default 64; a panel\n"
+           "                        times both unless this is given)\n"
            "  --hbm-budget-gb=N     cap on device table memory\n"
            "  --db=<folder>         (fastibs) another sample in the same reference pass\n"
            "  --db-list=<file>      (fastibs) sample folders, one per line\n"
@@ -209,7 +210,10 @@ inline bool parseOptions(int argc, char **argv, int first, Options &o, bool pane
             g.loadFactor = f;
         }
         else if (key == "--bucket-bytes" && (val == "64" || val == "128"))
+        {
             g.bucketSlots = val == "64" ? 8 : 16;
+            g.bucketAuto = false;
+        }
         else if (key == "--hbm-budget-gb" && parseInt(val, 1, 1 << 20, v))
             g.hbmBudget = (uint64_t)v << 30;
         else if (panel && key == "--db" && !val.empty())

@@ -34,6 +34,7 @@ public:
         int depth = 3;
         double loadFactor = 0.6;
         uint32_t bucketSlots = 8;
+        bool bucketAuto = true; // a panel may time 64 B vs 128 B buckets
         uint64_t hbmBudget = 0;
         unsigned loadThreads = 0;
     };
@@ -44,6 +45,8 @@ public:
     }
 
     // Reads the header only; the device table is built on first use.
+    uint64_t kmerCount() const { return info_.total_kmers; }
+
     explicit KmerDatabase(std::string sourcePath, bool listing = false)
         : sourcePath_(std::move(sourcePath))
     {

@@ -132,6 +132,54 @@ public:
         out.bytes = nsw * 8 + new_ * 8 + win.size() * sizeof(hipibs_window);
     }
 
+    // Bucket width: the smallest database's table is built with 64 B and
+    // with 128 B buckets and scanned over the first 1024 windows; the faster
+    // width builds every table of the panel.
+    void tuneBucketWidth(const DeviceReference &dref)
+    {
+        using clk = std::chrono::steady_clock;
+        auto &g = KmerDatabase::gpuOptions();
+        size_t pick = 0;
+        for (size_t i = 1; i < dbs_.size(); ++i)
+            if (dbs_[i]->kmerCount() < dbs_[pick]->kmerCount())
+                pick = i;
+        const uint32_t nw = std::min<uint32_t>(dref.nWindows, 1024);
+        const uint64_t wb = hipibs_panel_work_bytes(nw, 1, 1);
+        hipibs::DeviceBuffer stats(g.device, (uint64_t)nw * sizeof(hipibs_window_stats));
+        hipibs::DeviceBuffer work(g.device, std::max<uint64_t>(wb, 8));
+        const hipibs_scan_config c{1, 16, {0, 0}};
+        double best = 1e30;
+        uint32_t bestSlots = g.bucketSlots;
+        for (uint32_t slots : {8u, 16u})
+        {
+            g.bucketSlots = slots;
+            dbs_[pick]->initGpuIndex();
+            const hipibs_table *t = dbs_[pick]->table();
+            double sec = 0;
+            for (int rep = 0; rep < 3; ++rep)
+            {
+                const auto t0 = clk::now();
+                hipibs::check(hipibs_panel_scan_device(&t, 1, dref.seq.as<uint64_t>(),
+                                                       dref.exc.as<uint64_t>(), dref.nBases,
+                                                       dref.windows.as<hipibs_window>(), nw,
+                                                       stats.as<hipibs_window_stats>(), &c,
+                                                       work.get(), wb, nullptr),
+                              "bucket-width trial");
+                hipibs::check(hipibs_dev_synchronize(g.device), "bucket-width trial");
+                if (rep > 0)
+                    sec += std::chrono::duration<double>(clk::now() - t0).count();
+            }
+            dbs_[pick]->releaseGpuIndex();
+            if (sec < best)
+            {
+                best = sec;
+                bestSlots = slots;
+            }
+        }
+        g.bucketSlots = bestSlots;
+        std::cout << "Autotuned bucket width: " << bestSlots * 8 << " B" << std::endl;
+    }
+
     // outPaths[i] is database i's TSV; an empty path skips the database.
     // Device-resident path: the reference is packed and uploaded once;
     // each memory-budgeted round builds its tables and replays one captured
@@ -155,6 +203,8 @@ public:
         std::cout << "Panel: " << dbs_.size() << " databases in " << rounds.size()
                   << " device round(s); reference " << dref.nBases << " bases, "
                   << dref.nWindows << " windows, ingest " << ingestSec << " s" << std::endl;
+        if (g.bucketAuto && dref.nWindows)
+            tuneBucketWidth(dref);
         hipibs_scan_config cfg{0, 0, {0, 0}};
         bool tuned = false;
         double buildSec = 0, scanSec = 0, writeSec = 0;
@@ -186,7 +236,7 @@ public:
                 hipibs::check(hipibs_panel_autotune(tabs.data(), nt, dref.seq.as<uint64_t>(),
                                                     dref.exc.as<uint64_t>(), dref.nBases,
                                                     dref.windows.as<hipibs_window>(), dref.nWindows,
-                                                    2048, stats.as<hipibs_window_stats>(),
+                                                    1024, stats.as<hipibs_window_stats>(),
                                                     work.get(), wb, &cfg, &ms),
                               "panel autotune");
                 std::cout << "Autotuned panel launch: " << cfg.waves_per_cu << " waves/CU, "

@@ -57,10 +57,13 @@ typedef struct hipibs_table_opts {
     uint64_t hbm_budget;    /* max table bytes, 0 = no limit */
     uint32_t load_threads;  /* host threads of a KMC read, 0 = all CPUs */
     uint32_t minimizer_len; /* 0 (default): home bucket from a hash of the whole k-mer;
-                               5..17: from the k-mer's minimizer of that length, so
-                               consecutive positions share buckets (fewer requests on
-                               non-repetitive sequence, long overflow chains on
-                               repeat-rich sequence) */
+                               5..17: a minimizer table -- keys sharing a minimizer of
+                               that length share buckets, so consecutive positions of a
+                               reference share memory requests */
+    uint32_t minimizer_cap; /* minimizer tables: a minimizer with more keys than this
+                               (0 = 32) leaves one marker in the minimizer-ordered
+                               buckets and its keys go to a whole-k-mer part (repeat
+                               families would otherwise form long probe chains) */
 } hipibs_table_opts;
 
 typedef struct hipibs_window {

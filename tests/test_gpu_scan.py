@@ -80,22 +80,81 @@ def test_edge_scans_match_golden(k, expected):
     t.free()
 
 
-@pytest.mark.parametrize("slots,lf,mlen", [(8, 0.6, 0), (16, 0.6, 0), (8, 0.9, 0), (16, 0.95, 0),
-                                           (8, 0.3, 0), (16, 0.6, 13), (8, 0.6, 17),
-                                           (16, 0.9, 5), (16, 0.6, 17)])
-def test_table_geometry_does_not_change_results(small_corpus, slots, lf, mlen):
+@pytest.mark.parametrize("slots,lf,mlen,cap", [(8, 0.6, 0, 0), (16, 0.6, 0, 0), (8, 0.9, 0, 0),
+                                               (16, 0.95, 0, 0), (8, 0.3, 0, 0), (16, 0.6, 13, 0),
+                                               (8, 0.6, 17, 0), (16, 0.9, 5, 0), (16, 0.6, 17, 0),
+                                               (16, 0.6, 13, 1), (16, 0.9, 15, 2), (8, 0.6, 11, 4),
+                                               (16, 0.6, 9, 0)])
+def test_table_geometry_does_not_change_results(small_corpus, slots, lf, mlen, cap):
     """Bucket width, load factor and home-bucket rule (whole-k-mer hash,
-    minimizers of several lengths) never change the statistics."""
+    minimizer tables of several lengths and group caps, so that groups
+    overflow to the whole-k-mer part behind markers) never change the
+    statistics."""
     ref, samples = small_corpus
     seq = corpus.to_text(ref)
     keys = samples[1][0]
-    note(f"geometry slots={slots} lf={lf} mlen={mlen}")
-    t = hipibs.Table.from_keys(keys, 25, bucket_slots=slots, load_factor=lf, minimizer_len=mlen)
+    note(f"geometry slots={slots} lf={lf} mlen={mlen} cap={cap}")
+    t = hipibs.Table.from_keys(keys, 25, bucket_slots=slots, load_factor=lf, minimizer_len=mlen,
+                               minimizer_cap=cap)
     assert t.info()["minimizer_len"] == mlen
+    assert t.info()["n_keys"] == keys.size
     rows = hipibs.scan_sequences([t], [b"chr1"], [seq], 5000)[0]
     want = oracle_scan_rows([b"chr1"], [seq], 5000, 25, sorted_keys(keys))
     assert rows == want
     t.free()
+
+
+@pytest.mark.parametrize("k,mlen,cap", [(25, 13, 1), (25, 15, 3), (31, 17, 2), (32, 17, 1),
+                                        (15, 7, 2), (31, 13, 0)])
+def test_minimizer_table_every_build_path(tmp_path, k, mlen, cap):
+    """Minimizer tables built from host keys, device keys and a KMC database
+    hold exactly the key set (markers never answer as keys), and their scans
+    and hit masks equal the oracle's on the edge corpus."""
+    import ctypes as C
+    recs, keys = corpus.edge_config(k)
+    ids = [r[0] for r in recs]
+    seqs = [r[1] for r in recs]
+    note(f"minimizer table k={k} m={mlen} cap={cap}")
+    other = np.unique(corpus.splitmix(k, 91, 30000) & np.uint64((1 << (2 * k)) - 1 if k < 32
+                                                              else 0xFFFFFFFFFFFFFFFF))
+    other = np.unique(np.concatenate([np.minimum(other, corpus.revcomp(other, k)), keys[::4]]))
+    want_other = int(np.isin(other, keys).sum())
+    pre = str(tmp_path / "db")
+    hipibs.kmc_write(pre, keys, k, sig_len=9 if k > 9 else 5, counter_size=1, version=0x200,
+                     min_count=1, max_count=255)
+    d_keys = D.from_array(keys)
+    opts = dict(minimizer_len=mlen, minimizer_cap=cap)
+    tables = [hipibs.Table.from_keys(keys, k, **opts),
+              hipibs.Table.from_device_keys(d_keys.ptr, keys.size, k, **opts),
+              hipibs.Table.from_kmc(pre, **opts)]
+    hipibs.synchronize()
+    want_rows = oracle_scan_rows(ids, seqs, 5000, k, sorted_keys(keys))
+    for t in tables:
+        assert t.info()["n_keys"] == keys.size
+        assert t.probe_count(keys) == keys.size
+        assert t.probe_count(other) == want_other
+        rows = hipibs.scan_sequences([t], ids, seqs, 5000)[0]
+        assert rows == want_rows
+    # hit mask of the longest contig against the oracle's coverage rule
+    c = max(range(len(seqs)), key=lambda i: len(seqs[i]))
+    seq = seqs[c]
+    sw, ew, _ = hipibs.pack_sequences([seq])
+    npos = len(seq) - k + 1
+    d_sw, d_ew = D.from_array(sw), D.from_array(ew)
+    d_hit = D.zeros(((npos + 31) // 32 + 1) * 4)
+    hipibs._check(hipibs._lib.hipibs_hitmask_device(tables[2].handle, d_sw.ptr, d_ew.ptr, 0, npos,
+                                                    len(seq), d_hit.ptr, None), "hitmask")
+    hipibs.synchronize()
+    words = d_hit.to_array(np.uint32, (npos + 31) // 32)
+    bits = ((words[:, None] >> np.arange(32, dtype=np.uint32)) & 1).reshape(-1)[:npos]
+    lib = oracle()
+    enc = np.zeros(npos, dtype=np.uint64)
+    val = np.zeros(npos, dtype=np.uint8)
+    assert lib.orc_encode_positions(seq, len(seq), k, enc.ctypes.data, val.ctypes.data) == npos
+    want_bits = (val.astype(bool) & np.isin(enc, keys)).astype(np.int64)
+    assert np.array_equal(bits.astype(np.int64), want_bits)
+    for t in tables:
+        t.free()
 
 
 def test_probe_count_intersect(small_corpus, expected):
