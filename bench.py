@@ -54,15 +54,17 @@ def parse():
                     help="reference slice (Mbp) of the CPU baseline sample; 0 = skip")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0x1B50200)
-    ap.add_argument("--serve-samples", type=int, default=48)
-    ap.add_argument("--serve-min-keys", type=float, default=0.05e9)
-    ap.add_argument("--serve-max-keys", type=float, default=0.3e9)
-    ap.add_argument("--serve-budget-gb", default="150,50",
+    ap.add_argument("--serve-samples", type=int, default=96)
+    ap.add_argument("--serve-min-keys", type=float, default=0.075e9)
+    ap.add_argument("--serve-max-keys", type=float, default=0.375e9)
+    ap.add_argument("--serve-preload", type=float, default=0.95,
+                    help="fraction of the first budget filled with preloaded tables")
+    ap.add_argument("--serve-budget-gb", default="100",
                     help="HBM budgets to run (GB); the first is used with every rate, "
                          "the others with the lowest rate")
     ap.add_argument("--serve-zipf", type=float, default=1.1,
                     help="sample popularity ~ 1/rank^s (0 = uniform)")
-    ap.add_argument("--serve-rates", default="20,100,200")
+    ap.add_argument("--serve-rates", default="5,20,50,100,200")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--workload", default="scan", choices=["scan", "panel", "serve", "large"],
                     help="scan: the single-sample device scan (default line); panel: "
@@ -475,10 +477,12 @@ def run_serve(args):
                                  max_windows_per_launch=65536)
         eng.set_reference(h_sw, h_ew, n)
         budget_used = 0
+        n_pre = 0
         for i in range(S):
             tb = int(np.ceil(np.ceil(keys[i].size / 0.6) / 8) * 64)
-            pre = budget_used + tb <= 0.7 * budget_gb * 1e9
+            pre = budget_used + tb <= args.serve_preload * budget_gb * 1e9
             budget_used += tb if pre else 0
+            n_pre += int(pre)
             eng.add_sample(i, keys[i], preload=pre)
         rr = np.random.default_rng(args.seed + int(rate))
         m = int(rate * args.serve_seconds)
@@ -514,7 +518,8 @@ def run_serve(args):
         c = eng.counters()
         eng.free()
         L = np.array(list(lat.values()))
-        res = {"budget_gb": budget_gb, "rate_req_s": rate, "requests": m, "p50_ms": float(np.percentile(L, 50)),
+        res = {"budget_gb": budget_gb, "rate_req_s": rate, "requests": m, "preloaded": n_pre,
+               "p50_ms": float(np.percentile(L, 50)),
                "p99_ms": float(np.percentile(L, 99)), "mean_ms": float(L.mean()),
                "probes_per_s": probes / wall, "wall_s": wall, "launches": c["launches"],
                "admissions": c["admissions"], "evictions": c["evictions"],
