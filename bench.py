@@ -966,16 +966,16 @@ def run_large(args):
 
 def main():
     args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
     if args.workload == "panel":
         return run_panel(args)
     if args.workload == "serve":
         return run_serve(args)
     if args.workload == "large":
         return run_large(args)
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is a design note only)", file=sys.stderr)
-        sys.exit(2)
     import numpy as np
     import torch
 
