@@ -5,9 +5,12 @@
 // finished launches, handing their statistics to requests and unpinning
 // their tables; (2) if a launch slot is free, fills it from the pending
 // requests in arrival order -- a request whose sample is not resident is
-// admitted by building its table from the registered host keys, after
-// evicting least-recently-used tables that no launch holds, as far as the
-// HBM budget requires -- and launches one mixed scan in which every
+// admitted by building its table from the registered host keys on a
+// builder thread (at most one build per builder in flight), after evicting
+// tables that no launch holds, the least requested first, as far as the
+// HBM budget requires; a table is only evicted for a sample requested at
+// least as often, unless the waiting request is older than kForceAdmitMs
+// -- and launches one mixed scan in which every
 // window names its own table, so requests for different samples, ranges
 // and window sizes share a launch; (3) sleeps until a submission or a
 // completion if nothing moved. Up to max_inflight launches are in flight
@@ -55,7 +58,12 @@ struct Sample
     bool building = false; // queued on or running in the builder thread
     int pins = 0;
     uint64_t lastUse = 0;
+    uint64_t hits = 0; // requests received, halved every kAgeEvery requests
 };
+
+// Admission policy constants.
+constexpr uint64_t kAgeEvery = 4096; // requests between halvings of the hit counts
+constexpr double kForceAdmitMs = 200.0; // a request waiting longer may evict anything
 
 struct Built
 {
@@ -99,6 +107,7 @@ struct hipibs_serve
     uint64_t nBases = 0;
     std::unordered_map<uint32_t, Sample> samples;
     uint64_t resident = 0, residentTables = 0, useClock = 0; // worker-owned
+    uint64_t buildsInFlight = 0, requestsSeen = 0;           // worker-owned
     std::vector<Slot> slots;
     std::deque<std::shared_ptr<Request>> incoming, pending;
     std::unordered_map<uint64_t, std::shared_ptr<Request>> completed;
@@ -126,18 +135,21 @@ double ms(Clock::time_point a, Clock::time_point b)
     return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
-// Evicts unpinned tables, least recently used first, until `need` more
-// bytes fit the budget; false if they cannot.
-bool makeRoom(hipibs_serve *s, uint64_t need)
+// Evicts unpinned tables until `need` more bytes fit the budget: the
+// least requested first (ties: least recently used). Without `force`, a
+// table requested more often than the candidate (candHits) is never
+// evicted for it. False if the bytes cannot be freed.
+bool makeRoom(hipibs_serve *s, uint64_t need, uint64_t candHits, bool force)
 {
     while (s->resident + need > s->budget)
     {
         Sample *victim = nullptr;
         for (auto &kv : s->samples)
             if (kv.second.table && kv.second.pins == 0 &&
-                (!victim || kv.second.lastUse < victim->lastUse))
+                (!victim || kv.second.hits < victim->hits ||
+                 (kv.second.hits == victim->hits && kv.second.lastUse < victim->lastUse)))
                 victim = &kv.second;
-        if (!victim)
+        if (!victim || (!force && victim->hits > candHits))
             return false;
         hipibs_table_free(victim->table);
         victim->table = nullptr;
@@ -157,10 +169,11 @@ uint64_t tableBytesFor(const hipibs_serve *s, uint64_t n)
     return buckets * slots * 8;
 }
 
-// Builds sample id's table now (preload) -- or, async, queues it on the
+// Builds sample id's table now (preload) -- or, async, queues it on a
 // builder thread -- after making room. 0 = resident, 1 = not (yet), <0 =
 // error. The bytes of a queued build are reserved against the budget.
-int ensureResident(hipibs_serve *s, uint32_t id, bool async)
+// `force`: the waiting request is old enough to evict any idle table.
+int ensureResident(hipibs_serve *s, uint32_t id, bool async, bool force)
 {
     Sample &sm = s->samples.at(id);
     sm.lastUse = ++s->useClock;
@@ -168,16 +181,19 @@ int ensureResident(hipibs_serve *s, uint32_t id, bool async)
         return 0;
     if (sm.building)
         return 1;
+    if (async && s->buildsInFlight >= s->builders.size())
+        return 1;
     const uint64_t need = tableBytesFor(s, sm.n);
     if (need > s->budget)
         return setError(HIPIBS_E_NOMEM, "a sample table alone exceeds the serving HBM budget");
-    if (!makeRoom(s, need))
+    if (!makeRoom(s, need, sm.hits, force || !async))
         return 1;
     sm.bytes = need;
     s->resident += need;
     if (async)
     {
         sm.building = true;
+        s->buildsInFlight += 1;
         {
             std::lock_guard<std::mutex> lk(s->bmu);
             s->buildQueue.push_back(id);
@@ -241,6 +257,7 @@ int collectBuilds(hipibs_serve *s, bool &progress)
     {
         Sample &sm = s->samples.at(b.id);
         sm.building = false;
+        s->buildsInFlight -= 1;
         if (!b.table)
             return setError(HIPIBS_E_HIP, "table build failed: " + b.error);
         sm.table = b.table;
@@ -300,7 +317,8 @@ int fillAndLaunch(hipibs_serve *s, Slot &sl, bool &launched)
          it != s->pending.end() && used < s->maxWindows && bases < LAUNCH_BASES;)
     {
         const std::shared_ptr<Request> &r = *it;
-        const int rc = ensureResident(s, r->sample, true);
+        const int rc = ensureResident(s, r->sample, true,
+                                      ms(r->tSubmit, Clock::now()) > kForceAdmitMs);
         if (rc < 0)
             return rc;
         if (rc == 1)
@@ -388,6 +406,10 @@ void workerLoop(hipibs_serve *s)
             std::lock_guard<std::mutex> lk(s->mu);
             while (!s->incoming.empty())
             {
+                s->samples.at(s->incoming.front()->sample).hits += 1;
+                if (++s->requestsSeen % kAgeEvery == 0)
+                    for (auto &kv : s->samples)
+                        kv.second.hits /= 2;
                 s->pending.push_back(std::move(s->incoming.front()));
                 s->incoming.pop_front();
                 progress = true;
@@ -545,7 +567,7 @@ extern "C" int hipibs_serve_add_sample(hipibs_serve *s, uint32_t sample_id, cons
     s->samples[sample_id] = sm;
     if (preload)
     {
-        const int rc = ensureResident(s, sample_id, false);
+        const int rc = ensureResident(s, sample_id, false, true);
         if (rc < 0)
             return rc;
         if (rc == 1)

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 namespace hipibs
@@ -641,6 +642,28 @@ extern "C" int hipibs_table_build_keys_device(const uint64_t *d_keys, uint64_t n
     return 0;
 }
 
+// memcpy on up to 8 host threads (staging copies into pinned memory are
+// memory-bandwidth bound; one thread reaches a fraction of it).
+static void parallelCopy(void *dst, const void *src, uint64_t bytes)
+{
+    const uint64_t piece = 16ull << 20;
+    const unsigned n = (unsigned)std::min<uint64_t>(8, (bytes + piece - 1) / piece);
+    if (n <= 1)
+    {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const uint64_t per = (bytes + n - 1) / n;
+    std::vector<std::thread> pool;
+    for (unsigned i = 0; i < n; ++i)
+        pool.emplace_back([=] {
+            const uint64_t a = std::min(bytes, (uint64_t)i * per), b = std::min(bytes, a + per);
+            std::memcpy(static_cast<char *>(dst) + a, static_cast<const char *>(src) + a, b - a);
+        });
+    for (auto &th : pool)
+        th.join();
+}
+
 // Streams host keys through two pinned staging buffers (copy of chunk i+1
 // overlaps the insert of chunk i).
 int hipibsBuildFromHostChunks(hipibs_table *t, const uint64_t *hKeys, uint64_t n)
@@ -669,7 +692,7 @@ int hipibsBuildFromHostChunks(hipibs_table *t, const uint64_t *hKeys, uint64_t n
             rc = setError(HIPIBS_E_HIP, "event wait failed");
             break;
         }
-        std::memcpy(pinned[b], hKeys + off, m * 8);
+        parallelCopy(pinned[b], hKeys + off, m * 8);
         if (hipMemcpyAsync(dev[b], pinned[b], m * 8, hipMemcpyHostToDevice, s) != hipSuccess)
         {
             rc = setError(HIPIBS_E_HIP, "key upload failed");
@@ -937,7 +960,7 @@ extern "C" int hipibs_table_probe_count(const hipibs_table *t, const uint64_t *h
             rc = setError(HIPIBS_E_HIP, "event wait failed");
             break;
         }
-        std::memcpy(pinned[b], h_keys + off, m * 8);
+        parallelCopy(pinned[b], h_keys + off, m * 8);
         if (hipMemcpyAsync(dev[b], pinned[b], m * 8, hipMemcpyHostToDevice, s) != hipSuccess)
         {
             rc = setError(HIPIBS_E_HIP, "key upload failed");

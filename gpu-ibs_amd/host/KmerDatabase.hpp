@@ -18,6 +18,7 @@
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+#include <future>
 #include <iostream>
 #include <string>
 #include <thread>
@@ -218,10 +219,22 @@ public:
             t = n;
             return s;
         };
-        initGpuIndex();
-        const double indexSec = lap();
+        // the reference is read and cut into windows on its own thread while
+        // the table is built
         ReferenceWindows rw;
-        readReferenceWindows(refPath, windowSize, rw);
+        auto refRead = std::async(std::launch::async,
+                                  [&] { readReferenceWindows(refPath, windowSize, rw); });
+        try
+        {
+            initGpuIndex();
+        }
+        catch (...)
+        {
+            refRead.wait();
+            throw;
+        }
+        const double indexSec = lap();
+        refRead.get();
         lap();
         std::cout << "Kmer size: " << kmerSize << "\nK-mer path: fused device encode+probe+"
                   << "window reduction (MI355X)" << std::endl;
@@ -285,11 +298,22 @@ public:
     void produceMapping(std::string refPath, std::string outPath)
     {
         using clk = std::chrono::steady_clock;
-        initGpuIndex();
+        std::cout << "Reading sequences" << std::endl;
+        auto refRead = std::async(std::launch::async, [&] {
+            return readSequenceFile(refPath, refPath.find(".gz") != std::string::npos);
+        });
+        try
+        {
+            initGpuIndex();
+        }
+        catch (...)
+        {
+            refRead.wait();
+            throw;
+        }
         std::cout << "Kmer size: " << kmerSize << "\nK-mer path: device hit mask + coverage"
                   << std::endl;
-        std::cout << "Reading sequences" << std::endl;
-        ParseResult r = readSequenceFile(refPath, refPath.find(".gz") != std::string::npos);
+        ParseResult r = refRead.get();
         const auto t0 = clk::now();
         const int dev = gpuOptions().device;
         const unsigned threads = std::min(64u, seqio_detail::hostThreads());
