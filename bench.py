@@ -523,10 +523,13 @@ def run_serve(args):
                "p99_ms": float(np.percentile(L, 99)), "mean_ms": float(L.mean()),
                "probes_per_s": probes / wall, "wall_s": wall, "launches": c["launches"],
                "admissions": c["admissions"], "evictions": c["evictions"],
+               "build_ms_mean": c["build_us"] / 1e3 / max(1, c["admissions"] - n_pre),
+               "evict_ms_mean": c["evict_us"] / 1e3 / max(1, c["evictions"]),
                "resident_tables_end": c["resident_tables"]}
         log(f"budget {budget_gb} GB, rate {rate}/s: p50 {res['p50_ms']:.1f} ms p99 {res['p99_ms']:.1f} ms, "
             f"{res['probes_per_s']/1e9:.2f} G probes/s, {c['launches']} launches, "
-            f"{c['admissions']} admissions, {c['evictions']} evictions")
+            f"{c['admissions']} admissions ({res['build_ms_mean']:.0f} ms per build), "
+            f"{c['evictions']} evictions ({res['evict_ms_mean']:.1f} ms per free)")
         results.append(res)
     out = {"metric": "served p50/p99 query latency", "unit": "ms", "n_gpus": 1,
            "zipf": args.serve_zipf,
@@ -873,6 +876,7 @@ def run_large(args):
                                 [(names[c], ref_codes[int(offs[c]):int(offs[c + 1])])
                                  for c in range(len(lens))])
             fasta_s = time.time() - t1
+            n_keys = int(keys.numel())
             t1 = time.time()
             hkeys = keys.cpu().numpy().view(np.uint64)
             hipibs.kmc_write(os.path.join(root, "sample", "sample"), hkeys, k, sig_len=9,
@@ -912,7 +916,13 @@ def run_large(args):
                           for c, (s_, cm) in enumerate(chk)))
             log(f"fastibsmapper end to end: {map_s:.1f} s, {msize/1e9:.2f} GB of coverage text; "
                 f"size, comma counts and value sums match the device coverage: {map_ok}")
+            isect_s, iout = run([os.path.join(bindir, "KDBIntersect"), db, db])
+            isect_count = int(iout.strip().splitlines()[-1])
+            log(f"KDBIntersect (self) end to end: {isect_s:.1f} s, count {isect_count} "
+                f"(== keys: {isect_count == n_keys})")
             e2e = {"fastibs_s": scan_s, "fastibsmapper_s": map_s, "fasta_write_s": fasta_s,
+                   "kdbintersect_self_s": isect_s, "kdbintersect_count": isect_count,
+                   "kdbintersect_count_equals_keys": isect_count == n_keys,
                    "kmc_write_s": kmc_s, "tsv_equals_device_stats": tsv_same,
                    "coverage_text_bytes": msize, "coverage_text_checks_pass": map_ok,
                    "fastibs_log": [l for l in out.splitlines() if l.strip()][-6:],

@@ -151,12 +151,15 @@ bool makeRoom(hipibs_serve *s, uint64_t need, uint64_t candHits, bool force)
                 victim = &kv.second;
         if (!victim || (!force && victim->hits > candHits))
             return false;
+        const auto t0 = Clock::now();
         hipibs_table_free(victim->table);
+        const uint64_t us = (uint64_t)(ms(t0, Clock::now()) * 1000.0);
         victim->table = nullptr;
         s->resident -= victim->bytes;
         s->residentTables -= 1;
         std::lock_guard<std::mutex> lk(s->mu);
         s->counters.evictions += 1;
+        s->counters.evict_us += us;
     }
     return true;
 }
@@ -236,7 +239,13 @@ void builderLoop(hipibs_serve *s)
         hipibs_table_opts o = s->topts;
         o.device = s->device;
         hipibs_table *t = nullptr;
+        const auto t0 = Clock::now();
         const int rc = hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &t);
+        const uint64_t us = (uint64_t)(ms(t0, Clock::now()) * 1000.0);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->counters.build_us += us;
+        }
         {
             std::lock_guard<std::mutex> lk(s->bmu);
             s->built.push_back(Built{id, rc ? nullptr : t, rc ? hipibs_last_error() : ""});

@@ -66,7 +66,7 @@ class ScanConfig(C.Structure):
 class ServeCounters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "launches", "windows",
                                          "admissions", "evictions", "resident_bytes",
-                                         "resident_tables")]
+                                         "resident_tables", "build_us", "evict_us")]
 
 
 class HostSlot(C.Structure):

@@ -336,13 +336,17 @@ typedef struct hipibs_serve_counters {
     uint64_t submitted, completed, launches, windows;
     uint64_t admissions, evictions;   /* tables built / freed for memory */
     uint64_t resident_bytes, resident_tables;
+    uint64_t build_us;                /* summed wall time of table builds */
+    uint64_t evict_us;                /* summed wall time of eviction frees */
 } hipibs_serve_counters;
 
 /* A serving engine on `device` for k-mers of length k: sample tables are
  * built (with topts) on demand from registered host keys and kept resident
- * within hbm_budget bytes, least recently used unpinned tables evicted
- * first; up to max_inflight launches of at most max_windows_per_launch
- * windows run at once, each batching windows of any pending requests.
+ * within hbm_budget bytes; idle tables are evicted least requested first
+ * (request counts halved every 4096 requests), and only for a sample
+ * requested at least as often unless its request has waited 200 ms; up to
+ * max_inflight launches of at most max_windows_per_launch windows run at
+ * once, each batching windows of any pending requests.
  * No counterpart in the reference (it has no serving mode). */
 int hipibs_serve_create(int device, uint32_t k, uint64_t hbm_budget,
                         const hipibs_table_opts *topts, int max_inflight,
