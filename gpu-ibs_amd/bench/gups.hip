@@ -49,6 +49,62 @@ __device__ inline U64x2 loadBucketPart(const uint64_t *p)
     }
 }
 
+// Policies 4..7: four 16 B loads issued together in one asm block with
+// cache-policy bits the builtins cannot express (4: sc0, 5: sc0 nt,
+// 6: sc1 nt, 7: nt, the last as a control for the builtin's nt); the block
+// waits for its loads before it ends.
+#define GUPS_LOAD4(CPOL)                                                                         \
+    asm volatile("global_load_dwordx4 %0, %4, off " CPOL "\n"                                     \
+                 "global_load_dwordx4 %1, %5, off " CPOL "\n"                                     \
+                 "global_load_dwordx4 %2, %6, off " CPOL "\n"                                     \
+                 "global_load_dwordx4 %3, %7, off " CPOL "\n"                                     \
+                 "s_waitcnt vmcnt(0)"                                                            \
+                 : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3])                            \
+                 : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3])                                    \
+                 : "memory")
+
+template <int POLICY>
+__device__ inline void loadFour(const uint64_t *const a[4], U64x2 v[4])
+{
+    if constexpr (POLICY == 4)
+        GUPS_LOAD4("sc0");
+    else if constexpr (POLICY == 5)
+        GUPS_LOAD4("sc0 nt");
+    else if constexpr (POLICY == 6)
+        GUPS_LOAD4("sc1 nt");
+    else
+        GUPS_LOAD4("nt");
+}
+
+template <int S, int POLICY>
+__global__ __launch_bounds__(256) void gupsAsmKernel(const uint64_t *buf, uint64_t nBuckets,
+                                                     uint64_t iters, uint64_t seed, uint64_t *out)
+{
+    constexpr int LPQ = S / 2;
+    const int lane = threadIdx.x & 63;
+    const int g = lane / LPQ, sub = lane % LPQ;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint64_t acc = 0;
+    for (uint64_t it = 0; it < iters; ++it)
+    {
+        const uint64_t *a[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+            const uint64_t h = mix(seed ^ (((wave * iters + it) * 4 + r) * 64 + g));
+            const uint64_t b = ((h >> 32) * nBuckets) >> 32;
+            a[r] = buf + b * S + sub * 2;
+        }
+        U64x2 v[4];
+        loadFour<POLICY>(a, v);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            acc ^= v[r].x ^ v[r].y;
+    }
+    if (acc == 0x0123456789ABCDEFull)
+        out[0] = acc;
+}
+
 template <int S, int ROUNDS, int POLICY = 0>
 __global__ __launch_bounds__(256) void gupsKernel(const uint64_t *buf, uint64_t nBuckets,
                                                   uint64_t iters, uint64_t seed, uint64_t *out)
@@ -94,12 +150,14 @@ extern "C" int hipibs_bench_gups(const void *d_buf, uint64_t n_buckets, uint32_t
 }
 
 // As hipibs_bench_gups with the load policy of gupsKernel (0..3; policies
-// 1..3 with 64 B buckets and 4 or 8 rounds only).
+// 1..3 with 64 B buckets and 4 or 8 rounds only) or gupsAsmKernel (4..7,
+// 64 B buckets, 4 rounds).
 extern "C" int hipibs_bench_gups_policy(const void *d_buf, uint64_t n_buckets,
                                         uint32_t bucket_slots, uint64_t n_reads, int rounds,
                                         int waves_per_cu, int policy, double *ms, void *stream)
 {
-    if (policy < 0 || policy > 3 || (policy && (bucket_slots != 8 || (rounds != 4 && rounds != 8))))
+    if (policy < 0 || policy > 7 || (policy && (bucket_slots != 8 || (rounds != 4 && rounds != 8))) ||
+        (policy >= 4 && rounds != 4))
         return -1;
     if (!d_buf || !ms || n_buckets == 0 || n_buckets >= (1ull << 32))
         return -1;
@@ -136,6 +194,15 @@ extern "C" int hipibs_bench_gups_policy(const void *d_buf, uint64_t n_buckets,
     GUPS_CASE(8, 4, 3)
     GUPS_CASE(8, 8, 3)
 #undef GUPS_CASE
+#define GUPS_ASM_CASE(PP)                                                                        \
+    if (bucket_slots == 8 && rounds == 4 && policy == PP)                                        \
+        hipLaunchKernelGGL((gupsAsmKernel<8, PP>), dim3(blocks), dim3(256), 0, s, buf,           \
+                           n_buckets, iters, 0x5EEDull, out);
+    GUPS_ASM_CASE(4)
+    GUPS_ASM_CASE(5)
+    GUPS_ASM_CASE(6)
+    GUPS_ASM_CASE(7)
+#undef GUPS_ASM_CASE
     (void)hipEventRecord(e1, s);
     hipError_t err = hipEventSynchronize(e1);
     float t = 0;

@@ -246,6 +246,8 @@ struct hipibs_table
     }
 };
 
+struct HostStaging;
+
 namespace hipibs
 {
 // Launch helpers implemented in the .hip files (all asynchronous).
@@ -266,6 +268,13 @@ uint32_t autoSegsPerWindow(uint32_t nWindows, int nTables, uint32_t maxPositions
 uint64_t scanWorkBytes(uint32_t nWindows, int nTables, uint32_t segsPerWindow);
 int kmcReadKeys(const std::string &prefix, uint64_t *keys, uint64_t cap, uint64_t *nOut,
                 unsigned threads, hipibs_kmc_info *infoOut);
+
+// Reusable staging for builds from host keys (pinned + device buffers of
+// chunkKeys keys each, and a stream).
+int hostStagingCreate(int device, uint64_t chunkKeys, HostStaging **out);
+void hostStagingFree(HostStaging *st);
+int buildTableFromHost(const uint64_t *hKeys, uint64_t n, uint32_t k, const hipibs_table_opts *opts,
+                       HostStaging *st, hipibs_table **out);
 
 // The raw records of a KMC suffix file, for readers that decode on the
 // device: record r of bucket j (lut[j] <= r < lut[j+1]) holds the key

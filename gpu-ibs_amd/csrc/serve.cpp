@@ -122,6 +122,7 @@ struct hipibs_serve
     // into tables while the worker keeps launching resident requests
     std::vector<std::thread> builders;
     std::deque<uint32_t> buildQueue;
+    std::deque<hipibs_table *> freeQueue; // evicted tables, freed by the builders
     std::vector<Built> built;
     std::mutex bmu;
     std::condition_variable bcv;
@@ -151,15 +152,17 @@ bool makeRoom(hipibs_serve *s, uint64_t need, uint64_t candHits, bool force)
                 victim = &kv.second;
         if (!victim || (!force && victim->hits > candHits))
             return false;
-        const auto t0 = Clock::now();
-        hipibs_table_free(victim->table);
-        const uint64_t us = (uint64_t)(ms(t0, Clock::now()) * 1000.0);
+        // freeing can wait for the device; the builders do it off this thread
+        {
+            std::lock_guard<std::mutex> lk(s->bmu);
+            s->freeQueue.push_back(victim->table);
+        }
+        s->bcv.notify_one();
         victim->table = nullptr;
         s->resident -= victim->bytes;
         s->residentTables -= 1;
         std::lock_guard<std::mutex> lk(s->mu);
         s->counters.evictions += 1;
-        s->counters.evict_us += us;
     }
     return true;
 }
@@ -224,23 +227,52 @@ int ensureResident(hipibs_serve *s, uint32_t id, bool async, bool force)
 void builderLoop(hipibs_serve *s)
 {
     (void)hipSetDevice(s->device);
+    HostStaging *staging = nullptr; // kept across builds
     for (;;)
     {
-        uint32_t id;
+        uint32_t id = 0;
+        bool haveBuild = false;
+        std::vector<hipibs_table *> toFree;
         {
             std::unique_lock<std::mutex> lk(s->bmu);
-            s->bcv.wait(lk, [s] { return s->stop || !s->buildQueue.empty(); });
+            s->bcv.wait(lk, [s] {
+                return s->stop || !s->buildQueue.empty() || !s->freeQueue.empty();
+            });
+            toFree.assign(s->freeQueue.begin(), s->freeQueue.end());
+            s->freeQueue.clear();
+            if (!s->stop && !s->buildQueue.empty())
+            {
+                id = s->buildQueue.front();
+                s->buildQueue.pop_front();
+                haveBuild = true;
+            }
+        }
+        if (!toFree.empty())
+        {
+            const auto t0 = Clock::now();
+            for (hipibs_table *t : toFree)
+                hipibs_table_free(t);
+            const uint64_t us = (uint64_t)(ms(t0, Clock::now()) * 1000.0);
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->counters.evict_us += us;
+        }
+        if (!haveBuild)
+        {
             if (s->stop)
-                return;
-            id = s->buildQueue.front();
-            s->buildQueue.pop_front();
+                break;
+            continue;
         }
         const Sample &sm = s->samples.at(id); // keys and n never change
         hipibs_table_opts o = s->topts;
         o.device = s->device;
         hipibs_table *t = nullptr;
         const auto t0 = Clock::now();
-        const int rc = hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &t);
+        int rc = 0;
+        if (!staging)
+            rc = hostStagingCreate(s->device, 32ull << 20, &staging);
+        if (!rc)
+            rc = o.minimizer_len ? hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &t)
+                                 : buildTableFromHost(sm.keys, sm.n, s->k, &o, staging, &t);
         const uint64_t us = (uint64_t)(ms(t0, Clock::now()) * 1000.0);
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -252,6 +284,7 @@ void builderLoop(hipibs_serve *s)
         }
         s->cvWork.notify_one();
     }
+    hostStagingFree(staging);
 }
 
 // Worker side of a finished build: the table becomes resident.
@@ -477,6 +510,9 @@ void destroy(hipibs_serve *s)
             b.join();
     for (auto &b : s->built)
         hipibs_table_free(b.table);
+    for (hipibs_table *t : s->freeQueue)
+        hipibs_table_free(t);
+    s->freeQueue.clear();
     (void)hipSetDevice(s->device);
     for (auto &sl : s->slots)
     {

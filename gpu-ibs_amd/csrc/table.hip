@@ -642,12 +642,13 @@ extern "C" int hipibs_table_build_keys_device(const uint64_t *d_keys, uint64_t n
     return 0;
 }
 
-// memcpy on up to 8 host threads (staging copies into pinned memory are
-// memory-bandwidth bound; one thread reaches a fraction of it).
+// memcpy on up to 4 host threads (staging copies into pinned memory are
+// memory-bandwidth bound; one thread reaches a fraction of it, and builds
+// may run on several threads at once).
 static void parallelCopy(void *dst, const void *src, uint64_t bytes)
 {
     const uint64_t piece = 16ull << 20;
-    const unsigned n = (unsigned)std::min<uint64_t>(8, (bytes + piece - 1) / piece);
+    const unsigned n = (unsigned)std::min<uint64_t>(4, (bytes + piece - 1) / piece);
     if (n <= 1)
     {
         std::memcpy(dst, src, bytes);
@@ -664,60 +665,124 @@ static void parallelCopy(void *dst, const void *src, uint64_t bytes)
         th.join();
 }
 
-// Streams host keys through two pinned staging buffers (copy of chunk i+1
-// overlaps the insert of chunk i).
-int hipibsBuildFromHostChunks(hipibs_table *t, const uint64_t *hKeys, uint64_t n)
+// Two pinned host buffers, two device buffers and a stream for staging host
+// keys into table builds; callers that build often keep one.
+struct HostStaging
 {
-    const uint64_t chunk = std::min<uint64_t>(n, 32ull << 20);
-    hipStream_t s;
-    HIPIBS_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int device = 0;
+    uint64_t chunk = 0; // keys per buffer
     uint64_t *pinned[2] = {nullptr, nullptr};
     uint64_t *dev[2] = {nullptr, nullptr};
     hipEvent_t done[2] = {nullptr, nullptr};
-    int rc = 0;
-    for (int i = 0; i < 2 && !rc; ++i)
+    hipStream_t stream = nullptr;
+};
+
+namespace hipibs
+{
+
+void hostStagingFree(HostStaging *st)
+{
+    if (!st)
+        return;
+    (void)hipSetDevice(st->device);
+    if (st->stream)
+        (void)hipStreamSynchronize(st->stream);
+    for (int i = 0; i < 2; ++i)
     {
-        if (hipHostMalloc(reinterpret_cast<void **>(&pinned[i]), chunk * 8) != hipSuccess ||
-            hipMalloc(reinterpret_cast<void **>(&dev[i]), chunk * 8) != hipSuccess ||
-            hipEventCreateWithFlags(&done[i], hipEventDisableTiming) != hipSuccess)
-            rc = setError(HIPIBS_E_NOMEM, "staging allocation failed");
+        if (st->pinned[i])
+            (void)hipHostFree(st->pinned[i]);
+        if (st->dev[i])
+            (void)hipFree(st->dev[i]);
+        if (st->done[i])
+            (void)hipEventDestroy(st->done[i]);
     }
+    if (st->stream)
+        (void)hipStreamDestroy(st->stream);
+    delete st;
+}
+
+int hostStagingCreate(int device, uint64_t chunkKeys, HostStaging **out)
+{
+    *out = nullptr;
+    HIPIBS_CHECK(hipSetDevice(device));
+    auto *st = new HostStaging();
+    st->device = device;
+    st->chunk = std::max<uint64_t>(chunkKeys, 1);
+    int rc = 0;
+    if (hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = setError(HIPIBS_E_HIP, "stream creation failed");
+    for (int i = 0; i < 2 && !rc; ++i)
+        if (hipHostMalloc(reinterpret_cast<void **>(&st->pinned[i]), st->chunk * 8) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void **>(&st->dev[i]), st->chunk * 8) != hipSuccess ||
+            hipEventCreateWithFlags(&st->done[i], hipEventDisableTiming) != hipSuccess)
+            rc = setError(HIPIBS_E_NOMEM, "staging allocation failed");
+    if (rc)
+    {
+        hostStagingFree(st);
+        return rc;
+    }
+    *out = st;
+    return 0;
+}
+
+// Streams host keys through the staging buffers into t (the copy of chunk
+// i+1 overlaps the upload and insert of chunk i) and reads back the build
+// counters.
+static int buildFromHostChunks(hipibs_table *t, const uint64_t *hKeys, uint64_t n,
+                               HostStaging *st)
+{
+    const uint64_t chunk = st->chunk;
+    hipStream_t s = st->stream;
+    int rc = 0;
     bool used[2] = {false, false};
     for (uint64_t off = 0, it = 0; off < n && !rc; off += chunk, ++it)
     {
         const int b = (int)(it & 1);
         const uint64_t m = std::min(chunk, n - off);
-        if (used[b] && hipEventSynchronize(done[b]) != hipSuccess)
+        if (used[b] && hipEventSynchronize(st->done[b]) != hipSuccess)
         {
             rc = setError(HIPIBS_E_HIP, "event wait failed");
             break;
         }
-        parallelCopy(pinned[b], hKeys + off, m * 8);
-        if (hipMemcpyAsync(dev[b], pinned[b], m * 8, hipMemcpyHostToDevice, s) != hipSuccess)
+        parallelCopy(st->pinned[b], hKeys + off, m * 8);
+        if (hipMemcpyAsync(st->dev[b], st->pinned[b], m * 8, hipMemcpyHostToDevice, s) !=
+            hipSuccess)
         {
             rc = setError(HIPIBS_E_HIP, "key upload failed");
             break;
         }
-        rc = launchTableInsert(t, dev[b], m, s);
-        if (!rc && hipEventRecord(done[b], s) != hipSuccess)
+        rc = launchTableInsert(t, st->dev[b], m, s);
+        if (!rc && hipEventRecord(st->done[b], s) != hipSuccess)
             rc = setError(HIPIBS_E_HIP, "event record failed");
         used[b] = true;
     }
     if (!rc)
         rc = finishTable(t, s);
-    hipStreamSynchronize(s);
-    for (int i = 0; i < 2; ++i)
-    {
-        if (pinned[i])
-            hipHostFree(pinned[i]);
-        if (dev[i])
-            hipFree(dev[i]);
-        if (done[i])
-            hipEventDestroy(done[i]);
-    }
-    hipStreamDestroy(s);
+    (void)hipStreamSynchronize(s);
     return rc;
 }
+
+// hipibs_table_build_keys with the caller's staging (its stream also clears
+// the new table, so no other stream is involved).
+int buildTableFromHost(const uint64_t *hKeys, uint64_t n, uint32_t k, const hipibs_table_opts *opts,
+                       HostStaging *st, hipibs_table **out)
+{
+    *out = nullptr;
+    hipibs_table *t = nullptr;
+    int rc = allocTable(n, k, opts, &t, st->stream);
+    if (rc)
+        return rc;
+    rc = buildFromHostChunks(t, hKeys, n, st);
+    if (rc)
+    {
+        hipibs_table_free(t);
+        return rc;
+    }
+    *out = t;
+    return 0;
+}
+
+} // namespace hipibs
 
 // Streams the records of a KMC database through two pinned staging buffers:
 // the host only reads raw records (parallel preads, 4-9 B per key instead of
@@ -844,20 +909,14 @@ extern "C" int hipibs_table_build_keys(const uint64_t *h_keys, uint64_t n, uint3
         hipFree(dAll);
         return rc;
     }
-    hipibs_table *t = nullptr;
-    int rc = allocTable(n, k, opts, &t, nullptr);
+    HostStaging *st = nullptr;
+    int rc = hostStagingCreate(opts ? opts->device : 0, std::min<uint64_t>(std::max<uint64_t>(n, 1),
+                                                                         32ull << 20), &st);
     if (rc)
         return rc;
-    rc = hipStreamSynchronize(nullptr) == hipSuccess ? 0 : setError(HIPIBS_E_HIP, "table clear failed");
-    if (!rc)
-        rc = hipibsBuildFromHostChunks(t, h_keys, n);
-    if (rc)
-    {
-        hipibs_table_free(t);
-        return rc;
-    }
-    *out = t;
-    return 0;
+    rc = buildTableFromHost(h_keys, n, k, opts, st, out);
+    hostStagingFree(st);
+    return rc;
 }
 
 extern "C" int hipibs_table_free(hipibs_table *t)

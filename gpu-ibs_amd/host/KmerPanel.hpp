@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <future>
 #include <cmath>
 #include <memory>
 
@@ -39,8 +40,8 @@ public:
         return buckets * g.bucketSlots * 8;
     }
 
-    // Groups of database indices, in order, each within the memory budget.
-    std::vector<std::vector<size_t>> planRounds() const
+    // Device bytes the panel's tables may take.
+    uint64_t tableBudget() const
     {
         const auto &g = KmerDatabase::gpuOptions();
         uint64_t freeB = 0, totalB = 0;
@@ -49,6 +50,15 @@ public:
         uint64_t budget = freeB > reserve ? freeB - reserve : 0;
         if (g.hbmBudget)
             budget = std::min(budget, g.hbmBudget);
+        return budget;
+    }
+
+    // Groups of database indices, in order, each within `budget` bytes
+    // (0 = the whole table budget).
+    std::vector<std::vector<size_t>> planRounds(uint64_t budget = 0) const
+    {
+        if (budget == 0)
+            budget = tableBudget();
         std::vector<std::vector<size_t>> rounds;
         std::vector<size_t> cur;
         uint64_t used = 0;
@@ -199,7 +209,12 @@ public:
         packReference(rw, g.device, dref);
         const double ingestSec = std::chrono::duration<double>(clk::now() - t0).count();
         reserveBytes_ = dref.bytes;
-        const auto rounds = planRounds();
+        // More than one memory group: plan them at half the budget, so the
+        // next group's tables can be built while the current group scans.
+        auto rounds = planRounds();
+        const bool overlap = rounds.size() > 1;
+        if (overlap)
+            rounds = planRounds(tableBudget() / 2);
         std::cout << "Panel: " << dbs_.size() << " databases in " << rounds.size()
                   << " device round(s); reference " << dref.nBases << " bases, "
                   << dref.nWindows << " windows, ingest " << ingestSec << " s" << std::endl;
@@ -207,24 +222,39 @@ public:
             tuneBucketWidth(dref);
         hipibs_scan_config cfg{0, 0, {0, 0}};
         bool tuned = false;
-        double buildSec = 0, scanSec = 0, writeSec = 0;
+        double buildSec = 0, waitSec = 0, scanSec = 0, writeSec = 0;
+        std::vector<std::vector<size_t>> todos;
         for (const auto &round : rounds)
         {
             std::vector<size_t> todo;
             for (size_t i : round)
                 if (!outPaths[i].empty())
                     todo.push_back(i);
-            if (todo.empty())
-                continue;
+            if (!todo.empty())
+                todos.push_back(todo);
+        }
+        // builds of group r+1 run on their own thread during the scan of r
+        auto buildGroup = [this](const std::vector<size_t> &todo) {
+            const auto tb = clk::now();
+            for (size_t i : todo)
+                dbs_[i]->initGpuIndex();
+            return std::chrono::duration<double>(clk::now() - tb).count();
+        };
+        std::future<double> nextBuild;
+        if (!todos.empty())
+            nextBuild = std::async(std::launch::async, buildGroup, todos[0]);
+        for (size_t r = 0; r < todos.size(); ++r)
+        {
+            const std::vector<size_t> &todo = todos[r];
             auto tb = clk::now();
+            buildSec += nextBuild.get(); // builder time (overlapped when grouped)
+            if (r + 1 < todos.size())
+                nextBuild = std::async(std::launch::async, buildGroup, todos[r + 1]);
             std::vector<const hipibs_table *> tabs;
             for (size_t i : todo)
-            {
-                dbs_[i]->initGpuIndex();
                 tabs.push_back(dbs_[i]->table());
-            }
             auto ts = clk::now();
-            buildSec += std::chrono::duration<double>(ts - tb).count();
+            waitSec += std::chrono::duration<double>(ts - tb).count();
             const int nt = (int)tabs.size();
             hipibs::DeviceBuffer stats(g.device, std::max<uint64_t>(1, (uint64_t)dref.nWindows * nt) *
                                                      sizeof(hipibs_window_stats));
@@ -285,7 +315,8 @@ public:
             writeSec += std::chrono::duration<double>(clk::now() - tw).count();
         }
         std::cout << "Panel phases: ingest " << ingestSec << " s, table builds " << buildSec
-                  << " s, scans " << scanSec << " s, write " << writeSec << " s" << std::endl;
+                  << " s (waited " << waitSec << " s" << (overlap ? ", overlapped with scans" : "")
+                  << "), scans " << scanSec << " s, write " << writeSec << " s" << std::endl;
     }
 
 private:
