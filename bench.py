@@ -1084,7 +1084,7 @@ def main():
         gbuf = torch.empty(info["bytes"], dtype=torch.uint8, device="cuda")
         S = info["bucket_slots"]
         # policy 0: plain loads; 1: non-temporal loads, as the probe issues them
-        for policy in ((0, 1) if S == 8 else (0,)):
+        for policy in (0, 1):
             for rounds in (4, 8):
                 for wpc in (16, 32):
                     t_ms = C.c_double(0)

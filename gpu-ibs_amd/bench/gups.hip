@@ -149,15 +149,15 @@ extern "C" int hipibs_bench_gups(const void *d_buf, uint64_t n_buckets, uint32_t
                                     0, ms, stream);
 }
 
-// As hipibs_bench_gups with the load policy of gupsKernel (0..3; policies
-// 1..3 with 64 B buckets and 4 or 8 rounds only) or gupsAsmKernel (4..7,
+// As hipibs_bench_gups with the load policy of gupsKernel (0..3, with 4 or
+// 8 rounds; policies 2..3 with 64 B buckets only) or gupsAsmKernel (4..7,
 // 64 B buckets, 4 rounds).
 extern "C" int hipibs_bench_gups_policy(const void *d_buf, uint64_t n_buckets,
                                         uint32_t bucket_slots, uint64_t n_reads, int rounds,
                                         int waves_per_cu, int policy, double *ms, void *stream)
 {
-    if (policy < 0 || policy > 7 || (policy && (bucket_slots != 8 || (rounds != 4 && rounds != 8))) ||
-        (policy >= 4 && rounds != 4))
+    if (policy < 0 || policy > 7 || (policy && (rounds != 4 && rounds != 8)) ||
+        (policy >= 2 && bucket_slots != 8) || (policy >= 4 && rounds != 4))
         return -1;
     if (!d_buf || !ms || n_buckets == 0 || n_buckets >= (1ull << 32))
         return -1;
@@ -189,6 +189,8 @@ extern "C" int hipibs_bench_gups_policy(const void *d_buf, uint64_t n_buckets,
     GUPS_CASE(16, 8, 0)
     GUPS_CASE(8, 4, 1)
     GUPS_CASE(8, 8, 1)
+    GUPS_CASE(16, 4, 1)
+    GUPS_CASE(16, 8, 1)
     GUPS_CASE(8, 4, 2)
     GUPS_CASE(8, 8, 2)
     GUPS_CASE(8, 4, 3)

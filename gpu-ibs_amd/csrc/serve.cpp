@@ -170,7 +170,7 @@ bool makeRoom(hipibs_serve *s, uint64_t need, uint64_t candHits, bool force)
 uint64_t tableBytesFor(const hipibs_serve *s, uint64_t n)
 {
     const double lf = s->topts.load_factor > 0 ? s->topts.load_factor : 0.6;
-    const uint32_t slots = s->topts.bucket_slots ? s->topts.bucket_slots : 8;
+    const uint32_t slots = s->topts.bucket_slots ? s->topts.bucket_slots : 16;
     const uint64_t buckets = std::max<uint64_t>(2, (uint64_t)std::ceil(std::ceil((double)std::max<uint64_t>(n, 1) / lf) / slots));
     return buckets * slots * 8;
 }

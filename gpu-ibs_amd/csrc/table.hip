@@ -398,7 +398,7 @@ static int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts,
     if (mlen && (mlen < 5 || mlen > 17 || mlen > k))
         return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (off) or 5..min(17, k)");
     if (o.bucket_slots == 0)
-        o.bucket_slots = mlen ? 16 : 8;
+        o.bucket_slots = 16; // one 128 B bucket = one memory request on gfx950
     if (o.bucket_slots != 8 && o.bucket_slots != 16)
         return setError(HIPIBS_E_ARG, "bucket_slots must be 8 or 16");
     if (!(o.load_factor > 0 && o.load_factor < 0.98))

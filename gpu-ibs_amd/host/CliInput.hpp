@@ -165,7 +165,7 @@ inline const char *optionsUsage()
            "  --batch-mb=N          bases per streamed batch, in MiB (default 128)\n"
            "  --streams=N           batches in flight (default 3)\n"
            "  --load-factor=F       device table load factor (default 0.6)\n"
-           "  --bucket-bytes=64|128 device table bucket width (default 64; a panel\n"
+           "  --bucket-bytes=64|128 device table bucket width (default 128; a panel\n"
            "                        times both unless this is given)\n"
            "  --hbm-budget-gb=N     cap on device table memory\n"
            "  --db=<folder>         (fastibs) another sample in the same reference pass\n"

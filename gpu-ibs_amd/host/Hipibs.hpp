@@ -29,7 +29,7 @@ inline hipibs_table_opts defaultOpts()
 {
     hipibs_table_opts o{};
     o.load_factor = 0.6;
-    o.bucket_slots = 8;
+    o.bucket_slots = 16;
     return o;
 }
 

@@ -34,7 +34,7 @@ public:
         uint64_t batchBases = 128ull << 20;
         int depth = 3;
         double loadFactor = 0.6;
-        uint32_t bucketSlots = 8;
+        uint32_t bucketSlots = 16;
         bool bucketAuto = true; // a panel may time 64 B vs 128 B buckets
         uint64_t hbmBudget = 0;
         unsigned loadThreads = 0;

@@ -52,7 +52,7 @@ typedef struct hipibs_scan hipibs_scan;
  * length. */
 typedef struct hipibs_table_opts {
     double load_factor;     /* keys / slots, default 0.6 */
-    uint32_t bucket_slots;  /* 8 (64 B bucket) or 16 (128 B bucket), default 8 */
+    uint32_t bucket_slots;  /* 16 (128 B bucket, one memory request; default) or 8 (64 B) */
     int device;             /* HIP device ordinal, default 0 */
     uint64_t hbm_budget;    /* max table bytes, 0 = no limit */
     uint32_t load_threads;  /* host threads of a KMC read, 0 = all CPUs */
