@@ -1095,11 +1095,13 @@ def main():
                                                        C.byref(t_ms), None)
                     reads = blib.hipibs_bench_gups_reads(S, probes, rounds, wpc)
                     if rc == 0 and t_ms.value > 0:
+                        # G bucket reads per second (each read = one probe's request)
                         gups[f"{'nt_' if policy else ''}r{rounds}_w{wpc}"] = \
-                            reads * S * 8 / (t_ms.value / 1e3) / 1e9
+                            reads / (t_ms.value / 1e3) / 1e9
         del gbuf
         torch.cuda.empty_cache()
-        log(f"random {S*8} B-sector ceiling (GB/s): {gups}")
+        log(f"random {S*8} B-bucket read ceiling (G reads/s): "
+            f"{ {k: round(v, 1) for k, v in gups.items()} }")
     except Exception as e:
         log(f"gups ceiling skipped: {e}")
 
@@ -1166,11 +1168,11 @@ def main():
                      "kernel": f"scanKernel<{info['bucket_slots']}>", "kernel_ms": k_ms,
                      "algorithmic_bytes": alg_bytes,
                      "probe_rate_gps": probes / (k_ms / 1e3) / 1e9,
-                     "measured_random_sector_ceiling_gbs": max(gups.values()) if gups else None,
+                     "measured_random_read_ceiling_gps": max(gups.values()) if gups else None,
                      "frac_of_measured_random_ceiling":
-                         (alg_bytes / (k_ms / 1e3) / 1e9) / max(gups.values()) if gups else None,
-                     "hbm_line_bytes_per_probe": 128,
-                     "gups_sweep_gbs": gups},
+                         (probes / (k_ms / 1e3) / 1e9) / max(gups.values()) if gups else None,
+                     "hbm_request_bytes_per_probe": 128,
+                     "gups_sweep_g_reads_per_s": gups},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

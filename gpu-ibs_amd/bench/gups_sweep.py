@@ -1,9 +1,13 @@
-"""Random-bucket read rate vs working-set size (measurement only).
+"""Random-bucket read rate vs working-set size and load policy (measurement only).
 
-    python3 -u gpu-ibs_amd/bench/gups_sweep.py
+    python3 -u gpu-ibs_amd/bench/gups_sweep.py            # sizes x bucket widths, plain loads
+    python3 -u gpu-ibs_amd/bench/gups_sweep.py policies   # load policies at 27 GB and 256 MB
 
-Drives hipibs_bench_gups (gups.hip) over device buffers from 2 MB to 27 GB,
-with 64 B and 128 B buckets and 4 or 8 loads in flight per wave."""
+Drives hipibs_bench_gups_policy (gups.hip) over device buffers from 2 MB to
+27 GB, with 64 B and 128 B buckets and 4 or 8 loads in flight per wave.
+Policies: 0 plain, 1 non-temporal (as the table probe loads), 2 / 3 relaxed
+8 B atomic loads at system / device scope, 4 sc0, 5 sc0 nt, 6 sc1 nt, 7 nt
+written in asm (a control for 1)."""
 import ctypes as C
 import os
 import sys
@@ -13,22 +17,40 @@ sys.path.insert(0, os.path.dirname(HERE))
 import hipibs  # noqa: E402
 
 b = C.CDLL(os.path.join(os.path.dirname(HERE), "lib", "libhipibs_bench.so"))
-b.hipibs_bench_gups.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_int, C.c_int,
-                                C.POINTER(C.c_double), C.c_void_p]
+b.hipibs_bench_gups_policy.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_int,
+                                       C.c_int, C.c_int, C.POINTER(C.c_double), C.c_void_p]
 b.hipibs_bench_gups_reads.argtypes = [C.c_uint32, C.c_uint64, C.c_int, C.c_int]
 b.hipibs_bench_gups_reads.restype = C.c_uint64
+N = 500_000_000
 
-for mb in [2, 16, 64, 128, 192, 256, 384, 512, 1024, 4096, 27000]:
-    buf = hipibs.DeviceBuffer.zeros(mb << 20)
-    for S in (8, 16):
-        nb = (mb << 20) // (S * 8)
-        for rounds in (4, 8):
-            t = C.c_double(0)
-            n = 500_000_000
-            b.hipibs_bench_gups(buf.ptr, nb, S, n, rounds, 32, C.byref(t), None)  # warm-up
-            b.hipibs_bench_gups(buf.ptr, nb, S, n, rounds, 32, C.byref(t), None)
-            reads = b.hipibs_bench_gups_reads(S, n, rounds, 32)
-            print(f"size {mb:6d} MB bucket {S * 8:3d} B rounds {rounds}: "
-                  f"{reads / (t.value / 1e3) / 1e9:7.1f} G buckets/s  "
-                  f"{reads * S * 8 / (t.value / 1e3) / 1e9:7.0f} GB/s", flush=True)
-    buf.free()
+
+def rate(buf, mb, S, rounds, policy, wpc=32):
+    nb = (mb << 20) // (S * 8)
+    t = C.c_double(0)
+    for _ in range(2):  # warm-up, then the measurement
+        rc = b.hipibs_bench_gups_policy(buf.ptr, nb, S, N, rounds, wpc, policy, C.byref(t), None)
+    if rc != 0:
+        return None
+    reads = b.hipibs_bench_gups_reads(S, N, rounds, wpc)
+    return reads / (t.value / 1e3) / 1e9, reads * S * 8 / (t.value / 1e3) / 1e9
+
+
+if len(sys.argv) > 1 and sys.argv[1] == "policies":
+    for mb in (256, 27000):
+        buf = hipibs.DeviceBuffer.zeros(mb << 20)
+        for policy in range(8):
+            for S in (8, 16):
+                r = rate(buf, mb, S, 4, policy)
+                if r:
+                    print(f"size {mb:6d} MB bucket {S * 8:3d} B policy {policy}: "
+                          f"{r[0]:7.1f} G buckets/s", flush=True)
+        buf.free()
+else:
+    for mb in [2, 16, 64, 128, 192, 256, 384, 512, 1024, 4096, 27000]:
+        buf = hipibs.DeviceBuffer.zeros(mb << 20)
+        for S in (8, 16):
+            for rounds in (4, 8):
+                r = rate(buf, mb, S, rounds, 0)
+                print(f"size {mb:6d} MB bucket {S * 8:3d} B rounds {rounds}: "
+                      f"{r[0]:7.1f} G buckets/s  {r[1]:7.0f} GB/s", flush=True)
+        buf.free()
