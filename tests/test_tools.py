@@ -79,12 +79,15 @@ def test_fastibs_matches_golden_and_skips_existing(small_layout, tmp_path):
 
 
 @pytest.mark.gpu
-def test_fastibs_panel_equals_singles(small_layout, tmp_path):
+@pytest.mark.parametrize("resident", [2, 1])
+def test_fastibs_panel_equals_singles(small_layout, tmp_path, resident):
+    """Panel runs in 2 or 3 memory groups (the next group's tables built
+    while the current one scans) give the single-sample TSVs."""
     out = tmp_path / "panel"
     lst = tmp_path / "list.txt"
     lst.write_text(f"# panel\n{small_layout / 's2'}\n")
     r = tool("fastibs", small_layout / "s0", small_layout / "refs", out, 50000,
-             f"--db={small_layout / 's1'}", f"--db-list={lst}", "--resident=2")
+             f"--db={small_layout / 's1'}", f"--db-list={lst}", f"--resident={resident}")
     assert r.returncode == 0, r.stderr
     for i in range(3):
         got = (out / f"s{i}_v_small_50000.tsv").read_bytes()
