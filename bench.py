@@ -75,8 +75,8 @@ def parse():
     ap.add_argument("--e2e", type=int, default=1,
                     help="large: also run fastibs / fastibsmapper end to end from files")
     ap.add_argument("--panel-samples", type=int, default=48)
-    ap.add_argument("--panel-min-keys", type=float, default=0.02e9)
-    ap.add_argument("--panel-max-keys", type=float, default=0.25e9)
+    ap.add_argument("--panel-min-keys", type=float, default=0.05e9)
+    ap.add_argument("--panel-max-keys", type=float, default=0.625e9)
     ap.add_argument("--panel-groups", type=int, default=3,
                     help="panel: HBM budget = the panel's table bytes / this, so the panel "
                          "runs in at least this many memory-budgeted groups")
