@@ -27,6 +27,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -52,6 +53,7 @@ struct Request
 struct Sample
 {
     const uint64_t *keys = nullptr;
+    std::string kmcPrefix; // set: the table is built from this KMC database
     uint64_t n = 0;
     hipibs_table *table = nullptr;
     uint64_t bytes = 0;    // bytes counted against the budget (reserved while building)
@@ -209,7 +211,8 @@ int ensureResident(hipibs_serve *s, uint32_t id, bool async, bool force)
     }
     hipibs_table_opts o = s->topts;
     o.device = s->device;
-    int rc = hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &sm.table);
+    int rc = sm.kmcPrefix.empty() ? hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &sm.table)
+                                  : hipibs_table_build_kmc(sm.kmcPrefix.c_str(), &o, &sm.table);
     if (rc)
     {
         s->resident -= need;
@@ -271,8 +274,14 @@ void builderLoop(hipibs_serve *s)
         if (!staging)
             rc = hostStagingCreate(s->device, 32ull << 20, &staging);
         if (!rc)
-            rc = o.minimizer_len ? hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &t)
-                                 : buildTableFromHost(sm.keys, sm.n, s->k, &o, staging, &t);
+        {
+            if (!sm.kmcPrefix.empty())
+                rc = hipibs_table_build_kmc(sm.kmcPrefix.c_str(), &o, &t);
+            else if (o.minimizer_len)
+                rc = hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &t);
+            else
+                rc = buildTableFromHost(sm.keys, sm.n, s->k, &o, staging, &t);
+        }
         const uint64_t us = (uint64_t)(ms(t0, Clock::now()) * 1000.0);
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -595,21 +604,15 @@ extern "C" int hipibs_serve_set_reference(hipibs_serve *s, const uint64_t *h_seq
     return 0;
 }
 
-extern "C" int hipibs_serve_add_sample(hipibs_serve *s, uint32_t sample_id, const uint64_t *h_keys,
-                                       uint64_t n_keys, int preload)
+static int addSample(hipibs_serve *s, uint32_t sample_id, Sample sm, int preload)
 {
-    if (!s || (n_keys && !h_keys))
-        return setError(HIPIBS_E_ARG, "null argument");
     if (!s->dSeq)
         return setError(HIPIBS_E_ARG, "set the reference first");
     if (s->worker.joinable())
         return setError(HIPIBS_E_ARG, "samples are registered before the first request");
     if (s->samples.count(sample_id))
         return setError(HIPIBS_E_ARG, "sample id already registered");
-    Sample sm;
-    sm.keys = h_keys;
-    sm.n = n_keys;
-    s->samples[sample_id] = sm;
+    s->samples[sample_id] = std::move(sm);
     if (preload)
     {
         const int rc = ensureResident(s, sample_id, false, true);
@@ -619,6 +622,34 @@ extern "C" int hipibs_serve_add_sample(hipibs_serve *s, uint32_t sample_id, cons
             return setError(HIPIBS_E_NOMEM, "preloaded samples exceed the HBM budget");
     }
     return 0;
+}
+
+extern "C" int hipibs_serve_add_sample(hipibs_serve *s, uint32_t sample_id, const uint64_t *h_keys,
+                                       uint64_t n_keys, int preload)
+{
+    if (!s || (n_keys && !h_keys))
+        return setError(HIPIBS_E_ARG, "null argument");
+    Sample sm;
+    sm.keys = h_keys;
+    sm.n = n_keys;
+    return addSample(s, sample_id, std::move(sm), preload);
+}
+
+extern "C" int hipibs_serve_add_sample_kmc(hipibs_serve *s, uint32_t sample_id,
+                                           const char *kmc_prefix, int preload)
+{
+    if (!s || !kmc_prefix)
+        return setError(HIPIBS_E_ARG, "null argument");
+    hipibs_kmc_info info{};
+    int rc = hipibs_kmc_header(kmc_prefix, &info);
+    if (rc)
+        return rc;
+    if (info.kmer_length != s->k)
+        return setError(HIPIBS_E_ARG, "the KMC database's k differs from the engine's");
+    Sample sm;
+    sm.kmcPrefix = kmc_prefix;
+    sm.n = info.total_kmers;
+    return addSample(s, sample_id, std::move(sm), preload);
 }
 
 extern "C" int hipibs_serve_submit(hipibs_serve *s, uint64_t request_id, uint32_t sample_id,

@@ -136,6 +136,7 @@ _SIGS = [
                                       C.c_int, C.c_uint32, C.POINTER(vp)]),
     ("hipibs_serve_set_reference", C.c_int, [vp, u64p, u64p, C.c_uint64]),
     ("hipibs_serve_add_sample", C.c_int, [vp, C.c_uint32, u64p, C.c_uint64, C.c_int]),
+    ("hipibs_serve_add_sample_kmc", C.c_int, [vp, C.c_uint32, C.c_char_p, C.c_int]),
     ("hipibs_serve_submit", C.c_int, [vp, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64,
                                       C.c_uint32]),
     ("hipibs_serve_poll", C.c_int, [vp, u64p, C.c_uint32, C.c_int, u32p]),
@@ -637,6 +638,10 @@ class ServeEngine:
         self._keep.append(keys)
         _check(_lib.hipibs_serve_add_sample(self.handle, sample_id, _ptr(keys), keys.size,
                                             int(preload)), "hipibs_serve_add_sample")
+
+    def add_sample_kmc(self, sample_id: int, kmc_prefix: str, preload: bool = False) -> None:
+        _check(_lib.hipibs_serve_add_sample_kmc(self.handle, sample_id, kmc_prefix.encode(),
+                                                int(preload)), "hipibs_serve_add_sample_kmc")
 
     def submit(self, request_id: int, sample_id: int, base: int, n_bases: int, W: int) -> None:
         _check(_lib.hipibs_serve_submit(self.handle, request_id, sample_id, base, n_bases, W),

@@ -363,6 +363,13 @@ int hipibs_serve_set_reference(hipibs_serve *s, const uint64_t *h_seq_words,
 int hipibs_serve_add_sample(hipibs_serve *s, uint32_t sample_id, const uint64_t *h_keys,
                             uint64_t n_keys, int preload);
 
+/* Registers a sample by its KMC database (the prefix of .kmc_pre/.kmc_suf,
+ * which must stay readable): admissions stream its raw records through the
+ * device decode, so the host keeps no key array. Same rules as
+ * hipibs_serve_add_sample; the database's k must be the engine's. */
+int hipibs_serve_add_sample_kmc(hipibs_serve *s, uint32_t sample_id, const char *kmc_prefix,
+                                int preload);
+
 /* Queues a scan of bases [base, base+n_bases) of the reference (one
  * contig's range) in windows of W (chunkSequences geometry within the
  * range) against one sample. Returns at once. */

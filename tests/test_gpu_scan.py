@@ -462,11 +462,12 @@ def test_panel_graph_replays_and_autotune(small_corpus):
         t.free()
 
 
-def test_serving_results_same_alone_and_in_the_mix(small_corpus):
+def test_serving_results_same_alone_and_in_the_mix(small_corpus, tmp_path):
     """Continuous batching: every request's statistics are identical when
     served alone and when served mixed with other samples, ranges and
     window sizes, with an HBM budget that forces evictions and
-    re-admissions; and equal to the oracle."""
+    re-admissions (one sample registered by its KMC database in the mix);
+    and equal to the oracle."""
     import time
     ref, samples = small_corpus
     seq = corpus.to_text(ref)
@@ -483,11 +484,18 @@ def test_serving_results_same_alone_and_in_the_mix(small_corpus):
         base = int(rng.integers(0, n - ln))
         reqs.append((i, sid, base, ln, W))
 
-    def serve(batch, budget, burst):
+    kmc2 = str(tmp_path / "s2")
+    hipibs.kmc_write(kmc2, keys[2], 25, p=9, sig_len=9, counter_size=1, version=0x200,
+                     min_count=1, max_count=255)
+
+    def serve(batch, budget, burst, via_kmc=False):
         eng = hipibs.ServeEngine(25, budget, max_inflight=3, max_windows_per_launch=512)
         eng.set_reference(sw, ew, n)
         for sid in range(3):
-            eng.add_sample(sid, keys[sid])
+            if via_kmc and sid == 2:
+                eng.add_sample_kmc(sid, kmc2)
+            else:
+                eng.add_sample(sid, keys[sid])
         out = {}
         pending = set()
         for j, (rid, sid, base, ln, W) in enumerate(batch):
@@ -509,7 +517,7 @@ def test_serving_results_same_alone_and_in_the_mix(small_corpus):
     for r in reqs:
         o, _ = serve([r], 3 * one_table, False)
         alone.update(o)
-    mixed, c = serve(reqs, int(1.5 * one_table), True)
+    mixed, c = serve(reqs, int(1.5 * one_table), True, via_kmc=True)
     assert c["evictions"] > 0 and c["admissions"] > 3
     assert c["launches"] < len(reqs)  # requests shared launches
     for rid, sid, base, ln, W in reqs:
