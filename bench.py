@@ -50,7 +50,7 @@ def parse():
                          "whole-k-mer part (0 = library default)")
     ap.add_argument("--load-factor", type=float, default=0.6)
     ap.add_argument("--segs", type=int, default=0, help="segments per window (0 = auto)")
-    ap.add_argument("--cpu-mbp", type=float, default=256.0,
+    ap.add_argument("--cpu-mbp", type=float, default=1536.0,
                     help="reference slice (Mbp) of the CPU baseline sample; 0 = skip")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0x1B50200)
