@@ -15,6 +15,7 @@
 #include <charconv>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
@@ -22,6 +23,7 @@
 #include <iostream>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 class KmerDatabase
@@ -115,6 +117,120 @@ public:
         so.batchBases = std::max<uint64_t>(1 << 16, window.end - window.start + 128);
         so.depth = 1;
         return hipibs::scanWindowsStreamed({table_.get()}, one, so)[0][0];
+    }
+
+    // ---- the reference's process-wide switches, kept for its callers ----
+
+    // k > 32 needs the reference's string path, which this build does not
+    // have: the constructor rejects such databases.
+    static bool needsStringPath(uint32_t k) { return k > 32; }
+
+    // Host threads for parsing, packing and KMC reads: FASTIBS_THREADS=N,
+    // else every CPU this process may use.
+    static unsigned poolThreads()
+    {
+        if (const char *v = std::getenv("FASTIBS_THREADS"))
+        {
+            const long n = std::atol(v);
+            if (n > 0)
+                return (unsigned)n;
+        }
+        return seqio_detail::hostThreads();
+    }
+
+    // One device per process (the --device ordinal).
+    static int devicesToUse(int available) { return available > 0 ? 1 : 0; }
+
+    // The device path is the only path: there is no CPU probe and no
+    // separate synchronous device path.
+    static bool gpuRequested() { return true; }
+    static bool gpuSyncRequested() { return false; }
+
+    // One window against several databases (panel mode's per-window call):
+    // out[i] = dbs[i]->getStatsFromSequence(window), from one device scan
+    // over all the databases' tables (all of one k).
+    static void getStatsFromSequenceMulti(const SeqWindow &window,
+                                          const std::vector<KmerDatabase *> &dbs, Window *out)
+    {
+        if (dbs.empty())
+            return;
+        std::vector<const hipibs_table *> tabs;
+        for (KmerDatabase *d : dbs)
+        {
+            d->initGpuIndex();
+            tabs.push_back(d->table());
+        }
+        std::vector<SeqWindow> one{window};
+        hipibs::ScanOptions so;
+        so.batchBases = std::max<uint64_t>(1 << 16, window.end - window.start + 128);
+        so.depth = 1;
+        const auto res = hipibs::scanWindowsStreamed(tabs, one, so);
+        for (size_t i = 0; i < dbs.size(); ++i)
+            out[i] = res[i][0];
+    }
+
+    // A mapper chunk's hits into its contig's difference array: +1 at every
+    // hit position p (contig coordinates, chunk.start + offset) and -1 at
+    // p + k, added atomically (chunks of one contig may run concurrently).
+    void accumulateMappingChunk(const SeqWindow &chunk, std::vector<int> &diff)
+    {
+        const std::vector<uint32_t> words = hitWords(chunk.seq);
+        const size_t npos = chunk.seq.size() >= kmerSize ? chunk.seq.size() - kmerSize + 1 : 0;
+        for (size_t i = 0; i < npos; ++i)
+            if ((words[i >> 5] >> (i & 31)) & 1u)
+            {
+                const size_t p = chunk.start + i;
+                __atomic_fetch_add(&diff[p], 1, __ATOMIC_RELAXED);
+                __atomic_fetch_sub(&diff[p + kmerSize], 1, __ATOMIC_RELAXED);
+            }
+    }
+
+    // The reference's coverage line from a difference array: prefix sums
+    // joined by ',' (to_chars), no trailing separator.
+    static std::string serializeCoverage(const std::vector<int> &diff, size_t seqSize)
+    {
+        std::string out;
+        out.reserve(2 * seqSize);
+        char buf[12];
+        int coverage = 0;
+        for (size_t i = 0; i < seqSize; ++i)
+        {
+            coverage += diff[i];
+            const auto r = std::to_chars(buf, buf + sizeof(buf), coverage);
+            out.append(buf, r.ptr);
+            out.push_back(',');
+        }
+        if (!out.empty())
+            out.pop_back();
+        return out;
+    }
+
+    // One contig's (id, coverage line), hit mask, coverage and text made on
+    // the device.
+    std::pair<std::string, std::string> getMappingFromSequence(const std::string &id,
+                                                               const std::string &sequence)
+    {
+        initGpuIndex();
+        const int dev = gpuOptions().device;
+        const uint64_t n = sequence.size();
+        if (n == 0)
+            return {id, std::string()};
+        const uint64_t npos = n >= kmerSize ? n - kmerSize + 1 : 0;
+        const std::vector<uint32_t> words = hitWords(sequence);
+        hipibs::DeviceBuffer dhit(dev, std::max<uint64_t>(1, words.size()) * 4), dcov(dev, n),
+            dtxt(dev, 4 * n);
+        if (!words.empty())
+            hipibs::check(hipibs_memcpy(dhit.get(), words.data(), words.size() * 4, 1), "upload");
+        hipibs::check(hipibs_coverage_device(dhit.as<uint32_t>(), npos, n, kmerSize,
+                                             dcov.as<uint8_t>(), nullptr),
+                      "coverage");
+        uint64_t len = 0;
+        hipibs::check(hipibs_coverage_text_device(dcov.as<uint8_t>(), n, dtxt.as<char>(), 4 * n,
+                                                  &len, nullptr),
+                      "coverage text");
+        std::string text(len, '\0');
+        hipibs::check(hipibs_memcpy(text.data(), dtxt.get(), len, 2), "download");
+        return {id, std::move(text)};
     }
 
     struct ReferenceWindows
@@ -381,6 +497,32 @@ public:
     uint32_t kmerSize = 0;
 
 private:
+    // Hit mask of every position of seq (bit i of word i/32: the k-mer at
+    // i is valid and in the table), from the device.
+    std::vector<uint32_t> hitWords(std::string_view seq)
+    {
+        initGpuIndex();
+        const int dev = gpuOptions().device;
+        const uint64_t n = seq.size();
+        const uint64_t npos = n >= kmerSize ? n - kmerSize + 1 : 0;
+        std::vector<uint32_t> words((npos + 31) / 32, 0);
+        if (npos == 0)
+            return words;
+        const uint64_t nsw = (n + 31) / 32 + HIPIBS_PACK_PAD_WORDS;
+        const uint64_t new_ = (n + 63) / 64 + HIPIBS_PACK_PAD_WORDS;
+        std::vector<uint64_t> sw(nsw, 0), ew(new_, 0);
+        hipibs::check(hipibs_pack_bases(seq.data(), n, 0, sw.data(), ew.data()), "pack");
+        hipibs::DeviceBuffer dsw(dev, nsw * 8), dew(dev, new_ * 8), dhit(dev, (words.size() + 1) * 4);
+        hipibs::check(hipibs_memcpy(dsw.get(), sw.data(), nsw * 8, 1), "upload");
+        hipibs::check(hipibs_memcpy(dew.get(), ew.data(), new_ * 8, 1), "upload");
+        hipibs::check(hipibs_hitmask_device(table_.get(), dsw.as<uint64_t>(), dew.as<uint64_t>(), 0,
+                                            npos, n, dhit.as<uint32_t>(), nullptr),
+                      "hit mask");
+        hipibs::check(hipibs_dev_synchronize(dev), "hit mask");
+        hipibs::check(hipibs_memcpy(words.data(), dhit.get(), words.size() * 4, 2), "download");
+        return words;
+    }
+
     std::string sourcePath_;
     hipibs_kmc_info info_{};
     hipibs::TablePtr table_;

@@ -113,6 +113,28 @@ def test_kdbintersect_matches_golden(small_layout, expected):
         assert r.stdout == f"{want}\n"
 
 
+@pytest.mark.gpu
+def test_kmerdatabase_members_match_golden(small_layout, tmp_path, expected):
+    """The KmerDatabase members the tools do not call (per-contig mapping,
+    chunked mapping into a difference array + serializeCoverage, the
+    per-window multi-database and single-database statistics) give the
+    golden coverage file and TSVs."""
+    out = tmp_path / "api"
+    out.mkdir()
+    dbs = [small_layout / f"s{i}" / "sample" for i in range(3)]
+    r = tool("test_api", small_layout / "refs" / "small_genome.fasta", out, 50000, *dbs)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert ("needsStringPath(25) 0, needsStringPath(33) 1, devicesToUse(4) 1, gpuRequested 1, "
+            "gpuSyncRequested 0, poolThreads>0 1") in r.stdout
+    want_cov = expected["small"]["s0"]["coverage_sha256"]
+    assert sha256((out / "mapping.txt").read_bytes()) == want_cov
+    assert sha256((out / "mapping_chunks.txt").read_bytes()) == want_cov
+    for i in range(3):
+        want = open(os.path.join(ROOT, "tests", "golden", f"small_s{i}_50000.tsv"), "rb").read()
+        assert (out / f"multi_{i}.tsv").read_bytes() == want, i
+        assert (out / f"single_{i}.tsv").read_bytes() == want, i
+
+
 def test_cpp_host_layer_checks():
     r = tool("test_host")
     assert r.returncode == 0, r.stdout + r.stderr
