@@ -7,7 +7,10 @@
 //                             threads + serializeCoverage(diff, n)
 //   <out>/multi_<i>.tsv       getStatsFromSequenceMulti per window, database i's rows
 //   <out>/single_<i>.tsv      getStatsFromSequence per window
-// and prints the process-wide switches.
+// and prints the process-wide switches and, for DeviceKmerIndex built from
+// the second database, probeCountHost / probeHost of the first database's
+// keys.
+#include "DeviceKmerIndex.hpp"
 #include "KmerDatabase.hpp"
 
 #include <atomic>
@@ -82,6 +85,24 @@ int main(int argc, char **argv)
         {
             KmerDatabase::writeStats(out + "/multi_" + std::to_string(i) + ".tsv", multi[i]);
             KmerDatabase::writeStats(out + "/single_" + std::to_string(i) + ".tsv", single[i]);
+        }
+        if (dbs.size() > 1)
+        {
+            auto shards = DeviceKmerIndex::buildShards(argv[5], 1);
+            DeviceKmerIndex &idx = *shards[0];
+            std::vector<uint64_t> q(dbs[0]->totalKmers());
+            uint64_t nq = 0;
+            hipibs::check(hipibs_kmc_read_keys(argv[4], q.data(), q.size(), &nq, 0), "read keys");
+            q.resize(nq);
+            std::vector<uint8_t> hits;
+            idx.probeHost(q, hits);
+            uint64_t sum = 0;
+            for (uint8_t h : hits)
+                sum += h;
+            std::cout << "intersect 0-1: probeCountHost " << idx.probeCountHost(q)
+                      << " probeHost " << sum << " devices " << DeviceKmerIndex::deviceCount()
+                      << " footprint>0 " << (idx.footprintBytes() > 0) << " freeMemory>0 "
+                      << (DeviceKmerIndex::freeMemory(idx.device()) > 0) << std::endl;
         }
         std::cout << "needsStringPath(" << k << ") " << KmerDatabase::needsStringPath(k)
                   << ", needsStringPath(33) " << KmerDatabase::needsStringPath(33)

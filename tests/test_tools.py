@@ -118,7 +118,8 @@ def test_kmerdatabase_members_match_golden(small_layout, tmp_path, expected):
     """The KmerDatabase members the tools do not call (per-contig mapping,
     chunked mapping into a difference array + serializeCoverage, the
     per-window multi-database and single-database statistics) give the
-    golden coverage file and TSVs."""
+    golden coverage file and TSVs; DeviceKmerIndex's probeHost and
+    probeCountHost give the golden intersect count."""
     out = tmp_path / "api"
     out.mkdir()
     dbs = [small_layout / f"s{i}" / "sample" for i in range(3)]
@@ -126,6 +127,8 @@ def test_kmerdatabase_members_match_golden(small_layout, tmp_path, expected):
     assert r.returncode == 0, r.stdout + r.stderr
     assert ("needsStringPath(25) 0, needsStringPath(33) 1, devicesToUse(4) 1, gpuRequested 1, "
             "gpuSyncRequested 0, poolThreads>0 1") in r.stdout
+    n01 = expected["small"]["intersect"]["0-1"]
+    assert f"intersect 0-1: probeCountHost {n01} probeHost {n01} devices 1" in r.stdout
     want_cov = expected["small"]["s0"]["coverage_sha256"]
     assert sha256((out / "mapping.txt").read_bytes()) == want_cov
     assert sha256((out / "mapping_chunks.txt").read_bytes()) == want_cov
