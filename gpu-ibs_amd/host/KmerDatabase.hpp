@@ -365,45 +365,6 @@ public:
                   << st.waitSec << " s), write " << writeSec << " s" << std::endl;
     }
 
-    // Coverage line: values joined by ',' (serializeCoverage,
-    // src/KmerDatabase.hpp:756-772), built on host threads.
-    static std::string serializeCoverage(const uint8_t *cov, size_t n)
-    {
-        if (n == 0)
-            return std::string();
-        const unsigned T = std::max(1u, std::min(std::thread::hardware_concurrency(), 64u));
-        const size_t per = (n + T - 1) / T;
-        std::vector<std::string> parts(T);
-        auto work = [&](unsigned i) {
-            const size_t lo = std::min(n, (size_t)i * per), hi = std::min(n, lo + per);
-            std::string &s = parts[i];
-            s.reserve((hi - lo) * 3);
-            for (size_t j = lo; j < hi; ++j)
-            {
-                const unsigned v = cov[j];
-                if (v >= 100)
-                    s.push_back((char)('0' + v / 100));
-                if (v >= 10)
-                    s.push_back((char)('0' + (v / 10) % 10));
-                s.push_back((char)('0' + v % 10));
-                s.push_back(',');
-            }
-        };
-        std::vector<std::thread> pool;
-        for (unsigned i = 0; i < T; ++i)
-            pool.emplace_back(work, i);
-        for (auto &th : pool)
-            th.join();
-        size_t total = 0;
-        for (auto &p : parts)
-            total += p.size();
-        std::string out;
-        out.reserve(total);
-        for (auto &p : parts)
-            out += p;
-        out.pop_back();
-        return out;
-    }
 
     // Per-base coverage of every contig (fastibsmapper): device hit mask
     // and k-window popcount, serialised on the host.
