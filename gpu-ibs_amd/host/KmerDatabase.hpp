@@ -365,9 +365,6 @@ public:
                   << st.waitSec << " s), write " << writeSec << " s" << std::endl;
     }
 
-
-    // Per-base coverage of every contig (fastibsmapper): device hit mask
-    // and k-window popcount, serialised on the host.
     // Per-base coverage of every contig (produceMappingGpu,
     // src/KmerDatabase.hpp:1544-1581): the contig is packed on host threads,
     // its hit mask, coverage and coverage text are made on the device, and
