@@ -80,8 +80,9 @@ public:
     bool gpuActive() const { return table_ != nullptr; }
     const hipibs_table *table() const { return table_.get(); }
 
-    // Builds the device table (reads and repacks the database on the host
-    // threads, streams the keys to the device, inserts them).
+    // Builds the device table: the database's raw records are read on host
+    // threads into pinned buffers, decoded into keys and inserted on the
+    // device.
     void initGpuIndex()
     {
         if (table_)

@@ -154,7 +154,8 @@ int hipibs_pack_bases(const char *h_text, uint64_t n, uint64_t at, uint64_t *h_s
 /* Replaces KmcRaIndex(prefix) + DeviceKmerIndex::buildShards
  * (src/gpu/KmcRaIndex.hpp:124-135, src/gpu/DeviceKmerIndex.cu:144-183):
  * reads the database and builds a device-resident bucketised open-addressing
- * table of its canonical keys. */
+ * table of its canonical keys (the raw suffix records are read on host
+ * threads into pinned buffers and decoded into keys on the device). */
 int hipibs_table_build_kmc(const char *prefix, const hipibs_table_opts *opts,
                            hipibs_table **out);
 
