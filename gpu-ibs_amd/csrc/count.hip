@@ -1,0 +1,709 @@
+// k-mer counting from reads on the device: the step the reference's users
+// run with KMC 3.2.4 before the scan (README.md:95: `kmc -k25 -ci2 -cs255`;
+// scripts/make_test_corpus.sh:123: `kmc -k<K> -fm -ci1`). KMC is not part
+// of the reference's sources; its published counting rules, restated here:
+//   - every k-mer of every read (a k-mer never spans two records),
+//   - a k-mer holding a byte other than ACGTacgt is skipped,
+//   - counted in canonical form, min(code, reverse complement),
+//   - kept when min_count <= count <= max_count (-ci, -cx), the stored count
+//     clamped to counter_max (-cs).
+//
+// Counting table in HBM: n_buckets buckets of 128 B, 256 B aligned: the
+// bucket's 8 keys (EMPTY = all ones) in its first 64 B, their 8 counts in
+// the second, so the count's atomic add goes to the line the key lookup has
+// just read (one memory line per counted k-mer). Home bucket bucketOf(key),
+// linear probing at bucket granularity, wrapping. The host grows the table
+// between chunks so that it stays at or below 0.7 load even if every
+// position of the next chunk were a new key: an insert always finds a slot.
+//
+// Reads arrive as text: the bases of each record followed by one '\n' (any
+// byte outside ACGTacgt ends a run of valid bases), in pinned chunks of
+// `chunk_bytes`; a record cut by a chunk end continues in the next chunk
+// after its last k-1 bases, so every k-mer is counted exactly once.
+#include "hipibs_internal.hpp"
+#include "wave_probe.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace hipibs
+{
+namespace
+{
+
+constexpr int kCountSlots = 8;     // keys per bucket
+constexpr int kCountWords = 16;    // 64-bit words per bucket (keys, then counts)
+constexpr int kLanePositions = 32; // k-mer positions one lane rolls over
+constexpr int kCountBlock = 256;
+constexpr int kTilePositions = kCountBlock * kLanePositions;
+constexpr int kTileBytes = kTilePositions + 32; // + k - 1 <= 31 bases, padded
+constexpr double kMaxLoad = 0.7;                // grow above this
+constexpr double kGrowLoad = 0.5;               // load right after growing
+constexpr uint64_t kDefaultChunk = 128ull << 20;
+
+// ACGT / acgt -> 0..3, any other byte 4 (the codec's base table,
+// src/KmerCodec.hpp:105-118). The low two bits of (b >> 1) ^ (b >> 2) are
+// 0, 1, 2, 3 for A, C, G, T in either case; kValid marks the eight letters
+// among the bytes 0x40..0x7f.
+__device__ inline uint32_t baseCode(uint32_t b)
+{
+    constexpr uint64_t kValid = (1ull << 1) | (1ull << 3) | (1ull << 7) | (1ull << 20) |
+                                (1ull << 33) | (1ull << 35) | (1ull << 39) | (1ull << 52);
+    const bool ok = (b >> 6) == 1u && ((kValid >> (b & 63u)) & 1u);
+    return ok ? (((b >> 1) ^ (b >> 2)) & 3u) : 4u;
+}
+
+// Adds `add` to the count of key, storing the key first if it is absent.
+// Keys are read with plain loads: a stored key never changes, and an EMPTY
+// read from a stale line is settled by the compare-and-swap. Returns false
+// only when every bucket was visited (excluded by the host's sizing).
+__device__ inline bool countAdd(uint64_t *tab, uint64_t nBuckets, uint64_t key, uint64_t add,
+                                bool *isNew)
+{
+    uint64_t b = bucketOf(key, nBuckets);
+    *isNew = false;
+    for (uint64_t d = 0; d < nBuckets; ++d)
+    {
+        unsigned long long *bucket = reinterpret_cast<unsigned long long *>(tab + b * kCountWords);
+        const ulonglong2 *kv = reinterpret_cast<const ulonglong2 *>(bucket);
+        uint64_t keys[kCountSlots];
+#pragma unroll
+        for (int q = 0; q < kCountSlots / 2; ++q)
+        {
+            const ulonglong2 v = kv[q];
+            keys[2 * q] = v.x;
+            keys[2 * q + 1] = v.y;
+        }
+#pragma unroll
+        for (int s = 0; s < kCountSlots; ++s)
+        {
+            uint64_t cur = keys[s];
+            if (cur == EMPTY)
+            {
+                const unsigned long long prev = atomicCAS(bucket + s, EMPTY, key);
+                *isNew = prev == EMPTY;
+                cur = *isNew ? key : prev;
+            }
+            if (cur == key)
+            {
+                atomicAdd(bucket + kCountSlots + s, (unsigned long long)add);
+                return true;
+            }
+        }
+        b = b + 1 == nBuckets ? 0 : b + 1;
+    }
+    return false;
+}
+
+__device__ inline uint32_t waveSum(uint32_t v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// counters: [0] distinct keys stored, [1] k-mers counted, [2] 1 if an
+// insert found no slot.
+__global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__restrict__ text,
+                                                           uint64_t n, uint32_t k, uint64_t *tab,
+                                                           uint64_t nBuckets,
+                                                           unsigned long long *counters)
+{
+    __shared__ __align__(16) uint8_t tile[kTileBytes];
+    const uint64_t nPos = n >= k ? n - k + 1 : 0;
+    const uint64_t nTiles = (nPos + kTilePositions - 1) / kTilePositions;
+    const uint64_t mask = k == 32 ? ~0ull : (1ull << (2 * k)) - 1;
+    const uint32_t shift = 2 * (k - 1);
+    uint32_t counted = 0, fresh = 0, full = 0;
+    for (uint64_t t = blockIdx.x; t < nTiles; t += gridDim.x)
+    {
+        const uint64_t base = t * kTilePositions;
+        const uint32_t bytes = (uint32_t)min<uint64_t>(kTilePositions + k - 1, n - base);
+        __syncthreads(); // the previous tile is no longer read
+        for (uint32_t i = threadIdx.x * 16; i < bytes; i += kCountBlock * 16)
+        {
+            if (i + 16 <= bytes)
+                *reinterpret_cast<uint4 *>(tile + i) =
+                    *reinterpret_cast<const uint4 *>(text + base + i);
+            else
+                for (uint32_t j = i; j < bytes; ++j)
+                    tile[j] = text[base + j];
+        }
+        __syncthreads();
+        // Positions [start, start + 32) of the tile: the k-mer at position p
+        // ends at byte p + k - 1; `run` counts the valid bytes ending at i.
+        const uint32_t start = threadIdx.x * kLanePositions;
+        const uint32_t end = min<uint32_t>(bytes, start + kLanePositions + k - 1);
+        uint64_t fwd = 0, rc = 0;
+        uint32_t run = 0;
+        for (uint32_t i = start; i < end; ++i)
+        {
+            const uint32_t c = baseCode(tile[i]);
+            if (c > 3)
+            {
+                run = 0;
+                continue;
+            }
+            ++run;
+            fwd = ((fwd << 2) | c) & mask;
+            rc = (rc >> 2) | ((uint64_t)(3 - c) << shift);
+            if (run >= k)
+            {
+                bool isNew;
+                full |= !countAdd(tab, nBuckets, min(fwd, rc), 1, &isNew);
+                counted += 1;
+                fresh += isNew;
+            }
+        }
+    }
+    counted = waveSum(counted);
+    fresh = waveSum(fresh);
+    full = waveSum(full);
+    if ((threadIdx.x & 63) == 0)
+    {
+        if (fresh)
+            atomicAdd(counters + 0, (unsigned long long)fresh);
+        if (counted)
+            atomicAdd(counters + 1, (unsigned long long)counted);
+        if (full)
+            atomicExch(counters + 2, 1ull);
+    }
+}
+
+__global__ __launch_bounds__(256) void countFillKernel(uint64_t *tab, uint64_t words)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        tab[i] = (i % kCountWords) < kCountSlots ? EMPTY : 0ull;
+}
+
+// Moves every (key, count) of the old table into the new one.
+__global__ __launch_bounds__(256) void countRehashKernel(const uint64_t *__restrict__ old,
+                                                         uint64_t oldBuckets, uint64_t *tab,
+                                                         uint64_t nBuckets,
+                                                         unsigned long long *counters)
+{
+    const uint64_t slots = oldBuckets * kCountSlots;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots;
+         i += (uint64_t)gridDim.x * blockDim.x)
+    {
+        const uint64_t b = i / kCountSlots, s = i % kCountSlots;
+        const uint64_t key = old[b * kCountWords + s];
+        if (key == EMPTY)
+            continue;
+        bool isNew;
+        if (!countAdd(tab, nBuckets, key, old[b * kCountWords + kCountSlots + s], &isNew))
+            atomicExch(counters + 2, 1ull);
+    }
+}
+
+// Keys with lo <= count <= hi, in no particular order, with their counts
+// clamped to cmax (counts may be null). filt: [0] kept, [1] below lo,
+// [2] above hi. A wave claims its output range with one atomic add.
+__global__ __launch_bounds__(256) void countExtractKernel(const uint64_t *__restrict__ tab,
+                                                          uint64_t nBuckets, uint64_t lo,
+                                                          uint64_t hi, uint64_t cmax,
+                                                          uint64_t *keys, uint32_t *counts,
+                                                          uint64_t cap,
+                                                          unsigned long long *filt)
+{
+    const uint64_t slots = nBuckets * kCountSlots;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t below = 0, above = 0;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x; w < slots; w += stride)
+    {
+        const uint64_t i = w + threadIdx.x;
+        uint64_t key = EMPTY, c = 0;
+        if (i < slots)
+        {
+            const uint64_t b = i / kCountSlots, s = i % kCountSlots;
+            key = tab[b * kCountWords + s];
+            c = tab[b * kCountWords + kCountSlots + s];
+        }
+        const bool present = key != EMPTY;
+        const bool keep = present && c >= lo && c <= hi;
+        below += present && c < lo;
+        above += present && c > hi;
+        const uint64_t m = __ballot(keep);
+        if (!m)
+            continue;
+        uint64_t at = 0;
+        if (laneId() == __builtin_ctzll(m))
+            at = atomicAdd(filt + 0, (unsigned long long)__popcll(m));
+        at = __shfl(at, __builtin_ctzll(m), 64);
+        if (keep)
+        {
+            const uint64_t o = at + maskedRank(m);
+            if (o < cap)
+            {
+                keys[o] = key;
+                if (counts)
+                    counts[o] = (uint32_t)min(c, cmax);
+            }
+        }
+    }
+    below = waveSum(below);
+    above = waveSum(above);
+    if ((threadIdx.x & 63) == 0)
+    {
+        if (below)
+            atomicAdd(filt + 1, (unsigned long long)below);
+        if (above)
+            atomicAdd(filt + 2, (unsigned long long)above);
+    }
+}
+
+int cuCount(int device)
+{
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    return cus;
+}
+
+uint32_t gridFor(uint64_t items, uint64_t perBlock, int device)
+{
+    const uint64_t want = (items + perBlock - 1) / perBlock;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)cuCount(device) * 8));
+}
+
+} // namespace
+} // namespace hipibs
+
+using namespace hipibs;
+
+struct hipibs_counter : ReadSink
+{
+    int device = 0;
+    uint32_t k = 0;
+    uint64_t chunkBytes = 0;
+    hipStream_t stream = nullptr;
+    uint64_t *dTab = nullptr;
+    uint64_t nBuckets = 0;
+    unsigned long long *dCounters = nullptr; // [0] distinct, [1] counted, [2] full, [3..5] filter
+    unsigned long long *hCounters = nullptr; // pinned copy
+    char *hText[2] = {nullptr, nullptr};     // pinned chunks
+    uint8_t *dText = nullptr;
+    int cur = 0;          // chunk being filled
+    uint64_t len = 0;     // bytes in it
+    uint64_t recLen = 0;  // bases of the open record so far
+    uint64_t reads = 0;   // records
+    uint64_t distinct = 0, counted = 0;
+    int err = 0;
+
+    int failed() const override { return err; }
+    void bases(const char *p, size_t n) override
+    {
+        while (n && !err)
+        {
+            if (len == chunkBytes)
+            {
+                flush(true);
+                continue;
+            }
+            const size_t m = std::min<size_t>(n, chunkBytes - len);
+            std::memcpy(hText[cur] + len, p, m);
+            len += m;
+            recLen += m;
+            p += m;
+            n -= m;
+        }
+    }
+    void recordEnd() override
+    {
+        if (err)
+            return;
+        if (len == chunkBytes)
+            flush(false);
+        hText[cur][len++] = '\n';
+        recLen = 0;
+        reads += 1;
+    }
+
+    // Submits the filled chunk; with midRecord the next chunk starts with the
+    // open record's last k-1 bases (fewer if it is shorter).
+    void flush(bool midRecord)
+    {
+        const uint64_t carry = midRecord ? std::min<uint64_t>(recLen, k - 1) : 0;
+        const int done = cur;
+        const uint64_t doneLen = len;
+        err = submit(done, doneLen);
+        cur ^= 1;
+        len = 0;
+        if (!err && carry)
+        {
+            std::memcpy(hText[cur], hText[done] + doneLen - carry, carry);
+            len = carry;
+        }
+    }
+
+    // Waits for the previous chunk, then copies chunk `slot` to the device
+    // and counts it. The other pinned chunk is free once this returns.
+    int submit(int slot, uint64_t n)
+    {
+        int rc = sync();
+        if (rc)
+            return rc;
+        const uint64_t positions = n >= k ? n - k + 1 : 0;
+        if (!positions)
+            return 0;
+        rc = reserve(distinct + positions);
+        if (rc)
+            return rc;
+        HIPIBS_CHECK(hipMemcpyAsync(dText, hText[slot], n, hipMemcpyHostToDevice, stream));
+        countKernel<<<gridFor(positions, kTilePositions, device), kCountBlock, 0, stream>>>(
+            dText, n, k, dTab, nBuckets, dCounters);
+        HIPIBS_CHECK(hipGetLastError());
+        HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 3 * sizeof(unsigned long long),
+                                    hipMemcpyDeviceToHost, stream));
+        return 0;
+    }
+
+    int sync()
+    {
+        HIPIBS_CHECK(hipStreamSynchronize(stream));
+        if (hCounters[2])
+            return setError(HIPIBS_E_FULL, "k-mer counter: an insert found no free slot");
+        distinct = hCounters[0];
+        counted = hCounters[1];
+        return 0;
+    }
+
+    // Makes room for `need` keys at <= kMaxLoad, growing to kGrowLoad
+    // (called with the stream idle).
+    int reserve(uint64_t need)
+    {
+        if ((double)need <= kMaxLoad * (double)(nBuckets * kCountSlots))
+            return 0;
+        uint64_t want = (uint64_t)((double)need / kGrowLoad) / kCountSlots + 1;
+        want = std::max<uint64_t>(want, 2 * nBuckets);
+        want = std::max<uint64_t>(want, 1024);
+        if (want >= (1ull << 32))
+            return setError(HIPIBS_E_NOMEM, "k-mer counter: more than 2^32 buckets needed");
+        uint64_t *t = nullptr;
+        const hipError_t e = hipMalloc(reinterpret_cast<void **>(&t), want * kCountWords * 8);
+        if (e != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return setError(HIPIBS_E_NOMEM,
+                            "k-mer counter: no device memory for " + std::to_string(want) +
+                                " buckets (" + std::to_string(distinct) + " distinct k-mers so far)");
+        }
+        const uint64_t words = want * kCountWords;
+        countFillKernel<<<gridFor(words, 256 * 4, device), 256, 0, stream>>>(t, words);
+        HIPIBS_CHECK(hipGetLastError());
+        if (dTab)
+        {
+            countRehashKernel<<<gridFor(nBuckets * kCountSlots, 256 * 4, device), 256, 0, stream>>>(
+                dTab, nBuckets, t, want, dCounters);
+            HIPIBS_CHECK(hipGetLastError());
+            HIPIBS_CHECK(hipStreamSynchronize(stream));
+            HIPIBS_CHECK(hipFree(dTab));
+        }
+        dTab = t;
+        nBuckets = want;
+        return 0;
+    }
+
+    // Submits what is buffered and waits for every chunk.
+    int finish()
+    {
+        if (err)
+            return err;
+        if (len)
+        {
+            err = submit(cur, len);
+            cur ^= 1;
+            len = 0;
+            recLen = 0;
+            if (err)
+                return err;
+        }
+        return err = sync();
+    }
+
+    // Keys with lo <= count <= hi into dKeys (and clamped counts into dCounts
+    // if non-null), at most cap; filt receives kept / below / above.
+    int extract(uint64_t lo, uint64_t hi, uint64_t cmax, uint64_t *dKeys, uint32_t *dCounts,
+                uint64_t cap, hipibs_count_filter *filt)
+    {
+        HIPIBS_CHECK(hipMemsetAsync(dCounters + 3, 0, 3 * sizeof(unsigned long long), stream));
+        if (nBuckets)
+        {
+            countExtractKernel<<<gridFor(nBuckets * kCountSlots, 256 * 8, device), 256, 0,
+                                 stream>>>(dTab, nBuckets, lo, hi, cmax, dKeys, dCounts, cap,
+                                           dCounters + 3);
+            HIPIBS_CHECK(hipGetLastError());
+        }
+        HIPIBS_CHECK(hipMemcpyAsync(hCounters + 3, dCounters + 3, 3 * sizeof(unsigned long long),
+                                    hipMemcpyDeviceToHost, stream));
+        HIPIBS_CHECK(hipStreamSynchronize(stream));
+        filt->kept = hCounters[3];
+        filt->below = hCounters[4];
+        filt->above = hCounters[5];
+        return 0;
+    }
+};
+
+static int checkFilter(uint32_t lo, uint32_t hi, uint32_t cmax)
+{
+    if (lo > hi)
+        return setError(HIPIBS_E_ARG, "k-mer counter: min_count > max_count");
+    if (cmax == 0)
+        return setError(HIPIBS_E_ARG, "k-mer counter: counter_max must be >= 1");
+    return 0;
+}
+
+extern "C" int hipibs_counter_create(uint32_t k, int device, uint64_t expected_keys,
+                                     uint64_t chunk_bytes, hipibs_counter **out)
+{
+    if (!out)
+        return setError(HIPIBS_E_ARG, "null argument");
+    *out = nullptr;
+    if (k < 1 || k > 32)
+        return setError(HIPIBS_E_ARG, "k-mer counter: k must be 1..32");
+    if (chunk_bytes == 0)
+        chunk_bytes = kDefaultChunk;
+    if (chunk_bytes < 64 || chunk_bytes > (1ull << 32))
+        return setError(HIPIBS_E_ARG, "k-mer counter: chunk_bytes must be 64 B .. 4 GiB");
+    int nDev = 0;
+    if (hipGetDeviceCount(&nDev) != hipSuccess || nDev == 0)
+    {
+        (void)hipGetLastError();
+        return setError(HIPIBS_E_NODEV, "no usable GPU");
+    }
+    if (device < 0 || device >= nDev)
+        return setError(HIPIBS_E_ARG, "no such device");
+    HIPIBS_CHECK(hipSetDevice(device));
+    auto *c = new hipibs_counter();
+    c->device = device;
+    c->k = k;
+    c->chunkBytes = chunk_bytes;
+    auto fail = [c](int rc)
+    {
+        hipibs_counter_free(c);
+        return rc;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&c->dCounters), 8 * sizeof(unsigned long long));
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void **>(&c->hCounters), 8 * sizeof(unsigned long long),
+                          hipHostMallocDefault);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i)
+        e = hipHostMalloc(reinterpret_cast<void **>(&c->hText[i]), chunk_bytes,
+                          hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&c->dText), chunk_bytes);
+    if (e == hipSuccess)
+        e = hipMemset(c->dCounters, 0, 8 * sizeof(unsigned long long));
+    if (e != hipSuccess)
+        return fail(hipFail(e, "k-mer counter buffers"));
+    std::memset(c->hCounters, 0, 8 * sizeof(unsigned long long));
+    if (expected_keys)
+    {
+        const int rc = c->reserve(expected_keys);
+        if (rc)
+            return fail(rc);
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int hipibs_counter_add_text(hipibs_counter *c, const char *h_text, uint64_t n)
+{
+    if (!c || (n && !h_text))
+        return setError(HIPIBS_E_ARG, "null argument");
+    if (c->err)
+        return setError(c->err, "k-mer counter: an earlier call failed");
+    HIPIBS_CHECK(hipSetDevice(c->device));
+    c->bases(h_text, n);
+    c->recordEnd();
+    return c->err;
+}
+
+extern "C" int hipibs_counter_add_file(hipibs_counter *c, const char *path)
+{
+    if (!c || !path)
+        return setError(HIPIBS_E_ARG, "null argument");
+    if (c->err)
+        return setError(c->err, "k-mer counter: an earlier call failed");
+    HIPIBS_CHECK(hipSetDevice(c->device));
+    const int rc = readRecords(path, *c, nullptr);
+    if (rc && !c->err)
+        c->err = rc; // the file's records before the error may be counted: the counter is spent
+    return rc;
+}
+
+extern "C" int hipibs_counter_stats(hipibs_counter *c, hipibs_count_stats *out)
+{
+    if (!c || !out)
+        return setError(HIPIBS_E_ARG, "null argument");
+    HIPIBS_CHECK(hipSetDevice(c->device));
+    const int rc = c->finish();
+    if (rc)
+        return rc;
+    out->reads = c->reads;
+    out->kmers = c->counted;
+    out->distinct = c->distinct;
+    out->table_bytes = c->nBuckets * kCountWords * 8;
+    return 0;
+}
+
+extern "C" int hipibs_counter_keys(hipibs_counter *c, uint32_t min_count, uint32_t max_count,
+                                   uint32_t counter_max, uint64_t *h_keys, uint32_t *h_counts,
+                                   uint64_t cap, hipibs_count_filter *filt)
+{
+    if (!c || !filt || (cap && !h_keys))
+        return setError(HIPIBS_E_ARG, "null argument");
+    int rc = checkFilter(min_count, max_count, counter_max);
+    if (rc)
+        return rc;
+    HIPIBS_CHECK(hipSetDevice(c->device));
+    if ((rc = c->finish()))
+        return rc;
+    const uint64_t m = std::min<uint64_t>(cap, c->distinct);
+    uint64_t *dKeys = nullptr;
+    uint32_t *dCounts = nullptr;
+    hipError_t e = hipSuccess;
+    if (m)
+    {
+        e = hipMalloc(reinterpret_cast<void **>(&dKeys), m * 8);
+        if (e == hipSuccess && h_counts)
+            e = hipMalloc(reinterpret_cast<void **>(&dCounts), m * 4);
+    }
+    if (e == hipSuccess)
+        rc = c->extract(min_count, max_count, counter_max, dKeys, dCounts, m, filt);
+    const uint64_t got = std::min<uint64_t>(m, filt->kept);
+    if (e == hipSuccess && !rc && got)
+    {
+        e = hipMemcpy(h_keys, dKeys, got * 8, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && h_counts)
+            e = hipMemcpy(h_counts, dCounts, got * 4, hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(dKeys);
+    (void)hipFree(dCounts);
+    if (rc)
+        return rc;
+    if (e != hipSuccess)
+        return hipFail(e, "copying counted k-mers");
+    return 0;
+}
+
+extern "C" int hipibs_counter_build_table(hipibs_counter *c, uint32_t min_count,
+                                          uint32_t max_count, const hipibs_table_opts *opts,
+                                          hipibs_table **out, hipibs_count_filter *filt)
+{
+    if (!c || !out)
+        return setError(HIPIBS_E_ARG, "null argument");
+    *out = nullptr;
+    int rc = checkFilter(min_count, max_count, 1);
+    if (rc)
+        return rc;
+    hipibs_table_opts o{};
+    if (opts)
+        o = *opts;
+    else
+        o.device = c->device;
+    if (o.device != c->device)
+        return setError(HIPIBS_E_ARG, "k-mer counter: the table must be on the counter's device");
+    HIPIBS_CHECK(hipSetDevice(c->device));
+    if ((rc = c->finish()))
+        return rc;
+    hipibs_count_filter f{};
+    uint64_t *dKeys = nullptr;
+    if (c->distinct)
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dKeys), c->distinct * 8));
+    rc = c->extract(min_count, max_count, 1, dKeys, nullptr, c->distinct, &f);
+    if (!rc)
+        rc = hipibs_table_build_keys_device(dKeys, f.kept, c->k, &o, out, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(dKeys);
+    if (filt)
+        *filt = f;
+    return rc;
+}
+
+// KMC's counter width for -cs: the bytes that hold counter_max.
+static uint32_t counterBytes(uint32_t cmax)
+{
+    uint32_t b = 1;
+    while (b < 4 && (uint64_t)cmax >= (1ull << (8 * b)))
+        ++b;
+    return b;
+}
+
+// Prefix length of the written database: (k - p) % 4 == 0 and p <= 12, the
+// one with the smallest files (records of (k - p) / 4 + counter bytes each,
+// plus a LUT of 8 B per prefix per signature bin).
+static uint32_t lutPrefixLen(uint64_t n, uint32_t k, uint32_t cbytes, uint32_t bins)
+{
+    uint32_t best = k % 4;
+    double bestBytes = -1;
+    for (uint32_t p = k % 4; p <= std::min<uint32_t>(12, k); p += 4)
+    {
+        const double bytes = (double)n * ((k - p) / 4 + cbytes) +
+                             (double)(1ull << (2 * p)) * bins * 8.0;
+        if (bestBytes < 0 || bytes < bestBytes)
+        {
+            bestBytes = bytes;
+            best = p;
+        }
+    }
+    return best;
+}
+
+extern "C" int hipibs_counter_write_kmc(hipibs_counter *c, const char *prefix, uint32_t min_count,
+                                        uint32_t max_count, uint32_t counter_max,
+                                        hipibs_count_filter *filt)
+{
+    if (!c || !prefix)
+        return setError(HIPIBS_E_ARG, "null argument");
+    int rc = checkFilter(min_count, max_count, counter_max);
+    if (rc)
+        return rc;
+    HIPIBS_CHECK(hipSetDevice(c->device));
+    if ((rc = c->finish()))
+        return rc;
+    std::vector<uint64_t> keys(c->distinct);
+    std::vector<uint32_t> counts(c->distinct);
+    hipibs_count_filter f{};
+    rc = hipibs_counter_keys(c, min_count, max_count, counter_max, keys.data(), counts.data(),
+                             keys.size(), &f);
+    if (rc)
+        return rc;
+    if (filt)
+        *filt = f;
+    // KMC2 layout with 9-base signatures (KMC's default); k < 9 shortens
+    // them, and k < 5 writes the KMC1 layout, which has none.
+    const uint32_t sig = std::min<uint32_t>(9, c->k);
+    const uint32_t version = sig >= 5 ? 0x200 : 0;
+    const uint32_t cbytes = counterBytes(counter_max);
+    const uint32_t p = lutPrefixLen(f.kept, c->k, cbytes, version ? 512 : 1);
+    return kmcWrite(prefix, keys.data(), counts.data(), f.kept, c->k, p, version ? sig : 0, cbytes,
+                    version, min_count, max_count);
+}
+
+extern "C" void hipibs_counter_free(hipibs_counter *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    if (c->dTab)
+        (void)hipFree(c->dTab);
+    if (c->dCounters)
+        (void)hipFree(c->dCounters);
+    if (c->dText)
+        (void)hipFree(c->dText);
+    for (char *h : c->hText)
+        if (h)
+            (void)hipHostFree(h);
+    if (c->hCounters)
+        (void)hipHostFree(c->hCounters);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "hipibs.h"
+#include "reads.hpp"
 
 namespace hipibs
 {
@@ -293,6 +294,10 @@ struct KmcRecords
     ~KmcRecords();
 };
 int kmcOpenRecords(const std::string &prefix, KmcRecords &out);
+int kmcWrite(const std::string &prefix, const uint64_t *keys, const uint32_t *counts, uint64_t n,
+             uint32_t k, uint32_t p, uint32_t sigLen, uint32_t counterSize, uint32_t version,
+             uint32_t minCount, uint32_t maxCount);
+
 // Records [r0, r0+m) as stored (m * recBytes bytes) into dst, read by up to
 // `threads` parallel preads (0 = every usable CPU).
 int kmcReadRecords(const KmcRecords &src, uint64_t r0, uint64_t m, void *dst, unsigned threads);

@@ -69,6 +69,14 @@ class ServeCounters(C.Structure):
                                          "resident_tables", "build_us", "evict_us")]
 
 
+class CountStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("reads", "kmers", "distinct", "table_bytes")]
+
+
+class CountFilter(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("kept", "below", "above")]
+
+
 class HostSlot(C.Structure):
     _fields_ = [("seq_words", u64p), ("exc_words", u64p), ("windows", C.POINTER(Window)),
                 ("stats", C.POINTER(WindowStats)), ("seq_cap", C.c_uint64),
@@ -101,6 +109,18 @@ _SIGS = [
     ("hipibs_table_probe_count", C.c_int, [vp, u64p, C.c_uint64, u64p]),
     ("hipibs_table_probe_count_device", C.c_int, [vp, vp, C.c_uint64, vp, vp]),
     ("hipibs_table_probe_count_kmc", C.c_int, [vp, C.c_char_p, C.c_uint, u64p]),
+    ("hipibs_counter_create", C.c_int, [C.c_uint32, C.c_int, C.c_uint64, C.c_uint64,
+                                        C.POINTER(vp)]),
+    ("hipibs_counter_add_file", C.c_int, [vp, C.c_char_p]),
+    ("hipibs_counter_add_text", C.c_int, [vp, C.c_char_p, C.c_uint64]),
+    ("hipibs_counter_stats", C.c_int, [vp, C.POINTER(CountStats)]),
+    ("hipibs_counter_keys", C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, u64p, u32p,
+                                      C.c_uint64, C.POINTER(CountFilter)]),
+    ("hipibs_counter_build_table", C.c_int, [vp, C.c_uint32, C.c_uint32, C.POINTER(TableOpts),
+                                             C.POINTER(vp), C.POINTER(CountFilter)]),
+    ("hipibs_counter_write_kmc", C.c_int, [vp, C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                           C.POINTER(CountFilter)]),
+    ("hipibs_counter_free", None, [vp]),
     ("hipibs_coverage_text_device", C.c_int, [vp, C.c_uint64, vp, C.c_uint64, u64p, vp]),
     ("hipibs_coverage_text_stream", C.c_int, [vp, C.c_uint64, vp, vp, vp]),
     ("hipibs_scan_windows_device", C.c_int, [C.POINTER(vp), C.c_int, vp, vp, C.c_uint64, vp,
@@ -613,6 +633,73 @@ def _collect(eng: ScanEngine, slot: int, rng: Tuple[int, int], stats: np.ndarray
     n = hi - lo
     arr = st[:n * stats.shape[0]].view(np.int32).reshape(stats.shape[0], n, 4)
     stats[:, lo:hi, :] = arr
+
+
+class KmerCounter:
+    """Canonical k-mer counts of reads on the device (hipibs_counter): the
+    counting step KMC performs for the reference (`kmc -k -ci -cx -cs`)."""
+
+    def __init__(self, k: int, device: int = 0, expected_keys: int = 0, chunk_bytes: int = 0):
+        h = vp()
+        _check(_lib.hipibs_counter_create(k, device, expected_keys, chunk_bytes, C.byref(h)),
+               "hipibs_counter_create")
+        self.handle = h
+        self.k = k
+        self.device = device
+
+    def add_file(self, path: str) -> None:
+        _check(_lib.hipibs_counter_add_file(self.handle, str(path).encode()),
+               "hipibs_counter_add_file")
+
+    def add_text(self, text: bytes) -> None:
+        _check(_lib.hipibs_counter_add_text(self.handle, text, len(text)),
+               "hipibs_counter_add_text")
+
+    def stats(self) -> dict:
+        st = CountStats()
+        _check(_lib.hipibs_counter_stats(self.handle, C.byref(st)), "hipibs_counter_stats")
+        return {n: getattr(st, n) for n, _ in CountStats._fields_}
+
+    def keys(self, min_count: int = 1, max_count: int = 0xFFFFFFFF, counter_max: int = 255):
+        """(keys uint64, counts uint32, filter dict), in no particular order."""
+        n = self.stats()["distinct"]
+        keys = np.empty(n, dtype=np.uint64)
+        counts = np.empty(n, dtype=np.uint32)
+        f = CountFilter()
+        _check(_lib.hipibs_counter_keys(self.handle, min_count, max_count, counter_max,
+                                        _ptr(keys), _ptr(counts, u32p), n, C.byref(f)),
+               "hipibs_counter_keys")
+        kept = min(f.kept, n)
+        return keys[:kept], counts[:kept], {nm: getattr(f, nm) for nm, _ in CountFilter._fields_}
+
+    def build_table(self, min_count: int = 1, max_count: int = 0xFFFFFFFF, **kw) -> "Table":
+        kw.setdefault("device", self.device)
+        o = _opts(**kw)
+        h = vp()
+        f = CountFilter()
+        _check(_lib.hipibs_counter_build_table(self.handle, min_count, max_count, C.byref(o),
+                                               C.byref(h), C.byref(f)),
+               "hipibs_counter_build_table")
+        return Table(h.value)
+
+    def write_kmc(self, prefix: str, min_count: int = 2, max_count: int = 1_000_000_000,
+                  counter_max: int = 255) -> dict:
+        f = CountFilter()
+        _check(_lib.hipibs_counter_write_kmc(self.handle, str(prefix).encode(), min_count,
+                                             max_count, counter_max, C.byref(f)),
+               "hipibs_counter_write_kmc")
+        return {nm: getattr(f, nm) for nm, _ in CountFilter._fields_}
+
+    def free(self) -> None:
+        if self.handle:
+            _lib.hipibs_counter_free(self.handle)
+            self.handle = vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class ServeEngine:

@@ -1,6 +1,9 @@
 // Host-only checks of the C++ layer (run by tests/test_tools.py):
 // KmerCodec against a string brute force, window and mapper geometry, the
 // batch planner's invariants, the Window distance rule.
+// `test_host reads <file>` prints the records the k-mer counter's reader
+// takes from a read file, one line each.
+#include "../csrc/reads.hpp"
 #include "KmerCodec.hpp"
 #include "ScanPipeline.hpp"
 #include "SeqIO.hpp"
@@ -41,8 +44,30 @@ static bool stringCanonical(const std::string &w, uint64_t &code)
     return true;
 }
 
-int main()
+// Prints each record's bases followed by '\n'.
+struct PrintSink : hipibs::ReadSink
 {
+    void bases(const char *p, size_t n) override { std::fwrite(p, 1, n, stdout); }
+    void recordEnd() override { std::fputc('\n', stdout); }
+    int failed() const override { return 0; }
+};
+
+int main(int argc, char **argv)
+{
+    if (argc == 3 && std::string(argv[1]) == "reads")
+    {
+        PrintSink sink;
+        uint64_t n = 0;
+        const int rc = hipibs::readRecords(argv[2], sink, &n);
+        std::fflush(stdout);
+        if (rc)
+        {
+            std::fprintf(stderr, "%s\n", hipibs_last_error());
+            return 2;
+        }
+        std::fprintf(stderr, "records %llu\n", (unsigned long long)n);
+        return 0;
+    }
     std::mt19937_64 rng(20260830);
     const char alpha[] = "ACGTACGTacgtNnXY-";
     for (uint32_t k : {1u, 4u, 15u, 25u, 31u, 32u})

@@ -205,6 +205,64 @@ int hipibs_table_probe_count_device(const hipibs_table *t, const uint64_t *d_key
 int hipibs_table_probe_count_kmc(const hipibs_table *t, const char *kmc_prefix,
                                  unsigned load_threads, uint64_t *out);
 
+/* ---- k-mer counting from reads ------------------------------------------- */
+
+/* Replaces the `kmc -k<K> -ci<A> -cx<B> -cs<C>` step that produces the
+ * reference's sample databases (README.md:95, scripts/make_test_corpus.sh:123;
+ * KMC 3.2.4 is an external tool, not part of the reference's sources). The
+ * canonical k-mers of the reads are counted in a device hash table: a k-mer
+ * never spans two records and is skipped if it holds a byte other than
+ * ACGTacgt. The result becomes a device table (no files), host arrays, or a
+ * KMC database. A counter is used by one thread at a time. */
+typedef struct hipibs_counter hipibs_counter;
+
+typedef struct hipibs_count_stats {
+    uint64_t reads;       /* records (add_file) and add_text calls */
+    uint64_t kmers;       /* k-mers counted: positions with k valid bases */
+    uint64_t distinct;    /* distinct canonical k-mers */
+    uint64_t table_bytes; /* device bytes of the counting table */
+} hipibs_count_stats;
+
+typedef struct hipibs_count_filter {
+    uint64_t kept;  /* distinct k-mers with min_count <= count <= max_count */
+    uint64_t below; /* count < min_count */
+    uint64_t above; /* count > max_count */
+} hipibs_count_filter;
+
+/* k in 1..32. expected_keys (0 = unknown) sizes the first table; the table
+ * grows as needed. chunk_bytes (0 = 128 MiB) is the text moved to the device
+ * at a time: two pinned host buffers and one device buffer of that size. */
+int hipibs_counter_create(uint32_t k, int device, uint64_t expected_keys, uint64_t chunk_bytes,
+                          hipibs_counter **out);
+/* Counts a FASTA or FASTQ file, plain or gzip (decided by its first byte);
+ * asynchronous with respect to the device. After a failure the counter only
+ * accepts hipibs_counter_free. */
+int hipibs_counter_add_file(hipibs_counter *c, const char *path);
+/* Counts n bytes of text as one record: any byte other than ACGTacgt (a
+ * '\n', say) ends a run of k-mers, so several reads may be passed at once. */
+int hipibs_counter_add_text(hipibs_counter *c, const char *h_text, uint64_t n);
+/* Waits for the device and reports totals. */
+int hipibs_counter_stats(hipibs_counter *c, hipibs_count_stats *out);
+/* The k-mers with min_count <= count <= max_count, in no particular order,
+ * into h_keys (at most cap) and, if non-NULL, their counts clamped to
+ * counter_max into h_counts. */
+int hipibs_counter_keys(hipibs_counter *c, uint32_t min_count, uint32_t max_count,
+                        uint32_t counter_max, uint64_t *h_keys, uint32_t *h_counts,
+                        uint64_t cap, hipibs_count_filter *filt);
+/* A device table of the kept k-mers, built on the device from the counting
+ * table (opts as hipibs_table_build_kmc; NULL = defaults on the counter's
+ * device). filt may be NULL. */
+int hipibs_counter_build_table(hipibs_counter *c, uint32_t min_count, uint32_t max_count,
+                               const hipibs_table_opts *opts, hipibs_table **out,
+                               hipibs_count_filter *filt);
+/* Writes the kept k-mers as a KMC database (<prefix>.kmc_pre/.kmc_suf; the
+ * KMC2 layout with 9-base signatures, counters of the width counter_max
+ * needs; header min/max count = min_count / max_count), readable by CKMCFile,
+ * KmcRaIndex and hipibs_table_build_kmc. filt may be NULL. */
+int hipibs_counter_write_kmc(hipibs_counter *c, const char *prefix, uint32_t min_count,
+                             uint32_t max_count, uint32_t counter_max, hipibs_count_filter *filt);
+void hipibs_counter_free(hipibs_counter *c);
+
 /* ---- the fused scan (device-resident inputs) ------------------------------ */
 
 /* Fused encode + canonicalise + probe + per-window run-length reduction over

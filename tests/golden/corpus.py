@@ -166,15 +166,21 @@ def fasta(records, width: int = 80) -> bytes:
 SMALL_SEED = 0x1B50000 + 100 * 1
 
 
-def small_config(seed: int = SMALL_SEED, n: int = 1_000_000, k: int = 25):
-    """The 1 Mbp reference and its three samples' (keys, counts)."""
+def small_reads(seed: int = SMALL_SEED, n: int = 1_000_000):
+    """The 1 Mbp reference and its three samples' reads: 150 bp at 10x of
+    haplotypes with 0.1% / 1% / 5% SNPs, 0.5% substitution errors."""
     ref = make_reference(n, seed)
-    samples = []
+    reads = []
     for i, rate in enumerate((0.001, 0.01, 0.05)):
         hap = mutate(ref, rate, seed + i + 1)
-        reads = simulate_reads(hap, 10.0, 150, 0.005, seed + 10 + i)
-        samples.append(count_kmers(reads, k, ci=2, cs=255))
-    return ref, samples
+        reads.append(simulate_reads(hap, 10.0, 150, 0.005, seed + 10 + i))
+    return ref, reads
+
+
+def small_config(seed: int = SMALL_SEED, n: int = 1_000_000, k: int = 25):
+    """The 1 Mbp reference and its three samples' (keys, counts)."""
+    ref, reads = small_reads(seed, n)
+    return ref, [count_kmers(r, k, ci=2, cs=255) for r in reads]
 
 
 def edge_config(k: int, seed: int = 0x1B5E000):

@@ -1,0 +1,239 @@
+"""The k-mer counter (hipibs_counter_*, bin/kmc) against the oracle's
+counting (corpus.count_kmers: KMC's rules, pinned in test_oracle.py) and,
+through tables it builds, against the reference's golden TSVs. Bit-exact:
+keys and counts are integers."""
+import gzip
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import corpus
+from util import ROOT
+
+import hipibs
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(ROOT, "gpu-ibs_amd", "bin")
+ALL = 0xFFFFFFFF
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def fastq_bytes(reads, lower=False):
+    """FASTQ of code rows; quality strings cycle through '!'..'I', so some
+    quality lines start with '+' or '@'."""
+    out = []
+    for i, r in enumerate(reads):
+        s = corpus.to_text(r, lower=lower)
+        q = bytes(33 + (i + j) % 41 for j in range(len(s)))
+        out.append(b"@r%d\n%s\n+\n%s\n" % (i, s, q))
+    return b"".join(out)
+
+
+def fasta_bytes(records, width=60):
+    return corpus.fasta([(b"c%d" % i, corpus.to_text(r)) for i, r in enumerate(records)], width)
+
+
+def ragged_counts(records, k, ci=1, cs=ALL):
+    """count_kmers over records of any lengths."""
+    codes = []
+    for r in records:
+        if r.size >= k:
+            c, v = corpus.canonical_codes(r[None, :], k)
+            codes.append(c[v])
+    allc = np.concatenate(codes) if codes else np.zeros(0, np.uint64)
+    keys, counts = np.unique(allc, return_counts=True)
+    keep = counts >= ci
+    return keys[keep], np.minimum(counts[keep], cs).astype(np.uint32), allc.size
+
+
+def sorted_pairs(keys, counts):
+    o = np.argsort(keys, kind="stable")
+    return keys[o], counts[o]
+
+
+@pytest.fixture(scope="module")
+def small_reads():
+    return corpus.small_reads()
+
+
+def test_counts_match_oracle_fastq(tmp_path, small_reads):
+    _, reads = small_reads
+    r = reads[0]
+    path = tmp_path / "s0.fq"
+    path.write_bytes(fastq_bytes(r))
+    note("count s0 fastq")
+    c = hipibs.KmerCounter(25)
+    c.add_file(str(path))
+    st = c.stats()
+    want_k, want_c = corpus.count_kmers(r, 25, ci=1, cs=ALL)
+    _, valid = corpus.canonical_codes(r, 25)
+    assert st["reads"] == r.shape[0]
+    assert st["kmers"] == int(valid.sum())
+    assert st["distinct"] == want_k.size
+    keys, counts, filt = c.keys(1, ALL, ALL)
+    keys, counts = sorted_pairs(keys, counts)
+    np.testing.assert_array_equal(keys, want_k)
+    np.testing.assert_array_equal(counts, want_c)
+    assert filt == {"kept": want_k.size, "below": 0, "above": 0}
+    c.free()
+
+
+def test_reads_to_table_scan_matches_golden(tmp_path, small_reads, expected):
+    """reads -> GPU counts (-ci2) -> device table -> scan == the reference's
+    TSVs for the three samples; the KMC database the counter writes gives
+    the same table."""
+    ref, reads = small_reads
+    seq = corpus.to_text(ref)
+    for i, r in enumerate(reads):
+        note(f"reads -> table s{i}")
+        path = tmp_path / f"s{i}.fq.gz"
+        path.write_bytes(gzip.compress(fastq_bytes(r), compresslevel=1))
+        c = hipibs.KmerCounter(25)
+        c.add_file(str(path))
+        t = c.build_table(2, ALL)
+        assert t.info()["n_keys"] == expected["small"][f"s{i}"]["n_keys"]
+        prefix = str(tmp_path / f"db{i}")
+        f = c.write_kmc(prefix, 2, 1_000_000_000, 255)
+        assert f["kept"] == expected["small"][f"s{i}"]["n_keys"]
+        c.free()
+        info = hipibs.kmc_header(prefix)
+        assert (info.kmer_length, info.counter_size, info.min_count, info.total_kmers) == (
+            25, 1, 2, f["kept"])
+        t2 = hipibs.Table.from_kmc(prefix)
+        with open(os.path.join(ROOT, "tests", "golden", f"small_s{i}_50000.tsv"), "rb") as fh:
+            golden = fh.read()
+        for tab in (t, t2):
+            rows = hipibs.scan_sequences([tab], [b"chr1"], [seq], 50000)[0]
+            assert hipibs.format_tsv(rows) == golden
+            tab.free()
+
+
+@pytest.mark.parametrize("chunk", [64, 4096, 1 << 20])
+def test_chunk_seams_and_growth(tmp_path, chunk):
+    """Long multi-line FASTA records cut by small chunks, a table grown
+    from its smallest size many times: the counts do not change."""
+    rng = np.random.default_rng(chunk)
+    recs = []
+    for n in (200_000, 31, 32, 0, 90_001, 5):
+        r = rng.integers(0, 4, n).astype(np.uint8)
+        if n > 1000:
+            r[rng.integers(0, n, n // 500)] = 4
+            r[1000:1100] = 4
+        recs.append(r)
+    path = tmp_path / "long.fa"
+    path.write_bytes(fasta_bytes(recs))
+    k = 31
+    want_k, want_c, total = ragged_counts(recs, k)
+    note(f"chunk {chunk}")
+    c = hipibs.KmerCounter(k, chunk_bytes=chunk)
+    c.add_file(str(path))
+    st = c.stats()
+    assert st["kmers"] == total and st["distinct"] == want_k.size
+    assert st["reads"] == 6  # the empty record counts as a read with no k-mers
+    keys, counts, _ = c.keys(1, ALL, ALL)
+    keys, counts = sorted_pairs(keys, counts)
+    np.testing.assert_array_equal(keys, want_k)
+    np.testing.assert_array_equal(counts, want_c)
+    c.free()
+
+
+@pytest.mark.parametrize("k", [1, 9, 15, 32])
+def test_edge_reads(tmp_path, k):
+    """Reads shorter than k, all N, lowercase, CRLF line ends (the '\\r' is
+    a non-base byte), text passed directly, every k in range."""
+    rng = np.random.default_rng(100 + k)
+    reads = [rng.integers(0, 4, int(n)).astype(np.uint8) for n in rng.integers(0, 120, 300)]
+    reads[3] = np.full(50, 4, np.uint8)
+    reads[4] = np.zeros(k - 1 if k > 1 else 0, np.uint8)
+    reads[5] = np.zeros(k, np.uint8)
+    lines = []
+    for i, r in enumerate(reads):
+        s = corpus.to_text(r, lower=bool(i % 2))
+        lines.append(b"@q%d\r\n%s\r\n+\r\n%s\r\n" % (i, s, b"I" * len(s)))
+    path = tmp_path / "crlf.fq"
+    path.write_bytes(b"".join(lines))
+    want_k, want_c, total = ragged_counts(reads, k)
+    c = hipibs.KmerCounter(k, chunk_bytes=1000)
+    c.add_file(str(path))
+    # the same reads once more, as one text blob of '\n'-separated reads
+    c.add_text(b"\n".join(corpus.to_text(r) for r in reads))
+    st = c.stats()
+    assert st["kmers"] == 2 * total
+    keys, counts, _ = c.keys(1, ALL, ALL)
+    keys, counts = sorted_pairs(keys, counts)
+    np.testing.assert_array_equal(keys, want_k)
+    np.testing.assert_array_equal(counts, 2 * want_c)
+    c.free()
+
+
+def test_filters(small_reads):
+    _, reads = small_reads
+    r = reads[2][:20000]
+    c = hipibs.KmerCounter(25, expected_keys=10)
+    c.add_text(b"\n".join(corpus.to_text(x) for x in r))
+    all_k, all_c = corpus.count_kmers(r, 25, ci=1, cs=ALL)
+    for ci, cx, cs in ((2, ALL, 255), (3, 7, 5), (1, 1, 1), (50, ALL, 255)):
+        keys, counts, f = c.keys(ci, cx, cs)
+        keys, counts = sorted_pairs(keys, counts)
+        keep = (all_c >= ci) & (all_c <= cx)
+        np.testing.assert_array_equal(keys, all_k[keep])
+        np.testing.assert_array_equal(counts, np.minimum(all_c[keep], cs))
+        assert f == {"kept": int(keep.sum()), "below": int((all_c < ci).sum()),
+                     "above": int((all_c > cx).sum())}
+    c.free()
+
+
+def test_kmc_command(tmp_path, small_reads):
+    """bin/kmc with the reference README's options over an @list of gzip
+    FASTQ files: one database of both files' k-mers."""
+    _, reads = small_reads
+    halves = (reads[1][:30000], reads[1][30000:])
+    names = []
+    for i, h in enumerate(halves):
+        p = tmp_path / f"part{i}.fastq.gz"
+        p.write_bytes(gzip.compress(fastq_bytes(h), compresslevel=1))
+        names.append(str(p))
+    lst = tmp_path / "S1.reads"
+    lst.write_text("\n".join(names) + "\n")
+    (tmp_path / "tmp").mkdir()
+    out = str(tmp_path / "S1")
+    js = str(tmp_path / "S1.json")
+    r = subprocess.run([os.path.join(BIN, "kmc"), "-k25", "-ci2", "-cs255", "-t16", "-m32",
+                        f"-j{js}", f"@{lst}", out, str(tmp_path / "tmp")],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    want_k, _ = corpus.count_kmers(reads[1], 25, ci=2, cs=255)
+    assert "No. of unique counted k-mers       : %12d" % want_k.size in r.stdout
+    np.testing.assert_array_equal(np.sort(hipibs.kmc_read_keys(out)), want_k)
+    info = hipibs.kmc_header(out)
+    assert (info.kmc_version, info.signature_len, info.counter_size) == (0x200, 9, 1)
+    assert os.path.getsize(js) > 0
+
+
+def test_counter_errors(tmp_path):
+    for k in (0, 33):
+        with pytest.raises(hipibs.HipibsError):
+            hipibs.KmerCounter(k)
+    c = hipibs.KmerCounter(25)
+    with pytest.raises(hipibs.HipibsError, match="cannot open"):
+        c.add_file(str(tmp_path / "missing.fq"))
+    c.free()
+    bad = tmp_path / "bad.txt"
+    bad.write_bytes(b"ACGT\n")
+    c = hipibs.KmerCounter(25)
+    with pytest.raises(hipibs.HipibsError, match="unknown file type"):
+        c.add_file(str(bad))
+    with pytest.raises(hipibs.HipibsError, match="earlier call failed"):
+        c.add_text(b"ACGT")
+    c.free()
+    c = hipibs.KmerCounter(25)
+    with pytest.raises(hipibs.HipibsError):
+        c.keys(5, 2, 255)
+    c.free()

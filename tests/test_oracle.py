@@ -16,6 +16,10 @@ Each oracle function is checked against something independent of it:
   orc_intersect         the golden KDBIntersect counts and a Python set
   orc_kmc_read          keys written by the KMC writer, read back and listed
                         by the reference's KmcRaIndex when the databases were made
+  corpus.count_kmers    (the k-mer counter's oracle: KMC's counting rules) a
+                        dictionary count over substrings of each read's text
+                        with the string canonical form; the golden key counts
+                        of the three samples, which the reference's code read
 """
 import os
 
@@ -217,3 +221,46 @@ def test_oracle_kmc_reader(tmp_path, variant, expected):
     n = oracle().orc_kmc_read(pre.encode(), out.ctypes.data, out.size, C.byref(kk))
     assert n == keys.size == expected["kmc_formats"][name]["n_keys"] and kk.value == 25
     assert np.array_equal(np.sort(out), np.sort(keys))
+
+
+def brute_count(texts, k):
+    """Counts of canonical k-mers over the reads' text, by substrings."""
+    counts = {}
+    for t in texts:
+        for v in string_canonical(t, k):
+            if v is not None:
+                counts[v] = counts.get(v, 0) + 1
+    return counts
+
+
+@pytest.mark.parametrize("k", [1, 5, 25, 32])
+def test_count_kmers_against_string_count(k):
+    """count_kmers (numpy, vectorised over a read matrix) == a dictionary
+    count over every read's substrings; k-mers never span reads, N breaks
+    them, ci / cs filter and clamp."""
+    rng = np.random.default_rng(k)
+    base = rng.integers(0, 4, 400).astype(np.uint8)
+    reads = []
+    for i in range(60):
+        s = int(rng.integers(0, 400 - 40))
+        r = base[s:s + 40].copy()
+        if i % 3 == 0:
+            r = np.where(r < 4, 3 - r, 4)[::-1].copy()  # the other strand
+        if i % 7 == 0:
+            r[int(rng.integers(0, 40))] = 4
+        reads.append(r)
+    reads = np.stack(reads)
+    want = brute_count([corpus.to_text(r) for r in reads], k)
+    for ci, cs in ((1, 1 << 30), (2, 255), (3, 4)):
+        keys, counts = corpus.count_kmers(reads, k, ci=ci, cs=cs)
+        exp = {v: min(c, cs) for v, c in want.items() if c >= ci}
+        assert dict(zip(keys.tolist(), counts.tolist())) == exp
+    # the reads share k-mers, so counts above 1 and the filters are exercised
+    assert max(want.values()) > 4
+
+
+def test_count_kmers_reproduces_golden_key_counts(expected):
+    _, reads = corpus.small_reads()
+    for i, r in enumerate(reads):
+        keys, _ = corpus.count_kmers(r, 25, ci=2, cs=255)
+        assert keys.size == expected["small"][f"s{i}"]["n_keys"]

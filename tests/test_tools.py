@@ -1,5 +1,6 @@
 """The drop-in commands (fastibs, fastibsmapper, KDBIntersect): CLI rules
 and error paths on the CPU, byte-identical outputs on the GPU."""
+import gzip
 import os
 import subprocess
 
@@ -142,3 +143,40 @@ def test_cpp_host_layer_checks():
     r = tool("test_host")
     assert r.returncode == 0, r.stdout + r.stderr
     assert "host checks ok" in r.stdout
+
+
+def reader_records(path):
+    """Records the k-mer counter's reader takes from a read file."""
+    r = subprocess.run([os.path.join(BIN, "test_host"), "reads", str(path)], capture_output=True,
+                       timeout=60)
+    return r.returncode, r.stdout, r.stderr.decode()
+
+
+def test_read_file_reader(tmp_path):
+    """FASTQ with multi-line sequence and quality (quality lines starting with
+    '+' and '@'), empty lines and an empty record; multi-line FASTA; gzip;
+    CRLF kept as a non-base byte; bad input refused."""
+    fq = (b"@r1\nACGT\n+\n@@@@\n\n@r2 x\nAC\ngt\n+r2\n+I\nII\n@r3\n+\n@r4\nNNTT\n+\n"
+          b"@III\n")
+    cases = {
+        "a.fq": (fq, b"ACGT\nACgt\n\nNNTT\n", 4),
+        "a.fq.gz": (gzip.compress(fq), b"ACGT\nACgt\n\nNNTT\n", 4),
+        "b.fa": (b">a\nAC\nGT\n\n>b\n>c desc\nNNNN\r\nac\n", b"ACGT\n\nNNNN\rac\n", 3),
+        "c.fa": (b">only\n" + b"ACGT" * 5000 + b"\n", b"ACGT" * 5000 + b"\n", 1),
+    }
+    for name, (data, want, n) in cases.items():
+        p = tmp_path / name
+        p.write_bytes(data)
+        rc, out, err = reader_records(p)
+        assert rc == 0, (name, err)
+        assert out == want, name
+        assert f"records {n}" in err, name
+    for name, data, msg in (("d.txt", b"ACGT\n", "unknown file type"),
+                            ("e.fq", b"@r\nAC\n+\nII\nGG\n", "malformed FASTQ"),
+                            ("f.fq", b"", "empty file")):
+        p = tmp_path / name
+        p.write_bytes(data)
+        rc, _, err = reader_records(p)
+        assert rc == 2 and msg in err, (name, err)
+    rc, _, err = reader_records(tmp_path / "missing.fq")
+    assert rc == 2 and "cannot open" in err
