@@ -104,11 +104,12 @@ __device__ inline uint32_t waveSum(uint32_t v)
     return v;
 }
 
+// Counts the k-mers at positions skip .. n-k of text (16 B aligned).
 // counters: [0] distinct keys stored, [1] k-mers counted, [2] 1 if an
 // insert found no slot.
 __global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__restrict__ text,
-                                                           uint64_t n, uint32_t k, uint64_t *tab,
-                                                           uint64_t nBuckets,
+                                                           uint64_t n, uint64_t skip, uint32_t k,
+                                                           uint64_t *tab, uint64_t nBuckets,
                                                            unsigned long long *counters)
 {
     __shared__ __align__(16) uint8_t tile[kTileBytes];
@@ -149,7 +150,7 @@ __global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__rest
             ++run;
             fwd = ((fwd << 2) | c) & mask;
             rc = (rc >> 2) | ((uint64_t)(3 - c) << shift);
-            if (run >= k)
+            if (run >= k && base + i + 1 >= skip + k)
             {
                 bool isNew;
                 full |= !countAdd(tab, nBuckets, min(fwd, rc), 1, &isNew);
@@ -352,12 +353,44 @@ struct hipibs_counter : ReadSink
         if (rc)
             return rc;
         HIPIBS_CHECK(hipMemcpyAsync(dText, hText[slot], n, hipMemcpyHostToDevice, stream));
-        countKernel<<<gridFor(positions, kTilePositions, device), kCountBlock, 0, stream>>>(
-            dText, n, k, dTab, nBuckets, dCounters);
+        return launchCount(dText, n, 0);
+    }
+
+    // Counts positions skip .. n-k of device text (room reserved).
+    int launchCount(const uint8_t *text, uint64_t n, uint64_t skip)
+    {
+        countKernel<<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0, stream>>>(
+            text, n, skip, k, dTab, nBuckets, dCounters);
         HIPIBS_CHECK(hipGetLastError());
         HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 3 * sizeof(unsigned long long),
                                     hipMemcpyDeviceToHost, stream));
         return 0;
+    }
+
+    // Device-resident text in pieces of at most chunkBytes: a piece starts
+    // 16 B aligned at or before the previous piece's last k-1 bytes and
+    // skips the positions that piece counted.
+    int countDevice(const uint8_t *text, uint64_t n)
+    {
+        uint64_t off = 0, from = 0; // piece start, first position to count
+        while (n >= k && from <= n - k)
+        {
+            const uint64_t len = std::min<uint64_t>(chunkBytes, n - off);
+            if (len < k || off + len - k + 1 <= from)
+                break;
+            int rc = sync();
+            if (!rc)
+                rc = reserve(distinct + (off + len - k + 1 - from));
+            if (!rc)
+                rc = launchCount(text + off, len, from - off);
+            if (rc)
+                return rc;
+            if (off + len == n)
+                break;
+            from = off + len - k + 1;
+            off = from & ~15ull;
+        }
+        return sync();
     }
 
     int sync()
@@ -521,6 +554,24 @@ extern "C" int hipibs_counter_add_text(hipibs_counter *c, const char *h_text, ui
     c->bases(h_text, n);
     c->recordEnd();
     return c->err;
+}
+
+extern "C" int hipibs_counter_add_text_device(hipibs_counter *c, const char *d_text, uint64_t n)
+{
+    if (!c || (n && !d_text))
+        return setError(HIPIBS_E_ARG, "null argument");
+    if (reinterpret_cast<uintptr_t>(d_text) & 15)
+        return setError(HIPIBS_E_ARG, "k-mer counter: device text must be 16 B aligned");
+    if (c->err)
+        return setError(c->err, "k-mer counter: an earlier call failed");
+    HIPIBS_CHECK(hipSetDevice(c->device));
+    int rc = c->finish(); // text buffered by earlier calls goes first
+    if (!rc)
+        rc = c->countDevice(reinterpret_cast<const uint8_t *>(d_text), n);
+    if (rc)
+        return c->err = rc;
+    c->reads += 1;
+    return 0;
 }
 
 extern "C" int hipibs_counter_add_file(hipibs_counter *c, const char *path)

@@ -113,6 +113,7 @@ _SIGS = [
                                         C.POINTER(vp)]),
     ("hipibs_counter_add_file", C.c_int, [vp, C.c_char_p]),
     ("hipibs_counter_add_text", C.c_int, [vp, C.c_char_p, C.c_uint64]),
+    ("hipibs_counter_add_text_device", C.c_int, [vp, vp, C.c_uint64]),
     ("hipibs_counter_stats", C.c_int, [vp, C.POINTER(CountStats)]),
     ("hipibs_counter_keys", C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, u64p, u32p,
                                       C.c_uint64, C.POINTER(CountFilter)]),
@@ -654,6 +655,11 @@ class KmerCounter:
     def add_text(self, text: bytes) -> None:
         _check(_lib.hipibs_counter_add_text(self.handle, text, len(text)),
                "hipibs_counter_add_text")
+
+    def add_text_device(self, d_text: int, n: int) -> None:
+        """Counts n bytes of text at device address d_text (16 B aligned)."""
+        _check(_lib.hipibs_counter_add_text_device(self.handle, vp(d_text), n),
+               "hipibs_counter_add_text_device")
 
     def stats(self) -> dict:
         st = CountStats()

@@ -241,6 +241,11 @@ int hipibs_counter_add_file(hipibs_counter *c, const char *path);
 /* Counts n bytes of text as one record: any byte other than ACGTacgt (a
  * '\n', say) ends a run of k-mers, so several reads may be passed at once. */
 int hipibs_counter_add_text(hipibs_counter *c, const char *h_text, uint64_t n);
+/* Counts n bytes of device-resident text (16 B aligned, on the counter's
+ * device) as one record, by the rules of hipibs_counter_add_text; returns
+ * when the device has counted it. For callers whose reads are already in
+ * HBM. */
+int hipibs_counter_add_text_device(hipibs_counter *c, const char *d_text, uint64_t n);
 /* Waits for the device and reports totals. */
 int hipibs_counter_stats(hipibs_counter *c, hipibs_count_stats *out);
 /* The k-mers with min_count <= count <= max_count, in no particular order,

@@ -173,6 +173,29 @@ def test_edge_reads(tmp_path, k):
     c.free()
 
 
+@pytest.mark.parametrize("chunk", [64, 1000, 0])
+def test_device_text_equals_oracle(small_reads, chunk):
+    """Text already in HBM, counted in aligned overlapping pieces; a
+    misaligned pointer is refused."""
+    _, reads = small_reads
+    r = reads[0][:3000]
+    text = b"\n".join(corpus.to_text(x) for x in r)
+    want_k, want_c = corpus.count_kmers(r, 25, ci=1, cs=ALL)
+    _, valid = corpus.canonical_codes(r, 25)
+    buf = hipibs.DeviceBuffer.from_array(np.frombuffer(text, np.uint8))
+    c = hipibs.KmerCounter(25, chunk_bytes=chunk)
+    c.add_text_device(buf.ptr, len(text))
+    assert c.stats()["kmers"] == int(valid.sum())
+    keys, counts, _ = c.keys(1, ALL, ALL)
+    keys, counts = sorted_pairs(keys, counts)
+    np.testing.assert_array_equal(keys, want_k)
+    np.testing.assert_array_equal(counts, want_c)
+    with pytest.raises(hipibs.HipibsError, match="aligned"):
+        c.add_text_device(buf.ptr + 3, 100)
+    c.free()
+    buf.free()
+
+
 def test_filters(small_reads):
     _, reads = small_reads
     r = reads[2][:20000]
