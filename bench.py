@@ -17,6 +17,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -66,10 +67,16 @@ def parse():
                     help="sample popularity ~ 1/rank^s (0 = uniform)")
     ap.add_argument("--serve-rates", default="5,20,50,100,200")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
-    ap.add_argument("--workload", default="scan", choices=["scan", "panel", "serve", "large"],
+    ap.add_argument("--count-mbp", type=float, default=200.0,
+                    help="count workload: genome length in Mbp (10x of reads)")
+    ap.add_argument("--count-cpu-reads", type=float, default=2000000,
+                    help="count workload: reads the CPU oracle counts")
+    ap.add_argument("--workload", default="scan",
+                    choices=["scan", "panel", "serve", "large", "count"],
                     help="scan: the single-sample device scan (default line); panel: "
                          "end-to-end fastibs runs, N single-sample runs vs one panel run; "
-                         "serve: served queries; large: the k=31 4.2 Gbp scan + coverage")
+                         "serve: served queries; large: the k=31 4.2 Gbp scan + coverage; "
+                         "count: k-mer counting from reads")
     ap.add_argument("--cpu-mapper-mbp", type=float, default=8.0,
                     help="large: bases (Mbp) of the reference mapper's CPU sample")
     ap.add_argument("--e2e", type=int, default=1,
@@ -974,6 +981,180 @@ def run_large(args):
     print(json.dumps(out), flush=True)
 
 
+def gen_reads_text(torch, hap, n_reads, read_len, err, seed):
+    """Reads of a haplotype as device text: uniform starts, random strand,
+    i.i.d. substitution errors (N stays N); each read followed by '\n'."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    lut = torch.tensor(list(b"ACGTN"), dtype=torch.uint8, device="cuda")
+    out = torch.empty((n_reads, read_len + 1), dtype=torch.uint8, device="cuda")
+    out[:, read_len] = ord("\n")
+    ar = torch.arange(read_len, device="cuda")
+    for r0 in range(0, n_reads, 1 << 20):
+        m = min(1 << 20, n_reads - r0)
+        starts = torch.randint(0, hap.numel() - read_len + 1, (m,), device="cuda", generator=g)
+        r = hap[starts[:, None] + ar[None, :]]
+        e = (torch.rand(r.shape, device="cuda", generator=g) < err) & (r < 4)
+        d = torch.randint(1, 4, r.shape, dtype=torch.uint8, device="cuda", generator=g)
+        r = torch.where(e, (r + d) % 4, r)
+        flip = torch.rand(m, device="cuda", generator=g) < 0.5
+        r[flip] = torch.where(r[flip] < 4, 3 - r[flip], r[flip]).flip(1)
+        out[r0:r0 + m, :read_len] = lut[r.long()]
+    return out.view(-1)
+
+
+def run_count(args):
+    """k-mer counting from reads (hipibs_counter_*): 10x of 150 bp reads of
+    a 1%-SNP haplotype of a 200 Mbp repeat-rich genome (0.5% substitution
+    errors, both strands), k=25, as device text. A step counts every read
+    into a fresh counter (table growth included) and reports the totals."""
+    import numpy as np
+    import torch
+
+    import hipibs
+    assert torch.cuda.is_available(), "no GPU visible"
+    G, cov, L, err = int(args.count_mbp * 1e6) // 64 * 64, 10.0, 150, 0.005
+    n_reads = int(cov * G / L)
+    t0 = time.time()
+    ref = gen_reference(torch, G, args.seed + 50)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(args.seed + 51)
+    snp = (torch.rand(G, device="cuda", generator=g) < 0.01) & (ref < 4)
+    hap = torch.where(snp, (ref + torch.randint(1, 4, (G,), dtype=torch.uint8, device="cuda",
+                                                generator=g)) % 4, ref)
+    del ref, snp
+    text = gen_reads_text(torch, hap, n_reads, L, err, args.seed + 52)
+    del hap
+    torch.cuda.synchronize()
+    n = text.numel()
+    log(f"reads: {n_reads} x {L} bp as {n/1e9:.2f} GB of text ({time.time()-t0:.1f} s)")
+
+    def step():
+        c = hipibs.KmerCounter(K)
+        c.add_text_device(text.data_ptr(), n)
+        st = c.stats()
+        return c, st
+
+    for _ in range(args.warmup):
+        step()[0].free()
+    torch.cuda.synchronize()
+    stats = []
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        c, st = step()
+        stats.append(st)
+        c.free()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - ts
+    ms = elapsed / args.steps * 1e3
+    st = stats[-1]
+    stable = all((x["kmers"], x["distinct"]) == (st["kmers"], st["distinct"]) for x in stats)
+    k_ms = float(np.mean([x["kernel_ms"] for x in stats]))
+    launches = st["launches"]
+    alg_bytes = 80.0 * st["kmers"] + n  # per k-mer: 64 B of keys + the 8 B count read and written
+    log(f"count: {ms:.2f} ms/step, kernels {k_ms:.2f} ms in {launches} launches, "
+        f"{st['kmers']/1e9:.3f} G k-mers, {st['distinct']/1e6:.1f} M distinct, "
+        f"table {st['table_bytes']/1e9:.1f} GB, stable {stable}")
+
+    # the oracle's counter (oracle.c: encode, radix sort, run-length; KMC's
+    # rules) on the first reads, and the GPU on the same reads
+    cpu = None
+    match = None
+    try:
+        from util import oracle_count
+        n_cpu = min(n_reads, int(args.count_cpu_reads))
+        if n_cpu <= 0:
+            raise RuntimeError("skipped (--count-cpu-reads 0)")
+        sample = text[:n_cpu * (L + 1)].cpu().numpy().tobytes()
+        tc = time.perf_counter()
+        ck, cc, n_k = oracle_count(sample, K)
+        cpu_s = time.perf_counter() - tc
+        c = hipibs.KmerCounter(K)
+        c.add_text_device(text.data_ptr(), n_cpu * (L + 1))
+        gk, gc, _ = c.keys(1, 0xFFFFFFFF, 0xFFFFFFFF)
+        c.free()
+        o = np.argsort(gk)
+        match = bool(np.array_equal(gk[o], ck) and np.array_equal(gc[o], cc))
+        cpu = {"value": n_k / cpu_s, "unit": "k-mers counted/s", "cores": 1, "kind": "port",
+               "sample": f"the oracle's counter (C: encode, LSD radix sort, run-length; KMC's "
+                         f"rules) over the first {n_cpu} reads ({n_k} k-mers) in {cpu_s:.1f} s; "
+                         f"GPU keys and counts identical: {match}"}
+        log(f"cpu oracle: {n_k/cpu_s/1e6:.1f} M k-mers/s on {n_cpu} reads, identical {match}")
+    except Exception as e:  # the baseline is reported, never required
+        log(f"cpu baseline failed: {e}")
+
+    # end to end: bin/kmc on the reads as a FASTQ file (parse, count, write KMC)
+    e2e = None
+    if args.e2e:
+        try:
+            d = tempfile.mkdtemp(dir=args.workdir)
+            fq = os.path.join(d, "reads.fq")
+            host = text.cpu().numpy().reshape(n_reads, L + 1)
+            qual = np.full((1, L + 1), ord("I"), np.uint8)
+            qual[0, L] = ord("\n")
+            with open(fq, "wb") as f:
+                for r0 in range(0, n_reads, 1 << 18):
+                    blk = host[r0:r0 + (1 << 18)]
+                    m = blk.shape[0]
+                    rec = np.empty((m, 3 + (L + 1) + 2 + (L + 1)), np.uint8)
+                    rec[:, 0:3] = np.frombuffer(b"@r\n", np.uint8)
+                    rec[:, 3:3 + L + 1] = blk
+                    rec[:, 4 + L:6 + L] = np.frombuffer(b"+\n", np.uint8)
+                    rec[:, 6 + L:] = qual
+                    f.write(rec.tobytes())
+            del host
+            size = os.path.getsize(fq)
+            tk = time.perf_counter()
+            r = subprocess.run([os.path.join(ROOT, "gpu-ibs_amd", "bin", "kmc"), f"-k{K}",
+                                "-ci2", "-cs255", fq, os.path.join(d, "db"), d],
+                               capture_output=True, text=True, timeout=600)
+            kmc_s = time.perf_counter() - tk
+            ok = r.returncode == 0
+            e2e = {"kmc_wall_s": kmc_s, "fastq_bytes": size, "ok": ok,
+                   "stdout": r.stdout.strip().splitlines()[-8:] if ok else r.stderr[-300:]}
+            log(f"kmc -k{K} -ci2 -cs255 on a {size/1e9:.2f} GB FASTQ: {kmc_s:.2f} s, ok {ok}")
+            for name in ("reads.fq", "db.kmc_pre", "db.kmc_suf"):
+                if os.path.exists(os.path.join(d, name)):
+                    os.remove(os.path.join(d, name))
+            os.rmdir(d)
+        except Exception as e:
+            log(f"kmc end-to-end failed: {e}")
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "count_k25_200mbp_traffic.json")
+    if os.path.exists(tpath) and G == 200_000_000:
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_dispatch"]
+    value = st["kmers"] / (ms / 1e3)
+    achieved = alg_bytes / (k_ms / 1e3) / 1e9
+    out = {
+        "metric": "k-mers counted/s",
+        "value": value,
+        "unit": "k-mers counted/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (GPU-generated, seeded)",
+        "config": {"workload": f"k-mer counting, k={K}: {n_reads} reads of {L} bp (10x of a "
+                               f"{G/1e6:.0f} Mbp genome, 1% SNPs, 0.5% errors) as device text",
+                   "text_bytes": n, "kmers": st["kmers"], "distinct": st["distinct"],
+                   "table_bytes": st["table_bytes"], "repeat_stable": stable,
+                   "kmc_end_to_end": e2e},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "countKernel", "kernel_ms_per_step": k_ms,
+                     "launches_per_step": launches,
+                     "algorithmic_bytes": alg_bytes,
+                     "kmers_per_s_in_kernel": st["kmers"] / (k_ms / 1e3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -986,6 +1167,8 @@ def main():
         return run_serve(args)
     if args.workload == "large":
         return run_large(args)
+    if args.workload == "count":
+        return run_count(args)
     import numpy as np
     import torch
 

@@ -180,7 +180,9 @@ __global__ __launch_bounds__(256) void countFillKernel(uint64_t *tab, uint64_t w
         tab[i] = (i % kCountWords) < kCountSlots ? EMPTY : 0ull;
 }
 
-// Moves every (key, count) of the old table into the new one.
+// Moves every (key, count) of the old table into the new one. Keys are
+// distinct, so the thread that stores a key is the only writer of its
+// count: a plain store.
 __global__ __launch_bounds__(256) void countRehashKernel(const uint64_t *__restrict__ old,
                                                          uint64_t oldBuckets, uint64_t *tab,
                                                          uint64_t nBuckets,
@@ -194,8 +196,31 @@ __global__ __launch_bounds__(256) void countRehashKernel(const uint64_t *__restr
         const uint64_t key = old[b * kCountWords + s];
         if (key == EMPTY)
             continue;
-        bool isNew;
-        if (!countAdd(tab, nBuckets, key, old[b * kCountWords + kCountSlots + s], &isNew))
+        const uint64_t c = old[b * kCountWords + kCountSlots + s];
+        uint64_t h = bucketOf(key, nBuckets);
+        bool stored = false;
+        for (uint64_t d = 0; d < nBuckets && !stored; ++d)
+        {
+            unsigned long long *bucket =
+                reinterpret_cast<unsigned long long *>(tab + h * kCountWords);
+            const ulonglong2 *kv = reinterpret_cast<const ulonglong2 *>(bucket);
+            uint64_t keys[kCountSlots];
+#pragma unroll
+            for (int q = 0; q < kCountSlots / 2; ++q)
+            {
+                const ulonglong2 v = kv[q];
+                keys[2 * q] = v.x;
+                keys[2 * q + 1] = v.y;
+            }
+            for (int q = 0; q < kCountSlots && !stored; ++q)
+                if (keys[q] == EMPTY && atomicCAS(bucket + q, EMPTY, key) == EMPTY)
+                {
+                    bucket[kCountSlots + q] = c;
+                    stored = true;
+                }
+            h = h + 1 == nBuckets ? 0 : h + 1;
+        }
+        if (!stored)
             atomicExch(counters + 2, 1ull);
     }
 }
@@ -284,8 +309,12 @@ struct hipibs_counter : ReadSink
     uint64_t nBuckets = 0;
     unsigned long long *dCounters = nullptr; // [0] distinct, [1] counted, [2] full, [3..5] filter
     unsigned long long *hCounters = nullptr; // pinned copy
-    char *hText[2] = {nullptr, nullptr};     // pinned chunks
+    char *hText[2] = {nullptr, nullptr};     // pinned chunks (host text only)
     uint8_t *dText = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr}; // around the last count launch
+    bool evPending = false;
+    double kernelMs = 0;                   // device time of the count launches
+    uint64_t launches = 0;
     int cur = 0;          // chunk being filled
     uint64_t len = 0;     // bytes in it
     uint64_t recLen = 0;  // bases of the open record so far
@@ -359,9 +388,13 @@ struct hipibs_counter : ReadSink
     // Counts positions skip .. n-k of device text (room reserved).
     int launchCount(const uint8_t *text, uint64_t n, uint64_t skip)
     {
+        HIPIBS_CHECK(hipEventRecord(ev[0], stream));
         countKernel<<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0, stream>>>(
             text, n, skip, k, dTab, nBuckets, dCounters);
         HIPIBS_CHECK(hipGetLastError());
+        HIPIBS_CHECK(hipEventRecord(ev[1], stream));
+        evPending = true;
+        launches += 1;
         HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 3 * sizeof(unsigned long long),
                                     hipMemcpyDeviceToHost, stream));
         return 0;
@@ -396,6 +429,13 @@ struct hipibs_counter : ReadSink
     int sync()
     {
         HIPIBS_CHECK(hipStreamSynchronize(stream));
+        if (evPending)
+        {
+            float ms = 0;
+            HIPIBS_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            kernelMs += ms;
+            evPending = false;
+        }
         if (hCounters[2])
             return setError(HIPIBS_E_FULL, "k-mer counter: an insert found no free slot");
         distinct = hCounters[0];
@@ -403,8 +443,8 @@ struct hipibs_counter : ReadSink
         return 0;
     }
 
-    // Makes room for `need` keys at <= kMaxLoad, growing to kGrowLoad
-    // (called with the stream idle).
+    // Makes room for `need` keys at <= kMaxLoad, growing to kGrowLoad and
+    // at least doubling (called with the stream idle).
     int reserve(uint64_t need)
     {
         if ((double)need <= kMaxLoad * (double)(nBuckets * kCountSlots))
@@ -436,6 +476,18 @@ struct hipibs_counter : ReadSink
         }
         dTab = t;
         nBuckets = want;
+        return 0;
+    }
+
+    // The pinned chunks and the device chunk, on first use of host text.
+    int hostBuffers()
+    {
+        if (dText)
+            return 0;
+        for (int i = 0; i < 2; ++i)
+            HIPIBS_CHECK(hipHostMalloc(reinterpret_cast<void **>(&hText[i]), chunkBytes,
+                                       hipHostMallocDefault));
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dText), chunkBytes));
         return 0;
     }
 
@@ -525,10 +577,7 @@ extern "C" int hipibs_counter_create(uint32_t k, int device, uint64_t expected_k
         e = hipHostMalloc(reinterpret_cast<void **>(&c->hCounters), 8 * sizeof(unsigned long long),
                           hipHostMallocDefault);
     for (int i = 0; i < 2 && e == hipSuccess; ++i)
-        e = hipHostMalloc(reinterpret_cast<void **>(&c->hText[i]), chunk_bytes,
-                          hipHostMallocDefault);
-    if (e == hipSuccess)
-        e = hipMalloc(reinterpret_cast<void **>(&c->dText), chunk_bytes);
+        e = hipEventCreate(&c->ev[i]);
     if (e == hipSuccess)
         e = hipMemset(c->dCounters, 0, 8 * sizeof(unsigned long long));
     if (e != hipSuccess)
@@ -551,6 +600,8 @@ extern "C" int hipibs_counter_add_text(hipibs_counter *c, const char *h_text, ui
     if (c->err)
         return setError(c->err, "k-mer counter: an earlier call failed");
     HIPIBS_CHECK(hipSetDevice(c->device));
+    if ((c->err = c->hostBuffers()))
+        return c->err;
     c->bases(h_text, n);
     c->recordEnd();
     return c->err;
@@ -581,6 +632,8 @@ extern "C" int hipibs_counter_add_file(hipibs_counter *c, const char *path)
     if (c->err)
         return setError(c->err, "k-mer counter: an earlier call failed");
     HIPIBS_CHECK(hipSetDevice(c->device));
+    if ((c->err = c->hostBuffers()))
+        return c->err;
     const int rc = readRecords(path, *c, nullptr);
     if (rc && !c->err)
         c->err = rc; // the file's records before the error may be counted: the counter is spent
@@ -598,6 +651,8 @@ extern "C" int hipibs_counter_stats(hipibs_counter *c, hipibs_count_stats *out)
     out->reads = c->reads;
     out->kmers = c->counted;
     out->distinct = c->distinct;
+    out->kernel_ms = c->kernelMs;
+    out->launches = c->launches;
     out->table_bytes = c->nBuckets * kCountWords * 8;
     return 0;
 }
@@ -754,6 +809,9 @@ extern "C" void hipibs_counter_free(hipibs_counter *c)
             (void)hipHostFree(h);
     if (c->hCounters)
         (void)hipHostFree(c->hCounters);
+    for (hipEvent_t e : c->ev)
+        if (e)
+            (void)hipEventDestroy(e);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;

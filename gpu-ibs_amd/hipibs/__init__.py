@@ -70,7 +70,8 @@ class ServeCounters(C.Structure):
 
 
 class CountStats(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in ("reads", "kmers", "distinct", "table_bytes")]
+    _fields_ = [(n, C.c_uint64) for n in ("reads", "kmers", "distinct", "table_bytes",
+                                         "launches")] + [("kernel_ms", C.c_double)]
 
 
 class CountFilter(C.Structure):

@@ -221,6 +221,8 @@ typedef struct hipibs_count_stats {
     uint64_t kmers;       /* k-mers counted: positions with k valid bases */
     uint64_t distinct;    /* distinct canonical k-mers */
     uint64_t table_bytes; /* device bytes of the counting table */
+    uint64_t launches;    /* counting kernel launches */
+    double kernel_ms;     /* their device time (HIP events on the counter's stream) */
 } hipibs_count_stats;
 
 typedef struct hipibs_count_filter {
@@ -231,7 +233,8 @@ typedef struct hipibs_count_filter {
 
 /* k in 1..32. expected_keys (0 = unknown) sizes the first table; the table
  * grows as needed. chunk_bytes (0 = 128 MiB) is the text moved to the device
- * at a time: two pinned host buffers and one device buffer of that size. */
+ * at a time: two pinned host buffers and one device buffer of that size,
+ * allocated when host text first arrives. */
 int hipibs_counter_create(uint32_t k, int device, uint64_t expected_keys, uint64_t chunk_bytes,
                           hipibs_counter **out);
 /* Counts a FASTA or FASTQ file, plain or gzip (decided by its first byte);

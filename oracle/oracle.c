@@ -372,3 +372,70 @@ int64_t orc_scan_contig(const char *seq, int64_t n, int64_t W, uint32_t k, const
     }
     return nw;
 }
+
+/* k-mer counting with KMC 3.2.4's rules, the step that makes the
+ * reference's sample databases (README.md:95 `kmc -k25 -ci2 -cs255`; KMC
+ * is an external tool, not in the reference's sources): every position of
+ * text whose k bytes are ACGTacgt (a record separator or N ends a run)
+ * gives its canonical code (orc_encode_positions); the codes are sorted and
+ * each distinct code is written with its number of occurrences, ascending.
+ * Writes at most cap pairs; returns the number of distinct codes and the
+ * number of k-mers in *n_kmers. */
+static void radix_sort_u64(uint64_t *a, uint64_t *tmp, int64_t n)
+{
+    for (int shift = 0; shift < 64; shift += 8)
+    {
+        int64_t count[257] = {0};
+        for (int64_t i = 0; i < n; ++i)
+            count[((a[i] >> shift) & 255) + 1] += 1;
+        for (int b = 0; b < 256; ++b)
+            count[b + 1] += count[b];
+        for (int64_t i = 0; i < n; ++i)
+            tmp[count[(a[i] >> shift) & 255]++] = a[i];
+        memcpy(a, tmp, (size_t)n * sizeof(uint64_t));
+    }
+}
+
+int64_t orc_count_kmers(const char *text, int64_t n, uint32_t k, uint64_t *keys,
+                        uint32_t *counts, int64_t cap, int64_t *n_kmers)
+{
+    *n_kmers = 0;
+    if (k == 0 || k > 32 || n < (int64_t)k)
+        return 0;
+    const int64_t np = n - k + 1;
+    uint64_t *codes = malloc((size_t)np * sizeof(uint64_t));
+    uint64_t *tmp = malloc((size_t)np * sizeof(uint64_t));
+    uint8_t *valid = malloc((size_t)np);
+    if (!codes || !tmp || !valid)
+    {
+        free(codes);
+        free(tmp);
+        free(valid);
+        return -1;
+    }
+    orc_encode_positions(text, n, k, codes, valid);
+    int64_t m = 0;
+    for (int64_t i = 0; i < np; ++i)
+        if (valid[i])
+            codes[m++] = codes[i];
+    free(valid);
+    radix_sort_u64(codes, tmp, m);
+    free(tmp);
+    int64_t d = 0;
+    for (int64_t i = 0; i < m;)
+    {
+        int64_t j = i;
+        while (j < m && codes[j] == codes[i])
+            ++j;
+        if (d < cap)
+        {
+            keys[d] = codes[i];
+            counts[d] = (uint32_t)(j - i);
+        }
+        ++d;
+        i = j;
+    }
+    free(codes);
+    *n_kmers = m;
+    return d;
+}

@@ -16,10 +16,11 @@ Each oracle function is checked against something independent of it:
   orc_intersect         the golden KDBIntersect counts and a Python set
   orc_kmc_read          keys written by the KMC writer, read back and listed
                         by the reference's KmcRaIndex when the databases were made
-  corpus.count_kmers    (the k-mer counter's oracle: KMC's counting rules) a
-                        dictionary count over substrings of each read's text
-                        with the string canonical form; the golden key counts
-                        of the three samples, which the reference's code read
+  orc_count_kmers /     (KMC's counting rules, the k-mer counter's oracle) a
+  corpus.count_kmers    dictionary count over substrings of each read's text
+                        with the string canonical form; each other on the
+                        small samples' reads; the golden key counts of the
+                        three samples, whose databases the reference's code read
 """
 import os
 
@@ -27,8 +28,8 @@ import numpy as np
 import pytest
 
 import corpus
-from util import (ROOT, coverage_line, oracle, oracle_coverage_text, oracle_scan_rows, sha256,
-                  sorted_keys)
+from util import (ROOT, coverage_line, oracle, oracle_count, oracle_coverage_text,
+                  oracle_scan_rows, sha256, sorted_keys)
 
 import hipibs
 
@@ -255,6 +256,9 @@ def test_count_kmers_against_string_count(k):
         keys, counts = corpus.count_kmers(reads, k, ci=ci, cs=cs)
         exp = {v: min(c, cs) for v, c in want.items() if c >= ci}
         assert dict(zip(keys.tolist(), counts.tolist())) == exp
+    ok, oc, nk = oracle_count(b"\n".join(corpus.to_text(r, lower=True) for r in reads), k)
+    assert dict(zip(ok.tolist(), oc.tolist())) == want
+    assert nk == sum(want.values())
     # the reads share k-mers, so counts above 1 and the filters are exercised
     assert max(want.values()) > 4
 
@@ -262,5 +266,9 @@ def test_count_kmers_against_string_count(k):
 def test_count_kmers_reproduces_golden_key_counts(expected):
     _, reads = corpus.small_reads()
     for i, r in enumerate(reads):
-        keys, _ = corpus.count_kmers(r, 25, ci=2, cs=255)
-        assert keys.size == expected["small"][f"s{i}"]["n_keys"]
+        keys, counts = corpus.count_kmers(r, 25, ci=1, cs=1 << 31)
+        assert int((counts >= 2).sum()) == expected["small"][f"s{i}"]["n_keys"]
+        if i == 0:
+            ok, oc, _ = oracle_count(b"\n".join(corpus.to_text(x) for x in r), 25)
+            np.testing.assert_array_equal(ok, keys)
+            np.testing.assert_array_equal(oc, counts)

@@ -32,6 +32,8 @@ def oracle():
         lib.orc_kmc_read.restype = i64
         lib.orc_scan_contig.argtypes = [C.c_char_p, i64, i64, u32, vp, i64, vp, i64]
         lib.orc_scan_contig.restype = i64
+        lib.orc_count_kmers.argtypes = [C.c_char_p, i64, u32, vp, vp, i64, C.POINTER(i64)]
+        lib.orc_count_kmers.restype = i64
         _orc = lib
     return _orc
 
@@ -76,3 +78,16 @@ def coverage_line(cov):
 
 def sha256(b):
     return hashlib.sha256(b).hexdigest()
+
+
+def oracle_count(text, k):
+    """(keys, counts, k-mers) of text by the oracle's counter (ascending)."""
+    lib = oracle()
+    cap = max(1, len(text) - k + 1)
+    keys = np.zeros(cap, dtype=np.uint64)
+    counts = np.zeros(cap, dtype=np.uint32)
+    nk = C.c_int64(0)
+    d = lib.orc_count_kmers(text, len(text), k, keys.ctypes.data, counts.ctypes.data, cap,
+                            C.byref(nk))
+    assert 0 <= d <= cap
+    return keys[:d], counts[:d], nk.value
