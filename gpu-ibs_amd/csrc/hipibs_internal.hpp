@@ -24,6 +24,10 @@ namespace hipibs
 
 constexpr uint64_t EMPTY = ~0ull;
 
+// A k-mer of 33..64 bases as one host integer (device code keeps the
+// (high, low) word pair).
+typedef unsigned __int128 Wide;
+
 // Bucket of a key: a multiply-xorshift mix, then the high 32 bits scaled
 // onto [0, n_buckets) (n_buckets < 2^32).
 __host__ __device__ inline uint64_t bucketOf(uint64_t key, uint64_t nBuckets)
@@ -32,6 +36,12 @@ __host__ __device__ inline uint64_t bucketOf(uint64_t key, uint64_t nBuckets)
     h ^= h >> 29;
     h *= 0xBF58476D1CE4E5B9ull;
     return ((h >> 32) * nBuckets) >> 32;
+}
+
+// Bucket of a 33..64-base key (high word, low word).
+__host__ __device__ inline uint64_t bucketOfWide(uint64_t hi, uint64_t lo, uint64_t nBuckets)
+{
+    return bucketOf(lo ^ (hi * 0xD6E8FEB86659FD93ull) ^ (hi >> 17), nBuckets);
 }
 
 // Reverses the order of the 32 2-bit groups of x.

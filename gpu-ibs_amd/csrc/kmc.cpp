@@ -193,8 +193,8 @@ int loadPre(const std::string &prefix, PreFile &pf, bool withLut)
     const hipibs_kmc_info &I = pf.info;
     if (I.mode != 0)
         return setError(HIPIBS_E_IO, "KMC: Quake-compatible counters are not supported");
-    if (I.kmer_length == 0 || I.kmer_length > 32)
-        return setError(HIPIBS_E_IO, "KMC: k must be in 1..32 for this loader");
+    if (I.kmer_length == 0 || I.kmer_length > 64)
+        return setError(HIPIBS_E_IO, "KMC: k must be in 1..64 for this loader");
     if (I.lut_prefix_length > I.kmer_length || (I.kmer_length - I.lut_prefix_length) % 4 != 0)
         return setError(HIPIBS_E_IO, "KMC: unexpected suffix geometry");
     if (I.lut_prefix_length > 15)
@@ -341,6 +341,7 @@ int kmcReadKeys(const std::string &prefix, uint64_t *keys, uint64_t cap, uint64_
         return setError(HIPIBS_E_ARG, "null key buffer");
     const uint32_t shift = 2 * (I.kmer_length - I.lut_prefix_length);
     const uint64_t prefixMask = (1ull << (2 * I.lut_prefix_length)) - 1;
+    const bool wide = I.kmer_length > 32; // two words per key: high, low
     const std::vector<uint64_t> &lut = pf.lut;
     const uint64_t nBuckets = lut.size() - 1;
 
@@ -378,6 +379,17 @@ int kmcReadKeys(const std::string &prefix, uint64_t *keys, uint64_t cap, uint64_
             {
                 while (j + 1 < nBuckets && std::min<uint64_t>(lut[j + 1], total) <= r)
                     ++j;
+                if (wide)
+                {
+                    Wide v = 0;
+                    for (uint32_t b = 0; b < sufBytes; ++b)
+                        v = (v << 8) | q[b];
+                    if (shift < 128)
+                        v |= (Wide)(j & prefixMask) << shift;
+                    keys[2 * r] = (uint64_t)(v >> 64);
+                    keys[2 * r + 1] = (uint64_t)v;
+                    continue;
+                }
                 uint64_t v = 0;
                 for (uint32_t b = 0; b < sufBytes; ++b)
                     v = (v << 8) | q[b];
@@ -439,7 +451,8 @@ std::vector<uint32_t> normTable(uint32_t len)
     return norm;
 }
 
-uint32_t signatureOf(uint64_t code, uint32_t k, uint32_t len, const std::vector<uint32_t> &norm)
+template <typename Key>
+uint32_t signatureOf(Key code, uint32_t k, uint32_t len, const std::vector<uint32_t> &norm)
 {
     const uint32_t mask = (1u << (2 * len)) - 1;
     uint32_t cur = 0, best = 0xffffffffu;
@@ -452,14 +465,28 @@ uint32_t signatureOf(uint64_t code, uint32_t k, uint32_t len, const std::vector<
     return best;
 }
 
-} // namespace
-
-int kmcWrite(const std::string &prefix, const uint64_t *keys, const uint32_t *counts, uint64_t n,
-             uint32_t k, uint32_t p, uint32_t sigLen, uint32_t counterSize, uint32_t version,
-             uint32_t minCount, uint32_t maxCount)
+// Key i of a caller's array: one word (k <= 32) or a (high, low) pair.
+template <typename Key> Key keyAt(const uint64_t *keys, uint64_t i);
+template <> uint64_t keyAt<uint64_t>(const uint64_t *keys, uint64_t i)
 {
-    if (k == 0 || k > 32 || p > k || (k - p) % 4 != 0 || p > 12)
-        return setError(HIPIBS_E_ARG, "kmc_write: need k <= 32, p <= 12, (k - p) % 4 == 0");
+    return keys[i];
+}
+template <> Wide keyAt<Wide>(const uint64_t *keys, uint64_t i)
+{
+    return ((Wide)keys[2 * i] << 64) | keys[2 * i + 1];
+}
+
+// All ones in the low `bits` bits of Key.
+template <typename Key> Key lowMask(uint32_t bits)
+{
+    return bits >= 8 * sizeof(Key) ? ~(Key)0 : (((Key)1 << bits) - 1);
+}
+
+template <typename Key>
+int kmcWriteT(const std::string &prefix, const uint64_t *keys, const uint32_t *counts, uint64_t n,
+              uint32_t k, uint32_t p, uint32_t sigLen, uint32_t counterSize, uint32_t version,
+              uint32_t minCount, uint32_t maxCount)
+{
     if (version != 0 && version != 0x200)
         return setError(HIPIBS_E_ARG, "kmc_write: version must be 0 or 0x200");
     if (version == 0x200 && (sigLen < 5 || sigLen > 11 || sigLen > k))
@@ -483,13 +510,14 @@ int kmcWrite(const std::string &prefix, const uint64_t *keys, const uint32_t *co
     // in parallel, a counting sort by bin, then each bin sorted on its own.
     struct Rec
     {
-        uint64_t key;
+        Key key;
         uint64_t count;
     };
-    const uint64_t kmask = k == 32 ? ~0ull : (1ull << (2 * k)) - 1;
+    const Key kmask = lowMask<Key>(2 * k);
     const uint32_t shift = 2 * (k - p);
-    auto prefixOf = [shift](uint64_t key) { return shift >= 64 ? 0ull : key >> shift; };
-    const uint64_t sufMask = shift >= 64 ? ~0ull : ((1ull << shift) - 1);
+    auto prefixOf = [shift](Key key)
+    { return shift >= 8 * sizeof(Key) ? 0ull : (uint64_t)(key >> shift); };
+    const Key sufMask = lowMask<Key>(shift);
     unsigned T = std::max(1u, std::min(usableCpus(), 64u));
     if (n < (1ull << 20))
         T = 1;
@@ -516,10 +544,11 @@ int kmcWrite(const std::string &prefix, const uint64_t *keys, const uint32_t *co
                  const uint64_t hi = std::min<uint64_t>(n, lo + per);
                  for (uint64_t i = lo; i < hi; ++i)
                  {
-                     if (keys[i] & ~kmask)
+                     const Key key = keyAt<Key>(keys, i);
+                     if (key & ~kmask)
                          wide = true;
                      const uint32_t b =
-                         version == 0x200 ? sigMap[signatureOf(keys[i] & kmask, k, sigLen, norm)] : 0;
+                         version == 0x200 ? sigMap[signatureOf(key & kmask, k, sigLen, norm)] : 0;
                      binOf[i] = (uint16_t)b;
                      hist[t][b] += 1;
                  }
@@ -548,7 +577,7 @@ int kmcWrite(const std::string &prefix, const uint64_t *keys, const uint32_t *co
                  const uint64_t lo = std::min<uint64_t>(n, (uint64_t)t * per);
                  const uint64_t hi = std::min<uint64_t>(n, lo + per);
                  for (uint64_t i = lo; i < hi; ++i)
-                     recs[pos[binOf[i]]++] = Rec{keys[i], counts ? counts[i] : 1ull};
+                     recs[pos[binOf[i]]++] = Rec{keyAt<Key>(keys, i), counts ? counts[i] : 1ull};
              });
     std::vector<uint16_t>().swap(binOf);
     std::vector<uint64_t> lut((uint64_t)nBins * single + 1, 0);
@@ -599,7 +628,7 @@ int kmcWrite(const std::string &prefix, const uint64_t *keys, const uint32_t *co
         buf.insert(buf.end(), {'K', 'M', 'C', 'S'});
         for (const Rec &o : recs)
         {
-            const uint64_t suf = o.key & sufMask;
+            const Key suf = o.key & sufMask;
             for (int b = (int)sufBytes - 1; b >= 0; --b)
                 buf.push_back((unsigned char)(suf >> (8 * b)));
             const uint64_t c = std::min<uint64_t>(o.count, cmax);
@@ -660,6 +689,20 @@ int kmcWrite(const std::string &prefix, const uint64_t *keys, const uint32_t *co
             return setError(HIPIBS_E_IO, "write failed: " + path);
     }
     return 0;
+}
+
+} // namespace
+
+int kmcWrite(const std::string &prefix, const uint64_t *keys, const uint32_t *counts, uint64_t n,
+             uint32_t k, uint32_t p, uint32_t sigLen, uint32_t counterSize, uint32_t version,
+             uint32_t minCount, uint32_t maxCount)
+{
+    if (k == 0 || k > 64 || p > k || (k - p) % 4 != 0 || p > 12)
+        return setError(HIPIBS_E_ARG, "kmc_write: need k <= 64, p <= 12, (k - p) % 4 == 0");
+    return k <= 32 ? kmcWriteT<uint64_t>(prefix, keys, counts, n, k, p, sigLen, counterSize,
+                                         version, minCount, maxCount)
+                   : kmcWriteT<Wide>(prefix, keys, counts, n, k, p, sigLen, counterSize, version,
+                                     minCount, maxCount);
 }
 
 } // namespace hipibs

@@ -302,6 +302,9 @@ static int preparePanel(const hipibs_table *const *tables, int n_tables, const u
         return rc;
     if (tables[0]->mlen)
         return setError(HIPIBS_E_ARG, "panel scans need whole-k-mer bucket tables");
+    if (tables[0]->k > 32)
+        return setError(HIPIBS_E_ARG, "panel scans need k <= 32 (scan tables of k > 32 one "
+                                      "launch per table with hipibs_scan_windows_device)");
     if (n_windows && (!d_seq || !d_exc || !d_windows || !d_stats))
         return setError(HIPIBS_E_ARG, "null buffer");
     const uint32_t segs = cfg && cfg->segs_per_window ? cfg->segs_per_window : 1;

@@ -34,8 +34,9 @@ constexpr int SCAN_WAVES = SCAN_BLOCK / 64;
 
 // MIXED: every window names its own table (hipibs_window.reserved) and its
 // result goes to row w; otherwise grid.y is the table and results are
-// table-major.
-template <int S, bool MIXED>
+// table-major. WIDE: k = 33..64, two-word codes (wideCanonical) probed in a
+// table of 16 B keys (waveProbeWide); a fourth sequence word is read.
+template <int S, bool MIXED, bool WIDE = false>
 __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
     const TableView *__restrict__ tables, const uint64_t *__restrict__ seq,
     const uint64_t *__restrict__ exc, uint64_t nBases, const hipibs_window *__restrict__ windows,
@@ -54,9 +55,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
 
     const int lane = laneId();
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint64_t kmask1 = (1ull << k) - 1ull;
-    const uint64_t kmask2 = k == 32 ? ~0ull : (1ull << (2 * k)) - 1ull;
-    const uint32_t fwdShift = 64 - 2 * k;
+    const uint64_t kmask1 = k >= 64 ? ~0ull : (1ull << k) - 1ull;
+    const uint64_t kmask2 = k >= 32 ? ~0ull : (1ull << (2 * k)) - 1ull;
+    const uint32_t fwdShift = k >= 32 ? 0u : 64 - 2 * k;
     const uint64_t lanesBelow = (1ull << lane) - 1ull;
 
     uint32_t nV = 0, nH = 0, lead = 0, T0 = 0, uVar = 0, uDist = 0;
@@ -90,7 +91,17 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
             const uint64_t fwd = reverse2(x) >> fwdShift;
             const uint64_t canon = fwd < rc ? fwd : rc;
             uint64_t H;
-            if (mlen)
+            if constexpr (WIDE)
+            {
+                const uint64_t s3 = seq[2 * m + 3];
+                const uint64_t c = lane < 32 ? s2 : s3;
+                const uint64_t x1 = j ? ((b >> (2 * j)) | (c << (64 - 2 * j))) : b;
+                uint64_t hi, lo;
+                wideCanonical(xm, x1, k, hi, lo);
+                H = waveProbeWide(T, hi, lo,
+                                  valid ? (uint32_t)bucketOfWide(hi, lo, T.nBuckets) : 0u, valid);
+            }
+            else if (mlen)
             {
                 // canonical m-mer hashes at positions p and p+64, then the
                 // minimum over each k-mer's window of `win` m-mers by a
@@ -212,7 +223,7 @@ __global__ void foldKernel(const SegSummary *__restrict__ work, uint32_t nItems,
 // Mapper: hit mask over positions pos0 .. pos0+nPos-1 (bit i = position
 // pos0+i), one 64-position word per wave step, each lane gathering its own
 // k-mer (positions need not be aligned to the packed words).
-template <int S>
+template <int S, bool WIDE = false>
 __global__ __launch_bounds__(256) void hitmaskKernel(const TableView T, const uint64_t *seq,
                                                      const uint64_t *exc, uint64_t pos0,
                                                      uint64_t nPos, uint64_t contigEnd, uint32_t k,
@@ -222,33 +233,47 @@ __global__ __launch_bounds__(256) void hitmaskKernel(const TableView T, const ui
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nWaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     const uint64_t nChunks = (nPos + 63) >> 6;
-    const uint64_t kmask1 = (1ull << k) - 1ull;
-    const uint64_t kmask2 = k == 32 ? ~0ull : (1ull << (2 * k)) - 1ull;
-    const uint32_t fwdShift = 64 - 2 * k;
+    const uint64_t kmask1 = k >= 64 ? ~0ull : (1ull << k) - 1ull;
+    const uint64_t kmask2 = k >= 32 ? ~0ull : (1ull << (2 * k)) - 1ull;
+    const uint32_t fwdShift = k >= 32 ? 0u : 64 - 2 * k;
     const uint64_t nOutWords = (nPos + 31) >> 5;
     for (uint64_t c = wave; c < nChunks; c += nWaves)
     {
         const uint64_t i = (c << 6) + (uint64_t)lane;
         const uint64_t p = pos0 + i;
         bool valid = i < nPos && p + k <= contigEnd;
-        uint64_t canon = 0;
+        uint64_t canon = 0, chi = 0;
         if (valid)
         {
             const uint64_t wi = p >> 5;
             const uint32_t sh = (uint32_t)(p & 31) * 2;
             const uint64_t a = seq[wi], b = seq[wi + 1];
             uint64_t x = sh ? ((a >> sh) | (b << (64 - sh))) : a;
-            x &= kmask2;
             const uint64_t ej = p >> 6;
             const uint32_t es = (uint32_t)(p & 63);
             const uint64_t ea = exc[ej], eb = exc[ej + 1];
             const uint64_t ev = es ? ((ea >> es) | (eb << (64 - es))) : ea;
             valid = (ev & kmask1) == 0;
-            const uint64_t rc = ~x & kmask2;
-            const uint64_t fwd = reverse2(x) >> fwdShift;
-            canon = fwd < rc ? fwd : rc;
+            if constexpr (WIDE)
+            {
+                const uint64_t c2 = seq[wi + 2];
+                const uint64_t x1 = sh ? ((b >> sh) | (c2 << (64 - sh))) : b;
+                wideCanonical(x, x1, k, chi, canon);
+            }
+            else
+            {
+                x &= kmask2;
+                const uint64_t rc = ~x & kmask2;
+                const uint64_t fwd = reverse2(x) >> fwdShift;
+                canon = fwd < rc ? fwd : rc;
+            }
         }
-        const uint64_t H = waveLookup<S>(T, canon, valid);
+        uint64_t H;
+        if constexpr (WIDE)
+            H = waveProbeWide(T, chi, canon,
+                              valid ? (uint32_t)bucketOfWide(chi, canon, T.nBuckets) : 0u, valid);
+        else
+            H = waveLookup<S>(T, canon, valid);
         if (lane < 2)
         {
             const uint64_t o = 2 * c + (uint64_t)lane;
@@ -259,7 +284,7 @@ __global__ __launch_bounds__(256) void hitmaskKernel(const TableView T, const ui
 }
 
 // coverage[i] = popcount of hit bits p in [i-k+1, i] (clamped to the
-// positions), as a byte (k <= 32).
+// positions), as a byte (k <= 64).
 __global__ void coverageKernel(const uint32_t *__restrict__ hitWords, uint64_t nPos,
                                uint64_t nBases, uint32_t k, uint8_t *__restrict__ cov)
 {
@@ -272,17 +297,14 @@ __global__ void coverageKernel(const uint32_t *__restrict__ hitWords, uint64_t n
         {
             const uint64_t lo = i + 1 >= k ? i + 1 - k : 0;
             const uint64_t hi = std::min<uint64_t>(i, nPos - 1);
-            if (lo <= hi)
+            // the <= 64 bits lo..hi, one 32-bit word at a time
+            for (uint64_t q = lo; q <= hi && q >> 5 < nWords;)
             {
-                const uint64_t w0 = lo >> 5;
-                const uint32_t off = (uint32_t)(lo & 31);
-                const uint32_t len = (uint32_t)(hi - lo + 1);
-                uint64_t v = hitWords[w0];
-                if (w0 + 1 < nWords)
-                    v |= (uint64_t)hitWords[w0 + 1] << 32;
-                v >>= off;
-                v &= (len >= 64) ? ~0ull : ((1ull << len) - 1ull);
-                c = (uint32_t)__popcll(v);
+                const uint32_t off = (uint32_t)(q & 31);
+                const uint64_t take = std::min<uint64_t>(32 - off, hi - q + 1);
+                const uint32_t bits = hitWords[q >> 5] >> off;
+                c += (uint32_t)__popc(take >= 32 ? bits : bits & ((1u << take) - 1u));
+                q += take;
             }
         }
         cov[i] = (uint8_t)c;
@@ -384,7 +406,10 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
         }
         HIPIBS_CHECK(hipEventRecord(tEv[0], s));
     }
-    if (tables[0]->bucketSlots == 8)
+    if (k > 32)
+        hipLaunchKernelGGL((scanKernel<16, false, true>), grid, dim3(SCAN_BLOCK), 0, s, dViews,
+                           dSeq, dExc, nBases, dWindows, nWindows, segs, k, dStats, work);
+    else if (tables[0]->bucketSlots == 8)
         hipLaunchKernelGGL((scanKernel<8, false>), grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc,
                            nBases, dWindows, nWindows, segs, k, dStats, work);
     else
@@ -755,7 +780,11 @@ extern "C" int hipibs_hitmask_device(const hipibs_table *t, const uint64_t *d_se
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const uint64_t chunks = (n_pos + 63) / 64;
     const uint64_t grid = std::min<uint64_t>((chunks + 3) / 4, (uint64_t)deviceCUs(t->device) * 16);
-    if (t->bucketSlots == 8)
+    if (t->k > 32)
+        hipLaunchKernelGGL((hitmaskKernel<16, true>), dim3((uint32_t)grid), dim3(256), 0, s,
+                           t->view(), d_seq_words, d_exc_words, pos0, n_pos, contig_end, t->k,
+                           d_hit_words);
+    else if (t->bucketSlots == 8)
         hipLaunchKernelGGL(hitmaskKernel<8>, dim3((uint32_t)grid), dim3(256), 0, s, t->view(),
                            d_seq_words, d_exc_words, pos0, n_pos, contig_end, t->k, d_hit_words);
     else
@@ -768,7 +797,7 @@ extern "C" int hipibs_hitmask_device(const hipibs_table *t, const uint64_t *d_se
 extern "C" int hipibs_coverage_device(const uint32_t *d_hit_words, uint64_t n_pos, uint64_t n_bases,
                                       uint32_t k, uint8_t *d_cov, void *stream)
 {
-    if (k == 0 || k > 32 || (n_bases && !d_cov) || (n_pos && !d_hit_words))
+    if (k == 0 || k > 64 || (n_bases && !d_cov) || (n_pos && !d_hit_words))
         return setError(HIPIBS_E_ARG, "bad argument");
     if (n_pos > n_bases)
         return setError(HIPIBS_E_ARG, "more positions than bases");

@@ -90,6 +90,51 @@ __global__ __launch_bounds__(256) void insertKernel(uint64_t *slots, const Table
         atomicMax(counters + 2, (unsigned long long)localDisp);
 }
 
+// One 33..64-base key per thread (keys[2i] high word, keys[2i+1] low word)
+// into a table of 8 keys per 128 B bucket. The keys of one build must be
+// distinct (KMC databases are; host key arrays are made so): a thread
+// claims the first slot whose high word it can move from EMPTY to its own
+// and then writes the low word, and never needs to compare with a key
+// being stored. Counters as insertKernel.
+__global__ __launch_bounds__(256) void insertWideKernel(uint64_t *slots, const TableView T,
+                                                        const uint64_t *keys, uint64_t n,
+                                                        unsigned long long *counters)
+{
+    uint64_t localNew = 0;
+    uint32_t localDisp = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        const uint64_t hi = keys[2 * i], lo = keys[2 * i + 1];
+        uint64_t b = bucketOfWide(hi, lo, T.nBuckets);
+        bool stored = false;
+        for (uint64_t d = 0; d < T.nBuckets && !stored; ++d)
+        {
+            unsigned long long *bucket = reinterpret_cast<unsigned long long *>(slots + b * 16);
+            for (int q = 0; q < 8; ++q)
+            {
+                const unsigned long long cur = __hip_atomic_load(
+                    bucket + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (cur == EMPTY && atomicCAS(bucket + 2 * q, EMPTY, hi) == EMPTY)
+                {
+                    bucket[2 * q + 1] = lo;
+                    stored = true;
+                    localNew += 1;
+                    localDisp = max(localDisp, (uint32_t)min<uint64_t>(d, 0xffffffffu));
+                    break;
+                }
+            }
+            b = b + 1 == T.nBuckets ? 0 : b + 1;
+        }
+        if (!stored)
+            atomicExch(counters + 1, 1ull);
+    }
+    if (localNew)
+        atomicAdd(counters + 0, (unsigned long long)localNew);
+    if (localDisp)
+        atomicMax(counters + 2, (unsigned long long)localDisp);
+}
+
 // ---- minimizer tables ------------------------------------------------------
 // A group = the keys sharing a 32-bit minimizer hash. The group map holds
 // one 64-bit entry per group: hash in the low 32 bits, key count in bits
@@ -313,6 +358,72 @@ __global__ __launch_bounds__(256) void kmcDecodeKernel(const uint8_t *__restrict
     }
 }
 
+// kmcDecodeKernel for k = 33..64: record i gives keys[2i] (high word) and
+// keys[2i+1] (low word); the suffix holds up to 16 bytes.
+__global__ __launch_bounds__(256) void kmcDecodeWideKernel(
+    const uint8_t *__restrict__ raw, uint64_t r0, uint64_t m, uint32_t recBytes,
+    uint32_t sufBytes, const uint64_t *__restrict__ lut, uint64_t j0, uint64_t j1,
+    uint64_t prefixMask, uint32_t shift, uint64_t *__restrict__ keys)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * kDecodePerThread;
+    for (uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kDecodePerThread;
+         i0 < m; i0 += stride)
+    {
+        uint64_t j = lutBucketOf(lut, j0, j1, r0 + i0);
+        const uint64_t iEnd = i0 + kDecodePerThread < m ? i0 + kDecodePerThread : m;
+        for (uint64_t i = i0; i < iEnd; ++i)
+        {
+            const uint64_t r = r0 + i;
+            if (j < j1 && lut[j + 1] <= r)
+                j = lutBucketOf(lut, j + 1, j1, r);
+            const uint8_t *q = raw + i * recBytes;
+            uint64_t hi = 0, lo = 0;
+            for (uint32_t b = 0; b < sufBytes; ++b)
+            {
+                hi = (hi << 8) | (lo >> 56);
+                lo = (lo << 8) | q[b];
+            }
+            const uint64_t pre = j & prefixMask;
+            if (shift < 64)
+            {
+                lo |= pre << shift;
+                hi |= shift ? pre >> (64 - shift) : 0ull;
+            }
+            else if (shift < 128)
+                hi |= pre << (shift - 64);
+            keys[2 * i] = hi;
+            keys[2 * i + 1] = lo;
+        }
+    }
+}
+
+// Wide-key membership: codes[2i], codes[2i+1] = (high, low) of query i;
+// otherwise as probeKernel.
+__global__ __launch_bounds__(256) void probeWideKernel(const TableView T, const uint64_t *codes,
+                                                       const uint8_t *valid, uint64_t n,
+                                                       uint8_t *hits, unsigned long long *count)
+{
+    const int lane = laneId();
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nWaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t nChunks = (n + 63) >> 6;
+    uint64_t local = 0;
+    for (uint64_t c = wave; c < nChunks; c += nWaves)
+    {
+        const uint64_t i = (c << 6) + (uint64_t)lane;
+        const bool in = i < n;
+        const bool v = in && (valid == nullptr || valid[i] != 0);
+        const uint64_t hi = v ? codes[2 * i] : 0ull, lo = v ? codes[2 * i + 1] : 0ull;
+        const uint64_t H =
+            waveProbeWide(T, hi, lo, v ? (uint32_t)bucketOfWide(hi, lo, T.nBuckets) : 0u, v);
+        if (hits && in)
+            hits[i] = (uint8_t)((H >> lane) & 1ull);
+        local += (uint64_t)__popcll(H);
+    }
+    if (count && lane == 0 && local)
+        atomicAdd(count, (unsigned long long)local);
+}
+
 // 64 queries per wave per step; hits (nullable) gets one byte per query,
 // count (nullable) the number of hits. Grid-stride over 64-query chunks.
 template <int S>
@@ -355,7 +466,10 @@ int launchTableInsert(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStr
         return 0;
     const int grid = gridFor(n, 256, t->device);
     auto *ctr = reinterpret_cast<unsigned long long *>(t->dCounters);
-    if (t->bucketSlots == 8)
+    if (t->k > 32)
+        hipLaunchKernelGGL(insertWideKernel, dim3(grid), dim3(256), 0, s, t->dSlots, t->view(),
+                           dKeys, n, ctr);
+    else if (t->bucketSlots == 8)
         hipLaunchKernelGGL(insertKernel<8>, dim3(grid), dim3(256), 0, s, t->dSlots, t->view(),
                            dKeys, n, ctr);
     else
@@ -373,7 +487,10 @@ int launchProbe(const hipibs_table *t, const uint64_t *dCodes, const uint8_t *dV
     const uint64_t waves = (n + 63) / 64;
     const int grid = gridFor(waves * 64, 256, t->device);
     auto *cnt = reinterpret_cast<unsigned long long *>(dCount);
-    if (t->bucketSlots == 8)
+    if (t->k > 32)
+        hipLaunchKernelGGL(probeWideKernel, dim3(grid), dim3(256), 0, s, t->view(), dCodes,
+                           dValid, n, dHits, cnt);
+    else if (t->bucketSlots == 8)
         hipLaunchKernelGGL(probeKernel<8>, dim3(grid), dim3(256), 0, s, t->view(), dCodes, dValid,
                            n, dHits, cnt);
     else
@@ -392,8 +509,13 @@ static int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts,
         o = *opts;
     if (o.load_factor <= 0)
         o.load_factor = 0.6;
-    if (k == 0 || k > 32)
-        return setError(HIPIBS_E_ARG, "k must be in 1..32");
+    if (k == 0 || k > 64)
+        return setError(HIPIBS_E_ARG, "k must be in 1..64");
+    const bool wide = k > 32; // 8 keys of 16 B per 128 B bucket
+    if (wide && o.minimizer_len)
+        return setError(HIPIBS_E_ARG, "minimizer tables need k <= 32");
+    if (wide && o.bucket_slots == 8)
+        return setError(HIPIBS_E_ARG, "tables of k > 32 have 128 B buckets (bucket_slots 16)");
     const uint32_t mlen = o.minimizer_len;
     if (mlen && (mlen < 5 || mlen > 17 || mlen > k))
         return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (off) or 5..min(17, k)");
@@ -404,7 +526,7 @@ static int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts,
     if (!(o.load_factor > 0 && o.load_factor < 0.98))
         return setError(HIPIBS_E_ARG, "load_factor must be in (0, 0.98)");
     const double slotsWanted = std::ceil((double)std::max<uint64_t>(nKeys, 1) / o.load_factor);
-    uint64_t nBuckets = (uint64_t)std::ceil(slotsWanted / o.bucket_slots);
+    uint64_t nBuckets = (uint64_t)std::ceil(slotsWanted / (wide ? 8 : o.bucket_slots));
     nBuckets = std::max<uint64_t>(nBuckets, 2);
     if (nBuckets >= (1ull << 32))
         return setError(HIPIBS_E_ARG, "table too large: more than 2^32 buckets");
@@ -624,6 +746,8 @@ extern "C" int hipibs_table_build_keys_device(const uint64_t *d_keys, uint64_t n
         return setError(HIPIBS_E_ARG, "null argument");
     *out = nullptr;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (opts && opts->minimizer_len && k > 32)
+        return setError(HIPIBS_E_ARG, "minimizer tables need k <= 32");
     if (opts && opts->minimizer_len)
         return buildMinimizerTable(d_keys, n, k, opts, s, out);
     hipibs_table *t = nullptr;
@@ -812,7 +936,8 @@ static int streamKmcKeys(const KmcRecords &src, int device, unsigned threads, hi
             hipMalloc(reinterpret_cast<void **>(&raw[i]), rawBytes) != hipSuccess ||
             hipEventCreateWithFlags(&copied[i], hipEventDisableTiming) != hipSuccess)
             rc = setError(HIPIBS_E_NOMEM, "staging allocation failed");
-    if (!rc && (hipMalloc(reinterpret_cast<void **>(&dKeys), chunk * 8) != hipSuccess ||
+    const bool wide = k > 32; // two words per key
+    if (!rc && (hipMalloc(reinterpret_cast<void **>(&dKeys), chunk * (wide ? 16 : 8)) != hipSuccess ||
                 hipMalloc(reinterpret_cast<void **>(&dLut), src.lut.size() * 8) != hipSuccess))
         rc = setError(HIPIBS_E_NOMEM, "device allocation for the KMC decode failed");
     if (!rc && hipMemcpyAsync(dLut, src.lut.data(), src.lut.size() * 8, hipMemcpyHostToDevice,
@@ -848,8 +973,12 @@ static int streamKmcKeys(const KmcRecords &src, int device, unsigned threads, hi
         used[b] = true;
         const uint64_t j0 = bucketOfRecord(r0), j1 = bucketOfRecord(r0 + m - 1);
         const int grid = gridFor((m + kDecodePerThread - 1) / kDecodePerThread, 256, device);
-        hipLaunchKernelGGL(kmcDecodeKernel, dim3(grid), dim3(256), 0, s, raw[b], r0, m,
-                           src.recBytes, src.sufBytes, dLut, j0, j1, prefixMask, shift, dKeys);
+        if (wide)
+            hipLaunchKernelGGL(kmcDecodeWideKernel, dim3(grid), dim3(256), 0, s, raw[b], r0, m,
+                               src.recBytes, src.sufBytes, dLut, j0, j1, prefixMask, shift, dKeys);
+        else
+            hipLaunchKernelGGL(kmcDecodeKernel, dim3(grid), dim3(256), 0, s, raw[b], r0, m,
+                               src.recBytes, src.sufBytes, dLut, j0, j1, prefixMask, shift, dKeys);
         if (hipGetLastError() != hipSuccess)
         {
             rc = setError(HIPIBS_E_HIP, "KMC decode launch failed");
@@ -894,6 +1023,34 @@ extern "C" int hipibs_table_build_keys(const uint64_t *h_keys, uint64_t n, uint3
     if (!out || (n && !h_keys))
         return setError(HIPIBS_E_ARG, "null argument");
     *out = nullptr;
+    if (k > 32 && k <= 64)
+    {
+        // distinct (high, low) pairs, then one upload and the wide insert
+        std::vector<Wide> w(n);
+        for (uint64_t i = 0; i < n; ++i)
+            w[i] = ((Wide)h_keys[2 * i] << 64) | h_keys[2 * i + 1];
+        std::sort(w.begin(), w.end());
+        w.erase(std::unique(w.begin(), w.end()), w.end());
+        std::vector<uint64_t> flat(2 * w.size());
+        for (size_t i = 0; i < w.size(); ++i)
+        {
+            flat[2 * i] = (uint64_t)(w[i] >> 64);
+            flat[2 * i + 1] = (uint64_t)w[i];
+        }
+        HIPIBS_CHECK(hipSetDevice(opts ? opts->device : 0));
+        uint64_t *dAll = nullptr;
+        if (hipMalloc(reinterpret_cast<void **>(&dAll), std::max<size_t>(flat.size(), 1) * 8) !=
+            hipSuccess)
+            return setError(HIPIBS_E_NOMEM, "device key buffer allocation failed");
+        int rc = hipMemcpy(dAll, flat.data(), flat.size() * 8, hipMemcpyHostToDevice) == hipSuccess
+                     ? 0
+                     : setError(HIPIBS_E_HIP, "key upload failed");
+        if (!rc)
+            rc = hipibs_table_build_keys_device(dAll, w.size(), k, opts, out, nullptr);
+        (void)hipDeviceSynchronize();
+        hipFree(dAll);
+        return rc;
+    }
     if (opts && opts->minimizer_len)
     {
         // the group counts need every key: upload them all first
@@ -989,6 +1146,7 @@ extern "C" int hipibs_table_probe_count(const hipibs_table *t, const uint64_t *h
     std::lock_guard<std::mutex> lock(mu);
     HIPIBS_CHECK(hipSetDevice(t->device));
     const uint64_t chunk = std::min<uint64_t>(n, 16ull << 20);
+    const uint64_t kw = t->k > 32 ? 2 : 1; // words per key
     hipStream_t s;
     HIPIBS_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     uint64_t *pinned[2] = {nullptr, nullptr};
@@ -997,8 +1155,8 @@ extern "C" int hipibs_table_probe_count(const hipibs_table *t, const uint64_t *h
     hipEvent_t done[2] = {nullptr, nullptr};
     int rc = 0;
     for (int i = 0; i < 2 && !rc; ++i)
-        if (hipHostMalloc(reinterpret_cast<void **>(&pinned[i]), chunk * 8) != hipSuccess ||
-            hipMalloc(reinterpret_cast<void **>(&dev[i]), chunk * 8) != hipSuccess ||
+        if (hipHostMalloc(reinterpret_cast<void **>(&pinned[i]), chunk * 8 * kw) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void **>(&dev[i]), chunk * 8 * kw) != hipSuccess ||
             hipEventCreateWithFlags(&done[i], hipEventDisableTiming) != hipSuccess)
             rc = setError(HIPIBS_E_NOMEM, "staging allocation failed");
     if (!rc && hipMalloc(reinterpret_cast<void **>(&dCount), 8) != hipSuccess)
@@ -1019,8 +1177,8 @@ extern "C" int hipibs_table_probe_count(const hipibs_table *t, const uint64_t *h
             rc = setError(HIPIBS_E_HIP, "event wait failed");
             break;
         }
-        parallelCopy(pinned[b], h_keys + off, m * 8);
-        if (hipMemcpyAsync(dev[b], pinned[b], m * 8, hipMemcpyHostToDevice, s) != hipSuccess)
+        parallelCopy(pinned[b], h_keys + off * kw, m * 8 * kw);
+        if (hipMemcpyAsync(dev[b], pinned[b], m * 8 * kw, hipMemcpyHostToDevice, s) != hipSuccess)
         {
             rc = setError(HIPIBS_E_HIP, "key upload failed");
             break;
@@ -1064,6 +1222,8 @@ extern "C" int hipibs_table_build_kmc(const char *prefix, const hipibs_table_opt
         return rc;
     const uint64_t n = src.total;
     hipibs_table *t = nullptr;
+    if (opts && opts->minimizer_len && src.info.kmer_length > 32)
+        return setError(HIPIBS_E_ARG, "minimizer tables need k <= 32");
     if (opts && opts->minimizer_len)
     {
         // the group counts need every key: decode them all into one buffer

@@ -235,6 +235,114 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, uint32_t 
     return H;
 }
 
+// waveProbe for tables of 33..64-base keys: a bucket is 8 keys of 16 B
+// (high word, low word), so 8 lanes read one query's bucket, one key each;
+// a slot whose high word is EMPTY is free (a canonical code's high word is
+// never all ones). Must be called by all 64 lanes.
+__device__ inline uint64_t waveProbeWide(const TableView T, uint64_t hi, uint64_t lo,
+                                         uint32_t home, bool valid)
+{
+    constexpr int LPQ = 8, QPR = 8, ROUNDS = 8, S = 16;
+    constexpr uint32_t GMASK = 0xFFu;
+    const int lane = laneId();
+    const int g = lane / LPQ, sub = lane % LPQ;
+    const uint64_t V = ballot(valid);
+    uint64_t qh[ROUNDS], ql[ROUNDS];
+    uint32_t qb[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r)
+    {
+        qh[r] = shuffle64(hi, r * QPR + g);
+        ql[r] = shuffle64(lo, r * QPR + g);
+        qb[r] = shuffle32(home, r * QPR + g);
+    }
+    SlotPair kv[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r)
+    {
+        kv[r] = SlotPair{EMPTY, EMPTY};
+        if ((V >> (r * QPR + g)) & 1ull)
+            kv[r] = loadPair(T.slots + (uint64_t)qb[r] * S + (uint32_t)sub * 2u);
+    }
+    uint64_t M[ROUNDS], E[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r)
+    {
+        const bool vq = (V >> (r * QPR + g)) & 1ull;
+        M[r] = ballot(vq && kv[r].a == qh[r] && kv[r].b == ql[r]);
+        E[r] = ballot(vq && kv[r].a == EMPTY);
+    }
+    const int rOwn = lane / QPR, gOwn = lane % QPR;
+    uint64_t Mo = M[0], Eo = E[0];
+#pragma unroll
+    for (int r = 1; r < ROUNDS; ++r)
+        if (rOwn == r)
+        {
+            Mo = M[r];
+            Eo = E[r];
+        }
+    const bool found = ((Mo >> (gOwn * LPQ)) & GMASK) != 0;
+    const bool hasEmpty = ((Eo >> (gOwn * LPQ)) & GMASK) != 0;
+    uint64_t H = ballot(valid && found);
+    uint64_t NC = ballot(valid && !found && !hasEmpty);
+    if (NC == 0 || T.maxDisp == 0)
+        return H;
+    // Overflow rounds: pending lane i continues at home_i + disp_i.
+    uint32_t disp = 1;
+    while (NC)
+    {
+        const bool pending = (NC >> lane) & 1ull;
+        const uint32_t nPend = (uint32_t)__popcll(NC);
+        const bool serving = (uint32_t)g < nPend;
+        const int src = serving ? selectBit(NC, (uint32_t)g) : 0;
+        const uint64_t qhi = shuffle64(hi, src), qlo = shuffle64(lo, src);
+        const uint64_t qhome = shuffle32(home, src);
+        const uint32_t qdisp = shuffle32(disp, src);
+        SlotPair kv2{EMPTY, EMPTY};
+        if (serving)
+        {
+            uint64_t b = qhome + qdisp;
+            if (b >= T.nBuckets)
+                b -= T.nBuckets;
+            kv2 = loadPair(T.slots + b * S + (uint32_t)sub * 2u);
+        }
+        const uint64_t M2 = ballot(serving && kv2.a == qhi && kv2.b == qlo);
+        const uint64_t E2 = ballot(serving && kv2.a == EMPTY);
+        const uint32_t rank = maskedRank(NC);
+        const bool served = pending && rank < (uint32_t)QPR;
+        const uint32_t sh = (served ? rank : 0u) * (uint32_t)LPQ;
+        const bool f2 = served && ((M2 >> sh) & GMASK) != 0;
+        const bool e2 = served && ((E2 >> sh) & GMASK) != 0;
+        const bool done = served && (f2 || e2 || disp >= T.maxDisp);
+        H |= ballot(f2);
+        if (served && !done)
+            disp += 1;
+        NC &= ~ballot(done);
+    }
+    return H;
+}
+
+// Canonical code of the k = 33..64 bases whose low-first 2-bit packing is
+// x0 (bases 0..31) and x1 (bases 32..63, only the first k-32 used): the
+// reverse complement is the complement of the packing; the forward code
+// puts the first base in the most significant bits, through the 2-bit
+// group reversal of each word.
+__device__ inline void wideCanonical(uint64_t x0, uint64_t x1, uint32_t k, uint64_t &hi,
+                                     uint64_t &lo)
+{
+    const uint32_t k2 = k - 32; // bases in the high word
+    const uint64_t m2 = k2 == 32 ? ~0ull : (1ull << (2 * k2)) - 1ull;
+    x1 &= m2;
+    const uint64_t rhi = ~x1 & m2, rlo = ~x0;
+    const uint64_t r0 = reverse2(x0);
+    const uint64_t r1 = reverse2(x1) >> (64 - 2 * k2);
+    const uint64_t fhi = k2 == 32 ? r0 : r0 >> (64 - 2 * k2);
+    const uint64_t flo = k2 == 32 ? r1 : (r0 << (2 * k2)) | r1;
+    const bool f = fhi < rhi || (fhi == rhi && flo < rlo);
+    hi = f ? fhi : rhi;
+    lo = f ? flo : rlo;
+}
+
 // Membership in a minimizer table of the codes whose minimizer hash is
 // minHash: the minimizer-ordered part first; queries that meet their
 // group's marker there continue in the whole-k-mer part.

@@ -220,25 +220,37 @@ def kmc_header(prefix: str) -> KmcInfo:
 
 
 def kmc_read_keys(prefix: str, threads: int = 0) -> np.ndarray:
+    """Keys in on-disk order: uint64 (k <= 32) or (n, 2) high/low words."""
     info = kmc_header(prefix)
-    keys = np.empty(info.total_kmers, dtype=np.uint64)
+    wide = info.kmer_length > 32
+    keys = np.empty((info.total_kmers, 2) if wide else info.total_kmers, dtype=np.uint64)
     n = C.c_uint64(0)
-    _check(_lib.hipibs_kmc_read_keys(prefix.encode(), _ptr(keys), keys.size, C.byref(n), threads),
-           "hipibs_kmc_read_keys")
+    _check(_lib.hipibs_kmc_read_keys(prefix.encode(), _ptr(keys), info.total_kmers, C.byref(n),
+                                     threads), "hipibs_kmc_read_keys")
     return keys
+
+
+def _key_count(keys: np.ndarray, k: int) -> int:
+    """Number of keys in a uint64 array (k <= 32) or an (n, 2) one."""
+    if k > 32:
+        if keys.ndim != 2 or keys.shape[1] != 2:
+            raise ValueError("keys of k > 32 are an (n, 2) array of (high, low) words")
+        return keys.shape[0]
+    return keys.size
 
 
 def kmc_write(prefix: str, keys: np.ndarray, k: int, counts: Optional[np.ndarray] = None,
               p: Optional[int] = None, sig_len: int = 9, counter_size: int = 1,
               version: int = 0x200, min_count: int = 1, max_count: int = 255) -> None:
     keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    n = _key_count(keys, k)
     if p is None:
         p = _default_lut_prefix(k)
     cp = None
     if counts is not None:
         counts = np.ascontiguousarray(counts, dtype=np.uint32)
         cp = _ptr(counts, u32p)
-    _check(_lib.hipibs_kmc_write(prefix.encode(), _ptr(keys), cp, keys.size, k, p, sig_len,
+    _check(_lib.hipibs_kmc_write(prefix.encode(), _ptr(keys), cp, n, k, p, sig_len,
                                  counter_size, version, min_count, max_count), "hipibs_kmc_write")
 
 
@@ -425,11 +437,12 @@ class Table:
 
     @classmethod
     def from_keys(cls, keys: np.ndarray, k: int, **kw) -> "Table":
+        """keys: uint64 codes (k <= 32) or an (n, 2) array of (high, low)."""
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         h = vp()
         o = _opts(**kw)
-        _check(_lib.hipibs_table_build_keys(_ptr(keys), keys.size, k, C.byref(o), C.byref(h)),
-               "hipibs_table_build_keys")
+        _check(_lib.hipibs_table_build_keys(_ptr(keys), _key_count(keys, k), k, C.byref(o),
+                                            C.byref(h)), "hipibs_table_build_keys")
         return cls(h.value)
 
     @classmethod
@@ -463,8 +476,9 @@ class Table:
 
     def probe_count(self, keys: np.ndarray) -> int:
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        n = _key_count(keys, self.info()["k"])
         out = C.c_uint64(0)
-        _check(_lib.hipibs_table_probe_count(self.handle, _ptr(keys), keys.size, C.byref(out)),
+        _check(_lib.hipibs_table_probe_count(self.handle, _ptr(keys), n, C.byref(out)),
                "hipibs_table_probe_count")
         return out.value
 

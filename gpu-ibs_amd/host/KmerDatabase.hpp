@@ -56,9 +56,9 @@ public:
         (void)listing;
         hipibs::check(hipibs_kmc_header(sourcePath_.c_str(), &info_), "reading the KMC header");
         kmerSize = info_.kmer_length;
-        if (kmerSize == 0 || kmerSize > 32)
+        if (kmerSize == 0 || needsStringPath(kmerSize))
             throw std::runtime_error("k = " + std::to_string(kmerSize) +
-                                     ": this build supports k <= 32 only");
+                                     ": this build supports k <= 64 only");
     }
 
     uint32_t getKmerSize() const { return kmerSize; }
@@ -122,9 +122,10 @@ public:
 
     // ---- the reference's process-wide switches, kept for its callers ----
 
-    // k > 32 needs the reference's string path, which this build does not
-    // have: the constructor rejects such databases.
-    static bool needsStringPath(uint32_t k) { return k > 32; }
+    // The device path holds a k-mer in one 64-bit word (k <= 32) or two
+    // (k <= 64); a larger k would need the reference's string path, which
+    // this build does not have: the constructor rejects such databases.
+    static bool needsStringPath(uint32_t k) { return k > 64; }
 
     // Host threads for parsing, packing and KMC reads: FASTIBS_THREADS=N,
     // else every CPU this process may use.

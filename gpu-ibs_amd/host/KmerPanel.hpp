@@ -218,7 +218,10 @@ public:
         std::cout << "Panel: " << dbs_.size() << " databases in " << rounds.size()
                   << " device round(s); reference " << dref.nBases << " bases, "
                   << dref.nWindows << " windows, ingest " << ingestSec << " s" << std::endl;
-        if (g.bucketAuto && dref.nWindows)
+        // k > 32: 128 B buckets only, and the tables of a group are scanned
+        // by the multi-table scan launch (the panel kernel holds 64-bit codes)
+        const bool wide = dbs_[0]->getKmerSize() > 32;
+        if (g.bucketAuto && dref.nWindows && !wide)
             tuneBucketWidth(dref);
         hipibs_scan_config cfg{0, 0, {0, 0}};
         bool tuned = false;
@@ -258,9 +261,15 @@ public:
             const int nt = (int)tabs.size();
             hipibs::DeviceBuffer stats(g.device, std::max<uint64_t>(1, (uint64_t)dref.nWindows * nt) *
                                                      sizeof(hipibs_window_stats));
-            const uint64_t wb = hipibs_panel_work_bytes(dref.nWindows, nt, 4);
+            uint32_t segs = 0;
+            const uint64_t wb =
+                wide ? std::max<uint64_t>(8, hipibs_scan_work_bytes(dref.nWindows, nt,
+                                                                     (uint32_t)windowSize,
+                                                                     dbs_[0]->getKmerSize(), 0,
+                                                                     &segs))
+                     : hipibs_panel_work_bytes(dref.nWindows, nt, 4);
             hipibs::DeviceBuffer work(g.device, wb);
-            if (!tuned && dref.nWindows)
+            if (!tuned && dref.nWindows && !wide)
             {
                 double ms = 0;
                 hipibs::check(hipibs_panel_autotune(tabs.data(), nt, dref.seq.as<uint64_t>(),
@@ -274,7 +283,20 @@ public:
                 tuned = true;
             }
             std::vector<hipibs_window_stats> host((uint64_t)dref.nWindows * nt);
-            if (dref.nWindows)
+            if (dref.nWindows && wide)
+            {
+                hipibs::check(hipibs_scan_windows_device(
+                                  tabs.data(), nt, dref.seq.as<uint64_t>(), dref.exc.as<uint64_t>(),
+                                  dref.nBases, dref.windows.as<hipibs_window>(), dref.nWindows,
+                                  (uint32_t)windowSize, stats.as<hipibs_window_stats>(), segs,
+                                  work.get(), wb, nullptr),
+                              "scan of the group");
+                hipibs::check(hipibs_dev_synchronize(g.device), "scan of the group");
+                hipibs::check(hipibs_memcpy(host.data(), stats.get(),
+                                            host.size() * sizeof(hipibs_window_stats), 2),
+                              "statistics download");
+            }
+            else if (dref.nWindows)
             {
                 hipibs_graph *graph = nullptr;
                 hipibs::check(hipibs_panel_graph_create(tabs.data(), nt, dref.seq.as<uint64_t>(),

@@ -105,7 +105,8 @@ int main(int argc, char **argv)
                       << (DeviceKmerIndex::freeMemory(idx.device()) > 0) << std::endl;
         }
         std::cout << "needsStringPath(" << k << ") " << KmerDatabase::needsStringPath(k)
-                  << ", needsStringPath(33) " << KmerDatabase::needsStringPath(33)
+                  << ", needsStringPath(64) " << KmerDatabase::needsStringPath(64)
+                  << ", needsStringPath(65) " << KmerDatabase::needsStringPath(65)
                   << ", devicesToUse(4) " << KmerDatabase::devicesToUse(4) << ", gpuRequested "
                   << KmerDatabase::gpuRequested() << ", gpuSyncRequested "
                   << KmerDatabase::gpuSyncRequested() << ", poolThreads>0 "

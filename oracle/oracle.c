@@ -439,3 +439,159 @@ int64_t orc_count_kmers(const char *text, int64_t n, uint32_t k, uint64_t *keys,
     *n_kmers = m;
     return d;
 }
+
+/* ---- k = 33..64: two-word codes ---------------------------------------
+ * The reference takes its string path for k > 32 (KmerDatabase.hpp:104-123):
+ * getCanonicalKmersLegacy (src/Utils.hpp:102-125) keeps every k-length
+ * substring made only of ACGT after upper-casing, canonical = the smaller
+ * of it and its reverse complement in string order; the statistics loop is
+ * getStatsFromSequenceLegacy (src/KmerDatabase.hpp:675-709), coverage
+ * getMappingFromSequenceLegacy (:804-845). Restated here with the code of
+ * a k-mer as a 128-bit integer (first base most significant, A=0 C=1 G=2
+ * T=3), whose numeric order on ACGT strings is the string order. Keys are
+ * (high word, low word) pairs, sorted as 128-bit integers. */
+typedef unsigned __int128 u128;
+
+static u128 wide_of(const uint64_t *p)
+{
+    return ((u128)p[0] << 64) | p[1];
+}
+
+int64_t orc_encode_positions_wide(const char *seq, int64_t n, uint32_t k, uint64_t *codes,
+                                  uint8_t *valid)
+{
+    if (k < 33 || k > 64 || n < (int64_t)k)
+        return 0;
+    const u128 mask = k == 64 ? ~(u128)0 : (((u128)1 << (2 * k)) - 1);
+    const uint32_t rcShift = 2 * (k - 1);
+    u128 fwd = 0, rc = 0;
+    uint32_t run = 0;
+    for (int64_t j = 0; j < n; ++j)
+    {
+        int c = base_code((unsigned char)seq[j]);
+        if (c > 3)
+        {
+            run = 0;
+            c = 0;
+        }
+        else
+            ++run;
+        fwd = ((fwd << 2) | (u128)c) & mask;
+        rc = (rc >> 2) | ((u128)(3 - c) << rcShift);
+        if (j >= (int64_t)k - 1)
+        {
+            const int64_t i = j - (k - 1);
+            const int ok = run >= k;
+            const u128 v = ok ? (fwd < rc ? fwd : rc) : 0;
+            valid[i] = (uint8_t)ok;
+            codes[2 * i] = (uint64_t)(v >> 64);
+            codes[2 * i + 1] = (uint64_t)v;
+        }
+    }
+    return n - k + 1;
+}
+
+static int cmp_u128(const void *a, const void *b)
+{
+    const u128 x = wide_of((const uint64_t *)a), y = wide_of((const uint64_t *)b);
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+void orc_sort_keys_wide(uint64_t *keys, int64_t n)
+{
+    qsort(keys, (size_t)n, 2 * sizeof(uint64_t), cmp_u128);
+}
+
+static int member_wide(const uint64_t *keys, int64_t nkeys, u128 code)
+{
+    int64_t lo = 0, hi = nkeys - 1;
+    while (lo <= hi)
+    {
+        const int64_t mid = lo + (hi - lo) / 2;
+        const u128 v = wide_of(keys + 2 * mid);
+        if (v == code)
+            return 1;
+        if (v < code)
+            lo = mid + 1;
+        else
+            hi = mid - 1;
+    }
+    return 0;
+}
+
+/* Hit flag per position (valid positions only) against sorted pair keys. */
+static void hits_wide(const char *seq, int64_t n, uint32_t k, const uint64_t *keys,
+                      int64_t nkeys, uint8_t *valid, uint8_t *hit)
+{
+    const int64_t np = n - k + 1;
+    uint64_t *codes = (uint64_t *)malloc((size_t)np * 16);
+    orc_encode_positions_wide(seq, n, k, codes, valid);
+    for (int64_t i = 0; i < np; ++i)
+        hit[i] = valid[i] && member_wide(keys, nkeys, wide_of(codes + 2 * i));
+    free(codes);
+}
+
+/* Window rows as orc_scan_contig, for k = 33..64. */
+int64_t orc_scan_contig_wide(const char *seq, int64_t n, int64_t W, uint32_t k,
+                             const uint64_t *keys, int64_t nkeys, int64_t *out, int64_t cap)
+{
+    if (W <= (int64_t)k || k < 33 || k > 64)
+        return -1;
+    int64_t nw = 0;
+    for (int64_t j = 0; j < n; j += W - k)
+    {
+        const int64_t e = j + W > n ? n : j + W;
+        if (nw < cap)
+        {
+            int32_t st[4] = {0, 0, 0, 0};
+            if (e - j >= (int64_t)k)
+            {
+                const int64_t np = e - j - k + 1;
+                uint8_t *valid = (uint8_t *)malloc((size_t)np);
+                uint8_t *hit = (uint8_t *)malloc((size_t)np);
+                hits_wide(seq + j, e - j, k, keys, nkeys, valid, hit);
+                orc_stats_from_flags(valid, hit, np, k, st);
+                free(valid);
+                free(hit);
+            }
+            int64_t *o = out + 6 * nw;
+            o[0] = j;
+            o[1] = e;
+            o[2] = st[0];
+            o[3] = st[1];
+            o[4] = st[2];
+            o[5] = st[3];
+        }
+        ++nw;
+    }
+    return nw;
+}
+
+/* Per-base coverage of one contig as orc_coverage, for k = 33..64. */
+void orc_coverage_wide(const char *seq, int64_t n, uint32_t k, const uint64_t *keys,
+                       int64_t nkeys, int32_t *cov)
+{
+    int32_t *diff = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+    if (n >= (int64_t)k && k >= 33 && k <= 64)
+    {
+        const int64_t np = n - k + 1;
+        uint8_t *valid = (uint8_t *)malloc((size_t)np);
+        uint8_t *hit = (uint8_t *)malloc((size_t)np);
+        hits_wide(seq, n, k, keys, nkeys, valid, hit);
+        for (int64_t i = 0; i < np; ++i)
+            if (hit[i])
+            {
+                diff[i] += 1;
+                diff[i + k] -= 1;
+            }
+        free(valid);
+        free(hit);
+    }
+    int32_t c = 0;
+    for (int64_t i = 0; i < n; ++i)
+    {
+        c += diff[i];
+        cov[i] = c;
+    }
+    free(diff);
+}

@@ -13,6 +13,8 @@
 //   CKmerAPI::from_int                          KMC/kmc_api/kmer_api.h:521-544
 //   KmcRaIndex (load, probe, forEachKmer)       src/gpu/KmcRaIndex.hpp
 //   fastibs::probeBatched (CPU-BATCHED)         src/CpuProbeBatched.hpp:214-268
+//   getCanonicalKmersLegacy, reverseComplement  src/Utils.hpp:102-125 (k > 32)
+//   CKmerAPI::from_string                       KMC/kmc_api/kmer_api.h
 // What this file adds is glue: the TSV / coverage text formats of
 // KmerDatabase::writeStats / serializeCoverage (src/KmerDatabase.hpp:756-772,
 // :935-948; KmerDatabase.hpp itself needs Boost, absent here) and the
@@ -189,6 +191,137 @@ int ref_coverage(const char *dbPrefix, const char *fasta, const char *outPath)
         fprintf(stderr, "ref_coverage: %s\n", e.what());
         return -2;
     }
+}
+
+// The reference's string path (CPU-BASE, the one it takes for k > 32):
+// getCanonicalKmersLegacy + CKmerAPI::from_string + CKMCFile::IsKmer per
+// window, with the gap loop of KmerDatabase::getStatsFromSequenceLegacy
+// (src/KmerDatabase.hpp:675-709; glue, that header needs Boost).
+int ref_scan_tsv_legacy(const char *dbPrefix, const char *fasta, int W, const char *outTsv)
+{
+    try
+    {
+        const bool gz = std::string(fasta).find(".gz") != std::string::npos;
+        ParseResult parsed = readSequenceFile(fasta, gz);
+        CKMCFile db;
+        if (!openRa(db, dbPrefix))
+            return -1;
+        const uint32_t k = db.KmerLength();
+        std::vector<SeqWindow> windows;
+        chunkSequences(parsed.ids, parsed.sequences, (size_t)W, k, windows);
+        std::vector<Window> rows;
+        rows.reserve(windows.size());
+        for (const auto &w : windows)
+        {
+            Window stats;
+            stats.id = *w.id;
+            stats.start = (int)w.start;
+            stats.end = (int)w.end;
+            std::string seq(w.seq);
+            const auto kmers = getCanonicalKmersLegacy(seq, (int)k);
+            stats.totalKmers = (int)kmers.size();
+            int gap = 0;
+            CKmerAPI km;
+            for (const auto &s : kmers)
+            {
+                km.from_string(s);
+                if (!db.IsKmer(km))
+                    gap += 1;
+                else
+                {
+                    stats.observedKmers += 1;
+                    if (gap > 0)
+                    {
+                        stats.addVariationOriginal(gap, (int)k);
+                        gap = 0;
+                    }
+                }
+            }
+            if (gap > 0)
+                stats.addVariationOriginal(gap, (int)k);
+            rows.push_back(stats);
+        }
+        writeTsv(outTsv, rows);
+        return (int)rows.size();
+    }
+    catch (const std::exception &e)
+    {
+        fprintf(stderr, "ref_scan_tsv_legacy: %s\n", e.what());
+        return -2;
+    }
+}
+
+// The string path's coverage (KmerDatabase::getMappingFromSequenceLegacy,
+// src/KmerDatabase.hpp:804-845; glue): per position, the upper-cased
+// k-mer, skipped if not ACGT, canonical by reverseComplement and string
+// order, CKMCFile::IsKmer; a hit adds 1 to its k bases.
+int ref_coverage_legacy(const char *dbPrefix, const char *fasta, const char *outPath)
+{
+    try
+    {
+        const bool gz = std::string(fasta).find(".gz") != std::string::npos;
+        ParseResult parsed = readSequenceFile(fasta, gz);
+        CKMCFile db;
+        if (!openRa(db, dbPrefix))
+            return -1;
+        const size_t k = db.KmerLength();
+        std::ofstream f(outPath);
+        for (size_t c = 0; c < parsed.sequences.size(); ++c)
+        {
+            const std::string &seq = parsed.sequences[c];
+            std::vector<int> cov(seq.size(), 0);
+            for (size_t i = 0; i + k <= seq.size(); ++i)
+            {
+                std::string km = seq.substr(i, k);
+                for (auto &ch : km)
+                    ch = (char)toupper((unsigned char)ch);
+                if (km.find_first_not_of("ACGT") != std::string::npos)
+                    continue;
+                const std::string rc = reverseComplement(km);
+                CKmerAPI api;
+                api.from_string(km > rc ? rc : km);
+                if (db.IsKmer(api))
+                    for (size_t j = 0; j < k; ++j)
+                        cov[i + j] += 1;
+            }
+            std::string line;
+            char buf[12];
+            for (size_t i = 0; i < seq.size(); ++i)
+            {
+                auto r = std::to_chars(buf, buf + sizeof(buf), cov[i]);
+                line.append(buf, r.ptr);
+                line.push_back(',');
+            }
+            if (!line.empty())
+                line.pop_back();
+            f << parsed.ids[c] << '\n' << line << std::endl;
+        }
+        return (int)parsed.sequences.size();
+    }
+    catch (const std::exception &e)
+    {
+        fprintf(stderr, "ref_coverage_legacy: %s\n", e.what());
+        return -2;
+    }
+}
+
+// Membership of n k-mer strings of length k (concatenated) through
+// CKmerAPI::from_string + CKMCFile::IsKmer.
+long long ref_kmc_is_kmer_strings(const char *prefix, const char *kmers, long long n, uint8_t *hit)
+{
+    CKMCFile db;
+    if (!openRa(db, prefix))
+        return -1;
+    const size_t k = db.KmerLength();
+    CKmerAPI api;
+    long long c = 0;
+    for (long long i = 0; i < n; ++i)
+    {
+        api.from_string(std::string(kmers + i * k, k));
+        hit[i] = db.IsKmer(api) ? 1 : 0;
+        c += hit[i];
+    }
+    return c;
 }
 
 // Every k-mer of a database via KmcRaIndex::forEachKmer; returns the count.

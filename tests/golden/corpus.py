@@ -138,6 +138,46 @@ def sequence_kmers(seqs, k: int) -> np.ndarray:
     return np.unique(np.concatenate(out))
 
 
+def canonical_codes_wide(codes: np.ndarray, k: int):
+    """canonical_codes for k = 33..64: codes as (high, low) uint64 words of
+    the 2k-bit code (first base most significant); returns (hi, lo, valid)."""
+    R, L = codes.shape
+    P = L - k + 1
+    fh = np.zeros((R, P), dtype=np.uint64)
+    fl = np.zeros((R, P), dtype=np.uint64)
+    rh = np.zeros((R, P), dtype=np.uint64)
+    rl = np.zeros((R, P), dtype=np.uint64)
+    bad = np.zeros((R, P), dtype=bool)
+    c64 = codes.astype(np.uint64) & np.uint64(3)
+    for j in range(k):
+        col = c64[:, j:j + P]
+        fh = (fh << np.uint64(2)) | (fl >> np.uint64(62))
+        fl = (fl << np.uint64(2)) | col
+        comp = np.uint64(3) - col
+        if j < 32:
+            rl |= comp << np.uint64(2 * j)
+        else:
+            rh |= comp << np.uint64(2 * (j - 32))
+        bad |= codes[:, j:j + P] > 3
+    if k < 64:
+        fh &= np.uint64((1 << (2 * (k - 32))) - 1)
+    f_lt = (fh < rh) | ((fh == rh) & (fl < rl))
+    return np.where(f_lt, fh, rh), np.where(f_lt, fl, rl), ~bad
+
+
+def sequence_kmers_wide(seqs, k: int) -> np.ndarray:
+    """Distinct canonical k-mers (k = 33..64) of whole sequences as an
+    (n, 2) array of (high, low) words, sorted."""
+    out = []
+    for s in seqs:
+        if s.size >= k:
+            h, l, v = canonical_codes_wide(s[None, :], k)
+            out.append(np.stack([h[v], l[v]], axis=1))
+    if not out:
+        return np.zeros((0, 2), dtype=np.uint64)
+    return np.unique(np.concatenate(out), axis=0)
+
+
 def revcomp(codes: np.ndarray, k: int) -> np.ndarray:
     """Reverse complement of 2-bit packed k-mer codes."""
     c = np.asarray(codes, dtype=np.uint64)
@@ -213,6 +253,6 @@ def edge_config(k: int, seed: int = 0x1B5E000):
             for a, b in ((20_000, 25_000), (100_000, 102_000), (180_000, 180_100)):
                 keep[a:b] = False
         sample_seqs.append(s[keep])
-    keys = sequence_kmers(sample_seqs, k)
+    keys = sequence_kmers(sample_seqs, k) if k <= 32 else sequence_kmers_wide(sample_seqs, k)
     records = [(name, to_text(codes, lower)) for name, codes, lower in contigs]
     return records, keys

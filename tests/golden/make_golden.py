@@ -9,7 +9,11 @@ key, through the reference's KmcRaIndex and CKMCFile. Expected outputs:
   - scan TSVs through the reference's CPU-OPT probe (CKMCFile::IsKmer) and
     CPU-BATCHED probe (KmcRaIndex + probeBatched), which must agree;
   - fastibsmapper coverage files (SHA-256 + size);
-  - KDBIntersect counts.
+  - KDBIntersect counts;
+  - for k > 32 (the edge corpus at k = 33, 47, 64), scan TSVs and coverage
+    through the reference's string path (getCanonicalKmersLegacy +
+    CKmerAPI::from_string + CKMCFile::IsKmer), the databases first checked
+    key for key through CKMCFile with the k-mers as strings.
 Only data is committed (expected.json and the small TSVs); the reference
 never travels.
 """
@@ -33,6 +37,8 @@ import corpus  # noqa: E402
 import hipibs  # noqa: E402
 
 EDGE_KS = (25, 31, 32, 15)
+WIDE_KS = (33, 47, 64)
+WIDE_WS = (50000, 1000, 100)
 EDGE_WS = (50000, 5000, 1000, 100, 33)
 SMALL_W = 50000
 
@@ -50,7 +56,30 @@ def load_ref():
     lib.ref_ra_probe.restype = C.c_longlong
     lib.ref_intersect.argtypes = [C.c_char_p, C.c_char_p]
     lib.ref_intersect.restype = C.c_longlong
+    lib.ref_scan_tsv_legacy.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p]
+    lib.ref_coverage_legacy.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
+    lib.ref_kmc_is_kmer_strings.argtypes = [C.c_char_p, C.c_char_p, C.c_longlong, C.c_void_p]
+    lib.ref_kmc_is_kmer_strings.restype = C.c_longlong
     return lib
+
+
+def wide_strings(keys, k):
+    """(n, 2) high/low codes as concatenated k-mer strings."""
+    out = []
+    for hi, lo in keys.tolist():
+        v = (hi << 64) | lo
+        out.append("".join("ACGT"[(v >> (2 * (k - 1 - i))) & 3] for i in range(k)))
+    return "".join(out).encode()
+
+
+def check_db_wide(ref, prefix, keys, k):
+    """Every key of a k > 32 database is found by CKMCFile, given as a string,
+    and the library's reader lists exactly the keys."""
+    hit = np.zeros(keys.shape[0], dtype=np.uint8)
+    assert ref.ref_kmc_is_kmer_strings(prefix.encode(), wide_strings(keys, k), keys.shape[0],
+                                       hit.ctypes.data) == keys.shape[0], "CKMCFile misses a key"
+    back = hipibs.kmc_read_keys(prefix)
+    assert np.array_equal(np.unique(back, axis=0), keys), "reader lists other keys"
 
 
 def sha(path):
@@ -140,6 +169,28 @@ def main():
             e["coverage_sha256"], e["coverage_bytes"] = sha(cov)
             exp["edge"][str(k)] = e
             print("edge k", k, "keys", keys.size)
+
+        # ---- k > 32: the reference's string path ----
+        exp["edge_wide"] = {}
+        for k in WIDE_KS:
+            recs, keys = corpus.edge_config(k)
+            fa = os.path.join(td, f"wide{k}_genome.fasta")
+            with open(fa, "wb") as f:
+                f.write(corpus.fasta(recs))
+            p = os.path.join(td, f"wide{k}")
+            hipibs.kmc_write(p, keys, k, counts=None, sig_len=9, counter_size=1,
+                             version=0x200, min_count=1, max_count=255)
+            check_db_wide(ref, p, keys, k)
+            e = {"n_keys": int(keys.shape[0]), "fasta_sha256": sha(fa)[0], "tsv": {}}
+            for W in WIDE_WS + (k + 1,):
+                out = os.path.join(td, f"w{k}_{W}.tsv")
+                assert ref.ref_scan_tsv_legacy(p.encode(), fa.encode(), W, out.encode()) > 0
+                e["tsv"][str(W)] = dict(zip(("sha256", "bytes"), sha(out)))
+            cov = os.path.join(td, f"w{k}.cov")
+            assert ref.ref_coverage_legacy(p.encode(), fa.encode(), cov.encode()) == len(recs)
+            e["coverage_sha256"], e["coverage_bytes"] = sha(cov)
+            exp["edge_wide"][str(k)] = e
+            print("edge k", k, "keys", keys.shape[0])
 
         # ---- KMC format variants: one key set, several layouts ----
         recs, keys = corpus.edge_config(25)

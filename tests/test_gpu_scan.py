@@ -7,7 +7,8 @@ import numpy as np
 import pytest
 
 import corpus
-from util import ROOT, coverage_line, oracle, oracle_scan_rows, sha256, sorted_keys
+from util import (ROOT, coverage_line, oracle, oracle_scan_rows, sha256, sorted_keys,
+                  sorted_keys_wide)
 
 import hipibs
 
@@ -526,3 +527,79 @@ def test_serving_results_same_alone_and_in_the_mix(small_corpus, tmp_path):
         want = oracle_scan_rows([b"c"], [seq[base:base + ln]], W, 25, sorted_keys(keys[sid]))
         got = [tuple(int(x) for x in row) for row in mixed[rid]]
         assert got == [w[3:] for w in want], rid
+
+
+def coverage_text_gpu(t, ids, seqs, k):
+    """fastibsmapper file content from the device hit mask + coverage."""
+    parts = []
+    for cid, s in zip(ids, seqs):
+        n = len(s)
+        npos = n - k + 1 if n >= k else 0
+        sw, ew, _ = hipibs.pack_sequences([s])
+        d_sw, d_ew = D.from_array(sw), D.from_array(ew)
+        d_hit = D.zeros(((npos + 31) // 32 + 1) * 4)
+        d_cov = D.zeros(max(n, 1))
+        if npos:
+            hipibs._check(hipibs._lib.hipibs_hitmask_device(t.handle, d_sw.ptr, d_ew.ptr, 0, npos,
+                                                            n, d_hit.ptr, None), "hitmask")
+        hipibs._check(hipibs._lib.hipibs_coverage_device(d_hit.ptr, npos, n, k, d_cov.ptr, None),
+                      "coverage")
+        hipibs.synchronize()
+        cov = d_cov.to_array(np.uint8, n) if n else np.zeros(0, np.uint8)
+        parts.append(cid + b"\n" + coverage_line(cov) + b"\n")
+    return b"".join(parts)
+
+
+@pytest.mark.parametrize("k", [33, 47, 64])
+def test_wide_tables_match_reference_string_path(tmp_path, k, expected):
+    """k > 32 (two-word keys): tables built from a KMC database (16 B
+    suffixes decoded on the device) and from host keys with duplicates scan
+    to the reference's string-path TSVs and coverage; membership and the
+    streamed intersect against set arithmetic."""
+    recs, keys = corpus.edge_config(k)
+    ids = [r[0] for r in recs]
+    seqs = [r[1] for r in recs]
+    p = str(tmp_path / "db")
+    hipibs.kmc_write(p, keys, k, counts=None, sig_len=9, counter_size=1, version=0x200,
+                     min_count=1, max_count=255)
+    note(f"wide k={k}: builds")
+    t1 = hipibs.Table.from_kmc(p)
+    t2 = hipibs.Table.from_keys(np.concatenate([keys, keys[:5000]]), k)
+    for t in (t1, t2):
+        info = t.info()
+        assert (info["n_keys"], info["k"]) == (keys.shape[0], k)
+    e = expected["edge_wide"][str(k)]
+    for W, want in e["tsv"].items():
+        note(f"wide k={k} W={W}")
+        both = hipibs.scan_sequences([t1, t2], ids, seqs, int(W), batch_bases=1 << 22)
+        for rows in both:
+            tsv = hipibs.format_tsv(rows)
+            assert (sha256(tsv), len(tsv)) == (want["sha256"], want["bytes"]), (k, W)
+    note(f"wide k={k}: coverage")
+    cov = coverage_text_gpu(t1, ids, seqs, k)
+    assert (sha256(cov), len(cov)) == (e["coverage_sha256"], e["coverage_bytes"])
+    assert t1.probe_count(keys) == keys.shape[0]
+    rng = np.random.default_rng(k)
+    q = rng.integers(0, 1 << 63, size=(20000, 2), dtype=np.uint64)
+    q[:, 0] &= np.uint64((1 << (2 * (k - 32))) - 1) if k < 64 else np.uint64(~0 & (2**64 - 1))
+    q[:10000] = keys[rng.integers(0, keys.shape[0], 10000)]
+    pair = np.dtype([("h", "<u8"), ("l", "<u8")])
+    want_hits = int(np.isin(np.ascontiguousarray(q).view(pair),
+                            np.ascontiguousarray(keys).view(pair)).sum())
+    assert t2.probe_count(q) == want_hits
+    sub = np.unique(np.concatenate([keys[::3], sorted_keys_wide(q[10000:])]), axis=0)
+    p2 = str(tmp_path / "db2")
+    hipibs.kmc_write(p2, sub, k, counts=None, sig_len=9, counter_size=0, version=0x200,
+                     min_count=1, max_count=1)
+    inter = int(np.isin(sub.view(pair), np.ascontiguousarray(keys).view(pair)).sum())
+    assert t1.probe_count_kmc(p2) == inter
+    t1.free()
+    t2.free()
+
+
+def test_wide_tables_refused_where_unsupported(tmp_path):
+    recs, keys = corpus.edge_config(47)
+    with pytest.raises(hipibs.HipibsError, match="minimizer"):
+        hipibs.Table.from_keys(keys[:100], 47, minimizer_len=9)
+    with pytest.raises(hipibs.HipibsError, match="128 B"):
+        hipibs.Table.from_keys(keys[:100], 47, bucket_slots=8)

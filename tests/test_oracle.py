@@ -16,6 +16,10 @@ Each oracle function is checked against something independent of it:
   orc_intersect         the golden KDBIntersect counts and a Python set
   orc_kmc_read          keys written by the KMC writer, read back and listed
                         by the reference's KmcRaIndex when the databases were made
+  orc_*_wide            (k = 33..64) the string oracle for k up to 64; the
+                        golden TSVs and coverage of the reference's string
+                        path (getCanonicalKmersLegacy + CKMCFile::IsKmer on
+                        strings) at k = 33, 47, 64
   orc_count_kmers /     (KMC's counting rules, the k-mer counter's oracle) a
   corpus.count_kmers    dictionary count over substrings of each read's text
                         with the string canonical form; each other on the
@@ -28,8 +32,9 @@ import numpy as np
 import pytest
 
 import corpus
-from util import (ROOT, coverage_line, oracle, oracle_count, oracle_coverage_text,
-                  oracle_scan_rows, sha256, sorted_keys)
+from util import (REF_PATH, ROOT, coverage_line, oracle, oracle_count, oracle_coverage_text,
+                  oracle_coverage_text_wide, oracle_scan_rows, oracle_scan_rows_wide, sha256,
+                  sorted_keys, sorted_keys_wide)
 
 import hipibs
 
@@ -272,3 +277,69 @@ def test_count_kmers_reproduces_golden_key_counts(expected):
             ok, oc, _ = oracle_count(b"\n".join(corpus.to_text(x) for x in r), 25)
             np.testing.assert_array_equal(ok, keys)
             np.testing.assert_array_equal(oc, counts)
+
+
+@pytest.mark.parametrize("k", [33, 40, 47, 63, 64])
+def test_wide_codec_against_string_oracle(k):
+    rng = np.random.default_rng(k)
+    seq = bytes(rng.choice(list(b"ACGTacgt" * 25 + b"N"), 3000).tolist())
+    np_ = len(seq) - k + 1
+    codes = np.zeros(2 * np_, dtype=np.uint64)
+    valid = np.zeros(np_, dtype=np.uint8)
+    assert oracle().orc_encode_positions_wide(seq, len(seq), k, codes.ctypes.data,
+                                              valid.ctypes.data) == np_
+    want = string_canonical(seq, k)
+    assert sum(v is not None for v in want) > 100
+    for i, w in enumerate(want):
+        assert bool(valid[i]) == (w is not None), i
+        if w is not None:
+            assert (int(codes[2 * i]) << 64) | int(codes[2 * i + 1]) == w, i
+
+
+@pytest.mark.parametrize("k", [33, 47, 64])
+def test_wide_edge_tsv_and_coverage_match_reference(k, expected):
+    """The oracle's two-word path against the reference's string path."""
+    recs, keys = corpus.edge_config(k)
+    ids = [r[0] for r in recs]
+    seqs = [r[1] for r in recs]
+    ks = sorted_keys_wide(keys)
+    e = expected["edge_wide"][str(k)]
+    assert e["n_keys"] == ks.shape[0]
+    for W, want in e["tsv"].items():
+        tsv = hipibs.format_tsv(oracle_scan_rows_wide(ids, seqs, int(W), k, ks))
+        assert (sha256(tsv), len(tsv)) == (want["sha256"], want["bytes"]), (k, W)
+    cov = oracle_coverage_text_wide(ids, seqs, k, ks)
+    assert (sha256(cov), len(cov)) == (e["coverage_sha256"], e["coverage_bytes"])
+
+
+@pytest.mark.parametrize("k", [33, 48, 64])
+def test_wide_kmc_writer_round_trip(tmp_path, k):
+    """KMC databases of k > 32 written by the library read back key for key,
+    and the reference's CKMCFile finds each key given as a string."""
+    import ctypes as C
+    recs, keys = corpus.edge_config(k)
+    keys = keys[::7]
+    p = str(tmp_path / "db")
+    counts = (np.arange(keys.shape[0]) % 300 + 1).astype(np.uint32)
+    hipibs.kmc_write(p, keys, k, counts=counts, sig_len=9, counter_size=2, version=0x200,
+                     min_count=1, max_count=65535)
+    back = hipibs.kmc_read_keys(p)
+    np.testing.assert_array_equal(np.unique(back, axis=0), keys)
+    info = hipibs.kmc_header(p)
+    assert (info.kmer_length, info.counter_size, info.total_kmers) == (k, 2, keys.shape[0])
+    if k < 64:  # a key wider than 2k bits (every 128-bit value is a 64-mer code)
+        bad = keys[:2].copy()
+        bad[0, 0] = np.uint64(1) << np.uint64(63)
+        with pytest.raises(hipibs.HipibsError):
+            hipibs.kmc_write(str(tmp_path / "bad"), bad, k)
+    if not os.path.exists(REF_PATH):
+        return
+    ref = C.CDLL(REF_PATH)
+    ref.ref_kmc_is_kmer_strings.argtypes = [C.c_char_p, C.c_char_p, C.c_longlong, C.c_void_p]
+    ref.ref_kmc_is_kmer_strings.restype = C.c_longlong
+    sub = keys[::10]
+    text = "".join("".join("ACGT"[(((int(h) << 64) | int(l)) >> (2 * (k - 1 - i))) & 3]
+                           for i in range(k)) for h, l in sub.tolist()).encode()
+    hit = np.zeros(sub.shape[0], dtype=np.uint8)
+    assert ref.ref_kmc_is_kmer_strings(p.encode(), text, sub.shape[0],
+                                       hit.ctypes.data) == sub.shape[0]

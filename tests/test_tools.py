@@ -115,6 +115,41 @@ def test_kdbintersect_matches_golden(small_layout, expected):
 
 
 @pytest.mark.gpu
+def test_tools_with_k_above_32(tmp_path, expected):
+    """k = 47 databases (two-word keys): the fastibs TSV and the
+    fastibsmapper file equal the reference's string path; a panel run equals
+    single runs; KDBIntersect counts the shared keys."""
+    k = 47
+    recs, keys = corpus.edge_config(k)
+    (tmp_path / "refs").mkdir()
+    (tmp_path / "refs" / "edge_genome.fasta").write_bytes(corpus.fasta(recs))
+    for name, ks in (("w0", keys), ("w1", keys[::2])):
+        (tmp_path / name).mkdir()
+        hipibs.kmc_write(str(tmp_path / name / "sample"), ks, k, counts=None, sig_len=9,
+                         counter_size=1, version=0x200, min_count=1, max_count=255)
+    e = expected["edge_wide"][str(k)]
+    r = tool("fastibs", tmp_path / "w0", tmp_path / "refs", tmp_path / "one", 1000)
+    assert r.returncode == 0, r.stderr
+    single0 = (tmp_path / "one" / "w0_v_edge_1000.tsv").read_bytes()
+    assert sha256(single0) == e["tsv"]["1000"]["sha256"]
+    r = tool("fastibs", tmp_path / "w1", tmp_path / "refs", tmp_path / "one", 1000)
+    assert r.returncode == 0, r.stderr
+    r = tool("fastibs", tmp_path / "w0", tmp_path / "refs", tmp_path / "panel", 1000,
+             f"--db={tmp_path / 'w1'}")
+    assert r.returncode == 0, r.stderr
+    for name in ("w0", "w1"):
+        assert (tmp_path / "panel" / f"{name}_v_edge_1000.tsv").read_bytes() == \
+            (tmp_path / "one" / f"{name}_v_edge_1000.tsv").read_bytes(), name
+    r = tool("fastibsmapper", tmp_path / "w0", tmp_path / "refs", tmp_path / "map")
+    assert r.returncode == 0, r.stderr
+    assert sha256((tmp_path / "map" / "w0_v_edge_genome.txt").read_bytes()) == \
+        e["coverage_sha256"]
+    r = tool("KDBIntersect", tmp_path / "w0", tmp_path / "w1")
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == f"{keys[::2].shape[0]}\n"
+
+
+@pytest.mark.gpu
 def test_kmerdatabase_members_match_golden(small_layout, tmp_path, expected):
     """The KmerDatabase members the tools do not call (per-contig mapping,
     chunked mapping into a difference array + serializeCoverage, the
@@ -126,7 +161,8 @@ def test_kmerdatabase_members_match_golden(small_layout, tmp_path, expected):
     dbs = [small_layout / f"s{i}" / "sample" for i in range(3)]
     r = tool("test_api", small_layout / "refs" / "small_genome.fasta", out, 50000, *dbs)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert ("needsStringPath(25) 0, needsStringPath(33) 1, devicesToUse(4) 1, gpuRequested 1, "
+    assert ("needsStringPath(25) 0, needsStringPath(64) 0, needsStringPath(65) 1, "
+            "devicesToUse(4) 1, gpuRequested 1, "
             "gpuSyncRequested 0, poolThreads>0 1") in r.stdout
     n01 = expected["small"]["intersect"]["0-1"]
     assert f"intersect 0-1: probeCountHost {n01} probeHost {n01} devices 1" in r.stdout

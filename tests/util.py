@@ -34,6 +34,11 @@ def oracle():
         lib.orc_scan_contig.restype = i64
         lib.orc_count_kmers.argtypes = [C.c_char_p, i64, u32, vp, vp, i64, C.POINTER(i64)]
         lib.orc_count_kmers.restype = i64
+        lib.orc_encode_positions_wide.argtypes = [C.c_char_p, i64, u32, vp, vp]
+        lib.orc_encode_positions_wide.restype = i64
+        lib.orc_scan_contig_wide.argtypes = [C.c_char_p, i64, i64, u32, vp, i64, vp, i64]
+        lib.orc_scan_contig_wide.restype = i64
+        lib.orc_coverage_wide.argtypes = [C.c_char_p, i64, u32, vp, i64, vp]
         _orc = lib
     return _orc
 
@@ -64,6 +69,36 @@ def oracle_coverage_text(ids, seqs, k, keys_sorted):
     for cid, s in zip(ids, seqs):
         cov = np.zeros(len(s), dtype=np.int32)
         lib.orc_coverage(s, len(s), k, keys_sorted.ctypes.data, keys_sorted.size, cov.ctypes.data)
+        parts.append(cid + b"\n" + coverage_line(cov) + b"\n")
+    return b"".join(parts)
+
+
+def sorted_keys_wide(keys):
+    """(n, 2) high/low codes sorted as 128-bit integers, contiguous."""
+    return np.ascontiguousarray(np.unique(np.asarray(keys, dtype=np.uint64), axis=0))
+
+
+def oracle_scan_rows_wide(ids, seqs, W, k, keys2):
+    """TSV rows for k = 33..64 through the oracle (keys2 from sorted_keys_wide)."""
+    lib = oracle()
+    rows = []
+    for cid, s in zip(ids, seqs):
+        cap = len(s) // max(1, W - k) + 2
+        out = np.zeros((cap, 6), dtype=np.int64)
+        n = lib.orc_scan_contig_wide(s, len(s), W, k, keys2.ctypes.data, keys2.shape[0],
+                                     out.ctypes.data, cap)
+        assert 0 <= n <= cap
+        for r in out[:n]:
+            rows.append((cid,) + tuple(int(x) for x in r))
+    return rows
+
+
+def oracle_coverage_text_wide(ids, seqs, k, keys2):
+    lib = oracle()
+    parts = []
+    for cid, s in zip(ids, seqs):
+        cov = np.zeros(len(s), dtype=np.int32)
+        lib.orc_coverage_wide(s, len(s), k, keys2.ctypes.data, keys2.shape[0], cov.ctypes.data)
         parts.append(cid + b"\n" + coverage_line(cov) + b"\n")
     return b"".join(parts)
 
