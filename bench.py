@@ -71,12 +71,15 @@ def parse():
                     help="count workload: genome length in Mbp (10x of reads)")
     ap.add_argument("--count-cpu-reads", type=float, default=2000000,
                     help="count workload: reads the CPU oracle counts")
+    ap.add_argument("--wide-k", type=int, default=47, help="wide workload: k (33..64)")
+    ap.add_argument("--wide-cpu-mbp", type=float, default=8.0,
+                    help="wide workload: Mbp of the reference's string-path CPU sample")
     ap.add_argument("--workload", default="scan",
-                    choices=["scan", "panel", "serve", "large", "count"],
+                    choices=["scan", "panel", "serve", "large", "count", "wide"],
                     help="scan: the single-sample device scan (default line); panel: "
                          "end-to-end fastibs runs, N single-sample runs vs one panel run; "
                          "serve: served queries; large: the k=31 4.2 Gbp scan + coverage; "
-                         "count: k-mer counting from reads")
+                         "count: k-mer counting from reads; wide: the scan at k = 47")
     ap.add_argument("--cpu-mapper-mbp", type=float, default=8.0,
                     help="large: bases (Mbp) of the reference mapper's CPU sample")
     ap.add_argument("--e2e", type=int, default=1,
@@ -981,6 +984,183 @@ def run_large(args):
     print(json.dumps(out), flush=True)
 
 
+def encode_wide(torch, sw, ew, n, k, npos=None):
+    """(npos, 2) int64 two-word canonical codes of the first npos positions
+    (default all) of one contig of n bases (all ones where invalid)."""
+    import hipibs
+    npos = n - k + 1 if npos is None else npos
+    out = torch.empty((npos, 2), dtype=torch.int64, device="cuda")
+    hipibs._check(hipibs._lib.hipibs_encode_device(sw.data_ptr(), ew.data_ptr(), 0, npos, n, k,
+                                                   out.data_ptr(), None, None), "encode")
+    return out
+
+
+def run_wide(args):
+    """The single-sample scan at k = 47 (two-word keys, the reference's
+    string path): the default workload's 2 Gbp reference against the
+    distinct 47-mers of 1.6e9 haplotype positions (1% SNPs; repeat copies
+    collapse) plus those of a random 0.4 Gbp sequence, 16 B per key in
+    128 B buckets at load 0.6."""
+    import numpy as np
+    import torch
+
+    import hipibs
+    assert torch.cuda.is_available(), "no GPU visible"
+    k = args.wide_k
+    n = int(args.gbp * 1e9) // 64 * 64
+    t0 = time.time()
+    ref_codes = gen_reference(torch, n, args.seed)
+    sw, ew = pack(torch, ref_codes)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(args.seed + 1)
+    hap = ref_codes.clone()
+    snp = (torch.rand(n, device="cuda", generator=g) < 0.01) & (hap < 4)
+    hap[snp] = (hap[snp] + torch.randint(1, 4, (n,), dtype=torch.uint8, device="cuda",
+                                         generator=g)[snp]) % 4
+    del snp
+    hsw, hew = pack(torch, hap)
+    del hap
+    # the first 80% of the haplotype's positions and a random sequence's
+    # k-mers; invalid positions (all ones) and repeated k-mers (repeat
+    # copies) are left to the table build, which skips and collapses them.
+    # (No row filtering in torch here: its 2-D gathers of ~1e8 rows have
+    # returned damaged rows on this stack.)
+    n_hap = min(n - k + 1, int(0.8 * args.keys))
+    hc = encode_wide(torch, hsw, hew, n, k, n_hap)
+    del hsw, hew
+    n_rnd = max(64, int(args.keys) - n_hap) // 64 * 64 + 64
+    rnd = torch.randint(0, 4, (n_rnd,), dtype=torch.uint8, device="cuda", generator=g)
+    rsw, rew = pack(torch, rnd)
+    rc = encode_wide(torch, rsw, rew, rnd.numel(), k)
+    del rnd, rsw, rew
+    keys = torch.cat([hc, rc])
+    del hc, rc
+    torch.cuda.empty_cache()
+    torch.cuda.synchronize()
+    log(f"k={k}: {keys.shape[0]/1e9:.3f} G sample k-mers ({time.time()-t0:.1f} s); "
+        f"building the table")
+    tb = time.time()
+    table = hipibs.Table.from_device_keys(keys.data_ptr(), keys.shape[0], k)
+    build_s = time.time() - tb
+    info = table.info()
+    cpu_slice = int(args.wide_cpu_mbp * 1e6) // 64 * 64
+    del keys
+    torch.cuda.empty_cache()
+    log(f"table: {info['n_keys']/1e9:.3f} G keys, {info['bytes']/1e9:.1f} GB, build {build_s:.2f} s")
+
+    wins = hipibs.chunk_sequences([n], W, k)
+    win = np.zeros(len(wins), dtype=hipibs.WINDOW_DTYPE)
+    win["base"] = np.array([w[1] for w in wins], dtype=np.uint64)
+    win["n_bases"] = np.array([w[2] - w[1] for w in wins], dtype=np.uint32)
+    d_win = torch.from_numpy(win.view(np.uint8)).cuda()
+    d_stats = torch.zeros(len(wins) * 4, dtype=torch.int32, device="cuda")
+    segs = C.c_uint32(0)
+    wb = hipibs._lib.hipibs_scan_work_bytes(len(wins), 1, W, k, 0, C.byref(segs))
+    d_work = torch.zeros(max(wb, 8), dtype=torch.uint8, device="cuda")
+    tabs = hipibs._table_array([table])
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        hipibs._check(hipibs._lib.hipibs_scan_windows_device(
+            tabs, 1, sw.data_ptr(), ew.data_ptr(), n, d_win.data_ptr(), len(wins), W,
+            d_stats.data_ptr(), 0, d_work.data_ptr(), wb, stream), "scan")
+
+    hipibs._lib.hipibs_set_profiling(1)
+    for _ in range(args.warmup):
+        step()
+    kms = []
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        kms.append(hipibs._lib.hipibs_last_kernel_ms())
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - ts) / args.steps * 1e3
+    st = d_stats.cpu().numpy().reshape(-1, 4).copy()
+    probes = int(st[:, 0].astype(np.int64).sum())
+    observed = int(st[:, 1].astype(np.int64).sum())
+    k_ms = float(np.mean(kms))
+    alg_bytes = 128.0 * probes + 0.375 * n
+    log(f"scan k={k}: {ms:.2f} ms/step, kernel {k_ms:.2f} ms, {probes/1e9:.3f} G probes, "
+        f"hit rate {observed/max(1, probes):.3f}")
+
+    # the reference's string path on a slice, and the GPU on the same slice
+    cpu = None
+    if cpu_slice and os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libref.so")):
+        try:
+            ref = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref.so"))
+            ref.ref_scan_tsv_legacy.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p]
+            # the sample's keys that occur in the slice, found by probing the table
+            slice_codes = ref_codes[:cpu_slice]
+            ssw, sew = pack(torch, slice_codes)
+            sc = encode_wide(torch, ssw, sew, cpu_slice, k)
+            ok = (sc[:, 0] != -1).to(torch.uint8)
+            hit = torch.zeros(sc.shape[0], dtype=torch.uint8, device="cuda")
+            hipibs._check(hipibs._lib.hipibs_table_probe(table.handle, sc.data_ptr(),
+                                                         ok.data_ptr(), sc.shape[0],
+                                                         hit.data_ptr(), None), "probe")
+            hc_ = sc.cpu().numpy().view(np.uint64)
+            sub = np.unique(hc_[hit.cpu().numpy() == 1], axis=0)
+            with tempfile.TemporaryDirectory(dir=args.workdir) as td:
+                pre = os.path.join(td, "slice_db")
+                hipibs.kmc_write(pre, sub, k, sig_len=9, counter_size=1, version=0x200,
+                                 min_count=1, max_count=255)
+                fa = os.path.join(td, "slice.fasta")
+                write_fasta(fa, slice_codes.cpu().numpy())
+                out = os.path.join(td, "slice.tsv")
+                tc = time.perf_counter()
+                nrows = ref.ref_scan_tsv_legacy(pre.encode(), fa.encode(), W, out.encode())
+                cpu_s = time.perf_counter() - tc
+                ref_tsv = open(out, "rb").read()
+                t2 = hipibs.Table.from_keys(sub, k)
+                rows = hipibs.scan_sequences([t2], [b"chr1"],
+                                             [np.frombuffer(b"ACGTN", np.uint8)[
+                                                 slice_codes.cpu().numpy()].tobytes()], W)[0]
+                t2.free()
+                same = hipibs.format_tsv(rows) == ref_tsv
+            sprobes = sum(r[3] for r in rows)
+            cpu = {"value": sprobes / cpu_s, "unit": "ref k-mers probed/s", "cores": 1,
+                   "kind": "reference",
+                   "sample": f"first {cpu_slice/1e6:.0f} Mbp of the reference ({nrows} windows) "
+                             f"vs a KMC2 database of the sample's keys that occur there "
+                             f"({sub.shape[0]} keys); the reference's string path for k > 32 "
+                             f"(getCanonicalKmersLegacy + CKmerAPI::from_string + "
+                             f"CKMCFile::IsKmer), one thread; TSV identical to the GPU's: {same}"}
+            log(f"reference string path: {sprobes/cpu_s/1e6:.2f} M probes/s on {cpu_slice/1e6:.0f}"
+                f" Mbp, TSV identical {same}")
+        except Exception as e:
+            log(f"cpu baseline failed: {e}")
+    out = {
+        "metric": "ref k-mers probed/s",
+        "value": probes / (ms / 1e3),
+        "unit": "ref k-mers probed/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64x2",
+        "data": "synthetic (GPU-generated, seeded)",
+        "config": {"workload": f"single-sample scan, k={k} (two-word keys): 2 Gbp synthetic "
+                               f"reference vs ~2e9-key sample, W=50 kbp",
+                   "reference_bases": n, "windows": len(wins), "k": k, "window": W,
+                   "sample_keys": int(info["n_keys"]), "table_bytes": int(info["bytes"]),
+                   "table_build_s": build_s, "segs_per_window": segs.value,
+                   "probes_per_step": probes, "hit_rate": observed / max(1, probes)},
+        "roofline": {"bound": "hbm", "achieved": alg_bytes / (k_ms / 1e3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg_bytes / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "scanKernel<16, false, true>", "kernel_ms": k_ms,
+                     "algorithmic_bytes": alg_bytes,
+                     "probe_rate_gps": probes / (k_ms / 1e3) / 1e9},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    table.free()
+
+
 def gen_reads_text(torch, hap, n_reads, read_len, err, seed):
     """Reads of a haplotype as device text: uniform starts, random strand,
     i.i.d. substitution errors (N stays N); each read followed by '\n'."""
@@ -1169,6 +1349,8 @@ def main():
         return run_large(args)
     if args.workload == "count":
         return run_count(args)
+    if args.workload == "wide":
+        return run_wide(args)
     import numpy as np
     import torch
 

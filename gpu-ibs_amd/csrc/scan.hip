@@ -312,19 +312,20 @@ __global__ void coverageKernel(const uint32_t *__restrict__ hitWords, uint64_t n
 }
 
 // Canonical code per position (all ones when invalid).
+template <bool WIDE>
 __global__ void encodeKernel(const uint64_t *__restrict__ seq, const uint64_t *__restrict__ exc,
                              uint64_t pos0, uint64_t nPos, uint64_t contigEnd, uint32_t k,
                              uint64_t *__restrict__ codes, uint8_t *__restrict__ valid)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t kmask1 = (1ull << k) - 1ull;
-    const uint64_t kmask2 = k == 32 ? ~0ull : (1ull << (2 * k)) - 1ull;
-    const uint32_t fwdShift = 64 - 2 * k;
+    const uint64_t kmask1 = k >= 64 ? ~0ull : (1ull << k) - 1ull;
+    const uint64_t kmask2 = k >= 32 ? ~0ull : (1ull << (2 * k)) - 1ull;
+    const uint32_t fwdShift = k >= 32 ? 0u : 64 - 2 * k;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nPos; i += stride)
     {
         const uint64_t p = pos0 + i;
         bool ok = p + k <= contigEnd;
-        uint64_t canon = EMPTY;
+        uint64_t canon = EMPTY, hi = EMPTY;
         if (ok)
         {
             const uint64_t wi = p >> 5;
@@ -337,14 +338,27 @@ __global__ void encodeKernel(const uint64_t *__restrict__ seq, const uint64_t *_
             const uint64_t ea = exc[ej], eb = exc[ej + 1];
             const uint64_t ev = es ? ((ea >> es) | (eb << (64 - es))) : ea;
             ok = (ev & kmask1) == 0;
-            if (ok)
+            if (ok && WIDE)
+            {
+                const uint64_t x0 = sh ? ((a >> sh) | (b << (64 - sh))) : a;
+                const uint64_t c = seq[wi + 2];
+                const uint64_t x1 = sh ? ((b >> sh) | (c << (64 - sh))) : b;
+                wideCanonical(x0, x1, k, hi, canon);
+            }
+            else if (ok)
             {
                 const uint64_t rc = ~x & kmask2;
                 const uint64_t fwd = reverse2(x) >> fwdShift;
                 canon = fwd < rc ? fwd : rc;
             }
         }
-        codes[i] = canon;
+        if (WIDE)
+        {
+            codes[2 * i] = hi;
+            codes[2 * i + 1] = canon;
+        }
+        else
+            codes[i] = canon;
         if (valid)
             valid[i] = ok ? 1 : 0;
     }
@@ -836,7 +850,7 @@ extern "C" int hipibs_encode_device(const uint64_t *d_seq_words, const uint64_t 
                                     uint64_t pos0, uint64_t n_pos, uint64_t contig_end, uint32_t k,
                                     uint64_t *d_codes, uint8_t *d_valid, void *stream)
 {
-    if (k == 0 || k > 32 || (n_pos && (!d_seq_words || !d_exc_words || !d_codes)))
+    if (k == 0 || k > 64 || (n_pos && (!d_seq_words || !d_exc_words || !d_codes)))
         return setError(HIPIBS_E_ARG, "bad argument");
     if (n_pos == 0)
         return 0;
@@ -844,8 +858,12 @@ extern "C" int hipibs_encode_device(const uint64_t *d_seq_words, const uint64_t 
     int dev = 0;
     (void)hipGetDevice(&dev);
     const uint64_t grid = std::min<uint64_t>((n_pos + 255) / 256, (uint64_t)deviceCUs(dev) * 16);
-    hipLaunchKernelGGL(encodeKernel, dim3((uint32_t)grid), dim3(256), 0, s, d_seq_words,
-                       d_exc_words, pos0, n_pos, contig_end, k, d_codes, d_valid);
+    if (k > 32)
+        hipLaunchKernelGGL(encodeKernel<true>, dim3((uint32_t)grid), dim3(256), 0, s, d_seq_words,
+                           d_exc_words, pos0, n_pos, contig_end, k, d_codes, d_valid);
+    else
+        hipLaunchKernelGGL(encodeKernel<false>, dim3((uint32_t)grid), dim3(256), 0, s,
+                           d_seq_words, d_exc_words, pos0, n_pos, contig_end, k, d_codes, d_valid);
     HIPIBS_CHECK(hipGetLastError());
     return 0;
 }

@@ -91,11 +91,20 @@ __global__ __launch_bounds__(256) void insertKernel(uint64_t *slots, const Table
 }
 
 // One 33..64-base key per thread (keys[2i] high word, keys[2i+1] low word)
-// into a table of 8 keys per 128 B bucket. The keys of one build must be
-// distinct (KMC databases are; host key arrays are made so): a thread
-// claims the first slot whose high word it can move from EMPTY to its own
-// and then writes the low word, and never needs to compare with a key
-// being stored. Counters as insertKernel.
+// into a table of 8 keys per 128 B bucket; a key already stored is not
+// stored again. A thread claims a slot by moving its high word from EMPTY
+// to its own (CAS) and then publishes the low word with a device-scope
+// store. A thread that meets its own high word in a claimed slot reads the
+// low word -- waiting, at most kWideSpin reads, while it still reads EMPTY
+// (the claimer's store follows its CAS directly; within a wave the claim
+// step precedes the compare step in program order) -- and stops if it is
+// its own. A low word that really is all ones outlasts the wait and the key
+// is stored in a further slot: a harmless extra copy. A walk ends after
+// kWideMaxWalk buckets (a few at load <= 0.98), then the table reports
+// overflow. A row whose high word is all ones holds no key and is skipped.
+// Counters as insertKernel.
+constexpr uint64_t kWideMaxWalk = 4096;
+constexpr uint32_t kWideSpin = 1u << 14;
 __global__ __launch_bounds__(256) void insertWideKernel(uint64_t *slots, const TableView T,
                                                         const uint64_t *keys, uint64_t n,
                                                         unsigned long long *counters)
@@ -103,30 +112,48 @@ __global__ __launch_bounds__(256) void insertWideKernel(uint64_t *slots, const T
     uint64_t localNew = 0;
     uint32_t localDisp = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t walk = T.nBuckets < kWideMaxWalk ? T.nBuckets : kWideMaxWalk;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     {
         const uint64_t hi = keys[2 * i], lo = keys[2 * i + 1];
+        if (hi == EMPTY)
+            continue; // no key (the encoder's mark of an invalid position)
         uint64_t b = bucketOfWide(hi, lo, T.nBuckets);
-        bool stored = false;
-        for (uint64_t d = 0; d < T.nBuckets && !stored; ++d)
+        bool done = false;
+        for (uint64_t d = 0; d < walk && !done; ++d)
         {
             unsigned long long *bucket = reinterpret_cast<unsigned long long *>(slots + b * 16);
-            for (int q = 0; q < 8; ++q)
+            for (int q = 0; q < 8 && !done; ++q)
             {
-                const unsigned long long cur = __hip_atomic_load(
-                    bucket + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (cur == EMPTY && atomicCAS(bucket + 2 * q, EMPTY, hi) == EMPTY)
+                unsigned long long *hw = bucket + 2 * q, *lw = bucket + 2 * q + 1;
+                unsigned long long cur =
+                    __hip_atomic_load(hw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (cur == EMPTY)
                 {
-                    bucket[2 * q + 1] = lo;
-                    stored = true;
-                    localNew += 1;
-                    localDisp = max(localDisp, (uint32_t)min<uint64_t>(d, 0xffffffffu));
-                    break;
+                    const unsigned long long prev = atomicCAS(hw, EMPTY, hi);
+                    if (prev == EMPTY)
+                    {
+                        __hip_atomic_store(lw, (unsigned long long)lo, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                        done = true;
+                        localNew += 1;
+                        localDisp = max(localDisp, (uint32_t)min<uint64_t>(d, 0xffffffffu));
+                    }
+                    else
+                        cur = prev;
+                }
+                if (!done && cur == hi)
+                {
+                    unsigned long long l =
+                        __hip_atomic_load(lw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    for (uint32_t w = 0; l == EMPTY && w < kWideSpin; ++w)
+                        l = __hip_atomic_load(lw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    done = l == lo;
                 }
             }
             b = b + 1 == T.nBuckets ? 0 : b + 1;
         }
-        if (!stored)
+        if (!done)
             atomicExch(counters + 1, 1ull);
     }
     if (localNew)
@@ -580,7 +607,9 @@ static int finishTable(hipibs_table *t, hipStream_t s)
     HIPIBS_CHECK(hipMemcpyAsync(c, t->dCounters, sizeof(c), hipMemcpyDeviceToHost, s));
     HIPIBS_CHECK(hipStreamSynchronize(s));
     if (c[1])
-        return setError(HIPIBS_E_FULL, "table overflow: no free slot for a key");
+        return setError(HIPIBS_E_FULL, t->k > 32 ? "table overflow: a key found no free slot "
+                                                   "within 4096 buckets"
+                                                 : "table overflow: no free slot for a key");
     t->nKeys = c[0];
     t->maxDisp = (uint32_t)c[2];
     return 0;
@@ -1025,28 +1054,17 @@ extern "C" int hipibs_table_build_keys(const uint64_t *h_keys, uint64_t n, uint3
     *out = nullptr;
     if (k > 32 && k <= 64)
     {
-        // distinct (high, low) pairs, then one upload and the wide insert
-        std::vector<Wide> w(n);
-        for (uint64_t i = 0; i < n; ++i)
-            w[i] = ((Wide)h_keys[2 * i] << 64) | h_keys[2 * i + 1];
-        std::sort(w.begin(), w.end());
-        w.erase(std::unique(w.begin(), w.end()), w.end());
-        std::vector<uint64_t> flat(2 * w.size());
-        for (size_t i = 0; i < w.size(); ++i)
-        {
-            flat[2 * i] = (uint64_t)(w[i] >> 64);
-            flat[2 * i + 1] = (uint64_t)w[i];
-        }
+        // two words per key: one upload, then the wide insert
         HIPIBS_CHECK(hipSetDevice(opts ? opts->device : 0));
         uint64_t *dAll = nullptr;
-        if (hipMalloc(reinterpret_cast<void **>(&dAll), std::max<size_t>(flat.size(), 1) * 8) !=
+        if (hipMalloc(reinterpret_cast<void **>(&dAll), std::max<uint64_t>(2 * n, 1) * 8) !=
             hipSuccess)
             return setError(HIPIBS_E_NOMEM, "device key buffer allocation failed");
-        int rc = hipMemcpy(dAll, flat.data(), flat.size() * 8, hipMemcpyHostToDevice) == hipSuccess
+        int rc = hipMemcpy(dAll, h_keys, 2 * n * 8, hipMemcpyHostToDevice) == hipSuccess
                      ? 0
                      : setError(HIPIBS_E_HIP, "key upload failed");
         if (!rc)
-            rc = hipibs_table_build_keys_device(dAll, w.size(), k, opts, out, nullptr);
+            rc = hipibs_table_build_keys_device(dAll, n, k, opts, out, nullptr);
         (void)hipDeviceSynchronize();
         hipFree(dAll);
         return rc;

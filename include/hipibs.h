@@ -9,7 +9,15 @@
  *
  * Data formats shared by every entry point:
  *  - k-mer key: canonical 2-bit packed code, A=0 C=1 G=2 T=3, first base in
- *    the most significant used bits, k <= 32 (src/KmerCodec.hpp:18-60).
+ *    the most significant used bits (src/KmerCodec.hpp:18-60). k <= 32: one
+ *    uint64_t per key. 33 <= k <= 64 (the reference's string path,
+ *    src/KmerDatabase.hpp:104-123, :675-709): two uint64_t per key, the high
+ *    word first, so every key array, code array and count argument below
+ *    holds 2n words for n keys when the table's (or database's) k is above
+ *    32. Tables of k > 32 have 128 B buckets of 8 keys, and their builds
+ *    skip keys whose high word is all ones (hipibs_encode_device's mark of
+ *    an invalid position); panel launches, minimizer tables, serving and the
+ *    k-mer counter take k <= 32.
  *  - packed bases: base b of a stream lives at bits 2*(b%32) of 64-bit word
  *    b/32 (code of a non-ACGT base is 0); its exception bit (set when the
  *    byte is not one of ACGTacgt) is bit b%64 of word b/64 of a separate
@@ -356,6 +364,7 @@ int hipibs_hitmask_device(const hipibs_table *t, const uint64_t *d_seq_words,
 /* Canonical code of every position p in [pos0, pos0+n_pos) of a packed
  * stream: d_codes[p-pos0] = canonical code when the k bases at p are all
  * ACGT and p+k <= contig_end, else all ones; d_valid (nullable) gets 1/0.
+ * k <= 64; for k > 32 d_codes holds two words per position (high first).
  * The encode half of the fused kernel on its own, as
  * KmerCodec::encodePositions (src/KmerCodec.hpp:67-100). Asynchronous. */
 int hipibs_encode_device(const uint64_t *d_seq_words, const uint64_t *d_exc_words, uint64_t pos0,

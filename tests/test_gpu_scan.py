@@ -264,6 +264,32 @@ def test_encode_device_matches_oracle():
         assert np.all(codes[ov == 0] == np.uint64(2 ** 64 - 1))
 
 
+def test_encode_device_wide_matches_oracle():
+    """Two-word codes (k = 33..64): (high, low) per position."""
+    rng = np.random.default_rng(12)
+    for k in (33, 47, 63, 64):
+        c = rng.integers(0, 4, 20000).astype(np.uint8)
+        c[rng.random(20000) < 0.003] = 4
+        s = corpus.to_text(c, lower=bool(k % 2))
+        sw, ew, _ = hipibs.pack_sequences([s])
+        d_sw, d_ew = D.from_array(sw), D.from_array(ew)
+        np_ = len(s) - k + 1
+        d_codes = D.zeros(np_ * 16)
+        d_valid = D.zeros(np_)
+        hipibs._check(hipibs._lib.hipibs_encode_device(d_sw.ptr, d_ew.ptr, 0, np_, len(s), k,
+                                                       d_codes.ptr, d_valid.ptr, None), "encode")
+        hipibs.synchronize()
+        codes = d_codes.to_array(np.uint64, 2 * np_).reshape(np_, 2)
+        valid = d_valid.to_array(np.uint8, np_)
+        oc = np.zeros(2 * np_, dtype=np.uint64)
+        ov = np.zeros(np_, dtype=np.uint8)
+        oracle().orc_encode_positions_wide(s, len(s), k, oc.ctypes.data, ov.ctypes.data)
+        oc = oc.reshape(np_, 2)
+        assert np.array_equal(valid, ov) and ov.sum() > 1000
+        assert np.array_equal(codes[ov == 1], oc[ov == 1])
+        assert np.all(codes[ov == 0] == np.uint64(2 ** 64 - 1))
+
+
 def test_probe_bytes_matches_membership(small_corpus):
     _, samples = small_corpus
     keys = samples[0][0]
@@ -595,6 +621,27 @@ def test_wide_tables_match_reference_string_path(tmp_path, k, expected):
     assert t1.probe_count_kmc(p2) == inter
     t1.free()
     t2.free()
+
+
+def test_wide_device_build_stores_repeated_keys_once():
+    """Device keys of k > 32 may repeat, as for k <= 32: 100,000 copies of one
+    key among distinct ones are stored once, and quickly; rows whose high
+    word is all ones (the encoder's invalid mark) are skipped."""
+    import time
+    recs, keys = corpus.edge_config(47)
+    none = np.full((1000, 2), 2 ** 64 - 1, dtype=np.uint64)  # rows with no key: skipped
+    rep = np.ascontiguousarray(np.concatenate([np.repeat(keys[:1], 100_000, axis=0), keys,
+                                               keys[::5], none]))
+    rng = np.random.default_rng(5)
+    rep = np.ascontiguousarray(rep[rng.permutation(rep.shape[0])])
+    d = D.from_array(rep)
+    t0 = time.time()
+    t = hipibs.Table.from_device_keys(d.ptr, rep.shape[0], 47)
+    assert time.time() - t0 < 30
+    assert t.info()["n_keys"] == keys.shape[0]
+    assert t.probe_count(keys) == keys.shape[0]
+    t.free()
+    d.free()
 
 
 def test_wide_tables_refused_where_unsupported(tmp_path):
