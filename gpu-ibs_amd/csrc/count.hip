@@ -33,15 +33,31 @@ namespace hipibs
 namespace
 {
 
-constexpr int kCountSlots = 8;     // keys per bucket
-constexpr int kCountWords = 16;    // 64-bit words per bucket (keys, then counts)
+constexpr int kCountWords = 16;    // 64-bit words per 128 B bucket
 constexpr int kLanePositions = 32; // k-mer positions one lane rolls over
 constexpr int kCountBlock = 256;
 constexpr int kTilePositions = kCountBlock * kLanePositions;
-constexpr int kTileBytes = kTilePositions + 32; // + k - 1 <= 31 bases, padded
+constexpr int kTileBytes = kTilePositions + 64; // + k - 1 <= 63 bases
 constexpr double kMaxLoad = 0.7;                // grow above this
 constexpr double kGrowLoad = 0.5;               // load right after growing
 constexpr uint64_t kDefaultChunk = 128ull << 20;
+constexpr uint32_t kWideSpin = 1u << 14;        // reads of a low word being published
+
+// Bucket layout. k <= 32: 8 keys in words 0..7, their counts in 8..15.
+// k = 33..64: 5 keys as (high, low) in words 0..9, counts in 10..14, word
+// 15 unused.
+template <bool WIDE> struct Layout
+{
+    static constexpr int kSlots = 8;
+    static constexpr int kKeyWords = 8;
+    __host__ __device__ static int countWord(int s) { return 8 + s; }
+};
+template <> struct Layout<true>
+{
+    static constexpr int kSlots = 5;
+    static constexpr int kKeyWords = 10;
+    __host__ __device__ static int countWord(int s) { return 10 + s; }
+};
 
 // ACGT / acgt -> 0..3, any other byte 4 (the codec's base table,
 // src/KmerCodec.hpp:105-118). The low two bits of (b >> 1) ^ (b >> 2) are
@@ -68,16 +84,16 @@ __device__ inline bool countAdd(uint64_t *tab, uint64_t nBuckets, uint64_t key, 
     {
         unsigned long long *bucket = reinterpret_cast<unsigned long long *>(tab + b * kCountWords);
         const ulonglong2 *kv = reinterpret_cast<const ulonglong2 *>(bucket);
-        uint64_t keys[kCountSlots];
+        uint64_t keys[8];
 #pragma unroll
-        for (int q = 0; q < kCountSlots / 2; ++q)
+        for (int q = 0; q < 4; ++q)
         {
             const ulonglong2 v = kv[q];
             keys[2 * q] = v.x;
             keys[2 * q + 1] = v.y;
         }
 #pragma unroll
-        for (int s = 0; s < kCountSlots; ++s)
+        for (int s = 0; s < 8; ++s)
         {
             uint64_t cur = keys[s];
             if (cur == EMPTY)
@@ -88,8 +104,59 @@ __device__ inline bool countAdd(uint64_t *tab, uint64_t nBuckets, uint64_t key, 
             }
             if (cur == key)
             {
-                atomicAdd(bucket + kCountSlots + s, (unsigned long long)add);
+                atomicAdd(bucket + Layout<false>::countWord(s), (unsigned long long)add);
                 return true;
+            }
+        }
+        b = b + 1 == nBuckets ? 0 : b + 1;
+    }
+    return false;
+}
+
+// countAdd for k = 33..64. A slot is claimed by moving its high word from
+// EMPTY to the key's (CAS), and its low word is then published with a
+// device-scope store; a thread meeting its own high word reads the low
+// word, waiting (at most kWideSpin reads) while it still reads all ones,
+// and counts there if it is its own. The one canonical code whose low word
+// is all ones (k = 64: A^32 T^32) is counted by the caller outside the
+// table, so a published low word is never all ones.
+__device__ inline bool countAddWide(uint64_t *tab, uint64_t nBuckets, uint64_t hi, uint64_t lo,
+                                    uint64_t add, bool *isNew)
+{
+    uint64_t b = bucketOfWide(hi, lo, nBuckets);
+    *isNew = false;
+    for (uint64_t d = 0; d < nBuckets; ++d)
+    {
+        unsigned long long *bucket = reinterpret_cast<unsigned long long *>(tab + b * kCountWords);
+        for (int s = 0; s < Layout<true>::kSlots; ++s)
+        {
+            unsigned long long *hw = bucket + 2 * s, *lw = bucket + 2 * s + 1;
+            unsigned long long cur =
+                __hip_atomic_load(hw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == EMPTY)
+            {
+                const unsigned long long prev = atomicCAS(hw, EMPTY, hi);
+                if (prev == EMPTY)
+                {
+                    __hip_atomic_store(lw, (unsigned long long)lo, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    atomicAdd(bucket + Layout<true>::countWord(s), (unsigned long long)add);
+                    *isNew = true;
+                    return true;
+                }
+                cur = prev;
+            }
+            if (cur == hi)
+            {
+                unsigned long long l =
+                    __hip_atomic_load(lw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (uint32_t w = 0; l == EMPTY && w < kWideSpin; ++w)
+                    l = __hip_atomic_load(lw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (l == lo)
+                {
+                    atomicAdd(bucket + Layout<true>::countWord(s), (unsigned long long)add);
+                    return true;
+                }
             }
         }
         b = b + 1 == nBuckets ? 0 : b + 1;
@@ -106,7 +173,8 @@ __device__ inline uint32_t waveSum(uint32_t v)
 
 // Counts the k-mers at positions skip .. n-k of text (16 B aligned).
 // counters: [0] distinct keys stored, [1] k-mers counted, [2] 1 if an
-// insert found no slot.
+// insert found no slot, [6] count of the k = 64 code A^32 T^32.
+template <bool WIDE>
 __global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__restrict__ text,
                                                            uint64_t n, uint64_t skip, uint32_t k,
                                                            uint64_t *tab, uint64_t nBuckets,
@@ -115,8 +183,10 @@ __global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__rest
     __shared__ __align__(16) uint8_t tile[kTileBytes];
     const uint64_t nPos = n >= k ? n - k + 1 : 0;
     const uint64_t nTiles = (nPos + kTilePositions - 1) / kTilePositions;
-    const uint64_t mask = k == 32 ? ~0ull : (1ull << (2 * k)) - 1;
-    const uint32_t shift = 2 * (k - 1);
+    // k <= 32: one word; k > 32: the high word holds the first k - 32 bases
+    const uint64_t mask = WIDE ? (k == 64 ? ~0ull : (1ull << (2 * (k - 32))) - 1)
+                               : (k == 32 ? ~0ull : (1ull << (2 * k)) - 1);
+    const uint32_t shift = WIDE ? 2 * (k - 33) : 2 * (k - 1);
     uint32_t counted = 0, fresh = 0, full = 0;
     for (uint64_t t = blockIdx.x; t < nTiles; t += gridDim.x)
     {
@@ -137,7 +207,7 @@ __global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__rest
         // ends at byte p + k - 1; `run` counts the valid bytes ending at i.
         const uint32_t start = threadIdx.x * kLanePositions;
         const uint32_t end = min<uint32_t>(bytes, start + kLanePositions + k - 1);
-        uint64_t fwd = 0, rc = 0;
+        uint64_t fwd = 0, rc = 0, fhi = 0, rhi = 0;
         uint32_t run = 0;
         for (uint32_t i = start; i < end; ++i)
         {
@@ -148,12 +218,32 @@ __global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__rest
                 continue;
             }
             ++run;
-            fwd = ((fwd << 2) | c) & mask;
-            rc = (rc >> 2) | ((uint64_t)(3 - c) << shift);
+            if (WIDE)
+            {
+                fhi = ((fhi << 2) | (fwd >> 62)) & mask;
+                fwd = (fwd << 2) | c;
+                rc = (rc >> 2) | (rhi << 62);
+                rhi = (rhi >> 2) | ((uint64_t)(3 - c) << shift);
+            }
+            else
+            {
+                fwd = ((fwd << 2) | c) & mask;
+                rc = (rc >> 2) | ((uint64_t)(3 - c) << shift);
+            }
             if (run >= k && base + i + 1 >= skip + k)
             {
                 bool isNew;
-                full |= !countAdd(tab, nBuckets, min(fwd, rc), 1, &isNew);
+                if (WIDE)
+                {
+                    const bool f = fhi < rhi || (fhi == rhi && fwd < rc);
+                    const uint64_t h = f ? fhi : rhi, l = f ? fwd : rc;
+                    if (l == EMPTY) // only A^32 T^32 (k = 64)
+                        isNew = atomicAdd(counters + 6, 1ull) == 0;
+                    else
+                        full |= !countAddWide(tab, nBuckets, h, l, 1, &isNew);
+                }
+                else
+                    full |= !countAdd(tab, nBuckets, min(fwd, rc), 1, &isNew);
                 counted += 1;
                 fresh += isNew;
             }
@@ -173,51 +263,53 @@ __global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__rest
     }
 }
 
-__global__ __launch_bounds__(256) void countFillKernel(uint64_t *tab, uint64_t words)
+__global__ __launch_bounds__(256) void countFillKernel(uint64_t *tab, uint64_t words, int keyWords)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
          i += (uint64_t)gridDim.x * blockDim.x)
-        tab[i] = (i % kCountWords) < kCountSlots ? EMPTY : 0ull;
+        tab[i] = (int)(i % kCountWords) < keyWords ? EMPTY : 0ull;
 }
 
 // Moves every (key, count) of the old table into the new one. Keys are
 // distinct, so the thread that stores a key is the only writer of its
-// count: a plain store.
+// count: a plain store (the low word of a two-word key: a device-scope
+// store, as the counting insert publishes it).
+template <bool WIDE>
 __global__ __launch_bounds__(256) void countRehashKernel(const uint64_t *__restrict__ old,
                                                          uint64_t oldBuckets, uint64_t *tab,
                                                          uint64_t nBuckets,
                                                          unsigned long long *counters)
 {
-    const uint64_t slots = oldBuckets * kCountSlots;
+    using L = Layout<WIDE>;
+    const uint64_t slots = oldBuckets * L::kSlots;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots;
          i += (uint64_t)gridDim.x * blockDim.x)
     {
-        const uint64_t b = i / kCountSlots, s = i % kCountSlots;
-        const uint64_t key = old[b * kCountWords + s];
+        const uint64_t b = i / L::kSlots, s = i % L::kSlots;
+        const uint64_t *ob = old + b * kCountWords;
+        const uint64_t key = WIDE ? ob[2 * s] : ob[s];
         if (key == EMPTY)
             continue;
-        const uint64_t c = old[b * kCountWords + kCountSlots + s];
-        uint64_t h = bucketOf(key, nBuckets);
+        const uint64_t lo = WIDE ? ob[2 * s + 1] : 0;
+        const uint64_t c = ob[L::countWord((int)s)];
+        uint64_t h = WIDE ? bucketOfWide(key, lo, nBuckets) : bucketOf(key, nBuckets);
         bool stored = false;
         for (uint64_t d = 0; d < nBuckets && !stored; ++d)
         {
             unsigned long long *bucket =
                 reinterpret_cast<unsigned long long *>(tab + h * kCountWords);
-            const ulonglong2 *kv = reinterpret_cast<const ulonglong2 *>(bucket);
-            uint64_t keys[kCountSlots];
-#pragma unroll
-            for (int q = 0; q < kCountSlots / 2; ++q)
+            for (int q = 0; q < L::kSlots && !stored; ++q)
             {
-                const ulonglong2 v = kv[q];
-                keys[2 * q] = v.x;
-                keys[2 * q + 1] = v.y;
-            }
-            for (int q = 0; q < kCountSlots && !stored; ++q)
-                if (keys[q] == EMPTY && atomicCAS(bucket + q, EMPTY, key) == EMPTY)
+                unsigned long long *kw = bucket + (WIDE ? 2 * q : q);
+                if (*kw == EMPTY && atomicCAS(kw, EMPTY, key) == EMPTY)
                 {
-                    bucket[kCountSlots + q] = c;
+                    if (WIDE)
+                        __hip_atomic_store(kw + 1, (unsigned long long)lo, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    bucket[L::countWord(q)] = c;
                     stored = true;
                 }
+            }
             h = h + 1 == nBuckets ? 0 : h + 1;
         }
         if (!stored)
@@ -225,9 +317,11 @@ __global__ __launch_bounds__(256) void countRehashKernel(const uint64_t *__restr
     }
 }
 
-// Keys with lo <= count <= hi, in no particular order, with their counts
-// clamped to cmax (counts may be null). filt: [0] kept, [1] below lo,
-// [2] above hi. A wave claims its output range with one atomic add.
+// Keys with lo <= count <= hi, in no particular order (two words each when
+// WIDE), with their counts clamped to cmax (counts may be null). filt: [0]
+// kept, [1] below lo, [2] above hi. A wave claims its output range with
+// one atomic add.
+template <bool WIDE>
 __global__ __launch_bounds__(256) void countExtractKernel(const uint64_t *__restrict__ tab,
                                                           uint64_t nBuckets, uint64_t lo,
                                                           uint64_t hi, uint64_t cmax,
@@ -235,18 +329,21 @@ __global__ __launch_bounds__(256) void countExtractKernel(const uint64_t *__rest
                                                           uint64_t cap,
                                                           unsigned long long *filt)
 {
-    const uint64_t slots = nBuckets * kCountSlots;
+    using L = Layout<WIDE>;
+    const uint64_t slots = nBuckets * L::kSlots;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint32_t below = 0, above = 0;
     for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x; w < slots; w += stride)
     {
         const uint64_t i = w + threadIdx.x;
-        uint64_t key = EMPTY, c = 0;
+        uint64_t key = EMPTY, low = 0, c = 0;
         if (i < slots)
         {
-            const uint64_t b = i / kCountSlots, s = i % kCountSlots;
-            key = tab[b * kCountWords + s];
-            c = tab[b * kCountWords + kCountSlots + s];
+            const uint64_t b = i / L::kSlots, s = i % L::kSlots;
+            const uint64_t *tb = tab + b * kCountWords;
+            key = WIDE ? tb[2 * s] : tb[s];
+            low = WIDE ? tb[2 * s + 1] : 0;
+            c = tb[L::countWord((int)s)];
         }
         const bool present = key != EMPTY;
         const bool keep = present && c >= lo && c <= hi;
@@ -264,7 +361,13 @@ __global__ __launch_bounds__(256) void countExtractKernel(const uint64_t *__rest
             const uint64_t o = at + maskedRank(m);
             if (o < cap)
             {
-                keys[o] = key;
+                if (WIDE)
+                {
+                    keys[2 * o] = key;
+                    keys[2 * o + 1] = low;
+                }
+                else
+                    keys[o] = key;
                 if (counts)
                     counts[o] = (uint32_t)min(c, cmax);
             }
@@ -303,11 +406,13 @@ struct hipibs_counter : ReadSink
 {
     int device = 0;
     uint32_t k = 0;
+    bool wide = false; // k > 32: two-word keys, 5 per bucket
     uint64_t chunkBytes = 0;
     hipStream_t stream = nullptr;
     uint64_t *dTab = nullptr;
     uint64_t nBuckets = 0;
-    unsigned long long *dCounters = nullptr; // [0] distinct, [1] counted, [2] full, [3..5] filter
+    unsigned long long *dCounters = nullptr; // [0] distinct, [1] counted, [2] full, [3..5] filter,
+                                             // [6] count of A^32 T^32 (k = 64)
     unsigned long long *hCounters = nullptr; // pinned copy
     char *hText[2] = {nullptr, nullptr};     // pinned chunks (host text only)
     uint8_t *dText = nullptr;
@@ -389,8 +494,12 @@ struct hipibs_counter : ReadSink
     int launchCount(const uint8_t *text, uint64_t n, uint64_t skip)
     {
         HIPIBS_CHECK(hipEventRecord(ev[0], stream));
-        countKernel<<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0, stream>>>(
-            text, n, skip, k, dTab, nBuckets, dCounters);
+        if (wide)
+            countKernel<true><<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0,
+                                stream>>>(text, n, skip, k, dTab, nBuckets, dCounters);
+        else
+            countKernel<false><<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0,
+                                 stream>>>(text, n, skip, k, dTab, nBuckets, dCounters);
         HIPIBS_CHECK(hipGetLastError());
         HIPIBS_CHECK(hipEventRecord(ev[1], stream));
         evPending = true;
@@ -447,9 +556,10 @@ struct hipibs_counter : ReadSink
     // at least doubling (called with the stream idle).
     int reserve(uint64_t need)
     {
-        if ((double)need <= kMaxLoad * (double)(nBuckets * kCountSlots))
+        const int slots = wide ? Layout<true>::kSlots : Layout<false>::kSlots;
+        if ((double)need <= kMaxLoad * (double)(nBuckets * slots))
             return 0;
-        uint64_t want = (uint64_t)((double)need / kGrowLoad) / kCountSlots + 1;
+        uint64_t want = (uint64_t)((double)need / kGrowLoad) / slots + 1;
         want = std::max<uint64_t>(want, 2 * nBuckets);
         want = std::max<uint64_t>(want, 1024);
         if (want >= (1ull << 32))
@@ -464,12 +574,17 @@ struct hipibs_counter : ReadSink
                                 " buckets (" + std::to_string(distinct) + " distinct k-mers so far)");
         }
         const uint64_t words = want * kCountWords;
-        countFillKernel<<<gridFor(words, 256 * 4, device), 256, 0, stream>>>(t, words);
+        countFillKernel<<<gridFor(words, 256 * 4, device), 256, 0, stream>>>(
+            t, words, wide ? Layout<true>::kKeyWords : Layout<false>::kKeyWords);
         HIPIBS_CHECK(hipGetLastError());
         if (dTab)
         {
-            countRehashKernel<<<gridFor(nBuckets * kCountSlots, 256 * 4, device), 256, 0, stream>>>(
-                dTab, nBuckets, t, want, dCounters);
+            if (wide)
+                countRehashKernel<true><<<gridFor(nBuckets * slots, 256 * 4, device), 256, 0,
+                                          stream>>>(dTab, nBuckets, t, want, dCounters);
+            else
+                countRehashKernel<false><<<gridFor(nBuckets * slots, 256 * 4, device), 256, 0,
+                                           stream>>>(dTab, nBuckets, t, want, dCounters);
             HIPIBS_CHECK(hipGetLastError());
             HIPIBS_CHECK(hipStreamSynchronize(stream));
             HIPIBS_CHECK(hipFree(dTab));
@@ -508,25 +623,52 @@ struct hipibs_counter : ReadSink
         return err = sync();
     }
 
-    // Keys with lo <= count <= hi into dKeys (and clamped counts into dCounts
-    // if non-null), at most cap; filt receives kept / below / above.
+    // Keys with lo <= count <= hi into dKeys (two words each for k > 32;
+    // and clamped counts into dCounts if non-null), at most cap; filt
+    // receives kept / below / above.
     int extract(uint64_t lo, uint64_t hi, uint64_t cmax, uint64_t *dKeys, uint32_t *dCounts,
                 uint64_t cap, hipibs_count_filter *filt)
     {
         HIPIBS_CHECK(hipMemsetAsync(dCounters + 3, 0, 3 * sizeof(unsigned long long), stream));
         if (nBuckets)
         {
-            countExtractKernel<<<gridFor(nBuckets * kCountSlots, 256 * 8, device), 256, 0,
-                                 stream>>>(dTab, nBuckets, lo, hi, cmax, dKeys, dCounts, cap,
-                                           dCounters + 3);
+            const uint64_t slots =
+                nBuckets * (wide ? Layout<true>::kSlots : Layout<false>::kSlots);
+            if (wide)
+                countExtractKernel<true><<<gridFor(slots, 256 * 8, device), 256, 0, stream>>>(
+                    dTab, nBuckets, lo, hi, cmax, dKeys, dCounts, cap, dCounters + 3);
+            else
+                countExtractKernel<false><<<gridFor(slots, 256 * 8, device), 256, 0, stream>>>(
+                    dTab, nBuckets, lo, hi, cmax, dKeys, dCounts, cap, dCounters + 3);
             HIPIBS_CHECK(hipGetLastError());
         }
-        HIPIBS_CHECK(hipMemcpyAsync(hCounters + 3, dCounters + 3, 3 * sizeof(unsigned long long),
+        HIPIBS_CHECK(hipMemcpyAsync(hCounters + 3, dCounters + 3, 4 * sizeof(unsigned long long),
                                     hipMemcpyDeviceToHost, stream));
         HIPIBS_CHECK(hipStreamSynchronize(stream));
         filt->kept = hCounters[3];
         filt->below = hCounters[4];
         filt->above = hCounters[5];
+        // the k = 64 code A^32 T^32, counted outside the table
+        const uint64_t sc = hCounters[6];
+        if (sc)
+        {
+            if (sc < lo)
+                filt->below += 1;
+            else if (sc > hi)
+                filt->above += 1;
+            else
+            {
+                const uint64_t o = filt->kept++;
+                if (o < cap)
+                {
+                    const uint64_t key[2] = {0, EMPTY};
+                    const uint32_t c = (uint32_t)std::min<uint64_t>(sc, cmax);
+                    HIPIBS_CHECK(hipMemcpy(dKeys + 2 * o, key, 16, hipMemcpyHostToDevice));
+                    if (dCounts)
+                        HIPIBS_CHECK(hipMemcpy(dCounts + o, &c, 4, hipMemcpyHostToDevice));
+                }
+            }
+        }
         return 0;
     }
 };
@@ -546,8 +688,8 @@ extern "C" int hipibs_counter_create(uint32_t k, int device, uint64_t expected_k
     if (!out)
         return setError(HIPIBS_E_ARG, "null argument");
     *out = nullptr;
-    if (k < 1 || k > 32)
-        return setError(HIPIBS_E_ARG, "k-mer counter: k must be 1..32");
+    if (k < 1 || k > 64)
+        return setError(HIPIBS_E_ARG, "k-mer counter: k must be 1..64");
     if (chunk_bytes == 0)
         chunk_bytes = kDefaultChunk;
     if (chunk_bytes < 64 || chunk_bytes > (1ull << 32))
@@ -564,6 +706,7 @@ extern "C" int hipibs_counter_create(uint32_t k, int device, uint64_t expected_k
     auto *c = new hipibs_counter();
     c->device = device;
     c->k = k;
+    c->wide = k > 32;
     c->chunkBytes = chunk_bytes;
     auto fail = [c](int rc)
     {
@@ -675,7 +818,7 @@ extern "C" int hipibs_counter_keys(hipibs_counter *c, uint32_t min_count, uint32
     hipError_t e = hipSuccess;
     if (m)
     {
-        e = hipMalloc(reinterpret_cast<void **>(&dKeys), m * 8);
+        e = hipMalloc(reinterpret_cast<void **>(&dKeys), m * 8 * (c->wide ? 2 : 1));
         if (e == hipSuccess && h_counts)
             e = hipMalloc(reinterpret_cast<void **>(&dCounts), m * 4);
     }
@@ -684,7 +827,7 @@ extern "C" int hipibs_counter_keys(hipibs_counter *c, uint32_t min_count, uint32
     const uint64_t got = std::min<uint64_t>(m, filt->kept);
     if (e == hipSuccess && !rc && got)
     {
-        e = hipMemcpy(h_keys, dKeys, got * 8, hipMemcpyDeviceToHost);
+        e = hipMemcpy(h_keys, dKeys, got * 8 * (c->wide ? 2 : 1), hipMemcpyDeviceToHost);
         if (e == hipSuccess && h_counts)
             e = hipMemcpy(h_counts, dCounts, got * 4, hipMemcpyDeviceToHost);
     }
@@ -720,7 +863,8 @@ extern "C" int hipibs_counter_build_table(hipibs_counter *c, uint32_t min_count,
     hipibs_count_filter f{};
     uint64_t *dKeys = nullptr;
     if (c->distinct)
-        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dKeys), c->distinct * 8));
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dKeys),
+                               c->distinct * 8 * (c->wide ? 2 : 1)));
     rc = c->extract(min_count, max_count, 1, dKeys, nullptr, c->distinct, &f);
     if (!rc)
         rc = hipibs_table_build_keys_device(dKeys, f.kept, c->k, &o, out, c->stream);
@@ -772,11 +916,11 @@ extern "C" int hipibs_counter_write_kmc(hipibs_counter *c, const char *prefix, u
     HIPIBS_CHECK(hipSetDevice(c->device));
     if ((rc = c->finish()))
         return rc;
-    std::vector<uint64_t> keys(c->distinct);
+    std::vector<uint64_t> keys(c->distinct * (c->wide ? 2 : 1));
     std::vector<uint32_t> counts(c->distinct);
     hipibs_count_filter f{};
     rc = hipibs_counter_keys(c, min_count, max_count, counter_max, keys.data(), counts.data(),
-                             keys.size(), &f);
+                             c->distinct, &f);
     if (rc)
         return rc;
     if (filt)

@@ -682,9 +682,10 @@ class KmerCounter:
         return {n: getattr(st, n) for n, _ in CountStats._fields_}
 
     def keys(self, min_count: int = 1, max_count: int = 0xFFFFFFFF, counter_max: int = 255):
-        """(keys uint64, counts uint32, filter dict), in no particular order."""
+        """(keys, counts uint32, filter dict), in no particular order; keys
+        uint64 (k <= 32) or (n, 2) high/low words."""
         n = self.stats()["distinct"]
-        keys = np.empty(n, dtype=np.uint64)
+        keys = np.empty((n, 2) if self.k > 32 else n, dtype=np.uint64)
         counts = np.empty(n, dtype=np.uint32)
         f = CountFilter()
         _check(_lib.hipibs_counter_keys(self.handle, min_count, max_count, counter_max,

@@ -7,7 +7,7 @@
 // FASTQ reads needs:
 //
 //   kmc [options] <input file | @list of input files> <output prefix> <work dir>
-//     -k<len>   k-mer length, 1..32 (default 25)
+//     -k<len>   k-mer length, 1..64 (default 25)
 //     -ci<n>    drop k-mers seen fewer than n times (default 2)
 //     -cx<n>    drop k-mers seen more than n times (default 1e9)
 //     -cs<n>    largest stored count (default 255)
@@ -34,7 +34,7 @@ int usage()
 {
     std::cerr << "usage: kmc [options] <input file | @file with input names> <output prefix> "
                  "<working directory>\n"
-                 "  -k<len> (1..32, default 25)  -ci<n> (default 2)  -cx<n> (default 1e9)\n"
+                 "  -k<len> (1..64, default 25)  -ci<n> (default 2)  -cx<n> (default 1e9)\n"
                  "  -cs<n> (default 255)  -fq/-fa/-fm  -j<json file>\n";
     return 1;
 }
@@ -76,9 +76,9 @@ int main(int argc, char **argv)
         {
             if (!number(a.substr(2), v) || v < 1)
                 return usage();
-            if (v > 32)
+            if (v > 64)
             {
-                std::cerr << "kmc: k = " << v << " is not supported (k <= 32)\n";
+                std::cerr << "kmc: k = " << v << " is not supported (k <= 64)\n";
                 return 1;
             }
             k = (uint32_t)v;

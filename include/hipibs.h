@@ -16,8 +16,8 @@
  *    holds 2n words for n keys when the table's (or database's) k is above
  *    32. Tables of k > 32 have 128 B buckets of 8 keys, and their builds
  *    skip keys whose high word is all ones (hipibs_encode_device's mark of
- *    an invalid position); panel launches, minimizer tables, serving and the
- *    k-mer counter take k <= 32.
+ *    an invalid position); panel launches, minimizer tables and serving take
+ *    k <= 32.
  *  - packed bases: base b of a stream lives at bits 2*(b%32) of 64-bit word
  *    b/32 (code of a non-ACGT base is 0); its exception bit (set when the
  *    byte is not one of ACGTacgt) is bit b%64 of word b/64 of a separate
@@ -239,7 +239,8 @@ typedef struct hipibs_count_filter {
     uint64_t above; /* count > max_count */
 } hipibs_count_filter;
 
-/* k in 1..32. expected_keys (0 = unknown) sizes the first table; the table
+/* k in 1..64 (for k > 32, key arrays hold two words per key, high first).
+ * expected_keys (0 = unknown) sizes the first table; the table
  * grows as needed. chunk_bytes (0 = 128 MiB) is the text moved to the device
  * at a time: two pinned host buffers and one device buffer of that size,
  * allocated when host text first arrives. */

@@ -226,6 +226,13 @@ def small_config(seed: int = SMALL_SEED, n: int = 1_000_000, k: int = 25):
 def edge_config(k: int, seed: int = 0x1B5E000):
     """Edge-case reference contigs and a sample built from a mutated copy
     (1% SNPs plus three dropout blocks), every k-mer of the copy kept."""
+    records, sample_seqs = edge_sequences(k, seed)
+    keys = sequence_kmers(sample_seqs, k) if k <= 32 else sequence_kmers_wide(sample_seqs, k)
+    return records, keys
+
+
+def edge_sequences(k: int, seed: int = 0x1B5E000):
+    """edge_config's reference records and the sample's sequences (codes)."""
     def with_n_runs(a):
         a = a.copy()
         for st, ln in ((1000, 1), (5000, k - 1), (9000, k), (13000, 3 * k)):
@@ -253,6 +260,5 @@ def edge_config(k: int, seed: int = 0x1B5E000):
             for a, b in ((20_000, 25_000), (100_000, 102_000), (180_000, 180_100)):
                 keep[a:b] = False
         sample_seqs.append(s[keep])
-    keys = sequence_kmers(sample_seqs, k) if k <= 32 else sequence_kmers_wide(sample_seqs, k)
     records = [(name, to_text(codes, lower)) for name, codes, lower in contigs]
-    return records, keys
+    return records, sample_seqs

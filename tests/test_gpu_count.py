@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import corpus
-from util import ROOT
+from util import ROOT, sha256
 
 import hipibs
 
@@ -241,7 +241,7 @@ def test_kmc_command(tmp_path, small_reads):
 
 
 def test_counter_errors(tmp_path):
-    for k in (0, 33):
+    for k in (0, 65):
         with pytest.raises(hipibs.HipibsError):
             hipibs.KmerCounter(k)
     c = hipibs.KmerCounter(25)
@@ -260,3 +260,106 @@ def test_counter_errors(tmp_path):
     with pytest.raises(hipibs.HipibsError):
         c.keys(5, 2, 255)
     c.free()
+
+
+def wide_counts(records, k):
+    """(keys (n, 2), counts, k-mers) of records of any lengths, k = 33..64."""
+    rows = []
+    for r in records:
+        if r.size >= k:
+            h, l, v = corpus.canonical_codes_wide(r[None, :], k)
+            rows.append(np.stack([h[v], l[v]], axis=1))
+    allr = np.concatenate(rows) if rows else np.zeros((0, 2), np.uint64)
+    keys, counts = np.unique(allr, axis=0, return_counts=True)
+    return keys, counts.astype(np.uint32), allr.shape[0]
+
+
+def sorted_rows(keys, counts):
+    o = np.lexsort((keys[:, 1], keys[:, 0]))
+    return keys[o], counts[o]
+
+
+@pytest.mark.parametrize("k", [33, 47, 64])
+def test_wide_counts_match_oracle(tmp_path, k):
+    """k = 33..64: FASTQ reads of a small genome (so k-mers repeat) cut into
+    64 B pieces with many table growths; at k = 64 the one code whose low
+    word is all ones (A^32 T^32) is among them."""
+    rng = np.random.default_rng(k)
+    genome = rng.integers(0, 4, 20000).astype(np.uint8)
+    reads = []
+    for i in range(1500):
+        ln = int(rng.integers(40, 150))
+        st = int(rng.integers(0, genome.size - ln))
+        r = genome[st:st + ln].copy()
+        if i % 2:
+            r = (3 - r)[::-1].copy()
+        if i % 9 == 0:
+            r[int(rng.integers(0, ln))] = 4
+        reads.append(r)
+    pal = np.array([0] * 32 + [3] * 32, dtype=np.uint8)  # A^32 T^32, its own reverse complement
+    for j in range(3):
+        reads.append(np.concatenate([genome[:10], pal, genome[j:j + 5]]))
+    path = tmp_path / "r.fq"
+    path.write_bytes(b"".join(b"@r%d\n%s\n+\n%s\n" % (i, corpus.to_text(r), b"I" * r.size)
+                              for i, r in enumerate(reads)))
+    want_k, want_c, total = wide_counts(reads, k)
+    note(f"wide count k={k}")
+    c = hipibs.KmerCounter(k, chunk_bytes=64)
+    c.add_file(str(path))
+    st = c.stats()
+    assert (st["kmers"], st["distinct"]) == (total, want_k.shape[0])
+    keys, counts, _ = c.keys(1, ALL, ALL)
+    keys, counts = sorted_rows(keys, counts)
+    np.testing.assert_array_equal(keys, want_k)
+    np.testing.assert_array_equal(counts, want_c)
+    if k == 64:
+        i = np.flatnonzero((want_k[:, 0] == 0) & (want_k[:, 1] == np.uint64(2 ** 64 - 1)))
+        assert i.size == 1 and want_c[i[0]] >= 3
+    f = c.keys(2, 5, 3)[2]
+    assert f["kept"] == int(((want_c >= 2) & (want_c <= 5)).sum())
+    c.free()
+
+
+def test_wide_counter_to_table_matches_reference_string_path(tmp_path, expected):
+    """k = 47: the edge corpus's sample sequences counted (-ci1) give the
+    golden key set; their table, built directly and through the KMC database
+    the counter writes, scans to the reference string path's TSV."""
+    k = 47
+    recs, sample = corpus.edge_sequences(k)
+    _, keys = corpus.edge_config(k)
+    path = tmp_path / "sample.fa"
+    path.write_bytes(corpus.fasta([(b"s%d" % i, corpus.to_text(s)) for i, s in enumerate(sample)]))
+    c = hipibs.KmerCounter(k)
+    c.add_file(str(path))
+    got, _, _ = c.keys(1, ALL, ALL)
+    got = got[np.lexsort((got[:, 1], got[:, 0]))]
+    np.testing.assert_array_equal(got, keys)
+    t = c.build_table(1, ALL)
+    prefix = str(tmp_path / "db")
+    c.write_kmc(prefix, 1, 1_000_000_000, 255)
+    c.free()
+    t2 = hipibs.Table.from_kmc(prefix)
+    want = expected["edge_wide"][str(k)]["tsv"]["1000"]
+    ids = [r[0] for r in recs]
+    seqs = [r[1] for r in recs]
+    for tab in (t, t2):
+        tsv = hipibs.format_tsv(hipibs.scan_sequences([tab], ids, seqs, 1000)[0])
+        assert (sha256(tsv), len(tsv)) == (want["sha256"], want["bytes"])
+        tab.free()
+
+
+def test_kmc_command_k35(tmp_path):
+    """`kmc -k35 -ci1 -fm` as the reference's test script runs it for its
+    k > 32 corpus (scripts/run_tests.sh:70)."""
+    recs, sample = corpus.edge_sequences(35)
+    fa = tmp_path / "sample_reads.fasta"
+    fa.write_bytes(corpus.fasta([(b"s%d" % i, corpus.to_text(s)) for i, s in enumerate(sample)]))
+    (tmp_path / "tmp").mkdir()
+    out = str(tmp_path / "sample")
+    r = subprocess.run([os.path.join(BIN, "kmc"), "-k35", "-ci1", "-fm", "-t4", str(fa), out,
+                        str(tmp_path / "tmp")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    _, keys = corpus.edge_config(35)
+    back = hipibs.kmc_read_keys(out)
+    np.testing.assert_array_equal(np.unique(back, axis=0), keys)
+    assert hipibs.kmc_header(out).kmer_length == 35
