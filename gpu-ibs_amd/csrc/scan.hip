@@ -454,7 +454,11 @@ int launchScanMixed(int bucketSlots, uint32_t k, const TableView *dViews, const 
     if (nWindows == 0)
         return 0;
     const dim3 grid(nWindows, 1);
-    if (bucketSlots == 8)
+    if (k > 32)
+        hipLaunchKernelGGL((scanKernel<16, true, true>), grid, dim3(SCAN_BLOCK), 0, s, dViews,
+                           dSeq, dExc, nBases, dWindows, nWindows, 1u, k, dStats,
+                           (SegSummary *)nullptr);
+    else if (bucketSlots == 8)
         hipLaunchKernelGGL((scanKernel<8, true>), grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc,
                            nBases, dWindows, nWindows, 1u, k, dStats, (SegSummary *)nullptr);
     else

@@ -173,7 +173,9 @@ uint64_t tableBytesFor(const hipibs_serve *s, uint64_t n)
 {
     const double lf = s->topts.load_factor > 0 ? s->topts.load_factor : 0.6;
     const uint32_t slots = s->topts.bucket_slots ? s->topts.bucket_slots : 16;
-    const uint64_t buckets = std::max<uint64_t>(2, (uint64_t)std::ceil(std::ceil((double)std::max<uint64_t>(n, 1) / lf) / slots));
+    const uint32_t keysPerBucket = s->k > 32 ? 8 : slots; // k > 32: 8 keys of 16 B
+    const uint64_t buckets = std::max<uint64_t>(
+        2, (uint64_t)std::ceil(std::ceil((double)std::max<uint64_t>(n, 1) / lf) / keysPerBucket));
     return buckets * slots * 8;
 }
 
@@ -277,7 +279,7 @@ void builderLoop(hipibs_serve *s)
         {
             if (!sm.kmcPrefix.empty())
                 rc = hipibs_table_build_kmc(sm.kmcPrefix.c_str(), &o, &t);
-            else if (o.minimizer_len)
+            else if (o.minimizer_len || s->k > 32)
                 rc = hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &t);
             else
                 rc = buildTableFromHost(sm.keys, sm.n, s->k, &o, staging, &t);
@@ -551,7 +553,7 @@ extern "C" int hipibs_serve_create(int device, uint32_t k, uint64_t hbm_budget,
                                    const hipibs_table_opts *topts, int max_inflight,
                                    uint32_t max_windows_per_launch, hipibs_serve **out)
 {
-    if (!out || k == 0 || k > 32 || max_inflight < 1 || max_inflight > 16 ||
+    if (!out || k == 0 || k > 64 || max_inflight < 1 || max_inflight > 16 ||
         max_windows_per_launch == 0 || hbm_budget == 0)
         return setError(HIPIBS_E_ARG, "bad serving engine arguments");
     *out = nullptr;

@@ -735,6 +735,7 @@ class ServeEngine:
         _check(_lib.hipibs_serve_create(device, k, int(hbm_budget), C.byref(o), max_inflight,
                                         max_windows_per_launch, C.byref(h)), "hipibs_serve_create")
         self.handle = h
+        self.k = k
         self._keep = []
 
     def set_reference(self, seq_words: np.ndarray, exc_words: np.ndarray, n_bases: int) -> None:
@@ -743,10 +744,12 @@ class ServeEngine:
                                                n_bases), "hipibs_serve_set_reference")
 
     def add_sample(self, sample_id: int, keys: np.ndarray, preload: bool = False) -> None:
+        """keys: uint64 codes (k <= 32) or an (n, 2) array of (high, low)."""
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         self._keep.append(keys)
-        _check(_lib.hipibs_serve_add_sample(self.handle, sample_id, _ptr(keys), keys.size,
-                                            int(preload)), "hipibs_serve_add_sample")
+        _check(_lib.hipibs_serve_add_sample(self.handle, sample_id, _ptr(keys),
+                                            _key_count(keys, self.k), int(preload)),
+               "hipibs_serve_add_sample")
 
     def add_sample_kmc(self, sample_id: int, kmc_prefix: str, preload: bool = False) -> None:
         _check(_lib.hipibs_serve_add_sample_kmc(self.handle, sample_id, kmc_prefix.encode(),

@@ -16,8 +16,8 @@
  *    holds 2n words for n keys when the table's (or database's) k is above
  *    32. Tables of k > 32 have 128 B buckets of 8 keys, and their builds
  *    skip keys whose high word is all ones (hipibs_encode_device's mark of
- *    an invalid position); panel launches, minimizer tables and serving take
- *    k <= 32.
+ *    an invalid position); panel-kernel launches (hipibs_panel_*) and
+ *    minimizer tables take k <= 32.
  *  - packed bases: base b of a stream lives at bits 2*(b%32) of 64-bit word
  *    b/32 (code of a non-ACGT base is 0); its exception bit (set when the
  *    byte is not one of ACGTacgt) is bit b%64 of word b/64 of a separate

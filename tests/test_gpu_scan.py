@@ -7,8 +7,8 @@ import numpy as np
 import pytest
 
 import corpus
-from util import (ROOT, coverage_line, oracle, oracle_scan_rows, sha256, sorted_keys,
-                  sorted_keys_wide)
+from util import (ROOT, coverage_line, oracle, oracle_scan_rows, oracle_scan_rows_wide, sha256,
+                  sorted_keys, sorted_keys_wide)
 
 import hipibs
 
@@ -647,6 +647,53 @@ def test_wide_device_build_stores_repeated_keys_once():
     assert t.probe_count(keys) == keys.shape[0]
     t.free()
     d.free()
+
+
+def test_wide_serving_equals_single_scans(tmp_path, expected):
+    """Serving at k = 47: samples from host keys and from a KMC database,
+    a budget of one table (evictions, re-admissions), mixed windows; every
+    request equals the reference string path's rows for its range."""
+    k = 47
+    recs, keys = corpus.edge_config(k)
+    seq = recs[0][1]  # chr1, 240 kbp
+    sw, ew, _ = hipibs.pack_sequences([seq])
+    n = len(seq)
+    samples = [keys, keys[::2], keys[1::3]]
+    kmc = str(tmp_path / "s2")
+    hipibs.kmc_write(kmc, samples[2], k, sig_len=9, counter_size=1, version=0x200,
+                     min_count=1, max_count=255)
+    one = hipibs.Table.from_keys(samples[0], k)
+    budget = int(1.5 * one.info()["bytes"])
+    one.free()
+    eng = hipibs.ServeEngine(k, budget, max_inflight=3, max_windows_per_launch=512)
+    eng.set_reference(sw, ew, n)
+    eng.add_sample(0, samples[0])
+    eng.add_sample(1, samples[1])
+    eng.add_sample_kmc(2, kmc)
+    rng = np.random.default_rng(9)
+    reqs = []
+    for i in range(24):
+        ln = int(rng.integers(100, n // 2))
+        reqs.append((i, i % 3, int(rng.integers(0, n - ln)), ln, int(rng.choice([1000, 5000]))))
+    out = {}
+    for phase in (reqs[:12], reqs[12:]):
+        for r in phase:
+            eng.submit(*r)
+        pending = {r[0] for r in phase}
+        while pending:
+            for rid in eng.poll(timeout_ms=2000):
+                out[rid] = eng.fetch(rid)[0]
+                pending.discard(rid)
+    c = eng.counters()
+    eng.free()
+    # at most two of the three tables fit together, and the second phase asks
+    # for all three again
+    assert c["evictions"] >= 1 and c["admissions"] >= 4
+    for rid, sid, base, ln, W in reqs:
+        want = oracle_scan_rows_wide([b"c"], [seq[base:base + ln]], W, k,
+                                     sorted_keys_wide(samples[sid]))
+        got = [tuple(int(x) for x in row) for row in out[rid]]
+        assert got == [w[3:] for w in want], rid
 
 
 def test_wide_tables_refused_where_unsupported(tmp_path):
