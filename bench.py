@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--keys", type=float, default=2.0e9, help="target distinct keys")
     ap.add_argument("--bucket-slots", type=int, default=0, help="0 = library default")
     ap.add_argument("--minimizer-len", type=int, default=0,
-                    help="0 = whole-k-mer hash buckets (default), 5..17 = minimizer buckets")
+                    help="0 = whole-k-mer hash buckets (default), 5..min(31, k) = minimizer buckets")
     ap.add_argument("--minimizer-cap", type=int, default=0,
                     help="minimizer tables: keys per minimizer group before it moves to the "
                          "whole-k-mer part (0 = library default)")

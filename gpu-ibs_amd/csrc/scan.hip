@@ -36,7 +36,9 @@ constexpr int SCAN_WAVES = SCAN_BLOCK / 64;
 // result goes to row w; otherwise grid.y is the table and results are
 // table-major. WIDE: k = 33..64, two-word codes (wideCanonical) probed in a
 // table of 16 B keys (waveProbeWide); a fourth sequence word is read.
-template <int S, bool MIXED, bool WIDE = false>
+// GROUPED: every table is a minimizer table, probed with waveProbeGrouped
+// (each run of positions sharing a minimizer loads its home line once).
+template <int S, bool MIXED, bool WIDE = false, bool GROUPED = false>
 __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
     const TableView *__restrict__ tables, const uint64_t *__restrict__ seq,
     const uint64_t *__restrict__ exc, uint64_t nBases, const hipibs_window *__restrict__ windows,
@@ -44,6 +46,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
     SegSummary *__restrict__ work)
 {
     __shared__ SegSummary waveSum[SCAN_WAVES];
+    __shared__ __align__(16) uint64_t groupStage[GROUPED ? SCAN_WAVES : 1]
+                                                [GROUPED ? GroupStage<S>::kWords : 2];
     const uint32_t unit = blockIdx.x;
     const uint32_t w = unit / segs, s = unit - w * segs;
     const hipibs_window win = windows[w];
@@ -101,7 +105,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
                 H = waveProbeWide(T, hi, lo,
                                   valid ? (uint32_t)bucketOfWide(hi, lo, T.nBuckets) : 0u, valid);
             }
-            else if (mlen)
+            else if (GROUPED || mlen)
             {
                 // canonical m-mer hashes at positions p and p+64, then the
                 // minimum over each k-mer's window of `win` m-mers by a
@@ -130,7 +134,10 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
                 const int src = (lane + d) & 63;
                 const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
                 const uint32_t minh = min(L, lane + d < 64 ? l2 : h2);
-                H = waveProbeMinimizer<S>(T, canon, minh, valid);
+                if constexpr (GROUPED)
+                    H = waveProbeGrouped<S>(T, canon, minh, valid, groupStage[wv]);
+                else
+                    H = waveProbeMinimizer<S>(T, canon, minh, valid);
             }
             else
                 H = waveProbe<S>(T, canon, (uint32_t)bucketOf(canon, T.nBuckets), valid);
@@ -423,6 +430,12 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
     if (k > 32)
         hipLaunchKernelGGL((scanKernel<16, false, true>), grid, dim3(SCAN_BLOCK), 0, s, dViews,
                            dSeq, dExc, nBases, dWindows, nWindows, segs, k, dStats, work);
+    else if (tables[0]->mlen && tables[0]->bucketSlots == 8)
+        hipLaunchKernelGGL((scanKernel<8, false, false, true>), grid, dim3(SCAN_BLOCK), 0, s,
+                           dViews, dSeq, dExc, nBases, dWindows, nWindows, segs, k, dStats, work);
+    else if (tables[0]->mlen)
+        hipLaunchKernelGGL((scanKernel<16, false, false, true>), grid, dim3(SCAN_BLOCK), 0, s,
+                           dViews, dSeq, dExc, nBases, dWindows, nWindows, segs, k, dStats, work);
     else if (tables[0]->bucketSlots == 8)
         hipLaunchKernelGGL((scanKernel<8, false>), grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc,
                            nBases, dWindows, nWindows, segs, k, dStats, work);

@@ -544,8 +544,8 @@ static int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts,
     if (wide && o.bucket_slots == 8)
         return setError(HIPIBS_E_ARG, "tables of k > 32 have 128 B buckets (bucket_slots 16)");
     const uint32_t mlen = o.minimizer_len;
-    if (mlen && (mlen < 5 || mlen > 17 || mlen > k))
-        return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (off) or 5..min(17, k)");
+    if (mlen && (mlen < 5 || mlen > 31 || mlen > k))
+        return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (off) or 5..min(31, k)");
     if (o.bucket_slots == 0)
         o.bucket_slots = 16; // one 128 B bucket = one memory request on gfx950
     if (o.bucket_slots != 8 && o.bucket_slots != 16)
@@ -658,8 +658,8 @@ static int buildMinimizerTable(const uint64_t *dKeys, uint64_t n, uint32_t k,
     const uint32_t mlen = o.minimizer_len;
     if (k == 0 || k > 32)
         return setError(HIPIBS_E_ARG, "k must be in 1..32");
-    if (mlen < 5 || mlen > 17 || mlen > k)
-        return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (off) or 5..min(17, k)");
+    if (mlen < 5 || mlen > 31 || mlen > k)
+        return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (off) or 5..min(31, k)");
     if (o.bucket_slots == 0)
         o.bucket_slots = 16;
     if (o.bucket_slots != 8 && o.bucket_slots != 16)

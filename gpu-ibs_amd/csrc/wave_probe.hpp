@@ -110,6 +110,63 @@ __device__ inline int selectBit(uint64_t m, uint32_t n)
     return pos;
 }
 
+// Continues the queries of mask NC (home bucket full without a match) in
+// the buckets after home, in compact rounds of QPR queries: each stops at
+// its code (H), at `alt` (ALT only: A), at an EMPTY slot or after the
+// table's maximum displacement. Must be called by all 64 lanes.
+template <int S, bool ALT>
+__device__ inline void probeOverflow(const TableView &T, uint64_t code, uint32_t home, uint64_t alt,
+                                     uint64_t NC, uint64_t &H, uint64_t &A)
+{
+    constexpr int LPQ = S / 2;
+    constexpr int QPR = 64 / LPQ;
+    constexpr uint32_t GMASK = (1u << LPQ) - 1u;
+    const int lane = laneId();
+    const int g = lane / LPQ, sub = lane % LPQ;
+    // Overflow rounds: pending lane i continues at home_i + disp_i.
+    uint32_t disp = 1;
+    while (NC)
+    {
+        const bool pending = (NC >> lane) & 1ull;
+        const uint32_t nPend = (uint32_t)__popcll(NC);
+        // group g serves the pending lane of rank g
+        const bool serving = (uint32_t)g < nPend;
+        const int src = serving ? selectBit(NC, (uint32_t)g) : 0;
+        const uint64_t qcode = shuffle64(code, src);
+        const uint64_t qhome = shuffle32(home, src);
+        const uint32_t qdisp = shuffle32(disp, src);
+        uint64_t qalt = 0;
+        if constexpr (ALT)
+            qalt = shuffle64(alt, src);
+        SlotPair kv2{EMPTY, EMPTY};
+        if (serving)
+        {
+            uint64_t b = qhome + qdisp;
+            if (b >= T.nBuckets)
+                b -= T.nBuckets;
+            kv2 = loadPair(T.slots + b * S + (uint32_t)sub * 2u);
+        }
+        const uint64_t M2 = ballot(serving && (kv2.a == qcode || kv2.b == qcode));
+        const uint64_t E2 = ballot(serving && (kv2.a == EMPTY || kv2.b == EMPTY));
+        uint64_t A2 = 0;
+        if constexpr (ALT)
+            A2 = ballot(serving && (kv2.a == qalt || kv2.b == qalt));
+        const uint32_t rank = maskedRank(NC);
+        const bool served = pending && rank < (uint32_t)QPR;
+        const uint32_t sh = (served ? rank : 0u) * (uint32_t)LPQ;
+        const bool f2 = served && ((M2 >> sh) & GMASK) != 0;
+        const bool e2 = served && ((E2 >> sh) & GMASK) != 0;
+        const bool a2 = ALT && served && !f2 && ((A2 >> sh) & GMASK) != 0;
+        const bool done = served && (f2 || a2 || e2 || disp >= T.maxDisp);
+        H |= ballot(f2);
+        if constexpr (ALT)
+            A |= ballot(a2);
+        if (served && !done)
+            disp += 1;
+        NC &= ~ballot(done);
+    }
+}
+
 // Returns the 64-bit hit mask of the wave: bit i set when lane i's query is
 // valid and its code is stored in the table; `home` is the query's home
 // bucket (homeOf). Must be called by all 64 lanes. With ALT, a query also
@@ -181,55 +238,8 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, uint32_t 
     if constexpr (ALT)
         A = ballot(valid && !found && marked);
     uint64_t NC = ballot(valid && !found && !marked && !hasEmpty);
-    if (NC == 0 || T.maxDisp == 0)
-    {
-        if constexpr (ALT)
-            *altHit = A;
-        return H;
-    }
-
-    // Overflow rounds: pending lane i continues at home_i + disp_i.
-    uint32_t disp = 1;
-    while (NC)
-    {
-        const bool pending = (NC >> lane) & 1ull;
-        const uint32_t nPend = (uint32_t)__popcll(NC);
-        // group g serves the pending lane of rank g
-        const bool serving = (uint32_t)g < nPend;
-        const int src = serving ? selectBit(NC, (uint32_t)g) : 0;
-        const uint64_t qcode = shuffle64(code, src);
-        const uint64_t qhome = shuffle32(home, src);
-        const uint32_t qdisp = shuffle32(disp, src);
-        uint64_t qalt = 0;
-        if constexpr (ALT)
-            qalt = shuffle64(alt, src);
-        SlotPair kv2{EMPTY, EMPTY};
-        if (serving)
-        {
-            uint64_t b = qhome + qdisp;
-            if (b >= T.nBuckets)
-                b -= T.nBuckets;
-            kv2 = loadPair(T.slots + b * S + (uint32_t)sub * 2u);
-        }
-        const uint64_t M2 = ballot(serving && (kv2.a == qcode || kv2.b == qcode));
-        const uint64_t E2 = ballot(serving && (kv2.a == EMPTY || kv2.b == EMPTY));
-        uint64_t A2 = 0;
-        if constexpr (ALT)
-            A2 = ballot(serving && (kv2.a == qalt || kv2.b == qalt));
-        const uint32_t rank = maskedRank(NC);
-        const bool served = pending && rank < (uint32_t)QPR;
-        const uint32_t sh = (served ? rank : 0u) * (uint32_t)LPQ;
-        const bool f2 = served && ((M2 >> sh) & GMASK) != 0;
-        const bool e2 = served && ((E2 >> sh) & GMASK) != 0;
-        const bool a2 = ALT && served && !f2 && ((A2 >> sh) & GMASK) != 0;
-        const bool done = served && (f2 || a2 || e2 || disp >= T.maxDisp);
-        H |= ballot(f2);
-        if constexpr (ALT)
-            A |= ballot(a2);
-        if (served && !done)
-            disp += 1;
-        NC &= ~ballot(done);
-    }
+    if (NC && T.maxDisp)
+        probeOverflow<S, ALT>(T, code, home, alt, NC, H, A);
     if constexpr (ALT)
         *altHit = A;
     return H;
@@ -357,6 +367,123 @@ __device__ inline uint64_t waveProbeMinimizer(const TableView &T, uint64_t code,
     {
         const TableView T2{T.slots2, T.nBuckets2, T.maxDisp2, 0u, T.k, 0u, nullptr, 0ull};
         const bool v2 = (A >> laneId()) & 1ull;
+        H |= waveProbe<8>(T2, code, v2 ? (uint32_t)bucketOf(code, T.nBuckets2) : 0u, v2);
+    }
+    return H;
+}
+
+// Per-wave LDS staging of waveProbeGrouped: up to kLines home lines, each
+// S slots plus 16 B of padding (lanes that read different lines at the same
+// offset then fall in different banks), and per line the run's
+// (minimizer hash << 32 | home bucket).
+template <int S>
+struct GroupStage
+{
+    static constexpr int kLines = 32;
+    static constexpr int kStride = S + 2; // 64-bit words per staged line
+    static constexpr int kWords = kLines * kStride + kLines;
+};
+
+// waveProbeMinimizer for the scan's consecutive positions: lanes in runs of
+// equal minimizer hash share one home bucket, so the wave loads each run's
+// home line once (LPQ lanes x 16 B, up to 32 lines per pass), stages the
+// lines in LDS and every lane searches its run's line there. The line's
+// flags come from the lanes that loaded it: the run's marker seen (the
+// group lives in the whole-k-mer part) and an EMPTY last slot (slots fill
+// from the first, so the key is not displaced). Queries whose home line is
+// full without their key or marker continue in probeOverflow; marked ones
+// probe the whole-k-mer part. `stage` is this wave's GroupStage<S>::kWords
+// words of LDS, 16 B aligned. Must be called by all 64 lanes.
+template <int S>
+__device__ inline uint64_t waveProbeGrouped(const TableView &T, uint64_t code, uint32_t minHash,
+                                            bool valid, uint64_t *stage)
+{
+    using G = GroupStage<S>;
+    constexpr int LPQ = S / 2;           // lanes per line
+    constexpr int LPR = 64 / LPQ;        // lines per load round
+    constexpr int ROUNDS = G::kLines / LPR;
+    constexpr uint32_t GMASK = (1u << LPQ) - 1u;
+    uint64_t *lines = stage;
+    uint64_t *meta = stage + G::kLines * G::kStride;
+    const int lane = laneId();
+    const int g = lane / LPQ, sub = lane % LPQ;
+    const uint64_t V = ballot(valid);
+    const uint32_t prevHash = shuffle32(minHash, lane > 0 ? lane - 1 : 0);
+    const bool prevValid = lane > 0 && ((V >> ((lane - 1) & 63)) & 1ull);
+    const bool leader = valid && !(prevValid && prevHash == minHash);
+    const uint64_t LM = ballot(leader);
+    const uint32_t nL = (uint32_t)__popcll(LM);
+    const uint32_t below = maskedRank(LM);
+    // a valid lane that is not a leader has a valid lane of its run below it
+    const uint32_t run = leader ? below : below - 1u;
+    const uint32_t home = valid ? (uint32_t)bucketOfMinimizer(minHash, T.nBuckets) : 0u;
+    uint64_t H = 0, A = 0, NC = 0;
+    for (uint32_t base = 0; base < nL; base += G::kLines)
+    {
+        if (leader && below - base < (uint32_t)G::kLines)
+            meta[below - base] = ((uint64_t)minHash << 32) | home;
+        const uint32_t nLines = min(nL - base, (uint32_t)G::kLines);
+        SlotPair kv[ROUNDS];
+        uint64_t mk[ROUNDS];
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r)
+        {
+            const uint32_t li = (uint32_t)(r * LPR + g);
+            kv[r] = SlotPair{EMPTY, EMPTY};
+            mk[r] = EMPTY;
+            if (li < nLines)
+            {
+                const uint64_t e = meta[li];
+                mk[r] = markerOf((uint32_t)(e >> 32));
+                kv[r] = loadPair(T.slots + (uint64_t)(uint32_t)e * S + (uint32_t)sub * 2u);
+            }
+        }
+        uint64_t MK[ROUNDS], EF[ROUNDS];
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r)
+        {
+            const uint32_t li = (uint32_t)(r * LPR + g);
+            const bool act = li < nLines;
+            if (act)
+                *reinterpret_cast<U64x2 *>(lines + li * G::kStride + sub * 2) =
+                    U64x2{kv[r].a, kv[r].b};
+            MK[r] = ballot(act && (kv[r].a == mk[r] || kv[r].b == mk[r]));
+            EF[r] = ballot(act && sub == LPQ - 1 && kv[r].b == EMPTY);
+        }
+        const uint32_t li = run - base;
+        const bool mine = valid && li < (uint32_t)G::kLines;
+        bool found = false;
+        if (mine)
+        {
+            const uint64_t *ln = lines + li * G::kStride;
+#pragma unroll
+            for (int q = 0; q < S; q += 2)
+            {
+                const U64x2 v = *reinterpret_cast<const U64x2 *>(ln + q);
+                found = found || v.x == code || v.y == code;
+            }
+        }
+        const uint32_t myR = mine ? li / LPR : 0u, myG = mine ? li % LPR : 0u;
+        uint64_t mkr = MK[0], efr = EF[0];
+#pragma unroll
+        for (int r = 1; r < ROUNDS; ++r)
+            if (myR == (uint32_t)r)
+            {
+                mkr = MK[r];
+                efr = EF[r];
+            }
+        const bool marked = mine && !found && ((mkr >> (myG * LPQ)) & GMASK) != 0;
+        const bool hasEmpty = ((efr >> (myG * LPQ + LPQ - 1)) & 1ull) != 0;
+        H |= ballot(mine && found);
+        A |= ballot(marked);
+        NC |= ballot(mine && !found && !marked && !hasEmpty);
+    }
+    if (NC && T.maxDisp)
+        probeOverflow<S, true>(T, code, home, markerOf(minHash), NC, H, A);
+    if (A)
+    {
+        const TableView T2{T.slots2, T.nBuckets2, T.maxDisp2, 0u, T.k, 0u, nullptr, 0ull};
+        const bool v2 = (A >> lane) & 1ull;
         H |= waveProbe<8>(T2, code, v2 ? (uint32_t)bucketOf(code, T.nBuckets2) : 0u, v2);
     }
     return H;

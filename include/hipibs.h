@@ -65,7 +65,7 @@ typedef struct hipibs_table_opts {
     uint64_t hbm_budget;    /* max table bytes, 0 = no limit */
     uint32_t load_threads;  /* host threads of a KMC read, 0 = all CPUs */
     uint32_t minimizer_len; /* 0 (default): home bucket from a hash of the whole k-mer;
-                               5..17: a minimizer table -- keys sharing a minimizer of
+                               5..min(31, k): a minimizer table -- keys sharing a minimizer of
                                that length share buckets, so consecutive positions of a
                                reference share memory requests */
     uint32_t minimizer_cap; /* minimizer tables: a minimizer with more keys than this

@@ -85,7 +85,8 @@ def test_edge_scans_match_golden(k, expected):
                                                (16, 0.95, 0, 0), (8, 0.3, 0, 0), (16, 0.6, 13, 0),
                                                (8, 0.6, 17, 0), (16, 0.9, 5, 0), (16, 0.6, 17, 0),
                                                (16, 0.6, 13, 1), (16, 0.9, 15, 2), (8, 0.6, 11, 4),
-                                               (16, 0.6, 9, 0)])
+                                               (16, 0.6, 9, 0), (16, 0.35, 21, 16),
+                                               (8, 0.35, 25, 2)])
 def test_table_geometry_does_not_change_results(small_corpus, slots, lf, mlen, cap):
     """Bucket width, load factor and home-bucket rule (whole-k-mer hash,
     minimizer tables of several lengths and group caps, so that groups
@@ -106,7 +107,7 @@ def test_table_geometry_does_not_change_results(small_corpus, slots, lf, mlen, c
 
 
 @pytest.mark.parametrize("k,mlen,cap", [(25, 13, 1), (25, 15, 3), (31, 17, 2), (32, 17, 1),
-                                        (15, 7, 2), (31, 13, 0)])
+                                        (15, 7, 2), (31, 13, 0), (25, 21, 16), (31, 31, 1)])
 def test_minimizer_table_every_build_path(tmp_path, k, mlen, cap):
     """Minimizer tables built from host keys, device keys and a KMC database
     hold exactly the key set (markers never answer as keys), and their scans
@@ -156,6 +157,17 @@ def test_minimizer_table_every_build_path(tmp_path, k, mlen, cap):
     assert np.array_equal(bits.astype(np.int64), want_bits)
     for t in tables:
         t.free()
+
+
+def test_minimizer_len_limits():
+    """Minimizer lengths outside 5..min(31, k) are refused."""
+    keys = corpus.sequence_kmers([corpus.random_bases(5, 5, 4000)], 32)
+    for k, m in ((32, 32), (25, 26), (25, 4)):
+        with pytest.raises(hipibs.HipibsError, match="minimizer_len"):
+            hipibs.Table.from_keys(keys[:100] >> np.uint64(2 * (32 - k)), k, minimizer_len=m)
+    t = hipibs.Table.from_keys(keys, 32, minimizer_len=31)
+    assert t.probe_count(keys) == keys.size
+    t.free()
 
 
 def test_probe_count_intersect(small_corpus, expected):
