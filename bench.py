@@ -44,12 +44,16 @@ def parse():
     ap.add_argument("--gbp", type=float, default=2.0, help="reference length in Gbp")
     ap.add_argument("--keys", type=float, default=2.0e9, help="target distinct keys")
     ap.add_argument("--bucket-slots", type=int, default=0, help="0 = library default")
-    ap.add_argument("--minimizer-len", type=int, default=0,
-                    help="0 = whole-k-mer hash buckets (default), 5..min(31, k) = minimizer buckets")
-    ap.add_argument("--minimizer-cap", type=int, default=0,
+    ap.add_argument("--minimizer-len", type=int, default=-1,
+                    help="0 = whole-k-mer hash buckets, 5..min(31, k) = minimizer buckets; "
+                         "default: 21 for the scan workload, 0 otherwise")
+    ap.add_argument("--minimizer-cap", type=int, default=-1,
                     help="minimizer tables: keys per minimizer group before it moves to the "
-                         "whole-k-mer part (0 = library default)")
-    ap.add_argument("--load-factor", type=float, default=0.6)
+                         "whole-k-mer part (0 = library default); default: 16 for the scan "
+                         "workload")
+    ap.add_argument("--load-factor", type=float, default=0.0,
+                    help="keys per slot; default: 0.2 for the scan workload's minimizer "
+                         "table, 0.6 otherwise")
     ap.add_argument("--segs", type=int, default=0, help="segments per window (0 = auto)")
     ap.add_argument("--cpu-mbp", type=float, default=1536.0,
                     help="reference slice (Mbp) of the CPU baseline sample; 0 = skip")
@@ -91,7 +95,19 @@ def parse():
                     help="panel: HBM budget = the panel's table bytes / this, so the panel "
                          "runs in at least this many memory-budgeted groups")
     ap.add_argument("--workdir", default=os.environ.get("TMPDIR", "/tmp"))
-    return ap.parse_args()
+    args = ap.parse_args()
+    # The scan workload's table: minimizer buckets (m = 21, groups of up to 16
+    # keys) at load 0.2, so that each run of reference positions sharing a
+    # minimizer reads one 128 B line; the other workloads keep whole-k-mer
+    # tables at load 0.6.
+    minimizer = args.workload == "scan" and args.minimizer_len != 0
+    if args.minimizer_len < 0:
+        args.minimizer_len = 21 if minimizer else 0
+    if args.minimizer_cap < 0:
+        args.minimizer_cap = 16 if minimizer else 0
+    if args.load_factor <= 0:
+        args.load_factor = 0.2 if minimizer else 0.6
+    return args
 
 
 # ---- synthetic data on the GPU ---------------------------------------------------
@@ -1496,13 +1512,19 @@ def main():
     # HBM traffic per launch, measured with rocprofv3 --pmc on this default
     # workload and calibrated against a plain copy kernel
     traffic = None
+    requests_per_probe = None
     tpath = os.path.join(ROOT, "profiles", "scan_k25_2gbp_traffic.json")
     if os.path.exists(tpath) and n == 2_000_000_000 and abs(args.keys - 2e9) < 1:
         with open(tpath) as f:
-            tj = json.load(f)
-        if (tj.get("bucket_slots") == info["bucket_slots"] and
-                tj.get("minimizer_len") == info["minimizer_len"]):
-            traffic = tj["traffic_bytes_per_dispatch"]
+            entries = json.load(f)["tables"]
+        for tj in entries:
+            if (tj["bucket_slots"] == info["bucket_slots"] and
+                    tj["minimizer_len"] == info["minimizer_len"] and
+                    tj["minimizer_cap"] == (args.minimizer_cap if info["minimizer_len"] else 0)
+                    and abs(tj["load_factor"] - args.load_factor) < 1e-9):
+                traffic = tj["traffic_bytes_per_dispatch"]
+                rq = tj["ea_read_requests_per_dispatch"]
+                requests_per_probe = (rq["128B"] + rq["64B"] + rq["32B"]) / probes
     value = probes / (ms / 1e3)
     out = {
         "metric": "ref k-mers probed/s",
@@ -1522,7 +1544,7 @@ def main():
                    "reference_bases": n, "windows": len(wins), "k": K, "window": W,
                    "sample_keys": int(info["n_keys"]), "table_bytes": int(info["bytes"]),
                    "bucket_slots": info["bucket_slots"], "load_factor": args.load_factor,
-                   "minimizer_len": info["minimizer_len"],
+                   "minimizer_len": info["minimizer_len"], "minimizer_cap": args.minimizer_cap,
                    "segs_per_window": segs.value, "scan_wall_s": ms / 1e3,
                    "table_build_s": build_s, "probes_per_step": probes,
                    "hit_rate": observed / max(1, probes), "repeat_stable": stable,
@@ -1530,13 +1552,16 @@ def main():
         "roofline": {"bound": "hbm", "achieved": alg_bytes / (k_ms / 1e3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg_bytes / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"scanKernel<{info['bucket_slots']}>", "kernel_ms": k_ms,
+                     "kernel": (f"scanKernel<{info['bucket_slots']}, grouped>"
+                                if info["minimizer_len"] else
+                                f"scanKernel<{info['bucket_slots']}>"), "kernel_ms": k_ms,
                      "algorithmic_bytes": alg_bytes,
                      "probe_rate_gps": probes / (k_ms / 1e3) / 1e9,
                      "measured_random_read_ceiling_gps": max(gups.values()) if gups else None,
                      "frac_of_measured_random_ceiling":
                          (probes / (k_ms / 1e3) / 1e9) / max(gups.values()) if gups else None,
-                     "hbm_request_bytes_per_probe": 128,
+                     "hbm_request_bytes": 128,
+                     "hbm_requests_per_probe": requests_per_probe,
                      "gups_sweep_g_reads_per_s": gups},
         "cpu_baseline": cpu,
     }

@@ -218,6 +218,33 @@ extern "C" int hipibs_bench_gups_policy(const void *d_buf, uint64_t n_buckets,
     return 0;
 }
 
+// Device memory of one allocation kind for the ceiling measurement: 0 =
+// hipMalloc's default (coarse-grained), 1 = fine-grained, 2 = uncached
+// (hipExtMallocWithFlags), filled with a byte pattern. Freeing uncached
+// memory that a kernel has read can corrupt later allocations, so
+// hipibs_bench_free refuses kind 2: such a buffer lives until the process
+// ends.
+extern "C" int hipibs_bench_alloc(uint64_t bytes, int kind, void **out)
+{
+    static const unsigned flags[3] = {hipDeviceMallocDefault, hipDeviceMallocFinegrained,
+                                      hipDeviceMallocUncached};
+    if (!out || kind < 0 || kind > 2 || bytes == 0)
+        return -1;
+    *out = nullptr;
+    if (hipExtMallocWithFlags(out, bytes, flags[kind]) != hipSuccess)
+        return -2;
+    if (hipMemset(*out, 0x5A, bytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return -3;
+    return 0;
+}
+
+extern "C" int hipibs_bench_free(void *p, int kind)
+{
+    if (kind == 2)
+        return -1;
+    return hipFree(p) == hipSuccess ? 0 : -2;
+}
+
 // Buckets actually read by hipibs_bench_gups for these arguments.
 extern "C" uint64_t hipibs_bench_gups_reads(uint32_t bucket_slots, uint64_t n_reads, int rounds,
                                             int waves_per_cu)

@@ -76,15 +76,18 @@ __host__ __device__ inline uint32_t minimizerOf(uint64_t key, uint32_t k, uint32
     return best;
 }
 
-// The minimum of hashes is skewed toward small values: re-mix it before
-// scaling onto the buckets.
+// The minimum of hashes is skewed toward small values: re-mix it (the
+// 32-bit finalizer of MurmurHash3, a bijection) before scaling onto the
+// buckets (n_buckets < 2^32).
 __host__ __device__ inline uint64_t bucketOfMinimizer(uint32_t minHash, uint64_t nBuckets)
 {
-    uint64_t x = (uint64_t)minHash + 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    x ^= x >> 31;
-    return ((x >> 32) * nBuckets) >> 32;
+    uint32_t x = minHash;
+    x ^= x >> 16;
+    x *= 0x85EBCA6Bu;
+    x ^= x >> 13;
+    x *= 0xC2B2AE35u;
+    x ^= x >> 16;
+    return ((uint64_t)x * (uint32_t)nBuckets) >> 32;
 }
 
 // Device view of one table, passed by value to kernels. A minimizer table

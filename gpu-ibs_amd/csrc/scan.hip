@@ -31,6 +31,9 @@ namespace hipibs
 
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_WAVES = SCAN_BLOCK / 64;
+// 64-position words per step of a minimizer-table scan (two words per step
+// keep twice the lines in flight per wave but cost a wave per SIMD: slower).
+constexpr int kGroupWords = 1;
 
 // MIXED: every window names its own table (hipibs_window.reserved) and its
 // result goes to row w; otherwise grid.y is the table and results are
@@ -47,7 +50,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
 {
     __shared__ SegSummary waveSum[SCAN_WAVES];
     __shared__ __align__(16) uint64_t groupStage[GROUPED ? SCAN_WAVES : 1]
-                                                [GROUPED ? GroupStage<S>::kWords : 2];
+                                                [GROUPED ? GroupStage<S, kGroupWords>::kWords : 2];
     const uint32_t unit = blockIdx.x;
     const uint32_t w = unit / segs, s = unit - w * segs;
     const hipibs_window win = windows[w];
@@ -62,7 +65,6 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
     const uint64_t kmask1 = k >= 64 ? ~0ull : (1ull << k) - 1ull;
     const uint64_t kmask2 = k >= 32 ? ~0ull : (1ull << (2 * k)) - 1ull;
     const uint32_t fwdShift = k >= 32 ? 0u : 64 - 2 * k;
-    const uint64_t lanesBelow = (1ull << lane) - 1ull;
 
     uint32_t nV = 0, nH = 0, lead = 0, T0 = 0, uVar = 0, uDist = 0;
     uint32_t lVar = 0, lDist = 0;
@@ -79,33 +81,22 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
         const uint64_t maskm2 = mlen ? (1ull << (2 * mlen)) - 1ull : 0ull;
         const uint32_t mShift = 64 - 2 * mlen;
         const int win = (int)(k - mlen + 1); // m-mers per k-mer
-        for (uint64_t m = mBeg; m < mEnd; ++m)
-        {
+        // Canonical code, validity and (minimizer tables) minimizer hash of
+        // position 64m + lane.
+        auto encode = [&](uint64_t m, uint64_t &canon, bool &valid, uint32_t &minh) {
             const uint64_t s0 = seq[2 * m], s1 = seq[2 * m + 1], s2 = seq[2 * m + 2];
             const uint64_t e0 = exc[m], e1 = exc[m + 1];
             const uint64_t a = lane < 32 ? s0 : s1;
             const uint64_t b = lane < 32 ? s1 : s2;
-            uint64_t x = j ? ((a >> (2 * j)) | (b << (64 - 2 * j))) : a;
-            const uint64_t xm = x; // low-first window of bases p .. p+31
-            x &= kmask2;
+            const uint64_t x = (j ? ((a >> (2 * j)) | (b << (64 - 2 * j))) : a) & kmask2;
             const uint64_t ev = lane ? ((e0 >> lane) | (e1 << (64 - lane))) : e0;
             const uint64_t p = (m << 6) + (uint64_t)lane;
-            const bool valid = ((ev & kmask1) == 0) && p >= g0 && p < g1;
+            valid = ((ev & kmask1) == 0) && p >= g0 && p < g1;
             const uint64_t rc = ~x & kmask2;
             const uint64_t fwd = reverse2(x) >> fwdShift;
-            const uint64_t canon = fwd < rc ? fwd : rc;
-            uint64_t H;
-            if constexpr (WIDE)
-            {
-                const uint64_t s3 = seq[2 * m + 3];
-                const uint64_t c = lane < 32 ? s2 : s3;
-                const uint64_t x1 = j ? ((b >> (2 * j)) | (c << (64 - 2 * j))) : b;
-                uint64_t hi, lo;
-                wideCanonical(xm, x1, k, hi, lo);
-                H = waveProbeWide(T, hi, lo,
-                                  valid ? (uint32_t)bucketOfWide(hi, lo, T.nBuckets) : 0u, valid);
-            }
-            else if (GROUPED || mlen)
+            canon = fwd < rc ? fwd : rc;
+            minh = 0;
+            if (GROUPED || mlen)
             {
                 // canonical m-mer hashes at positions p and p+64, then the
                 // minimum over each k-mer's window of `win` m-mers by a
@@ -114,10 +105,12 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
                 const uint64_t a2 = lane < 32 ? s2 : s3;
                 const uint64_t b2 = lane < 32 ? s3 : s4;
                 uint64_t y = j ? ((a2 >> (2 * j)) | (b2 << (64 - 2 * j))) : a2;
-                uint64_t ylo = xm & maskm2;
                 y &= maskm2;
-                const uint64_t rlo = ~ylo & maskm2, rhi = ~y & maskm2;
-                const uint64_t flo = reverse2(ylo) >> mShift, fhi = reverse2(y) >> mShift;
+                // the m-mer at p is the k-mer's first m bases: its forward
+                // code is the top of fwd, its reverse complement the bottom
+                // of rc
+                const uint64_t flo = fwd >> (2 * (k - mlen)), rlo = rc & maskm2;
+                const uint64_t rhi = ~y & maskm2, fhi = reverse2(y) >> mShift;
                 uint32_t L = mmerHash(flo < rlo ? flo : rlo);
                 uint32_t Hh = mmerHash(fhi < rhi ? fhi : rhi);
                 int span = 1;
@@ -133,15 +126,11 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
                 const int d = win - span;
                 const int src = (lane + d) & 63;
                 const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
-                const uint32_t minh = min(L, lane + d < 64 ? l2 : h2);
-                if constexpr (GROUPED)
-                    H = waveProbeGrouped<S>(T, canon, minh, valid, groupStage[wv]);
-                else
-                    H = waveProbeMinimizer<S>(T, canon, minh, valid);
+                minh = min(L, lane + d < 64 ? l2 : h2);
             }
-            else
-                H = waveProbe<S>(T, canon, (uint32_t)bucketOf(canon, T.nBuckets), valid);
-            const uint64_t V = ballot(valid);
+        };
+        // Run-length statistics of one word from its hit and valid masks.
+        auto fold = [&](uint64_t H, uint64_t V) {
             const uint64_t Miss = V & ~H;
             if (H)
             {
@@ -164,19 +153,17 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
                 T0 = (uint32_t)__popcll(Miss >> l);
                 // runs closed inside the word, one per hit lane with an
                 // earlier hit in the word
-                if ((H >> lane) & 1ull)
+                if (((H >> lane) & 1ull) && maskedRank(H))
                 {
-                    const uint64_t below = H & lanesBelow;
-                    if (below)
+                    // the previous hit: H's highest bit below this lane
+                    // (lane >= 1 here), moved to the top of the word
+                    const int prev = lane - 1 - __builtin_clzll(H << (64 - lane));
+                    // misses between the previous hit and this lane
+                    const uint32_t gap = maskedRank(Miss & ~((2ull << prev) - 1ull));
+                    if (gap)
                     {
-                        const int prev = 63 - __builtin_clzll(below);
-                        const uint32_t gap =
-                            (uint32_t)__popcll(Miss & lanesBelow & ~((2ull << prev) - 1ull));
-                        if (gap)
-                        {
-                            lVar += 1;
-                            lDist += (uint32_t)gapDistance((int32_t)gap, (int32_t)k);
-                        }
+                        lVar += 1;
+                        lDist += (uint32_t)gapDistance((int32_t)gap, (int32_t)k);
                     }
                 }
             }
@@ -184,7 +171,68 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
                 T0 += (uint32_t)__popcll(Miss);
             nV += (uint32_t)__popcll(V);
             nH += (uint32_t)__popcll(H);
+        };
+        if constexpr (GROUPED)
+        {
+            // kGroupWords words per step: their home lines are in flight
+            // together
+            for (uint64_t m = mBeg; m < mEnd; m += kGroupWords)
+            {
+                uint64_t canon[kGroupWords], H[kGroupWords];
+                uint32_t minh[kGroupWords];
+                bool valid[kGroupWords];
+#pragma unroll
+                for (int i = 0; i < kGroupWords; ++i)
+                {
+                    canon[i] = 0;
+                    minh[i] = 0;
+                    valid[i] = false;
+                    if (m + i < mEnd)
+                        encode(m + i, canon[i], valid[i], minh[i]);
+                }
+                waveProbeGrouped<S, kGroupWords>(T, canon, minh, valid, groupStage[wv], H);
+#pragma unroll
+                for (int i = 0; i < kGroupWords; ++i)
+                    if (m + i < mEnd)
+                        fold(H[i], ballot(valid[i]));
+            }
         }
+        else
+            for (uint64_t m = mBeg; m < mEnd; ++m)
+            {
+                uint64_t H;
+                bool valid;
+                if constexpr (WIDE)
+                {
+                    const uint64_t s0 = seq[2 * m], s1 = seq[2 * m + 1], s2 = seq[2 * m + 2];
+                    const uint64_t s3 = seq[2 * m + 3];
+                    const uint64_t e0 = exc[m], e1 = exc[m + 1];
+                    const uint64_t a = lane < 32 ? s0 : s1;
+                    const uint64_t b = lane < 32 ? s1 : s2;
+                    const uint64_t c = lane < 32 ? s2 : s3;
+                    const uint64_t x0 = j ? ((a >> (2 * j)) | (b << (64 - 2 * j))) : a;
+                    const uint64_t x1 = j ? ((b >> (2 * j)) | (c << (64 - 2 * j))) : b;
+                    const uint64_t ev = lane ? ((e0 >> lane) | (e1 << (64 - lane))) : e0;
+                    const uint64_t p = (m << 6) + (uint64_t)lane;
+                    valid = ((ev & kmask1) == 0) && p >= g0 && p < g1;
+                    uint64_t hi, lo;
+                    wideCanonical(x0, x1, k, hi, lo);
+                    H = waveProbeWide(T, hi, lo,
+                                      valid ? (uint32_t)bucketOfWide(hi, lo, T.nBuckets) : 0u,
+                                      valid);
+                }
+                else
+                {
+                    uint64_t canon;
+                    uint32_t minh;
+                    encode(m, canon, valid, minh);
+                    if (mlen)
+                        H = waveProbeMinimizer<S>(T, canon, minh, valid);
+                    else
+                        H = waveProbe<S>(T, canon, (uint32_t)bucketOf(canon, T.nBuckets), valid);
+                }
+                fold(H, ballot(valid));
+            }
     }
     lVar = waveSumU32(lVar);
     lDist = waveSumU32(lDist);

@@ -372,121 +372,127 @@ __device__ inline uint64_t waveProbeMinimizer(const TableView &T, uint64_t code,
     return H;
 }
 
-// Per-wave LDS staging of waveProbeGrouped: up to kLines home lines, each
-// S slots plus 16 B of padding (lanes that read different lines at the same
-// offset then fall in different banks), and per line the run's
-// (minimizer hash << 32 | home bucket).
-template <int S>
+// Per-wave LDS staging of waveProbeGrouped over NW words: up to kLines home
+// lines, each S slots plus 16 B of padding (lanes that read different lines
+// at the same offset then fall in different banks), and the lines' home
+// buckets.
+template <int S, int NW>
 struct GroupStage
 {
-    static constexpr int kLines = 32;
+    static constexpr int kLines = 32 * NW;
     static constexpr int kStride = S + 2; // 64-bit words per staged line
-    static constexpr int kWords = kLines * kStride + kLines;
+    static constexpr int kWords = kLines * kStride + kLines / 2;
 };
 
-// waveProbeMinimizer for the scan's consecutive positions: lanes in runs of
-// equal minimizer hash share one home bucket, so the wave loads each run's
-// home line once (LPQ lanes x 16 B, up to 32 lines per pass), stages the
-// lines in LDS and every lane searches its run's line there. The line's
-// flags come from the lanes that loaded it: the run's marker seen (the
-// group lives in the whole-k-mer part) and an EMPTY last slot (slots fill
-// from the first, so the key is not displaced). Queries whose home line is
-// full without their key or marker continue in probeOverflow; marked ones
-// probe the whole-k-mer part. `stage` is this wave's GroupStage<S>::kWords
-// words of LDS, 16 B aligned. Must be called by all 64 lanes.
-template <int S>
-__device__ inline uint64_t waveProbeGrouped(const TableView &T, uint64_t code, uint32_t minHash,
-                                            bool valid, uint64_t *stage)
+// waveProbeMinimizer for the scan's consecutive positions, NW words of 64
+// positions at once (lane i holds position i of every word): lanes in runs
+// of equal minimizer hash share one home bucket, so the wave loads each
+// run's home line once (LPQ lanes x 16 B; the lines of all NW words are in
+// flight together, up to kLines per pass), stages the lines in LDS and
+// every lane searches its runs' lines there for its code, its group's
+// marker (the group lives in the whole-k-mer part) and an EMPTY last slot
+// (slots fill from the first, so then the key is not displaced). Queries
+// whose home line is full without their key or marker continue in
+// probeOverflow; marked ones probe the whole-k-mer part. `stage` is this
+// wave's GroupStage<S, NW>::kWords words of LDS, 16 B aligned. H[w] gets
+// word w's hit mask. Must be called by all 64 lanes.
+template <int S, int NW>
+__device__ inline void waveProbeGrouped(const TableView &T, const uint64_t (&code)[NW],
+                                        const uint32_t (&minHash)[NW], const bool (&valid)[NW],
+                                        uint64_t *stage, uint64_t (&H)[NW])
 {
-    using G = GroupStage<S>;
+    using G = GroupStage<S, NW>;
     constexpr int LPQ = S / 2;           // lanes per line
     constexpr int LPR = 64 / LPQ;        // lines per load round
     constexpr int ROUNDS = G::kLines / LPR;
-    constexpr uint32_t GMASK = (1u << LPQ) - 1u;
     uint64_t *lines = stage;
-    uint64_t *meta = stage + G::kLines * G::kStride;
+    uint32_t *homes = reinterpret_cast<uint32_t *>(stage + G::kLines * G::kStride);
     const int lane = laneId();
     const int g = lane / LPQ, sub = lane % LPQ;
-    const uint64_t V = ballot(valid);
-    const uint32_t prevHash = shuffle32(minHash, lane > 0 ? lane - 1 : 0);
-    const bool prevValid = lane > 0 && ((V >> ((lane - 1) & 63)) & 1ull);
-    const bool leader = valid && !(prevValid && prevHash == minHash);
-    const uint64_t LM = ballot(leader);
-    const uint32_t nL = (uint32_t)__popcll(LM);
-    const uint32_t below = maskedRank(LM);
-    // a valid lane that is not a leader has a valid lane of its run below it
-    const uint32_t run = leader ? below : below - 1u;
-    const uint32_t home = valid ? (uint32_t)bucketOfMinimizer(minHash, T.nBuckets) : 0u;
-    uint64_t H = 0, A = 0, NC = 0;
+    uint32_t run[NW], home[NW], below[NW];
+    bool leader[NW];
+    uint32_t nL = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+    {
+        const uint64_t V = ballot(valid[w]);
+        const uint32_t prevHash = shuffle32(minHash[w], lane > 0 ? lane - 1 : 0);
+        const bool prevValid = lane > 0 && ((V >> ((lane - 1) & 63)) & 1ull);
+        leader[w] = valid[w] && !(prevValid && prevHash == minHash[w]);
+        const uint64_t LM = ballot(leader[w]);
+        // runs are numbered across the words: word w's after word w-1's
+        below[w] = nL + maskedRank(LM);
+        // a valid lane that is not a leader has a valid lane of its run below
+        run[w] = leader[w] ? below[w] : below[w] - 1u;
+        home[w] = valid[w] ? (uint32_t)bucketOfMinimizer(minHash[w], T.nBuckets) : 0u;
+        nL += (uint32_t)__popcll(LM);
+        H[w] = 0;
+    }
+    uint64_t A[NW], NC[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+        A[w] = NC[w] = 0;
     for (uint32_t base = 0; base < nL; base += G::kLines)
     {
-        if (leader && below - base < (uint32_t)G::kLines)
-            meta[below - base] = ((uint64_t)minHash << 32) | home;
+#pragma unroll
+        for (int w = 0; w < NW; ++w)
+            if (leader[w] && below[w] - base < (uint32_t)G::kLines)
+                homes[below[w] - base] = home[w];
         const uint32_t nLines = min(nL - base, (uint32_t)G::kLines);
-        SlotPair kv[ROUNDS];
-        uint64_t mk[ROUNDS];
+        U64x2 kv[ROUNDS];
 #pragma unroll
         for (int r = 0; r < ROUNDS; ++r)
         {
             const uint32_t li = (uint32_t)(r * LPR + g);
-            kv[r] = SlotPair{EMPTY, EMPTY};
-            mk[r] = EMPTY;
             if (li < nLines)
-            {
-                const uint64_t e = meta[li];
-                mk[r] = markerOf((uint32_t)(e >> 32));
-                kv[r] = loadPair(T.slots + (uint64_t)(uint32_t)e * S + (uint32_t)sub * 2u);
-            }
+                kv[r] = __builtin_nontemporal_load((GlobalU64x2)(reinterpret_cast<const U64x2 *>(
+                    T.slots + (uint64_t)homes[li] * S + (uint32_t)sub * 2u)));
         }
-        uint64_t MK[ROUNDS], EF[ROUNDS];
 #pragma unroll
         for (int r = 0; r < ROUNDS; ++r)
         {
             const uint32_t li = (uint32_t)(r * LPR + g);
-            const bool act = li < nLines;
-            if (act)
-                *reinterpret_cast<U64x2 *>(lines + li * G::kStride + sub * 2) =
-                    U64x2{kv[r].a, kv[r].b};
-            MK[r] = ballot(act && (kv[r].a == mk[r] || kv[r].b == mk[r]));
-            EF[r] = ballot(act && sub == LPQ - 1 && kv[r].b == EMPTY);
+            if (li < nLines)
+                *reinterpret_cast<U64x2 *>(lines + li * G::kStride + sub * 2) = kv[r];
         }
-        const uint32_t li = run - base;
-        const bool mine = valid && li < (uint32_t)G::kLines;
-        bool found = false;
-        if (mine)
+#pragma unroll
+        for (int w = 0; w < NW; ++w)
         {
-            const uint64_t *ln = lines + li * G::kStride;
-#pragma unroll
-            for (int q = 0; q < S; q += 2)
+            const uint32_t li = run[w] - base;
+            const bool mine = valid[w] && li < (uint32_t)G::kLines;
+            const uint64_t marker = markerOf(minHash[w]);
+            bool found = false, marked = false, hasEmpty = false;
+            if (mine)
             {
-                const U64x2 v = *reinterpret_cast<const U64x2 *>(ln + q);
-                found = found || v.x == code || v.y == code;
+                const uint64_t *ln = lines + li * G::kStride;
+#pragma unroll
+                for (int q = 0; q < S; q += 2)
+                {
+                    const U64x2 v = *reinterpret_cast<const U64x2 *>(ln + q);
+                    found = found || v.x == code[w] || v.y == code[w];
+                    marked = marked || v.x == marker || v.y == marker;
+                    if (q == S - 2)
+                        hasEmpty = v.y == EMPTY;
+                }
             }
+            H[w] |= ballot(mine && found);
+            A[w] |= ballot(mine && !found && marked);
+            NC[w] |= ballot(mine && !found && !marked && !hasEmpty);
         }
-        const uint32_t myR = mine ? li / LPR : 0u, myG = mine ? li % LPR : 0u;
-        uint64_t mkr = MK[0], efr = EF[0];
+    }
 #pragma unroll
-        for (int r = 1; r < ROUNDS; ++r)
-            if (myR == (uint32_t)r)
-            {
-                mkr = MK[r];
-                efr = EF[r];
-            }
-        const bool marked = mine && !found && ((mkr >> (myG * LPQ)) & GMASK) != 0;
-        const bool hasEmpty = ((efr >> (myG * LPQ + LPQ - 1)) & 1ull) != 0;
-        H |= ballot(mine && found);
-        A |= ballot(marked);
-        NC |= ballot(mine && !found && !marked && !hasEmpty);
-    }
-    if (NC && T.maxDisp)
-        probeOverflow<S, true>(T, code, home, markerOf(minHash), NC, H, A);
-    if (A)
+    for (int w = 0; w < NW; ++w)
     {
-        const TableView T2{T.slots2, T.nBuckets2, T.maxDisp2, 0u, T.k, 0u, nullptr, 0ull};
-        const bool v2 = (A >> lane) & 1ull;
-        H |= waveProbe<8>(T2, code, v2 ? (uint32_t)bucketOf(code, T.nBuckets2) : 0u, v2);
+        if (NC[w] && T.maxDisp)
+            probeOverflow<S, true>(T, code[w], home[w], markerOf(minHash[w]), NC[w], H[w], A[w]);
+        if (A[w])
+        {
+            const TableView T2{T.slots2, T.nBuckets2, T.maxDisp2, 0u, T.k, 0u, nullptr, 0ull};
+            const bool v2 = (A[w] >> lane) & 1ull;
+            H[w] |= waveProbe<8>(T2, code[w], v2 ? (uint32_t)bucketOf(code[w], T.nBuckets2) : 0u,
+                                 v2);
+        }
     }
-    return H;
 }
 
 // Membership of each lane's code in T, whole-k-mer or minimizer table.
