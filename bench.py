@@ -1225,23 +1225,34 @@ def run_count(args):
     n = text.numel()
     log(f"reads: {n_reads} x {L} bp as {n/1e9:.2f} GB of text ({time.time()-t0:.1f} s)")
 
+    phases = []
+
     def step():
+        t = [time.perf_counter()]
         c = hipibs.KmerCounter(K)
+        t.append(time.perf_counter())
         c.add_text_device(text.data_ptr(), n)
+        t.append(time.perf_counter())
         st = c.stats()
-        return c, st
+        t.append(time.perf_counter())
+        c.free()
+        t.append(time.perf_counter())
+        phases.append([(b - a) * 1e3 for a, b in zip(t, t[1:])])
+        return st
 
     for _ in range(args.warmup):
-        step()[0].free()
+        step()
     torch.cuda.synchronize()
+    phases.clear()
     stats = []
     ts = time.perf_counter()
     for _ in range(args.steps):
-        c, st = step()
-        stats.append(st)
-        c.free()
+        stats.append(step())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - ts
+    ph = np.mean(np.array(phases), axis=0)
+    log(f"count step phases (ms): create {ph[0]:.1f}, add text {ph[1]:.1f}, "
+        f"stats (counts the flush) {ph[2]:.1f}, free {ph[3]:.1f}")
     ms = elapsed / args.steps * 1e3
     st = stats[-1]
     stable = all((x["kmers"], x["distinct"]) == (st["kmers"], st["distinct"]) for x in stats)
@@ -1337,6 +1348,9 @@ def run_count(args):
         "data": "synthetic (GPU-generated, seeded)",
         "config": {"workload": f"k-mer counting, k={K}: {n_reads} reads of {L} bp (10x of a "
                                f"{G/1e6:.0f} Mbp genome, 1% SNPs, 0.5% errors) as device text",
+                   "step_phase_ms": {"create": ph[0], "add_text": ph[1], "stats_flush": ph[2],
+                                     "free": ph[3]},
+                   "spilled": st["spilled"], "flushes": st["flushes"],
                    "text_bytes": n, "kmers": st["kmers"], "distinct": st["distinct"],
                    "table_bytes": st["table_bytes"], "repeat_stable": stable,
                    "kmc_end_to_end": e2e},

@@ -10,11 +10,20 @@
 //
 // Counting table in HBM: n_buckets buckets of 128 B, 256 B aligned: the
 // bucket's 8 keys (EMPTY = all ones) in its first 64 B, their 8 counts in
-// the second, so the count's atomic add goes to the line the key lookup has
-// just read (one memory line per counted k-mer). Home bucket bucketOf(key),
-// linear probing at bucket granularity, wrapping. The host grows the table
-// between chunks so that it stays at or below 0.7 load even if every
-// position of the next chunk were a new key: an insert always finds a slot.
+// the second. Home bucket bucketOf(key), linear probing at bucket
+// granularity, wrapping.
+//
+// k <= 32 (sorted-slice counting): the k-mers of up to 16 chunks are hashed
+// (keyMix, invertible) into a device buffer with a distinct-count sketch;
+// the host sizes the table from the sketch (a power of two of buckets), the
+// hashes are grouped by slice (512 buckets = 64 KB, the top bits of the
+// hash) in two bucketing passes, and one workgroup per slice counts its
+// k-mers in LDS and writes the slice back; the few k-mers whose probe runs
+// past their slice are counted in the global table with atomics afterwards.
+// k = 33..64: every k-mer is counted in the global table with memory-side
+// atomics as it is read, and the host grows the table between chunks so
+// that it stays at or below 0.7 load even if every position of the next
+// chunk were a new key.
 //
 // Reads arrive as text: the bases of each record followed by one '\n' (any
 // byte outside ACGTacgt ends a run of valid bases), in pinned chunks of
@@ -24,6 +33,9 @@
 #include "wave_probe.hpp"
 
 #include <algorithm>
+#include <mutex>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -263,6 +275,298 @@ __global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__rest
     }
 }
 
+// ---- sorted-slice counting (k <= 32) ---------------------------------------
+// The k-mers of a flush are first written as h = keyMix(code), the table is
+// sized, h is sorted by its top `sbits` bits (n_buckets = 2^(sbits+9), so
+// those bits are the slice: the 512-bucket range that holds h's home
+// bucket), and one workgroup per slice counts the slice's k-mers in LDS
+// with LDS atomics before writing the slice back. A k-mer whose probe runs
+// past the slice's last bucket is spilled and counted in the global table
+// afterwards. No atomic reaches memory except the spills'.
+constexpr int kSliceLog = 9;
+constexpr int kSliceBuckets = 1 << kSliceLog; // 64 KB of keys and counts
+constexpr int kSliceBlock = 512;
+constexpr int kHllLog = 10; // 1024 registers of a HyperLogLog sketch
+constexpr int kHllSample = 3; // of the k-mers with hash bits 20..22 zero
+
+// Writes h = keyMix(code) of the k-mers at positions skip .. n-k of text to
+// out (append position: counters[8]), counts them in counters[1], and
+// feeds the sampled ones to the sketch `hll` (register = max rank).
+__global__ __launch_bounds__(kCountBlock) void countHashKernel(const uint8_t *__restrict__ text,
+                                                               uint64_t n, uint64_t skip,
+                                                               uint32_t k, uint64_t *out,
+                                                               uint32_t *hll,
+                                                               unsigned long long *counters)
+{
+    __shared__ __align__(16) uint8_t tile[kTileBytes];
+    __shared__ uint32_t reg[1 << kHllLog];
+    __shared__ uint64_t stage[kTilePositions];
+    __shared__ uint32_t nStage;
+    __shared__ unsigned long long stageAt;
+    for (int i = threadIdx.x; i < (1 << kHllLog); i += kCountBlock)
+        reg[i] = 0;
+    if (threadIdx.x == 0)
+        nStage = 0;
+    const uint64_t nPos = n >= k ? n - k + 1 : 0;
+    const uint64_t nTiles = (nPos + kTilePositions - 1) / kTilePositions;
+    const uint64_t mask = k == 32 ? ~0ull : (1ull << (2 * k)) - 1;
+    const uint32_t shift = 2 * (k - 1);
+    const int lane = laneId();
+    uint32_t counted = 0;
+    for (uint64_t t = blockIdx.x; t < nTiles; t += gridDim.x)
+    {
+        const uint64_t base = t * kTilePositions;
+        const uint32_t bytes = (uint32_t)min<uint64_t>(kTilePositions + k - 1, n - base);
+        __syncthreads(); // the previous tile is no longer read
+        for (uint32_t i = threadIdx.x * 16; i < bytes; i += kCountBlock * 16)
+        {
+            if (i + 16 <= bytes)
+                *reinterpret_cast<uint4 *>(tile + i) =
+                    *reinterpret_cast<const uint4 *>(text + base + i);
+            else
+                for (uint32_t j = i; j < bytes; ++j)
+                    tile[j] = text[base + j];
+        }
+        __syncthreads();
+        // positions [start, start + 32) of the tile, every lane the same
+        // number of steps so that the wave appends its k-mers together
+        const uint32_t start = threadIdx.x * kLanePositions;
+        uint64_t fwd = 0, rc = 0;
+        uint32_t run = 0;
+        for (uint32_t d = 0; d < kLanePositions + k - 1; ++d)
+        {
+            const uint32_t i = start + d;
+            const uint32_t c = i < bytes ? baseCode(tile[i]) : 4u;
+            run = c > 3 ? 0 : run + 1;
+            fwd = ((fwd << 2) | (c & 3)) & mask;
+            rc = (rc >> 2) | ((uint64_t)(3 - (c & 3)) << shift);
+            const bool emit = run >= k && base + i + 1 >= skip + k;
+            const uint64_t h = keyMix(min(fwd, rc));
+            const uint64_t m = __ballot(emit);
+            if (m)
+            {
+                uint32_t at = 0;
+                if (lane == __builtin_ctzll(m))
+                    at = atomicAdd(&nStage, (uint32_t)__popcll(m));
+                at = __shfl(at, __builtin_ctzll(m), 64);
+                if (emit)
+                {
+                    stage[at + maskedRank(m)] = h;
+                    if (((h >> 20) & ((1u << kHllSample) - 1)) == 0)
+                    {
+                        const uint32_t rank =
+                            (uint32_t)__builtin_ctzll((h >> 23) | (1ull << 40)) + 1u;
+                        atomicMax(&reg[h & ((1u << kHllLog) - 1)], rank);
+                    }
+                }
+                counted += emit;
+            }
+        }
+        // the tile's hashes go out with one global append
+        __syncthreads();
+        if (threadIdx.x == 0)
+            stageAt = nStage ? atomicAdd(counters + 8, (unsigned long long)nStage) : 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nStage; i += kCountBlock)
+            out[stageAt + i] = stage[i];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            nStage = 0;
+    }
+    counted = waveSum(counted);
+    if (lane == 0 && counted)
+        atomicAdd(counters + 1, (unsigned long long)counted);
+    __syncthreads();
+    for (int i = threadIdx.x; i < (1 << kHllLog); i += kCountBlock)
+        if (reg[i])
+            atomicMax(hll + i, reg[i]);
+}
+
+// Grouping by slice: most-significant-digit bucketing of the hashes, up to
+// kDigitBits more top bits per level (order inside a bucket is arbitrary).
+// A level's bucket of h is g = h >> (64 - bits), bits the top bits grouped
+// so far. The input of a level is grouped by its first bits - digit bits,
+// so a tile of kBucketTile consecutive elements covers few of the level's
+// buckets: a block counts its tile in an LDS window of kBucketWin buckets
+// from its first element's segment, and only buckets outside the window
+// (tiles crossing several small segments) go straight to global memory.
+constexpr int kDigitBits = 9;
+constexpr int kBucketTile = 1 << 16;
+constexpr int kBucketWin = 1024;
+constexpr int kBucketBlock = 512;
+
+__device__ inline uint64_t windowStart(const uint64_t *in, uint64_t t0, uint32_t bits,
+                                       uint32_t digit)
+{
+    return ((in[t0] >> (64 - bits)) >> digit) << digit;
+}
+
+// count[g] += elements of bucket g.
+__global__ __launch_bounds__(kBucketBlock) void bucketCountKernel(const uint64_t *__restrict__ in,
+                                                                  uint64_t n, uint32_t bits,
+                                                                  uint32_t digit,
+                                                                  unsigned long long *count)
+{
+    __shared__ uint32_t hist[kBucketWin];
+    const uint64_t t0 = (uint64_t)blockIdx.x * kBucketTile;
+    const uint64_t g0 = windowStart(in, t0, bits, digit);
+    for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
+        hist[i] = 0;
+    __syncthreads();
+    const uint64_t end = min<uint64_t>(n, t0 + kBucketTile);
+    for (uint64_t i = t0 + threadIdx.x; i < end; i += kBucketBlock)
+    {
+        const uint64_t g = in[i] >> (64 - bits);
+        if (g - g0 < (uint64_t)kBucketWin)
+            atomicAdd(&hist[g - g0], 1u);
+        else
+            atomicAdd(count + g, 1ull);
+    }
+    __syncthreads();
+    const uint64_t nBuckets = 1ull << bits;
+    for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
+        if (hist[i] && g0 + i < nBuckets)
+            atomicAdd(count + g0 + i, (unsigned long long)hist[i]);
+}
+
+// Writes every element at its bucket's cursor (cursor[g] starts at the
+// bucket's first output index): the tile is counted once in LDS, each of
+// its buckets reserves its run with one global add, and a second read of
+// the tile places every element at its run's base plus an LDS cursor.
+__global__ __launch_bounds__(kBucketBlock) void bucketScatterKernel(
+    const uint64_t *__restrict__ in, uint64_t n, uint32_t bits, uint32_t digit,
+    unsigned long long *cursor, uint64_t *__restrict__ out)
+{
+    __shared__ uint32_t hist[kBucketWin];
+    __shared__ unsigned long long base[kBucketWin];
+    const uint64_t t0 = (uint64_t)blockIdx.x * kBucketTile;
+    const uint64_t g0 = windowStart(in, t0, bits, digit);
+    const uint64_t end = min<uint64_t>(n, t0 + kBucketTile);
+    for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
+        hist[i] = 0;
+    __syncthreads();
+    for (uint64_t i = t0 + threadIdx.x; i < end; i += kBucketBlock)
+    {
+        const uint64_t g = in[i] >> (64 - bits);
+        if (g - g0 < (uint64_t)kBucketWin)
+            atomicAdd(&hist[g - g0], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
+    {
+        if (hist[i])
+            base[i] = atomicAdd(cursor + g0 + i, (unsigned long long)hist[i]);
+        hist[i] = 0;
+    }
+    __syncthreads();
+    for (uint64_t i = t0 + threadIdx.x; i < end; i += kBucketBlock)
+    {
+        const uint64_t v = in[i];
+        const uint64_t g = v >> (64 - bits);
+        const uint64_t at = g - g0 < (uint64_t)kBucketWin
+                                ? base[g - g0] + atomicAdd(&hist[g - g0], 1u)
+                                : atomicAdd(cursor + g, 1ull);
+        out[at] = v;
+    }
+}
+
+// One workgroup per slice: loads the slice's buckets into LDS (or EMPTY
+// keys and zero counts when the table is fresh), counts the slice's k-mers
+// there (the key by compare-and-swap into the first EMPTY slot from its
+// home, the count by an LDS atomic add), writes the slice back. A k-mer
+// whose probe reaches the slice's end without its key or an EMPTY slot is
+// appended to spill (counters[7]). counters[0] += keys stored.
+__global__ __launch_bounds__(kSliceBlock) void countSliceKernel(
+    const uint64_t *__restrict__ hs, const uint64_t *__restrict__ start, uint64_t *tab,
+    uint64_t nBuckets, int fresh, uint64_t *spill, unsigned long long *counters)
+{
+    __shared__ unsigned long long keys[kSliceBuckets * 8];
+    __shared__ unsigned long long cnt[kSliceBuckets * 8];
+    __shared__ uint32_t stored;
+    const uint64_t s = blockIdx.x;
+    const uint64_t lo = start[s], hi = start[s + 1];
+    if (lo >= hi)
+        return;
+    const uint64_t b0 = s << kSliceLog;
+    const uint32_t nb = (uint32_t)min<uint64_t>(kSliceBuckets, nBuckets - b0);
+    if (threadIdx.x == 0)
+        stored = 0;
+    for (uint32_t q = threadIdx.x; q < nb * kCountWords; q += kSliceBlock)
+    {
+        const uint32_t b = q / kCountWords, w = q % kCountWords;
+        const uint64_t v = fresh ? (w < 8 ? EMPTY : 0ull) : tab[(b0 + b) * kCountWords + w];
+        if (w < 8)
+            keys[b * 8 + w] = v;
+        else
+            cnt[b * 8 + w - 8] = v;
+    }
+    __syncthreads();
+    uint32_t mine = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += kSliceBlock)
+    {
+        const uint64_t h = hs[i];
+        const uint64_t key = keyUnmix(h);
+        const uint64_t home = ((h >> 32) * nBuckets) >> 32;
+        bool done = false;
+        for (uint32_t b = (uint32_t)(home - b0); b < nb && !done; ++b)
+            for (int q = 0; q < 8 && !done; ++q)
+            {
+                unsigned long long cur = keys[b * 8 + q];
+                if (cur == EMPTY)
+                {
+                    cur = atomicCAS(&keys[b * 8 + q], EMPTY, (unsigned long long)key);
+                    if (cur == EMPTY)
+                    {
+                        mine += 1;
+                        cur = key;
+                    }
+                }
+                if (cur == key)
+                {
+                    atomicAdd(&cnt[b * 8 + q], 1ull);
+                    done = true;
+                }
+            }
+        if (!done)
+            spill[atomicAdd(counters + 7, 1ull)] = h;
+    }
+    if (mine)
+        atomicAdd(&stored, mine);
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < nb * kCountWords; q += kSliceBlock)
+    {
+        const uint32_t b = q / kCountWords, w = q % kCountWords;
+        tab[(b0 + b) * kCountWords + w] = w < 8 ? keys[b * 8 + w] : cnt[b * 8 + w - 8];
+    }
+    if (threadIdx.x == 0 && stored)
+        atomicAdd(counters + 0, (unsigned long long)stored);
+}
+
+// The spilled k-mers (h form), counted in the global table.
+__global__ __launch_bounds__(256) void countSpillKernel(const uint64_t *__restrict__ spill,
+                                                        uint64_t n, uint64_t *tab,
+                                                        uint64_t nBuckets,
+                                                        unsigned long long *counters)
+{
+    uint32_t fresh = 0, full = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+    {
+        bool isNew;
+        full |= !countAdd(tab, nBuckets, keyUnmix(spill[i]), 1, &isNew);
+        fresh += isNew;
+    }
+    fresh = waveSum(fresh);
+    full = waveSum(full);
+    if ((threadIdx.x & 63) == 0)
+    {
+        if (fresh)
+            atomicAdd(counters + 0, (unsigned long long)fresh);
+        if (full)
+            atomicExch(counters + 2, 1ull);
+    }
+}
+
 __global__ __launch_bounds__(256) void countFillKernel(uint64_t *tab, uint64_t words, int keyWords)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
@@ -402,6 +706,61 @@ uint32_t gridFor(uint64_t items, uint64_t perBlock, int device)
 
 using namespace hipibs;
 
+namespace
+{
+// The hash buffers of sorted-slice counting are reused by the process's
+// next counter: the largest pair a freed counter returns is kept (one pair
+// per process; hipibs_counter_release_cache frees it). Allocating and
+// freeing tens of GB per counter can cost the runtime seconds.
+struct HashCache
+{
+    std::mutex m;
+    int device = -1;
+    uint64_t *p[2] = {nullptr, nullptr};
+    uint64_t cap = 0;
+} gHashCache;
+
+// Takes the cached pair if it is on `device` and holds `need` entries.
+bool takeCachedHash(int device, uint64_t need, uint64_t *out[2], uint64_t *cap)
+{
+    std::lock_guard<std::mutex> g(gHashCache.m);
+    if (!gHashCache.p[0] || gHashCache.device != device || gHashCache.cap < need)
+        return false;
+    out[0] = gHashCache.p[0];
+    out[1] = gHashCache.p[1];
+    *cap = gHashCache.cap;
+    gHashCache.p[0] = gHashCache.p[1] = nullptr;
+    gHashCache.cap = 0;
+    return true;
+}
+
+// Keeps a freed counter's pair if it is larger than the cached one (the
+// smaller pair is freed); called with the pair's device current.
+void returnHash(int device, uint64_t *p[2], uint64_t cap)
+{
+    std::lock_guard<std::mutex> g(gHashCache.m);
+    if (gHashCache.p[0] && gHashCache.cap >= cap)
+    {
+        for (int i = 0; i < 2; ++i)
+            (void)hipFree(p[i]);
+        return;
+    }
+    if (gHashCache.p[0])
+    {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(gHashCache.device);
+        for (int i = 0; i < 2; ++i)
+            (void)hipFree(gHashCache.p[i]);
+        (void)hipSetDevice(cur);
+    }
+    gHashCache.device = device;
+    gHashCache.p[0] = p[0];
+    gHashCache.p[1] = p[1];
+    gHashCache.cap = cap;
+}
+} // namespace
+
 struct hipibs_counter : ReadSink
 {
     int device = 0;
@@ -412,14 +771,32 @@ struct hipibs_counter : ReadSink
     uint64_t *dTab = nullptr;
     uint64_t nBuckets = 0;
     unsigned long long *dCounters = nullptr; // [0] distinct, [1] counted, [2] full, [3..5] filter,
-                                             // [6] count of A^32 T^32 (k = 64)
+                                             // [6] count of A^32 T^32 (k = 64), [7] spilled
+                                             // and [8] hashed k-mers of the open flush
     unsigned long long *hCounters = nullptr; // pinned copy
+    // sorted-slice counting (k <= 32): the hashed k-mers of the open flush
+    // (dHash[0]; grouped by slice through both buffers, the other one then
+    // takes the spills)
+    uint64_t *dHash[2] = {nullptr, nullptr};
+    uint64_t hashCap = 0;    // entries of each buffer
+    uint64_t hashBudget = 0; // most entries one flush holds
+    uint64_t nHash = 0;
+    uint32_t *dHll = nullptr; // the flush's distinct-count sketch
+    uint64_t *dStart = nullptr;
+    uint64_t startCap = 0;
+    hipEvent_t fev[2] = {nullptr, nullptr}; // around a flush's sort and slice counts
     char *hText[2] = {nullptr, nullptr};     // pinned chunks (host text only)
     uint8_t *dText = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr}; // around the last count launch
     bool evPending = false;
     double kernelMs = 0;                   // device time of the count launches
     uint64_t launches = 0;
+    uint64_t spilled = 0, flushes = 0;     // sorted-slice counting
+    // new keys a flush sizes the table for, per distinct k-mer the sketch
+    // estimates (environment HIPIBS_COUNT_SKETCH_SCALE; a small value
+    // undersizes the table so that most k-mers take the spill path, which
+    // the tests use)
+    double sketchScale = 1.15;
     int cur = 0;          // chunk being filled
     uint64_t len = 0;     // bytes in it
     uint64_t recLen = 0;  // bases of the open record so far
@@ -483,11 +860,239 @@ struct hipibs_counter : ReadSink
         const uint64_t positions = n >= k ? n - k + 1 : 0;
         if (!positions)
             return 0;
+        if (!wide)
+        {
+            HIPIBS_CHECK(hipMemcpyAsync(dText, hText[slot], n, hipMemcpyHostToDevice, stream));
+            return appendHashes(dText, n, 0);
+        }
         rc = reserve(distinct + positions);
         if (rc)
             return rc;
         HIPIBS_CHECK(hipMemcpyAsync(dText, hText[slot], n, hipMemcpyHostToDevice, stream));
         return launchCount(dText, n, 0);
+    }
+
+    // k <= 32: hashes the k-mers at positions skip .. n-k of device text
+    // into the open flush (called with the stream synchronised), making
+    // room first: the buffers grow up to hashBudget entries, beyond that
+    // the open flush is counted.
+    int appendHashes(const uint8_t *text, uint64_t n, uint64_t skip)
+    {
+        const uint64_t positions = n >= k + skip ? n - k + 1 - skip : 0;
+        if (!positions)
+            return 0;
+        int rc = 0;
+        if ((rc = budget()))
+            return rc;
+        if (positions > hashBudget)
+            return setError(HIPIBS_E_NOMEM, "k-mer counter: a text piece exceeds the flush budget");
+        if (nHash + positions > hashBudget && (rc = flushSorted()))
+            return rc;
+        if ((rc = hashRoom(nHash + positions)))
+            return rc;
+        if (!dHll)
+        {
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dHll), (4u << kHllLog)));
+            HIPIBS_CHECK(hipMemsetAsync(dHll, 0, 4u << kHllLog, stream));
+        }
+        HIPIBS_CHECK(hipEventRecord(ev[0], stream));
+        countHashKernel<<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0, stream>>>(
+            text, n, skip, k, dHash[0], dHll, dCounters);
+        HIPIBS_CHECK(hipGetLastError());
+        HIPIBS_CHECK(hipEventRecord(ev[1], stream));
+        evPending = true;
+        launches += 1;
+        HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 9 * sizeof(unsigned long long),
+                                    hipMemcpyDeviceToHost, stream));
+        nHash += positions; // an upper bound until the next sync
+        return 0;
+    }
+
+    // The most hashed k-mers one flush holds: 16 read chunks (2 Gi k-mers
+    // at the default chunk), both buffers within a quarter of the free
+    // memory, at least one chunk.
+    int budget()
+    {
+        if (hashBudget)
+            return 0;
+        size_t freeB = 0, totalB = 0;
+        HIPIBS_CHECK(hipMemGetInfo(&freeB, &totalB));
+        hashBudget = std::min<uint64_t>(16 * chunkBytes, freeB / 4 / 16);
+        hashBudget = std::max<uint64_t>(hashBudget, chunkBytes);
+        return 0;
+    }
+
+    // Grows the hash buffers to hold `need` entries (at most the budget),
+    // keeping the open flush's; called with the stream synchronised.
+    int hashRoom(uint64_t need)
+    {
+        if (need <= hashCap)
+            return 0;
+        if (!hashCap && takeCachedHash(device, need, dHash, &hashCap))
+            return 0;
+        {
+            const uint64_t want = std::min<uint64_t>(
+                hashBudget, std::max<uint64_t>({need, 2 * hashCap, 1ull << 22}));
+            uint64_t *nb[2] = {nullptr, nullptr};
+            for (int i = 0; i < 2; ++i)
+                if (hipMalloc(reinterpret_cast<void **>(&nb[i]), want * 8) != hipSuccess)
+                {
+                    (void)hipGetLastError();
+                    if (nb[0])
+                        (void)hipFree(nb[0]);
+                    return setError(HIPIBS_E_NOMEM, "k-mer counter: no device memory for " +
+                                                        std::to_string(want) + " hashed k-mers");
+                }
+            if (nHash)
+                HIPIBS_CHECK(hipMemcpyAsync(nb[0], dHash[0], nHash * 8, hipMemcpyDeviceToDevice,
+                                            stream));
+            HIPIBS_CHECK(hipStreamSynchronize(stream));
+            for (int i = 0; i < 2; ++i)
+            {
+                if (dHash[i])
+                    HIPIBS_CHECK(hipFree(dHash[i]));
+                dHash[i] = nb[i];
+            }
+            hashCap = want;
+        }
+        return 0;
+    }
+
+    // Distinct values among the hashed k-mers of the open flush, from the
+    // sketch (HyperLogLog over the 1/8 of them with hash bits 20..22 zero;
+    // linear counting while registers are empty).
+    double sketchEstimate()
+    {
+        uint32_t reg[1 << kHllLog];
+        if (hipMemcpy(reg, dHll, sizeof(reg), hipMemcpyDeviceToHost) != hipSuccess)
+            return -1;
+        const double m = (double)(1 << kHllLog);
+        double sum = 0;
+        int zeros = 0;
+        for (uint32_t r : reg)
+        {
+            sum += std::ldexp(1.0, -(int)r);
+            zeros += r == 0;
+        }
+        double e = 0.7213 / (1.0 + 1.079 / m) * m * m / sum;
+        if (e <= 2.5 * m && zeros)
+            e = m * std::log(m / zeros);
+        return e * (1 << kHllSample);
+    }
+
+    // Counts the open flush: sizes the table from the sketch, sorts the
+    // hashes by slice, counts every slice in LDS, then the spills in the
+    // global table (after growing it for them).
+    int flushSorted()
+    {
+        int rc = sync();
+        if (rc || wide)
+            return rc;
+        nHash = hCounters[8];
+        if (!nHash)
+            return 0;
+        const double est = sketchEstimate();
+        if (est < 0)
+            return setError(HIPIBS_E_HIP, "k-mer counter: reading the sketch failed");
+        const uint64_t newKeys = std::min<uint64_t>(nHash, (uint64_t)(est * sketchScale) + 4096);
+        if ((rc = reserve(distinct + newKeys)))
+            return rc;
+        const int lg = 63 - __builtin_clzll(nBuckets); // a power of two
+        const uint32_t sbits = lg > kSliceLog ? (uint32_t)(lg - kSliceLog) : 0u;
+        const uint64_t nSlices = sbits ? (1ull << sbits) : 1;
+        if (startCap < nSlices + 1)
+        {
+            if (dStart)
+                HIPIBS_CHECK(hipFree(dStart));
+            dStart = nullptr;
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dStart), (nSlices + 1) * 8));
+            startCap = nSlices + 1;
+        }
+        HIPIBS_CHECK(hipEventRecord(fev[0], stream));
+        // group the hashes by slice: levels of up to kDigitBits top bits,
+        // ping-ponging between the two buffers; the last level's bucket
+        // offsets are the slices' starts
+        uint64_t *src = dHash[0], *dst = dHash[1];
+        // sized for the last level: the resizes below never reallocate while
+        // a copy from it may be in flight
+        std::vector<uint64_t> offs(nSlices + 1, 0);
+        offs[nSlices] = nHash;
+        for (uint32_t bits = 0; bits < sbits;)
+        {
+            const uint32_t digit = std::min<uint32_t>(kDigitBits, sbits - bits);
+            bits += digit;
+            const uint64_t nb = 1ull << bits;
+            const uint32_t tiles = (uint32_t)((nHash + kBucketTile - 1) / kBucketTile);
+            HIPIBS_CHECK(hipMemsetAsync(dStart, 0, nb * 8, stream));
+            bucketCountKernel<<<tiles, kBucketBlock, 0, stream>>>(
+                src, nHash, bits, digit, reinterpret_cast<unsigned long long *>(dStart));
+            HIPIBS_CHECK(hipGetLastError());
+            offs.resize(nb + 1);
+            HIPIBS_CHECK(hipMemcpyAsync(offs.data(), dStart, nb * 8, hipMemcpyDeviceToHost, stream));
+            HIPIBS_CHECK(hipStreamSynchronize(stream));
+            uint64_t acc = 0;
+            for (uint64_t g = 0; g < nb; ++g)
+            {
+                const uint64_t c = offs[g];
+                offs[g] = acc;
+                acc += c;
+            }
+            offs[nb] = acc;
+            if (acc != nHash)
+                return setError(HIPIBS_E_HIP, "k-mer counter: slice grouping lost k-mers");
+            HIPIBS_CHECK(hipMemcpyAsync(dStart, offs.data(), nb * 8, hipMemcpyHostToDevice, stream));
+            bucketScatterKernel<<<tiles, kBucketBlock, 0, stream>>>(
+                src, nHash, bits, digit, reinterpret_cast<unsigned long long *>(dStart), dst);
+            HIPIBS_CHECK(hipGetLastError());
+            std::swap(src, dst);
+            launches += 2;
+        }
+        // the grouped hashes are in src; dst takes the spills
+        const uint64_t *sorted = src;
+        uint64_t *other = dst;
+        HIPIBS_CHECK(hipMemcpyAsync(dStart, offs.data(), (nSlices + 1) * 8, hipMemcpyHostToDevice,
+                                    stream));
+        countSliceKernel<<<(uint32_t)nSlices, kSliceBlock, 0, stream>>>(
+            sorted, dStart, dTab, nBuckets, distinct == 0, other, dCounters);
+        HIPIBS_CHECK(hipGetLastError());
+        HIPIBS_CHECK(hipEventRecord(fev[1], stream));
+        launches += 1;
+        HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 9 * sizeof(unsigned long long),
+                                    hipMemcpyDeviceToHost, stream));
+        HIPIBS_CHECK(hipStreamSynchronize(stream));
+        float ms = 0;
+        HIPIBS_CHECK(hipEventElapsedTime(&ms, fev[0], fev[1]));
+        kernelMs += ms;
+        distinct = hCounters[0];
+        const uint64_t spills = hCounters[7];
+        if (spills)
+        {
+            // the spills sit in `other`, which a growth leaves in place
+            if ((rc = reserve(distinct + spills)))
+                return rc;
+            HIPIBS_CHECK(hipEventRecord(fev[0], stream));
+            countSpillKernel<<<gridFor(spills, 256 * 4, device), 256, 0, stream>>>(
+                other, spills, dTab, nBuckets, dCounters);
+            HIPIBS_CHECK(hipGetLastError());
+            HIPIBS_CHECK(hipEventRecord(fev[1], stream));
+            launches += 1;
+            HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 9 * sizeof(unsigned long long),
+                                        hipMemcpyDeviceToHost, stream));
+            HIPIBS_CHECK(hipStreamSynchronize(stream));
+            HIPIBS_CHECK(hipEventElapsedTime(&ms, fev[0], fev[1]));
+            kernelMs += ms;
+            if (hCounters[2])
+                return setError(HIPIBS_E_FULL, "k-mer counter: an insert found no free slot");
+            distinct = hCounters[0];
+        }
+        spilled += spills;
+        HIPIBS_CHECK(hipMemsetAsync(dCounters + 7, 0, 2 * sizeof(unsigned long long), stream));
+        HIPIBS_CHECK(hipMemsetAsync(dHll, 0, 4u << kHllLog, stream));
+        HIPIBS_CHECK(hipStreamSynchronize(stream));
+        hCounters[7] = hCounters[8] = 0;
+        nHash = 0;
+        flushes += 1;
+        return 0;
     }
 
     // Counts positions skip .. n-k of device text (room reserved).
@@ -514,6 +1119,17 @@ struct hipibs_counter : ReadSink
     // skips the positions that piece counted.
     int countDevice(const uint8_t *text, uint64_t n)
     {
+        if (!wide && n >= k)
+        {
+            // room for the whole text's k-mers at once (within the budget)
+            int rc = sync();
+            if (!rc)
+                rc = budget();
+            if (!rc)
+                rc = hashRoom(std::min<uint64_t>(hashBudget, nHash + (n - k + 1)));
+            if (rc)
+                return rc;
+        }
         uint64_t off = 0, from = 0; // piece start, first position to count
         while (n >= k && from <= n - k)
         {
@@ -521,10 +1137,15 @@ struct hipibs_counter : ReadSink
             if (len < k || off + len - k + 1 <= from)
                 break;
             int rc = sync();
-            if (!rc)
-                rc = reserve(distinct + (off + len - k + 1 - from));
-            if (!rc)
-                rc = launchCount(text + off, len, from - off);
+            if (!rc && !wide)
+                rc = appendHashes(text + off, len, from - off);
+            else
+            {
+                if (!rc)
+                    rc = reserve(distinct + (off + len - k + 1 - from));
+                if (!rc)
+                    rc = launchCount(text + off, len, from - off);
+            }
             if (rc)
                 return rc;
             if (off + len == n)
@@ -562,6 +1183,8 @@ struct hipibs_counter : ReadSink
         uint64_t want = (uint64_t)((double)need / kGrowLoad) / slots + 1;
         want = std::max<uint64_t>(want, 2 * nBuckets);
         want = std::max<uint64_t>(want, 1024);
+        if (!wide) // a power of two: slices of the sorted-slice count
+            want = 1ull << (64 - __builtin_clzll(want - 1));
         if (want >= (1ull << 32))
             return setError(HIPIBS_E_NOMEM, "k-mer counter: more than 2^32 buckets needed");
         uint64_t *t = nullptr;
@@ -607,7 +1230,7 @@ struct hipibs_counter : ReadSink
     }
 
     // Submits what is buffered and waits for every chunk.
-    int finish()
+    int drain()
     {
         if (err)
             return err;
@@ -621,6 +1244,14 @@ struct hipibs_counter : ReadSink
                 return err;
         }
         return err = sync();
+    }
+
+    // drain, then counts the open flush.
+    int finish()
+    {
+        if (drain())
+            return err;
+        return err = flushSorted();
     }
 
     // Keys with lo <= count <= hi into dKeys (two words each for k > 32;
@@ -708,6 +1339,8 @@ extern "C" int hipibs_counter_create(uint32_t k, int device, uint64_t expected_k
     c->k = k;
     c->wide = k > 32;
     c->chunkBytes = chunk_bytes;
+    if (const char *sc = std::getenv("HIPIBS_COUNT_SKETCH_SCALE"))
+        c->sketchScale = std::min(100.0, std::max(0.0, std::atof(sc)));
     auto fail = [c](int rc)
     {
         hipibs_counter_free(c);
@@ -715,17 +1348,19 @@ extern "C" int hipibs_counter_create(uint32_t k, int device, uint64_t expected_k
     };
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess)
-        e = hipMalloc(reinterpret_cast<void **>(&c->dCounters), 8 * sizeof(unsigned long long));
+        e = hipMalloc(reinterpret_cast<void **>(&c->dCounters), 16 * sizeof(unsigned long long));
     if (e == hipSuccess)
-        e = hipHostMalloc(reinterpret_cast<void **>(&c->hCounters), 8 * sizeof(unsigned long long),
+        e = hipHostMalloc(reinterpret_cast<void **>(&c->hCounters), 16 * sizeof(unsigned long long),
                           hipHostMallocDefault);
     for (int i = 0; i < 2 && e == hipSuccess; ++i)
         e = hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i)
+        e = hipEventCreate(&c->fev[i]);
     if (e == hipSuccess)
-        e = hipMemset(c->dCounters, 0, 8 * sizeof(unsigned long long));
+        e = hipMemset(c->dCounters, 0, 16 * sizeof(unsigned long long));
     if (e != hipSuccess)
         return fail(hipFail(e, "k-mer counter buffers"));
-    std::memset(c->hCounters, 0, 8 * sizeof(unsigned long long));
+    std::memset(c->hCounters, 0, 16 * sizeof(unsigned long long));
     if (expected_keys)
     {
         const int rc = c->reserve(expected_keys);
@@ -759,7 +1394,7 @@ extern "C" int hipibs_counter_add_text_device(hipibs_counter *c, const char *d_t
     if (c->err)
         return setError(c->err, "k-mer counter: an earlier call failed");
     HIPIBS_CHECK(hipSetDevice(c->device));
-    int rc = c->finish(); // text buffered by earlier calls goes first
+    int rc = c->drain(); // text buffered by earlier calls goes first
     if (!rc)
         rc = c->countDevice(reinterpret_cast<const uint8_t *>(d_text), n);
     if (rc)
@@ -796,6 +1431,8 @@ extern "C" int hipibs_counter_stats(hipibs_counter *c, hipibs_count_stats *out)
     out->distinct = c->distinct;
     out->kernel_ms = c->kernelMs;
     out->launches = c->launches;
+    out->flushes = c->flushes;
+    out->spilled = c->spilled;
     out->table_bytes = c->nBuckets * kCountWords * 8;
     return 0;
 }
@@ -948,6 +1585,12 @@ extern "C" void hipibs_counter_free(hipibs_counter *c)
         (void)hipFree(c->dCounters);
     if (c->dText)
         (void)hipFree(c->dText);
+    if (c->dHash[0])
+        returnHash(c->device, c->dHash, c->hashCap);
+    if (c->dHll)
+        (void)hipFree(c->dHll);
+    if (c->dStart)
+        (void)hipFree(c->dStart);
     for (char *h : c->hText)
         if (h)
             (void)hipHostFree(h);
@@ -956,7 +1599,25 @@ extern "C" void hipibs_counter_free(hipibs_counter *c)
     for (hipEvent_t e : c->ev)
         if (e)
             (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->fev)
+        if (e)
+            (void)hipEventDestroy(e);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+extern "C" void hipibs_counter_release_cache(void)
+{
+    std::lock_guard<std::mutex> g(gHashCache.m);
+    if (!gHashCache.p[0])
+        return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(gHashCache.device);
+    for (int i = 0; i < 2; ++i)
+        (void)hipFree(gHashCache.p[i]);
+    (void)hipSetDevice(cur);
+    gHashCache.p[0] = gHashCache.p[1] = nullptr;
+    gHashCache.cap = 0;
 }

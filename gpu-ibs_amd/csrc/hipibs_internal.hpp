@@ -28,14 +28,40 @@ constexpr uint64_t EMPTY = ~0ull;
 // (high, low) word pair).
 typedef unsigned __int128 Wide;
 
-// Bucket of a key: a multiply-xorshift mix, then the high 32 bits scaled
-// onto [0, n_buckets) (n_buckets < 2^32).
+// The mix behind bucketOf: multiply-xorshift steps, each invertible, so
+// keyUnmix recovers the key.
+constexpr uint64_t kMixA = 0x9E3779B97F4A7C15ull, kMixB = 0xBF58476D1CE4E5B9ull;
+__host__ __device__ inline uint64_t keyMix(uint64_t key)
+{
+    uint64_t h = (key ^ (key >> 31)) * kMixA;
+    h ^= h >> 29;
+    return h * kMixB;
+}
+
+// Multiplicative inverse of an odd 64-bit constant (Newton's iteration,
+// each step doubles the correct low bits: 3 -> 96).
+constexpr uint64_t inverseOdd(uint64_t a)
+{
+    uint64_t x = a;
+    for (int i = 0; i < 5; ++i)
+        x *= 2 - a * x;
+    return x;
+}
+
+__host__ __device__ inline uint64_t keyUnmix(uint64_t h)
+{
+    constexpr uint64_t invA = inverseOdd(kMixA), invB = inverseOdd(kMixB);
+    uint64_t x = h * invB;
+    x ^= (x >> 29) ^ (x >> 58); // inverse of x ^= x >> 29
+    x *= invA;
+    return x ^ (x >> 31) ^ (x >> 62); // inverse of x ^= x >> 31
+}
+
+// Bucket of a key: the high 32 bits of keyMix scaled onto [0, n_buckets)
+// (n_buckets < 2^32).
 __host__ __device__ inline uint64_t bucketOf(uint64_t key, uint64_t nBuckets)
 {
-    uint64_t h = (key ^ (key >> 31)) * 0x9E3779B97F4A7C15ull;
-    h ^= h >> 29;
-    h *= 0xBF58476D1CE4E5B9ull;
-    return ((h >> 32) * nBuckets) >> 32;
+    return ((keyMix(key) >> 32) * nBuckets) >> 32;
 }
 
 // Bucket of a 33..64-base key (high word, low word).

@@ -71,7 +71,8 @@ class ServeCounters(C.Structure):
 
 class CountStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("reads", "kmers", "distinct", "table_bytes",
-                                         "launches")] + [("kernel_ms", C.c_double)]
+                                         "launches")] + [("kernel_ms", C.c_double)] + \
+        [(n, C.c_uint64) for n in ("flushes", "spilled")]
 
 
 class CountFilter(C.Structure):
@@ -123,6 +124,7 @@ _SIGS = [
     ("hipibs_counter_write_kmc", C.c_int, [vp, C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                            C.POINTER(CountFilter)]),
     ("hipibs_counter_free", None, [vp]),
+    ("hipibs_counter_release_cache", None, []),
     ("hipibs_coverage_text_device", C.c_int, [vp, C.c_uint64, vp, C.c_uint64, u64p, vp]),
     ("hipibs_coverage_text_stream", C.c_int, [vp, C.c_uint64, vp, vp, vp]),
     ("hipibs_scan_windows_device", C.c_int, [C.POINTER(vp), C.c_int, vp, vp, C.c_uint64, vp,

@@ -231,6 +231,9 @@ typedef struct hipibs_count_stats {
     uint64_t table_bytes; /* device bytes of the counting table */
     uint64_t launches;    /* counting kernel launches */
     double kernel_ms;     /* their device time (HIP events on the counter's stream) */
+    uint64_t flushes;     /* k <= 32: sorted-slice counting rounds */
+    uint64_t spilled;     /* k <= 32: k-mers whose probe left their 512-bucket slice and
+                             were counted in the global table */
 } hipibs_count_stats;
 
 typedef struct hipibs_count_filter {
@@ -278,7 +281,13 @@ int hipibs_counter_build_table(hipibs_counter *c, uint32_t min_count, uint32_t m
  * KmcRaIndex and hipibs_table_build_kmc. filt may be NULL. */
 int hipibs_counter_write_kmc(hipibs_counter *c, const char *prefix, uint32_t min_count,
                              uint32_t max_count, uint32_t counter_max, hipibs_count_filter *filt);
+/* Frees the counter. For k <= 32 its two buffers of hashed k-mers (8 B per
+ * k-mer of one counting round, up to 16 chunks' worth) are kept for the
+ * process's next counter on the same device: the largest pair freed so far,
+ * one pair per process. */
 void hipibs_counter_free(hipibs_counter *c);
+/* Frees the kept pair of hashed k-mer buffers (no counter needs to exist). */
+void hipibs_counter_release_cache(void);
 
 /* ---- the fused scan (device-resident inputs) ------------------------------ */
 

@@ -136,6 +136,8 @@ def test_chunk_seams_and_growth(tmp_path, chunk):
     c.add_file(str(path))
     st = c.stats()
     assert st["kmers"] == total and st["distinct"] == want_k.size
+    # a table sized from the sketch: probes rarely leave their slice
+    assert st["spilled"] <= total // 50, st
     assert st["reads"] == 6  # the empty record counts as a read with no k-mers
     keys, counts, _ = c.keys(1, ALL, ALL)
     keys, counts = sorted_pairs(keys, counts)
@@ -185,7 +187,11 @@ def test_device_text_equals_oracle(small_reads, chunk):
     buf = hipibs.DeviceBuffer.from_array(np.frombuffer(text, np.uint8))
     c = hipibs.KmerCounter(25, chunk_bytes=chunk)
     c.add_text_device(buf.ptr, len(text))
-    assert c.stats()["kmers"] == int(valid.sum())
+    st = c.stats()
+    assert st["kmers"] == int(valid.sum())
+    # a flush holds 16 chunks of k-mers: small chunks count in many rounds
+    # into a table that already holds keys
+    assert (st["flushes"] > 100) if chunk == 64 else (st["flushes"] >= 1)
     keys, counts, _ = c.keys(1, ALL, ALL)
     keys, counts = sorted_pairs(keys, counts)
     np.testing.assert_array_equal(keys, want_k)
@@ -194,6 +200,34 @@ def test_device_text_equals_oracle(small_reads, chunk):
         c.add_text_device(buf.ptr + 3, 100)
     c.free()
     buf.free()
+
+
+@pytest.mark.parametrize("chunk", [1000, 0])
+def test_spilled_kmers_counted_in_global_table(small_reads, monkeypatch, chunk):
+    """A table sized far below the sketch's estimate: most k-mers' probes
+    leave their slice and are counted by the spill path in the global
+    table, which grows for them; the counts stay exact. With 1000 B chunks
+    the spills of later rounds meet keys stored by earlier ones."""
+    _, reads = small_reads
+    r = reads[1][:4000]
+    text = b"\n".join(corpus.to_text(x) for x in r)
+    want_k, want_c = corpus.count_kmers(r, 25, ci=1, cs=ALL)
+    monkeypatch.setenv("HIPIBS_COUNT_SKETCH_SCALE", "0")
+    c = hipibs.KmerCounter(25, chunk_bytes=chunk)
+    monkeypatch.delenv("HIPIBS_COUNT_SKETCH_SCALE")
+    half = len(text) // 2
+    half = text.index(b"\n", half)
+    c.add_text(text[:half])
+    c.add_text(text[half + 1:])
+    st = c.stats()
+    note(f"spill test chunk {chunk}: {st}")
+    assert st["spilled"] > (st["kmers"] // 4 if chunk == 0 else 0)
+    keys, counts, _ = c.keys(1, ALL, ALL)
+    keys, counts = sorted_pairs(keys, counts)
+    np.testing.assert_array_equal(keys, want_k)
+    np.testing.assert_array_equal(counts, want_c)
+    assert st["distinct"] == want_k.size
+    c.free()
 
 
 def test_filters(small_reads):
