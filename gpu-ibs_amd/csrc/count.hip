@@ -430,15 +430,20 @@ __global__ __launch_bounds__(kBucketBlock) void bucketCountKernel(const uint64_t
 }
 
 // Writes every element at its bucket's cursor (cursor[g] starts at the
-// bucket's first output index): the tile is counted once in LDS, each of
-// its buckets reserves its run with one global add, and a second read of
-// the tile places every element at its run's base plus an LDS cursor.
+// bucket's first output index). The tile is counted once in LDS and each of
+// its buckets reserves its run with one global add; then sub-chunks of
+// kStageElems elements are ordered by bucket in LDS (a counting sort) and
+// written out, so that consecutive threads store consecutive addresses of
+// a run instead of single elements scattered over all of the tile's runs.
+constexpr int kStageElems = 8192;
 __global__ __launch_bounds__(kBucketBlock) void bucketScatterKernel(
     const uint64_t *__restrict__ in, uint64_t n, uint32_t bits, uint32_t digit,
     unsigned long long *cursor, uint64_t *__restrict__ out)
 {
-    __shared__ uint32_t hist[kBucketWin];
-    __shared__ unsigned long long base[kBucketWin];
+    __shared__ uint32_t hist[kBucketWin];  // tile counts, then per sub-chunk counts
+    __shared__ uint32_t start[kBucketWin]; // a sub-chunk's bucket starts in stage
+    __shared__ unsigned long long base[kBucketWin]; // next output index per bucket
+    __shared__ uint64_t stage[kStageElems];
     const uint64_t t0 = (uint64_t)blockIdx.x * kBucketTile;
     const uint64_t g0 = windowStart(in, t0, bits, digit);
     const uint64_t end = min<uint64_t>(n, t0 + kBucketTile);
@@ -459,14 +464,70 @@ __global__ __launch_bounds__(kBucketBlock) void bucketScatterKernel(
         hist[i] = 0;
     }
     __syncthreads();
-    for (uint64_t i = t0 + threadIdx.x; i < end; i += kBucketBlock)
+    constexpr int kPer = kStageElems / kBucketBlock;
+    for (uint64_t c0 = t0; c0 < end; c0 += kStageElems)
     {
-        const uint64_t v = in[i];
-        const uint64_t g = v >> (64 - bits);
-        const uint64_t at = g - g0 < (uint64_t)kBucketWin
-                                ? base[g - g0] + atomicAdd(&hist[g - g0], 1u)
-                                : atomicAdd(cursor + g, 1ull);
-        out[at] = v;
+        const uint32_t m = (uint32_t)min<uint64_t>(kStageElems, end - c0);
+        uint64_t v[kPer];
+        uint32_t r[kPer];
+        // ranks within the sub-chunk's buckets; outside the window: direct
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+        {
+            const uint32_t e = threadIdx.x + (uint32_t)j * kBucketBlock;
+            r[j] = 0xffffffffu;
+            if (e < m)
+            {
+                v[j] = in[c0 + e];
+                const uint64_t g = v[j] >> (64 - bits);
+                if (g - g0 < (uint64_t)kBucketWin)
+                    r[j] = atomicAdd(&hist[g - g0], 1u);
+                else
+                    out[atomicAdd(cursor + g, 1ull)] = v[j];
+            }
+        }
+        __syncthreads();
+        // exclusive scan of the sub-chunk's counts (kBucketWin = 2 x block)
+        {
+            const int a = 2 * threadIdx.x;
+            const uint32_t x0 = hist[a], x1 = hist[a + 1];
+            uint32_t sum = x0 + x1;
+            // inclusive scan of the pair sums across the block via LDS
+            start[threadIdx.x] = sum;
+            __syncthreads();
+            for (int off = 1; off < kBucketBlock; off <<= 1)
+            {
+                const uint32_t add = threadIdx.x >= off ? start[threadIdx.x - off] : 0u;
+                __syncthreads();
+                start[threadIdx.x] += add;
+                __syncthreads();
+            }
+            const uint32_t incl = start[threadIdx.x];
+            __syncthreads();
+            start[a] = incl - sum;
+            start[a + 1] = incl - sum + x0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+            if (r[j] != 0xffffffffu)
+                stage[start[(v[j] >> (64 - bits)) - g0] + r[j]] = v[j];
+        __syncthreads();
+        // staged elements of one bucket are adjacent: their stores are too
+        const uint32_t placed = start[kBucketWin - 1] + hist[kBucketWin - 1];
+        for (uint32_t e = threadIdx.x; e < placed; e += kBucketBlock)
+        {
+            const uint64_t x = stage[e];
+            const uint32_t b = (uint32_t)((x >> (64 - bits)) - g0);
+            out[base[b] + (e - start[b])] = x;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
+        {
+            base[i] += hist[i];
+            hist[i] = 0;
+        }
+        __syncthreads();
     }
 }
 
