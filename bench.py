@@ -1257,8 +1257,23 @@ def run_count(args):
     st = stats[-1]
     stable = all((x["kmers"], x["distinct"]) == (st["kmers"], st["distinct"]) for x in stats)
     k_ms = float(np.mean([x["kernel_ms"] for x in stats]))
+    stage_ms = np.mean(np.array([x["stage_ms"] for x in stats]), axis=0)
     launches = st["launches"]
-    alg_bytes = 80.0 * st["kmers"] + n  # per k-mer: 64 B of keys + the 8 B count read and written
+    # algorithmic bytes per stage and step (one counting round into a fresh
+    # table): hashing reads the text and writes 8 B per k-mer; each
+    # bucketing level reads and writes 8 B per k-mer (levels: 9 top bits of
+    # the 2^(log2(buckets) - 9) slices each); the slice counts read 8 B per
+    # k-mer and write the table once
+    nb = st["table_bytes"] // 128
+    levels = -(-max(0, nb.bit_length() - 1 - 9) // 9)
+    stage_names = ["countHashKernel", "bucketCountKernel + bucketScatterKernel",
+                   "countSliceKernel", "countSpillKernel"]
+    stage_bytes = [n + 8.0 * st["kmers"], 16.0 * st["kmers"] * levels,
+                   8.0 * st["kmers"] + st["table_bytes"], 0.0]
+    dom = int(np.argmax(stage_ms))
+    alg_bytes = stage_bytes[dom]
+    log(f"count stages (ms): hash {stage_ms[0]:.1f}, group ({levels} levels) {stage_ms[1]:.1f}, "
+        f"slices {stage_ms[2]:.1f}, spills {stage_ms[3]:.2f}")
     log(f"count: {ms:.2f} ms/step, kernels {k_ms:.2f} ms in {launches} launches, "
         f"{st['kmers']/1e9:.3f} G k-mers, {st['distinct']/1e6:.1f} M distinct, "
         f"table {st['table_bytes']/1e9:.1f} GB, stable {stable}")
@@ -1330,9 +1345,11 @@ def run_count(args):
     tpath = os.path.join(ROOT, "profiles", "count_k25_200mbp_traffic.json")
     if os.path.exists(tpath) and G == 200_000_000:
         with open(tpath) as f:
-            traffic = json.load(f)["traffic_bytes_per_dispatch"]
+            tj = json.load(f)
+        if tj.get("stage") == stage_names[dom]:
+            traffic = tj["traffic_bytes_per_step"]
     value = st["kmers"] / (ms / 1e3)
-    achieved = alg_bytes / (k_ms / 1e3) / 1e9
+    achieved = alg_bytes / (stage_ms[dom] / 1e3) / 1e9
     out = {
         "metric": "k-mers counted/s",
         "value": value,
@@ -1356,10 +1373,13 @@ def run_count(args):
                    "kmc_end_to_end": e2e},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "countKernel", "kernel_ms_per_step": k_ms,
-                     "launches_per_step": launches,
+                     "kernel": stage_names[dom], "kernel_ms_per_step": float(stage_ms[dom]),
                      "algorithmic_bytes": alg_bytes,
-                     "kmers_per_s_in_kernel": st["kmers"] / (k_ms / 1e3)},
+                     "stages": [{"kernel": nm, "ms": float(t), "algorithmic_bytes": b,
+                                 "achieved_gbs": b / (t / 1e3) / 1e9 if t > 0 else None}
+                                for nm, t, b in zip(stage_names, stage_ms, stage_bytes)],
+                     "all_kernels_ms_per_step": k_ms, "launches_per_step": launches,
+                     "kmers_per_s_in_kernels": st["kmers"] / (k_ms / 1e3)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

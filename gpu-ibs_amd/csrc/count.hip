@@ -845,12 +845,16 @@ struct hipibs_counter : ReadSink
     uint32_t *dHll = nullptr; // the flush's distinct-count sketch
     uint64_t *dStart = nullptr;
     uint64_t startCap = 0;
-    hipEvent_t fev[2] = {nullptr, nullptr}; // around a flush's sort and slice counts
     char *hText[2] = {nullptr, nullptr};     // pinned chunks (host text only)
     uint8_t *dText = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr}; // around the last count launch
-    bool evPending = false;
-    double kernelMs = 0;                   // device time of the count launches
+    // device time per stage (0 hashing or, k > 32, counting; 1 grouping by
+    // slice; 2 slice counts; 3 spills), from HIP events around every launch,
+    // read after the stream's next synchronisation
+    std::vector<hipEvent_t> evPool;
+    size_t evUsed = 0;
+    std::vector<std::pair<size_t, int>> evPending;
+    double stageMs[4] = {0, 0, 0, 0};
+    double kernelMs = 0;                   // their sum
     uint64_t launches = 0;
     uint64_t spilled = 0, flushes = 0;     // sorted-slice counting
     // new keys a flush sizes the table for, per distinct k-mer the sketch
@@ -866,6 +870,40 @@ struct hipibs_counter : ReadSink
     int err = 0;
 
     int failed() const override { return err; }
+
+    // Launches with HIP events around it, its time going to `stage`.
+    template <class F> int timed(int stage, F &&launch)
+    {
+        while (evUsed + 2 > evPool.size())
+        {
+            hipEvent_t e = nullptr;
+            HIPIBS_CHECK(hipEventCreate(&e));
+            evPool.push_back(e);
+        }
+        HIPIBS_CHECK(hipEventRecord(evPool[evUsed], stream));
+        launch();
+        HIPIBS_CHECK(hipGetLastError());
+        HIPIBS_CHECK(hipEventRecord(evPool[evUsed + 1], stream));
+        evPending.push_back({evUsed, stage});
+        evUsed += 2;
+        launches += 1;
+        return 0;
+    }
+
+    // Reads the times of the launches since the last call (stream idle).
+    int collectTimes()
+    {
+        for (const auto &pe : evPending)
+        {
+            float ms = 0;
+            HIPIBS_CHECK(hipEventElapsedTime(&ms, evPool[pe.first], evPool[pe.first + 1]));
+            stageMs[pe.second] += ms;
+            kernelMs += ms;
+        }
+        evPending.clear();
+        evUsed = 0;
+        return 0;
+    }
     void bases(const char *p, size_t n) override
     {
         while (n && !err)
@@ -956,13 +994,11 @@ struct hipibs_counter : ReadSink
             HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dHll), (4u << kHllLog)));
             HIPIBS_CHECK(hipMemsetAsync(dHll, 0, 4u << kHllLog, stream));
         }
-        HIPIBS_CHECK(hipEventRecord(ev[0], stream));
-        countHashKernel<<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0, stream>>>(
-            text, n, skip, k, dHash[0], dHll, dCounters);
-        HIPIBS_CHECK(hipGetLastError());
-        HIPIBS_CHECK(hipEventRecord(ev[1], stream));
-        evPending = true;
-        launches += 1;
+        if ((rc = timed(0, [&] {
+                 countHashKernel<<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0,
+                                   stream>>>(text, n, skip, k, dHash[0], dHll, dCounters);
+             })))
+            return rc;
         HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 9 * sizeof(unsigned long long),
                                     hipMemcpyDeviceToHost, stream));
         nHash += positions; // an upper bound until the next sync
@@ -1069,7 +1105,6 @@ struct hipibs_counter : ReadSink
             HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dStart), (nSlices + 1) * 8));
             startCap = nSlices + 1;
         }
-        HIPIBS_CHECK(hipEventRecord(fev[0], stream));
         // group the hashes by slice: levels of up to kDigitBits top bits,
         // ping-ponging between the two buffers; the last level's bucket
         // offsets are the slices' starts
@@ -1085,12 +1120,16 @@ struct hipibs_counter : ReadSink
             const uint64_t nb = 1ull << bits;
             const uint32_t tiles = (uint32_t)((nHash + kBucketTile - 1) / kBucketTile);
             HIPIBS_CHECK(hipMemsetAsync(dStart, 0, nb * 8, stream));
-            bucketCountKernel<<<tiles, kBucketBlock, 0, stream>>>(
-                src, nHash, bits, digit, reinterpret_cast<unsigned long long *>(dStart));
-            HIPIBS_CHECK(hipGetLastError());
+            if ((rc = timed(1, [&] {
+                     bucketCountKernel<<<tiles, kBucketBlock, 0, stream>>>(
+                         src, nHash, bits, digit, reinterpret_cast<unsigned long long *>(dStart));
+                 })))
+                return rc;
             offs.resize(nb + 1);
             HIPIBS_CHECK(hipMemcpyAsync(offs.data(), dStart, nb * 8, hipMemcpyDeviceToHost, stream));
             HIPIBS_CHECK(hipStreamSynchronize(stream));
+            if ((rc = collectTimes()))
+                return rc;
             uint64_t acc = 0;
             for (uint64_t g = 0; g < nb; ++g)
             {
@@ -1102,28 +1141,29 @@ struct hipibs_counter : ReadSink
             if (acc != nHash)
                 return setError(HIPIBS_E_HIP, "k-mer counter: slice grouping lost k-mers");
             HIPIBS_CHECK(hipMemcpyAsync(dStart, offs.data(), nb * 8, hipMemcpyHostToDevice, stream));
-            bucketScatterKernel<<<tiles, kBucketBlock, 0, stream>>>(
-                src, nHash, bits, digit, reinterpret_cast<unsigned long long *>(dStart), dst);
-            HIPIBS_CHECK(hipGetLastError());
+            if ((rc = timed(1, [&] {
+                     bucketScatterKernel<<<tiles, kBucketBlock, 0, stream>>>(
+                         src, nHash, bits, digit, reinterpret_cast<unsigned long long *>(dStart),
+                         dst);
+                 })))
+                return rc;
             std::swap(src, dst);
-            launches += 2;
         }
         // the grouped hashes are in src; dst takes the spills
         const uint64_t *sorted = src;
         uint64_t *other = dst;
         HIPIBS_CHECK(hipMemcpyAsync(dStart, offs.data(), (nSlices + 1) * 8, hipMemcpyHostToDevice,
                                     stream));
-        countSliceKernel<<<(uint32_t)nSlices, kSliceBlock, 0, stream>>>(
-            sorted, dStart, dTab, nBuckets, distinct == 0, other, dCounters);
-        HIPIBS_CHECK(hipGetLastError());
-        HIPIBS_CHECK(hipEventRecord(fev[1], stream));
-        launches += 1;
+        if ((rc = timed(2, [&] {
+                 countSliceKernel<<<(uint32_t)nSlices, kSliceBlock, 0, stream>>>(
+                     sorted, dStart, dTab, nBuckets, distinct == 0, other, dCounters);
+             })))
+            return rc;
         HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 9 * sizeof(unsigned long long),
                                     hipMemcpyDeviceToHost, stream));
         HIPIBS_CHECK(hipStreamSynchronize(stream));
-        float ms = 0;
-        HIPIBS_CHECK(hipEventElapsedTime(&ms, fev[0], fev[1]));
-        kernelMs += ms;
+        if ((rc = collectTimes()))
+            return rc;
         distinct = hCounters[0];
         const uint64_t spills = hCounters[7];
         if (spills)
@@ -1131,17 +1171,16 @@ struct hipibs_counter : ReadSink
             // the spills sit in `other`, which a growth leaves in place
             if ((rc = reserve(distinct + spills)))
                 return rc;
-            HIPIBS_CHECK(hipEventRecord(fev[0], stream));
-            countSpillKernel<<<gridFor(spills, 256 * 4, device), 256, 0, stream>>>(
-                other, spills, dTab, nBuckets, dCounters);
-            HIPIBS_CHECK(hipGetLastError());
-            HIPIBS_CHECK(hipEventRecord(fev[1], stream));
-            launches += 1;
+            if ((rc = timed(3, [&] {
+                     countSpillKernel<<<gridFor(spills, 256 * 4, device), 256, 0, stream>>>(
+                         other, spills, dTab, nBuckets, dCounters);
+                 })))
+                return rc;
             HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 9 * sizeof(unsigned long long),
                                         hipMemcpyDeviceToHost, stream));
             HIPIBS_CHECK(hipStreamSynchronize(stream));
-            HIPIBS_CHECK(hipEventElapsedTime(&ms, fev[0], fev[1]));
-            kernelMs += ms;
+            if ((rc = collectTimes()))
+                return rc;
             if (hCounters[2])
                 return setError(HIPIBS_E_FULL, "k-mer counter: an insert found no free slot");
             distinct = hCounters[0];
@@ -1159,17 +1198,16 @@ struct hipibs_counter : ReadSink
     // Counts positions skip .. n-k of device text (room reserved).
     int launchCount(const uint8_t *text, uint64_t n, uint64_t skip)
     {
-        HIPIBS_CHECK(hipEventRecord(ev[0], stream));
-        if (wide)
-            countKernel<true><<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0,
-                                stream>>>(text, n, skip, k, dTab, nBuckets, dCounters);
-        else
-            countKernel<false><<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0,
-                                 stream>>>(text, n, skip, k, dTab, nBuckets, dCounters);
-        HIPIBS_CHECK(hipGetLastError());
-        HIPIBS_CHECK(hipEventRecord(ev[1], stream));
-        evPending = true;
-        launches += 1;
+        const int rc = timed(0, [&] {
+            if (wide)
+                countKernel<true><<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0,
+                                    stream>>>(text, n, skip, k, dTab, nBuckets, dCounters);
+            else
+                countKernel<false><<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0,
+                                     stream>>>(text, n, skip, k, dTab, nBuckets, dCounters);
+        });
+        if (rc)
+            return rc;
         HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 3 * sizeof(unsigned long long),
                                     hipMemcpyDeviceToHost, stream));
         return 0;
@@ -1220,13 +1258,8 @@ struct hipibs_counter : ReadSink
     int sync()
     {
         HIPIBS_CHECK(hipStreamSynchronize(stream));
-        if (evPending)
-        {
-            float ms = 0;
-            HIPIBS_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-            kernelMs += ms;
-            evPending = false;
-        }
+        if (int rc = collectTimes())
+            return rc;
         if (hCounters[2])
             return setError(HIPIBS_E_FULL, "k-mer counter: an insert found no free slot");
         distinct = hCounters[0];
@@ -1413,10 +1446,6 @@ extern "C" int hipibs_counter_create(uint32_t k, int device, uint64_t expected_k
     if (e == hipSuccess)
         e = hipHostMalloc(reinterpret_cast<void **>(&c->hCounters), 16 * sizeof(unsigned long long),
                           hipHostMallocDefault);
-    for (int i = 0; i < 2 && e == hipSuccess; ++i)
-        e = hipEventCreate(&c->ev[i]);
-    for (int i = 0; i < 2 && e == hipSuccess; ++i)
-        e = hipEventCreate(&c->fev[i]);
     if (e == hipSuccess)
         e = hipMemset(c->dCounters, 0, 16 * sizeof(unsigned long long));
     if (e != hipSuccess)
@@ -1494,6 +1523,8 @@ extern "C" int hipibs_counter_stats(hipibs_counter *c, hipibs_count_stats *out)
     out->launches = c->launches;
     out->flushes = c->flushes;
     out->spilled = c->spilled;
+    for (int i = 0; i < 4; ++i)
+        out->stage_ms[i] = c->stageMs[i];
     out->table_bytes = c->nBuckets * kCountWords * 8;
     return 0;
 }
@@ -1657,12 +1688,8 @@ extern "C" void hipibs_counter_free(hipibs_counter *c)
             (void)hipHostFree(h);
     if (c->hCounters)
         (void)hipHostFree(c->hCounters);
-    for (hipEvent_t e : c->ev)
-        if (e)
-            (void)hipEventDestroy(e);
-    for (hipEvent_t e : c->fev)
-        if (e)
-            (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->evPool)
+        (void)hipEventDestroy(e);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;

@@ -72,7 +72,7 @@ class ServeCounters(C.Structure):
 class CountStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("reads", "kmers", "distinct", "table_bytes",
                                          "launches")] + [("kernel_ms", C.c_double)] + \
-        [(n, C.c_uint64) for n in ("flushes", "spilled")]
+        [(n, C.c_uint64) for n in ("flushes", "spilled")] + [("stage_ms", C.c_double * 4)]
 
 
 class CountFilter(C.Structure):
@@ -681,7 +681,9 @@ class KmerCounter:
     def stats(self) -> dict:
         st = CountStats()
         _check(_lib.hipibs_counter_stats(self.handle, C.byref(st)), "hipibs_counter_stats")
-        return {n: getattr(st, n) for n, _ in CountStats._fields_}
+        out = {n: getattr(st, n) for n, _ in CountStats._fields_}
+        out["stage_ms"] = list(st.stage_ms)
+        return out
 
     def keys(self, min_count: int = 1, max_count: int = 0xFFFFFFFF, counter_max: int = 255):
         """(keys, counts uint32, filter dict), in no particular order; keys

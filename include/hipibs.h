@@ -230,10 +230,13 @@ typedef struct hipibs_count_stats {
     uint64_t distinct;    /* distinct canonical k-mers */
     uint64_t table_bytes; /* device bytes of the counting table */
     uint64_t launches;    /* counting kernel launches */
-    double kernel_ms;     /* their device time (HIP events on the counter's stream) */
+    double kernel_ms;     /* their device time (HIP events around each launch on the
+                             counter's stream) */
     uint64_t flushes;     /* k <= 32: sorted-slice counting rounds */
     uint64_t spilled;     /* k <= 32: k-mers whose probe left their 512-bucket slice and
                              were counted in the global table */
+    double stage_ms[4];   /* kernel_ms by stage: hashing (k > 32: counting), grouping by
+                             slice, slice counts, spills */
 } hipibs_count_stats;
 
 typedef struct hipibs_count_filter {
