@@ -129,6 +129,43 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
                 minh = min(L, lane + d < 64 ? l2 : h2);
             }
         };
+        // The minimum over each k-mer's window of `win` m-mer hashes, L the
+        // hashes at positions 64m + lane and Hh at 64(m+1) + lane, by a
+        // doubling sliding minimum across the wave.
+        auto windowMin = [&](uint32_t L, uint32_t Hh) -> uint32_t {
+            int span = 1;
+            while (2 * span <= win)
+            {
+                const int src = (lane + span) & 63;
+                const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
+                const bool inLo = lane + span < 64;
+                L = min(L, inLo ? l2 : h2);
+                Hh = min(Hh, inLo ? h2 : 0xffffffffu);
+                span *= 2;
+            }
+            const int d = win - span;
+            const int src = (lane + d) & 63;
+            const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
+            return min(L, lane + d < 64 ? l2 : h2);
+        };
+        // Canonical code, validity and the hash of the m-mer at position
+        // 64m + lane (the k-mer's first m bases: the top of its forward code,
+        // the bottom of its reverse complement).
+        auto encodeFirst = [&](uint64_t m, uint64_t &canon, bool &valid, uint32_t &L) {
+            const uint64_t s0 = seq[2 * m], s1 = seq[2 * m + 1], s2 = seq[2 * m + 2];
+            const uint64_t e0 = exc[m], e1 = exc[m + 1];
+            const uint64_t a = lane < 32 ? s0 : s1;
+            const uint64_t b = lane < 32 ? s1 : s2;
+            const uint64_t x = (j ? ((a >> (2 * j)) | (b << (64 - 2 * j))) : a) & kmask2;
+            const uint64_t ev = lane ? ((e0 >> lane) | (e1 << (64 - lane))) : e0;
+            const uint64_t p = (m << 6) + (uint64_t)lane;
+            valid = ((ev & kmask1) == 0) && p >= g0 && p < g1;
+            const uint64_t rc = ~x & kmask2;
+            const uint64_t fwd = reverse2(x) >> fwdShift;
+            canon = fwd < rc ? fwd : rc;
+            const uint64_t flo = fwd >> (2 * (k - mlen)), rlo = rc & maskm2;
+            L = mmerHash(flo < rlo ? flo : rlo);
+        };
         // Run-length statistics of one word from its hit and valid masks.
         auto fold = [&](uint64_t H, uint64_t V) {
             const uint64_t Miss = V & ~H;
@@ -172,7 +209,32 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
             nV += (uint32_t)__popcll(V);
             nH += (uint32_t)__popcll(H);
         };
-        if constexpr (GROUPED)
+        if constexpr (GROUPED && kGroupWords == 1)
+        {
+            // one word behind: the next word's first m-mers (encoded for its
+            // own step) end this word's windows. Word mEnd is read within
+            // the packed arrays' pad words.
+            uint64_t canon[1] = {0};
+            uint32_t L = 0, minh[1] = {0};
+            bool valid[1] = {false};
+            if (mBeg < mEnd)
+                encodeFirst(mBeg, canon[0], valid[0], L);
+            for (uint64_t m = mBeg; m < mEnd; ++m)
+            {
+                uint64_t canonN;
+                bool validN;
+                uint32_t LN;
+                encodeFirst(m + 1, canonN, validN, LN);
+                minh[0] = windowMin(L, LN);
+                uint64_t H[1];
+                waveProbeGrouped<S, 1>(T, canon, minh, valid, groupStage[wv], H);
+                fold(H[0], ballot(valid[0]));
+                canon[0] = canonN;
+                valid[0] = validN;
+                L = LN;
+            }
+        }
+        else if constexpr (GROUPED)
         {
             // kGroupWords words per step: their home lines are in flight
             // together
