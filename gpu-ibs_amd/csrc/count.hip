@@ -401,11 +401,13 @@ __device__ inline uint64_t windowStart(const uint64_t *in, uint64_t t0, uint32_t
     return ((in[t0] >> (64 - bits)) >> digit) << digit;
 }
 
-// count[g] += elements of bucket g.
+// count[g] += elements of bucket g; tileHist[tile][i] = the tile's count of
+// bucket windowStart + i (its LDS window).
 __global__ __launch_bounds__(kBucketBlock) void bucketCountKernel(const uint64_t *__restrict__ in,
                                                                   uint64_t n, uint32_t bits,
                                                                   uint32_t digit,
-                                                                  unsigned long long *count)
+                                                                  unsigned long long *count,
+                                                                  uint32_t *tileHist)
 {
     __shared__ uint32_t hist[kBucketWin];
     const uint64_t t0 = (uint64_t)blockIdx.x * kBucketTile;
@@ -425,20 +427,24 @@ __global__ __launch_bounds__(kBucketBlock) void bucketCountKernel(const uint64_t
     __syncthreads();
     const uint64_t nBuckets = 1ull << bits;
     for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
+    {
         if (hist[i] && g0 + i < nBuckets)
             atomicAdd(count + g0 + i, (unsigned long long)hist[i]);
+        tileHist[(uint64_t)blockIdx.x * kBucketWin + i] = hist[i];
+    }
 }
 
 // Writes every element at its bucket's cursor (cursor[g] starts at the
-// bucket's first output index). The tile is counted once in LDS and each of
-// its buckets reserves its run with one global add; then sub-chunks of
+// bucket's first output index). Each of the tile's buckets (counted by
+// bucketCountKernel into tileHist) reserves its run with one global add;
+// then sub-chunks of
 // kStageElems elements are ordered by bucket in LDS (a counting sort) and
 // written out, so that consecutive threads store consecutive addresses of
 // a run instead of single elements scattered over all of the tile's runs.
 constexpr int kStageElems = 8192;
 __global__ __launch_bounds__(kBucketBlock) void bucketScatterKernel(
     const uint64_t *__restrict__ in, uint64_t n, uint32_t bits, uint32_t digit,
-    unsigned long long *cursor, uint64_t *__restrict__ out)
+    const uint32_t *__restrict__ tileHist, unsigned long long *cursor, uint64_t *__restrict__ out)
 {
     __shared__ uint32_t hist[kBucketWin];  // tile counts, then per sub-chunk counts
     __shared__ uint32_t start[kBucketWin]; // a sub-chunk's bucket starts in stage
@@ -448,19 +454,10 @@ __global__ __launch_bounds__(kBucketBlock) void bucketScatterKernel(
     const uint64_t g0 = windowStart(in, t0, bits, digit);
     const uint64_t end = min<uint64_t>(n, t0 + kBucketTile);
     for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
-        hist[i] = 0;
-    __syncthreads();
-    for (uint64_t i = t0 + threadIdx.x; i < end; i += kBucketBlock)
     {
-        const uint64_t g = in[i] >> (64 - bits);
-        if (g - g0 < (uint64_t)kBucketWin)
-            atomicAdd(&hist[g - g0], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
-    {
-        if (hist[i])
-            base[i] = atomicAdd(cursor + g0 + i, (unsigned long long)hist[i]);
+        const uint32_t c = tileHist[(uint64_t)blockIdx.x * kBucketWin + i];
+        if (c)
+            base[i] = atomicAdd(cursor + g0 + i, (unsigned long long)c);
         hist[i] = 0;
     }
     __syncthreads();
@@ -534,22 +531,30 @@ __global__ __launch_bounds__(kBucketBlock) void bucketScatterKernel(
 // One workgroup per slice: loads the slice's buckets into LDS (or EMPTY
 // keys and zero counts when the table is fresh), counts the slice's k-mers
 // there (the key by compare-and-swap into the first EMPTY slot from its
-// home, the count by an LDS atomic add), writes the slice back. A k-mer
+// home, the count by an LDS atomic add), writes the slice back; with
+// writeAll (an unfilled table) a slice without k-mers is written empty. A k-mer
 // whose probe reaches the slice's end without its key or an EMPTY slot is
 // appended to spill (counters[7]). counters[0] += keys stored.
 __global__ __launch_bounds__(kSliceBlock) void countSliceKernel(
     const uint64_t *__restrict__ hs, const uint64_t *__restrict__ start, uint64_t *tab,
-    uint64_t nBuckets, int fresh, uint64_t *spill, unsigned long long *counters)
+    uint64_t nBuckets, int fresh, int writeAll, uint64_t *spill, unsigned long long *counters)
 {
     __shared__ unsigned long long keys[kSliceBuckets * 8];
     __shared__ unsigned long long cnt[kSliceBuckets * 8];
     __shared__ uint32_t stored;
     const uint64_t s = blockIdx.x;
     const uint64_t lo = start[s], hi = start[s + 1];
-    if (lo >= hi)
-        return;
     const uint64_t b0 = s << kSliceLog;
     const uint32_t nb = (uint32_t)min<uint64_t>(kSliceBuckets, nBuckets - b0);
+    if (lo >= hi)
+    {
+        // no k-mers: an unfilled table still gets the slice's empty buckets
+        if (writeAll)
+            for (uint32_t q = threadIdx.x; q < nb * kCountWords; q += kSliceBlock)
+                tab[(b0 + q / kCountWords) * kCountWords + q % kCountWords] =
+                    q % kCountWords < 8 ? EMPTY : 0ull;
+        return;
+    }
     if (threadIdx.x == 0)
         stored = 0;
     for (uint32_t q = threadIdx.x; q < nb * kCountWords; q += kSliceBlock)
@@ -844,6 +849,9 @@ struct hipibs_counter : ReadSink
     uint64_t nHash = 0;
     uint32_t *dHll = nullptr; // the flush's distinct-count sketch
     uint64_t *dStart = nullptr;
+    bool tableUnfilled = false;    // dTab allocated but not yet written
+    uint32_t *dTileHist = nullptr; // a bucketing level's per-tile window counts
+    uint64_t tileHistCap = 0;      // tiles
     uint64_t startCap = 0;
     char *hText[2] = {nullptr, nullptr};     // pinned chunks (host text only)
     uint8_t *dText = nullptr;
@@ -1092,7 +1100,7 @@ struct hipibs_counter : ReadSink
         if (est < 0)
             return setError(HIPIBS_E_HIP, "k-mer counter: reading the sketch failed");
         const uint64_t newKeys = std::min<uint64_t>(nHash, (uint64_t)(est * sketchScale) + 4096);
-        if ((rc = reserve(distinct + newKeys)))
+        if ((rc = reserve(distinct + newKeys, distinct == 0)))
             return rc;
         const int lg = 63 - __builtin_clzll(nBuckets); // a power of two
         const uint32_t sbits = lg > kSliceLog ? (uint32_t)(lg - kSliceLog) : 0u;
@@ -1104,6 +1112,16 @@ struct hipibs_counter : ReadSink
             dStart = nullptr;
             HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dStart), (nSlices + 1) * 8));
             startCap = nSlices + 1;
+        }
+        const uint64_t nTiles = (nHash + kBucketTile - 1) / kBucketTile;
+        if (sbits && tileHistCap < nTiles)
+        {
+            if (dTileHist)
+                HIPIBS_CHECK(hipFree(dTileHist));
+            dTileHist = nullptr;
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileHist),
+                                   nTiles * kBucketWin * sizeof(uint32_t)));
+            tileHistCap = nTiles;
         }
         // group the hashes by slice: levels of up to kDigitBits top bits,
         // ping-ponging between the two buffers; the last level's bucket
@@ -1122,7 +1140,8 @@ struct hipibs_counter : ReadSink
             HIPIBS_CHECK(hipMemsetAsync(dStart, 0, nb * 8, stream));
             if ((rc = timed(1, [&] {
                      bucketCountKernel<<<tiles, kBucketBlock, 0, stream>>>(
-                         src, nHash, bits, digit, reinterpret_cast<unsigned long long *>(dStart));
+                         src, nHash, bits, digit, reinterpret_cast<unsigned long long *>(dStart),
+                         dTileHist);
                  })))
                 return rc;
             offs.resize(nb + 1);
@@ -1143,8 +1162,8 @@ struct hipibs_counter : ReadSink
             HIPIBS_CHECK(hipMemcpyAsync(dStart, offs.data(), nb * 8, hipMemcpyHostToDevice, stream));
             if ((rc = timed(1, [&] {
                      bucketScatterKernel<<<tiles, kBucketBlock, 0, stream>>>(
-                         src, nHash, bits, digit, reinterpret_cast<unsigned long long *>(dStart),
-                         dst);
+                         src, nHash, bits, digit, dTileHist,
+                         reinterpret_cast<unsigned long long *>(dStart), dst);
                  })))
                 return rc;
             std::swap(src, dst);
@@ -1156,7 +1175,8 @@ struct hipibs_counter : ReadSink
                                     stream));
         if ((rc = timed(2, [&] {
                  countSliceKernel<<<(uint32_t)nSlices, kSliceBlock, 0, stream>>>(
-                     sorted, dStart, dTab, nBuckets, distinct == 0, other, dCounters);
+                     sorted, dStart, dTab, nBuckets, distinct == 0, tableUnfilled, other,
+                     dCounters);
              })))
             return rc;
         HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 9 * sizeof(unsigned long long),
@@ -1164,6 +1184,7 @@ struct hipibs_counter : ReadSink
         HIPIBS_CHECK(hipStreamSynchronize(stream));
         if ((rc = collectTimes()))
             return rc;
+        tableUnfilled = false; // every slice written
         distinct = hCounters[0];
         const uint64_t spills = hCounters[7];
         if (spills)
@@ -1269,7 +1290,7 @@ struct hipibs_counter : ReadSink
 
     // Makes room for `need` keys at <= kMaxLoad, growing to kGrowLoad and
     // at least doubling (called with the stream idle).
-    int reserve(uint64_t need)
+    int reserve(uint64_t need, bool lazyFill = false)
     {
         const int slots = wide ? Layout<true>::kSlots : Layout<false>::kSlots;
         if ((double)need <= kMaxLoad * (double)(nBuckets * slots))
@@ -1291,9 +1312,15 @@ struct hipibs_counter : ReadSink
                                 " buckets (" + std::to_string(distinct) + " distinct k-mers so far)");
         }
         const uint64_t words = want * kCountWords;
-        countFillKernel<<<gridFor(words, 256 * 4, device), 256, 0, stream>>>(
-            t, words, wide ? Layout<true>::kKeyWords : Layout<false>::kKeyWords);
-        HIPIBS_CHECK(hipGetLastError());
+        // a new table with no keys to move in may stay unfilled when the
+        // caller's slice counts write every slice (lazyFill)
+        tableUnfilled = lazyFill && !dTab;
+        if (!tableUnfilled)
+        {
+            countFillKernel<<<gridFor(words, 256 * 4, device), 256, 0, stream>>>(
+                t, words, wide ? Layout<true>::kKeyWords : Layout<false>::kKeyWords);
+            HIPIBS_CHECK(hipGetLastError());
+        }
         if (dTab)
         {
             if (wide)
@@ -1683,6 +1710,8 @@ extern "C" void hipibs_counter_free(hipibs_counter *c)
         (void)hipFree(c->dHll);
     if (c->dStart)
         (void)hipFree(c->dStart);
+    if (c->dTileHist)
+        (void)hipFree(c->dTileHist);
     for (char *h : c->hText)
         if (h)
             (void)hipHostFree(h);
