@@ -568,33 +568,49 @@ __global__ __launch_bounds__(kSliceBlock) void countSliceKernel(
     }
     __syncthreads();
     uint32_t mine = 0;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += kSliceBlock)
+    // kSliceAhead hashes per thread are loaded before any is counted, so
+    // their loads are in flight together
+    constexpr int kSliceAhead = 4;
+    for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)kSliceBlock * kSliceAhead)
     {
-        const uint64_t h = hs[i];
-        const uint64_t key = keyUnmix(h);
-        const uint64_t home = ((h >> 32) * nBuckets) >> 32;
-        bool done = false;
-        for (uint32_t b = (uint32_t)(home - b0); b < nb && !done; ++b)
-            for (int q = 0; q < 8 && !done; ++q)
-            {
-                unsigned long long cur = keys[b * 8 + q];
-                if (cur == EMPTY)
+        uint64_t hv[kSliceAhead];
+#pragma unroll
+        for (int a = 0; a < kSliceAhead; ++a)
+        {
+            const uint64_t i = i0 + (uint64_t)a * kSliceBlock;
+            hv[a] = i < hi ? hs[i] : EMPTY;
+        }
+#pragma unroll
+        for (int a = 0; a < kSliceAhead; ++a)
+        {
+            if (i0 + (uint64_t)a * kSliceBlock >= hi)
+                break;
+            const uint64_t h = hv[a];
+            const uint64_t key = keyUnmix(h);
+            const uint64_t home = ((h >> 32) * nBuckets) >> 32;
+            bool done = false;
+            for (uint32_t b = (uint32_t)(home - b0); b < nb && !done; ++b)
+                for (int q = 0; q < 8 && !done; ++q)
                 {
-                    cur = atomicCAS(&keys[b * 8 + q], EMPTY, (unsigned long long)key);
+                    unsigned long long cur = keys[b * 8 + q];
                     if (cur == EMPTY)
                     {
-                        mine += 1;
-                        cur = key;
+                        cur = atomicCAS(&keys[b * 8 + q], EMPTY, (unsigned long long)key);
+                        if (cur == EMPTY)
+                        {
+                            mine += 1;
+                            cur = key;
+                        }
+                    }
+                    if (cur == key)
+                    {
+                        atomicAdd(&cnt[b * 8 + q], 1ull);
+                        done = true;
                     }
                 }
-                if (cur == key)
-                {
-                    atomicAdd(&cnt[b * 8 + q], 1ull);
-                    done = true;
-                }
-            }
-        if (!done)
-            spill[atomicAdd(counters + 7, 1ull)] = h;
+            if (!done)
+                spill[atomicAdd(counters + 7, 1ull)] = h;
+        }
     }
     if (mine)
         atomicAdd(&stored, mine);
