@@ -300,9 +300,12 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
                uint32_t segsPerWindow, void *dWork, uint64_t workBytes, hipStream_t s,
                TableView *dViews);
 int uploadViews(const hipibs_table *const *tables, int nTables, hipStream_t s, TableView **out);
+// grouped: every table the windows name is a minimizer table (probed with
+// waveProbeGrouped).
 int launchScanMixed(int bucketSlots, uint32_t k, const TableView *dViews, const uint64_t *dSeq,
                     const uint64_t *dExc, uint64_t nBases, const hipibs_window *dWindows,
-                    uint32_t nWindows, hipibs_window_stats *dStats, hipStream_t s);
+                    uint32_t nWindows, hipibs_window_stats *dStats, hipStream_t s,
+                    bool grouped = false);
 int checkTables(const hipibs_table *const *tables, int nTables);
 uint32_t autoSegsPerWindow(uint32_t nWindows, int nTables, uint32_t maxPositions);
 uint64_t scanWorkBytes(uint32_t nWindows, int nTables, uint32_t segsPerWindow);

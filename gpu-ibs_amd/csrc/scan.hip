@@ -572,7 +572,7 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
 // indexes dViews); results row i per window; one segment per window.
 int launchScanMixed(int bucketSlots, uint32_t k, const TableView *dViews, const uint64_t *dSeq,
                     const uint64_t *dExc, uint64_t nBases, const hipibs_window *dWindows,
-                    uint32_t nWindows, hipibs_window_stats *dStats, hipStream_t s)
+                    uint32_t nWindows, hipibs_window_stats *dStats, hipStream_t s, bool grouped)
 {
     if (nWindows == 0)
         return 0;
@@ -580,6 +580,14 @@ int launchScanMixed(int bucketSlots, uint32_t k, const TableView *dViews, const 
     if (k > 32)
         hipLaunchKernelGGL((scanKernel<16, true, true>), grid, dim3(SCAN_BLOCK), 0, s, dViews,
                            dSeq, dExc, nBases, dWindows, nWindows, 1u, k, dStats,
+                           (SegSummary *)nullptr);
+    else if (grouped && bucketSlots == 8)
+        hipLaunchKernelGGL((scanKernel<8, true, false, true>), grid, dim3(SCAN_BLOCK), 0, s,
+                           dViews, dSeq, dExc, nBases, dWindows, nWindows, 1u, k, dStats,
+                           (SegSummary *)nullptr);
+    else if (grouped)
+        hipLaunchKernelGGL((scanKernel<16, true, false, true>), grid, dim3(SCAN_BLOCK), 0, s,
+                           dViews, dSeq, dExc, nBases, dWindows, nWindows, 1u, k, dStats,
                            (SegSummary *)nullptr);
     else if (bucketSlots == 8)
         hipLaunchKernelGGL((scanKernel<8, true>), grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc,

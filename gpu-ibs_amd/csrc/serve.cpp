@@ -428,8 +428,9 @@ int fillAndLaunch(hipibs_serve *s, Slot &sl, bool &launched)
                                 hipMemcpyHostToDevice, sl.stream));
     HIPIBS_CHECK(hipMemcpyAsync(sl.dWin, sl.hWin, used * sizeof(hipibs_window),
                                 hipMemcpyHostToDevice, sl.stream));
+    // every table of the engine is built with the engine's options
     int rc = launchScanMixed(slots, s->k, sl.dViews, s->dSeq, s->dExc, s->nBases, sl.dWin, used,
-                             sl.dStats, sl.stream);
+                             sl.dStats, sl.stream, s->topts.minimizer_len != 0 && s->k <= 32);
     if (rc)
         return rc;
     HIPIBS_CHECK(hipMemcpyAsync(sl.hStats, sl.dStats, used * sizeof(hipibs_window_stats),

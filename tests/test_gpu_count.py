@@ -230,6 +230,42 @@ def test_spilled_kmers_counted_in_global_table(small_reads, monkeypatch, chunk):
     c.free()
 
 
+def test_hash_buffers_reused_across_counters(small_reads):
+    """Counters alive together each keep their own buffers; a freed
+    counter's buffers serve the next counter, larger or smaller, and
+    hipibs_counter_release_cache frees them between counters: every count
+    equals the oracle's."""
+    _, reads = small_reads
+    sets = [reads[0][:6000], reads[1][:1500], reads[2][:9000], reads[0][:500]]
+    texts = [b"\n".join(corpus.to_text(x) for x in r) for r in sets]
+    wants = [corpus.count_kmers(r, 25, ci=1, cs=ALL) for r in sets]
+
+    def check(c, i):
+        keys, counts, _ = c.keys(1, ALL, ALL)
+        keys, counts = sorted_pairs(keys, counts)
+        np.testing.assert_array_equal(keys, wants[i][0])
+        np.testing.assert_array_equal(counts, wants[i][1])
+
+    a, b = hipibs.KmerCounter(25), hipibs.KmerCounter(25)
+    a.add_text(texts[0])
+    b.add_text(texts[1])
+    check(a, 0)
+    check(b, 1)
+    a.free()
+    c = hipibs.KmerCounter(25)  # takes a's buffers
+    c.add_text(texts[2])
+    check(c, 2)
+    b.free()
+    c.free()
+    hipibs._lib.hipibs_counter_release_cache()
+    d = hipibs.KmerCounter(25)
+    d.add_text(texts[3])
+    check(d, 3)
+    d.free()
+    hipibs._lib.hipibs_counter_release_cache()
+    hipibs._lib.hipibs_counter_release_cache()  # nothing kept: no-op
+
+
 def test_filters(small_reads):
     _, reads = small_reads
     r = reads[2][:20000]

@@ -572,6 +572,47 @@ def test_serving_results_same_alone_and_in_the_mix(small_corpus, tmp_path):
         assert got == [w[3:] for w in want], rid
 
 
+def test_minimizer_serving_equals_oracle(small_corpus):
+    """Serving with minimizer tables (the grouped probe in the mixed-table
+    launch; a cap of 4 keys per group sends repeat groups to the whole-k-mer
+    part), a budget that forces evictions: every request equals the
+    oracle."""
+    ref, samples = small_corpus
+    seq = corpus.to_text(ref)
+    sw, ew, _ = hipibs.pack_sequences([seq])
+    n = len(seq)
+    keys = [s[0] for s in samples]
+    opts = dict(minimizer_len=21, minimizer_cap=4, load_factor=0.35)
+    one_table = hipibs.Table.from_keys(keys[0], 25, **opts).info()["bytes"]
+    rng = np.random.default_rng(7)
+    reqs = []
+    for i in range(18):
+        ln = int(rng.integers(30, n // 3))
+        reqs.append((i, i % 3, int(rng.integers(0, n - ln)), ln,
+                     int(rng.choice([1000, 5000, 10000]))))
+    eng = hipibs.ServeEngine(25, int(1.5 * one_table), max_inflight=3,
+                             max_windows_per_launch=512, **opts)
+    eng.set_reference(sw, ew, n)
+    for sid in range(3):
+        eng.add_sample(sid, keys[sid])
+    got = {}
+    for phase in (reqs[:9], reqs[9:]):
+        pending = set()
+        for r in phase:
+            eng.submit(*r)
+            pending.add(r[0])
+        while pending:
+            for rid in eng.poll(timeout_ms=2000):
+                got[rid] = eng.fetch(rid)[0]
+                pending.discard(rid)
+    c = eng.counters()
+    eng.free()
+    assert c["evictions"] >= 1
+    for rid, sid, base, ln, W in reqs:
+        want = oracle_scan_rows([b"c"], [seq[base:base + ln]], W, 25, sorted_keys(keys[sid]))
+        assert [tuple(int(x) for x in row) for row in got[rid]] == [w[3:] for w in want], rid
+
+
 def coverage_text_gpu(t, ids, seqs, k):
     """fastibsmapper file content from the device hit mask + coverage."""
     parts = []
