@@ -339,14 +339,19 @@ __global__ void foldKernel(const SegSummary *__restrict__ work, uint32_t nItems,
 
 // Mapper: hit mask over positions pos0 .. pos0+nPos-1 (bit i = position
 // pos0+i), one 64-position word per wave step, each lane gathering its own
-// k-mer (positions need not be aligned to the packed words).
-template <int S, bool WIDE = false>
+// k-mer (positions need not be aligned to the packed words). GROUPED: a
+// minimizer table, probed with waveProbeGrouped (the wave's lanes hold
+// consecutive positions, so runs of them share a minimizer).
+template <int S, bool WIDE = false, bool GROUPED = false>
 __global__ __launch_bounds__(256) void hitmaskKernel(const TableView T, const uint64_t *seq,
                                                      const uint64_t *exc, uint64_t pos0,
                                                      uint64_t nPos, uint64_t contigEnd, uint32_t k,
                                                      uint32_t *hitWords)
 {
+    __shared__ __align__(16) uint64_t
+        groupStage[GROUPED ? 4 : 1][GROUPED ? GroupStage<S, 1>::kWords : 2];
     const int lane = laneId();
+    const int wv = GROUPED ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nWaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     const uint64_t nChunks = (nPos + 63) >> 6;
@@ -389,6 +394,15 @@ __global__ __launch_bounds__(256) void hitmaskKernel(const TableView T, const ui
         if constexpr (WIDE)
             H = waveProbeWide(T, chi, canon,
                               valid ? (uint32_t)bucketOfWide(chi, canon, T.nBuckets) : 0u, valid);
+        else if constexpr (GROUPED)
+        {
+            const uint64_t code[1] = {canon};
+            const uint32_t minh[1] = {valid ? minimizerOf(canon, k, T.mlen) : 0u};
+            const bool v[1] = {valid};
+            uint64_t Hw[1];
+            waveProbeGrouped<S, 1>(T, code, minh, v, groupStage[wv], Hw);
+            H = Hw[0];
+        }
         else
             H = waveLookup<S>(T, canon, valid);
         if (lane < 2)
@@ -932,6 +946,14 @@ extern "C" int hipibs_hitmask_device(const hipibs_table *t, const uint64_t *d_se
     if (t->k > 32)
         hipLaunchKernelGGL((hitmaskKernel<16, true>), dim3((uint32_t)grid), dim3(256), 0, s,
                            t->view(), d_seq_words, d_exc_words, pos0, n_pos, contig_end, t->k,
+                           d_hit_words);
+    else if (t->mlen && t->bucketSlots == 8)
+        hipLaunchKernelGGL((hitmaskKernel<8, false, true>), dim3((uint32_t)grid), dim3(256), 0, s,
+                           t->view(), d_seq_words, d_exc_words, pos0, n_pos, contig_end, t->k,
+                           d_hit_words);
+    else if (t->mlen)
+        hipLaunchKernelGGL((hitmaskKernel<16, false, true>), dim3((uint32_t)grid), dim3(256), 0,
+                           s, t->view(), d_seq_words, d_exc_words, pos0, n_pos, contig_end, t->k,
                            d_hit_words);
     else if (t->bucketSlots == 8)
         hipLaunchKernelGGL(hitmaskKernel<8>, dim3((uint32_t)grid), dim3(256), 0, s, t->view(),

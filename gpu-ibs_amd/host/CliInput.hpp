@@ -168,6 +168,9 @@ inline const char *optionsUsage()
            "  --bucket-bytes=64|128 device table bucket width (default 128; a panel\n"
            "                        times both unless this is given)\n"
            "  --hbm-budget-gb=N     cap on device table memory\n"
+           "  --minimizer=M[,CAP]   minimizer table (k <= 32, M in 5..min(31, k), groups of\n"
+           "                        up to CAP keys, default 16): runs of positions that share\n"
+           "                        a minimizer read their bucket once; one sample per run\n"
            "  --db=<folder>         (fastibs) another sample in the same reference pass\n"
            "  --db-list=<file>      (fastibs) sample folders, one per line\n"
            "  --resident=N          (fastibs) at most N sample tables on the device at once\n";
@@ -216,6 +219,20 @@ inline bool parseOptions(int argc, char **argv, int first, Options &o, bool pane
         }
         else if (key == "--hbm-budget-gb" && parseInt(val, 1, 1 << 20, v))
             g.hbmBudget = (uint64_t)v << 30;
+        else if (key == "--minimizer" && !val.empty())
+        {
+            const size_t comma = val.find(',');
+            long long cap = 16;
+            if (!parseInt(val.substr(0, comma), 5, 31, v) ||
+                (comma != std::string::npos && !parseInt(val.substr(comma + 1), 1, 4096, cap)))
+            {
+                std::cerr << "Error: --minimizer=M[,CAP] needs M in 5..31 and CAP in 1..4096"
+                          << std::endl;
+                return false;
+            }
+            g.minimizerLen = (uint32_t)v;
+            g.minimizerCap = (uint32_t)cap;
+        }
         else if (panel && key == "--db" && !val.empty())
             o.dbs.push_back(val);
         else if (panel && key == "--db-list" && !val.empty())

@@ -40,6 +40,8 @@ public:
         bool bucketAuto = true; // a panel may time 64 B vs 128 B buckets
         uint64_t hbmBudget = 0;
         unsigned loadThreads = 0;
+        uint32_t minimizerLen = 0; // 0: whole-k-mer table
+        uint32_t minimizerCap = 0; // 0: the library's default group cap
     };
     static GpuOptions &gpuOptions()
     {
@@ -97,6 +99,8 @@ public:
         o.bucket_slots = g.bucketSlots;
         o.hbm_budget = g.hbmBudget;
         o.load_threads = g.loadThreads;
+        o.minimizer_len = g.minimizerLen;
+        o.minimizer_cap = g.minimizerCap;
         const auto t0 = std::chrono::steady_clock::now();
         table_ = hipibs::tableFromKmc(sourcePath_, o);
         const auto ti = hipibs::info(table_.get());

@@ -49,6 +49,16 @@ def test_help_and_bad_arguments(tmp_path):
     assert r.returncode == 1 and "no CPU path" in r.stderr
     r = tool("fastibs", tmp_path, tmp_path, tmp_path, "50000", "--bogus")
     assert r.returncode == 1 and "unknown" in r.stderr
+    for bad in ("--minimizer=4", "--minimizer=32", "--minimizer=21,0", "--minimizer=21,",
+                "--minimizer=x"):
+        r = tool("fastibs", tmp_path, tmp_path, tmp_path, "50000", bad)
+        assert r.returncode == 1 and "--minimizer=M[,CAP]" in r.stderr, bad
+
+
+def test_minimizer_tables_not_for_panels(small_layout, tmp_path):
+    r = tool("fastibs", small_layout / "s0", small_layout / "refs", tmp_path / "out", 50000,
+             f"--db={small_layout / 's1'}", "--minimizer=21")
+    assert r.returncode == 1 and "panel runs" in r.stderr
 
 
 def test_window_not_larger_than_k(small_layout, tmp_path):
@@ -100,6 +110,22 @@ def test_fastibs_panel_equals_singles(small_layout, tmp_path, resident):
 def test_fastibsmapper_matches_golden(small_layout, tmp_path, expected):
     out = tmp_path / "map"
     r = tool("fastibsmapper", small_layout / "s0", small_layout / "refs", out)
+    assert r.returncode == 0, r.stderr
+    assert sha256((out / "s0_v_small_genome.txt").read_bytes()) == \
+        expected["small"]["s0"]["coverage_sha256"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("opt", ["--minimizer=21", "--minimizer=13,4"])
+def test_tools_with_minimizer_tables_match_golden(small_layout, tmp_path, expected, opt):
+    """fastibs and fastibsmapper over minimizer tables (the grouped scan and
+    hit-mask kernels) write the whole-k-mer tables' bytes."""
+    out = tmp_path / "out"
+    r = tool("fastibs", small_layout / "s1", small_layout / "refs", out, 50000, opt)
+    assert r.returncode == 0, r.stderr
+    assert (out / "s1_v_small_50000.tsv").read_bytes() == (
+        open(os.path.join(ROOT, "tests", "golden", "small_s1_50000.tsv"), "rb").read())
+    r = tool("fastibsmapper", small_layout / "s0", small_layout / "refs", out, opt)
     assert r.returncode == 0, r.stderr
     assert sha256((out / "s0_v_small_genome.txt").read_bytes()) == \
         expected["small"]["s0"]["coverage_sha256"]
