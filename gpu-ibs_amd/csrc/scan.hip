@@ -111,42 +111,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
                 // of rc
                 const uint64_t flo = fwd >> (2 * (k - mlen)), rlo = rc & maskm2;
                 const uint64_t rhi = ~y & maskm2, fhi = reverse2(y) >> mShift;
-                uint32_t L = mmerHash(flo < rlo ? flo : rlo);
-                uint32_t Hh = mmerHash(fhi < rhi ? fhi : rhi);
-                int span = 1;
-                while (2 * span <= win)
-                {
-                    const int src = (lane + span) & 63;
-                    const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
-                    const bool inLo = lane + span < 64;
-                    L = min(L, inLo ? l2 : h2);
-                    Hh = min(Hh, inLo ? h2 : 0xffffffffu);
-                    span *= 2;
-                }
-                const int d = win - span;
-                const int src = (lane + d) & 63;
-                const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
-                minh = min(L, lane + d < 64 ? l2 : h2);
+                minh = waveWindowMin(mmerHash(flo < rlo ? flo : rlo),
+                                     mmerHash(fhi < rhi ? fhi : rhi), win);
             }
-        };
-        // The minimum over each k-mer's window of `win` m-mer hashes, L the
-        // hashes at positions 64m + lane and Hh at 64(m+1) + lane, by a
-        // doubling sliding minimum across the wave.
-        auto windowMin = [&](uint32_t L, uint32_t Hh) -> uint32_t {
-            int span = 1;
-            while (2 * span <= win)
-            {
-                const int src = (lane + span) & 63;
-                const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
-                const bool inLo = lane + span < 64;
-                L = min(L, inLo ? l2 : h2);
-                Hh = min(Hh, inLo ? h2 : 0xffffffffu);
-                span *= 2;
-            }
-            const int d = win - span;
-            const int src = (lane + d) & 63;
-            const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
-            return min(L, lane + d < 64 ? l2 : h2);
         };
         // Canonical code, validity and the hash of the m-mer at position
         // 64m + lane (the k-mer's first m bases: the top of its forward code,
@@ -225,7 +192,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scanKernel(
                 bool validN;
                 uint32_t LN;
                 encodeFirst(m + 1, canonN, validN, LN);
-                minh[0] = windowMin(L, LN);
+                minh[0] = waveWindowMin(L, LN, win);
                 uint64_t H[1];
                 waveProbeGrouped<S, 1>(T, canon, minh, valid, groupStage[wv], H);
                 fold(H[0], ballot(valid[0]));

@@ -372,6 +372,31 @@ __device__ inline uint64_t waveProbeMinimizer(const TableView &T, uint64_t code,
     return H;
 }
 
+// Sliding minimum of m-mer hashes across the wave: L holds the hash at
+// position base + lane, Hh the one at base + 64 + lane; lane i gets the
+// minimum over positions base + i .. base + i + win - 1 (win <= 65), the
+// minimizer hash of the k-mer at base + i when win = k - m + 1. Doubling
+// spans, then one overlapping step for the remainder. Must be called by
+// all 64 lanes.
+__device__ inline uint32_t waveWindowMin(uint32_t L, uint32_t Hh, int win)
+{
+    const int lane = laneId();
+    int span = 1;
+    while (2 * span <= win)
+    {
+        const int src = (lane + span) & 63;
+        const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
+        const bool inLo = lane + span < 64;
+        L = min(L, inLo ? l2 : h2);
+        Hh = min(Hh, inLo ? h2 : 0xffffffffu);
+        span *= 2;
+    }
+    const int d = win - span;
+    const int src = (lane + d) & 63;
+    const uint32_t l2 = shuffle32(L, src), h2 = shuffle32(Hh, src);
+    return min(L, lane + d < 64 ? l2 : h2);
+}
+
 // Per-wave LDS staging of waveProbeGrouped over NW words: up to kLines home
 // lines, each S slots plus 16 B of padding (lanes that read different lines
 // at the same offset then fall in different banks), and the lines' home
