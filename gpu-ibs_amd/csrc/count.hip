@@ -298,20 +298,25 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(const uint8_t *__
                                                                uint32_t *hll,
                                                                unsigned long long *counters)
 {
+    // each wave stages its k-mers' hashes in its own region (a lane emits
+    // at most kLanePositions per tile), counted in a wave-uniform register
+    constexpr int kWaves = kCountBlock / 64;
+    constexpr uint32_t kWaveStage = 64 * kLanePositions;
+    static_assert(kWaves * kWaveStage == kTilePositions, "one stage region per wave");
     __shared__ __align__(16) uint8_t tile[kTileBytes];
     __shared__ uint32_t reg[1 << kHllLog];
     __shared__ uint64_t stage[kTilePositions];
-    __shared__ uint32_t nStage;
+    __shared__ uint32_t waveCount[kWaves];
     __shared__ unsigned long long stageAt;
     for (int i = threadIdx.x; i < (1 << kHllLog); i += kCountBlock)
         reg[i] = 0;
-    if (threadIdx.x == 0)
-        nStage = 0;
     const uint64_t nPos = n >= k ? n - k + 1 : 0;
     const uint64_t nTiles = (nPos + kTilePositions - 1) / kTilePositions;
     const uint64_t mask = k == 32 ? ~0ull : (1ull << (2 * k)) - 1;
     const uint32_t shift = 2 * (k - 1);
     const int lane = laneId();
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    uint64_t *wstage = stage + wv * kWaveStage;
     uint32_t counted = 0;
     for (uint64_t t = blockIdx.x; t < nTiles; t += gridDim.x)
     {
@@ -329,29 +334,33 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(const uint8_t *__
         }
         __syncthreads();
         // positions [start, start + 32) of the tile, every lane the same
-        // number of steps so that the wave appends its k-mers together
+        // number of steps so that the wave stages its k-mers together
+        // (bytes start + d, read one aligned 4-byte word per 4 steps; the
+        // step count is the same for every lane)
         const uint32_t start = threadIdx.x * kLanePositions;
+        const uint32_t steps = kLanePositions + k - 1;
         uint64_t fwd = 0, rc = 0;
-        uint32_t run = 0;
-        for (uint32_t d = 0; d < kLanePositions + k - 1; ++d)
+        uint32_t run = 0, wcount = 0;
+        for (uint32_t d0 = 0; d0 < steps; d0 += 4)
         {
-            const uint32_t i = start + d;
-            const uint32_t c = i < bytes ? baseCode(tile[i]) : 4u;
-            run = c > 3 ? 0 : run + 1;
-            fwd = ((fwd << 2) | (c & 3)) & mask;
-            rc = (rc >> 2) | ((uint64_t)(3 - (c & 3)) << shift);
-            const bool emit = run >= k && base + i + 1 >= skip + k;
-            const uint64_t h = keyMix(min(fwd, rc));
-            const uint64_t m = __ballot(emit);
-            if (m)
+            const uint32_t word = *reinterpret_cast<const uint32_t *>(tile + start + d0);
+#pragma unroll
+            for (uint32_t b = 0; b < 4; ++b)
             {
-                uint32_t at = 0;
-                if (lane == __builtin_ctzll(m))
-                    at = atomicAdd(&nStage, (uint32_t)__popcll(m));
-                at = __shfl(at, __builtin_ctzll(m), 64);
+                const uint32_t d = d0 + b;
+                if (d >= steps)
+                    break;
+                const uint32_t i = start + d;
+                const uint32_t c = i < bytes ? baseCode((word >> (8 * b)) & 0xffu) : 4u;
+                run = c > 3 ? 0 : run + 1;
+                fwd = ((fwd << 2) | (c & 3)) & mask;
+                rc = (rc >> 2) | ((uint64_t)(3 - (c & 3)) << shift);
+                const bool emit = run >= k && base + i + 1 >= skip + k;
+                const uint64_t h = keyMix(min(fwd, rc));
+                const uint64_t m = __ballot(emit);
                 if (emit)
                 {
-                    stage[at + maskedRank(m)] = h;
+                    wstage[wcount + maskedRank(m)] = h;
                     if (((h >> 20) & ((1u << kHllSample) - 1)) == 0)
                     {
                         const uint32_t rank =
@@ -359,19 +368,31 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(const uint8_t *__
                         atomicMax(&reg[h & ((1u << kHllLog) - 1)], rank);
                     }
                 }
+                wcount += (uint32_t)__popcll(m);
                 counted += emit;
             }
         }
+        if (lane == 0)
+            waveCount[wv] = wcount;
         // the tile's hashes go out with one global append
         __syncthreads();
+        uint32_t pre[kWaves + 1];
+        pre[0] = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w)
+            pre[w + 1] = pre[w] + waveCount[w];
+        const uint32_t total = pre[kWaves];
         if (threadIdx.x == 0)
-            stageAt = nStage ? atomicAdd(counters + 8, (unsigned long long)nStage) : 0;
+            stageAt = total ? atomicAdd(counters + 8, (unsigned long long)total) : 0;
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nStage; i += kCountBlock)
-            out[stageAt + i] = stage[i];
-        __syncthreads();
-        if (threadIdx.x == 0)
-            nStage = 0;
+        for (uint32_t i = threadIdx.x; i < total; i += kCountBlock)
+        {
+            int w = 0;
+#pragma unroll
+            for (int q = 1; q < kWaves; ++q)
+                w += i >= pre[q];
+            out[stageAt + i] = stage[w * kWaveStage + (i - pre[w])];
+        }
     }
     counted = waveSum(counted);
     if (lane == 0 && counted)
