@@ -591,7 +591,7 @@ __global__ __launch_bounds__(kSliceBlock) void countSliceKernel(
     uint32_t mine = 0;
     // kSliceAhead hashes per thread are loaded before any is counted, so
     // their loads are in flight together
-    constexpr int kSliceAhead = 4;
+    constexpr int kSliceAhead = 16;
     for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)kSliceBlock * kSliceAhead)
     {
         uint64_t hv[kSliceAhead];
