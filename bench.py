@@ -924,15 +924,18 @@ def run_large(args):
                 return time.time() - t2, r.stdout
             db = os.path.join(root, "sample")
             refs = os.path.join(root, "refs")
+            # the tools build the same kind of table as the in-process scan
+            topts = ([f"--minimizer={args.minimizer_len},{args.minimizer_cap or 16}",
+                      f"--load-factor={args.load_factor}"] if args.minimizer_len else [])
             scan_s, out = run([os.path.join(bindir, "fastibs"), db, refs,
-                               os.path.join(root, "out"), str(W)])
+                               os.path.join(root, "out"), str(W)] + topts)
             tsv = os.path.join(root, "out", f"sample_v_large_{W}.tsv")
             rows = open(tsv, "rb").read().splitlines()[1:]
             tv = np.array([[int(x) for x in r.split(b"\t")[3:7]] for r in rows], dtype=np.int64)
             tsv_same = tv.shape == st.shape and bool(np.array_equal(tv, st))
             log(f"fastibs end to end: {scan_s:.1f} s; TSV equals the device statistics: {tsv_same}")
             map_s, mout = run([os.path.join(bindir, "fastibsmapper"), db, refs,
-                               os.path.join(root, "map")])
+                               os.path.join(root, "map")] + topts)
             mfiles = [f for f in os.listdir(os.path.join(root, "map")) if f.endswith(".txt")]
             mpath = os.path.join(root, "map", mfiles[0])
             msize = os.path.getsize(mpath)

@@ -170,7 +170,8 @@ inline const char *optionsUsage()
            "  --hbm-budget-gb=N     cap on device table memory\n"
            "  --minimizer=M[,CAP]   minimizer table (k <= 32, M in 5..min(31, k), groups of\n"
            "                        up to CAP keys, default 16): runs of positions that share\n"
-           "                        a minimizer read their bucket once; one sample per run\n"
+           "                        a minimizer read their bucket once; slower to build;\n"
+           "                        one sample per run\n"
            "  --db=<folder>         (fastibs) another sample in the same reference pass\n"
            "  --db-list=<file>      (fastibs) sample folders, one per line\n"
            "  --resident=N          (fastibs) at most N sample tables on the device at once\n";
