@@ -130,8 +130,10 @@ __device__ inline bool countAdd(uint64_t *tab, uint64_t nBuckets, uint64_t key, 
 // device-scope store; a thread meeting its own high word reads the low
 // word, waiting (at most kWideSpin reads) while it still reads all ones,
 // and counts there if it is its own. The one canonical code whose low word
-// is all ones (k = 64: A^32 T^32) is counted by the caller outside the
-// table, so a published low word is never all ones.
+// is all ones (k = 64: A^32 T^32) and the one whose high word is all ones
+// (k = 64: T^32 A^32, which would read as a free slot) are counted by the
+// caller outside the table, so a published low word is never all ones and
+// a claimed high word never EMPTY.
 __device__ inline bool countAddWide(uint64_t *tab, uint64_t nBuckets, uint64_t hi, uint64_t lo,
                                     uint64_t add, bool *isNew)
 {
@@ -185,7 +187,8 @@ __device__ inline uint32_t waveSum(uint32_t v)
 
 // Counts the k-mers at positions skip .. n-k of text (16 B aligned).
 // counters: [0] distinct keys stored, [1] k-mers counted, [2] 1 if an
-// insert found no slot, [6] count of the k = 64 code A^32 T^32.
+// insert found no slot, [6] count of the k = 64 code A^32 T^32, [9] count
+// of the k = 64 code T^32 A^32.
 template <bool WIDE>
 __global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__restrict__ text,
                                                            uint64_t n, uint64_t skip, uint32_t k,
@@ -251,6 +254,8 @@ __global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__rest
                     const uint64_t h = f ? fhi : rhi, l = f ? fwd : rc;
                     if (l == EMPTY) // only A^32 T^32 (k = 64)
                         isNew = atomicAdd(counters + 6, 1ull) == 0;
+                    else if (h == EMPTY) // only T^32 A^32 (k = 64)
+                        isNew = atomicAdd(counters + 9, 1ull) == 0;
                     else
                         full |= !countAddWide(tab, nBuckets, h, l, 1, &isNew);
                 }
@@ -875,7 +880,8 @@ struct hipibs_counter : ReadSink
     uint64_t nBuckets = 0;
     unsigned long long *dCounters = nullptr; // [0] distinct, [1] counted, [2] full, [3..5] filter,
                                              // [6] count of A^32 T^32 (k = 64), [7] spilled
-                                             // and [8] hashed k-mers of the open flush
+                                             // and [8] hashed k-mers of the open flush,
+                                             // [9] count of T^32 A^32 (k = 64)
     unsigned long long *hCounters = nullptr; // pinned copy
     // sorted-slice counting (k <= 32): the hashed k-mers of the open flush
     // (dHash[0]; grouped by slice through both buffers, the other one then
@@ -1431,16 +1437,23 @@ struct hipibs_counter : ReadSink
                     dTab, nBuckets, lo, hi, cmax, dKeys, dCounts, cap, dCounters + 3);
             HIPIBS_CHECK(hipGetLastError());
         }
-        HIPIBS_CHECK(hipMemcpyAsync(hCounters + 3, dCounters + 3, 4 * sizeof(unsigned long long),
+        HIPIBS_CHECK(hipMemcpyAsync(hCounters + 3, dCounters + 3, 7 * sizeof(unsigned long long),
                                     hipMemcpyDeviceToHost, stream));
         HIPIBS_CHECK(hipStreamSynchronize(stream));
         filt->kept = hCounters[3];
         filt->below = hCounters[4];
         filt->above = hCounters[5];
-        // the k = 64 code A^32 T^32, counted outside the table
-        const uint64_t sc = hCounters[6];
-        if (sc)
+        // the k = 64 codes A^32 T^32 and T^32 A^32, counted outside the table
+        const struct
         {
+            uint64_t count;
+            uint64_t key[2];
+        } special[2] = {{hCounters[6], {0, EMPTY}}, {hCounters[9], {EMPTY, 0}}};
+        for (const auto &sp : special)
+        {
+            const uint64_t sc = sp.count;
+            if (!sc)
+                continue;
             if (sc < lo)
                 filt->below += 1;
             else if (sc > hi)
@@ -1450,9 +1463,8 @@ struct hipibs_counter : ReadSink
                 const uint64_t o = filt->kept++;
                 if (o < cap)
                 {
-                    const uint64_t key[2] = {0, EMPTY};
                     const uint32_t c = (uint32_t)std::min<uint64_t>(sc, cmax);
-                    HIPIBS_CHECK(hipMemcpy(dKeys + 2 * o, key, 16, hipMemcpyHostToDevice));
+                    HIPIBS_CHECK(hipMemcpy(dKeys + 2 * o, sp.key, 16, hipMemcpyHostToDevice));
                     if (dCounts)
                         HIPIBS_CHECK(hipMemcpy(dCounts + o, &c, 4, hipMemcpyHostToDevice));
                 }

@@ -176,24 +176,38 @@ extern "C" int hipibs_scan_submit(hipibs_scan *s, int slot, uint32_t n_windows, 
     HIPIBS_CHECK(hipSetDevice(s->device));
     const uint64_t seqCopy = n_seq_words + HIPIBS_PACK_PAD_WORDS;
     const uint64_t excCopy = n_exc_words + HIPIBS_PACK_PAD_WORDS;
-    HIPIBS_CHECK(hipMemcpyAsync(sl.dSeq, sl.host.seq_words, seqCopy * 8, hipMemcpyHostToDevice,
-                                sl.stream));
-    HIPIBS_CHECK(hipMemcpyAsync(sl.dExc, sl.host.exc_words, excCopy * 8, hipMemcpyHostToDevice,
-                                sl.stream));
-    if (n_windows)
-        HIPIBS_CHECK(hipMemcpyAsync(sl.dWin, sl.host.windows, n_windows * sizeof(hipibs_window),
-                                    hipMemcpyHostToDevice, sl.stream));
+    // Once a copy is queued, the slot's host buffers are in use: a failure
+    // from then on waits for the stream before returning, so the caller may
+    // refill the slot at once.
+    auto failQueued = [&](int rc)
+    {
+        (void)hipStreamSynchronize(sl.stream);
+        return rc;
+    };
+    hipError_t e = hipMemcpyAsync(sl.dSeq, sl.host.seq_words, seqCopy * 8, hipMemcpyHostToDevice,
+                                  sl.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(sl.dExc, sl.host.exc_words, excCopy * 8, hipMemcpyHostToDevice,
+                           sl.stream);
+    if (e == hipSuccess && n_windows)
+        e = hipMemcpyAsync(sl.dWin, sl.host.windows, n_windows * sizeof(hipibs_window),
+                           hipMemcpyHostToDevice, sl.stream);
+    if (e != hipSuccess)
+        return failQueued(hipFail(e, "scan batch upload"));
     const int nT = (int)s->tables.size();
     int rc = launchScan(s->tables.data(), nT, sl.dSeq, sl.dExc, nBases, sl.dWin, n_windows, maxBases,
                         sl.dStats, 0, sl.dWork, sl.workBytes, sl.stream, s->dViews);
     if (rc)
-        return rc;
+        return failQueued(rc);
     // results are table-major with stride n_windows
     if (n_windows)
-        HIPIBS_CHECK(hipMemcpyAsync(sl.host.stats, sl.dStats,
-                                    (uint64_t)n_windows * nT * sizeof(hipibs_window_stats),
-                                    hipMemcpyDeviceToHost, sl.stream));
-    HIPIBS_CHECK(hipEventRecord(sl.done, sl.stream));
+        e = hipMemcpyAsync(sl.host.stats, sl.dStats,
+                           (uint64_t)n_windows * nT * sizeof(hipibs_window_stats),
+                           hipMemcpyDeviceToHost, sl.stream);
+    if (e == hipSuccess)
+        e = hipEventRecord(sl.done, sl.stream);
+    if (e != hipSuccess)
+        return failQueued(hipFail(e, "scan statistics download"));
     sl.inFlight = true;
     return 0;
 }

@@ -6,8 +6,12 @@
 // in one of the next `max_disp` buckets (linear probing at bucket
 // granularity, wrapping). A probe therefore stops at the first of: key
 // found, an EMPTY slot in the bucket, or max_disp+1 buckets visited.
-// Canonical codes are never all ones (k < 32: < 2^62; k = 32:
-// min(x, ~x) < 2^63), so EMPTY can never collide with a key.
+// Canonical codes of k <= 32 are never all ones (k < 32: < 2^62; k = 32:
+// min(x, ~x) < 2^63), so EMPTY can never collide with a key. Keys of
+// 33..64 bases are (high, low) word pairs whose free mark is an all-ones
+// high word; the one canonical code with such a high word is k = 64's
+// T^32 A^32 (high all ones, low 0; its reverse complement is itself), which
+// a table records in a flag of its own (TableView::wideTop).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -131,7 +135,16 @@ struct TableView
     uint32_t maxDisp2;
     const uint64_t *slots2; // the whole-k-mer part of a minimizer table
     uint64_t nBuckets2;
+    uint32_t wideTop; // k = 64: 1 if the key T^32 A^32 (high word EMPTY, low 0) is present
+    uint32_t pad;
 };
+
+// k = 64's T^32 A^32 as a (high, low) pair: the only canonical code whose
+// high word is EMPTY.
+__host__ __device__ inline bool isWideTop(uint64_t hi, uint64_t lo)
+{
+    return hi == EMPTY && lo == 0;
+}
 
 // The marker of a minimizer group (by its 32-bit minimizer hash) that keeps
 // its keys in the whole-k-mer part: first and last base T, so it is never a
@@ -273,16 +286,19 @@ struct hipibs_table
     uint64_t *dSlots = nullptr;
     uint64_t *dCounters = nullptr; // [0] inserted, [1] full flag, [2] max displacement,
                                    // minimizer tables: [3] keys in the whole-k-mer part,
-                                   // [4] its max displacement, [5] markers
+                                   // [4] its max displacement, [5] markers;
+                                   // k > 32: [3] 1 if T^32 A^32 was inserted
     // minimizer tables only: the whole-k-mer part and the group cap
     uint64_t *dSlots2 = nullptr;
     uint64_t nBuckets2 = 0;
     uint32_t maxDisp2 = 0;
     uint32_t mcap = 0;
     uint64_t nKeys2 = 0, nMarkers = 0;
+    uint32_t wideTop = 0; // k = 64: T^32 A^32 is a key (held outside the buckets)
     hipibs::TableView view() const
     {
-        return hipibs::TableView{dSlots, nBuckets, maxDisp, mlen, k, maxDisp2, dSlots2, nBuckets2};
+        return hipibs::TableView{dSlots,   nBuckets, maxDisp,   mlen,    k,
+                                 maxDisp2, dSlots2,  nBuckets2, wideTop, 0u};
     }
 };
 

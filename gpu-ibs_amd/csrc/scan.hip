@@ -592,15 +592,13 @@ struct ViewCache
 };
 static thread_local ViewCache tViews;
 
+// Every field counts: a freed table's address may come back for a table of
+// another shape (views hold no padding bytes of undefined value: view()
+// sets every member).
 static bool sameViews(const std::vector<TableView> &a, const std::vector<TableView> &b)
 {
-    if (a.size() != b.size())
-        return false;
-    for (size_t i = 0; i < a.size(); ++i)
-        if (a[i].slots != b[i].slots || a[i].nBuckets != b[i].nBuckets ||
-            a[i].maxDisp != b[i].maxDisp)
-            return false;
-    return true;
+    return a.size() == b.size() &&
+           (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(TableView)) == 0);
 }
 
 int uploadViews(const hipibs_table *const *tables, int nTables, hipStream_t s, TableView **out)

@@ -20,6 +20,8 @@
 // batched (the per-window scan is independent of the launch).
 #include "hipibs_internal.hpp"
 
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -55,8 +57,10 @@ struct Sample
     const uint64_t *keys = nullptr;
     std::string kmcPrefix; // set: the table is built from this KMC database
     uint64_t n = 0;
+    uint64_t kmcStage = 0; // KMC samples: device bytes of the record decode's staging
     hipibs_table *table = nullptr;
-    uint64_t bytes = 0;    // bytes counted against the budget (reserved while building)
+    uint64_t bytes = 0;    // bytes counted against the budget: while building the table's
+                           // estimate plus the build's transient memory, then the table's own
     bool building = false; // queued on or running in the builder thread
     int pins = 0;
     uint64_t lastUse = 0;
@@ -162,21 +166,68 @@ bool makeRoom(hipibs_serve *s, uint64_t need, uint64_t candHits, bool force)
         s->bcv.notify_one();
         victim->table = nullptr;
         s->resident -= victim->bytes;
+        victim->bytes = 0;
         s->residentTables -= 1;
         std::lock_guard<std::mutex> lk(s->mu);
         s->counters.evictions += 1;
+        s->counters.resident_bytes = s->resident;
+        s->counters.resident_tables = s->residentTables;
     }
     return true;
 }
 
+// Upper bound of the bytes of a table of n keys built with the engine's
+// options (the sizing rules of the table builds).
 uint64_t tableBytesFor(const hipibs_serve *s, uint64_t n)
 {
     const double lf = s->topts.load_factor > 0 ? s->topts.load_factor : 0.6;
     const uint32_t slots = s->topts.bucket_slots ? s->topts.bucket_slots : 16;
     const uint32_t keysPerBucket = s->k > 32 ? 8 : slots; // k > 32: 8 keys of 16 B
+    if (s->topts.minimizer_len && s->k <= 32)
+    {
+        // Two parts: minimizer-ordered buckets (keys of groups of at most
+        // cap keys + one marker per larger group) and a whole-k-mer part
+        // of 64 B buckets; together at most n keys + n / (cap + 1) markers
+        // at the load factor, each part rounded up to whole buckets (>= 2).
+        const uint32_t cap = s->topts.minimizer_cap ? s->topts.minimizer_cap : 32;
+        const double items = (double)n + std::ceil((double)n / (cap + 1.0));
+        const uint64_t slotBytes = (uint64_t)std::ceil(std::ceil(items / lf) * 8.0);
+        return slotBytes + 3ull * slots * 8 + 3ull * 64;
+    }
     const uint64_t buckets = std::max<uint64_t>(
         2, (uint64_t)std::ceil(std::ceil((double)std::max<uint64_t>(n, 1) / lf) / keysPerBucket));
     return buckets * slots * 8;
+}
+
+// Device bytes a build holds beside its table while it runs: a KMC
+// sample's record staging and decoded keys, and for a minimizer table every
+// key on the device plus the group map (a power of two >= n entries of
+// 8 B). (Each builder thread's host-key staging, two chunks of 32 Mi keys,
+// is allocated once and held outside the budget.)
+uint64_t buildTransientFor(const hipibs_serve *s, const Sample &sm)
+{
+    const uint64_t keyBytes = s->k > 32 ? 16 : 8;
+    uint64_t b = 0;
+    if (!sm.kmcPrefix.empty())
+        b += sm.kmcStage + std::min<uint64_t>(sm.n, 32ull << 20) * keyBytes;
+    if (s->topts.minimizer_len && s->k <= 32)
+    {
+        uint64_t map = 1024;
+        while (map < sm.n)
+            map <<= 1;
+        b += sm.n * keyBytes + map * 8;
+    }
+    return b;
+}
+
+// After a build: the sample's reservation becomes its table's real size.
+void settleBytes(hipibs_serve *s, Sample &sm)
+{
+    uint64_t bytes = sm.bytes;
+    if (sm.table)
+        (void)hipibs_table_info(sm.table, nullptr, &bytes, nullptr, nullptr, nullptr, nullptr);
+    s->resident = s->resident - sm.bytes + bytes;
+    sm.bytes = bytes;
 }
 
 // Builds sample id's table now (preload) -- or, async, queues it on a
@@ -193,9 +244,11 @@ int ensureResident(hipibs_serve *s, uint32_t id, bool async, bool force)
         return 1;
     if (async && s->buildsInFlight >= s->builders.size())
         return 1;
-    const uint64_t need = tableBytesFor(s, sm.n);
-    if (need > s->budget)
+    const uint64_t tableBytes = tableBytesFor(s, sm.n);
+    if (tableBytes > s->budget)
         return setError(HIPIBS_E_NOMEM, "a sample table alone exceeds the serving HBM budget");
+    // the build's transient memory is reserved as far as the budget allows
+    const uint64_t need = std::min(s->budget, tableBytes + buildTransientFor(s, sm));
     if (!makeRoom(s, need, sm.hits, force || !async))
         return 1;
     sm.bytes = need;
@@ -221,6 +274,7 @@ int ensureResident(hipibs_serve *s, uint32_t id, bool async, bool force)
         sm.bytes = 0;
         return rc;
     }
+    settleBytes(s, sm);
     s->residentTables += 1;
     std::lock_guard<std::mutex> lk(s->mu);
     s->counters.admissions += 1;
@@ -312,8 +366,13 @@ int collectBuilds(hipibs_serve *s, bool &progress)
         sm.building = false;
         s->buildsInFlight -= 1;
         if (!b.table)
+        {
+            s->resident -= sm.bytes;
+            sm.bytes = 0;
             return setError(HIPIBS_E_HIP, "table build failed: " + b.error);
+        }
         sm.table = b.table;
+        settleBytes(s, sm);
         s->residentTables += 1;
         progress = true;
         std::lock_guard<std::mutex> lk(s->mu);
@@ -652,6 +711,14 @@ extern "C" int hipibs_serve_add_sample_kmc(hipibs_serve *s, uint32_t sample_id,
     Sample sm;
     sm.kmcPrefix = kmc_prefix;
     sm.n = info.total_kmers;
+    // the decode's staging: two raw record chunks (of at most 32 Mi records),
+    // and the LUT, which the .kmc_pre file holds (its size bounds the LUT's)
+    const uint64_t recBytes =
+        (info.kmer_length - info.lut_prefix_length) / 4 + info.counter_size;
+    struct stat st{};
+    const uint64_t preBytes =
+        stat((std::string(kmc_prefix) + ".kmc_pre").c_str(), &st) == 0 ? (uint64_t)st.st_size : 0;
+    sm.kmcStage = 2 * std::min<uint64_t>(info.total_kmers, 32ull << 20) * recBytes + preBytes;
     return addSample(s, sample_id, std::move(sm), preload);
 }
 

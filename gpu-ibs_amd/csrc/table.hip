@@ -101,8 +101,9 @@ __global__ __launch_bounds__(256) void insertKernel(uint64_t *slots, const Table
 // its own. A low word that really is all ones outlasts the wait and the key
 // is stored in a further slot: a harmless extra copy. A walk ends after
 // kWideMaxWalk buckets (a few at load <= 0.98), then the table reports
-// overflow. A row whose high word is all ones holds no key and is skipped.
-// Counters as insertKernel.
+// overflow. A row whose high word is all ones holds no key and is skipped,
+// except k = 64's T^32 A^32 (low word 0), which sets counters[3] instead
+// of taking a slot. Counters as insertKernel.
 constexpr uint64_t kWideMaxWalk = 4096;
 constexpr uint32_t kWideSpin = 1u << 14;
 __global__ __launch_bounds__(256) void insertWideKernel(uint64_t *slots, const TableView T,
@@ -117,7 +118,13 @@ __global__ __launch_bounds__(256) void insertWideKernel(uint64_t *slots, const T
     {
         const uint64_t hi = keys[2 * i], lo = keys[2 * i + 1];
         if (hi == EMPTY)
-            continue; // no key (the encoder's mark of an invalid position)
+        {
+            // T^32 A^32 is stored as a flag; any other row with an all-ones
+            // high word is no key (the encoder's mark of an invalid position)
+            if (T.k == 64 && isWideTop(hi, lo) && atomicCAS(counters + 3, 0ull, 1ull) == 0ull)
+                localNew += 1;
+            continue;
+        }
         uint64_t b = bucketOfWide(hi, lo, T.nBuckets);
         bool done = false;
         for (uint64_t d = 0; d < walk && !done; ++d)
@@ -612,6 +619,8 @@ static int finishTable(hipibs_table *t, hipStream_t s)
                                                  : "table overflow: no free slot for a key");
     t->nKeys = c[0];
     t->maxDisp = (uint32_t)c[2];
+    if (t->k > 32)
+        t->wideTop = c[3] ? 1u : 0u;
     return 0;
 }
 

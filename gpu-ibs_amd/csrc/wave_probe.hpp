@@ -247,8 +247,10 @@ __device__ inline uint64_t waveProbe(const TableView T, uint64_t code, uint32_t 
 
 // waveProbe for tables of 33..64-base keys: a bucket is 8 keys of 16 B
 // (high word, low word), so 8 lanes read one query's bucket, one key each;
-// a slot whose high word is EMPTY is free (a canonical code's high word is
-// never all ones). Must be called by all 64 lanes.
+// a slot whose high word is EMPTY is free. The one canonical code with an
+// all-ones high word (k = 64: T^32 A^32) never enters the buckets: its
+// presence is T.wideTop, and a lane querying it answers from that flag.
+// Must be called by all 64 lanes.
 __device__ inline uint64_t waveProbeWide(const TableView T, uint64_t hi, uint64_t lo,
                                          uint32_t home, bool valid)
 {
@@ -256,6 +258,9 @@ __device__ inline uint64_t waveProbeWide(const TableView T, uint64_t hi, uint64_
     constexpr uint32_t GMASK = 0xFFu;
     const int lane = laneId();
     const int g = lane / LPQ, sub = lane % LPQ;
+    const bool top = valid && isWideTop(hi, lo);
+    const uint64_t topHit = T.wideTop ? ballot(top) : 0ull;
+    valid = valid && !top;
     const uint64_t V = ballot(valid);
     uint64_t qh[ROUNDS], ql[ROUNDS];
     uint32_t qb[ROUNDS];
@@ -293,7 +298,7 @@ __device__ inline uint64_t waveProbeWide(const TableView T, uint64_t hi, uint64_
         }
     const bool found = ((Mo >> (gOwn * LPQ)) & GMASK) != 0;
     const bool hasEmpty = ((Eo >> (gOwn * LPQ)) & GMASK) != 0;
-    uint64_t H = ballot(valid && found);
+    uint64_t H = ballot(valid && found) | topHit;
     uint64_t NC = ballot(valid && !found && !hasEmpty);
     if (NC == 0 || T.maxDisp == 0)
         return H;
@@ -397,6 +402,18 @@ __device__ inline uint32_t waveWindowMin(uint32_t L, uint32_t Hh, int win)
     return min(L, lane + d < 64 ? l2 : h2);
 }
 
+// Orders this wave's LDS accesses around a hand-over between lanes: the
+// writes before it are visible to every lane of the wave after it, and no
+// access moves across it. Wavefront-scope fences emit no instructions (a
+// wave's LDS operations complete in order); they keep the compiler from
+// reordering the accesses of different lanes, which it cannot see.
+__device__ inline void waveLdsSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Per-wave LDS staging of waveProbeGrouped over NW words: up to kLines home
 // lines, each S slots plus 16 B of padding (lanes that read different lines
 // at the same offset then fall in different banks), and the lines' home
@@ -463,6 +480,7 @@ __device__ inline void waveProbeGrouped(const TableView &T, const uint64_t (&cod
         for (int w = 0; w < NW; ++w)
             if (leader[w] && below[w] - base < (uint32_t)G::kLines)
                 homes[below[w] - base] = home[w];
+        waveLdsSync(); // homes[] are read by the lanes that load the lines
         const uint32_t nLines = min(nL - base, (uint32_t)G::kLines);
         U64x2 kv[ROUNDS];
 #pragma unroll
@@ -480,6 +498,7 @@ __device__ inline void waveProbeGrouped(const TableView &T, const uint64_t (&cod
             if (li < nLines)
                 *reinterpret_cast<U64x2 *>(lines + li * G::kStride + sub * 2) = kv[r];
         }
+        waveLdsSync(); // lines[] are searched by the lanes of their runs
 #pragma unroll
         for (int w = 0; w < NW; ++w)
         {
@@ -504,6 +523,7 @@ __device__ inline void waveProbeGrouped(const TableView &T, const uint64_t (&cod
             A[w] |= ballot(mine && !found && marked);
             NC[w] |= ballot(mine && !found && !marked && !hasEmpty);
         }
+        waveLdsSync(); // the next pass overwrites homes[] and lines[]
     }
 #pragma unroll
     for (int w = 0; w < NW; ++w)

@@ -353,7 +353,8 @@ def sorted_rows(keys, counts):
 def test_wide_counts_match_oracle(tmp_path, k):
     """k = 33..64: FASTQ reads of a small genome (so k-mers repeat) cut into
     64 B pieces with many table growths; at k = 64 the one code whose low
-    word is all ones (A^32 T^32) is among them."""
+    word is all ones (A^32 T^32) and the one whose high word is all ones
+    (T^32 A^32) are among them."""
     rng = np.random.default_rng(k)
     genome = rng.integers(0, 4, 20000).astype(np.uint8)
     reads = []
@@ -367,8 +368,11 @@ def test_wide_counts_match_oracle(tmp_path, k):
             r[int(rng.integers(0, ln))] = 4
         reads.append(r)
     pal = np.array([0] * 32 + [3] * 32, dtype=np.uint8)  # A^32 T^32, its own reverse complement
+    top = np.array([3] * 32 + [0] * 32, dtype=np.uint8)  # T^32 A^32, likewise
     for j in range(3):
         reads.append(np.concatenate([genome[:10], pal, genome[j:j + 5]]))
+    for j in range(4):
+        reads.append(np.concatenate([genome[j:j + 7], top, genome[:3]]))
     path = tmp_path / "r.fq"
     path.write_bytes(b"".join(b"@r%d\n%s\n+\n%s\n" % (i, corpus.to_text(r), b"I" * r.size)
                               for i, r in enumerate(reads)))
@@ -383,8 +387,16 @@ def test_wide_counts_match_oracle(tmp_path, k):
     np.testing.assert_array_equal(keys, want_k)
     np.testing.assert_array_equal(counts, want_c)
     if k == 64:
-        i = np.flatnonzero((want_k[:, 0] == 0) & (want_k[:, 1] == np.uint64(2 ** 64 - 1)))
+        ones = np.uint64(2 ** 64 - 1)
+        i = np.flatnonzero((want_k[:, 0] == 0) & (want_k[:, 1] == ones))
         assert i.size == 1 and want_c[i[0]] >= 3
+        j = np.flatnonzero((want_k[:, 0] == ones) & (want_k[:, 1] == 0))
+        assert j.size == 1 and want_c[j[0]] >= 4
+        # both travel into a table built from the counts, and probe as present
+        t = c.build_table(1, ALL)
+        assert t.info()["n_keys"] == want_k.shape[0]
+        assert t.probe_count(want_k[np.concatenate([i, j])]) == 2
+        t.free()
     f = c.keys(2, 5, 3)[2]
     assert f["kept"] == int(((want_c >= 2) & (want_c <= 5)).sum())
     c.free()

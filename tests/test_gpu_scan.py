@@ -613,6 +613,48 @@ def test_minimizer_serving_equals_oracle(small_corpus):
         assert [tuple(int(x) for x in row) for row in got[rid]] == [w[3:] for w in want], rid
 
 
+def test_minimizer_serving_budget_counts_real_table_bytes(small_corpus):
+    """The serving budget holds minimizer tables at their real size: three
+    samples of the same keys (tables of equal bytes) under a budget of two
+    and a half tables; after every phase the engine's resident bytes are
+    exactly its resident tables' bytes, never above the budget, and every
+    request equals the oracle."""
+    ref, samples = small_corpus
+    seq = corpus.to_text(ref)
+    sw, ew, _ = hipibs.pack_sequences([seq])
+    n = len(seq)
+    keys = samples[0][0]
+    opts = dict(minimizer_len=21, minimizer_cap=8, load_factor=0.3)
+    one_table = hipibs.Table.from_keys(keys, 25, **opts).info()["bytes"]
+    budget = int(2.5 * one_table)
+    eng = hipibs.ServeEngine(25, budget, max_inflight=2, max_windows_per_launch=256, **opts)
+    eng.set_reference(sw, ew, n)
+    for sid in range(3):
+        eng.add_sample(sid, keys)
+    want = {}
+    for phase in range(4):
+        pending = set()
+        for j in range(3):
+            rid = phase * 3 + j
+            sid = (phase + j) % 3
+            eng.submit(rid, sid, 1000 * rid, 200_000, 10_000)
+            pending.add(rid)
+            want[rid] = sid
+        while pending:
+            for rid in eng.poll(timeout_ms=2000):
+                rows = eng.fetch(rid)[0]
+                base = 1000 * rid
+                ref_rows = oracle_scan_rows([b"c"], [seq[base:base + 200_000]], 10_000, 25,
+                                            sorted_keys(keys))
+                assert [tuple(int(x) for x in r) for r in rows] == [w[3:] for w in ref_rows]
+                pending.discard(rid)
+        c = eng.counters()
+        assert c["resident_bytes"] == c["resident_tables"] * one_table, c
+        assert c["resident_bytes"] <= budget
+    assert eng.counters()["evictions"] >= 1
+    eng.free()
+
+
 def coverage_text_gpu(t, ids, seqs, k):
     """fastibsmapper file content from the device hit mask + coverage."""
     parts = []
@@ -679,6 +721,58 @@ def test_wide_tables_match_reference_string_path(tmp_path, k, expected):
     assert t1.probe_count_kmc(p2) == inter
     t1.free()
     t2.free()
+
+
+def test_k64_palindromes_scan_like_the_oracle(tmp_path):
+    """k = 64: T^32 A^32 is the one canonical code whose high word is all ones
+    (the tables' free mark) and A^32 T^32 the one whose low word is. Contigs
+    holding both, with N runs beside them, scan and map like the oracle
+    whether the table holds them (from host keys, from device keys, from a
+    KMC database) or not."""
+    k = 64
+    rng = np.random.default_rng(64)
+    pal = np.array([0] * 32 + [3] * 32, dtype=np.uint8)
+    top = np.array([3] * 32 + [0] * 32, dtype=np.uint8)
+    contigs = []
+    for i in range(6):
+        parts = []
+        for j in range(8):
+            parts.append(rng.integers(0, 4, int(rng.integers(1, 300))).astype(np.uint8))
+            parts.append(top if (i + j) % 2 else pal)
+            if j % 3 == 2:
+                parts.append(np.full(int(rng.integers(1, 70)), 4, np.uint8))
+        contigs.append(np.concatenate(parts))
+    ids = [b"c%d" % i for i in range(len(contigs))]
+    seqs = [corpus.to_text(c) for c in contigs]
+    every = sorted_keys_wide(np.concatenate([corpus.sequence_kmers_wide([c], k) for c in contigs]))
+    ones = np.uint64(2 ** 64 - 1)
+    is_top = (every[:, 0] == ones) & (every[:, 1] == 0)
+    is_pal = (every[:, 0] == 0) & (every[:, 1] == ones)
+    assert is_top.sum() == 1 and is_pal.sum() == 1
+    half = every[rng.random(every.shape[0]) < 0.5]
+    with_both = sorted_keys_wide(np.concatenate([half, every[is_top | is_pal]]))
+    without_top = np.ascontiguousarray(with_both[~((with_both[:, 0] == ones) &
+                                                   (with_both[:, 1] == 0))])
+    for keys, name in ((with_both, "with"), (without_top, "without")):
+        p = str(tmp_path / f"db_{name}")
+        hipibs.kmc_write(p, keys, k, counts=None, sig_len=9, counter_size=1, version=0x200,
+                         min_count=1, max_count=255)
+        d = D.from_array(np.ascontiguousarray(keys[::-1]))
+        tabs = [hipibs.Table.from_keys(keys, k), hipibs.Table.from_kmc(p),
+                hipibs.Table.from_device_keys(d.ptr, keys.shape[0], k)]
+        for W in (100, 1000):
+            note(f"k=64 palindromes ({name} T^32A^32) W={W}")
+            want = oracle_scan_rows_wide(ids, seqs, W, k, keys)
+            for t, rows in zip(tabs, hipibs.scan_sequences(tabs, ids, seqs, W)):
+                assert t.info()["n_keys"] == keys.shape[0]
+                assert rows == want
+        assert tabs[0].probe_count(every) == keys.shape[0]
+        cov = coverage_text_gpu(tabs[1], ids, seqs, k)
+        from util import oracle_coverage_text_wide
+        assert cov == oracle_coverage_text_wide(ids, seqs, k, keys)
+        for t in tabs:
+            t.free()
+        d.free()
 
 
 def test_wide_device_build_stores_repeated_keys_once():
