@@ -314,7 +314,7 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
                const uint64_t *dExc, uint64_t nBases, const hipibs_window *dWindows,
                uint32_t nWindows, uint32_t maxWindowBases, hipibs_window_stats *dStats,
                uint32_t segsPerWindow, void *dWork, uint64_t workBytes, hipStream_t s,
-               TableView *dViews);
+               TableView *dViews, bool profile = true);
 int uploadViews(const hipibs_table *const *tables, int nTables, hipStream_t s, TableView **out);
 // grouped: every table the windows name is a minimizer table (probed with
 // waveProbeGrouped).

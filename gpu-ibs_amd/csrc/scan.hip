@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace hipibs
@@ -496,7 +497,7 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
                const uint64_t *dExc, uint64_t nBases, const hipibs_window *dWindows,
                uint32_t nWindows, uint32_t maxWindowBases, hipibs_window_stats *dStats,
                uint32_t segsPerWindow, void *dWork, uint64_t workBytes, hipStream_t s,
-               TableView *dViews)
+               TableView *dViews, bool profile)
 {
     const uint32_t k = tables[0]->k;
     const uint32_t maxPos = maxWindowBases >= k ? maxWindowBases - k + 1 : 0;
@@ -509,7 +510,8 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
         return 0;
     const dim3 grid(nWindows * segs, (uint32_t)nTables);
     auto *work = reinterpret_cast<SegSummary *>(dWork);
-    if (tProfile)
+    profile = profile && tProfile;
+    if (profile)
     {
         if (!tEv[0])
         {
@@ -534,7 +536,7 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
         hipLaunchKernelGGL((scanKernel<16, false>), grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc,
                            nBases, dWindows, nWindows, segs, k, dStats, work);
     HIPIBS_CHECK(hipGetLastError());
-    if (tProfile)
+    if (profile)
     {
         HIPIBS_CHECK(hipEventRecord(tEv[1], s));
         tEvRecorded = true;
@@ -895,6 +897,110 @@ extern "C" int hipibs_scan_windows_device(const hipibs_table *const *tables, int
         return rc;
     return launchScan(tables, n_tables, d_seq_words, d_exc_words, n_bases, d_windows, n_windows,
                       max_window_bases, d_stats, segs_per_window, d_work, work_bytes, s, views);
+}
+
+// ---- the scan without allocation -----------------------------------------
+// Scratch of hipibs_scan_windows_device_noalloc: the launch's table views
+// (at most kNoAllocTables, in the first kNoAllocViewBytes), then the
+// segment summaries of a window split over several blocks. One copy per
+// GPU, held until the process ends; every call on a GPU shares it (calls
+// that overlap on one GPU race). Nothing in it needs clearing: every block
+// writes its summary before the fold reads it.
+constexpr int kNoAllocTables = 64;
+constexpr uint64_t kNoAllocViewBytes = 4096;
+constexpr uint64_t kNoAllocScratchBytes = 16ull << 20;
+static_assert(kNoAllocTables * sizeof(TableView) <= kNoAllocViewBytes, "views exceed their part");
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+// Device address of `scratch` per device (the host-side name is not one):
+// taken once per device, with that device current.
+constexpr int kMaxDevices = 64;
+static std::mutex gScratchMu;
+static void *gScratch[kMaxDevices] = {};
+
+static int scratchOfDevice(int device, uint8_t **out)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return setError(HIPIBS_E_ARG, "device ordinal out of range for the scan scratch");
+    std::lock_guard<std::mutex> lk(gScratchMu);
+    if (!gScratch[device])
+    {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return hipFail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                           "hipGetSymbolAddress of the scan scratch");
+        gScratch[device] = p;
+    }
+    *out = static_cast<uint8_t *>(gScratch[device]);
+    return 0;
+}
+
+// Table views travel as kernel arguments (no host buffer outlives the call).
+constexpr int kViewsPerLaunch = 16;
+struct ViewBatch
+{
+    TableView v[kViewsPerLaunch];
+};
+
+__global__ void setViewsKernel(const ViewBatch b, uint32_t n, TableView *dst)
+{
+    if (threadIdx.x < n)
+        dst[threadIdx.x] = b.v[threadIdx.x];
+}
+
+extern "C" uint64_t hipibs_scan_noalloc_max_work_bytes(void)
+{
+    return kNoAllocScratchBytes - kNoAllocViewBytes;
+}
+
+extern "C" int hipibs_scan_windows_device_noalloc(const hipibs_table *const *tables, int n_tables,
+                                                  const uint64_t *d_seq_words,
+                                                  const uint64_t *d_exc_words, uint64_t n_bases,
+                                                  const hipibs_window *d_windows,
+                                                  uint32_t n_windows, uint32_t max_window_bases,
+                                                  hipibs_window_stats *d_stats,
+                                                  uint32_t segs_per_window, void *stream)
+{
+    int rc = checkTables(tables, n_tables);
+    if (rc)
+        return rc;
+    if (n_tables > kNoAllocTables)
+        return setError(HIPIBS_E_ARG, "the scan without allocation takes at most " +
+                                          std::to_string(kNoAllocTables) + " tables");
+    if (n_windows && (!d_seq_words || !d_exc_words || !d_windows || !d_stats))
+        return setError(HIPIBS_E_ARG, "null buffer");
+    const uint32_t k = tables[0]->k;
+    const uint32_t maxPos = max_window_bases >= k ? max_window_bases - k + 1 : 0;
+    const uint32_t segs =
+        segs_per_window ? segs_per_window : autoSegsPerWindow(n_windows, n_tables, maxPos);
+    const uint64_t work = scanWorkBytes(n_windows, n_tables, segs);
+    if (work > kNoAllocScratchBytes - kNoAllocViewBytes)
+        return setError(HIPIBS_E_ARG,
+                        "windows x tables x segments need " + std::to_string(work) +
+                            " bytes of scratch, more than the " +
+                            std::to_string(kNoAllocScratchBytes - kNoAllocViewBytes) +
+                            " the scan without allocation holds: fewer windows per call or "
+                            "segs_per_window = 1");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    HIPIBS_CHECK(hipSetDevice(tables[0]->device));
+    uint8_t *base = nullptr;
+    rc = scratchOfDevice(tables[0]->device, &base);
+    if (rc)
+        return rc;
+    auto *views = reinterpret_cast<TableView *>(base);
+    for (int c = 0; c < n_tables; c += kViewsPerLaunch)
+    {
+        ViewBatch b{};
+        const int m = std::min(kViewsPerLaunch, n_tables - c);
+        for (int i = 0; i < m; ++i)
+            b.v[i] = tables[c + i]->view();
+        hipLaunchKernelGGL(setViewsKernel, dim3(1), dim3(64), 0, s, b, (uint32_t)m, views + c);
+        HIPIBS_CHECK(hipGetLastError());
+    }
+    return launchScan(tables, n_tables, d_seq_words, d_exc_words, n_bases, d_windows, n_windows,
+                      max_window_bases, d_stats, segs, base + kNoAllocViewBytes,
+                      kNoAllocScratchBytes - kNoAllocViewBytes, s, views, false);
 }
 
 extern "C" int hipibs_hitmask_device(const hipibs_table *t, const uint64_t *d_seq_words,

@@ -132,6 +132,10 @@ _SIGS = [
                                              C.c_uint64, vp]),
     ("hipibs_scan_work_bytes", C.c_uint64, [C.c_uint32, C.c_int, C.c_uint32, C.c_uint32,
                                             C.c_uint32, u32p]),
+    ("hipibs_scan_windows_device_noalloc", C.c_int, [C.POINTER(vp), C.c_int, vp, vp,
+                                                     C.c_uint64, vp, C.c_uint32, C.c_uint32, vp,
+                                                     C.c_uint32, vp]),
+    ("hipibs_scan_noalloc_max_work_bytes", C.c_uint64, []),
     ("hipibs_hitmask_device", C.c_int, [vp, vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, vp]),
     ("hipibs_coverage_device", C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint32, vp, vp]),
     ("hipibs_encode_device", C.c_int, [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32,

@@ -312,6 +312,27 @@ int hipibs_scan_windows_device(const hipibs_table *const *tables, int n_tables,
                                hipibs_window_stats *d_stats, uint32_t segs_per_window,
                                void *d_work, uint64_t work_bytes, void *stream);
 
+/* hipibs_scan_windows_device for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). No counterpart in the reference. It calls no allocating
+ * routine: its scratch (the launch's table views and segment summaries) is
+ * a library-held device array of 16 MiB per GPU, kept until the process
+ * ends. Limits: at most 64 tables, and n_windows x n_tables x (resolved
+ * segments per window, when above 1) x 24 B at most
+ * hipibs_scan_noalloc_max_work_bytes(); larger calls fail with
+ * HIPIBS_E_ARG. Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (on different streams, or from several threads) race:
+ * serialise them. Asynchronous on `stream`. */
+int hipibs_scan_windows_device_noalloc(const hipibs_table *const *tables, int n_tables,
+                                       const uint64_t *d_seq_words, const uint64_t *d_exc_words,
+                                       uint64_t n_bases, const hipibs_window *d_windows,
+                                       uint32_t n_windows, uint32_t max_window_bases,
+                                       hipibs_window_stats *d_stats, uint32_t segs_per_window,
+                                       void *stream);
+
+/* Segment-summary bytes hipibs_scan_windows_device_noalloc can hold. */
+uint64_t hipibs_scan_noalloc_max_work_bytes(void);
+
 /* Scratch bytes of hipibs_scan_windows_device for these arguments (0 when
  * the resolved split is one segment per window); *segs_resolved (nullable)
  * receives the split the library will use. */
