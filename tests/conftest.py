@@ -4,6 +4,16 @@ import sys
 
 import pytest
 
+# torch (used by the full-size configuration tests for device memory and
+# random generation) brings its own copy of the HIP runtime, loaded under a
+# file name of its own: imported after libhipibs.so it would be a second HIP
+# runtime in the process, and one of the two then finds no device. Imported
+# first, it is the process's one runtime and libhipibs.so binds to it.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (os.path.join(ROOT, "gpu-ibs_amd"), os.path.join(ROOT, "tests"),
           os.path.join(ROOT, "tests", "golden")):
