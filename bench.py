@@ -57,7 +57,11 @@ def parse():
     ap.add_argument("--segs", type=int, default=0, help="segments per window (0 = auto)")
     ap.add_argument("--cpu-mbp", type=float, default=1536.0,
                     help="reference slice (Mbp) of the CPU baseline sample; 0 = skip")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0 = every CPU of the process's affinity, "
+                         "capped by OMP_NUM_THREADS when set)")
+    ap.add_argument("--cpu-passes", type=int, default=3,
+                    help="scan workload: passes of the reference's CPU whole scan (median)")
     ap.add_argument("--seed", type=int, default=0x1B50200)
     ap.add_argument("--serve-samples", type=int, default=96)
     ap.add_argument("--serve-min-keys", type=float, default=0.075e9)
@@ -79,15 +83,19 @@ def parse():
     ap.add_argument("--wide-cpu-mbp", type=float, default=8.0,
                     help="wide workload: Mbp of the reference's string-path CPU sample")
     ap.add_argument("--workload", default="scan",
-                    choices=["scan", "panel", "serve", "large", "count", "wide"],
+                    choices=["scan", "panel", "serve", "large", "count", "wide", "gups", "copy"],
                     help="scan: the single-sample device scan (default line); panel: "
                          "end-to-end fastibs runs, N single-sample runs vs one panel run; "
                          "serve: served queries; large: the k=31 4.2 Gbp scan + coverage; "
-                         "count: k-mer counting from reads; wide: the scan at k = 47")
+                         "count: k-mer counting from reads; wide: the scan at k = 47; "
+                         "gups: the random-bucket read ceiling (--gups-mode); copy: a plain "
+                         "2 GiB copy that calibrates rocprofv3's byte counters")
+    ap.add_argument("--gups-mode", default="sizes", choices=["sizes", "policies", "alloc"])
     ap.add_argument("--cpu-mapper-mbp", type=float, default=8.0,
                     help="large: bases (Mbp) of the reference mapper's CPU sample")
     ap.add_argument("--e2e", type=int, default=1,
-                    help="large: also run fastibs / fastibsmapper end to end from files")
+                    help="scan, large: also run the tools end to end from files (scan: "
+                         "fastibs, and the reference's CPU whole scan of the same files)")
     ap.add_argument("--panel-samples", type=int, default=48)
     ap.add_argument("--panel-min-keys", type=float, default=0.05e9)
     ap.add_argument("--panel-max-keys", type=float, default=0.625e9)
@@ -219,6 +227,181 @@ def make_sample_keys(torch, hap_codes, target, k, seed):
 
 # ---- CPU baseline (the reference's CPU-BATCHED scan, compiled from its sources) ----
 
+def cpu_threads(args):
+    """Threads of the CPU baseline: every CPU this process may run on
+    (sched_getaffinity), capped by OMP_NUM_THREADS when the machine sets it
+    (a GPU box allots one GPU's job its CPU share through it); --cpu-threads
+    overrides."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def host_info():
+    """nproc, affinity, the full CPU model, THP mode and NUMA layout of the
+    host (the records the reference's CPU baseline protocol asks for)."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        fields = dict(l.split(":", 1) for l in out.splitlines() if ":" in l)
+        fields = {k.strip(): v.strip() for k, v in fields.items()}
+        info["cpu_model"] = fields.get("Model name")
+        info["sockets"] = fields.get("Socket(s)")
+        info["threads_per_core"] = fields.get("Thread(s) per core")
+        info["numa_nodes"] = fields.get("NUMA node(s)")
+        info["numa_cpus"] = {k: v for k, v in fields.items()
+                             if k.startswith("NUMA node") and "CPU" in k}
+    except Exception:
+        pass
+    for name, path in (("thp_enabled", "/sys/kernel/mm/transparent_hugepage/enabled"),
+                       ("thp_defrag", "/sys/kernel/mm/transparent_hugepage/defrag")):
+        try:
+            with open(path) as f:
+                info[name] = f.read().strip()
+        except OSError:
+            info[name] = None
+    try:
+        with open("/proc/meminfo") as f:
+            info["mem_total_kb"] = int(next(l for l in f if l.startswith("MemTotal")).split()[1])
+    except Exception:
+        pass
+    return info
+
+
+def run_cmd(cmd):
+    """(wall seconds, stdout) of a command that must succeed."""
+    t0 = time.time()
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{os.path.basename(cmd[0])} failed: {r.stderr[-2000:]}")
+    return time.time() - t0, r.stdout
+
+
+def fastibs_phases(out):
+    """The phase lines fastibs prints: table build and scan phases."""
+    import re
+    ph = {}
+    m = re.search(r"built in ([0-9.e+-]+) s", out)
+    if m:
+        ph["table_build_s"] = float(m.group(1))
+    m = re.search(r"Scan phases: index ([0-9.e+-]+) s, read ([0-9.e+-]+) s, windows "
+                  r"([0-9.e+-]+) s, pipeline ([0-9.e+-]+) s \((\d+) batches, pack ([0-9.e+-]+) "
+                  r"s, device wait ([0-9.e+-]+) s\), write ([0-9.e+-]+) s", out)
+    if m:
+        keys = ["index_s", "fasta_read_s", "windows_s", "pipeline_s", "batches", "pack_s",
+                "device_wait_s", "write_s"]
+        ph.update({k: float(v) for k, v in zip(keys, m.groups())})
+    return ph
+
+
+def end_to_end_and_cpu(torch, ref_codes, host_keys, wins, st, args, n):
+    """The drop-in end to end on this workload, and the reference's
+    CPU-BATCHED whole scan beside it. The reference is written as FASTA and
+    the sample as a KMC2 database (p = 9, 9-base signatures, 1-byte
+    counters); then (1) `fastibs` runs with its default table and with the
+    other table kind, wall time and phases, each TSV checked against the
+    device statistics of the timed scan; (2) the reference's own code
+    (KmcRaIndex + readSequenceFile + chunkSequences + probeBatched K=16 +
+    statistics + TSV, compiled from its sources) scans the same files on
+    the host's threads, args.cpu_passes times; median probe rate and whole
+    scan time; its TSV must equal fastibs'. Files are read from the page
+    cache (written just before)."""
+    import shutil
+
+    import numpy as np
+
+    import hipibs
+    root = os.path.join(args.workdir, "hipibs_bench_e2e")
+    shutil.rmtree(root, ignore_errors=True)
+    os.makedirs(os.path.join(root, "refs"))
+    free = shutil.disk_usage(root).free
+    need = n * 1.05 + host_keys.size * 6.0 + 2e9
+    if need > free * 0.9:
+        log(f"end to end skipped: {need/1e9:.0f} GB of files, {free/1e9:.0f} GB free")
+        return None, None, None
+    t0 = time.time()
+    fasta = os.path.join(root, "refs", "c2_genome.fasta")
+    write_fasta(fasta, ref_codes.cpu().numpy())
+    db = os.path.join(root, "c2_sample")
+    os.makedirs(db)
+    hipibs.kmc_write(os.path.join(db, "sample"), host_keys, K, p=9, sig_len=9, counter_size=1,
+                     version=0x200, min_count=1, max_count=255)
+    log(f"end to end: FASTA + KMC2 database ({host_keys.size/1e9:.2f} G keys) written in "
+        f"{time.time()-t0:.1f} s")
+    want_tsv = hipibs.format_tsv([(b"chr1", w[1], w[2]) + tuple(int(x) for x in st[i])
+                                  for i, w in enumerate(wins)])
+    fastibs = os.path.join(ROOT, "gpu-ibs_amd", "bin", "fastibs")
+    name = f"c2_sample_v_c2_{W}.tsv"
+    e2e = {}
+    tsvs = {}
+    for label, extra in (("default_table", []),
+                         ("minimizer_21_16_load_0.2", ["--minimizer=21,16", "--load-factor=0.2"]),
+                         ("whole_kmer_load_0.6", ["--minimizer=0", "--load-factor=0.6"])):
+        out_dir = os.path.join(root, "out_" + label)
+        try:
+            wall, out = run_cmd([fastibs, db, os.path.join(root, "refs"), out_dir, str(W)] + extra)
+        except RuntimeError as e:
+            log(f"fastibs {label}: {e}")
+            continue
+        tsv = open(os.path.join(out_dir, name), "rb").read()
+        tsvs[label] = tsv
+        tbl = [l for l in out.splitlines() if l.startswith("Device table")]
+        e2e[label] = {"wall_s": wall, "args": extra, **fastibs_phases(out),
+                      "table": tbl[0] if tbl else None,
+                      "tsv_equals_device_stats": tsv == want_tsv}
+        log(f"fastibs {label}: {wall:.2f} s end to end; {e2e[label]}")
+    # the reference's CPU-BATCHED whole scan of the same files
+    cpu = None
+    lib_path = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+    if os.path.exists(lib_path) and args.cpu_passes > 0:
+        ref = C.CDLL(lib_path)
+        f = ref.ref_batched_whole_scan
+        f.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_void_p,
+                      C.POINTER(C.c_longlong)]
+        f.restype = C.c_longlong
+        threads = cpu_threads(args)
+        passes = []
+        same = True
+        for i in range(args.cpu_passes):
+            ph = np.zeros(5)
+            probes = C.c_longlong(0)
+            out_tsv = os.path.join(root, f"cpu_{i}.tsv")
+            t1 = time.time()
+            nw = f(os.path.join(db, "sample").encode(), fasta.encode(), W, out_tsv.encode(), 16,
+                   threads, ph.ctypes.data, C.byref(probes))
+            wall = time.time() - t1
+            if nw != len(wins):
+                raise RuntimeError(f"reference whole scan returned {nw}")
+            tsv = open(out_tsv, "rb").read()
+            same &= tsv == want_tsv
+            passes.append({"whole_scan_s": wall, "index_load_s": ph[0], "fasta_read_s": ph[1],
+                           "windows_s": ph[2], "probe_stats_s": ph[3], "write_s": ph[4],
+                           "probes": probes.value})
+            log(f"cpu pass {i}: whole scan {wall:.1f} s (index {ph[0]:.1f}, read {ph[1]:.1f}, "
+                f"probe+stats {ph[3]:.1f} s, {probes.value/ph[3]/1e6:.1f} M probes/s on "
+                f"{threads} threads); TSV equals the GPU's: {tsv == want_tsv}")
+        med = lambda key: float(np.median([p[key] for p in passes]))
+        probes = passes[0]["probes"]
+        cpu = {"value": probes / med("probe_stats_s"), "unit": "ref k-mers probed/s",
+               "cores": threads, "kind": "reference",
+               "sample": f"the whole workload: {n/1e9:.1f} Gbp reference (FASTA) vs the "
+                         f"{host_keys.size/1e9:.2f} G-key sample as a KMC2 database; the "
+                         f"reference's CPU-BATCHED whole scan (KmcRaIndex load, "
+                         f"readSequenceFile, chunkSequences, probeBatched K=16, codec, "
+                         f"run-length statistics, TSV), {threads} threads, median of "
+                         f"{len(passes)} passes; value = probes / probe-phase seconds",
+               "whole_scan_s": med("whole_scan_s"), "probe_phase_s": med("probe_stats_s"),
+               "index_load_s": med("index_load_s"), "fasta_read_s": med("fasta_read_s"),
+               "passes": passes, "tsv_identical_to_gpu": same, "host": host_info()}
+    shutil.rmtree(root, ignore_errors=True)
+    return e2e, cpu, (tsvs.get("default_table") == want_tsv if tsvs else None)
+
+
 def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_check, k=K,
                  keys_target=None, mapper_bp=0, gpu_coverage=None):
     """The reference's CPU-BATCHED scan on a slice of the reference, and
@@ -241,7 +424,7 @@ def cpu_baseline(torch, ref_codes, sw, ew, hap_codes_slice, n_slice, args, gpu_c
                                      C.c_int, C.c_int, C.c_void_p]
     ref.ref_batched_scan.restype = C.c_double
     ref.ref_coverage.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p]
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = cpu_threads(args)
     frac = n_slice / ref_codes.numel()
     keys = make_sample_keys(torch, hap_codes_slice, (keys_target or args.keys) * frac, k,
                             args.seed + 99)
@@ -1388,6 +1571,86 @@ def run_count(args):
     print(json.dumps(out), flush=True)
 
 
+def run_gups(args):
+    """Random-bucket read rate vs working-set size, load policy and
+    allocation kind (gpu-ibs_amd/bench/gups.hip): the ceiling the probes are
+    measured against. --gups-mode sizes: buffers of 2 MB-27 GB, 64 B and
+    128 B buckets, 4 or 8 loads in flight per wave; policies: 0 plain, 1
+    non-temporal (as the table probes load), 2 / 3 relaxed 8 B atomic loads
+    at system / device scope, 4 sc0, 5 sc0 nt, 6 sc1 nt, 7 nt written in
+    asm (a control for 1), at 256 MB and 27 GB; alloc: hipMalloc's default,
+    fine-grained and uncached device memory at 24 GiB (an uncached buffer
+    is never freed). One line per measurement on stdout."""
+    import hipibs
+    b = C.CDLL(os.path.join(ROOT, "gpu-ibs_amd", "lib", "libhipibs_bench.so"))
+    b.hipibs_bench_gups_policy.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64,
+                                           C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                           C.c_void_p]
+    b.hipibs_bench_gups_reads.argtypes = [C.c_uint32, C.c_uint64, C.c_int, C.c_int]
+    b.hipibs_bench_gups_reads.restype = C.c_uint64
+    b.hipibs_bench_alloc.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]
+    b.hipibs_bench_free.argtypes = [C.c_void_p, C.c_int]
+    N = 500_000_000
+
+    def rate(ptr, mb, S, rounds, policy, wpc=32):
+        nb = (mb << 20) // (S * 8)
+        t = C.c_double(0)
+        for _ in range(2):  # warm-up, then the measurement
+            rc = b.hipibs_bench_gups_policy(ptr, nb, S, N, rounds, wpc, policy, C.byref(t), None)
+        if rc != 0:
+            return None
+        reads = b.hipibs_bench_gups_reads(S, N, rounds, wpc)
+        return reads / (t.value / 1e3) / 1e9, reads * S * 8 / (t.value / 1e3) / 1e9
+
+    if args.gups_mode == "alloc":
+        mb = 24 << 10
+        for kind, name in enumerate(("default", "fine-grained", "uncached")):
+            p = C.c_void_p()
+            if b.hipibs_bench_alloc(mb << 20, kind, C.byref(p)) != 0:
+                print(f"{name}: allocation failed", flush=True)
+                continue
+            for S in (8, 16):
+                for policy in (0, 1):
+                    r = rate(p.value, mb, S, 8, policy)
+                    print(f"alloc {name:12s} size {mb:6d} MB bucket {S * 8:3d} B policy {policy}: "
+                          f"{r[0]:7.1f} G buckets/s", flush=True)
+            if kind != 2:
+                b.hipibs_bench_free(p, kind)
+    elif args.gups_mode == "policies":
+        for mb in (256, 27000):
+            buf = hipibs.DeviceBuffer.zeros(mb << 20)
+            for policy in range(8):
+                for S in (8, 16):
+                    r = rate(buf.ptr, mb, S, 4, policy)
+                    if r:
+                        print(f"size {mb:6d} MB bucket {S * 8:3d} B policy {policy}: "
+                              f"{r[0]:7.1f} G buckets/s", flush=True)
+            buf.free()
+    else:
+        for mb in [2, 16, 64, 128, 192, 256, 384, 512, 1024, 4096, 27000]:
+            buf = hipibs.DeviceBuffer.zeros(mb << 20)
+            for S in (8, 16):
+                for rounds in (4, 8):
+                    r = rate(buf.ptr, mb, S, rounds, 0)
+                    print(f"size {mb:6d} MB bucket {S * 8:3d} B rounds {rounds}: "
+                          f"{r[0]:7.1f} G buckets/s  {r[1]:7.0f} GB/s", flush=True)
+            buf.free()
+
+
+def run_copy(args):
+    """A plain 2 GiB device-to-device copy, three times: the known-size
+    reference that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters
+    (run it under `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`)."""
+    import torch
+    x = torch.ones(1 << 28, dtype=torch.int64, device="cuda")  # 2 GiB
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("copied 3 x 2 GiB", flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -1404,6 +1667,10 @@ def main():
         return run_count(args)
     if args.workload == "wide":
         return run_wide(args)
+    if args.workload == "gups":
+        return run_gups(args)
+    if args.workload == "copy":
+        return run_copy(args)
     import numpy as np
     import torch
 
@@ -1429,8 +1696,6 @@ def main():
     hap_codes = encode(torch, hsw, hew, n, K)
     del hsw, hew
     torch.cuda.synchronize()
-    cpu_slice = int(args.cpu_mbp * 1e6) // 64 * 64 if args.cpu_mbp > 0 else 0
-    hap_slice = hap_codes[:cpu_slice - K + 1].clone() if cpu_slice else None
     keys = make_sample_keys(torch, hap_codes, args.keys, K, args.seed)
     del hap_codes
     torch.cuda.synchronize()
@@ -1443,7 +1708,7 @@ def main():
                                           minimizer_cap=args.minimizer_cap)
     build_s = time.time() - tb
     info = table.info()
-    n_keys = keys.numel()
+    host_keys = keys.cpu().numpy().view(np.uint64) if args.e2e else None
     del keys
     torch.cuda.empty_cache()
     log(f"table: {info['n_keys']/1e9:.3f} G keys, {info['bytes']/1e9:.1f} GB, build {build_s:.2f} s")
@@ -1523,33 +1788,28 @@ def main():
     except Exception as e:
         log(f"gups ceiling skipped: {e}")
 
-    cpu = None
-    cpu_match = None
-    if cpu_slice:
-        def gpu_check(t, n_slice, swins):
-            wv = np.zeros(len(swins), dtype=hipibs.WINDOW_DTYPE)
-            wv["base"] = np.array([w[1] for w in swins], dtype=np.uint64)
-            wv["n_bases"] = np.array([w[2] - w[1] for w in swins], dtype=np.uint32)
-            dw = torch.from_numpy(wv.view(np.uint8)).cuda()
-            ds = torch.zeros(len(swins) * 4, dtype=torch.int32, device="cuda")
-            s2 = C.c_uint32(0)
-            wb2 = hipibs._lib.hipibs_scan_work_bytes(len(swins), 1, W, K, 0, C.byref(s2))
-            dwk = torch.zeros(max(wb2, 8), dtype=torch.uint8, device="cuda")
-            hipibs._check(hipibs._lib.hipibs_scan_windows_device(
-                hipibs._table_array([t]), 1, sw.data_ptr(), ew.data_ptr(), n_slice,
-                dw.data_ptr(), len(swins), W, ds.data_ptr(), 0, dwk.data_ptr(), wb2, None),
-                "scan")
-            torch.cuda.synchronize()
-            return ds.cpu().numpy().reshape(-1, 4)
+    # the drop-in end to end (files -> fastibs -> TSV) and the reference's
+    # CPU whole scan of the same files; the device's table and buffers are
+    # released first (fastibs builds its own)
+    kernel_name = (f"scanKernel<{info['bucket_slots']}, grouped>" if info["minimizer_len"]
+                   else f"scanKernel<{info['bucket_slots']}>")
+    table.free()
+    table = None
+    del d_work, d_stats, d_win, sw, ew
+    torch.cuda.empty_cache()
+    e2e = cpu = e2e_match = None
+    if args.e2e:
         try:
-            cpu, cpu_match = cpu_baseline(torch, ref_codes, sw, ew, hap_slice, cpu_slice, args,
-                                          gpu_check)
-        except Exception as e:  # the baseline is reported, never required
-            log(f"cpu baseline failed: {e}")
+            e2e, cpu, e2e_match = end_to_end_and_cpu(torch, ref_codes, host_keys, wins, st, args,
+                                                     n)
+        except Exception as e:  # reported, never required
+            log(f"end to end / cpu baseline failed: {e}")
+    del host_keys
     # HBM traffic per launch, measured with rocprofv3 --pmc on this default
     # workload and calibrated against a plain copy kernel
     traffic = None
     requests_per_probe = None
+    traffic_source = None
     tpath = os.path.join(ROOT, "profiles", "scan_k25_2gbp_traffic.json")
     if os.path.exists(tpath) and n == 2_000_000_000 and abs(args.keys - 2e9) < 1:
         with open(tpath) as f:
@@ -1560,6 +1820,10 @@ def main():
                     tj["minimizer_cap"] == (args.minimizer_cap if info["minimizer_len"] else 0)
                     and abs(tj["load_factor"] - args.load_factor) < 1e-9):
                 traffic = tj["traffic_bytes_per_dispatch"]
+                traffic_source = ("stored: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this "
+                                  "workload and table geometry, calibrated by a plain copy "
+                                  "(profiles/scan_k25_2gbp_traffic.json); not measured in "
+                                  "this run")
                 rq = tj["ea_read_requests_per_dispatch"]
                 requests_per_probe = (rq["128B"] + rq["64B"] + rq["32B"]) / probes
     value = probes / (ms / 1e3)
@@ -1582,16 +1846,22 @@ def main():
                    "sample_keys": int(info["n_keys"]), "table_bytes": int(info["bytes"]),
                    "bucket_slots": info["bucket_slots"], "load_factor": args.load_factor,
                    "minimizer_len": info["minimizer_len"], "minimizer_cap": args.minimizer_cap,
-                   "segs_per_window": segs.value, "scan_wall_s": ms / 1e3,
+                   "segs_per_window": segs.value,
+                   "scan_wall_s": (e2e or {}).get("default_table", {}).get("wall_s"),
+                   "scan_wall_s_note": "the whole fastibs command (KMC2 database and FASTA "
+                                       "from the page cache -> TSV) with its default table; "
+                                       "per-table runs and phases in end_to_end",
+                   "kernel_ms": k_ms, "device_step_ms": ms,
                    "table_build_s": build_s, "probes_per_step": probes,
                    "hit_rate": observed / max(1, probes), "repeat_stable": stable,
-                   "cpu_gpu_sample_identical": cpu_match},
+                   "fastibs_tsv_equals_device_stats": e2e_match,
+                   "cpu_gpu_identical": (cpu or {}).get("tsv_identical_to_gpu"),
+                   "end_to_end": e2e},
         "roofline": {"bound": "hbm", "achieved": alg_bytes / (k_ms / 1e3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg_bytes / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": (f"scanKernel<{info['bucket_slots']}, grouped>"
-                                if info["minimizer_len"] else
-                                f"scanKernel<{info['bucket_slots']}>"), "kernel_ms": k_ms,
+                     "traffic_source": traffic_source,
+                     "kernel": kernel_name, "kernel_ms": k_ms,
                      "algorithmic_bytes": alg_bytes,
                      "probe_rate_gps": probes / (k_ms / 1e3) / 1e9,
                      "measured_random_read_ceiling_gps": max(gups.values()) if gups else None,
@@ -1603,7 +1873,6 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    table.free()
 
 
 if __name__ == "__main__":

@@ -164,14 +164,16 @@ inline const char *optionsUsage()
            "  --device=N            HIP device ordinal (default 0)\n"
            "  --batch-mb=N          bases per streamed batch, in MiB (default 128)\n"
            "  --streams=N           batches in flight (default 3)\n"
-           "  --load-factor=F       device table load factor (default 0.6)\n"
+           "  --load-factor=F       device table load factor (default 0.2 for minimizer\n"
+           "                        tables, 0.3 if memory is short, 0.6 for whole-k-mer)\n"
            "  --bucket-bytes=64|128 device table bucket width (default 128; a panel\n"
            "                        times both unless this is given)\n"
            "  --hbm-budget-gb=N     cap on device table memory\n"
            "  --minimizer=M[,CAP]   minimizer table (k <= 32, M in 5..min(31, k), groups of\n"
            "                        up to CAP keys, default 16): runs of positions that share\n"
-           "                        a minimizer read their bucket once; slower to build;\n"
-           "                        one sample per run\n"
+           "                        a minimizer read their bucket once; one sample per run.\n"
+           "                        Default for one sample at k = 13..32: M = k - 4;\n"
+           "                        --minimizer=0: whole-k-mer table (panels, intersect)\n"
            "  --db=<folder>         (fastibs) another sample in the same reference pass\n"
            "  --db-list=<file>      (fastibs) sample folders, one per line\n"
            "  --resident=N          (fastibs) at most N sample tables on the device at once\n";
@@ -224,14 +226,18 @@ inline bool parseOptions(int argc, char **argv, int first, Options &o, bool pane
         {
             const size_t comma = val.find(',');
             long long cap = 16;
-            if (!parseInt(val.substr(0, comma), 5, 31, v) ||
-                (comma != std::string::npos && !parseInt(val.substr(comma + 1), 1, 4096, cap)))
+            if (val == "0")
+                v = 0;
+            else if (!parseInt(val.substr(0, comma), 5, 31, v) ||
+                     (comma != std::string::npos &&
+                      !parseInt(val.substr(comma + 1), 1, 4096, cap)))
             {
-                std::cerr << "Error: --minimizer=M[,CAP] needs M in 5..31 and CAP in 1..4096"
+                std::cerr << "Error: --minimizer=M[,CAP] needs M in 5..31 and CAP in 1..4096 "
+                             "(or 0: whole-k-mer table)"
                           << std::endl;
                 return false;
             }
-            g.minimizerLen = (uint32_t)v;
+            g.minimizerLen = (int)v;
             g.minimizerCap = (uint32_t)cap;
         }
         else if (panel && key == "--db" && !val.empty())

@@ -74,7 +74,7 @@ static int run(int argc, char **argv)
         }
     }
     dbDirs.insert(dbDirs.begin(), sourcePath);
-    if (dbDirs.size() > 1 && KmerDatabase::gpuOptions().minimizerLen)
+    if (dbDirs.size() > 1 && KmerDatabase::gpuOptions().minimizerLen > 0)
     {
         cerr << "Error: --minimizer: panel runs (--db, --db-list) need whole-k-mer tables"
              << endl;

@@ -50,7 +50,7 @@ static int run(int argc, char **argv)
     const int small = 1 - big;
     KmerDatabase db(p[big]);
     const auto t0 = chrono::steady_clock::now();
-    db.initGpuIndex();
+    db.initGpuIndex(KmerDatabase::TableUse::Lookup);
     const auto t1 = chrono::steady_clock::now();
     uint64_t count = 0;
     hipibs::check(hipibs_table_probe_count_kmc(db.table(), p[small].c_str(),

@@ -35,13 +35,13 @@ public:
         int device = 0;
         uint64_t batchBases = 128ull << 20;
         int depth = 3;
-        double loadFactor = 0.6;
+        double loadFactor = 0; // 0: chosen with the table kind (0.3 minimizer, 0.6 whole k-mer)
         uint32_t bucketSlots = 16;
         bool bucketAuto = true; // a panel may time 64 B vs 128 B buckets
         uint64_t hbmBudget = 0;
         unsigned loadThreads = 0;
-        uint32_t minimizerLen = 0; // 0: whole-k-mer table
-        uint32_t minimizerCap = 0; // 0: the library's default group cap
+        int minimizerLen = -1;     // -1: chosen by use; 0: whole-k-mer table; 5..31: minimizer
+        uint32_t minimizerCap = 0; // 0: 16 keys per minimizer group
     };
     static GpuOptions &gpuOptions()
     {
@@ -49,6 +49,59 @@ public:
         return o;
     }
 
+    // What a table is probed with: the consecutive positions of a reference
+    // (scan, mapper), or keys in no order (intersect) / many tables per
+    // launch (panel).
+    enum class TableUse
+    {
+        Scan,
+        Lookup,
+    };
+
+    // The table options for a database of n k-mers of length k. Unless
+    // given, scans of k = 13..32 get a minimizer table (m = k - 4, groups of
+    // up to 16 keys, load 0.2: 40 B per key; runs of positions sharing a
+    // minimizer read one bucket), at load 0.3 (26.7 B per key) when that
+    // does not fit 80% of the device's free memory with its build, else a
+    // whole-k-mer table; every other use a whole-k-mer table at load 0.6
+    // (13.3 B per key).
+    static hipibs_table_opts tableOptions(uint32_t k, TableUse use, uint64_t n = 0)
+    {
+        const GpuOptions &g = gpuOptions();
+        hipibs_table_opts o = hipibs::defaultOpts();
+        o.device = g.device;
+        o.bucket_slots = g.bucketSlots;
+        o.hbm_budget = g.hbmBudget;
+        o.load_threads = g.loadThreads;
+        uint32_t m = 0;
+        if (g.minimizerLen > 0)
+            m = (uint32_t)g.minimizerLen;
+        else if (g.minimizerLen < 0 && use == TableUse::Scan && k >= 13 && k <= 32)
+            m = k - 4;
+        double lf = g.loadFactor;
+        if (m && g.minimizerLen < 0 && lf <= 0 && n)
+        {
+            // the build holds every key (8 B) and a group map (<= 16 B per key) beside
+            // the table
+            uint64_t freeB = 0, totalB = 0;
+            if (hipibs_device_mem(g.device, &freeB, &totalB) == 0 && freeB)
+            {
+                auto fits = [&](double load) {
+                    return (double)n * (8.0 * 1.07 / load + 24.0) <= 0.8 * (double)freeB;
+                };
+                if (fits(0.2))
+                    lf = 0.2;
+                else if (fits(0.3))
+                    lf = 0.3;
+                else
+                    m = 0;
+            }
+        }
+        o.minimizer_len = m;
+        o.minimizer_cap = m ? (g.minimizerCap ? g.minimizerCap : 16) : 0;
+        o.load_factor = lf > 0 ? lf : (m ? 0.2 : 0.6);
+        return o;
+    }
     // Reads the header only; the device table is built on first use.
     uint64_t kmerCount() const { return info_.total_kmers; }
 
@@ -82,30 +135,27 @@ public:
     bool gpuActive() const { return table_ != nullptr; }
     const hipibs_table *table() const { return table_.get(); }
 
-    // Builds the device table: the database's raw records are read on host
-    // threads into pinned buffers, decoded into keys and inserted on the
-    // device.
-    void initGpuIndex()
+    // Builds the device table (kind and geometry by tableOptions): the
+    // database's raw records are read on host threads into pinned buffers,
+    // decoded into keys and inserted on the device. A table built for one
+    // use serves the others too (only more slowly).
+    void initGpuIndex(TableUse use = TableUse::Scan)
     {
         if (table_)
             return;
         std::string why;
         if (hipibs::deviceCount(why) < 1)
             throw std::runtime_error("no usable GPU: " + why);
-        const GpuOptions &g = gpuOptions();
-        hipibs_table_opts o = hipibs::defaultOpts();
-        o.device = g.device;
-        o.load_factor = g.loadFactor;
-        o.bucket_slots = g.bucketSlots;
-        o.hbm_budget = g.hbmBudget;
-        o.load_threads = g.loadThreads;
-        o.minimizer_len = g.minimizerLen;
-        o.minimizer_cap = g.minimizerCap;
+        const hipibs_table_opts o = tableOptions(kmerSize, use, info_.total_kmers);
         const auto t0 = std::chrono::steady_clock::now();
         table_ = hipibs::tableFromKmc(sourcePath_, o);
         const auto ti = hipibs::info(table_.get());
         std::cout << "Device table: " << ti.keys << " keys, " << ti.bytes / 1e9 << " GB, "
-                  << ti.slots * 8 << " B buckets, built in "
+                  << ti.slots * 8 << " B buckets, "
+                  << (o.minimizer_len ? "minimizer m = " + std::to_string(o.minimizer_len) +
+                                            ", cap " + std::to_string(o.minimizer_cap)
+                                      : std::string("whole k-mer"))
+                  << ", load " << o.load_factor << ", built in "
                   << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count()
                   << " s" << std::endl;
     }
@@ -160,19 +210,38 @@ public:
     {
         if (dbs.empty())
             return;
-        std::vector<const hipibs_table *> tabs;
         for (KmerDatabase *d : dbs)
-        {
-            d->initGpuIndex();
-            tabs.push_back(d->table());
-        }
+            d->initGpuIndex(dbs.size() > 1 ? TableUse::Lookup : TableUse::Scan);
         std::vector<SeqWindow> one{window};
         hipibs::ScanOptions so;
         so.batchBases = std::max<uint64_t>(1 << 16, window.end - window.start + 128);
         so.depth = 1;
-        const auto res = hipibs::scanWindowsStreamed(tabs, one, so);
+        // one launch per table kind (a table built earlier for a single-sample
+        // scan may be a minimizer table; one launch takes one kind)
+        std::vector<bool> done(dbs.size(), false);
         for (size_t i = 0; i < dbs.size(); ++i)
-            out[i] = res[i][0];
+        {
+            if (done[i])
+                continue;
+            uint32_t mi = 0;
+            hipibs_table_minimizer_len(dbs[i]->table(), &mi);
+            std::vector<size_t> idx;
+            std::vector<const hipibs_table *> tabs;
+            for (size_t j = i; j < dbs.size(); ++j)
+            {
+                uint32_t mj = 0;
+                hipibs_table_minimizer_len(dbs[j]->table(), &mj);
+                if (!done[j] && mj == mi)
+                {
+                    idx.push_back(j);
+                    tabs.push_back(dbs[j]->table());
+                    done[j] = true;
+                }
+            }
+            const auto res = hipibs::scanWindowsStreamed(tabs, one, so);
+            for (size_t t = 0; t < idx.size(); ++t)
+                out[idx[t]] = res[t][0];
+        }
     }
 
     // A mapper chunk's hits into its contig's difference array: +1 at every

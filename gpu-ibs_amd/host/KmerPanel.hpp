@@ -34,8 +34,10 @@ public:
     uint64_t tableBytes(size_t i) const
     {
         const auto &g = KmerDatabase::gpuOptions();
-        const double slots = std::ceil((double)std::max<uint64_t>(1, dbs_[i]->totalKmers()) /
-                                       g.loadFactor);
+        const double lf =
+            KmerDatabase::tableOptions(dbs_[i]->getKmerSize(), KmerDatabase::TableUse::Lookup)
+                .load_factor;
+        const double slots = std::ceil((double)std::max<uint64_t>(1, dbs_[i]->totalKmers()) / lf);
         const uint64_t buckets = std::max<uint64_t>(2, (uint64_t)std::ceil(slots / g.bucketSlots));
         return buckets * g.bucketSlots * 8;
     }
@@ -163,7 +165,7 @@ public:
         for (uint32_t slots : {8u, 16u})
         {
             g.bucketSlots = slots;
-            dbs_[pick]->initGpuIndex();
+            dbs_[pick]->initGpuIndex(KmerDatabase::TableUse::Lookup);
             const hipibs_table *t = dbs_[pick]->table();
             double sec = 0;
             for (int rep = 0; rep < 3; ++rep)
@@ -240,7 +242,7 @@ public:
         auto buildGroup = [this](const std::vector<size_t> &todo) {
             const auto tb = clk::now();
             for (size_t i : todo)
-                dbs_[i]->initGpuIndex();
+                dbs_[i]->initGpuIndex(KmerDatabase::TableUse::Lookup);
             return std::chrono::duration<double>(clk::now() - tb).count();
         };
         std::future<double> nextBuild;

@@ -483,4 +483,75 @@ double ref_batched_scan(void *h, const char *seq, const long long *wStart, const
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// The reference's CPU-BATCHED whole scan of one FASTA against one KMC
+// database, phase by phase as its processReference runs it
+// (src/KmerDatabase.hpp:950-1065 with FASTIBS_CPU_BATCHED): open the
+// KmcRaIndex, read the FASTA (readSequenceFile), cut windows
+// (chunkSequences), per window codec -> probeBatched(K) -> statistics on
+// `threads` threads, write the TSV. phases[0..4] = index load, FASTA read,
+// windows, probe + statistics, write (seconds); returns the number of
+// windows, or < 0 on failure. probes receives the valid positions probed.
+long long ref_batched_whole_scan(const char *dbPrefix, const char *fasta, int W,
+                                 const char *outTsv, int K, int threads, double *phases,
+                                 long long *probes)
+{
+    try
+    {
+        using clk = std::chrono::steady_clock;
+        auto t = clk::now();
+        auto lap = [&t]() {
+            const auto n = clk::now();
+            const double s = std::chrono::duration<double>(n - t).count();
+            t = n;
+            return s;
+        };
+        KmcRaIndex idx(dbPrefix);
+        const uint32_t k = idx.getParams().kmerLength;
+        phases[0] = lap();
+        const bool gz = std::string(fasta).find(".gz") != std::string::npos;
+        ParseResult parsed = readSequenceFile(fasta, gz);
+        phases[1] = lap();
+        std::vector<SeqWindow> windows;
+        chunkSequences(parsed.ids, parsed.sequences, (size_t)W, k, windows);
+        phases[2] = lap();
+        std::vector<Window> rows(windows.size());
+        std::atomic<long long> next{0}, probed{0};
+        auto worker = [&]()
+        {
+            KmerCodec codec(k);
+            std::vector<uint64_t> codes;
+            std::vector<uint8_t> valid, hit;
+            long long mine = 0;
+            for (;;)
+            {
+                const long long i = next.fetch_add(1);
+                if (i >= (long long)windows.size())
+                    break;
+                codec.encodePositions(windows[i].seq, codes, valid);
+                hit.assign(codes.size(), 0);
+                fastibs::probeBatched(idx, codes.data(), valid.data(), codes.size(), hit.data(),
+                                      (unsigned)K);
+                rows[i] = statsOf(windows[i], k, valid, hit);
+                mine += rows[i].totalKmers;
+            }
+            probed += mine;
+        };
+        std::vector<std::thread> pool;
+        for (int i = 0; i < std::max(1, threads); ++i)
+            pool.emplace_back(worker);
+        for (auto &th : pool)
+            th.join();
+        phases[3] = lap();
+        writeTsv(outTsv, rows);
+        phases[4] = lap();
+        *probes = probed.load();
+        return (long long)rows.size();
+    }
+    catch (const std::exception &e)
+    {
+        fprintf(stderr, "ref_batched_whole_scan: %s\n", e.what());
+        return -2;
+    }
+}
+
 } // extern "C"

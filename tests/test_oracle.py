@@ -343,3 +343,34 @@ def test_wide_kmc_writer_round_trip(tmp_path, k):
     hit = np.zeros(sub.shape[0], dtype=np.uint8)
     assert ref.ref_kmc_is_kmer_strings(p.encode(), text, sub.shape[0],
                                        hit.ctypes.data) == sub.shape[0]
+
+
+def test_reference_batched_whole_scan_writes_the_golden_tsv(tmp_path, small_corpus):
+    """The CPU baseline's whole-scan driver (the reference's KmcRaIndex,
+    readSequenceFile, chunkSequences, probeBatched and statistics, timed
+    phase by phase) writes the golden TSV of each small sample."""
+    import ctypes as C
+    if not os.path.exists(REF_PATH):
+        pytest.skip("oracle/_ref/libref.so not built (no reference sources)")
+    ref_lib = C.CDLL(REF_PATH)
+    f = ref_lib.ref_batched_whole_scan
+    f.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_void_p,
+                  C.POINTER(C.c_longlong)]
+    f.restype = C.c_longlong
+    ref, samples = small_corpus
+    fa = tmp_path / "ref.fasta"
+    fa.write_bytes(corpus.fasta([(b"chr1", corpus.to_text(ref))]))
+    for i, (keys, counts) in enumerate(samples):
+        p = str(tmp_path / f"s{i}")
+        hipibs.kmc_write(p, keys, 25, counts=counts, p=9, sig_len=9, counter_size=1,
+                         version=0x200, min_count=2, max_count=255)
+        out = tmp_path / f"s{i}.tsv"
+        phases = np.zeros(5)
+        probes = C.c_longlong(0)
+        n = f(p.encode(), str(fa).encode(), 50000, str(out).encode(), 16, 3,
+              phases.ctypes.data, C.byref(probes))
+        assert n == 21 and (phases >= 0).all()
+        with open(os.path.join(ROOT, "tests", "golden", f"small_s{i}_50000.tsv"), "rb") as g:
+            want = g.read()
+        assert out.read_bytes() == want
+        assert probes.value == sum(int(l.split(b"\t")[3]) for l in want.splitlines()[1:])
