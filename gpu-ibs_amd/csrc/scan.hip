@@ -382,31 +382,90 @@ __global__ __launch_bounds__(256) void hitmaskKernel(const TableView T, const ui
     }
 }
 
-// coverage[i] = popcount of hit bits p in [i-k+1, i] (clamped to the
-// positions), as a byte (k <= 64).
-__global__ void coverageKernel(const uint32_t *__restrict__ hitWords, uint64_t nPos,
-                               uint64_t nBases, uint32_t k, uint8_t *__restrict__ cov)
+// Bit j of the low 8 bits of b as byte j (0 or 1) of a 64-bit word.
+__device__ inline uint64_t bitsToBytes(uint32_t b)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t x = ((uint64_t)(b & 0xFFu) * 0x0101010101010101ull) & 0x8040201008040201ull;
+    const uint64_t y = ((x & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | x;
+    return (y >> 7) & 0x0101010101010101ull;
+}
+
+// Running sums of the bytes of v (byte j = sum of bytes 0..j; sums < 256).
+__device__ inline uint64_t bytePrefix(uint64_t v)
+{
+    v += v << 8;
+    v += v << 16;
+    return v + (v << 32);
+}
+
+// coverage[i] = popcount of hit bits p in [i-k+1, i] (clamped to the
+// positions), as a byte (k <= 64). A wave takes 1024 consecutive bases,
+// 16 per lane (bases i0..i0+15 of lane l, i0 = wave base + 16 l): lane l
+// loads hit word (wave base)/32 - 2 + l, and each lane takes the four
+// words of positions i0-64 .. i0+63 from its neighbours (ds_bpermute).
+// The lane's first base counts its window; the other 15 come in bytes at
+// once -- coverage[i0+t] = coverage[i0] + (hits entering at i0+1..i0+t) -
+// (hits leaving at i0+1-k..i0+t-k), both running sums of bits spread to
+// bytes -- and the 16 bytes are one 16 B store (consecutive lanes,
+// consecutive stores).
+constexpr uint32_t kCovPerWave = 1024;
+__global__ __launch_bounds__(256) void coverageKernel(const uint32_t *__restrict__ hitWords,
+                                                      uint64_t nPos, uint64_t nBases, uint32_t k,
+                                                      uint8_t *__restrict__ cov)
+{
+    const int lane = laneId();
+    const uint64_t nWaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     const uint64_t nWords = (nPos + 31) >> 5;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nBases; i += stride)
+    const uint64_t nChunks = (nBases + kCovPerWave - 1) / kCovPerWave;
+    const bool vec = (reinterpret_cast<uintptr_t>(cov) & 15) == 0;
+    for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < nChunks;
+         c += nWaves)
     {
-        uint32_t c = 0;
-        if (nPos)
+        const uint64_t W0 = c * kCovPerWave;
+        // word (W0/32) - 2 + lane (positions < 0 or >= nPos read as 0)
+        const int64_t w = (int64_t)(W0 >> 5) - 2 + lane;
+        uint32_t v = 0;
+        if (w >= 0 && (uint64_t)w < nWords)
         {
-            const uint64_t lo = i + 1 >= k ? i + 1 - k : 0;
-            const uint64_t hi = std::min<uint64_t>(i, nPos - 1);
-            // the <= 64 bits lo..hi, one 32-bit word at a time
-            for (uint64_t q = lo; q <= hi && q >> 5 < nWords;)
-            {
-                const uint32_t off = (uint32_t)(q & 31);
-                const uint64_t take = std::min<uint64_t>(32 - off, hi - q + 1);
-                const uint32_t bits = hitWords[q >> 5] >> off;
-                c += (uint32_t)__popc(take >= 32 ? bits : bits & ((1u << take) - 1u));
-                q += take;
-            }
+            v = hitWords[w];
+            const int64_t last = (int64_t)nPos - 32 * w; // positions of word w below nPos
+            if (last < 32)
+                v &= (1u << last) - 1u;
         }
-        cov[i] = (uint8_t)c;
+        const uint64_t i0 = W0 + 16 * (uint64_t)lane;
+        // this lane's words (W0/32) + lane/2 - 2 .. +1 are lanes lane/2 .. lane/2+3
+        uint64_t wv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            wv[q] = shuffle32(v, (lane >> 1) + q);
+        // L: positions i0-64 .. i0-1, R: positions i0 .. i0+63 (bit 0 = the first)
+        uint64_t L, R;
+        if (lane & 1)
+        {
+            L = (wv[0] >> 16) | (wv[1] << 16) | (wv[2] << 48);
+            R = (wv[2] >> 16) | (wv[3] << 16);
+        }
+        else
+        {
+            L = wv[0] | (wv[1] << 32);
+            R = wv[2] | (wv[3] << 32);
+        }
+        const uint64_t c0 = (R & 1ull) + (k > 1 ? (uint64_t)__popcll(L >> (65 - k)) : 0ull);
+        const uint32_t in = (uint32_t)R & 0xFFFEu;
+        const uint32_t out = (uint32_t)(k == 64 ? L : (L >> (64 - k)) | (R << k)) & 0xFFFEu;
+        const uint64_t in0 = bytePrefix(bitsToBytes(in)), out0 = bytePrefix(bitsToBytes(out));
+        const uint64_t in1 = bytePrefix(bitsToBytes(in >> 8)) + (in0 >> 56) * 0x0101010101010101ull;
+        const uint64_t out1 =
+            bytePrefix(bitsToBytes(out >> 8)) + (out0 >> 56) * 0x0101010101010101ull;
+        const uint64_t base = c0 * 0x0101010101010101ull;
+        // (base + entered) >= left in every byte: no borrow crosses a byte
+        const uint64_t lo = base + in0 - out0, hi = base + in1 - out1;
+        if (vec && i0 + 16 <= nBases)
+            *reinterpret_cast<uint4 *>(cov + i0) =
+                make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+        else
+            for (uint64_t j = 0; j < 16 && i0 + j < nBases; ++j)
+                cov[i0 + j] = (uint8_t)((j < 8 ? lo >> (8 * j) : hi >> (8 * (j - 8))) & 0xFF);
     }
 }
 
@@ -1048,7 +1107,8 @@ extern "C" int hipibs_coverage_device(const uint32_t *d_hit_words, uint64_t n_po
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     int dev = 0;
     (void)hipGetDevice(&dev);
-    const uint64_t grid = std::min<uint64_t>((n_bases + 255) / 256, (uint64_t)deviceCUs(dev) * 16);
+    const uint64_t waves = (n_bases + kCovPerWave - 1) / kCovPerWave;
+    const uint64_t grid = std::min<uint64_t>((waves + 3) / 4, (uint64_t)deviceCUs(dev) * 16);
     hipLaunchKernelGGL(coverageKernel, dim3((uint32_t)grid), dim3(256), 0, s, d_hit_words, n_pos,
                        n_bases, k, d_cov);
     HIPIBS_CHECK(hipGetLastError());

@@ -222,6 +222,37 @@ def test_coverage_matches_golden(small_corpus, expected):
     t.free()
 
 
+def test_coverage_kernel_against_brute_force():
+    """hipibs_coverage_device on random hit masks: every k in 1..64 class,
+    lengths around the 16-base chunks and 32-bit words, contigs shorter
+    than k, an output pointer off 16 B alignment, bits past n_pos ignored."""
+    rng = np.random.default_rng(77)
+    cases = [(k, n) for k in (1, 2, 15, 25, 31, 32, 33, 47, 63, 64)
+             for n in (0, 1, 15, 16, 17, 31, 33, 63, 64, 65, 100, 1000, 4099)]
+    for k, n in cases:
+        npos = n - k + 1 if n >= k else 0
+        bits = (rng.random(max(npos, 1)) < 0.6)[:npos]
+        words = np.zeros((npos + 31) // 32 + 2, dtype=np.uint32)
+        for p in np.flatnonzero(bits):
+            words[p // 32] |= np.uint32(1) << np.uint32(p % 32)
+        if npos % 32:
+            words[npos // 32] |= ~((np.uint32(1) << np.uint32(npos % 32)) - np.uint32(1))
+        want = np.zeros(n, dtype=np.int64)
+        for p in np.flatnonzero(bits):
+            want[p:p + k] += 1
+        d_hit = D.from_array(words)
+        for off in (0, 1):
+            d_cov = D.zeros(n + 32)
+            ptr = d_cov.ptr + off
+            hipibs._check(hipibs._lib.hipibs_coverage_device(d_hit.ptr, npos, n, k, ptr, None),
+                          "coverage")
+            hipibs.synchronize()
+            got = d_cov.to_array(np.uint8, n + 32)[off:off + n].astype(np.int64)
+            assert np.array_equal(got, want), (k, n, off)
+            d_cov.free()
+        d_hit.free()
+
+
 @pytest.mark.parametrize("n", [1, 2, 15, 16, 17, 4095, 4096, 4097, 12288, 100003])
 def test_coverage_text_matches_the_serialisation(n):
     """Device text of a coverage line == the values joined by ',' (1-3
