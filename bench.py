@@ -46,14 +46,14 @@ def parse():
     ap.add_argument("--bucket-slots", type=int, default=0, help="0 = library default")
     ap.add_argument("--minimizer-len", type=int, default=-1,
                     help="0 = whole-k-mer hash buckets, 5..min(31, k) = minimizer buckets; "
-                         "default: 21 for the scan workload, 0 otherwise")
+                         "default: k - 4 for the scan and large workloads, 0 otherwise")
     ap.add_argument("--minimizer-cap", type=int, default=-1,
                     help="minimizer tables: keys per minimizer group before it moves to the "
                          "whole-k-mer part (0 = library default); default: 16 for the scan "
-                         "workload")
+                         "and large workloads")
     ap.add_argument("--load-factor", type=float, default=0.0,
-                    help="keys per slot; default: 0.2 for the scan workload's minimizer "
-                         "table, 0.6 otherwise")
+                    help="keys per slot; default: 0.2 for the minimizer tables of the scan "
+                         "and large workloads, 0.6 otherwise")
     ap.add_argument("--segs", type=int, default=0, help="segments per window (0 = auto)")
     ap.add_argument("--cpu-mbp", type=float, default=1536.0,
                     help="reference slice (Mbp) of the CPU baseline sample; 0 = skip")
@@ -104,13 +104,13 @@ def parse():
                          "runs in at least this many memory-budgeted groups")
     ap.add_argument("--workdir", default=os.environ.get("TMPDIR", "/tmp"))
     args = ap.parse_args()
-    # The scan workload's table: minimizer buckets (m = 21, groups of up to 16
-    # keys) at load 0.2, so that each run of reference positions sharing a
-    # minimizer reads one 128 B line; the other workloads keep whole-k-mer
-    # tables at load 0.6.
-    minimizer = args.workload == "scan" and args.minimizer_len != 0
+    # The single-sample scan workloads build what fastibs builds by default:
+    # minimizer buckets (m = k - 4, groups of up to 16 keys) at load 0.2, so
+    # that each run of reference positions sharing a minimizer reads one
+    # 128 B line; the other workloads keep whole-k-mer tables at load 0.6.
+    minimizer = args.workload in ("scan", "large") and args.minimizer_len != 0
     if args.minimizer_len < 0:
-        args.minimizer_len = 21 if minimizer else 0
+        args.minimizer_len = ((LARGE_K if args.workload == "large" else K) - 4) if minimizer else 0
     if args.minimizer_cap < 0:
         args.minimizer_cap = 16 if minimizer else 0
     if args.load_factor <= 0:
@@ -212,17 +212,28 @@ def random_canonical(torch, m, k, g):
     return torch.minimum(x, rc)
 
 
-def make_sample_keys(torch, hap_codes, target, k, seed):
+def make_sample_keys(torch, hap_codes, target, k, seed, n_parts=4):
     """~target distinct keys: 80% drawn from the distinct k-mers of the
-    haplotype, the rest random canonical codes (sequencing-error k-mers)."""
+    haplotype, the rest random canonical codes (sequencing-error k-mers).
+    Made distinct class by class (a hash of the code splits them into
+    n_parts disjoint classes), so no single sort spans ~2e9 codes."""
     g = torch.Generator(device="cuda")
     g.manual_seed(seed + 7)
-    hk = torch.unique(hap_codes[hap_codes >= 0])
+    v = hap_codes[hap_codes >= 0]
+    cls = code_class(torch, v, n_parts).to(torch.uint8)
+    hk = []
+    for p in range(n_parts):
+        hk.append(torch.unique(v[cls == p]))
+    del v, cls
+    n_all = sum(x.numel() for x in hk)
     n_hap = int(0.8 * target)
-    if hk.numel() > n_hap:
-        hk = hk[torch.rand(hk.numel(), device="cuda", generator=g) < n_hap / hk.numel()]
-    rnd = random_canonical(torch, max(0, int(target) - hk.numel()), k, g)
-    return torch.unique(torch.cat([hk, rnd]))
+    if n_all > n_hap:
+        hk = [x[torch.rand(x.numel(), device="cuda", generator=g) < n_hap / n_all] for x in hk]
+    rnd = random_canonical(torch, max(0, int(target) - sum(x.numel() for x in hk)), k, g)
+    rc = code_class(torch, rnd, n_parts)
+    out = [torch.unique(torch.cat([hk[p], rnd[rc == p]])) for p in range(n_parts)]
+    del hk, rnd, rc
+    return torch.cat(out)
 
 
 # ---- CPU baseline (the reference's CPU-BATCHED scan, compiled from its sources) ----
@@ -1108,8 +1119,9 @@ def run_large(args):
             db = os.path.join(root, "sample")
             refs = os.path.join(root, "refs")
             # the tools build the same kind of table as the in-process scan
-            topts = ([f"--minimizer={args.minimizer_len},{args.minimizer_cap or 16}",
-                      f"--load-factor={args.load_factor}"] if args.minimizer_len else [])
+            topts = ([f"--minimizer={args.minimizer_len},{args.minimizer_cap or 16}"]
+                     if args.minimizer_len else ["--minimizer=0"])
+            topts.append(f"--load-factor={args.load_factor}")
             scan_s, out = run([os.path.join(bindir, "fastibs"), db, refs,
                                os.path.join(root, "out"), str(W)] + topts)
             tsv = os.path.join(root, "out", f"sample_v_large_{W}.tsv")
@@ -1163,7 +1175,8 @@ def run_large(args):
                    "windows": len(wins), "k": k, "window": W,
                    "sample_keys": int(info["n_keys"]), "table_bytes": int(info["bytes"]),
                    "bucket_slots": info["bucket_slots"], "load_factor": args.load_factor,
-                   "segs_per_window": segs.value, "scan_wall_s": ms / 1e3,
+                   "minimizer_len": info["minimizer_len"], "minimizer_cap": args.minimizer_cap,
+                   "segs_per_window": segs.value, "device_step_ms": ms,
                    "table_build_s": build_s, "probes_per_step": probes,
                    "hit_rate": observed / max(1, probes), "repeat_stable": stable,
                    "introgressed_blocks": blocks,
@@ -1178,7 +1191,9 @@ def run_large(args):
         "roofline": {"bound": "hbm", "achieved": alg_bytes / (k_ms / 1e3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg_bytes / (k_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"scanKernel<{info['bucket_slots']}>", "kernel_ms": k_ms,
+                     "kernel": (f"scanKernel<{info['bucket_slots']}, grouped>"
+                                if info["minimizer_len"] else f"scanKernel<{info['bucket_slots']}>"),
+                     "kernel_ms": k_ms,
                      "algorithmic_bytes": alg_bytes,
                      "probe_rate_gps": probes / (k_ms / 1e3) / 1e9},
         "cpu_baseline": cpu,
