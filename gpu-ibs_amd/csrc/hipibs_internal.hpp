@@ -295,6 +295,10 @@ struct hipibs_table
     uint32_t mcap = 0;
     uint64_t nKeys2 = 0, nMarkers = 0;
     uint32_t wideTop = 0; // k = 64: T^32 A^32 is a key (held outside the buckets)
+    // slots and counters from hipMallocAsync (the device's default memory
+    // pool): freed with hipFreeAsync, so neither allocation nor free waits
+    // for the device
+    bool streamOrdered = false;
     hipibs::TableView view() const
     {
         return hipibs::TableView{dSlots,   nBuckets, maxDisp,   mlen,    k,
@@ -327,6 +331,49 @@ uint32_t autoSegsPerWindow(uint32_t nWindows, int nTables, uint32_t maxPositions
 uint64_t scanWorkBytes(uint32_t nWindows, int nTables, uint32_t segsPerWindow);
 int kmcReadKeys(const std::string &prefix, uint64_t *keys, uint64_t cap, uint64_t *nOut,
                 unsigned threads, hipibs_kmc_info *infoOut);
+
+// Table construction steps shared by the build paths: slots sized for
+// nKeys and cleared on s; after the inserts, the build counters read back
+// (synchronises s) and checked.
+int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts, hipibs_table **out,
+               hipStream_t s, bool streamOrdered = false);
+int finishTable(hipibs_table *t, hipStream_t s);
+// Frees t; a stream-ordered table's memory goes back to the pool after the
+// work already on s.
+void freeTableOn(hipibs_table *t, hipStream_t s);
+
+// A sample's keys kept compactly in host memory (the serving engine's copy
+// for re-admissions): each 2k-bit code is split into a prefix of
+// prefixBits (<= 22) bits and a suffix of sufBytes bytes; keys are stored
+// grouped by prefix (any order inside a group), offs[p] = index of the
+// first key of prefix p, offs[2^prefixBits] = n, suffixes little-endian,
+// sufBytes each. Pinned host memory where granted.
+struct KeyStore
+{
+    uint32_t k = 0, prefixBits = 0, sufBytes = 0;
+    uint64_t n = 0;
+    uint32_t *offs = nullptr;
+    uint8_t *suf = nullptr;
+    bool pinned = false;
+    uint64_t hostBytes() const { return n * sufBytes + ((1ull << prefixBits) + 1) * 4; }
+};
+void keyStoreGeometry(uint32_t k, uint32_t &prefixBits, uint32_t &sufBytes);
+// From n device keys (one word each, two for k > 32): a counting sort by
+// prefix on s (device memory for n * sufBytes bytes and the offsets while
+// it runs). Rejects codes wider than 2k bits and n >= 2^32.
+int keyStoreFromDevice(const uint64_t *dKeys, uint64_t n, uint32_t k, int device, hipStream_t s,
+                       KeyStore &out);
+void keyStoreFree(KeyStore &ks);
+// A builder's reusable device staging (two streams, double buffers of
+// 16 Mi keys' suffixes and decoded keys, the offsets).
+struct StoreStaging;
+int storeStagingCreate(int device, StoreStaging **out);
+void storeStagingFree(StoreStaging *st);
+// A table from a key copy: suffix chunks uploaded on one stream while the
+// other decodes and inserts the previous one (a minimizer table decodes
+// every key first, into device memory held for the build).
+int buildTableFromStore(const KeyStore &ks, const hipibs_table_opts *opts, StoreStaging *st,
+                        hipibs_table **out, bool streamOrdered = false);
 
 // Reusable staging for builds from host keys (pinned + device buffers of
 // chunkKeys keys each, and a stream).

@@ -1,7 +1,8 @@
 // Panel scan: one persistent launch scans a group of sample tables over a
 // shared packed reference. Every 64-position word is encoded once (codec,
-// validity, the table-independent part of the bucket hash) and probed
-// against each table of the group in turn; each table's run-length state
+// validity, the table-independent part of the bucket hash -- or, for
+// minimizer tables, the positions' minimizers) and probed against each
+// table of the group in turn; each table's run-length state
 // lives in LDS (wave-uniform part + per-lane accumulators), so the result
 // per (table, window) is the same as a single-table scan. Blocks pull
 // (window, segment) items from a counter until the work runs out, so the
@@ -35,7 +36,11 @@ __device__ inline uint64_t keyMixHi(uint64_t key)
     return h >> 32;
 }
 
-template <int S, int MT>
+// GROUPED: every table is a minimizer table of the same minimizer length;
+// each word's minimizers are computed once and every table is probed with
+// waveProbeGrouped (a run of positions sharing a minimizer loads its home
+// line once).
+template <int S, int MT, bool GROUPED = false>
 __global__ __launch_bounds__(PANEL_BLOCK) void panelKernel(
     const TableView *__restrict__ tables, int nT, const uint64_t *__restrict__ seq,
     const uint64_t *__restrict__ exc, uint64_t nBases, const hipibs_window *__restrict__ windows,
@@ -47,6 +52,8 @@ __global__ __launch_bounds__(PANEL_BLOCK) void panelKernel(
     __shared__ WaveState st[PANEL_WAVES][MT];
     __shared__ uint32_t lacc[PANEL_WAVES][MT][64][2];
     __shared__ SegSummary sum[PANEL_WAVES][MT];
+    __shared__ __align__(16) uint64_t groupStage[GROUPED ? PANEL_WAVES : 1]
+                                                [GROUPED ? GroupStage<S, 1>::kWords : 2];
     __shared__ uint32_t item;
     const int lane = laneId();
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -85,8 +92,13 @@ __global__ __launch_bounds__(PANEL_BLOCK) void panelKernel(
             const uint64_t per = (nWords + PANEL_WAVES - 1) / PANEL_WAVES;
             const uint64_t mBeg = mFirst + std::min<uint64_t>(nWords, (uint64_t)wv * per);
             const uint64_t mEnd = mFirst + std::min<uint64_t>(nWords, (uint64_t)(wv + 1) * per);
-            for (uint64_t m = mBeg; m < mEnd; ++m)
-            {
+            const uint32_t mlen = GROUPED ? tv[0].mlen : 0u;
+            const uint64_t maskm2 = GROUPED ? (1ull << (2 * mlen)) - 1ull : 0ull;
+            const int win = (int)(k - mlen + 1); // m-mers per k-mer
+            // canonical code and validity of position 64m + lane; GROUPED:
+            // the hash of its first m-mer (the top of the forward code, the
+            // bottom of the reverse complement)
+            auto encodeWord = [&](uint64_t m, uint64_t &canon, bool &valid, uint32_t &L) {
                 const uint64_t s0 = seq[2 * m], s1 = seq[2 * m + 1], s2 = seq[2 * m + 2];
                 const uint64_t e0 = exc[m], e1 = exc[m + 1];
                 const uint64_t a = lane < 32 ? s0 : s1;
@@ -95,18 +107,61 @@ __global__ __launch_bounds__(PANEL_BLOCK) void panelKernel(
                 x &= kmask2;
                 const uint64_t ev = lane ? ((e0 >> lane) | (e1 << (64 - lane))) : e0;
                 const uint64_t p = (m << 6) + (uint64_t)lane;
-                const bool valid = ((ev & kmask1) == 0) && p >= g0 && p < g1;
+                valid = ((ev & kmask1) == 0) && p >= g0 && p < g1;
                 const uint64_t rc = ~x & kmask2;
                 const uint64_t fwd = reverse2(x) >> fwdShift;
-                const uint64_t canon = fwd < rc ? fwd : rc;
-                const uint64_t hi = keyMixHi(canon);
+                canon = fwd < rc ? fwd : rc;
+                L = 0;
+                if constexpr (GROUPED)
+                {
+                    const uint64_t flo = fwd >> (2 * (k - mlen)), rlo = rc & maskm2;
+                    L = mmerHash(flo < rlo ? flo : rlo);
+                }
+            };
+            uint64_t canon = 0;
+            bool valid = false;
+            uint32_t L = 0;
+            if (GROUPED && mBeg < mEnd)
+                encodeWord(mBeg, canon, valid, L);
+            for (uint64_t m = mBeg; m < mEnd; ++m)
+            {
+                uint32_t minh = 0;
+                uint64_t hi = 0, canonN = 0;
+                bool validN = false;
+                uint32_t LN = 0;
+                if constexpr (GROUPED)
+                {
+                    // one word behind: the next word's first m-mers end this
+                    // word's minimizer windows (word mEnd is read within the
+                    // packed arrays' pad words)
+                    encodeWord(m + 1, canonN, validN, LN);
+                    minh = waveWindowMin(L, LN, win);
+                }
+                else
+                {
+                    encodeWord(m, canon, valid, L);
+                    hi = keyMixHi(canon);
+                }
                 const uint64_t V = ballot(valid);
                 nV += (uint32_t)__popcll(V);
                 for (int t = 0; t < nT; ++t)
                 {
                     const TableView T = tv[t];
-                    const uint32_t home = (uint32_t)((hi * T.nBuckets) >> 32);
-                    const uint64_t H = waveProbe<S>(T, canon, home, valid);
+                    uint64_t H;
+                    if constexpr (GROUPED)
+                    {
+                        const uint64_t code[1] = {canon};
+                        const uint32_t mh[1] = {minh};
+                        const bool v[1] = {valid};
+                        uint64_t Hw[1];
+                        waveProbeGrouped<S, 1>(T, code, mh, v, groupStage[wv], Hw);
+                        H = Hw[0];
+                    }
+                    else
+                    {
+                        const uint32_t home = (uint32_t)((hi * T.nBuckets) >> 32);
+                        H = waveProbe<S>(T, canon, home, valid);
+                    }
                     const uint64_t Miss = V & ~H;
                     WaveState ws = st[wv][t];
                     if (H)
@@ -151,6 +206,12 @@ __global__ __launch_bounds__(PANEL_BLOCK) void panelKernel(
                     ws.nH += (uint32_t)__popcll(H);
                     if (lane == 0)
                         st[wv][t] = ws;
+                }
+                if constexpr (GROUPED)
+                {
+                    canon = canonN;
+                    valid = validN;
+                    L = LN;
                 }
             }
         }
@@ -256,28 +317,36 @@ static int enqueuePanel(const PanelLaunch &L, hipStream_t s)
     {
         const int base = g * PANEL_MAX_TABLES;
         const int n = std::min(PANEL_MAX_TABLES, L.nTables - base);
-#define PANEL_LAUNCH(SS, MT)                                                                     \
-    hipLaunchKernelGGL((panelKernel<SS, MT>), dim3(blocks), dim3(PANEL_BLOCK), 0, s, views + base, \
-                       n, L.seq, L.exc, L.nBases, L.windows, L.nWindows, L.segs, k, L.stats, sums, \
-                       counters + g, (uint32_t)base, L.statsStride)
+#define PANEL_LAUNCH(SS, MT, GR)                                                                 \
+    hipLaunchKernelGGL((panelKernel<SS, MT, GR>), dim3(blocks), dim3(PANEL_BLOCK), 0, s,          \
+                       views + base, n, L.seq, L.exc, L.nBases, L.windows, L.nWindows, L.segs, k, \
+                       L.stats, sums, counters + g, (uint32_t)base, L.statsStride)
+#define PANEL_LAUNCH_MT(SS, GR)                                                                  \
+    do                                                                                           \
+    {                                                                                            \
+        if (n <= 4)                                                                              \
+            PANEL_LAUNCH(SS, 4, GR);                                                             \
+        else if (n <= 8)                                                                         \
+            PANEL_LAUNCH(SS, 8, GR);                                                             \
+        else                                                                                     \
+            PANEL_LAUNCH(SS, 16, GR);                                                            \
+    } while (0)
+        const bool grouped = t0->mlen != 0; // checkTables: every table alike
         if (t0->bucketSlots == 8)
         {
-            if (n <= 4)
-                PANEL_LAUNCH(8, 4);
-            else if (n <= 8)
-                PANEL_LAUNCH(8, 8);
+            if (grouped)
+                PANEL_LAUNCH_MT(8, true);
             else
-                PANEL_LAUNCH(8, 16);
+                PANEL_LAUNCH_MT(8, false);
         }
         else
         {
-            if (n <= 4)
-                PANEL_LAUNCH(16, 4);
-            else if (n <= 8)
-                PANEL_LAUNCH(16, 8);
+            if (grouped)
+                PANEL_LAUNCH_MT(16, true);
             else
-                PANEL_LAUNCH(16, 16);
+                PANEL_LAUNCH_MT(16, false);
         }
+#undef PANEL_LAUNCH_MT
 #undef PANEL_LAUNCH
         HIPIBS_CHECK(hipGetLastError());
     }
@@ -300,8 +369,6 @@ static int preparePanel(const hipibs_table *const *tables, int n_tables, const u
     int rc = checkTables(tables, n_tables);
     if (rc)
         return rc;
-    if (tables[0]->mlen)
-        return setError(HIPIBS_E_ARG, "panel scans need whole-k-mer bucket tables");
     if (tables[0]->k > 32)
         return setError(HIPIBS_E_ARG, "panel scans need k <= 32 (scan tables of k > 32 one "
                                       "launch per table with hipibs_scan_windows_device)");

@@ -5,8 +5,9 @@
 // finished launches, handing their statistics to requests and unpinning
 // their tables; (2) if a launch slot is free, fills it from the pending
 // requests in arrival order -- a request whose sample is not resident is
-// admitted by building its table from the registered host keys on a
-// builder thread (at most one build per builder in flight), after evicting
+// admitted by building its table from the engine's compact host copy of
+// the sample's keys on a builder thread (at most one build per builder in
+// flight), after evicting
 // tables that no launch holds, the least requested first, as far as the
 // HBM budget requires; a table is only evicted for a sample requested at
 // least as often, unless the waiting request is older than kForceAdmitMs
@@ -54,7 +55,7 @@ struct Request
 
 struct Sample
 {
-    const uint64_t *keys = nullptr;
+    KeyStore store;        // the keys (samples registered by key arrays)
     std::string kmcPrefix; // set: the table is built from this KMC database
     uint64_t n = 0;
     uint64_t kmcStage = 0; // KMC samples: device bytes of the record decode's staging
@@ -202,8 +203,9 @@ uint64_t tableBytesFor(const hipibs_serve *s, uint64_t n)
 // Device bytes a build holds beside its table while it runs: a KMC
 // sample's record staging and decoded keys, and for a minimizer table every
 // key on the device plus the group map (a power of two >= n entries of
-// 8 B). (Each builder thread's host-key staging, two chunks of 32 Mi keys,
-// is allocated once and held outside the budget.)
+// 8 B). (Each builder thread's staging for builds from the key copies --
+// two chunks of 16 Mi keys' suffixes and decoded keys, and the prefix
+// offsets -- is allocated once and held outside the budget.)
 uint64_t buildTransientFor(const hipibs_serve *s, const Sample &sm)
 {
     const uint64_t keyBytes = s->k > 32 ? 16 : 8;
@@ -230,11 +232,13 @@ void settleBytes(hipibs_serve *s, Sample &sm)
     sm.bytes = bytes;
 }
 
-// Builds sample id's table now (preload) -- or, async, queues it on a
-// builder thread -- after making room. 0 = resident, 1 = not (yet), <0 =
-// error. The bytes of a queued build are reserved against the budget.
-// `force`: the waiting request is old enough to evict any idle table.
-int ensureResident(hipibs_serve *s, uint32_t id, bool async, bool force)
+// Builds sample id's table now (preload: from its keys on the device,
+// dKeys, or its KMC database) -- or, async, queues it on a builder thread
+// -- after making room. 0 = resident, 1 = not (yet), <0 = error. The bytes
+// of a queued build are reserved against the budget. `force`: the waiting
+// request is old enough to evict any idle table.
+int ensureResident(hipibs_serve *s, uint32_t id, bool async, bool force,
+                   const uint64_t *dKeys = nullptr)
 {
     Sample &sm = s->samples.at(id);
     sm.lastUse = ++s->useClock;
@@ -266,8 +270,10 @@ int ensureResident(hipibs_serve *s, uint32_t id, bool async, bool force)
     }
     hipibs_table_opts o = s->topts;
     o.device = s->device;
-    int rc = sm.kmcPrefix.empty() ? hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &sm.table)
-                                  : hipibs_table_build_kmc(sm.kmcPrefix.c_str(), &o, &sm.table);
+    int rc = !sm.kmcPrefix.empty()
+                 ? hipibs_table_build_kmc(sm.kmcPrefix.c_str(), &o, &sm.table)
+                 : dKeys ? hipibs_table_build_keys_device(dKeys, sm.n, s->k, &o, &sm.table, nullptr)
+                         : setError(HIPIBS_E_ARG, "no device keys for a preload");
     if (rc)
     {
         s->resident -= need;
@@ -286,7 +292,7 @@ int ensureResident(hipibs_serve *s, uint32_t id, bool async, bool force)
 void builderLoop(hipibs_serve *s)
 {
     (void)hipSetDevice(s->device);
-    HostStaging *staging = nullptr; // kept across builds
+    StoreStaging *staging = nullptr; // kept across builds
     for (;;)
     {
         uint32_t id = 0;
@@ -310,7 +316,7 @@ void builderLoop(hipibs_serve *s)
         {
             const auto t0 = Clock::now();
             for (hipibs_table *t : toFree)
-                hipibs_table_free(t);
+                freeTableOn(t, nullptr); // stream-ordered tables: no wait for the device
             const uint64_t us = (uint64_t)(ms(t0, Clock::now()) * 1000.0);
             std::lock_guard<std::mutex> lk(s->mu);
             s->counters.evict_us += us;
@@ -327,16 +333,14 @@ void builderLoop(hipibs_serve *s)
         hipibs_table *t = nullptr;
         const auto t0 = Clock::now();
         int rc = 0;
-        if (!staging)
-            rc = hostStagingCreate(s->device, 32ull << 20, &staging);
-        if (!rc)
+        if (!sm.kmcPrefix.empty())
+            rc = hipibs_table_build_kmc(sm.kmcPrefix.c_str(), &o, &t);
+        else
         {
-            if (!sm.kmcPrefix.empty())
-                rc = hipibs_table_build_kmc(sm.kmcPrefix.c_str(), &o, &t);
-            else if (o.minimizer_len || s->k > 32)
-                rc = hipibs_table_build_keys(sm.keys, sm.n, s->k, &o, &t);
-            else
-                rc = buildTableFromHost(sm.keys, sm.n, s->k, &o, staging, &t);
+            if (!staging)
+                rc = storeStagingCreate(s->device, &staging);
+            if (!rc)
+                rc = buildTableFromStore(sm.store, &o, staging, &t, true);
         }
         const uint64_t us = (uint64_t)(ms(t0, Clock::now()) * 1000.0);
         {
@@ -349,7 +353,7 @@ void builderLoop(hipibs_serve *s)
         }
         s->cvWork.notify_one();
     }
-    hostStagingFree(staging);
+    storeStagingFree(staging);
 }
 
 // Worker side of a finished build: the table becomes resident.
@@ -601,9 +605,17 @@ void destroy(hipibs_serve *s)
             (void)hipStreamDestroy(sl.stream);
     }
     for (auto &kv : s->samples)
+    {
         hipibs_table_free(kv.second.table);
+        keyStoreFree(kv.second.store);
+    }
     (void)hipFree(s->dSeq);
     (void)hipFree(s->dExc);
+    // the pool's retained memory goes back once the frees above are done
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceSynchronize() == hipSuccess &&
+        hipDeviceGetDefaultMemPool(&pool, s->device) == hipSuccess && pool)
+        (void)hipMemPoolTrimTo(pool, 0);
     delete s;
 }
 
@@ -625,6 +637,16 @@ extern "C" int hipibs_serve_create(int device, uint32_t k, uint64_t hbm_budget,
     if (topts)
         s->topts = *topts;
     s->maxWindows = max_windows_per_launch;
+    // admissions allocate tables from the device's default memory pool and
+    // evictions return them there (hipMallocAsync / hipFreeAsync): freed
+    // memory stays in the pool for the next admission instead of going back
+    // to the driver, and neither call waits for the device
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool)
+    {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     s->slots.resize(max_inflight);
     bool ok = true;
     for (auto &sl : s->slots)
@@ -666,7 +688,7 @@ extern "C" int hipibs_serve_set_reference(hipibs_serve *s, const uint64_t *h_seq
     return 0;
 }
 
-static int addSample(hipibs_serve *s, uint32_t sample_id, Sample sm, int preload)
+static int checkRegistration(hipibs_serve *s, uint32_t sample_id)
 {
     if (!s->dSeq)
         return setError(HIPIBS_E_ARG, "set the reference first");
@@ -674,10 +696,26 @@ static int addSample(hipibs_serve *s, uint32_t sample_id, Sample sm, int preload
         return setError(HIPIBS_E_ARG, "samples are registered before the first request");
     if (s->samples.count(sample_id))
         return setError(HIPIBS_E_ARG, "sample id already registered");
+    return 0;
+}
+
+static int addSample(hipibs_serve *s, uint32_t sample_id, Sample sm, int preload,
+                     const uint64_t *dKeys)
+{
+    int rc = checkRegistration(s, sample_id);
+    if (rc)
+    {
+        keyStoreFree(sm.store);
+        return rc;
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->counters.host_bytes += sm.store.suf ? sm.store.hostBytes() : 0;
+    }
     s->samples[sample_id] = std::move(sm);
     if (preload)
     {
-        const int rc = ensureResident(s, sample_id, false, true);
+        rc = ensureResident(s, sample_id, false, true, dKeys);
         if (rc < 0)
             return rc;
         if (rc == 1)
@@ -686,15 +724,52 @@ static int addSample(hipibs_serve *s, uint32_t sample_id, Sample sm, int preload
     return 0;
 }
 
+// A sample from keys on the device: the engine's host copy, then (preload)
+// the table from the same device keys.
+static int addSampleDevice(hipibs_serve *s, uint32_t sample_id, const uint64_t *dKeys,
+                           uint64_t n, int preload)
+{
+    Sample sm;
+    sm.n = n;
+    int rc = keyStoreFromDevice(dKeys, n, s->k, s->device, nullptr, sm.store);
+    if (rc)
+        return rc;
+    return addSample(s, sample_id, std::move(sm), preload, dKeys);
+}
+
 extern "C" int hipibs_serve_add_sample(hipibs_serve *s, uint32_t sample_id, const uint64_t *h_keys,
                                        uint64_t n_keys, int preload)
 {
     if (!s || (n_keys && !h_keys))
         return setError(HIPIBS_E_ARG, "null argument");
-    Sample sm;
-    sm.keys = h_keys;
-    sm.n = n_keys;
-    return addSample(s, sample_id, std::move(sm), preload);
+    int rc = checkRegistration(s, sample_id);
+    if (rc)
+        return rc;
+    HIPIBS_CHECK(hipSetDevice(s->device));
+    const uint64_t bytes = std::max<uint64_t>(n_keys, 1) * (s->k > 32 ? 16 : 8);
+    uint64_t *dKeys = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&dKeys), bytes) != hipSuccess)
+        return setError(HIPIBS_E_NOMEM, "device memory to register a sample's keys");
+    rc = n_keys && hipMemcpy(dKeys, h_keys, n_keys * (s->k > 32 ? 16 : 8),
+                             hipMemcpyHostToDevice) != hipSuccess
+             ? setError(HIPIBS_E_HIP, "key upload failed")
+             : 0;
+    if (!rc)
+        rc = addSampleDevice(s, sample_id, dKeys, n_keys, preload);
+    (void)hipFree(dKeys);
+    return rc;
+}
+
+extern "C" int hipibs_serve_add_sample_device(hipibs_serve *s, uint32_t sample_id,
+                                              const uint64_t *d_keys, uint64_t n_keys, int preload)
+{
+    if (!s || (n_keys && !d_keys))
+        return setError(HIPIBS_E_ARG, "null argument");
+    int rc = checkRegistration(s, sample_id);
+    if (rc)
+        return rc;
+    HIPIBS_CHECK(hipSetDevice(s->device));
+    return addSampleDevice(s, sample_id, d_keys, n_keys, preload);
 }
 
 extern "C" int hipibs_serve_add_sample_kmc(hipibs_serve *s, uint32_t sample_id,
@@ -719,7 +794,7 @@ extern "C" int hipibs_serve_add_sample_kmc(hipibs_serve *s, uint32_t sample_id,
     const uint64_t preBytes =
         stat((std::string(kmc_prefix) + ".kmc_pre").c_str(), &st) == 0 ? (uint64_t)st.st_size : 0;
     sm.kmcStage = 2 * std::min<uint64_t>(info.total_kmers, 32ull << 20) * recBytes + preBytes;
-    return addSample(s, sample_id, std::move(sm), preload);
+    return addSample(s, sample_id, std::move(sm), preload, nullptr);
 }
 
 extern "C" int hipibs_serve_submit(hipibs_serve *s, uint64_t request_id, uint32_t sample_id,

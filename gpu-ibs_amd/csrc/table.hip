@@ -535,8 +535,8 @@ int launchProbe(const hipibs_table *t, const uint64_t *dCodes, const uint8_t *dV
 }
 
 // Allocates and clears a table sized for nKeys keys.
-static int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts,
-                      hipibs_table **out, hipStream_t s)
+int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts, hipibs_table **out,
+               hipStream_t s, bool streamOrdered)
 {
     hipibs_table_opts o{};
     if (opts)
@@ -577,19 +577,22 @@ static int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts,
     t->mlen = mlen;
     t->nBuckets = nBuckets;
     t->bytes = bytes;
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(&t->dSlots), bytes);
+    t->streamOrdered = streamOrdered;
+    auto alloc = [&](void **p, uint64_t b) {
+        return streamOrdered ? hipMallocAsync(p, b, s) : hipMalloc(p, b);
+    };
+    hipError_t e = alloc(reinterpret_cast<void **>(&t->dSlots), bytes);
     if (e != hipSuccess)
     {
         delete t;
-        return setError(HIPIBS_E_NOMEM, "hipMalloc of " + std::to_string(bytes) +
+        return setError(HIPIBS_E_NOMEM, "allocation of " + std::to_string(bytes) +
                                              " table bytes failed: " + hipGetErrorString(e));
     }
-    e = hipMalloc(reinterpret_cast<void **>(&t->dCounters), 4 * sizeof(uint64_t));
+    e = alloc(reinterpret_cast<void **>(&t->dCounters), 4 * sizeof(uint64_t));
     if (e != hipSuccess)
     {
-        hipFree(t->dSlots);
-        delete t;
-        return setError(HIPIBS_E_NOMEM, "hipMalloc of table counters failed");
+        freeTableOn(t, s);
+        return setError(HIPIBS_E_NOMEM, "allocation of table counters failed");
     }
     const uint64_t nSlots = nBuckets * o.bucket_slots;
     hipLaunchKernelGGL(fillKernel, dim3(gridFor(nSlots, 256, o.device)), dim3(256), 0, s,
@@ -598,17 +601,26 @@ static int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts,
     e = hipGetLastError();
     if (e != hipSuccess)
     {
-        hipFree(t->dSlots);
-        hipFree(t->dCounters);
-        delete t;
+        freeTableOn(t, s);
         return hipFail(e, "table clear");
     }
     *out = t;
     return 0;
 }
 
+void freeTableOn(hipibs_table *t, hipStream_t s)
+{
+    if (!t)
+        return;
+    (void)hipSetDevice(t->device);
+    for (uint64_t *p : {t->dSlots, t->dSlots2, t->dCounters})
+        if (p)
+            (void)(t->streamOrdered ? hipFreeAsync(p, s) : hipFree(p));
+    delete t;
+}
+
 // Reads back the build counters; fails when the table overflowed.
-static int finishTable(hipibs_table *t, hipStream_t s)
+int finishTable(hipibs_table *t, hipStream_t s)
 {
     uint64_t c[4] = {0, 0, 0, 0};
     HIPIBS_CHECK(hipMemcpyAsync(c, t->dCounters, sizeof(c), hipMemcpyDeviceToHost, s));
@@ -1105,16 +1117,7 @@ extern "C" int hipibs_table_build_keys(const uint64_t *h_keys, uint64_t n, uint3
 
 extern "C" int hipibs_table_free(hipibs_table *t)
 {
-    if (!t)
-        return 0;
-    hipSetDevice(t->device);
-    if (t->dSlots)
-        hipFree(t->dSlots);
-    if (t->dSlots2)
-        hipFree(t->dSlots2);
-    if (t->dCounters)
-        hipFree(t->dCounters);
-    delete t;
+    freeTableOn(t, nullptr);
     return 0;
 }
 

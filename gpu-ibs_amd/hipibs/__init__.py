@@ -66,7 +66,8 @@ class ScanConfig(C.Structure):
 class ServeCounters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "launches", "windows",
                                          "admissions", "evictions", "resident_bytes",
-                                         "resident_tables", "build_us", "evict_us")]
+                                         "resident_tables", "build_us", "evict_us",
+                                         "host_bytes")]
 
 
 class CountStats(C.Structure):
@@ -164,6 +165,7 @@ _SIGS = [
                                       C.c_int, C.c_uint32, C.POINTER(vp)]),
     ("hipibs_serve_set_reference", C.c_int, [vp, u64p, u64p, C.c_uint64]),
     ("hipibs_serve_add_sample", C.c_int, [vp, C.c_uint32, u64p, C.c_uint64, C.c_int]),
+    ("hipibs_serve_add_sample_device", C.c_int, [vp, C.c_uint32, vp, C.c_uint64, C.c_int]),
     ("hipibs_serve_add_sample_kmc", C.c_int, [vp, C.c_uint32, C.c_char_p, C.c_int]),
     ("hipibs_serve_submit", C.c_int, [vp, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64,
                                       C.c_uint32]),
@@ -754,10 +756,17 @@ class ServeEngine:
     def add_sample(self, sample_id: int, keys: np.ndarray, preload: bool = False) -> None:
         """keys: uint64 codes (k <= 32) or an (n, 2) array of (high, low)."""
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
-        self._keep.append(keys)
         _check(_lib.hipibs_serve_add_sample(self.handle, sample_id, _ptr(keys),
                                             _key_count(keys, self.k), int(preload)),
                "hipibs_serve_add_sample")
+
+    def add_sample_device(self, sample_id: int, d_keys: int, n_keys: int,
+                          preload: bool = False) -> None:
+        """d_keys: device address of n_keys codes (two words each for k > 32),
+        complete before the call."""
+        _check(_lib.hipibs_serve_add_sample_device(self.handle, sample_id, vp(d_keys), n_keys,
+                                                   int(preload)),
+               "hipibs_serve_add_sample_device")
 
     def add_sample_kmc(self, sample_id: int, kmc_prefix: str, preload: bool = False) -> None:
         _check(_lib.hipibs_serve_add_sample_kmc(self.handle, sample_id, kmc_prefix.encode(),

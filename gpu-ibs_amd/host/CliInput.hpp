@@ -169,11 +169,12 @@ inline const char *optionsUsage()
            "  --bucket-bytes=64|128 device table bucket width (default 128; a panel\n"
            "                        times both unless this is given)\n"
            "  --hbm-budget-gb=N     cap on device table memory\n"
-           "  --minimizer=M[,CAP]   minimizer table (k <= 32, M in 5..min(31, k), groups of\n"
+           "  --minimizer=M[,CAP]   minimizer tables (k <= 32, M in 5..min(31, k), groups of\n"
            "                        up to CAP keys, default 16): runs of positions that share\n"
-           "                        a minimizer read their bucket once; one sample per run.\n"
-           "                        Default for one sample at k = 13..32: M = k - 4;\n"
-           "                        --minimizer=0: whole-k-mer table (panels, intersect)\n"
+           "                        a minimizer read their bucket once. Default for one\n"
+           "                        sample at k = 13..32: M = k - 4; panels and intersects\n"
+           "                        default to whole-k-mer tables (M given: minimizer tables\n"
+           "                        for every sample); --minimizer=0: whole-k-mer tables\n"
            "  --db=<folder>         (fastibs) another sample in the same reference pass\n"
            "  --db-list=<file>      (fastibs) sample folders, one per line\n"
            "  --resident=N          (fastibs) at most N sample tables on the device at once\n";

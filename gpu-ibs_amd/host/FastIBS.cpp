@@ -74,12 +74,6 @@ static int run(int argc, char **argv)
         }
     }
     dbDirs.insert(dbDirs.begin(), sourcePath);
-    if (dbDirs.size() > 1 && KmerDatabase::gpuOptions().minimizerLen > 0)
-    {
-        cerr << "Error: --minimizer: panel runs (--db, --db-list) need whole-k-mer tables"
-             << endl;
-        return 1;
-    }
     vector<string> prefixes, names;
     for (auto &d : dbDirs)
     {

@@ -30,16 +30,50 @@ public:
     uint32_t getKmerSize() const { return dbs_[0]->getKmerSize(); }
     size_t size() const { return dbs_.size(); }
 
-    // Bytes the device table of database i will take.
+    // Bytes the device table of database i will take (the sizing rules of
+    // the table builds: whole-k-mer buckets of 16 keys, or 8 two-word keys
+    // for k > 32; a minimizer table's two parts hold at most n keys plus a
+    // marker per group of more than cap keys, each part rounded up to whole
+    // buckets).
     uint64_t tableBytes(size_t i) const
     {
-        const auto &g = KmerDatabase::gpuOptions();
-        const double lf =
-            KmerDatabase::tableOptions(dbs_[i]->getKmerSize(), KmerDatabase::TableUse::Lookup)
-                .load_factor;
-        const double slots = std::ceil((double)std::max<uint64_t>(1, dbs_[i]->totalKmers()) / lf);
-        const uint64_t buckets = std::max<uint64_t>(2, (uint64_t)std::ceil(slots / g.bucketSlots));
-        return buckets * g.bucketSlots * 8;
+        const uint32_t k = dbs_[i]->getKmerSize();
+        const uint64_t n = std::max<uint64_t>(1, dbs_[i]->totalKmers());
+        const hipibs_table_opts o =
+            KmerDatabase::tableOptions(k, KmerDatabase::TableUse::Lookup);
+        const uint32_t slots = o.bucket_slots ? o.bucket_slots : 16;
+        if (o.minimizer_len)
+        {
+            const double items = (double)n + std::ceil((double)n / (o.minimizer_cap + 1.0));
+            return (uint64_t)std::ceil(std::ceil(items / o.load_factor) * 8.0) +
+                   3ull * slots * 8 + 3ull * 64;
+        }
+        const uint32_t perBucket = k > 32 ? 8 : slots;
+        const uint64_t buckets = std::max<uint64_t>(
+            2, (uint64_t)std::ceil(std::ceil((double)n / o.load_factor) / perBucket));
+        return buckets * slots * 8;
+    }
+
+    // Device bytes a build holds beside the tables already resident: a
+    // minimizer table's build holds every key (8 B) and its group map (a
+    // power of two >= n entries of 8 B); a whole-k-mer build from KMC
+    // records streams them through small fixed staging (in the reserve).
+    uint64_t buildTransientBytes() const
+    {
+        const hipibs_table_opts o =
+            KmerDatabase::tableOptions(getKmerSize(), KmerDatabase::TableUse::Lookup);
+        if (!o.minimizer_len)
+            return 0;
+        uint64_t most = 0;
+        for (auto &d : dbs_)
+        {
+            const uint64_t n = d->totalKmers();
+            uint64_t map = 1024;
+            while (map < n)
+                map <<= 1;
+            most = std::max<uint64_t>(most, n * 8 + map * 8);
+        }
+        return most;
     }
 
     // Device bytes the panel's tables may take.
@@ -48,7 +82,7 @@ public:
         const auto &g = KmerDatabase::gpuOptions();
         uint64_t freeB = 0, totalB = 0;
         hipibs::check(hipibs_device_mem(g.device, &freeB, &totalB), "device memory query");
-        const uint64_t reserve = reserveBytes_ + (4ull << 30);
+        const uint64_t reserve = reserveBytes_ + (4ull << 30) + buildTransientBytes();
         uint64_t budget = freeB > reserve ? freeB - reserve : 0;
         if (g.hbmBudget)
             budget = std::min(budget, g.hbmBudget);

@@ -448,10 +448,12 @@ typedef struct hipibs_serve_counters {
     uint64_t resident_bytes, resident_tables;
     uint64_t build_us;                /* summed wall time of table builds */
     uint64_t evict_us;                /* summed wall time of eviction frees */
+    uint64_t host_bytes;              /* the engine's host copies of sample keys */
 } hipibs_serve_counters;
 
 /* A serving engine on `device` for k-mers of length k: sample tables are
- * built (with topts) on demand from registered host keys and kept resident
+ * built (with topts) on demand from the engine's host copy of each
+ * registered sample's keys (or its KMC database) and kept resident
  * within hbm_budget bytes; idle tables are evicted least requested first
  * (request counts halved every 4096 requests), and only for a sample
  * requested at least as often unless its request has waited 200 ms; up to
@@ -467,11 +469,23 @@ int hipibs_serve_create(int device, uint32_t k, uint64_t hbm_budget,
 int hipibs_serve_set_reference(hipibs_serve *s, const uint64_t *h_seq_words,
                                const uint64_t *h_exc_words, uint64_t n_bases);
 
-/* Registers a sample's canonical keys (host memory the caller keeps valid
- * for the engine's lifetime); preload != 0 builds its table now. All
- * samples are registered before the first request. */
+/* Registers a sample's canonical keys (n_keys codes of 2k bits, two words
+ * each for k > 32); preload != 0 builds its table now. The engine keeps
+ * its own compact copy in host memory (pinned where the system grants
+ * it): each code's top <= 22 bits are implied by its position among keys
+ * grouped by them, the rest kept as whole bytes -- 4 B per key at k = 25,
+ * 5 B at k = 31 -- plus 4 B per prefix value; admissions upload that copy
+ * and decode it on the device. The caller's array may be freed on return.
+ * Registration uses device memory for the keys and their suffixes while
+ * it runs. All samples are registered before the first request; fewer
+ * than 2^32 keys per sample. */
 int hipibs_serve_add_sample(hipibs_serve *s, uint32_t sample_id, const uint64_t *h_keys,
                             uint64_t n_keys, int preload);
+
+/* hipibs_serve_add_sample from keys in device memory (complete before the
+ * call); a preloaded table is built from them directly. */
+int hipibs_serve_add_sample_device(hipibs_serve *s, uint32_t sample_id, const uint64_t *d_keys,
+                                   uint64_t n_keys, int preload);
 
 /* Registers a sample by its KMC database (the prefix of .kmc_pre/.kmc_suf,
  * which must stay readable): admissions stream its raw records through the

@@ -453,24 +453,35 @@ def _packed_windows(seq, W, k):
     return D.from_array(sw), D.from_array(ew), D.from_array(win), wins
 
 
-@pytest.mark.parametrize("n_tables", [1, 3, 17])
-def test_panel_kernel_equals_single_scans(small_corpus, n_tables):
+PANEL_MINIMIZER = {"whole": dict(bucket_slots=8),
+                   "m21": dict(minimizer_len=21, minimizer_cap=4, load_factor=0.35,
+                               bucket_slots=16),
+                   "m15s8": dict(minimizer_len=15, minimizer_cap=16, load_factor=0.3,
+                                 bucket_slots=8)}
+
+
+@pytest.mark.parametrize("n_tables,kind", [(1, "whole"), (3, "whole"), (17, "whole"),
+                                           (1, "m21"), (3, "m21"), (17, "m21"), (5, "m15s8")])
+def test_panel_kernel_equals_single_scans(small_corpus, n_tables, kind):
     """The persistent one-encode-many-probes panel kernel (16 tables per
     launch, so 17 spans two) gives each table's single-scan statistics, for
-    several launch configurations."""
+    several launch configurations; over whole-k-mer tables and over
+    minimizer tables (each word's minimizers computed once, every table
+    probed with the grouped probe; a cap of 4 sends repeat groups to the
+    whole-k-mer part)."""
     import ctypes as C
     ref, samples = small_corpus
     seq = corpus.to_text(ref)
     keysets = [samples[i % 3][0][: (samples[i % 3][0].size * (i + 5)) // (n_tables + 5)]
                for i in range(n_tables)]
-    tabs = [hipibs.Table.from_keys(ks, 25, bucket_slots=(8 if i % 2 == 0 else 8))
-            for i, ks in enumerate(keysets)]
+    opts = PANEL_MINIMIZER[kind]
+    tabs = [hipibs.Table.from_keys(ks, 25, **opts) for ks in keysets]
     d_sw, d_ew, d_win, wins = _packed_windows(seq, 20000, 25)
     nw = len(wins)
     arr = hipibs._table_array(tabs)
     want = [oracle_scan_rows([b"chr1"], [seq], 20000, 25, sorted_keys(ks)) for ks in keysets]
     for segs, wpc in ((1, 16), (3, 8), (2, 32)):
-        note(f"panel n={n_tables} segs={segs} wpc={wpc}")
+        note(f"panel n={n_tables} {kind} segs={segs} wpc={wpc}")
         wb = hipibs._lib.hipibs_panel_work_bytes(nw, n_tables, segs)
         d_work = D.zeros(wb)
         d_st = D.zeros(nw * n_tables * 16)
@@ -601,6 +612,79 @@ def test_serving_results_same_alone_and_in_the_mix(small_corpus, tmp_path):
         want = oracle_scan_rows([b"c"], [seq[base:base + ln]], W, 25, sorted_keys(keys[sid]))
         got = [tuple(int(x) for x in row) for row in mixed[rid]]
         assert got == [w[3:] for w in want], rid
+
+
+@pytest.mark.parametrize("k", [11, 12, 21, 25, 31, 32, 33, 47, 64])
+def test_serving_key_copy_round_trip(k):
+    """Admissions rebuild tables from the engine's compact host copy of a
+    sample's keys (the top <= 22 bits of each code implied by its position
+    among keys grouped by them, the rest kept as whole bytes: 0 bytes at
+    k = 11, 1 at k = 12, ..., 14 at k = 64). A budget of 1.2 times the
+    largest table makes nearly every request re-admit its sample from that
+    copy. The
+    reference holds T^i A^(k-i) runs, so codes whose prefix is all ones
+    are hit; samples come from host keys and from device keys (with A^k,
+    code 0). Every request equals the oracle."""
+    wide = k > 32
+    rnd = corpus.random_bases(0x5E7E, k, 40_000)
+    runs = np.concatenate([np.full(k, 3, np.uint8), np.full(k, 0, np.uint8),
+                           np.full(k + 3, 0, np.uint8)])
+    codes = np.concatenate([rnd, runs, corpus.random_bases(0x5E7E, 100 + k, 2_000)])
+    seq = corpus.to_text(codes)
+    sw, ew, _ = hipibs.pack_sequences([seq])
+    n = len(seq)
+    every = (corpus.sequence_kmers_wide if wide else corpus.sequence_kmers)([codes], k)
+    rng = np.random.default_rng(k)
+    samples = [every, every[rng.random(every.shape[0]) < 0.5], every[::3]]
+    samples = [np.ascontiguousarray(x) for x in samples]
+    one = hipibs.Table.from_keys(samples[0], k)
+    budget = int(1.2 * one.info()["bytes"])
+    one.free()
+    eng = hipibs.ServeEngine(k, budget, max_inflight=2, max_windows_per_launch=256)
+    eng.set_reference(sw, ew, n)
+    eng.add_sample(0, samples[0])
+    eng.add_sample(1, samples[1])
+    d = hipibs.DeviceBuffer.from_array(samples[2])
+    eng.add_sample_device(2, d.ptr, samples[2].shape[0])
+    d.free()
+    reqs = []
+    for i in range(9):
+        ln = int(rng.integers(2 * k, n))
+        reqs.append((i, i % 3, int(rng.integers(0, n - ln + 1)), ln, int(rng.choice([300, 5000]))))
+    reqs.append((9, 0, rnd.size - 5, 3 * k + 10, 2 * k))  # the T/A runs
+    out = {}
+    for r in reqs:  # one at a time: each admits its sample again
+        eng.submit(*r)
+        got = []
+        while not got:
+            got = eng.poll(timeout_ms=2000)
+        assert got == [r[0]]
+        out[r[0]] = eng.fetch(r[0])[0]
+    c = eng.counters()
+    eng.free()
+    assert c["evictions"] >= 6 and c["admissions"] >= 8
+    bpk = {11: 0, 12: 1, 21: 3, 25: 4, 31: 5, 32: 6, 33: 6, 47: 9, 64: 14}[k]
+    assert c["host_bytes"] == sum(x.shape[0] for x in samples) * bpk + 3 * 4 * (
+        (1 << (2 * k - 8 * bpk)) + 1)
+    for rid, sid, base, ln, W in reqs:
+        if wide:
+            want = oracle_scan_rows_wide([b"c"], [seq[base:base + ln]], W, k,
+                                         sorted_keys_wide(samples[sid]))
+        else:
+            want = oracle_scan_rows([b"c"], [seq[base:base + ln]], W, k, sorted_keys(samples[sid]))
+        assert [tuple(int(x) for x in row) for row in out[rid]] == [w[3:] for w in want], rid
+
+
+def test_serving_rejects_codes_wider_than_2k_bits():
+    seq = corpus.to_text(corpus.random_bases(3, 3, 5_000))
+    sw, ew, _ = hipibs.pack_sequences([seq])
+    for k, bad in ((25, np.array([1 << 50], dtype=np.uint64)),
+                   (40, np.array([[1 << 16, 0]], dtype=np.uint64))):
+        eng = hipibs.ServeEngine(k, 1 << 30, max_inflight=1, max_windows_per_launch=64)
+        eng.set_reference(sw, ew, len(seq))
+        with pytest.raises(hipibs.HipibsError, match="2k-bit"):
+            eng.add_sample(0, bad)
+        eng.free()
 
 
 def test_minimizer_serving_equals_oracle(small_corpus):

@@ -55,12 +55,6 @@ def test_help_and_bad_arguments(tmp_path):
         assert r.returncode == 1 and "--minimizer=M[,CAP]" in r.stderr, bad
 
 
-def test_minimizer_tables_not_for_panels(small_layout, tmp_path):
-    r = tool("fastibs", small_layout / "s0", small_layout / "refs", tmp_path / "out", 50000,
-             f"--db={small_layout / 's1'}", "--minimizer=21")
-    assert r.returncode == 1 and "panel runs" in r.stderr
-
-
 def test_window_not_larger_than_k(small_layout, tmp_path):
     r = tool("fastibs", small_layout / "s0", small_layout / "refs", tmp_path / "out", 25)
     assert r.returncode == 1 and "must be larger than" in r.stderr
@@ -90,15 +84,19 @@ def test_fastibs_matches_golden_and_skips_existing(small_layout, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("resident", [2, 1])
-def test_fastibs_panel_equals_singles(small_layout, tmp_path, resident):
+@pytest.mark.parametrize("resident,opts", [(2, ()), (1, ()), (2, ("--minimizer=21",)),
+                                           (1, ("--minimizer=13,4", "--load-factor=0.3"))])
+def test_fastibs_panel_equals_singles(small_layout, tmp_path, resident, opts):
     """Panel runs in 2 or 3 memory groups (the next group's tables built
-    while the current one scans) give the single-sample TSVs."""
+    while the current one scans) give the single-sample TSVs, over
+    whole-k-mer tables and over minimizer tables (the panel kernel's
+    grouped probe)."""
     out = tmp_path / "panel"
     lst = tmp_path / "list.txt"
     lst.write_text(f"# panel\n{small_layout / 's2'}\n")
     r = tool("fastibs", small_layout / "s0", small_layout / "refs", out, 50000,
-             f"--db={small_layout / 's1'}", f"--db-list={lst}", f"--resident={resident}")
+             f"--db={small_layout / 's1'}", f"--db-list={lst}", f"--resident={resident}",
+             *opts)
     assert r.returncode == 0, r.stderr
     for i in range(3):
         got = (out / f"s{i}_v_small_50000.tsv").read_bytes()
