@@ -223,7 +223,8 @@ __global__ __launch_bounds__(256) void groupCountKernel(const uint64_t *keys, ui
 }
 
 // tally[0] += keys of groups of at most cap keys, tally[1] += larger
-// groups (one marker each), tally[2] += their keys.
+// groups (one marker each), tally[2] += their keys; tally[4] = 1 if a
+// group's count reached 2^31 (bit 63 of its entry).
 __global__ __launch_bounds__(256) void groupTallyKernel(const unsigned long long *map,
                                                         uint64_t size, uint32_t cap,
                                                         unsigned long long *tally)
@@ -235,6 +236,8 @@ __global__ __launch_bounds__(256) void groupTallyKernel(const unsigned long long
         const unsigned long long e = map[i];
         if (e == EMPTY)
             continue;
+        if (e >> 63)
+            atomicExch(tally + 4, 1ull);
         const uint64_t c = (e >> 32) & 0x7fffffffull;
         if (c <= cap)
             a += c;
@@ -690,8 +693,11 @@ static int buildMinimizerTable(const uint64_t *dKeys, uint64_t n, uint32_t k,
     const uint32_t cap = o.minimizer_cap ? o.minimizer_cap : kDefaultMinimizerCap;
     if (cap > 4096)
         return setError(HIPIBS_E_ARG, "minimizer_cap must be at most 4096");
-    if (n >= (1ull << 31))
-        return setError(HIPIBS_E_ARG, "minimizer tables hold fewer than 2^31 keys");
+    // a group map entry holds its group's key count in bits 32..62 (bit 63
+    // is the placement's marker flag): with fewer than 2^32 keys a count
+    // never wraps, and one that reaches 2^31 shows as bit 63 in the tally
+    if (n >= (1ull << 32))
+        return setError(HIPIBS_E_ARG, "minimizer tables hold fewer than 2^32 keys");
     HIPIBS_CHECK(hipSetDevice(o.device));
     // group map: at most one group per key, so a map of >= n entries always
     // has room
@@ -699,7 +705,7 @@ static int buildMinimizerTable(const uint64_t *dKeys, uint64_t n, uint32_t k,
     unsigned long long *map = nullptr, *tally = nullptr;
     if (hipMalloc(reinterpret_cast<void **>(&map), mapSize * 8) != hipSuccess)
         return setError(HIPIBS_E_NOMEM, "hipMalloc of the minimizer group map failed");
-    if (hipMalloc(reinterpret_cast<void **>(&tally), 4 * 8) != hipSuccess)
+    if (hipMalloc(reinterpret_cast<void **>(&tally), 5 * 8) != hipSuccess)
     {
         hipFree(map);
         return setError(HIPIBS_E_NOMEM, "hipMalloc of the group tally failed");
@@ -708,13 +714,13 @@ static int buildMinimizerTable(const uint64_t *dKeys, uint64_t n, uint32_t k,
     hipLaunchKernelGGL(fillKernel, dim3(gridFor(mapSize, 256, dev)), dim3(256), 0, s,
                        reinterpret_cast<uint64_t *>(map), mapSize, EMPTY);
     hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, s, reinterpret_cast<uint64_t *>(tally),
-                       (uint64_t)4, 0ull);
+                       (uint64_t)5, 0ull);
     if (n)
         hipLaunchKernelGGL(groupCountKernel, dim3(gridFor(n, 256, dev)), dim3(256), 0, s, dKeys, n,
                            k, mlen, map, mapSize - 1, tally + 3);
     hipLaunchKernelGGL(groupTallyKernel, dim3(gridFor(mapSize, 256, dev)), dim3(256), 0, s, map,
                        mapSize, cap, tally);
-    uint64_t tl[4] = {0, 0, 0, 0};
+    uint64_t tl[5] = {0, 0, 0, 0, 0};
     int rc = 0;
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(tl, tally, sizeof(tl), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -722,6 +728,8 @@ static int buildMinimizerTable(const uint64_t *dKeys, uint64_t n, uint32_t k,
         rc = setError(HIPIBS_E_HIP, "minimizer group count failed");
     if (!rc && tl[3])
         rc = setError(HIPIBS_E_HIP, "minimizer group map overflow");
+    if (!rc && tl[4])
+        rc = setError(HIPIBS_E_ARG, "a minimizer group holds 2^31 keys or more");
     hipibs_table *t = nullptr;
     if (!rc)
     {
