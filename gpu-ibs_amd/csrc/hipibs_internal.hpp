@@ -17,7 +17,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "hipibs.h"
@@ -321,11 +323,12 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
                TableView *dViews, bool profile = true);
 int uploadViews(const hipibs_table *const *tables, int nTables, hipStream_t s, TableView **out);
 // grouped: every table the windows name is a minimizer table (probed with
-// waveProbeGrouped).
+// waveProbeGrouped). segs > 1 splits every window into that many blocks
+// (their summaries in dWork, nWindows * segs of them, folded in order).
 int launchScanMixed(int bucketSlots, uint32_t k, const TableView *dViews, const uint64_t *dSeq,
                     const uint64_t *dExc, uint64_t nBases, const hipibs_window *dWindows,
                     uint32_t nWindows, hipibs_window_stats *dStats, hipStream_t s,
-                    bool grouped = false);
+                    bool grouped = false, uint32_t segs = 1, SegSummary *dWork = nullptr);
 int checkTables(const hipibs_table *const *tables, int nTables);
 uint32_t autoSegsPerWindow(uint32_t nWindows, int nTables, uint32_t maxPositions);
 uint64_t scanWorkBytes(uint32_t nWindows, int nTables, uint32_t segsPerWindow);
@@ -382,13 +385,42 @@ void hostStagingFree(HostStaging *st);
 int buildTableFromHost(const uint64_t *hKeys, uint64_t n, uint32_t k, const hipibs_table_opts *opts,
                        HostStaging *st, hipibs_table **out);
 
+// An allocator whose default construction leaves the element as it is:
+// vectors sized for bulk reads (a KMC LUT is 1 GiB at p = 9) skip the
+// zero fill and are first touched by the threads that read into them.
+template <class T>
+struct NoInitAlloc : std::allocator<T>
+{
+    template <class U>
+    struct rebind
+    {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U> &) noexcept
+    {
+    }
+    template <class U>
+    void construct(U *p) noexcept
+    {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a)
+    {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+using LutVector = std::vector<uint64_t, NoInitAlloc<uint64_t>>;
+
 // The raw records of a KMC suffix file, for readers that decode on the
 // device: record r of bucket j (lut[j] <= r < lut[j+1]) holds the key
 // ((j mod 4^p) << 2(k-p)) | (its first sufBytes bytes, big-endian).
 struct KmcRecords
 {
     hipibs_kmc_info info{};
-    std::vector<uint64_t> lut; // one entry per bucket plus a guard > total
+    LutVector lut; // one entry per bucket plus a guard > total
     std::string path;
     int fd = -1;
     uint32_t sufBytes = 0, recBytes = 0;

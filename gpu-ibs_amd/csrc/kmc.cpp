@@ -128,9 +128,59 @@ struct PreFile
     hipibs_kmc_info info{};
     uint64_t headerOffset = 0;
     uint64_t fileBytes = 0;
-    std::vector<uint64_t> lut; // with the guard entry
+    LutVector lut; // with the guard entry
     std::vector<uint32_t> sigMap;
 };
+
+// preadAll of [off, off + bytes) into dst by up to `threads` threads in
+// pieces of at least 4 MiB (each thread also first-touches its part).
+bool preadParallel(int fd, void *dst, uint64_t bytes, uint64_t off, unsigned threads)
+{
+    const uint64_t minPiece = 4ull << 20;
+    threads = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>({threads, 64, bytes / minPiece}));
+    if (threads == 1)
+        return preadAll(fd, dst, bytes, off);
+    const uint64_t per = (bytes + threads - 1) / threads;
+    std::atomic<bool> bad{false};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < threads; ++t)
+        pool.emplace_back([&, t] {
+            const uint64_t a = std::min(bytes, (uint64_t)t * per), b = std::min(bytes, a + per);
+            if (a < b && !preadAll(fd, static_cast<char *>(dst) + a, b - a, off + a))
+                bad = true;
+        });
+    for (auto &th : pool)
+        th.join();
+    return !bad;
+}
+
+// lut[0 .. n-1] non-decreasing, checked on up to `threads` threads.
+bool monotonic(const uint64_t *lut, uint64_t n, unsigned threads)
+{
+    threads = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>({threads, 64, n / (1u << 20)}));
+    const uint64_t per = (n + threads - 1) / threads;
+    std::atomic<bool> bad{false};
+    auto check = [&](unsigned t) {
+        const uint64_t a = std::min(n, (uint64_t)t * per), b = std::min(n, a + per);
+        for (uint64_t j = std::max<uint64_t>(a, 1); j < b; ++j)
+            if (lut[j] < lut[j - 1])
+            {
+                bad = true;
+                return;
+            }
+    };
+    if (threads == 1)
+        check(0);
+    else
+    {
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < threads; ++t)
+            pool.emplace_back(check, t);
+        for (auto &th : pool)
+            th.join();
+    }
+    return !bad;
+}
 
 int readHeaderOf(int fd, const std::string &path, PreFile &pf)
 {
@@ -214,7 +264,7 @@ int loadPre(const std::string &prefix, PreFile &pf, bool withLut)
         const uint64_t entries = lutArea / 8 + 1;
         pf.lut.resize(entries);
         pf.sigMap.resize(sigEntries);
-        if (!preadAll(f.fd, pf.lut.data(), entries * 8, 4) ||
+        if (!preadParallel(f.fd, pf.lut.data(), entries * 8, 4, usableCpus()) ||
             !preadAll(f.fd, pf.sigMap.data(), sigEntries * 4, 4 + lutArea + 8))
             return setError(HIPIBS_E_IO, "short read of KMC LUT: " + path);
         const uint64_t nBins = (entries - 1) / single;
@@ -225,14 +275,13 @@ int loadPre(const std::string &prefix, PreFile &pf, bool withLut)
     else
     {
         pf.lut.resize(single + 1);
-        if (!preadAll(f.fd, pf.lut.data(), (single + 1) * 8, 4))
+        if (!preadParallel(f.fd, pf.lut.data(), (single + 1) * 8, 4, usableCpus()))
             return setError(HIPIBS_E_IO, "short read of KMC LUT: " + path);
     }
     // The guard entry, as both readers set it.
     pf.lut.back() = I.total_kmers + 1;
-    for (size_t j = 1; j + 1 < pf.lut.size(); ++j)
-        if (pf.lut[j] < pf.lut[j - 1])
-            return setError(HIPIBS_E_IO, "KMC: LUT is not monotonic");
+    if (!monotonic(pf.lut.data(), pf.lut.size() - 1, usableCpus()))
+        return setError(HIPIBS_E_IO, "KMC: LUT is not monotonic");
     return 0;
 }
 
@@ -342,7 +391,7 @@ int kmcReadKeys(const std::string &prefix, uint64_t *keys, uint64_t cap, uint64_
     const uint32_t shift = 2 * (I.kmer_length - I.lut_prefix_length);
     const uint64_t prefixMask = (1ull << (2 * I.lut_prefix_length)) - 1;
     const bool wide = I.kmer_length > 32; // two words per key: high, low
-    const std::vector<uint64_t> &lut = pf.lut;
+    const LutVector &lut = pf.lut;
     const uint64_t nBuckets = lut.size() - 1;
 
     if (threads == 0)

@@ -614,30 +614,37 @@ int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *d
 // indexes dViews); results row i per window; one segment per window.
 int launchScanMixed(int bucketSlots, uint32_t k, const TableView *dViews, const uint64_t *dSeq,
                     const uint64_t *dExc, uint64_t nBases, const hipibs_window *dWindows,
-                    uint32_t nWindows, hipibs_window_stats *dStats, hipStream_t s, bool grouped)
+                    uint32_t nWindows, hipibs_window_stats *dStats, hipStream_t s, bool grouped,
+                    uint32_t segs, SegSummary *dWork)
 {
     if (nWindows == 0)
         return 0;
-    const dim3 grid(nWindows, 1);
+    if (segs == 0 || (segs > 1 && !dWork))
+        return setError(HIPIBS_E_ARG, "segments need a summary buffer");
+    const dim3 grid(nWindows * segs, 1);
+    SegSummary *work = segs > 1 ? dWork : nullptr;
     if (k > 32)
         hipLaunchKernelGGL((scanKernel<16, true, true>), grid, dim3(SCAN_BLOCK), 0, s, dViews,
-                           dSeq, dExc, nBases, dWindows, nWindows, 1u, k, dStats,
-                           (SegSummary *)nullptr);
+                           dSeq, dExc, nBases, dWindows, nWindows, segs, k, dStats, work);
     else if (grouped && bucketSlots == 8)
         hipLaunchKernelGGL((scanKernel<8, true, false, true>), grid, dim3(SCAN_BLOCK), 0, s,
-                           dViews, dSeq, dExc, nBases, dWindows, nWindows, 1u, k, dStats,
-                           (SegSummary *)nullptr);
+                           dViews, dSeq, dExc, nBases, dWindows, nWindows, segs, k, dStats, work);
     else if (grouped)
         hipLaunchKernelGGL((scanKernel<16, true, false, true>), grid, dim3(SCAN_BLOCK), 0, s,
-                           dViews, dSeq, dExc, nBases, dWindows, nWindows, 1u, k, dStats,
-                           (SegSummary *)nullptr);
+                           dViews, dSeq, dExc, nBases, dWindows, nWindows, segs, k, dStats, work);
     else if (bucketSlots == 8)
         hipLaunchKernelGGL((scanKernel<8, true>), grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc,
-                           nBases, dWindows, nWindows, 1u, k, dStats, (SegSummary *)nullptr);
+                           nBases, dWindows, nWindows, segs, k, dStats, work);
     else
         hipLaunchKernelGGL((scanKernel<16, true>), grid, dim3(SCAN_BLOCK), 0, s, dViews, dSeq, dExc,
-                           nBases, dWindows, nWindows, 1u, k, dStats, (SegSummary *)nullptr);
+                           nBases, dWindows, nWindows, segs, k, dStats, work);
     HIPIBS_CHECK(hipGetLastError());
+    if (segs > 1)
+    {
+        hipLaunchKernelGGL(foldKernel, dim3((nWindows + 255) / 256), dim3(256), 0, s, dWork,
+                           nWindows, segs, k, dStats);
+        HIPIBS_CHECK(hipGetLastError());
+    }
     return 0;
 }
 

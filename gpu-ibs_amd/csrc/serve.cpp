@@ -70,7 +70,7 @@ struct Sample
 
 // Admission policy constants.
 constexpr uint64_t kAgeEvery = 4096; // requests between halvings of the hit counts
-constexpr double kForceAdmitMs = 200.0; // a request waiting longer may evict anything
+constexpr double kForceAdmitMs = 50.0; // a request waiting longer may evict anything
 
 struct Built
 {
@@ -82,6 +82,12 @@ struct Built
 // Bases of window positions one launch may take (keeps launches short so a
 // small request never waits behind a whole chromosome of others).
 constexpr uint64_t LAUNCH_BASES = 256ull << 20;
+// A launch of few windows splits each into segments (one block each) until
+// it has about kLaunchBlocksPerCu blocks per CU, keeping segments of at least
+// kMinSegBases bases; at most kSlotSegs summaries per launch.
+constexpr uint32_t kLaunchBlocksPerCu = 8;
+constexpr uint64_t kMinSegBases = 4096;
+constexpr uint32_t kSlotSegs = 1u << 15;
 
 struct Part
 {
@@ -96,6 +102,7 @@ struct Slot
     hipibs_window *hWin = nullptr, *dWin = nullptr;
     hipibs_window_stats *hStats = nullptr, *dStats = nullptr;
     TableView *hViews = nullptr, *dViews = nullptr;
+    SegSummary *dWork = nullptr; // segment summaries of a split launch
     bool busy = false;
     std::vector<Part> parts;
     std::vector<uint32_t> pinned;
@@ -110,6 +117,7 @@ struct hipibs_serve
     uint64_t budget = 0;
     hipibs_table_opts topts{};
     uint32_t maxWindows = 0;
+    int cus = 256; // compute units of the device
     uint64_t *dSeq = nullptr, *dExc = nullptr;
     uint64_t nBases = 0;
     std::unordered_map<uint32_t, Sample> samples;
@@ -491,9 +499,19 @@ int fillAndLaunch(hipibs_serve *s, Slot &sl, bool &launched)
                                 hipMemcpyHostToDevice, sl.stream));
     HIPIBS_CHECK(hipMemcpyAsync(sl.dWin, sl.hWin, used * sizeof(hipibs_window),
                                 hipMemcpyHostToDevice, sl.stream));
+    // few windows (a lone small request): segments keep the CUs busy, so
+    // its latency is not one block's walk over a whole window
+    uint32_t segs = 1;
+    const uint64_t want = (uint64_t)s->cus * kLaunchBlocksPerCu;
+    if (used < want)
+        segs = (uint32_t)std::min<uint64_t>(
+            {(want + used - 1) / used, std::max<uint64_t>(1, bases / used / kMinSegBases),
+             kSlotSegs / used});
+    segs = std::max<uint32_t>(segs, 1);
     // every table of the engine is built with the engine's options
     int rc = launchScanMixed(slots, s->k, sl.dViews, s->dSeq, s->dExc, s->nBases, sl.dWin, used,
-                             sl.dStats, sl.stream, s->topts.minimizer_len != 0 && s->k <= 32);
+                             sl.dStats, sl.stream, s->topts.minimizer_len != 0 && s->k <= 32,
+                             segs, sl.dWork);
     if (rc)
         return rc;
     HIPIBS_CHECK(hipMemcpyAsync(sl.hStats, sl.dStats, used * sizeof(hipibs_window_stats),
@@ -599,6 +617,7 @@ void destroy(hipibs_serve *s)
         (void)hipFree(sl.dWin);
         (void)hipFree(sl.dStats);
         (void)hipFree(sl.dViews);
+        (void)hipFree(sl.dWork);
         if (sl.done)
             (void)hipEventDestroy(sl.done);
         if (sl.stream)
@@ -659,8 +678,10 @@ extern "C" int hipibs_serve_create(int device, uint32_t k, uint64_t hbm_budget,
              hipHostMalloc(reinterpret_cast<void **>(&sl.hViews), W * sizeof(TableView)) == hipSuccess &&
              hipMalloc(reinterpret_cast<void **>(&sl.dWin), W * sizeof(hipibs_window)) == hipSuccess &&
              hipMalloc(reinterpret_cast<void **>(&sl.dStats), W * sizeof(hipibs_window_stats)) == hipSuccess &&
-             hipMalloc(reinterpret_cast<void **>(&sl.dViews), W * sizeof(TableView)) == hipSuccess;
+             hipMalloc(reinterpret_cast<void **>(&sl.dViews), W * sizeof(TableView)) == hipSuccess &&
+             hipMalloc(reinterpret_cast<void **>(&sl.dWork), kSlotSegs * sizeof(SegSummary)) == hipSuccess;
     }
+    (void)hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, device);
     if (!ok)
     {
         destroy(s);

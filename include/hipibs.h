@@ -456,7 +456,7 @@ typedef struct hipibs_serve_counters {
  * registered sample's keys (or its KMC database) and kept resident
  * within hbm_budget bytes; idle tables are evicted least requested first
  * (request counts halved every 4096 requests), and only for a sample
- * requested at least as often unless its request has waited 200 ms; up to
+ * requested at least as often unless its request has waited 50 ms; up to
  * max_inflight launches of at most max_windows_per_launch windows run at
  * once, each batching windows of any pending requests.
  * No counterpart in the reference (it has no serving mode). */
