@@ -1713,19 +1713,19 @@ def run_count(args):
     stage_ms = np.mean(np.array([x["stage_ms"] for x in stats]), axis=0)
     launches = st["launches"]
     # algorithmic bytes per stage and step (one counting round into a fresh
-    # table): hashing reads the text and writes 8 B per k-mer; each
-    # bucketing level reads and writes 8 B per k-mer (levels: 9 top bits of
-    # the 2^(log2(buckets) - 9) slices each); the slice counts read 8 B per
-    # k-mer and write the table once
+    # table): hashing reads the text and writes 8 B per k-mer; each pass of
+    # the slice sort reads and writes 8 B per k-mer (passes: 9 top bits of
+    # the 2^(log2(buckets) - 9) slices each, at least one); the slice counts
+    # read 8 B per k-mer and write the table once
     nb = st["table_bytes"] // 128
-    levels = -(-max(0, nb.bit_length() - 1 - 9) // 9)
-    stage_names = ["countHashKernel", "bucketCountKernel + bucketScatterKernel",
+    levels = max(1, -(-max(0, nb.bit_length() - 1 - 9) // 9))
+    stage_names = ["countHashKernel", "groupPassKernel + sliceStartKernel",
                    "countSliceKernel", "countSpillKernel"]
     stage_bytes = [n + 8.0 * st["kmers"], 16.0 * st["kmers"] * levels,
                    8.0 * st["kmers"] + st["table_bytes"], 0.0]
     dom = int(np.argmax(stage_ms))
     alg_bytes = stage_bytes[dom]
-    log(f"count stages (ms): hash {stage_ms[0]:.1f}, group ({levels} levels) {stage_ms[1]:.1f}, "
+    log(f"count stages (ms): hash {stage_ms[0]:.1f}, sort ({levels} passes) {stage_ms[1]:.1f}, "
         f"slices {stage_ms[2]:.1f}, spills {stage_ms[3]:.2f}")
     log(f"count: {ms:.2f} ms/step, kernels {k_ms:.2f} ms in {launches} launches, "
         f"{st['kmers']/1e9:.3f} G k-mers, {st['distinct']/1e6:.1f} M distinct, "

@@ -288,45 +288,60 @@ __global__ __launch_bounds__(kCountBlock) void countKernel(const uint8_t *__rest
 // with LDS atomics before writing the slice back. A k-mer whose probe runs
 // past the slice's last bucket is spilled and counted in the global table
 // afterwards. No atomic reaches memory except the spills'.
+//
+// The sort is least-significant-digit first over digits of `digitBits` top
+// bits (digit c = bits [64 - (c+1) digitBits, 64 - c digitBits) of h, at most
+// kMaxDigits of them): the hashing kernel counts every digit's histogram,
+// then each pass reads every hash once and writes it once, its place the
+// digit's start + the hashes of that digit in earlier tiles (decoupled
+// look-back across tiles) + its rank in its own tile. Passes after the first
+// rank stably, so the result is ordered by all the digits sorted so far.
 constexpr int kSliceLog = 9;
 constexpr int kSliceBuckets = 1 << kSliceLog; // 64 KB of keys and counts
 constexpr int kSliceBlock = 512;
 constexpr int kHllLog = 10; // 1024 registers of a HyperLogLog sketch
 constexpr int kHllSample = 3; // of the k-mers with hash bits 20..22 zero
+constexpr int kMaxDigits = 3;
+constexpr int kMaxDigitBits = 9;
+constexpr int kBins = 1 << kMaxDigitBits;
+constexpr int kHashWaves = kCountBlock / 64;
+constexpr uint32_t kRegion = 64 * kLanePositions; // hash slots of one wave per tile
 
-// Writes h = keyMix(code) of the k-mers at positions skip .. n-k of text to
-// out (append position: counters[8]), counts them in counters[1], and
-// feeds the sampled ones to the sketch `hll` (register = max rank).
-__global__ __launch_bounds__(kCountBlock) void countHashKernel(const uint8_t *__restrict__ text,
-                                                               uint64_t n, uint64_t skip,
-                                                               uint32_t k, uint64_t *out,
-                                                               uint32_t *hll,
-                                                               unsigned long long *counters)
+// Writes h = keyMix(code) of the k-mers at positions skip .. n-k of text.
+// Tile t (kTilePositions positions, the launch's tiles following tileBase)
+// owns slots [(tileBase + t) kTilePositions, +kTilePositions) of out: wave w
+// the region of kRegion slots from w kRegion, filled from its start, its
+// fill in regionCount[(tileBase + t) kHashWaves + w]. Counts the k-mers in
+// counters[1] and counters[8], every digit's histogram in digitHist
+// (kMaxDigits x kBins), and feeds the sampled ones to the sketch `hll`
+// (register = max rank).
+__global__ __launch_bounds__(kCountBlock) void countHashKernel(
+    const uint8_t *__restrict__ text, uint64_t n, uint64_t skip, uint32_t k, uint64_t tileBase,
+    uint32_t digitBits, uint64_t *__restrict__ out, uint32_t *__restrict__ regionCount,
+    uint32_t *hll, unsigned long long *digitHist, unsigned long long *counters)
 {
-    // each wave stages its k-mers' hashes in its own region (a lane emits
-    // at most kLanePositions per tile), counted in a wave-uniform register
-    constexpr int kWaves = kCountBlock / 64;
-    constexpr uint32_t kWaveStage = 64 * kLanePositions;
-    static_assert(kWaves * kWaveStage == kTilePositions, "one stage region per wave");
+    static_assert(kHashWaves * kRegion == kTilePositions, "one region per wave");
     __shared__ __align__(16) uint8_t tile[kTileBytes];
     __shared__ uint32_t reg[1 << kHllLog];
-    __shared__ uint64_t stage[kTilePositions];
-    __shared__ uint32_t waveCount[kWaves];
-    __shared__ unsigned long long stageAt;
+    __shared__ uint32_t hist[kMaxDigits][kBins];
     for (int i = threadIdx.x; i < (1 << kHllLog); i += kCountBlock)
         reg[i] = 0;
+    for (int i = threadIdx.x; i < kMaxDigits * kBins; i += kCountBlock)
+        (&hist[0][0])[i] = 0;
     const uint64_t nPos = n >= k ? n - k + 1 : 0;
     const uint64_t nTiles = (nPos + kTilePositions - 1) / kTilePositions;
     const uint64_t mask = k == 32 ? ~0ull : (1ull << (2 * k)) - 1;
     const uint32_t shift = 2 * (k - 1);
+    const uint32_t dmask = (1u << digitBits) - 1;
     const int lane = laneId();
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    uint64_t *wstage = stage + wv * kWaveStage;
     uint32_t counted = 0;
     for (uint64_t t = blockIdx.x; t < nTiles; t += gridDim.x)
     {
         const uint64_t base = t * kTilePositions;
         const uint32_t bytes = (uint32_t)min<uint64_t>(kTilePositions + k - 1, n - base);
+        const uint64_t region = (tileBase + t) * kHashWaves + wv;
+        uint64_t *wout = out + region * kRegion;
         __syncthreads(); // the previous tile is no longer read
         for (uint32_t i = threadIdx.x * 16; i < bytes; i += kCountBlock * 16)
         {
@@ -339,9 +354,8 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(const uint8_t *__
         }
         __syncthreads();
         // positions [start, start + 32) of the tile, every lane the same
-        // number of steps so that the wave stages its k-mers together
-        // (bytes start + d, read one aligned 4-byte word per 4 steps; the
-        // step count is the same for every lane)
+        // number of steps so that the wave writes its k-mers together
+        // (bytes start + d, read one aligned 4-byte word per 4 steps)
         const uint32_t start = threadIdx.x * kLanePositions;
         const uint32_t steps = kLanePositions + k - 1;
         uint64_t fwd = 0, rc = 0;
@@ -365,7 +379,10 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(const uint8_t *__
                 const uint64_t m = __ballot(emit);
                 if (emit)
                 {
-                    wstage[wcount + maskedRank(m)] = h;
+                    wout[wcount + maskedRank(m)] = h;
+#pragma unroll
+                    for (int dg = 0; dg < kMaxDigits; ++dg)
+                        atomicAdd(&hist[dg][(h >> (64 - (dg + 1) * digitBits)) & dmask], 1u);
                     if (((h >> 20) & ((1u << kHllSample) - 1)) == 0)
                     {
                         const uint32_t rank =
@@ -378,182 +395,249 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(const uint8_t *__
             }
         }
         if (lane == 0)
-            waveCount[wv] = wcount;
-        // the tile's hashes go out with one global append
-        __syncthreads();
-        uint32_t pre[kWaves + 1];
-        pre[0] = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w)
-            pre[w + 1] = pre[w] + waveCount[w];
-        const uint32_t total = pre[kWaves];
-        if (threadIdx.x == 0)
-            stageAt = total ? atomicAdd(counters + 8, (unsigned long long)total) : 0;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < total; i += kCountBlock)
-        {
-            int w = 0;
-#pragma unroll
-            for (int q = 1; q < kWaves; ++q)
-                w += i >= pre[q];
-            out[stageAt + i] = stage[w * kWaveStage + (i - pre[w])];
-        }
+            regionCount[region] = wcount;
     }
     counted = waveSum(counted);
     if (lane == 0 && counted)
+    {
         atomicAdd(counters + 1, (unsigned long long)counted);
+        atomicAdd(counters + 8, (unsigned long long)counted);
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < (1 << kHllLog); i += kCountBlock)
         if (reg[i])
             atomicMax(hll + i, reg[i]);
+    for (int i = threadIdx.x; i < kMaxDigits * kBins; i += kCountBlock)
+        if (const uint32_t c = (&hist[0][0])[i])
+            atomicAdd(digitHist + i, (unsigned long long)c);
 }
 
-// Grouping by slice: most-significant-digit bucketing of the hashes, up to
-// kDigitBits more top bits per level (order inside a bucket is arbitrary).
-// A level's bucket of h is g = h >> (64 - bits), bits the top bits grouped
-// so far. The input of a level is grouped by its first bits - digit bits,
-// so a tile of kBucketTile consecutive elements covers few of the level's
-// buckets: a block counts its tile in an LDS window of kBucketWin buckets
-// from its first element's segment, and only buckets outside the window
-// (tiles crossing several small segments) go straight to global memory.
-constexpr int kDigitBits = 9;
-constexpr int kBucketTile = 1 << 16;
-constexpr int kBucketWin = 1024;
-constexpr int kBucketBlock = 512;
-
-__device__ inline uint64_t windowStart(const uint64_t *in, uint64_t t0, uint32_t bits,
-                                       uint32_t digit)
+// A tile's published count of one digit for the look-back: the flag in the
+// top two bits (0 not yet, 1 the tile's own count, 2 the count of this and
+// every earlier tile), the count below. 32-bit words serve while every
+// digit holds fewer than 2^30 hashes.
+template <class D> struct LookBack
 {
-    return ((in[t0] >> (64 - bits)) >> digit) << digit;
-}
+    static constexpr int kShift = (int)sizeof(D) * 8 - 2;
+    static constexpr D kOwn = D(1) << kShift;
+    static constexpr D kAll = D(2) << kShift;
+    static constexpr D kMask = kOwn - 1;
+};
+constexpr int kGroupBlock = 512;
+constexpr int kGroupWaves = kGroupBlock / 64;
+constexpr int kGroupPer = 16; // hashes per thread
+constexpr int kGroupTile = kGroupBlock * kGroupPer;
+constexpr int kWaveHashes = 64 * kGroupPer;
+static_assert(kGroupTile == kTilePositions && kWaveHashes * 2 == (int)kRegion,
+              "a first-pass tile is one hashing tile, a wave half a region");
+constexpr uint32_t kLookBackSpins = 1u << 20; // polls of one descriptor before giving up
 
-// count[g] += elements of bucket g; tileHist[tile][i] = the tile's count of
-// bucket windowStart + i (its LDS window).
-__global__ __launch_bounds__(kBucketBlock) void bucketCountKernel(const uint64_t *__restrict__ in,
-                                                                  uint64_t n, uint32_t bits,
-                                                                  uint32_t digit,
-                                                                  unsigned long long *count,
-                                                                  uint32_t *tileHist)
+// One pass of the sort: tile t (taken in launch order from tileCounter)
+// places its hashes at digitStart[d] + the hashes of digit d in tiles < t +
+// their rank among its own. FIRST reads the hashing kernel's regions (any
+// order: ranks from LDS atomics); later passes read n dense hashes and rank
+// them stably (a wave ranks 64 hashes at a time from ballots of the digit's
+// bits, the waves of a block cover consecutive stretches). A look-back that
+// waits longer than kLookBackSpins polls sets *failed and takes the missing
+// counts as zero, so that the launch always ends.
+template <bool FIRST, class D>
+__global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
+    const uint64_t *__restrict__ in, const uint32_t *__restrict__ regionCount, uint64_t n,
+    uint32_t shift, uint32_t bits, const uint64_t *__restrict__ digitStart, D *desc,
+    unsigned int *tileCounter, unsigned long long *failed, uint64_t *__restrict__ out)
 {
-    __shared__ uint32_t hist[kBucketWin];
-    const uint64_t t0 = (uint64_t)blockIdx.x * kBucketTile;
-    const uint64_t g0 = windowStart(in, t0, bits, digit);
-    for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
-        hist[i] = 0;
+    using LB = LookBack<D>;
+    __shared__ uint64_t stage[kGroupTile];
+    __shared__ uint32_t cntRaw[kGroupWaves * kBins / 2]; // u16 [wave][bin], or u32 [bin]
+    __shared__ uint16_t localStart[kBins];
+    __shared__ uint64_t gbase[kBins];
+    __shared__ uint32_t waveTot[kGroupWaves];
+    __shared__ uint32_t tileSh, tileCount;
+    uint16_t(*waveCnt)[kBins] = reinterpret_cast<uint16_t(*)[kBins]>(cntRaw);
+    uint32_t *hist = cntRaw;
+    const int tid = threadIdx.x;
+    const int lane = laneId();
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t bins = 1u << bits, dmask = bins - 1;
+    if (tid == 0)
+        tileSh = atomicAdd(tileCounter, 1u);
+    for (int i = tid; i < kGroupWaves * kBins / 2; i += kGroupBlock)
+        cntRaw[i] = 0;
     __syncthreads();
-    const uint64_t end = min<uint64_t>(n, t0 + kBucketTile);
-    for (uint64_t i = t0 + threadIdx.x; i < end; i += kBucketBlock)
+    const uint64_t t = tileSh;
+    // this wave's stretch of the tile
+    const uint64_t *src;
+    uint32_t have;
+    if (FIRST)
     {
-        const uint64_t g = in[i] >> (64 - bits);
-        if (g - g0 < (uint64_t)kBucketWin)
-            atomicAdd(&hist[g - g0], 1u);
+        const uint64_t region = t * (kGroupTile / kRegion) + (w >> 1);
+        const uint32_t fill = regionCount[region];
+        const uint32_t e0 = (uint32_t)(w & 1) * kWaveHashes;
+        src = in + region * kRegion + e0;
+        have = fill > e0 ? min<uint32_t>(fill - e0, kWaveHashes) : 0u;
+    }
+    else
+    {
+        const uint64_t e0 = t * kGroupTile + (uint64_t)w * kWaveHashes;
+        src = in + e0;
+        have = n > e0 ? (uint32_t)min<uint64_t>(n - e0, kWaveHashes) : 0u;
+    }
+    uint64_t v[kGroupPer];
+    uint32_t rk[kGroupPer];
+#pragma unroll
+    for (int r = 0; r < kGroupPer; ++r)
+    {
+        const uint32_t idx = (uint32_t)r * 64 + lane;
+        v[r] = idx < have ? __builtin_nontemporal_load(src + idx) : 0ull;
+    }
+#pragma unroll
+    for (int r = 0; r < kGroupPer; ++r)
+    {
+        const bool ok = (uint32_t)r * 64 + lane < have;
+        const uint32_t d = (uint32_t)(v[r] >> shift) & dmask;
+        if (FIRST)
+        {
+            rk[r] = ok ? atomicAdd(&hist[d], 1u) : 0u;
+        }
         else
-            atomicAdd(count + g, 1ull);
+        {
+            // lanes holding the same digit, from one ballot per digit bit
+            uint64_t peers = __ballot(ok);
+            for (uint32_t b = 0; b < bits; ++b)
+            {
+                const bool set = (d >> b) & 1u;
+                const uint64_t m = __ballot(set);
+                peers &= set ? m : ~m;
+            }
+            const uint32_t below = maskedRank(peers);
+            const uint32_t before = ok ? waveCnt[w][d] : 0u;
+            // every lane has read its digit's count before a leader updates it
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (ok && below == 0)
+                waveCnt[w][d] = (uint16_t)(before + __popcll(peers));
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            rk[r] = before + below;
+        }
     }
     __syncthreads();
-    const uint64_t nBuckets = 1ull << bits;
-    for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
+    // the tile's count of digit `tid`; later passes: each wave's count
+    // becomes the wave's start among the tile's hashes of that digit
+    uint32_t tot = 0;
+    if ((uint32_t)tid < bins)
     {
-        if (hist[i] && g0 + i < nBuckets)
-            atomicAdd(count + g0 + i, (unsigned long long)hist[i]);
-        tileHist[(uint64_t)blockIdx.x * kBucketWin + i] = hist[i];
+        if (FIRST)
+            tot = hist[tid];
+        else
+            for (int ww = 0; ww < kGroupWaves; ++ww)
+            {
+                const uint32_t c = waveCnt[ww][tid];
+                waveCnt[ww][tid] = (uint16_t)tot;
+                tot += c;
+            }
+        __hip_atomic_store(desc + t * kBins + tid, (t == 0 ? LB::kAll : LB::kOwn) | (D)tot,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // exclusive scan of the counts over the digits: each digit's start in stage
+    uint32_t incl = tot;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const uint32_t y = shuffle32(incl, lane >= off ? lane - off : lane);
+        if (lane >= off)
+            incl += y;
+    }
+    if (lane == 63)
+        waveTot[w] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int ww = 0; ww < w; ++ww)
+        before += waveTot[ww];
+    if ((uint32_t)tid < bins)
+        localStart[tid] = (uint16_t)(before + incl - tot);
+    if ((uint32_t)tid == bins - 1)
+        tileCount = before + incl;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kGroupPer; ++r)
+        if ((uint32_t)r * 64 + lane < have)
+        {
+            const uint32_t d = (uint32_t)(v[r] >> shift) & dmask;
+            stage[localStart[d] + (FIRST ? 0u : (uint32_t)waveCnt[w][d]) + rk[r]] = v[r];
+        }
+    // look-back: the hashes of digit `tid` in the tiles before this one
+    if ((uint32_t)tid < bins)
+    {
+        D excl = 0;
+        if (t > 0)
+        {
+            uint64_t p = t - 1;
+            uint32_t spins = 0;
+            for (;;)
+            {
+                const D x = __hip_atomic_load(desc + p * kBins + tid, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+                const D flag = x >> LB::kShift;
+                if (flag == 0)
+                {
+                    if (++spins >= kLookBackSpins)
+                    {
+                        atomicExch(failed, 1ull);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += x & LB::kMask;
+                if (flag == 2 || p == 0)
+                    break;
+                --p;
+                spins = 0;
+            }
+            __hip_atomic_store(desc + t * kBins + tid, LB::kAll | ((excl + (D)tot) & LB::kMask),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        gbase[tid] = digitStart[tid] + (uint64_t)excl;
+    }
+    __syncthreads();
+    // the staged hashes of one digit are adjacent: so are their stores
+    const uint32_t cnt = tileCount;
+    for (uint32_t e = tid; e < cnt; e += kGroupBlock)
+    {
+        const uint64_t x = stage[e];
+        const uint32_t d = (uint32_t)(x >> shift) & dmask;
+        out[gbase[d] + (e - localStart[d])] = x;
     }
 }
 
-// Writes every element at its bucket's cursor (cursor[g] starts at the
-// bucket's first output index). Each of the tile's buckets (counted by
-// bucketCountKernel into tileHist) reserves its run with one global add;
-// then sub-chunks of
-// kStageElems elements are ordered by bucket in LDS (a counting sort) and
-// written out, so that consecutive threads store consecutive addresses of
-// a run instead of single elements scattered over all of the tile's runs.
-constexpr int kStageElems = 8192;
-__global__ __launch_bounds__(kBucketBlock) void bucketScatterKernel(
-    const uint64_t *__restrict__ in, uint64_t n, uint32_t bits, uint32_t digit,
-    const uint32_t *__restrict__ tileHist, unsigned long long *cursor, uint64_t *__restrict__ out)
+// start[s] = the first index of the sorted hashes whose top sbits bits are
+// >= s (s = 0 .. nSlices), by binary search.
+__global__ __launch_bounds__(256) void sliceStartKernel(const uint64_t *__restrict__ hs, uint64_t n,
+                                                        uint32_t sbits, uint64_t nSlices,
+                                                        uint64_t *start)
 {
-    __shared__ uint32_t hist[kBucketWin];  // tile counts, then per sub-chunk counts
-    __shared__ uint32_t start[kBucketWin]; // a sub-chunk's bucket starts in stage
-    __shared__ unsigned long long base[kBucketWin]; // next output index per bucket
-    __shared__ uint64_t stage[kStageElems];
-    const uint64_t t0 = (uint64_t)blockIdx.x * kBucketTile;
-    const uint64_t g0 = windowStart(in, t0, bits, digit);
-    const uint64_t end = min<uint64_t>(n, t0 + kBucketTile);
-    for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nSlices)
+        return;
+    if (sbits == 0 || s == nSlices)
     {
-        const uint32_t c = tileHist[(uint64_t)blockIdx.x * kBucketWin + i];
-        if (c)
-            base[i] = atomicAdd(cursor + g0 + i, (unsigned long long)c);
-        hist[i] = 0;
+        start[s] = s == 0 ? 0 : n;
+        return;
     }
-    __syncthreads();
-    constexpr int kPer = kStageElems / kBucketBlock;
-    for (uint64_t c0 = t0; c0 < end; c0 += kStageElems)
+    uint64_t lo = 0, hi = n;
+    while (lo < hi)
     {
-        const uint32_t m = (uint32_t)min<uint64_t>(kStageElems, end - c0);
-        uint64_t v[kPer];
-        uint32_t r[kPer];
-        // ranks within the sub-chunk's buckets; outside the window: direct
-#pragma unroll
-        for (int j = 0; j < kPer; ++j)
-        {
-            const uint32_t e = threadIdx.x + (uint32_t)j * kBucketBlock;
-            r[j] = 0xffffffffu;
-            if (e < m)
-            {
-                v[j] = in[c0 + e];
-                const uint64_t g = v[j] >> (64 - bits);
-                if (g - g0 < (uint64_t)kBucketWin)
-                    r[j] = atomicAdd(&hist[g - g0], 1u);
-                else
-                    out[atomicAdd(cursor + g, 1ull)] = v[j];
-            }
-        }
-        __syncthreads();
-        // exclusive scan of the sub-chunk's counts (kBucketWin = 2 x block)
-        {
-            const int a = 2 * threadIdx.x;
-            const uint32_t x0 = hist[a], x1 = hist[a + 1];
-            uint32_t sum = x0 + x1;
-            // inclusive scan of the pair sums across the block via LDS
-            start[threadIdx.x] = sum;
-            __syncthreads();
-            for (int off = 1; off < kBucketBlock; off <<= 1)
-            {
-                const uint32_t add = threadIdx.x >= off ? start[threadIdx.x - off] : 0u;
-                __syncthreads();
-                start[threadIdx.x] += add;
-                __syncthreads();
-            }
-            const uint32_t incl = start[threadIdx.x];
-            __syncthreads();
-            start[a] = incl - sum;
-            start[a + 1] = incl - sum + x0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < kPer; ++j)
-            if (r[j] != 0xffffffffu)
-                stage[start[(v[j] >> (64 - bits)) - g0] + r[j]] = v[j];
-        __syncthreads();
-        // staged elements of one bucket are adjacent: their stores are too
-        const uint32_t placed = start[kBucketWin - 1] + hist[kBucketWin - 1];
-        for (uint32_t e = threadIdx.x; e < placed; e += kBucketBlock)
-        {
-            const uint64_t x = stage[e];
-            const uint32_t b = (uint32_t)((x >> (64 - bits)) - g0);
-            out[base[b] + (e - start[b])] = x;
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < kBucketWin; i += kBucketBlock)
-        {
-            base[i] += hist[i];
-            hist[i] = 0;
-        }
-        __syncthreads();
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if ((hs[mid] >> (64 - sbits)) < s)
+            lo = mid + 1;
+        else
+            hi = mid;
     }
+    start[s] = lo;
 }
-
 // One workgroup per slice: loads the slice's buckets into LDS (or EMPTY
 // keys and zero counts when the table is fresh), counts the slice's k-mers
 // there (the key by compare-and-swap into the first EMPTY slot from its
@@ -881,7 +965,8 @@ struct hipibs_counter : ReadSink
     unsigned long long *dCounters = nullptr; // [0] distinct, [1] counted, [2] full, [3..5] filter,
                                              // [6] count of A^32 T^32 (k = 64), [7] spilled
                                              // and [8] hashed k-mers of the open flush,
-                                             // [9] count of T^32 A^32 (k = 64)
+                                             // [9] count of T^32 A^32 (k = 64),
+                                             // [10] a sort pass's look-back timed out
     unsigned long long *hCounters = nullptr; // pinned copy
     // sorted-slice counting (k <= 32): the hashed k-mers of the open flush
     // (dHash[0]; grouped by slice through both buffers, the other one then
@@ -889,12 +974,23 @@ struct hipibs_counter : ReadSink
     uint64_t *dHash[2] = {nullptr, nullptr};
     uint64_t hashCap = 0;    // entries of each buffer
     uint64_t hashBudget = 0; // most entries one flush holds
-    uint64_t nHash = 0;
-    uint32_t *dHll = nullptr; // the flush's distinct-count sketch
+    uint64_t nSlots = 0;     // slots the open flush's hashing tiles own in dHash[0]
+    uint64_t nHash = 0;      // hashes of the open flush (after a sync)
+    uint32_t *dRegion = nullptr;           // the fill of every hashing region
+    uint32_t *dHll = nullptr;              // the flush's distinct-count sketch
+    unsigned long long *dDigit = nullptr;  // the flush's digit histograms
+    uint64_t *dDigitStart = nullptr;       // their exclusive sums (kMaxDigits x (kBins + 1))
+    std::vector<unsigned long long> hDigit;
+    std::vector<uint64_t> hDigitStart;
+    void *dDesc = nullptr; // the sort's look-back words
+    uint64_t descBytes = 0;
+    unsigned int *dTileCounter = nullptr;
+    // digit width of the slice sort (environment HIPIBS_COUNT_DIGIT_BITS,
+    // 1..9: narrower digits take more passes, which the tests use)
+    uint32_t digitBits = kMaxDigitBits;
+    bool wideLookBack = false; // 64-bit look-back words always (HIPIBS_COUNT_LOOKBACK_WIDE=1)
     uint64_t *dStart = nullptr;
-    bool tableUnfilled = false;    // dTab allocated but not yet written
-    uint32_t *dTileHist = nullptr; // a bucketing level's per-tile window counts
-    uint64_t tileHistCap = 0;      // tiles
+    bool tableUnfilled = false; // dTab allocated but not yet written
     uint64_t startCap = 0;
     char *hText[2] = {nullptr, nullptr};     // pinned chunks (host text only)
     uint8_t *dText = nullptr;
@@ -1034,39 +1130,53 @@ struct hipibs_counter : ReadSink
         int rc = 0;
         if ((rc = budget()))
             return rc;
-        if (positions > hashBudget)
+        // the launch's tiles own whole slots ranges, the skipped positions' too
+        const uint64_t tiles = (n - k + 1 + kTilePositions - 1) / kTilePositions;
+        const uint64_t slots = tiles * kTilePositions;
+        if (slots > hashBudget)
             return setError(HIPIBS_E_NOMEM, "k-mer counter: a text piece exceeds the flush budget");
-        if (nHash + positions > hashBudget && (rc = flushSorted()))
+        if (nSlots + slots > hashBudget && (rc = flushSorted()))
             return rc;
-        if ((rc = hashRoom(nHash + positions)))
+        if ((rc = hashRoom(nSlots + slots)))
             return rc;
         if (!dHll)
         {
             HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dHll), (4u << kHllLog)));
             HIPIBS_CHECK(hipMemsetAsync(dHll, 0, 4u << kHllLog, stream));
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dDigit),
+                                   kMaxDigits * kBins * sizeof(unsigned long long)));
+            HIPIBS_CHECK(hipMemsetAsync(dDigit, 0, kMaxDigits * kBins * sizeof(unsigned long long),
+                                        stream));
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dRegion),
+                                   (hashBudget / kRegion + 1) * sizeof(uint32_t)));
         }
+        const uint64_t tileBase = nSlots / kTilePositions;
         if ((rc = timed(0, [&] {
                  countHashKernel<<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0,
-                                   stream>>>(text, n, skip, k, dHash[0], dHll, dCounters);
+                                   stream>>>(text, n, skip, k, tileBase, digitBits, dHash[0],
+                                             dRegion, dHll, dDigit, dCounters);
              })))
             return rc;
         HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 9 * sizeof(unsigned long long),
                                     hipMemcpyDeviceToHost, stream));
-        nHash += positions; // an upper bound until the next sync
+        nSlots += slots;
         return 0;
     }
 
-    // The most hashed k-mers one flush holds: 16 read chunks (2 Gi k-mers
-    // at the default chunk), both buffers within a quarter of the free
-    // memory, at least one chunk.
+    // The most hash slots one flush holds: 16 read chunks' hashing tiles
+    // (2 Gi k-mers at the default chunk), both buffers within a quarter of
+    // the free memory, at least one chunk's tiles.
     int budget()
     {
         if (hashBudget)
             return 0;
         size_t freeB = 0, totalB = 0;
         HIPIBS_CHECK(hipMemGetInfo(&freeB, &totalB));
-        hashBudget = std::min<uint64_t>(16 * chunkBytes, freeB / 4 / 16);
-        hashBudget = std::max<uint64_t>(hashBudget, chunkBytes);
+        const uint64_t chunkSlots =
+            (chunkBytes + kTilePositions - 1) / kTilePositions * kTilePositions;
+        hashBudget = std::min<uint64_t>(16 * chunkSlots, freeB / 4 / 16 / kTilePositions *
+                                                             kTilePositions);
+        hashBudget = std::max<uint64_t>(hashBudget, chunkSlots);
         return 0;
     }
 
@@ -1091,8 +1201,8 @@ struct hipibs_counter : ReadSink
                     return setError(HIPIBS_E_NOMEM, "k-mer counter: no device memory for " +
                                                         std::to_string(want) + " hashed k-mers");
                 }
-            if (nHash)
-                HIPIBS_CHECK(hipMemcpyAsync(nb[0], dHash[0], nHash * 8, hipMemcpyDeviceToDevice,
+            if (nSlots)
+                HIPIBS_CHECK(hipMemcpyAsync(nb[0], dHash[0], nSlots * 8, hipMemcpyDeviceToDevice,
                                             stream));
             HIPIBS_CHECK(hipStreamSynchronize(stream));
             for (int i = 0; i < 2; ++i)
@@ -1138,7 +1248,10 @@ struct hipibs_counter : ReadSink
             return rc;
         nHash = hCounters[8];
         if (!nHash)
+        {
+            nSlots = 0; // tiles without k-mers
             return 0;
+        }
         const double est = sketchEstimate();
         if (est < 0)
             return setError(HIPIBS_E_HIP, "k-mer counter: reading the sketch failed");
@@ -1148,6 +1261,11 @@ struct hipibs_counter : ReadSink
         const int lg = 63 - __builtin_clzll(nBuckets); // a power of two
         const uint32_t sbits = lg > kSliceLog ? (uint32_t)(lg - kSliceLog) : 0u;
         const uint64_t nSlices = sbits ? (1ull << sbits) : 1;
+        const uint32_t passes = std::max<uint32_t>(1, (sbits + digitBits - 1) / digitBits);
+        if (passes > (uint32_t)kMaxDigits)
+            return setError(HIPIBS_E_NOMEM, "k-mer counter: the table needs more than " +
+                                                std::to_string(kMaxDigits * digitBits) +
+                                                " bits of slice index");
         if (startCap < nSlices + 1)
         {
             if (dStart)
@@ -1156,77 +1274,102 @@ struct hipibs_counter : ReadSink
             HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dStart), (nSlices + 1) * 8));
             startCap = nSlices + 1;
         }
-        const uint64_t nTiles = (nHash + kBucketTile - 1) / kBucketTile;
-        if (sbits && tileHistCap < nTiles)
+        // every digit's start from its histogram
+        const uint32_t bins = 1u << digitBits;
+        hDigit.resize(kMaxDigits * kBins);
+        hDigitStart.assign(kMaxDigits * (kBins + 1), 0);
+        HIPIBS_CHECK(hipMemcpy(hDigit.data(), dDigit, hDigit.size() * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost));
+        uint64_t widest = 0;
+        for (int c = 0; c < kMaxDigits; ++c)
         {
-            if (dTileHist)
-                HIPIBS_CHECK(hipFree(dTileHist));
-            dTileHist = nullptr;
-            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileHist),
-                                   nTiles * kBucketWin * sizeof(uint32_t)));
-            tileHistCap = nTiles;
-        }
-        // group the hashes by slice: levels of up to kDigitBits top bits,
-        // ping-ponging between the two buffers; the last level's bucket
-        // offsets are the slices' starts
-        uint64_t *src = dHash[0], *dst = dHash[1];
-        // sized for the last level: the resizes below never reallocate while
-        // a copy from it may be in flight
-        std::vector<uint64_t> offs(nSlices + 1, 0);
-        offs[nSlices] = nHash;
-        for (uint32_t bits = 0; bits < sbits;)
-        {
-            const uint32_t digit = std::min<uint32_t>(kDigitBits, sbits - bits);
-            bits += digit;
-            const uint64_t nb = 1ull << bits;
-            const uint32_t tiles = (uint32_t)((nHash + kBucketTile - 1) / kBucketTile);
-            HIPIBS_CHECK(hipMemsetAsync(dStart, 0, nb * 8, stream));
-            if ((rc = timed(1, [&] {
-                     bucketCountKernel<<<tiles, kBucketBlock, 0, stream>>>(
-                         src, nHash, bits, digit, reinterpret_cast<unsigned long long *>(dStart),
-                         dTileHist);
-                 })))
-                return rc;
-            offs.resize(nb + 1);
-            HIPIBS_CHECK(hipMemcpyAsync(offs.data(), dStart, nb * 8, hipMemcpyDeviceToHost, stream));
-            HIPIBS_CHECK(hipStreamSynchronize(stream));
-            if ((rc = collectTimes()))
-                return rc;
             uint64_t acc = 0;
-            for (uint64_t g = 0; g < nb; ++g)
+            for (uint32_t d = 0; d < bins; ++d)
             {
-                const uint64_t c = offs[g];
-                offs[g] = acc;
-                acc += c;
+                hDigitStart[c * (kBins + 1) + d] = acc;
+                acc += hDigit[c * kBins + d];
+                widest = std::max<uint64_t>(widest, hDigit[c * kBins + d]);
             }
-            offs[nb] = acc;
+            hDigitStart[c * (kBins + 1) + bins] = acc;
             if (acc != nHash)
                 return setError(HIPIBS_E_HIP, "k-mer counter: slice grouping lost k-mers");
-            HIPIBS_CHECK(hipMemcpyAsync(dStart, offs.data(), nb * 8, hipMemcpyHostToDevice, stream));
+        }
+        if (!dDigitStart)
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dDigitStart),
+                                   hDigitStart.size() * sizeof(uint64_t)));
+        HIPIBS_CHECK(hipMemcpy(dDigitStart, hDigitStart.data(),
+                               hDigitStart.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+        // look-back words: 32 bits while no digit holds 2^30 hashes
+        const bool narrow = widest < (1ull << 30) && !wideLookBack;
+        const uint64_t tilesFirst = nSlots / kTilePositions;
+        const uint64_t tilesDense = (nHash + kGroupTile - 1) / kGroupTile;
+        const uint64_t tilesMax = std::max(tilesFirst, tilesDense);
+        const uint64_t wordBytes = narrow ? 4 : 8;
+        if (descBytes < tilesMax * kBins * wordBytes)
+        {
+            if (dDesc)
+                HIPIBS_CHECK(hipFree(dDesc));
+            dDesc = nullptr;
+            descBytes = 0;
+            HIPIBS_CHECK(hipMalloc(&dDesc, tilesMax * kBins * 8));
+            descBytes = tilesMax * kBins * 8;
+        }
+        if (!dTileCounter)
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileCounter), 16));
+        // the passes, least significant digit first, ping-ponging between
+        // the two buffers; then the slices' starts
+        uint64_t *src = dHash[0], *dst = dHash[1];
+        for (uint32_t q = 0; q < passes; ++q)
+        {
+            const uint32_t c = passes - 1 - q;
+            const uint32_t shift = 64 - (c + 1) * digitBits;
+            const uint64_t tiles = q == 0 ? tilesFirst : tilesDense;
+            const uint64_t *starts = dDigitStart + c * (kBins + 1);
+            HIPIBS_CHECK(hipMemsetAsync(dDesc, 0, tiles * kBins * wordBytes, stream));
+            HIPIBS_CHECK(hipMemsetAsync(dTileCounter, 0, 4, stream));
+            auto *failed = dCounters + 10;
             if ((rc = timed(1, [&] {
-                     bucketScatterKernel<<<tiles, kBucketBlock, 0, stream>>>(
-                         src, nHash, bits, digit, dTileHist,
-                         reinterpret_cast<unsigned long long *>(dStart), dst);
+                     if (q == 0 && narrow)
+                         groupPassKernel<true, uint32_t><<<(uint32_t)tiles, kGroupBlock, 0, stream>>>(
+                             src, dRegion, nHash, shift, digitBits, starts,
+                             static_cast<uint32_t *>(dDesc), dTileCounter, failed, dst);
+                     else if (q == 0)
+                         groupPassKernel<true, uint64_t><<<(uint32_t)tiles, kGroupBlock, 0, stream>>>(
+                             src, dRegion, nHash, shift, digitBits, starts,
+                             static_cast<uint64_t *>(dDesc), dTileCounter, failed, dst);
+                     else if (narrow)
+                         groupPassKernel<false, uint32_t><<<(uint32_t)tiles, kGroupBlock, 0, stream>>>(
+                             src, dRegion, nHash, shift, digitBits, starts,
+                             static_cast<uint32_t *>(dDesc), dTileCounter, failed, dst);
+                     else
+                         groupPassKernel<false, uint64_t><<<(uint32_t)tiles, kGroupBlock, 0, stream>>>(
+                             src, dRegion, nHash, shift, digitBits, starts,
+                             static_cast<uint64_t *>(dDesc), dTileCounter, failed, dst);
                  })))
                 return rc;
             std::swap(src, dst);
         }
+        if ((rc = timed(1, [&] {
+                 sliceStartKernel<<<(uint32_t)((nSlices + 1 + 255) / 256), 256, 0, stream>>>(
+                     src, nHash, sbits, nSlices, dStart);
+             })))
+            return rc;
         // the grouped hashes are in src; dst takes the spills
         const uint64_t *sorted = src;
         uint64_t *other = dst;
-        HIPIBS_CHECK(hipMemcpyAsync(dStart, offs.data(), (nSlices + 1) * 8, hipMemcpyHostToDevice,
-                                    stream));
         if ((rc = timed(2, [&] {
                  countSliceKernel<<<(uint32_t)nSlices, kSliceBlock, 0, stream>>>(
                      sorted, dStart, dTab, nBuckets, distinct == 0, tableUnfilled, other,
                      dCounters);
              })))
             return rc;
-        HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 9 * sizeof(unsigned long long),
+        HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 11 * sizeof(unsigned long long),
                                     hipMemcpyDeviceToHost, stream));
         HIPIBS_CHECK(hipStreamSynchronize(stream));
         if ((rc = collectTimes()))
             return rc;
+        if (hCounters[10])
+            return setError(HIPIBS_E_HIP, "k-mer counter: a sort pass's look-back timed out");
         tableUnfilled = false; // every slice written
         distinct = hCounters[0];
         const uint64_t spills = hCounters[7];
@@ -1252,9 +1395,12 @@ struct hipibs_counter : ReadSink
         spilled += spills;
         HIPIBS_CHECK(hipMemsetAsync(dCounters + 7, 0, 2 * sizeof(unsigned long long), stream));
         HIPIBS_CHECK(hipMemsetAsync(dHll, 0, 4u << kHllLog, stream));
+        HIPIBS_CHECK(hipMemsetAsync(dDigit, 0, kMaxDigits * kBins * sizeof(unsigned long long),
+                                    stream));
         HIPIBS_CHECK(hipStreamSynchronize(stream));
         hCounters[7] = hCounters[8] = 0;
         nHash = 0;
+        nSlots = 0;
         flushes += 1;
         return 0;
     }
@@ -1289,7 +1435,10 @@ struct hipibs_counter : ReadSink
             if (!rc)
                 rc = budget();
             if (!rc)
-                rc = hashRoom(std::min<uint64_t>(hashBudget, nHash + (n - k + 1)));
+                rc = hashRoom(std::min<uint64_t>(
+                    hashBudget, nSlots + (n - k + 1) +
+                                    (n / std::max<uint64_t>(chunkBytes - 64, 1) + 2) *
+                                        kTilePositions));
             if (rc)
                 return rc;
         }
@@ -1511,6 +1660,10 @@ extern "C" int hipibs_counter_create(uint32_t k, int device, uint64_t expected_k
     c->chunkBytes = chunk_bytes;
     if (const char *sc = std::getenv("HIPIBS_COUNT_SKETCH_SCALE"))
         c->sketchScale = std::min(100.0, std::max(0.0, std::atof(sc)));
+    if (const char *db = std::getenv("HIPIBS_COUNT_DIGIT_BITS"))
+        c->digitBits = (uint32_t)std::min(kMaxDigitBits, std::max(1, std::atoi(db)));
+    if (const char *lw = std::getenv("HIPIBS_COUNT_LOOKBACK_WIDE"))
+        c->wideLookBack = std::atoi(lw) != 0;
     auto fail = [c](int rc)
     {
         hipibs_counter_free(c);
@@ -1759,8 +1912,16 @@ extern "C" void hipibs_counter_free(hipibs_counter *c)
         (void)hipFree(c->dHll);
     if (c->dStart)
         (void)hipFree(c->dStart);
-    if (c->dTileHist)
-        (void)hipFree(c->dTileHist);
+    if (c->dRegion)
+        (void)hipFree(c->dRegion);
+    if (c->dDigit)
+        (void)hipFree(c->dDigit);
+    if (c->dDigitStart)
+        (void)hipFree(c->dDigitStart);
+    if (c->dDesc)
+        (void)hipFree(c->dDesc);
+    if (c->dTileCounter)
+        (void)hipFree(c->dTileCounter);
     for (char *h : c->hText)
         if (h)
             (void)hipHostFree(h);

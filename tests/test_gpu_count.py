@@ -230,6 +230,68 @@ def test_spilled_kmers_counted_in_global_table(small_reads, monkeypatch, chunk):
     c.free()
 
 
+def slice_passes(table_bytes, digit_bits):
+    """Passes of the counter's slice sort for a table of table_bytes (128 B
+    buckets, 512-bucket slices, digits of digit_bits top bits)."""
+    sbits = max(0, (table_bytes // 128).bit_length() - 1 - 9)
+    return max(1, -(-sbits // digit_bits))
+
+
+@pytest.mark.parametrize("bits,wide", [(9, "0"), (9, "1"), (4, "0"), (5, "1")])
+def test_slice_sort_passes_and_lookback_words(small_reads, monkeypatch, bits, wide):
+    """The slice sort over two or three digits (every pass after the first
+    ranks stably) with 32- or 64-bit look-back words. The table is sized for
+    every hashed k-mer (1.5 M, 2^19 buckets: 10 slice bits); with 512 values
+    per digit a tile of the second pass spans several values of the first
+    pass's digit, so a rank that is not stable would misplace k-mers. Keys
+    and counts equal the oracle's."""
+    _, reads = small_reads
+    r = reads[2][:12000]
+    text = b"\n".join(corpus.to_text(x) for x in r)
+    want_k, want_c = corpus.count_kmers(r, 25, ci=1, cs=ALL)
+    monkeypatch.setenv("HIPIBS_COUNT_DIGIT_BITS", str(bits))
+    monkeypatch.setenv("HIPIBS_COUNT_LOOKBACK_WIDE", wide)
+    monkeypatch.setenv("HIPIBS_COUNT_SKETCH_SCALE", "40")
+    c = hipibs.KmerCounter(25)
+    for v in ("HIPIBS_COUNT_DIGIT_BITS", "HIPIBS_COUNT_LOOKBACK_WIDE", "HIPIBS_COUNT_SKETCH_SCALE"):
+        monkeypatch.delenv(v)
+    c.add_text(text)
+    st = c.stats()
+    passes = slice_passes(st["table_bytes"], bits)
+    note(f"slice sort: digit bits {bits}, wide look-back {wide}, passes {passes}: {st}")
+    assert passes >= 2
+    keys, counts, _ = c.keys(1, ALL, ALL)
+    keys, counts = sorted_pairs(keys, counts)
+    np.testing.assert_array_equal(keys, want_k)
+    np.testing.assert_array_equal(counts, want_c)
+    c.free()
+
+
+def test_repetitive_reads_fill_few_digits(monkeypatch):
+    """Reads drawn from 40 short sequences: most digit values are empty,
+    a few hold every hash, and tiles of the later passes hold runs of one
+    value; with 3-bit digits the sort takes three passes."""
+    seqs = [corpus.random_bases(0x5EED, 70 + i, 150) for i in range(40)]
+    pick = corpus.splitmix(0x5EED, 200, 30000) % np.uint64(40)
+    r = np.stack([seqs[int(i)] for i in pick])
+    text = b"\n".join(corpus.to_text(x) for x in r)
+    want_k, want_c = corpus.count_kmers(r, 21, ci=1, cs=ALL)
+    monkeypatch.setenv("HIPIBS_COUNT_DIGIT_BITS", "3")
+    monkeypatch.setenv("HIPIBS_COUNT_SKETCH_SCALE", "100")
+    c = hipibs.KmerCounter(21)
+    monkeypatch.delenv("HIPIBS_COUNT_DIGIT_BITS")
+    monkeypatch.delenv("HIPIBS_COUNT_SKETCH_SCALE")
+    c.add_text(text)
+    st = c.stats()
+    note(f"repetitive reads: passes {slice_passes(st['table_bytes'], 3)}: {st}")
+    assert slice_passes(st["table_bytes"], 3) == 3
+    keys, counts, _ = c.keys(1, ALL, ALL)
+    keys, counts = sorted_pairs(keys, counts)
+    np.testing.assert_array_equal(keys, want_k)
+    np.testing.assert_array_equal(counts, want_c)
+    c.free()
+
+
 def test_hash_buffers_reused_across_counters(small_reads):
     """Counters alive together each keep their own buffers; a freed
     counter's buffers serve the next counter, larger or smaller, and
