@@ -304,6 +304,8 @@ constexpr int kHllSample = 3; // of the k-mers with hash bits 20..22 zero
 constexpr int kMaxDigits = 3;
 constexpr int kMaxDigitBits = 9;
 constexpr int kBins = 1 << kMaxDigitBits;
+constexpr int kTileDigits = 2; // per-tile counts of digits 1 and 2 (a first pass's digits)
+constexpr int kRanges = 8;     // the last pass's input ranges, one per XCD
 constexpr int kHashWaves = kCountBlock / 64;
 constexpr uint32_t kRegion = 64 * kLanePositions; // hash slots of one wave per tile
 
@@ -312,22 +314,34 @@ constexpr uint32_t kRegion = 64 * kLanePositions; // hash slots of one wave per 
 // owns slots [(tileBase + t) kTilePositions, +kTilePositions) of out: wave w
 // the region of kRegion slots from w kRegion, filled from its start, its
 // fill in regionCount[(tileBase + t) kHashWaves + w]. Counts the k-mers in
-// counters[1] and counters[8], every digit's histogram in digitHist
-// (kMaxDigits x kBins), and feeds the sampled ones to the sketch `hll`
-// (register = max rank).
+// counters[1] and counters[8]; the histograms of digits 1 and 2 in
+// digitHist (kMaxDigits x kBins, row 0 unused), digit 0's by range in
+// rangeHist (kRanges x kBins: the range is the top rangeBits bits of digit
+// 1, for the last pass of the sort); the tile's own counts of digits 1 and 2
+// in tileHist (kTileDigits x kBins u16 per tile: the first pass of a sort
+// over two or three digits reads them); and feeds the sampled ones to the
+// sketch `hll` (register = max rank).
 __global__ __launch_bounds__(kCountBlock) void countHashKernel(
     const uint8_t *__restrict__ text, uint64_t n, uint64_t skip, uint32_t k, uint64_t tileBase,
     uint32_t digitBits, uint64_t *__restrict__ out, uint32_t *__restrict__ regionCount,
-    uint32_t *hll, unsigned long long *digitHist, unsigned long long *counters)
+    uint16_t *__restrict__ tileHist, uint32_t rangeBits, uint32_t *hll,
+    unsigned long long *digitHist, unsigned long long *rangeHist, unsigned long long *counters)
 {
     static_assert(kHashWaves * kRegion == kTilePositions, "one region per wave");
+    static_assert(kTileDigits == kMaxDigits - 1, "per-tile counts of the digits below the top");
     __shared__ __align__(16) uint8_t tile[kTileBytes];
     __shared__ uint32_t reg[1 << kHllLog];
-    __shared__ uint32_t hist[kMaxDigits][kBins];
+    __shared__ uint32_t hist[kTileDigits][kBins];   // the block's, digits 1 and 2
+    __shared__ uint32_t rng[kRanges][kBins];        // the block's, digit 0 by range
+    __shared__ uint32_t tileCnt[kTileDigits][kBins]; // the current tile's, digits 1 and 2
     for (int i = threadIdx.x; i < (1 << kHllLog); i += kCountBlock)
         reg[i] = 0;
-    for (int i = threadIdx.x; i < kMaxDigits * kBins; i += kCountBlock)
+    for (int i = threadIdx.x; i < kTileDigits * kBins; i += kCountBlock)
         (&hist[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < kRanges * kBins; i += kCountBlock)
+        (&rng[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < kTileDigits * kBins; i += kCountBlock)
+        (&tileCnt[0][0])[i] = 0;
     const uint64_t nPos = n >= k ? n - k + 1 : 0;
     const uint64_t nTiles = (nPos + kTilePositions - 1) / kTilePositions;
     const uint64_t mask = k == 32 ? ~0ull : (1ull << (2 * k)) - 1;
@@ -380,9 +394,12 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(
                 if (emit)
                 {
                     wout[wcount + maskedRank(m)] = h;
+                    atomicAdd(&rng[(h >> (64 - digitBits - rangeBits)) & ((1u << rangeBits) - 1)]
+                                  [(h >> (64 - digitBits)) & dmask],
+                              1u);
 #pragma unroll
-                    for (int dg = 0; dg < kMaxDigits; ++dg)
-                        atomicAdd(&hist[dg][(h >> (64 - (dg + 1) * digitBits)) & dmask], 1u);
+                    for (int dg = 1; dg < kMaxDigits; ++dg)
+                        atomicAdd(&tileCnt[dg - 1][(h >> (64 - (dg + 1) * digitBits)) & dmask], 1u);
                     if (((h >> 20) & ((1u << kHllSample) - 1)) == 0)
                     {
                         const uint32_t rank =
@@ -396,6 +413,15 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(
         }
         if (lane == 0)
             regionCount[region] = wcount;
+        // the tile's counts of digits 1 and 2 out, added to the block's
+        __syncthreads();
+        for (int i = threadIdx.x; i < kTileDigits * kBins; i += kCountBlock)
+        {
+            const uint32_t c = (&tileCnt[0][0])[i];
+            tileHist[(tileBase + t) * (kTileDigits * kBins) + i] = (uint16_t)c;
+            (&hist[0][0])[i] += c;
+            (&tileCnt[0][0])[i] = 0;
+        }
     }
     counted = waveSum(counted);
     if (lane == 0 && counted)
@@ -407,9 +433,12 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(
     for (int i = threadIdx.x; i < (1 << kHllLog); i += kCountBlock)
         if (reg[i])
             atomicMax(hll + i, reg[i]);
-    for (int i = threadIdx.x; i < kMaxDigits * kBins; i += kCountBlock)
+    for (int i = threadIdx.x; i < kTileDigits * kBins; i += kCountBlock)
         if (const uint32_t c = (&hist[0][0])[i])
-            atomicAdd(digitHist + i, (unsigned long long)c);
+            atomicAdd(digitHist + kBins + i, (unsigned long long)c);
+    for (int i = threadIdx.x; i < kRanges * kBins; i += kCountBlock)
+        if (const uint32_t c = (&rng[0][0])[i])
+            atomicAdd(rangeHist + i, (unsigned long long)c);
 }
 
 // A tile's published count of one digit for the look-back: the flag in the
@@ -431,44 +460,101 @@ constexpr int kWaveHashes = 64 * kGroupPer;
 static_assert(kGroupTile == kTilePositions && kWaveHashes * 2 == (int)kRegion,
               "a first-pass tile is one hashing tile, a wave half a region");
 constexpr uint32_t kLookBackSpins = 1u << 20; // polls of one descriptor before giving up
+constexpr int kLookAhead = 8;                 // earlier tiles' words read per look-back step
 
-// One pass of the sort: tile t (taken in launch order from tileCounter)
-// places its hashes at digitStart[d] + the hashes of digit d in tiles < t +
-// their rank among its own. FIRST reads the hashing kernel's regions (any
-// order: ranks from LDS atomics); later passes read n dense hashes and rank
-// them stably (a wave ranks 64 hashes at a time from ballots of the digit's
-// bits, the waves of a block cover consecutive stretches). A look-back that
-// waits longer than kLookBackSpins polls sets *failed and takes the missing
-// counts as zero, so that the launch always ends.
-template <bool FIRST, class D>
+// One pass of the sort by the digit at `shift` (bits wide): tile t places
+// its hashes at the digit's start + the hashes of that digit in earlier
+// tiles + their rank among its own.
+//   kPassOffsets: the first pass, reading the hashing kernel's regions; each
+//     tile's places come from tileOffset (the per-tile counts the hashing
+//     kernel took, summed over tiles); ranks from LDS atomics.
+//   kPassLookBack: the same with the earlier tiles' counts found by a
+//     decoupled look-back (tiles taken in launch order from tileCounter) --
+//     the first pass when there is one digit only.
+//   kPassStable: a later pass over n dense hashes, already ordered by the
+//     digits below this one (prevShift .. shift), with look-back.
+//   kPassRanged: kPassStable for the last pass, its input cut into kRanges
+//     ranges by the top bits of the digit below (rangeMeta: kRanges + 1
+//     input starts, kRanges + 1 first tiles, kRanges x kBins output bases
+//     from the hashing kernel's counts by range): blocks of class
+//     blockIdx % kRanges take the tiles of their range in order (one
+//     counter per range), and a look-back stops at the range's first tile,
+//     so the ranges proceed side by side.
+//   Later passes rank stably. The rank
+//     keeps that order: a tile whose hashes hold at most two values of the
+//     lower digits ranks them by (digit, which value) with LDS atomics;
+//     otherwise a wave ranks 64 hashes at a time from ballots of the digit's
+//     bits, the waves of a block covering consecutive stretches.
+// A look-back that waits longer than kLookBackSpins polls sets *failed and
+// takes the missing counts as zero, so that the launch always ends.
+enum PassMode
+{
+    kPassOffsets,
+    kPassLookBack,
+    kPassStable,
+    kPassRanged
+};
+template <int MODE, class D>
 __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
     const uint64_t *__restrict__ in, const uint32_t *__restrict__ regionCount, uint64_t n,
-    uint32_t shift, uint32_t bits, const uint64_t *__restrict__ digitStart, D *desc,
+    uint32_t shift, uint32_t bits, uint32_t prevShift, const uint64_t *__restrict__ digitStart,
+    const uint32_t *__restrict__ tileOffset, const uint64_t *__restrict__ rangeMeta, D *desc,
     unsigned int *tileCounter, unsigned long long *failed, uint64_t *__restrict__ out)
 {
     using LB = LookBack<D>;
+    constexpr bool kRegions = MODE == kPassOffsets || MODE == kPassLookBack;
+    constexpr bool kStable = MODE == kPassStable || MODE == kPassRanged;
     __shared__ uint64_t stage[kGroupTile];
-    __shared__ uint32_t cntRaw[kGroupWaves * kBins / 2]; // u16 [wave][bin], or u32 [bin]
+    __shared__ uint32_t cntRaw[kGroupWaves * kBins / 2]; // u16 [wave][bin], or u32 [bin][2]
     __shared__ uint16_t localStart[kBins];
     __shared__ uint64_t gbase[kBins];
     __shared__ uint32_t waveTot[kGroupWaves];
     __shared__ uint32_t tileSh, tileCount;
+    __shared__ uint64_t keyEnds[2]; // lower-digit values of the tile's first and last hash
     uint16_t(*waveCnt)[kBins] = reinterpret_cast<uint16_t(*)[kBins]>(cntRaw);
-    uint32_t *hist = cntRaw;
+    uint32_t *hist = cntRaw; // [digit][value of the lower digits - the first one]
     const int tid = threadIdx.x;
     const int lane = laneId();
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t bins = 1u << bits, dmask = bins - 1;
+    const uint64_t pmask = shift > prevShift ? (1ull << (shift - prevShift)) - 1 : 0;
+    const uint32_t xr = blockIdx.x % kRanges; // kPassRanged: the block's range
     if (tid == 0)
-        tileSh = atomicAdd(tileCounter, 1u);
+    {
+        if (MODE == kPassOffsets)
+        {
+            // consecutive tiles on one XCD (blocks go to XCDs in turn), so
+            // that one L2 joins the runs that neighbouring tiles write
+            const uint32_t full = gridDim.x / kRanges, rem = gridDim.x % kRanges;
+            tileSh = xr * full + min(xr, rem) + blockIdx.x / kRanges;
+        }
+        else
+            tileSh = atomicAdd(tileCounter + (MODE == kPassRanged ? xr : 0u), 1u);
+    }
     for (int i = tid; i < kGroupWaves * kBins / 2; i += kGroupBlock)
         cntRaw[i] = 0;
     __syncthreads();
-    const uint64_t t = tileSh;
+    // the tile's descriptor row, the first row of its look-back, its input
+    uint64_t t = tileSh, tFirst = 0, inBegin = 0, inEnd = 0;
+    if (MODE == kPassRanged)
+    {
+        const uint64_t tiles = rangeMeta[kRanges + 1 + xr + 1] - rangeMeta[kRanges + 1 + xr];
+        if (t >= tiles)
+            return; // more blocks of this class than tiles in its range
+        inBegin = rangeMeta[xr] + t * kGroupTile;
+        inEnd = min<uint64_t>(rangeMeta[xr + 1], inBegin + kGroupTile);
+        tFirst = rangeMeta[kRanges + 1 + xr];
+        t += tFirst;
+    }
+    else if (!kRegions)
+    {
+        inBegin = t * kGroupTile;
+        inEnd = min<uint64_t>(n, inBegin + kGroupTile);
+    }
     // this wave's stretch of the tile
     const uint64_t *src;
     uint32_t have;
-    if (FIRST)
+    if (kRegions)
     {
         const uint64_t region = t * (kGroupTile / kRegion) + (w >> 1);
         const uint32_t fill = regionCount[region];
@@ -478,9 +564,9 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
     }
     else
     {
-        const uint64_t e0 = t * kGroupTile + (uint64_t)w * kWaveHashes;
+        const uint64_t e0 = inBegin + (uint64_t)w * kWaveHashes;
         src = in + e0;
-        have = n > e0 ? (uint32_t)min<uint64_t>(n - e0, kWaveHashes) : 0u;
+        have = inEnd > e0 ? (uint32_t)min<uint64_t>(inEnd - e0, kWaveHashes) : 0u;
     }
     uint64_t v[kGroupPer];
     uint32_t rk[kGroupPer];
@@ -490,17 +576,41 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
         const uint32_t idx = (uint32_t)r * 64 + lane;
         v[r] = idx < have ? __builtin_nontemporal_load(src + idx) : 0ull;
     }
-#pragma unroll
-    for (int r = 0; r < kGroupPer; ++r)
+    bool twoKeys = false; // kPassStable: at most two lower-digit values
+    uint64_t key0 = 0;
+    if (kStable)
     {
-        const bool ok = (uint32_t)r * 64 + lane < have;
-        const uint32_t d = (uint32_t)(v[r] >> shift) & dmask;
-        if (FIRST)
+        const uint64_t cnt = inEnd - inBegin;
+        const uint32_t lastW = (uint32_t)((cnt - 1) / kWaveHashes);
+        const uint32_t lastI = (uint32_t)((cnt - 1) % kWaveHashes);
+        if (w == 0 && lane == 0)
+            keyEnds[0] = (v[0] >> prevShift) & pmask;
+#pragma unroll
+        for (int r = 0; r < kGroupPer; ++r)
+            if ((uint32_t)w == lastW && (uint32_t)r * 64 + lane == lastI)
+                keyEnds[1] = (v[r] >> prevShift) & pmask;
+        __syncthreads();
+        key0 = keyEnds[0];
+        twoKeys = keyEnds[1] - key0 <= 1;
+    }
+    if (!kStable || twoKeys)
+    {
+#pragma unroll
+        for (int r = 0; r < kGroupPer; ++r)
         {
-            rk[r] = ok ? atomicAdd(&hist[d], 1u) : 0u;
+            const bool ok = (uint32_t)r * 64 + lane < have;
+            const uint32_t d = (uint32_t)(v[r] >> shift) & dmask;
+            const uint32_t sub = kStable ? (uint32_t)(((v[r] >> prevShift) & pmask) - key0) : 0u;
+            rk[r] = ok ? atomicAdd(&hist[2 * d + sub], 1u) : 0u;
         }
-        else
+    }
+    else
+    {
+#pragma unroll
+        for (int r = 0; r < kGroupPer; ++r)
         {
+            const bool ok = (uint32_t)r * 64 + lane < have;
+            const uint32_t d = (uint32_t)(v[r] >> shift) & dmask;
             // lanes holding the same digit, from one ballot per digit bit
             uint64_t peers = __ballot(ok);
             for (uint32_t b = 0; b < bits; ++b)
@@ -524,13 +634,18 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
         }
     }
     __syncthreads();
-    // the tile's count of digit `tid`; later passes: each wave's count
-    // becomes the wave's start among the tile's hashes of that digit
-    uint32_t tot = 0;
+    // the tile's count of digit `tid` (and of its first lower-digit value);
+    // the ballot path turns each wave's count into the wave's start among
+    // the tile's hashes of that digit
+    const bool atomics = !kStable || twoKeys;
+    uint32_t tot = 0, firstPart = 0;
     if ((uint32_t)tid < bins)
     {
-        if (FIRST)
-            tot = hist[tid];
+        if (atomics)
+        {
+            firstPart = hist[2 * tid];
+            tot = firstPart + hist[2 * tid + 1];
+        }
         else
             for (int ww = 0; ww < kGroupWaves; ++ww)
             {
@@ -538,8 +653,10 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
                 waveCnt[ww][tid] = (uint16_t)tot;
                 tot += c;
             }
-        __hip_atomic_store(desc + t * kBins + tid, (t == 0 ? LB::kAll : LB::kOwn) | (D)tot,
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (MODE != kPassOffsets)
+            __hip_atomic_store(desc + t * kBins + tid,
+                               (t == tFirst ? LB::kAll : LB::kOwn) | (D)tot, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
     // exclusive scan of the counts over the digits: each digit's start in stage
     uint32_t incl = tot;
@@ -557,7 +674,11 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
     for (int ww = 0; ww < w; ++ww)
         before += waveTot[ww];
     if ((uint32_t)tid < bins)
+    {
         localStart[tid] = (uint16_t)(before + incl - tot);
+        if (atomics)
+            hist[2 * tid + 1] = firstPart; // the second value's start within the digit
+    }
     if ((uint32_t)tid == bins - 1)
         tileCount = before + incl;
     __syncthreads();
@@ -566,41 +687,73 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
         if ((uint32_t)r * 64 + lane < have)
         {
             const uint32_t d = (uint32_t)(v[r] >> shift) & dmask;
-            stage[localStart[d] + (FIRST ? 0u : (uint32_t)waveCnt[w][d]) + rk[r]] = v[r];
+            uint32_t at = localStart[d] + rk[r];
+            if (!atomics)
+                at += waveCnt[w][d];
+            else if (kStable && ((v[r] >> prevShift) & pmask) != key0)
+                at += hist[2 * d + 1];
+            stage[at] = v[r];
         }
-    // look-back: the hashes of digit `tid` in the tiles before this one
     if ((uint32_t)tid < bins)
     {
-        D excl = 0;
-        if (t > 0)
+        uint64_t excl = 0;
+        if (MODE == kPassOffsets)
+            excl = tileOffset[t * kBins + tid];
+        else
         {
-            uint64_t p = t - 1;
-            uint32_t spins = 0;
-            for (;;)
+            // look-back: the hashes of digit `tid` in the tiles before this
+            // one, kLookAhead tiles' words loaded at once
+            D sum = 0;
+            if (t > tFirst)
             {
-                const D x = __hip_atomic_load(desc + p * kBins + tid, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-                const D flag = x >> LB::kShift;
-                if (flag == 0)
+                int64_t p = (int64_t)t - 1; // the nearest tile not yet added
+                uint32_t spins = 0;
+                for (bool done = false; !done;)
                 {
-                    if (++spins >= kLookBackSpins)
+                    D x[kLookAhead];
+#pragma unroll
+                    for (int j = 0; j < kLookAhead; ++j)
+                        x[j] = p - j >= (int64_t)tFirst
+                                   ? __hip_atomic_load(desc + (p - j) * kBins + tid, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT)
+                                   : LB::kAll; // before tFirst (never reached: tFirst is whole)
+                    // add words up to the first whole one; stop before one
+                    // not published yet (polled again)
+                    int used = 0;
+                    bool stop = false;
+#pragma unroll
+                    for (int j = 0; j < kLookAhead; ++j)
                     {
-                        atomicExch(failed, 1ull);
-                        break;
+                        const D flag = x[j] >> LB::kShift;
+                        const bool take = !stop && flag != 0;
+                        sum += take ? (x[j] & LB::kMask) : D(0);
+                        used = take ? j + 1 : used;
+                        done = done || (take && flag == 2);
+                        stop = stop || flag != 1;
                     }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+                    p -= used;
+                    if (done)
+                        break;
+                    if (used == 0)
+                    {
+                        if (++spins >= kLookBackSpins)
+                        {
+                            atomicExch(failed, 1ull);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    else
+                        spins = 0;
                 }
-                excl += x & LB::kMask;
-                if (flag == 2 || p == 0)
-                    break;
-                --p;
-                spins = 0;
+                __hip_atomic_store(desc + t * kBins + tid, LB::kAll | ((sum + (D)tot) & LB::kMask),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            __hip_atomic_store(desc + t * kBins + tid, LB::kAll | ((excl + (D)tot) & LB::kMask),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            excl = (MODE == kPassRanged ? rangeMeta[2 * (kRanges + 1) + xr * kBins + tid]
+                                        : digitStart[tid]) +
+                   (uint64_t)sum;
         }
-        gbase[tid] = digitStart[tid] + (uint64_t)excl;
+        gbase[tid] = excl;
     }
     __syncthreads();
     // the staged hashes of one digit are adjacent: so are their stores
@@ -610,6 +763,65 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
         const uint64_t x = stage[e];
         const uint32_t d = (uint32_t)(x >> shift) & dmask;
         out[gbase[d] + (e - localStart[d])] = x;
+    }
+}
+
+// The first pass's per-tile places from the hashing kernel's per-tile
+// counts (tileHist: kTileDigits x kBins u16 per tile, column `col`), in
+// groups of kScanTiles tiles: tileSumKernel sums each group, groupScanKernel
+// turns the sums into each group's start (the digit's start + the earlier
+// groups), tileOffsetKernel into each tile's.
+constexpr int kScanTiles = 512;
+__global__ __launch_bounds__(kBins) void tileSumKernel(const uint16_t *__restrict__ tileHist,
+                                                       uint64_t nTiles, uint32_t col,
+                                                       uint32_t bins, uint32_t *groupSum)
+{
+    const uint32_t d = threadIdx.x;
+    if (d >= bins)
+        return;
+    const uint64_t t0 = (uint64_t)blockIdx.x * kScanTiles;
+    const uint64_t t1 = min<uint64_t>(nTiles, t0 + kScanTiles);
+    uint32_t sum = 0;
+#pragma unroll 8
+    for (uint64_t t = t0; t < t1; ++t)
+        sum += tileHist[(t * kTileDigits + col) * kBins + d];
+    groupSum[(uint64_t)blockIdx.x * kBins + d] = sum;
+}
+
+__global__ __launch_bounds__(kBins) void groupScanKernel(uint32_t *groupSum, uint64_t nGroups,
+                                                         const uint64_t *__restrict__ digitStart,
+                                                         uint32_t bins)
+{
+    const uint32_t d = threadIdx.x;
+    if (d >= bins)
+        return;
+    uint32_t run = (uint32_t)digitStart[d];
+#pragma unroll 8
+    for (uint64_t g = 0; g < nGroups; ++g)
+    {
+        const uint32_t x = groupSum[g * kBins + d];
+        groupSum[g * kBins + d] = run;
+        run += x;
+    }
+}
+
+__global__ __launch_bounds__(kBins) void tileOffsetKernel(const uint16_t *__restrict__ tileHist,
+                                                          uint64_t nTiles, uint32_t col,
+                                                          uint32_t bins,
+                                                          const uint32_t *__restrict__ groupSum,
+                                                          uint32_t *tileOffset)
+{
+    const uint32_t d = threadIdx.x;
+    if (d >= bins)
+        return;
+    const uint64_t t0 = (uint64_t)blockIdx.x * kScanTiles;
+    const uint64_t t1 = min<uint64_t>(nTiles, t0 + kScanTiles);
+    uint32_t run = groupSum[(uint64_t)blockIdx.x * kBins + d];
+#pragma unroll 8
+    for (uint64_t t = t0; t < t1; ++t)
+    {
+        tileOffset[t * kBins + d] = run;
+        run += tileHist[(t * kTileDigits + col) * kBins + d];
     }
 }
 
@@ -977,8 +1189,16 @@ struct hipibs_counter : ReadSink
     uint64_t nSlots = 0;     // slots the open flush's hashing tiles own in dHash[0]
     uint64_t nHash = 0;      // hashes of the open flush (after a sync)
     uint32_t *dRegion = nullptr;           // the fill of every hashing region
+    uint16_t *dTileHist = nullptr;         // every hashing tile's counts of digits 1 and 2
+    uint32_t *dGroupSum = nullptr;         // the first pass's per-group sums, then starts
+    uint32_t *dTileOffset = nullptr;       // the first pass's per-tile places
+    uint64_t tileOffsetCap = 0;            // tiles
     uint32_t *dHll = nullptr;              // the flush's distinct-count sketch
     unsigned long long *dDigit = nullptr;  // the flush's digit histograms
+    unsigned long long *dRange = nullptr;  // the flush's digit-0 counts by range
+    uint64_t *dRangeMeta = nullptr;        // the last pass's ranges (see groupPassKernel)
+    std::vector<unsigned long long> hRange;
+    std::vector<uint64_t> hRangeMeta;
     uint64_t *dDigitStart = nullptr;       // their exclusive sums (kMaxDigits x (kBins + 1))
     std::vector<unsigned long long> hDigit;
     std::vector<uint64_t> hDigitStart;
@@ -1147,14 +1367,24 @@ struct hipibs_counter : ReadSink
                                    kMaxDigits * kBins * sizeof(unsigned long long)));
             HIPIBS_CHECK(hipMemsetAsync(dDigit, 0, kMaxDigits * kBins * sizeof(unsigned long long),
                                         stream));
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dRange),
+                                   kRanges * kBins * sizeof(unsigned long long)));
+            HIPIBS_CHECK(hipMemsetAsync(dRange, 0, kRanges * kBins * sizeof(unsigned long long),
+                                        stream));
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dRangeMeta),
+                                   (2 * (kRanges + 1) + kRanges * kBins) * sizeof(uint64_t)));
             HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dRegion),
                                    (hashBudget / kRegion + 1) * sizeof(uint32_t)));
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileHist),
+                                   (hashBudget / kTilePositions + 1) * kTileDigits * kBins *
+                                       sizeof(uint16_t)));
         }
         const uint64_t tileBase = nSlots / kTilePositions;
         if ((rc = timed(0, [&] {
                  countHashKernel<<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0,
                                    stream>>>(text, n, skip, k, tileBase, digitBits, dHash[0],
-                                             dRegion, dHll, dDigit, dCounters);
+                                             dRegion, dTileHist, rangeBits(), dHll, dDigit,
+                                             dRange, dCounters);
              })))
             return rc;
         HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 9 * sizeof(unsigned long long),
@@ -1176,9 +1406,14 @@ struct hipibs_counter : ReadSink
             (chunkBytes + kTilePositions - 1) / kTilePositions * kTilePositions;
         hashBudget = std::min<uint64_t>(16 * chunkSlots, freeB / 4 / 16 / kTilePositions *
                                                              kTilePositions);
+        // places within a flush are 32-bit (the first pass's per-tile ones)
+        hashBudget = std::min<uint64_t>(hashBudget, 1ull << 31);
         hashBudget = std::max<uint64_t>(hashBudget, chunkSlots);
         return 0;
     }
+
+    // Bits of digit 1 that cut the last pass's input into ranges.
+    uint32_t rangeBits() const { return std::min<uint32_t>(3, digitBits); }
 
     // Grows the hash buffers to hold `need` entries (at most the budget),
     // keeping the open flush's; called with the stream synchronised.
@@ -1280,6 +1515,15 @@ struct hipibs_counter : ReadSink
         hDigitStart.assign(kMaxDigits * (kBins + 1), 0);
         HIPIBS_CHECK(hipMemcpy(hDigit.data(), dDigit, hDigit.size() * sizeof(unsigned long long),
                                hipMemcpyDeviceToHost));
+        hRange.resize(kRanges * kBins);
+        HIPIBS_CHECK(hipMemcpy(hRange.data(), dRange, hRange.size() * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost));
+        for (uint32_t d = 0; d < bins; ++d)
+        {
+            hDigit[d] = 0;
+            for (int x = 0; x < kRanges; ++x)
+                hDigit[d] += hRange[x * kBins + d];
+        }
         uint64_t widest = 0;
         for (int c = 0; c < kMaxDigits; ++c)
         {
@@ -1303,7 +1547,7 @@ struct hipibs_counter : ReadSink
         const bool narrow = widest < (1ull << 30) && !wideLookBack;
         const uint64_t tilesFirst = nSlots / kTilePositions;
         const uint64_t tilesDense = (nHash + kGroupTile - 1) / kGroupTile;
-        const uint64_t tilesMax = std::max(tilesFirst, tilesDense);
+        const uint64_t tilesMax = std::max(tilesFirst, tilesDense + kRanges);
         const uint64_t wordBytes = narrow ? 4 : 8;
         if (descBytes < tilesMax * kBins * wordBytes)
         {
@@ -1315,38 +1559,130 @@ struct hipibs_counter : ReadSink
             descBytes = tilesMax * kBins * 8;
         }
         if (!dTileCounter)
-            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileCounter), 16));
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileCounter),
+                                   kRanges * sizeof(unsigned int)));
         // the passes, least significant digit first, ping-ponging between
-        // the two buffers; then the slices' starts
+        // the two buffers; then the slices' starts. The first pass over two
+        // or three digits takes its tiles' places from the hashing kernel's
+        // per-tile counts; every other pass finds them by look-back.
         uint64_t *src = dHash[0], *dst = dHash[1];
+        const uint32_t prevShift = 64 - passes * digitBits;
+        auto *failed = dCounters + 10;
         for (uint32_t q = 0; q < passes; ++q)
         {
             const uint32_t c = passes - 1 - q;
             const uint32_t shift = 64 - (c + 1) * digitBits;
-            const uint64_t tiles = q == 0 ? tilesFirst : tilesDense;
             const uint64_t *starts = dDigitStart + c * (kBins + 1);
-            HIPIBS_CHECK(hipMemsetAsync(dDesc, 0, tiles * kBins * wordBytes, stream));
-            HIPIBS_CHECK(hipMemsetAsync(dTileCounter, 0, 4, stream));
-            auto *failed = dCounters + 10;
-            if ((rc = timed(1, [&] {
-                     if (q == 0 && narrow)
-                         groupPassKernel<true, uint32_t><<<(uint32_t)tiles, kGroupBlock, 0, stream>>>(
-                             src, dRegion, nHash, shift, digitBits, starts,
-                             static_cast<uint32_t *>(dDesc), dTileCounter, failed, dst);
-                     else if (q == 0)
-                         groupPassKernel<true, uint64_t><<<(uint32_t)tiles, kGroupBlock, 0, stream>>>(
-                             src, dRegion, nHash, shift, digitBits, starts,
-                             static_cast<uint64_t *>(dDesc), dTileCounter, failed, dst);
-                     else if (narrow)
-                         groupPassKernel<false, uint32_t><<<(uint32_t)tiles, kGroupBlock, 0, stream>>>(
-                             src, dRegion, nHash, shift, digitBits, starts,
-                             static_cast<uint32_t *>(dDesc), dTileCounter, failed, dst);
-                     else
-                         groupPassKernel<false, uint64_t><<<(uint32_t)tiles, kGroupBlock, 0, stream>>>(
-                             src, dRegion, nHash, shift, digitBits, starts,
-                             static_cast<uint64_t *>(dDesc), dTileCounter, failed, dst);
-                 })))
-                return rc;
+            if (q == 0 && c > 0)
+            {
+                const uint64_t groups = (tilesFirst + kScanTiles - 1) / kScanTiles;
+                if (tileOffsetCap < tilesFirst)
+                {
+                    if (dGroupSum)
+                        HIPIBS_CHECK(hipFree(dGroupSum));
+                    if (dTileOffset)
+                        HIPIBS_CHECK(hipFree(dTileOffset));
+                    dGroupSum = nullptr;
+                    dTileOffset = nullptr;
+                    tileOffsetCap = 0;
+                    HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dGroupSum),
+                                           (groups + 1) * kBins * sizeof(uint32_t)));
+                    HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileOffset),
+                                           tilesFirst * kBins * sizeof(uint32_t)));
+                    tileOffsetCap = tilesFirst;
+                }
+                if ((rc = timed(1, [&] {
+                         tileSumKernel<<<(uint32_t)groups, kBins, 0, stream>>>(
+                             dTileHist, tilesFirst, c - 1, bins, dGroupSum);
+                         groupScanKernel<<<1, kBins, 0, stream>>>(dGroupSum, groups, starts, bins);
+                         tileOffsetKernel<<<(uint32_t)groups, kBins, 0, stream>>>(
+                             dTileHist, tilesFirst, c - 1, bins, dGroupSum, dTileOffset);
+                         groupPassKernel<kPassOffsets, uint32_t>
+                             <<<(uint32_t)tilesFirst, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, starts,
+                                 dTileOffset, nullptr, static_cast<uint32_t *>(dDesc), dTileCounter,
+                                 failed, dst);
+                     })))
+                    return rc;
+            }
+            else if (q > 0 && c == 0)
+            {
+                // the last pass in kRanges ranges: their input starts (the
+                // counts by range are ordered as the input is, by digit 1),
+                // first tiles, and output bases (digit 0's start + the
+                // earlier ranges' counts of it)
+                const uint32_t nr = 1u << rangeBits();
+                hRangeMeta.assign(2 * (kRanges + 1) + kRanges * kBins, 0);
+                uint64_t at = 0, tile = 0, longest = 0;
+                for (int x = 0; x <= kRanges; ++x)
+                {
+                    hRangeMeta[x] = at;
+                    hRangeMeta[kRanges + 1 + x] = tile;
+                    if (x == kRanges)
+                        break;
+                    uint64_t cnt = 0;
+                    for (uint32_t d = 0; d < bins; ++d)
+                    {
+                        hRangeMeta[2 * (kRanges + 1) + x * kBins + d] =
+                            hDigitStart[d] + (x ? hRangeMeta[2 * (kRanges + 1) + (x - 1) * kBins + d] -
+                                                      hDigitStart[d] + hRange[(x - 1) * kBins + d]
+                                                : 0);
+                        cnt += hRange[x * kBins + d];
+                    }
+                    if ((uint32_t)x >= nr && cnt)
+                        return setError(HIPIBS_E_HIP, "k-mer counter: a k-mer outside the ranges");
+                    const uint64_t tiles = (cnt + kGroupTile - 1) / kGroupTile;
+                    at += cnt;
+                    tile += tiles;
+                    longest = std::max(longest, tiles);
+                }
+                if (at != nHash)
+                    return setError(HIPIBS_E_HIP, "k-mer counter: slice grouping lost k-mers");
+                HIPIBS_CHECK(hipMemcpy(dRangeMeta, hRangeMeta.data(),
+                                       hRangeMeta.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+                HIPIBS_CHECK(hipMemsetAsync(dDesc, 0, tile * kBins * wordBytes, stream));
+                HIPIBS_CHECK(hipMemsetAsync(dTileCounter, 0, kRanges * sizeof(unsigned int), stream));
+                if ((rc = timed(1, [&] {
+                         const dim3 grid((uint32_t)(kRanges * longest));
+                         if (narrow)
+                             groupPassKernel<kPassRanged, uint32_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
+                                 dRangeMeta, static_cast<uint32_t *>(dDesc), dTileCounter, failed,
+                                 dst);
+                         else
+                             groupPassKernel<kPassRanged, uint64_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
+                                 dRangeMeta, static_cast<uint64_t *>(dDesc), dTileCounter, failed,
+                                 dst);
+                     })))
+                    return rc;
+            }
+            else
+            {
+                const uint64_t tiles = q == 0 ? tilesFirst : tilesDense;
+                HIPIBS_CHECK(hipMemsetAsync(dDesc, 0, tiles * kBins * wordBytes, stream));
+                HIPIBS_CHECK(hipMemsetAsync(dTileCounter, 0, 4, stream));
+                if ((rc = timed(1, [&] {
+                         const dim3 grid((uint32_t)tiles);
+                         if (q == 0 && narrow)
+                             groupPassKernel<kPassLookBack, uint32_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
+                                 nullptr, static_cast<uint32_t *>(dDesc), dTileCounter, failed, dst);
+                         else if (q == 0)
+                             groupPassKernel<kPassLookBack, uint64_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
+                                 nullptr, static_cast<uint64_t *>(dDesc), dTileCounter, failed, dst);
+                         else if (narrow)
+                             groupPassKernel<kPassStable, uint32_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
+                                 nullptr, static_cast<uint32_t *>(dDesc), dTileCounter, failed, dst);
+                         else
+                             groupPassKernel<kPassStable, uint64_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
+                                 nullptr, static_cast<uint64_t *>(dDesc), dTileCounter, failed, dst);
+                     })))
+                    return rc;
+            }
             std::swap(src, dst);
         }
         if ((rc = timed(1, [&] {
@@ -1396,6 +1732,8 @@ struct hipibs_counter : ReadSink
         HIPIBS_CHECK(hipMemsetAsync(dCounters + 7, 0, 2 * sizeof(unsigned long long), stream));
         HIPIBS_CHECK(hipMemsetAsync(dHll, 0, 4u << kHllLog, stream));
         HIPIBS_CHECK(hipMemsetAsync(dDigit, 0, kMaxDigits * kBins * sizeof(unsigned long long),
+                                    stream));
+        HIPIBS_CHECK(hipMemsetAsync(dRange, 0, kRanges * kBins * sizeof(unsigned long long),
                                     stream));
         HIPIBS_CHECK(hipStreamSynchronize(stream));
         hCounters[7] = hCounters[8] = 0;
@@ -1642,8 +1980,8 @@ extern "C" int hipibs_counter_create(uint32_t k, int device, uint64_t expected_k
         return setError(HIPIBS_E_ARG, "k-mer counter: k must be 1..64");
     if (chunk_bytes == 0)
         chunk_bytes = kDefaultChunk;
-    if (chunk_bytes < 64 || chunk_bytes > (1ull << 32))
-        return setError(HIPIBS_E_ARG, "k-mer counter: chunk_bytes must be 64 B .. 4 GiB");
+    if (chunk_bytes < 64 || chunk_bytes > (1ull << 30))
+        return setError(HIPIBS_E_ARG, "k-mer counter: chunk_bytes must be 64 B .. 1 GiB");
     int nDev = 0;
     if (hipGetDeviceCount(&nDev) != hipSuccess || nDev == 0)
     {
@@ -1914,8 +2252,18 @@ extern "C" void hipibs_counter_free(hipibs_counter *c)
         (void)hipFree(c->dStart);
     if (c->dRegion)
         (void)hipFree(c->dRegion);
+    if (c->dTileHist)
+        (void)hipFree(c->dTileHist);
+    if (c->dGroupSum)
+        (void)hipFree(c->dGroupSum);
+    if (c->dTileOffset)
+        (void)hipFree(c->dTileOffset);
     if (c->dDigit)
         (void)hipFree(c->dDigit);
+    if (c->dRange)
+        (void)hipFree(c->dRange);
+    if (c->dRangeMeta)
+        (void)hipFree(c->dRangeMeta);
     if (c->dDigitStart)
         (void)hipFree(c->dDigitStart);
     if (c->dDesc)

@@ -247,9 +247,9 @@ typedef struct hipibs_count_filter {
 
 /* k in 1..64 (for k > 32, key arrays hold two words per key, high first).
  * expected_keys (0 = unknown) sizes the first table; the table
- * grows as needed. chunk_bytes (0 = 128 MiB) is the text moved to the device
- * at a time: two pinned host buffers and one device buffer of that size,
- * allocated when host text first arrives. */
+ * grows as needed. chunk_bytes (0 = 128 MiB, at most 1 GiB) is the text
+ * moved to the device at a time: two pinned host buffers and one device
+ * buffer of that size, allocated when host text first arrives. */
 int hipibs_counter_create(uint32_t k, int device, uint64_t expected_keys, uint64_t chunk_bytes,
                           hipibs_counter **out);
 /* Counts a FASTA or FASTQ file, plain or gzip (decided by its first byte);
