@@ -850,19 +850,22 @@ __global__ __launch_bounds__(256) void sliceStartKernel(const uint64_t *__restri
     }
     start[s] = lo;
 }
-// One workgroup per slice: loads the slice's buckets into LDS (or EMPTY
-// keys and zero counts when the table is fresh), counts the slice's k-mers
-// there (the key by compare-and-swap into the first EMPTY slot from its
-// home, the count by an LDS atomic add), writes the slice back; with
-// writeAll (an unfilled table) a slice without k-mers is written empty. A k-mer
-// whose probe reaches the slice's end without its key or an EMPTY slot is
-// appended to spill (counters[7]). counters[0] += keys stored.
+// One workgroup per slice: loads the slice's keys into LDS (or EMPTY keys
+// when the table is fresh), counts the slice's k-mers there -- a bucket's 8
+// keys read as four 16 B words; the key's slot if it is there, else a
+// compare-and-swap into the bucket's first EMPTY slot (read again if another
+// key took it), then an LDS add to the slot's 32-bit count of this flush --
+// and writes the slice back, each count added to the table's (with writeAll,
+// an unfilled table, a slice without k-mers is written empty). A k-mer whose
+// probe reaches the slice's end without its key or an EMPTY slot is
+// appended to spill (counters[7]). counters[0] += keys stored. A flush
+// holds fewer than 2^32 k-mers, so no count of one flush overflows.
 __global__ __launch_bounds__(kSliceBlock) void countSliceKernel(
     const uint64_t *__restrict__ hs, const uint64_t *__restrict__ start, uint64_t *tab,
     uint64_t nBuckets, int fresh, int writeAll, uint64_t *spill, unsigned long long *counters)
 {
-    __shared__ unsigned long long keys[kSliceBuckets * 8];
-    __shared__ unsigned long long cnt[kSliceBuckets * 8];
+    __shared__ __align__(16) unsigned long long keys[kSliceBuckets * 8];
+    __shared__ uint32_t cnt[kSliceBuckets * 8];
     __shared__ uint32_t stored;
     const uint64_t s = blockIdx.x;
     const uint64_t lo = start[s], hi = start[s + 1];
@@ -879,14 +882,10 @@ __global__ __launch_bounds__(kSliceBlock) void countSliceKernel(
     }
     if (threadIdx.x == 0)
         stored = 0;
-    for (uint32_t q = threadIdx.x; q < nb * kCountWords; q += kSliceBlock)
+    for (uint32_t q = threadIdx.x; q < nb * 8; q += kSliceBlock)
     {
-        const uint32_t b = q / kCountWords, w = q % kCountWords;
-        const uint64_t v = fresh ? (w < 8 ? EMPTY : 0ull) : tab[(b0 + b) * kCountWords + w];
-        if (w < 8)
-            keys[b * 8 + w] = v;
-        else
-            cnt[b * 8 + w - 8] = v;
+        keys[q] = fresh ? EMPTY : tab[(b0 + q / 8) * kCountWords + q % 8];
+        cnt[q] = 0;
     }
     __syncthreads();
     uint32_t mine = 0;
@@ -911,25 +910,46 @@ __global__ __launch_bounds__(kSliceBlock) void countSliceKernel(
             const uint64_t key = keyUnmix(h);
             const uint64_t home = ((h >> 32) * nBuckets) >> 32;
             bool done = false;
-            for (uint32_t b = (uint32_t)(home - b0); b < nb && !done; ++b)
-                for (int q = 0; q < 8 && !done; ++q)
+            for (uint32_t b = (uint32_t)(home - b0); b < nb && !done;)
+            {
+                const ulonglong2 *kb = reinterpret_cast<const ulonglong2 *>(&keys[b * 8]);
+                uint64_t kk[8];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
                 {
-                    unsigned long long cur = keys[b * 8 + q];
-                    if (cur == EMPTY)
+                    const ulonglong2 x = kb[q];
+                    kk[2 * q] = x.x;
+                    kk[2 * q + 1] = x.y;
+                }
+                // keys take the first EMPTY slot and are never removed: a
+                // stored key sits before the first EMPTY one
+                int hit = 8, emp = 8;
+#pragma unroll
+                for (int q = 7; q >= 0; --q)
+                {
+                    hit = kk[q] == key ? q : hit;
+                    emp = kk[q] == EMPTY ? q : emp;
+                }
+                if (hit < emp)
+                {
+                    atomicAdd(&cnt[b * 8 + hit], 1u);
+                    done = true;
+                }
+                else if (emp < 8)
+                {
+                    const unsigned long long prev =
+                        atomicCAS(&keys[b * 8 + emp], EMPTY, (unsigned long long)key);
+                    if (prev == EMPTY || prev == key)
                     {
-                        cur = atomicCAS(&keys[b * 8 + q], EMPTY, (unsigned long long)key);
-                        if (cur == EMPTY)
-                        {
-                            mine += 1;
-                            cur = key;
-                        }
-                    }
-                    if (cur == key)
-                    {
-                        atomicAdd(&cnt[b * 8 + q], 1ull);
+                        mine += prev == EMPTY;
+                        atomicAdd(&cnt[b * 8 + emp], 1u);
                         done = true;
                     }
+                    // else another key took the slot: this bucket is read again
                 }
+                else
+                    ++b;
+            }
             if (!done)
                 spill[atomicAdd(counters + 7, 1ull)] = h;
         }
@@ -940,7 +960,13 @@ __global__ __launch_bounds__(kSliceBlock) void countSliceKernel(
     for (uint32_t q = threadIdx.x; q < nb * kCountWords; q += kSliceBlock)
     {
         const uint32_t b = q / kCountWords, w = q % kCountWords;
-        tab[(b0 + b) * kCountWords + w] = w < 8 ? keys[b * 8 + w] : cnt[b * 8 + w - 8];
+        uint64_t *dst = tab + (b0 + b) * kCountWords + w;
+        if (w < 8)
+            *dst = keys[b * 8 + w];
+        else if (fresh)
+            *dst = cnt[b * 8 + w - 8];
+        else if (const uint32_t c = cnt[b * 8 + w - 8])
+            *dst += c;
     }
     if (threadIdx.x == 0 && stored)
         atomicAdd(counters + 0, (unsigned long long)stored);
