@@ -306,8 +306,83 @@ constexpr int kMaxDigitBits = 9;
 constexpr int kBins = 1 << kMaxDigitBits;
 constexpr int kTileDigits = 2; // per-tile counts of digits 1 and 2 (a first pass's digits)
 constexpr int kRanges = 8;     // the last pass's input ranges, one per XCD
+
+// The value the sort orders a hash by, its digits read from the top bits:
+// the k-mer counter sorts its hashes by themselves (its table's slices are
+// hash prefixes); a table build sorts its keys' hashes by the slice of
+// their home bucket (monotone in the hash's top 32 bits for any bucket
+// count) put at the top of the word.
+struct SortKey
+{
+    uint64_t nBuckets; // 0: the hash itself
+    uint32_t sliceLog; // buckets per slice = 2^sliceLog
+    uint32_t sbits;    // bits of the slice index
+    __host__ __device__ uint64_t operator()(uint64_t h) const
+    {
+        if (!nBuckets)
+            return h;
+        if (!sbits)
+            return 0;
+        return ((((h >> 32) * nBuckets) >> 32) >> sliceLog) << (64 - sbits);
+    }
+};
 constexpr int kHashWaves = kCountBlock / 64;
 constexpr uint32_t kRegion = 64 * kLanePositions; // hash slots of one wave per tile
+
+// The counts of sort keys a hashing kernel takes for the sort, in LDS:
+// digit 0 by range (the top rangeBits bits of digit 1: the last pass's
+// ranges), digits 1 and 2 of the current tile (the first pass's places) and
+// of the block.
+struct DigitCounts
+{
+    uint32_t rng[kRanges][kBins];
+    uint32_t tile[kTileDigits][kBins];
+    uint32_t block[kTileDigits][kBins];
+
+    __device__ void clear()
+    {
+        for (int i = threadIdx.x; i < kRanges * kBins; i += blockDim.x)
+            (&rng[0][0])[i] = 0;
+        for (int i = threadIdx.x; i < kTileDigits * kBins; i += blockDim.x)
+        {
+            (&tile[0][0])[i] = 0;
+            (&block[0][0])[i] = 0;
+        }
+    }
+    __device__ void add(uint64_t key, uint32_t digitBits, uint32_t rangeBits)
+    {
+        const uint32_t dmask = (1u << digitBits) - 1;
+        atomicAdd(&rng[(key >> (64 - digitBits - rangeBits)) & ((1u << rangeBits) - 1)]
+                      [(key >> (64 - digitBits)) & dmask],
+                  1u);
+#pragma unroll
+        for (int dg = 1; dg < kMaxDigits; ++dg)
+            atomicAdd(&tile[dg - 1][(key >> (64 - (dg + 1) * digitBits)) & dmask], 1u);
+    }
+    // The tile's counts out as row `tileIndex` of tileHist and into the
+    // block's; call after a barrier that follows the tile's adds.
+    __device__ void endTile(uint16_t *tileHist, uint64_t tileIndex)
+    {
+        for (int i = threadIdx.x; i < kTileDigits * kBins; i += blockDim.x)
+        {
+            const uint32_t c = (&tile[0][0])[i];
+            tileHist[tileIndex * (kTileDigits * kBins) + i] = (uint16_t)c;
+            (&block[0][0])[i] += c;
+            (&tile[0][0])[i] = 0;
+        }
+    }
+    // The block's counts into the global histograms (digitHist rows 1 and 2);
+    // call after a barrier.
+    __device__ void flush(unsigned long long *digitHist, unsigned long long *rangeHist)
+    {
+        for (int i = threadIdx.x; i < kTileDigits * kBins; i += blockDim.x)
+            if (const uint32_t c = (&block[0][0])[i])
+                atomicAdd(digitHist + kBins + i, (unsigned long long)c);
+        for (int i = threadIdx.x; i < kRanges * kBins; i += blockDim.x)
+            if (const uint32_t c = (&rng[0][0])[i])
+                atomicAdd(rangeHist + i, (unsigned long long)c);
+    }
+};
 
 // Writes h = keyMix(code) of the k-mers at positions skip .. n-k of text.
 // Tile t (kTilePositions positions, the launch's tiles following tileBase)
@@ -331,22 +406,14 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(
     static_assert(kTileDigits == kMaxDigits - 1, "per-tile counts of the digits below the top");
     __shared__ __align__(16) uint8_t tile[kTileBytes];
     __shared__ uint32_t reg[1 << kHllLog];
-    __shared__ uint32_t hist[kTileDigits][kBins];   // the block's, digits 1 and 2
-    __shared__ uint32_t rng[kRanges][kBins];        // the block's, digit 0 by range
-    __shared__ uint32_t tileCnt[kTileDigits][kBins]; // the current tile's, digits 1 and 2
+    __shared__ DigitCounts dc;
     for (int i = threadIdx.x; i < (1 << kHllLog); i += kCountBlock)
         reg[i] = 0;
-    for (int i = threadIdx.x; i < kTileDigits * kBins; i += kCountBlock)
-        (&hist[0][0])[i] = 0;
-    for (int i = threadIdx.x; i < kRanges * kBins; i += kCountBlock)
-        (&rng[0][0])[i] = 0;
-    for (int i = threadIdx.x; i < kTileDigits * kBins; i += kCountBlock)
-        (&tileCnt[0][0])[i] = 0;
+    dc.clear();
     const uint64_t nPos = n >= k ? n - k + 1 : 0;
     const uint64_t nTiles = (nPos + kTilePositions - 1) / kTilePositions;
     const uint64_t mask = k == 32 ? ~0ull : (1ull << (2 * k)) - 1;
     const uint32_t shift = 2 * (k - 1);
-    const uint32_t dmask = (1u << digitBits) - 1;
     const int lane = laneId();
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     uint32_t counted = 0;
@@ -394,12 +461,7 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(
                 if (emit)
                 {
                     wout[wcount + maskedRank(m)] = h;
-                    atomicAdd(&rng[(h >> (64 - digitBits - rangeBits)) & ((1u << rangeBits) - 1)]
-                                  [(h >> (64 - digitBits)) & dmask],
-                              1u);
-#pragma unroll
-                    for (int dg = 1; dg < kMaxDigits; ++dg)
-                        atomicAdd(&tileCnt[dg - 1][(h >> (64 - (dg + 1) * digitBits)) & dmask], 1u);
+                    dc.add(h, digitBits, rangeBits);
                     if (((h >> 20) & ((1u << kHllSample) - 1)) == 0)
                     {
                         const uint32_t rank =
@@ -413,15 +475,8 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(
         }
         if (lane == 0)
             regionCount[region] = wcount;
-        // the tile's counts of digits 1 and 2 out, added to the block's
         __syncthreads();
-        for (int i = threadIdx.x; i < kTileDigits * kBins; i += kCountBlock)
-        {
-            const uint32_t c = (&tileCnt[0][0])[i];
-            tileHist[(tileBase + t) * (kTileDigits * kBins) + i] = (uint16_t)c;
-            (&hist[0][0])[i] += c;
-            (&tileCnt[0][0])[i] = 0;
-        }
+        dc.endTile(tileHist, tileBase + t);
     }
     counted = waveSum(counted);
     if (lane == 0 && counted)
@@ -433,12 +488,68 @@ __global__ __launch_bounds__(kCountBlock) void countHashKernel(
     for (int i = threadIdx.x; i < (1 << kHllLog); i += kCountBlock)
         if (reg[i])
             atomicMax(hll + i, reg[i]);
-    for (int i = threadIdx.x; i < kTileDigits * kBins; i += kCountBlock)
-        if (const uint32_t c = (&hist[0][0])[i])
-            atomicAdd(digitHist + kBins + i, (unsigned long long)c);
-    for (int i = threadIdx.x; i < kRanges * kBins; i += kCountBlock)
-        if (const uint32_t c = (&rng[0][0])[i])
-            atomicAdd(rangeHist + i, (unsigned long long)c);
+    dc.flush(digitHist, rangeHist);
+}
+
+// A table build's hashing: h = keyMix(key) of every row of keys[0..n) but
+// EMPTY ones, in countHashKernel's regions (tile t: rows [t kTilePositions,
+// +kTilePositions), wave w of it the kRegion rows from w kRegion, packed to
+// the region's start, the fill in regionCount) with its counts of the sort
+// key sk(h); counters[0] += hashes written.
+__global__ __launch_bounds__(kCountBlock) void keyHashKernel(
+    const uint64_t *__restrict__ keys, uint64_t n, SortKey sk, uint32_t digitBits,
+    uint32_t rangeBits, uint64_t *__restrict__ out, uint32_t *__restrict__ regionCount,
+    uint16_t *__restrict__ tileHist, unsigned long long *digitHist,
+    unsigned long long *rangeHist, unsigned long long *counters)
+{
+    __shared__ DigitCounts dc;
+    dc.clear();
+    const uint64_t nTiles = (n + kTilePositions - 1) / kTilePositions;
+    const int lane = laneId();
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    uint32_t written = 0;
+    for (uint64_t t = blockIdx.x; t < nTiles; t += gridDim.x)
+    {
+        const uint64_t region = t * kHashWaves + wv;
+        const uint64_t *src = keys + region * kRegion;
+        uint64_t *wout = out + region * kRegion;
+        const uint64_t left = n > region * kRegion ? n - region * kRegion : 0;
+        const uint32_t rows = (uint32_t)min<uint64_t>(left, kRegion);
+        __syncthreads(); // the previous tile's counts are out
+        uint32_t wcount = 0;
+        for (uint32_t r0 = 0; r0 < kRegion; r0 += 64 * 8)
+        {
+            uint64_t kv[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+            {
+                const uint32_t i = r0 + (uint32_t)j * 64 + lane;
+                kv[j] = i < rows ? __builtin_nontemporal_load(src + i) : EMPTY;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+            {
+                const bool ok = kv[j] != EMPTY;
+                const uint64_t m = __ballot(ok);
+                if (ok)
+                {
+                    const uint64_t h = keyMix(kv[j]);
+                    wout[wcount + maskedRank(m)] = h;
+                    dc.add(sk(h), digitBits, rangeBits);
+                }
+                wcount += (uint32_t)__popcll(m);
+            }
+        }
+        if (lane == 0)
+            regionCount[region] = wcount;
+        written += wcount;
+        __syncthreads();
+        dc.endTile(tileHist, t);
+    }
+    __syncthreads();
+    if (lane == 0 && written)
+        atomicAdd(counters, (unsigned long long)written);
+    dc.flush(digitHist, rangeHist);
 }
 
 // A tile's published count of one digit for the look-back: the flag in the
@@ -497,9 +608,10 @@ enum PassMode
 template <int MODE, class D>
 __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
     const uint64_t *__restrict__ in, const uint32_t *__restrict__ regionCount, uint64_t n,
-    uint32_t shift, uint32_t bits, uint32_t prevShift, const uint64_t *__restrict__ digitStart,
-    const uint32_t *__restrict__ tileOffset, const uint64_t *__restrict__ rangeMeta, D *desc,
-    unsigned int *tileCounter, unsigned long long *failed, uint64_t *__restrict__ out)
+    uint32_t shift, uint32_t bits, uint32_t prevShift, SortKey sk,
+    const uint64_t *__restrict__ digitStart, const uint32_t *__restrict__ tileOffset,
+    const uint64_t *__restrict__ rangeMeta, D *desc, unsigned int *tileCounter,
+    unsigned long long *failed, uint64_t *__restrict__ out)
 {
     using LB = LookBack<D>;
     constexpr bool kRegions = MODE == kPassOffsets || MODE == kPassLookBack;
@@ -584,11 +696,11 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
         const uint32_t lastW = (uint32_t)((cnt - 1) / kWaveHashes);
         const uint32_t lastI = (uint32_t)((cnt - 1) % kWaveHashes);
         if (w == 0 && lane == 0)
-            keyEnds[0] = (v[0] >> prevShift) & pmask;
+            keyEnds[0] = (sk(v[0]) >> prevShift) & pmask;
 #pragma unroll
         for (int r = 0; r < kGroupPer; ++r)
             if ((uint32_t)w == lastW && (uint32_t)r * 64 + lane == lastI)
-                keyEnds[1] = (v[r] >> prevShift) & pmask;
+                keyEnds[1] = (sk(v[r]) >> prevShift) & pmask;
         __syncthreads();
         key0 = keyEnds[0];
         twoKeys = keyEnds[1] - key0 <= 1;
@@ -599,8 +711,9 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
         for (int r = 0; r < kGroupPer; ++r)
         {
             const bool ok = (uint32_t)r * 64 + lane < have;
-            const uint32_t d = (uint32_t)(v[r] >> shift) & dmask;
-            const uint32_t sub = kStable ? (uint32_t)(((v[r] >> prevShift) & pmask) - key0) : 0u;
+            const uint64_t kv = sk(v[r]);
+            const uint32_t d = (uint32_t)(kv >> shift) & dmask;
+            const uint32_t sub = kStable ? (uint32_t)(((kv >> prevShift) & pmask) - key0) : 0u;
             rk[r] = ok ? atomicAdd(&hist[2 * d + sub], 1u) : 0u;
         }
     }
@@ -610,7 +723,7 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
         for (int r = 0; r < kGroupPer; ++r)
         {
             const bool ok = (uint32_t)r * 64 + lane < have;
-            const uint32_t d = (uint32_t)(v[r] >> shift) & dmask;
+            const uint32_t d = (uint32_t)(sk(v[r]) >> shift) & dmask;
             // lanes holding the same digit, from one ballot per digit bit
             uint64_t peers = __ballot(ok);
             for (uint32_t b = 0; b < bits; ++b)
@@ -686,11 +799,12 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
     for (int r = 0; r < kGroupPer; ++r)
         if ((uint32_t)r * 64 + lane < have)
         {
-            const uint32_t d = (uint32_t)(v[r] >> shift) & dmask;
+            const uint64_t kv = sk(v[r]);
+            const uint32_t d = (uint32_t)(kv >> shift) & dmask;
             uint32_t at = localStart[d] + rk[r];
             if (!atomics)
                 at += waveCnt[w][d];
-            else if (kStable && ((v[r] >> prevShift) & pmask) != key0)
+            else if (kStable && ((kv >> prevShift) & pmask) != key0)
                 at += hist[2 * d + 1];
             stage[at] = v[r];
         }
@@ -761,7 +875,7 @@ __global__ __launch_bounds__(kGroupBlock) void groupPassKernel(
     for (uint32_t e = tid; e < cnt; e += kGroupBlock)
     {
         const uint64_t x = stage[e];
-        const uint32_t d = (uint32_t)(x >> shift) & dmask;
+        const uint32_t d = (uint32_t)(sk(x) >> shift) & dmask;
         out[gbase[d] + (e - localStart[d])] = x;
     }
 }
@@ -825,11 +939,11 @@ __global__ __launch_bounds__(kBins) void tileOffsetKernel(const uint16_t *__rest
     }
 }
 
-// start[s] = the first index of the sorted hashes whose top sbits bits are
-// >= s (s = 0 .. nSlices), by binary search.
+// start[s] = the first index of the sorted hashes whose sort key's top
+// sbits bits are >= s (s = 0 .. nSlices), by binary search.
 __global__ __launch_bounds__(256) void sliceStartKernel(const uint64_t *__restrict__ hs, uint64_t n,
                                                         uint32_t sbits, uint64_t nSlices,
-                                                        uint64_t *start)
+                                                        SortKey sk, uint64_t *start)
 {
     const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s > nSlices)
@@ -843,7 +957,7 @@ __global__ __launch_bounds__(256) void sliceStartKernel(const uint64_t *__restri
     while (lo < hi)
     {
         const uint64_t mid = lo + (hi - lo) / 2;
-        if ((hs[mid] >> (64 - sbits)) < s)
+        if ((sk(hs[mid]) >> (64 - sbits)) < s)
             lo = mid + 1;
         else
             hi = mid;
@@ -852,7 +966,7 @@ __global__ __launch_bounds__(256) void sliceStartKernel(const uint64_t *__restri
 }
 // One workgroup per slice: loads the slice's keys into LDS (or EMPTY keys
 // when the table is fresh), counts the slice's k-mers there -- a bucket's 8
-// keys read as four 16 B words; the key's slot if it is there, else a
+// keys read as four 16 B words (stored swizzled against bank conflicts); the key's slot if it is there, else a
 // compare-and-swap into the bucket's first EMPTY slot (read again if another
 // key took it), then an LDS add to the slot's 32-bit count of this flush --
 // and writes the slice back, each count added to the table's (with writeAll,
@@ -861,115 +975,269 @@ __global__ __launch_bounds__(256) void sliceStartKernel(const uint64_t *__restri
 // appended to spill (counters[7]). counters[0] += keys stored. A flush
 // holds fewer than 2^32 k-mers, so no count of one flush overflows.
 __global__ __launch_bounds__(kSliceBlock) void countSliceKernel(
-    const uint64_t *__restrict__ hs, const uint64_t *__restrict__ start, uint64_t *tab,
-    uint64_t nBuckets, int fresh, int writeAll, uint64_t *spill, unsigned long long *counters)
+    const uint64_t *__restrict__ hs, const uint64_t *__restrict__ start, uint64_t nSlices,
+    uint64_t *tab, uint64_t nBuckets, int fresh, int writeAll, uint64_t *spill,
+    unsigned long long *counters)
 {
     __shared__ __align__(16) unsigned long long keys[kSliceBuckets * 8];
     __shared__ uint32_t cnt[kSliceBuckets * 8];
     __shared__ uint32_t stored;
-    const uint64_t s = blockIdx.x;
-    const uint64_t lo = start[s], hi = start[s + 1];
-    const uint64_t b0 = s << kSliceLog;
-    const uint32_t nb = (uint32_t)min<uint64_t>(kSliceBuckets, nBuckets - b0);
-    if (lo >= hi)
-    {
-        // no k-mers: an unfilled table still gets the slice's empty buckets
-        if (writeAll)
-            for (uint32_t q = threadIdx.x; q < nb * kCountWords; q += kSliceBlock)
-                tab[(b0 + q / kCountWords) * kCountWords + q % kCountWords] =
-                    q % kCountWords < 8 ? EMPTY : 0ull;
-        return;
-    }
     if (threadIdx.x == 0)
         stored = 0;
-    for (uint32_t q = threadIdx.x; q < nb * 8; q += kSliceBlock)
-    {
-        keys[q] = fresh ? EMPTY : tab[(b0 + q / 8) * kCountWords + q % 8];
-        cnt[q] = 0;
-    }
-    __syncthreads();
     uint32_t mine = 0;
     // kSliceAhead hashes per thread are loaded before any is counted, so
     // their loads are in flight together
     constexpr int kSliceAhead = 16;
-    for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)kSliceBlock * kSliceAhead)
+    do
     {
+        const uint64_t s = blockIdx.x;
+        if (s >= nSlices)
+            break;
+        const uint64_t lo = start[s], hi = start[s + 1];
+        const uint64_t b0 = s << kSliceLog;
+        const uint32_t nb = (uint32_t)min<uint64_t>(kSliceBuckets, nBuckets - b0);
+        if (lo >= hi)
+        {
+            // no k-mers: an unfilled table still gets the slice's empty buckets
+            if (writeAll)
+                for (uint32_t q = threadIdx.x; q < nb * kCountWords; q += kSliceBlock)
+                    tab[(b0 + q / kCountWords) * kCountWords + q % kCountWords] =
+                        q % kCountWords < 8 ? EMPTY : 0ull;
+            break;
+        }
         uint64_t hv[kSliceAhead];
+        // the first hashes are loaded before the LDS fill, so that the
+        // fill overlaps their latency
 #pragma unroll
         for (int a = 0; a < kSliceAhead; ++a)
         {
-            const uint64_t i = i0 + (uint64_t)a * kSliceBlock;
+            const uint64_t i = lo + threadIdx.x + (uint64_t)a * kSliceBlock;
             hv[a] = i < hi ? hs[i] : EMPTY;
         }
-#pragma unroll
-        for (int a = 0; a < kSliceAhead; ++a)
+        // a bucket's 16 B words are stored swizzled (word w of bucket b at
+        // w ^ (b mod 4)), so that lanes reading the same word of different
+        // buckets meet different LDS banks
+        for (uint32_t q = threadIdx.x; q < nb * 8; q += kSliceBlock)
         {
-            if (i0 + (uint64_t)a * kSliceBlock >= hi)
-                break;
-            const uint64_t h = hv[a];
-            const uint64_t key = keyUnmix(h);
-            const uint64_t home = ((h >> 32) * nBuckets) >> 32;
-            bool done = false;
-            for (uint32_t b = (uint32_t)(home - b0); b < nb && !done;)
+            const uint32_t b = q / 8, slot = q % 8;
+            keys[(b * 4 + ((slot >> 1) ^ (b & 3))) * 2 + (slot & 1)] =
+                fresh ? EMPTY : tab[(b0 + b) * kCountWords + slot];
+            cnt[q] = 0;
+        }
+        __syncthreads();
+        for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)kSliceBlock * kSliceAhead)
+        {
+            if (i0 != lo + threadIdx.x)
+#pragma unroll
+                for (int a = 0; a < kSliceAhead; ++a)
+                {
+                    const uint64_t i = i0 + (uint64_t)a * kSliceBlock;
+                    hv[a] = i < hi ? hs[i] : EMPTY;
+                }
+#pragma unroll
+            for (int a = 0; a < kSliceAhead; ++a)
             {
-                const ulonglong2 *kb = reinterpret_cast<const ulonglong2 *>(&keys[b * 8]);
-                uint64_t kk[8];
+                if (i0 + (uint64_t)a * kSliceBlock >= hi)
+                    break;
+                const uint64_t h = hv[a];
+                const uint64_t key = keyUnmix(h);
+                const uint64_t home = ((h >> 32) * nBuckets) >> 32;
+                bool done = false;
+                for (uint32_t b = (uint32_t)(home - b0); b < nb && !done;)
+                {
+                    const ulonglong2 *kb = reinterpret_cast<const ulonglong2 *>(keys) + b * 4;
+                    uint64_t kk[8];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                {
-                    const ulonglong2 x = kb[q];
-                    kk[2 * q] = x.x;
-                    kk[2 * q + 1] = x.y;
-                }
-                // keys take the first EMPTY slot and are never removed: a
-                // stored key sits before the first EMPTY one
-                int hit = 8, emp = 8;
-#pragma unroll
-                for (int q = 7; q >= 0; --q)
-                {
-                    hit = kk[q] == key ? q : hit;
-                    emp = kk[q] == EMPTY ? q : emp;
-                }
-                if (hit < emp)
-                {
-                    atomicAdd(&cnt[b * 8 + hit], 1u);
-                    done = true;
-                }
-                else if (emp < 8)
-                {
-                    const unsigned long long prev =
-                        atomicCAS(&keys[b * 8 + emp], EMPTY, (unsigned long long)key);
-                    if (prev == EMPTY || prev == key)
+                    for (int q = 0; q < 4; ++q)
                     {
-                        mine += prev == EMPTY;
-                        atomicAdd(&cnt[b * 8 + emp], 1u);
+                        const ulonglong2 x = kb[q ^ (b & 3)];
+                        kk[2 * q] = x.x;
+                        kk[2 * q + 1] = x.y;
+                    }
+                    // keys take the first EMPTY slot and are never removed:
+                    // a stored key sits before the first EMPTY one
+                    int hit = 8, emp = 8;
+#pragma unroll
+                    for (int q = 7; q >= 0; --q)
+                    {
+                        hit = kk[q] == key ? q : hit;
+                        emp = kk[q] == EMPTY ? q : emp;
+                    }
+                    if (hit < emp)
+                    {
+                        atomicAdd(&cnt[b * 8 + hit], 1u);
                         done = true;
                     }
-                    // else another key took the slot: this bucket is read again
+                    else if (emp < 8)
+                    {
+                        const unsigned long long prev = atomicCAS(
+                            &keys[(b * 4 + (((uint32_t)emp >> 1) ^ (b & 3))) * 2 + (emp & 1)],
+                            EMPTY, (unsigned long long)key);
+                        if (prev == EMPTY || prev == key)
+                        {
+                            mine += prev == EMPTY;
+                            atomicAdd(&cnt[b * 8 + emp], 1u);
+                            done = true;
+                        }
+                        // else another key took the slot: this bucket is read again
+                    }
+                    else
+                        ++b;
                 }
-                else
-                    ++b;
+                if (!done)
+                    spill[atomicAdd(counters + 7, 1ull)] = h;
             }
-            if (!done)
-                spill[atomicAdd(counters + 7, 1ull)] = h;
+        }
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < nb * kCountWords; q += kSliceBlock)
+        {
+            const uint32_t b = q / kCountWords, w = q % kCountWords;
+            uint64_t *dst = tab + (b0 + b) * kCountWords + w;
+            if (w < 8)
+                *dst = keys[(b * 4 + ((w >> 1) ^ (b & 3))) * 2 + (w & 1)];
+            else if (fresh)
+                *dst = cnt[b * 8 + w - 8];
+            else if (const uint32_t c = cnt[b * 8 + w - 8])
+                *dst += c;
+        }
+    } while (false);
+    if (mine)
+        atomicAdd(&stored, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && stored)
+        atomicAdd(counters + 0, (unsigned long long)stored);
+}
+
+// A table build's slice: buckets [s << SLICE_LOG, +2^SLICE_LOG) of a
+// whole-k-mer table of S key slots per bucket (32 KB), filled in LDS from
+// the slice's sorted hashes and written out whole, empty buckets too. A
+// key is placed as insertOne places it: from its home bucket on, in the
+// first EMPTY slot (claimed by compare-and-swap; a bucket is read again
+// when another key took it), stored once. A key whose walk passes the
+// slice's end is appended to spill (spillCount) for the global insert.
+// counters: [0] += keys stored, [2] = max displacement (as insertKernel).
+template <int S, int SLICE_LOG>
+__global__ __launch_bounds__(256) void tableSliceKernel(const uint64_t *__restrict__ hs,
+                                                        const uint64_t *__restrict__ start,
+                                                        uint64_t nSlices, uint64_t *slots,
+                                                        uint64_t nBuckets, uint64_t *spill,
+                                                        unsigned long long *spillCount,
+                                                        unsigned long long *counters)
+{
+    constexpr uint32_t kB = 1u << SLICE_LOG;
+    constexpr uint32_t W = S / 2; // 16 B words per bucket
+    __shared__ __align__(16) unsigned long long keys[kB * S];
+    __shared__ uint32_t stored, widest;
+    if (threadIdx.x == 0)
+        stored = widest = 0;
+    uint32_t mine = 0, disp = 0;
+    constexpr int kAhead = 8;
+    if (blockIdx.x < nSlices)
+    {
+        const uint64_t s = blockIdx.x;
+        const uint64_t b0 = s << SLICE_LOG;
+        const uint32_t nb = (uint32_t)min<uint64_t>(kB, nBuckets - b0);
+        const uint64_t lo = start[s], hi = start[s + 1];
+        uint64_t hv[kAhead];
+        // the first hashes are loaded before the LDS fill, so that the
+        // fill overlaps their latency
+#pragma unroll
+        for (int a = 0; a < kAhead; ++a)
+        {
+            const uint64_t i = lo + threadIdx.x + (uint64_t)a * blockDim.x;
+            hv[a] = i < hi ? hs[i] : 0;
+        }
+        for (uint32_t q = threadIdx.x; q < nb * S; q += blockDim.x)
+            keys[q] = EMPTY;
+        __syncthreads();
+        for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)blockDim.x * kAhead)
+        {
+            if (i0 != lo + threadIdx.x)
+#pragma unroll
+                for (int a = 0; a < kAhead; ++a)
+                {
+                    const uint64_t i = i0 + (uint64_t)a * blockDim.x;
+                    hv[a] = i < hi ? hs[i] : 0;
+                }
+#pragma unroll
+            for (int a = 0; a < kAhead; ++a)
+            {
+                if (i0 + (uint64_t)a * blockDim.x >= hi)
+                    break;
+                const uint64_t h = hv[a];
+                const uint64_t key = keyUnmix(h);
+                const uint32_t home = (uint32_t)((((h >> 32) * nBuckets) >> 32) - b0);
+                bool done = false;
+                for (uint32_t b = home; b < nb && !done;)
+                {
+                    // a bucket's 16 B words are stored swizzled (word w at
+                    // w ^ (b mod W)), so that lanes reading the same word of
+                    // different buckets meet different LDS banks
+                    const ulonglong2 *kb = reinterpret_cast<const ulonglong2 *>(keys) + b * W;
+                    uint64_t kk[S];
+#pragma unroll
+                    for (int q = 0; q < (int)W; ++q)
+                    {
+                        const ulonglong2 x = kb[q ^ (b & (W - 1))];
+                        kk[2 * q] = x.x;
+                        kk[2 * q + 1] = x.y;
+                    }
+                    // keys take the first EMPTY slot and are never removed: a
+                    // stored key sits before the first EMPTY one
+                    int hit = S, emp = S;
+#pragma unroll
+                    for (int q = S - 1; q >= 0; --q)
+                    {
+                        hit = kk[q] == key ? q : hit;
+                        emp = kk[q] == EMPTY ? q : emp;
+                    }
+                    if (hit < emp)
+                        done = true;
+                    else if (emp < S)
+                    {
+                        const uint32_t at =
+                            (b * W + (((uint32_t)emp >> 1) ^ (b & (W - 1)))) * 2 + (emp & 1);
+                        const unsigned long long prev =
+                            atomicCAS(&keys[at], EMPTY, (unsigned long long)key);
+                        if (prev == EMPTY)
+                        {
+                            mine += 1;
+                            disp = max(disp, b - home);
+                            done = true;
+                        }
+                        else if (prev == key)
+                            done = true;
+                        // else another key took the slot: this bucket is read again
+                    }
+                    else
+                        ++b;
+                }
+                if (!done)
+                    spill[atomicAdd(spillCount, 1ull)] = key;
+            }
+        }
+        __syncthreads();
+        // 16 B stores of the slice's keys, unswizzled
+        const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(keys);
+        ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(slots + b0 * S);
+        for (uint32_t q = threadIdx.x; q < nb * W; q += blockDim.x)
+        {
+            const uint32_t b = q / W;
+            dst[q] = src[b * W + ((q % W) ^ (b & (W - 1)))];
         }
     }
     if (mine)
         atomicAdd(&stored, mine);
+    if (disp)
+        atomicMax(&widest, disp);
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < nb * kCountWords; q += kSliceBlock)
+    if (threadIdx.x == 0)
     {
-        const uint32_t b = q / kCountWords, w = q % kCountWords;
-        uint64_t *dst = tab + (b0 + b) * kCountWords + w;
-        if (w < 8)
-            *dst = keys[b * 8 + w];
-        else if (fresh)
-            *dst = cnt[b * 8 + w - 8];
-        else if (const uint32_t c = cnt[b * 8 + w - 8])
-            *dst += c;
+        if (stored)
+            atomicAdd(counters + 0, (unsigned long long)stored);
+        if (widest)
+            atomicMax(counters + 2, (unsigned long long)widest);
     }
-    if (threadIdx.x == 0 && stored)
-        atomicAdd(counters + 0, (unsigned long long)stored);
 }
 
 // The spilled k-mers (h form), counted in the global table.
@@ -1131,6 +1399,293 @@ uint32_t gridFor(uint64_t items, uint64_t perBlock, int device)
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)cuCount(device) * 8));
 }
 
+// The buffers and steps of one sort by slice: a hashing kernel
+// (countHashKernel, keyHashKernel) fills the regions and the counts, then
+// sort() runs the passes of groupPassKernel and finds the slices' starts.
+// A k-mer counter and a table build each keep one.
+struct SliceSorter
+{
+    int device = 0;
+    uint32_t digitBits = kMaxDigitBits;
+    bool wideLookBack = false;            // 64-bit look-back words always
+    unsigned long long *dDigit = nullptr; // digit histograms (rows 1, 2; row 0 from dRange)
+    unsigned long long *dRange = nullptr; // digit 0 by range of digit 1
+    uint32_t *dRegion = nullptr;          // the fill of every hashing region
+    uint16_t *dTileHist = nullptr;        // every hashing tile's counts of digits 1 and 2
+    uint64_t tileCap = 0;                 // hashing tiles both hold
+    uint64_t *dDigitStart = nullptr;      // the digits' starts (kMaxDigits x (kBins + 1))
+    uint64_t *dRangeMeta = nullptr;       // the last pass's ranges (see groupPassKernel)
+    uint32_t *dGroupSum = nullptr;        // the first pass's per-group sums, then starts
+    uint32_t *dTileOffset = nullptr;      // the first pass's per-tile places
+    uint64_t tileOffsetCap = 0;
+    void *dDesc = nullptr; // look-back words
+    uint64_t descBytes = 0;
+    unsigned int *dTileCounter = nullptr;
+    uint64_t *dStart = nullptr; // the slices' starts
+    uint64_t startCap = 0;
+    std::vector<unsigned long long> hDigit, hRange;
+    std::vector<uint64_t> hDigitStart, hRangeMeta;
+
+    // Bits of digit 1 that cut the last pass's input into ranges.
+    uint32_t rangeBits() const { return std::min<uint32_t>(3, digitBits); }
+
+    // The histograms and small buffers, cleared (on the caller's device).
+    int init(hipStream_t s)
+    {
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dDigit),
+                               kMaxDigits * kBins * sizeof(unsigned long long)));
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dRange),
+                               kRanges * kBins * sizeof(unsigned long long)));
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dDigitStart),
+                               kMaxDigits * (kBins + 1) * sizeof(uint64_t)));
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dRangeMeta),
+                               (2 * (kRanges + 1) + kRanges * kBins) * sizeof(uint64_t)));
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileCounter),
+                               kRanges * sizeof(unsigned int)));
+        return clearCounts(s);
+    }
+
+    int clearCounts(hipStream_t s)
+    {
+        HIPIBS_CHECK(hipMemsetAsync(dDigit, 0, kMaxDigits * kBins * sizeof(unsigned long long), s));
+        HIPIBS_CHECK(hipMemsetAsync(dRange, 0, kRanges * kBins * sizeof(unsigned long long), s));
+        return 0;
+    }
+
+    // Regions and per-tile counts for `tiles` hashing tiles.
+    int reserveTiles(uint64_t tiles)
+    {
+        if (tiles <= tileCap)
+            return 0;
+        if (dRegion)
+            HIPIBS_CHECK(hipFree(dRegion));
+        if (dTileHist)
+            HIPIBS_CHECK(hipFree(dTileHist));
+        dRegion = nullptr;
+        dTileHist = nullptr;
+        tileCap = 0;
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dRegion),
+                               tiles * kHashWaves * sizeof(uint32_t)));
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileHist),
+                               tiles * kTileDigits * kBins * sizeof(uint16_t)));
+        tileCap = tiles;
+        return 0;
+    }
+
+    void release()
+    {
+        for (void *p : {(void *)dDigit, (void *)dRange, (void *)dRegion, (void *)dTileHist,
+                        (void *)dDigitStart, (void *)dRangeMeta, (void *)dGroupSum,
+                        (void *)dTileOffset, dDesc, (void *)dTileCounter, (void *)dStart})
+            if (p)
+                (void)hipFree(p);
+        *this = SliceSorter();
+    }
+
+    // Sorts the nHash hashes that the hashing kernel wrote in tilesFirst
+    // tiles of src by the top sbits bits of sk (passes over digitBits-bit
+    // digits, ping-ponging between src and dst); *sorted receives the
+    // buffer holding the result, dStart the nSlices + 1 slice starts. Every
+    // launch goes through timed(stage 1, launch); a look-back that timed out
+    // sets *failed (device). `who` names the caller in errors.
+    template <class Timed>
+    int sort(uint64_t *src, uint64_t *dst, uint64_t nHash, uint64_t tilesFirst, uint32_t sbits,
+             SortKey sk, unsigned long long *failed, hipStream_t stream, Timed &&timed,
+             uint64_t **sorted, const std::string &who)
+    {
+        int rc = 0;
+        const uint64_t nSlices = sbits ? (1ull << sbits) : 1;
+        const uint32_t passes = std::max<uint32_t>(1, (sbits + digitBits - 1) / digitBits);
+        if (passes > (uint32_t)kMaxDigits)
+            return setError(HIPIBS_E_NOMEM, who + ": the table needs more than " +
+                                                std::to_string(kMaxDigits * digitBits) +
+                                                " bits of slice index");
+        if (startCap < nSlices + 1)
+        {
+            if (dStart)
+                HIPIBS_CHECK(hipFree(dStart));
+            dStart = nullptr;
+            startCap = 0;
+            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dStart), (nSlices + 1) * 8));
+            startCap = nSlices + 1;
+        }
+        // every digit's start from its histogram
+        const uint32_t bins = 1u << digitBits;
+        hDigit.resize(kMaxDigits * kBins);
+        hDigitStart.assign(kMaxDigits * (kBins + 1), 0);
+        HIPIBS_CHECK(hipMemcpy(hDigit.data(), dDigit, hDigit.size() * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost));
+        hRange.resize(kRanges * kBins);
+        HIPIBS_CHECK(hipMemcpy(hRange.data(), dRange, hRange.size() * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost));
+        for (uint32_t d = 0; d < bins; ++d)
+        {
+            hDigit[d] = 0;
+            for (int x = 0; x < kRanges; ++x)
+                hDigit[d] += hRange[x * kBins + d];
+        }
+        uint64_t widest = 0;
+        for (int c = 0; c < kMaxDigits; ++c)
+        {
+            uint64_t acc = 0;
+            for (uint32_t d = 0; d < bins; ++d)
+            {
+                hDigitStart[c * (kBins + 1) + d] = acc;
+                acc += hDigit[c * kBins + d];
+                widest = std::max<uint64_t>(widest, hDigit[c * kBins + d]);
+            }
+            hDigitStart[c * (kBins + 1) + bins] = acc;
+            if (acc != nHash)
+                return setError(HIPIBS_E_HIP, who + ": the slice sort lost hashes");
+        }
+        HIPIBS_CHECK(hipMemcpy(dDigitStart, hDigitStart.data(),
+                               hDigitStart.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+        // look-back words: 32 bits while no digit holds 2^30 hashes
+        const bool narrow = widest < (1ull << 30) && !wideLookBack;
+        const uint64_t tilesDense = (nHash + kGroupTile - 1) / kGroupTile;
+        const uint64_t tilesMax = std::max(tilesFirst, tilesDense + kRanges);
+        const uint64_t wordBytes = narrow ? 4 : 8;
+        if (descBytes < tilesMax * kBins * wordBytes)
+        {
+            if (dDesc)
+                HIPIBS_CHECK(hipFree(dDesc));
+            dDesc = nullptr;
+            descBytes = 0;
+            HIPIBS_CHECK(hipMalloc(&dDesc, tilesMax * kBins * 8));
+            descBytes = tilesMax * kBins * 8;
+        }
+        // the passes, least significant digit first; then the slices'
+        // starts. The first pass over two or three digits takes its tiles'
+        // places from the hashing kernel's per-tile counts; every other
+        // pass finds them by look-back.
+        const uint32_t prevShift = 64 - passes * digitBits;
+        for (uint32_t q = 0; q < passes; ++q)
+        {
+            const uint32_t c = passes - 1 - q;
+            const uint32_t shift = 64 - (c + 1) * digitBits;
+            const uint64_t *starts = dDigitStart + c * (kBins + 1);
+            if (q == 0 && c > 0)
+            {
+                const uint64_t groups = (tilesFirst + kScanTiles - 1) / kScanTiles;
+                if (tileOffsetCap < tilesFirst)
+                {
+                    if (dGroupSum)
+                        HIPIBS_CHECK(hipFree(dGroupSum));
+                    if (dTileOffset)
+                        HIPIBS_CHECK(hipFree(dTileOffset));
+                    dGroupSum = nullptr;
+                    dTileOffset = nullptr;
+                    tileOffsetCap = 0;
+                    HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dGroupSum),
+                                           (groups + 1) * kBins * sizeof(uint32_t)));
+                    HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileOffset),
+                                           tilesFirst * kBins * sizeof(uint32_t)));
+                    tileOffsetCap = tilesFirst;
+                }
+                if ((rc = timed(1, [&] {
+                         tileSumKernel<<<(uint32_t)groups, kBins, 0, stream>>>(
+                             dTileHist, tilesFirst, c - 1, bins, dGroupSum);
+                         groupScanKernel<<<1, kBins, 0, stream>>>(dGroupSum, groups, starts, bins);
+                         tileOffsetKernel<<<(uint32_t)groups, kBins, 0, stream>>>(
+                             dTileHist, tilesFirst, c - 1, bins, dGroupSum, dTileOffset);
+                         groupPassKernel<kPassOffsets, uint32_t>
+                             <<<(uint32_t)tilesFirst, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, sk, starts,
+                                 dTileOffset, nullptr, static_cast<uint32_t *>(dDesc), dTileCounter,
+                                 failed, dst);
+                     })))
+                    return rc;
+            }
+            else if (q > 0 && c == 0)
+            {
+                // the last pass in kRanges ranges: their input starts (the
+                // counts by range are ordered as the input is, by digit 1),
+                // first tiles, and output bases (digit 0's start + the
+                // earlier ranges' counts of it)
+                const uint32_t nr = 1u << rangeBits();
+                hRangeMeta.assign(2 * (kRanges + 1) + kRanges * kBins, 0);
+                uint64_t at = 0, tile = 0, longest = 0;
+                for (int x = 0; x <= kRanges; ++x)
+                {
+                    hRangeMeta[x] = at;
+                    hRangeMeta[kRanges + 1 + x] = tile;
+                    if (x == kRanges)
+                        break;
+                    uint64_t cnt = 0;
+                    for (uint32_t d = 0; d < bins; ++d)
+                    {
+                        hRangeMeta[2 * (kRanges + 1) + x * kBins + d] =
+                            x ? hRangeMeta[2 * (kRanges + 1) + (x - 1) * kBins + d] +
+                                    hRange[(x - 1) * kBins + d]
+                              : hDigitStart[d];
+                        cnt += hRange[x * kBins + d];
+                    }
+                    if ((uint32_t)x >= nr && cnt)
+                        return setError(HIPIBS_E_HIP, who + ": a hash outside the ranges");
+                    const uint64_t tiles = (cnt + kGroupTile - 1) / kGroupTile;
+                    at += cnt;
+                    tile += tiles;
+                    longest = std::max(longest, tiles);
+                }
+                if (at != nHash)
+                    return setError(HIPIBS_E_HIP, who + ": the slice sort lost hashes");
+                HIPIBS_CHECK(hipMemcpy(dRangeMeta, hRangeMeta.data(),
+                                       hRangeMeta.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+                HIPIBS_CHECK(hipMemsetAsync(dDesc, 0, tile * kBins * wordBytes, stream));
+                HIPIBS_CHECK(hipMemsetAsync(dTileCounter, 0, kRanges * sizeof(unsigned int), stream));
+                if ((rc = timed(1, [&] {
+                         const dim3 grid((uint32_t)(kRanges * longest));
+                         if (narrow)
+                             groupPassKernel<kPassRanged, uint32_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, sk, starts,
+                                 nullptr, dRangeMeta, static_cast<uint32_t *>(dDesc), dTileCounter,
+                                 failed, dst);
+                         else
+                             groupPassKernel<kPassRanged, uint64_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, sk, starts,
+                                 nullptr, dRangeMeta, static_cast<uint64_t *>(dDesc), dTileCounter,
+                                 failed, dst);
+                     })))
+                    return rc;
+            }
+            else
+            {
+                const uint64_t tiles = q == 0 ? tilesFirst : tilesDense;
+                HIPIBS_CHECK(hipMemsetAsync(dDesc, 0, tiles * kBins * wordBytes, stream));
+                HIPIBS_CHECK(hipMemsetAsync(dTileCounter, 0, 4, stream));
+                if ((rc = timed(1, [&] {
+                         const dim3 grid((uint32_t)tiles);
+                         if (q == 0 && narrow)
+                             groupPassKernel<kPassLookBack, uint32_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, sk, starts, nullptr,
+                                 nullptr, static_cast<uint32_t *>(dDesc), dTileCounter, failed, dst);
+                         else if (q == 0)
+                             groupPassKernel<kPassLookBack, uint64_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, sk, starts, nullptr,
+                                 nullptr, static_cast<uint64_t *>(dDesc), dTileCounter, failed, dst);
+                         else if (narrow)
+                             groupPassKernel<kPassStable, uint32_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, sk, starts, nullptr,
+                                 nullptr, static_cast<uint32_t *>(dDesc), dTileCounter, failed, dst);
+                         else
+                             groupPassKernel<kPassStable, uint64_t><<<grid, kGroupBlock, 0, stream>>>(
+                                 src, dRegion, nHash, shift, digitBits, prevShift, sk, starts, nullptr,
+                                 nullptr, static_cast<uint64_t *>(dDesc), dTileCounter, failed, dst);
+                     })))
+                    return rc;
+            }
+            std::swap(src, dst);
+        }
+        if ((rc = timed(1, [&] {
+                 sliceStartKernel<<<(uint32_t)((nSlices + 1 + 255) / 256), 256, 0, stream>>>(
+                     src, nHash, sbits, nSlices, sk, dStart);
+             })))
+            return rc;
+        *sorted = src;
+        return 0;
+    }
+};
+
 } // namespace
 } // namespace hipibs
 
@@ -1189,6 +1744,52 @@ void returnHash(int device, uint64_t *p[2], uint64_t cap)
     gHashCache.p[1] = p[1];
     gHashCache.cap = cap;
 }
+
+// The sorter of table builds is kept for the process's next build (its
+// per-tile buffers grow with the largest build), one per process.
+struct BuildSorterCache
+{
+    std::mutex m;
+    int device = -1;
+    SliceSorter *so = nullptr;
+} gBuildSorter;
+
+SliceSorter *takeBuildSorter(int device)
+{
+    std::lock_guard<std::mutex> g(gBuildSorter.m);
+    if (!gBuildSorter.so || gBuildSorter.device != device)
+        return nullptr;
+    SliceSorter *so = gBuildSorter.so;
+    gBuildSorter.so = nullptr;
+    return so;
+}
+
+void returnBuildSorter(int device, SliceSorter *so)
+{
+    std::lock_guard<std::mutex> g(gBuildSorter.m);
+    if (gBuildSorter.so)
+    {
+        so->release();
+        delete so;
+        return;
+    }
+    gBuildSorter.device = device;
+    gBuildSorter.so = so;
+}
+
+void releaseBuildSorter()
+{
+    std::lock_guard<std::mutex> g(gBuildSorter.m);
+    if (!gBuildSorter.so)
+        return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(gBuildSorter.device);
+    gBuildSorter.so->release();
+    delete gBuildSorter.so;
+    gBuildSorter.so = nullptr;
+    (void)hipSetDevice(cur);
+}
 } // namespace
 
 struct hipibs_counter : ReadSink
@@ -1214,30 +1815,13 @@ struct hipibs_counter : ReadSink
     uint64_t hashBudget = 0; // most entries one flush holds
     uint64_t nSlots = 0;     // slots the open flush's hashing tiles own in dHash[0]
     uint64_t nHash = 0;      // hashes of the open flush (after a sync)
-    uint32_t *dRegion = nullptr;           // the fill of every hashing region
-    uint16_t *dTileHist = nullptr;         // every hashing tile's counts of digits 1 and 2
-    uint32_t *dGroupSum = nullptr;         // the first pass's per-group sums, then starts
-    uint32_t *dTileOffset = nullptr;       // the first pass's per-tile places
-    uint64_t tileOffsetCap = 0;            // tiles
-    uint32_t *dHll = nullptr;              // the flush's distinct-count sketch
-    unsigned long long *dDigit = nullptr;  // the flush's digit histograms
-    unsigned long long *dRange = nullptr;  // the flush's digit-0 counts by range
-    uint64_t *dRangeMeta = nullptr;        // the last pass's ranges (see groupPassKernel)
-    std::vector<unsigned long long> hRange;
-    std::vector<uint64_t> hRangeMeta;
-    uint64_t *dDigitStart = nullptr;       // their exclusive sums (kMaxDigits x (kBins + 1))
-    std::vector<unsigned long long> hDigit;
-    std::vector<uint64_t> hDigitStart;
-    void *dDesc = nullptr; // the sort's look-back words
-    uint64_t descBytes = 0;
-    unsigned int *dTileCounter = nullptr;
-    // digit width of the slice sort (environment HIPIBS_COUNT_DIGIT_BITS,
-    // 1..9: narrower digits take more passes, which the tests use)
-    uint32_t digitBits = kMaxDigitBits;
-    bool wideLookBack = false; // 64-bit look-back words always (HIPIBS_COUNT_LOOKBACK_WIDE=1)
-    uint64_t *dStart = nullptr;
+    uint32_t *dHll = nullptr; // the flush's distinct-count sketch
+    // the sort by slice (its digit width from the environment
+    // HIPIBS_SORT_DIGIT_BITS, 1..9: narrower digits take more passes, which
+    // the tests use; HIPIBS_SORT_LOOKBACK_WIDE=1 forces 64-bit look-back
+    // words)
+    SliceSorter sorter;
     bool tableUnfilled = false; // dTab allocated but not yet written
-    uint64_t startCap = 0;
     char *hText[2] = {nullptr, nullptr};     // pinned chunks (host text only)
     uint8_t *dText = nullptr;
     // device time per stage (0 hashing or, k > 32, counting; 1 grouping by
@@ -1389,28 +1973,17 @@ struct hipibs_counter : ReadSink
         {
             HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dHll), (4u << kHllLog)));
             HIPIBS_CHECK(hipMemsetAsync(dHll, 0, 4u << kHllLog, stream));
-            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dDigit),
-                                   kMaxDigits * kBins * sizeof(unsigned long long)));
-            HIPIBS_CHECK(hipMemsetAsync(dDigit, 0, kMaxDigits * kBins * sizeof(unsigned long long),
-                                        stream));
-            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dRange),
-                                   kRanges * kBins * sizeof(unsigned long long)));
-            HIPIBS_CHECK(hipMemsetAsync(dRange, 0, kRanges * kBins * sizeof(unsigned long long),
-                                        stream));
-            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dRangeMeta),
-                                   (2 * (kRanges + 1) + kRanges * kBins) * sizeof(uint64_t)));
-            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dRegion),
-                                   (hashBudget / kRegion + 1) * sizeof(uint32_t)));
-            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileHist),
-                                   (hashBudget / kTilePositions + 1) * kTileDigits * kBins *
-                                       sizeof(uint16_t)));
+            if ((rc = sorter.init(stream)) ||
+                (rc = sorter.reserveTiles(hashBudget / kTilePositions + 1)))
+                return rc;
         }
         const uint64_t tileBase = nSlots / kTilePositions;
         if ((rc = timed(0, [&] {
                  countHashKernel<<<gridFor(n - k + 1, kTilePositions, device), kCountBlock, 0,
-                                   stream>>>(text, n, skip, k, tileBase, digitBits, dHash[0],
-                                             dRegion, dTileHist, rangeBits(), dHll, dDigit,
-                                             dRange, dCounters);
+                                   stream>>>(text, n, skip, k, tileBase, sorter.digitBits,
+                                             dHash[0], sorter.dRegion, sorter.dTileHist,
+                                             sorter.rangeBits(), dHll, sorter.dDigit,
+                                             sorter.dRange, dCounters);
              })))
             return rc;
         HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 9 * sizeof(unsigned long long),
@@ -1437,9 +2010,6 @@ struct hipibs_counter : ReadSink
         hashBudget = std::max<uint64_t>(hashBudget, chunkSlots);
         return 0;
     }
-
-    // Bits of digit 1 that cut the last pass's input into ranges.
-    uint32_t rangeBits() const { return std::min<uint32_t>(3, digitBits); }
 
     // Grows the hash buffers to hold `need` entries (at most the budget),
     // keeping the open flush's; called with the stream synchronised.
@@ -1522,207 +2092,20 @@ struct hipibs_counter : ReadSink
         const int lg = 63 - __builtin_clzll(nBuckets); // a power of two
         const uint32_t sbits = lg > kSliceLog ? (uint32_t)(lg - kSliceLog) : 0u;
         const uint64_t nSlices = sbits ? (1ull << sbits) : 1;
-        const uint32_t passes = std::max<uint32_t>(1, (sbits + digitBits - 1) / digitBits);
-        if (passes > (uint32_t)kMaxDigits)
-            return setError(HIPIBS_E_NOMEM, "k-mer counter: the table needs more than " +
-                                                std::to_string(kMaxDigits * digitBits) +
-                                                " bits of slice index");
-        if (startCap < nSlices + 1)
-        {
-            if (dStart)
-                HIPIBS_CHECK(hipFree(dStart));
-            dStart = nullptr;
-            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dStart), (nSlices + 1) * 8));
-            startCap = nSlices + 1;
-        }
-        // every digit's start from its histogram
-        const uint32_t bins = 1u << digitBits;
-        hDigit.resize(kMaxDigits * kBins);
-        hDigitStart.assign(kMaxDigits * (kBins + 1), 0);
-        HIPIBS_CHECK(hipMemcpy(hDigit.data(), dDigit, hDigit.size() * sizeof(unsigned long long),
-                               hipMemcpyDeviceToHost));
-        hRange.resize(kRanges * kBins);
-        HIPIBS_CHECK(hipMemcpy(hRange.data(), dRange, hRange.size() * sizeof(unsigned long long),
-                               hipMemcpyDeviceToHost));
-        for (uint32_t d = 0; d < bins; ++d)
-        {
-            hDigit[d] = 0;
-            for (int x = 0; x < kRanges; ++x)
-                hDigit[d] += hRange[x * kBins + d];
-        }
-        uint64_t widest = 0;
-        for (int c = 0; c < kMaxDigits; ++c)
-        {
-            uint64_t acc = 0;
-            for (uint32_t d = 0; d < bins; ++d)
-            {
-                hDigitStart[c * (kBins + 1) + d] = acc;
-                acc += hDigit[c * kBins + d];
-                widest = std::max<uint64_t>(widest, hDigit[c * kBins + d]);
-            }
-            hDigitStart[c * (kBins + 1) + bins] = acc;
-            if (acc != nHash)
-                return setError(HIPIBS_E_HIP, "k-mer counter: slice grouping lost k-mers");
-        }
-        if (!dDigitStart)
-            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dDigitStart),
-                                   hDigitStart.size() * sizeof(uint64_t)));
-        HIPIBS_CHECK(hipMemcpy(dDigitStart, hDigitStart.data(),
-                               hDigitStart.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
-        // look-back words: 32 bits while no digit holds 2^30 hashes
-        const bool narrow = widest < (1ull << 30) && !wideLookBack;
-        const uint64_t tilesFirst = nSlots / kTilePositions;
-        const uint64_t tilesDense = (nHash + kGroupTile - 1) / kGroupTile;
-        const uint64_t tilesMax = std::max(tilesFirst, tilesDense + kRanges);
-        const uint64_t wordBytes = narrow ? 4 : 8;
-        if (descBytes < tilesMax * kBins * wordBytes)
-        {
-            if (dDesc)
-                HIPIBS_CHECK(hipFree(dDesc));
-            dDesc = nullptr;
-            descBytes = 0;
-            HIPIBS_CHECK(hipMalloc(&dDesc, tilesMax * kBins * 8));
-            descBytes = tilesMax * kBins * 8;
-        }
-        if (!dTileCounter)
-            HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileCounter),
-                                   kRanges * sizeof(unsigned int)));
-        // the passes, least significant digit first, ping-ponging between
-        // the two buffers; then the slices' starts. The first pass over two
-        // or three digits takes its tiles' places from the hashing kernel's
-        // per-tile counts; every other pass finds them by look-back.
-        uint64_t *src = dHash[0], *dst = dHash[1];
-        const uint32_t prevShift = 64 - passes * digitBits;
-        auto *failed = dCounters + 10;
-        for (uint32_t q = 0; q < passes; ++q)
-        {
-            const uint32_t c = passes - 1 - q;
-            const uint32_t shift = 64 - (c + 1) * digitBits;
-            const uint64_t *starts = dDigitStart + c * (kBins + 1);
-            if (q == 0 && c > 0)
-            {
-                const uint64_t groups = (tilesFirst + kScanTiles - 1) / kScanTiles;
-                if (tileOffsetCap < tilesFirst)
-                {
-                    if (dGroupSum)
-                        HIPIBS_CHECK(hipFree(dGroupSum));
-                    if (dTileOffset)
-                        HIPIBS_CHECK(hipFree(dTileOffset));
-                    dGroupSum = nullptr;
-                    dTileOffset = nullptr;
-                    tileOffsetCap = 0;
-                    HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dGroupSum),
-                                           (groups + 1) * kBins * sizeof(uint32_t)));
-                    HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileOffset),
-                                           tilesFirst * kBins * sizeof(uint32_t)));
-                    tileOffsetCap = tilesFirst;
-                }
-                if ((rc = timed(1, [&] {
-                         tileSumKernel<<<(uint32_t)groups, kBins, 0, stream>>>(
-                             dTileHist, tilesFirst, c - 1, bins, dGroupSum);
-                         groupScanKernel<<<1, kBins, 0, stream>>>(dGroupSum, groups, starts, bins);
-                         tileOffsetKernel<<<(uint32_t)groups, kBins, 0, stream>>>(
-                             dTileHist, tilesFirst, c - 1, bins, dGroupSum, dTileOffset);
-                         groupPassKernel<kPassOffsets, uint32_t>
-                             <<<(uint32_t)tilesFirst, kGroupBlock, 0, stream>>>(
-                                 src, dRegion, nHash, shift, digitBits, prevShift, starts,
-                                 dTileOffset, nullptr, static_cast<uint32_t *>(dDesc), dTileCounter,
-                                 failed, dst);
-                     })))
-                    return rc;
-            }
-            else if (q > 0 && c == 0)
-            {
-                // the last pass in kRanges ranges: their input starts (the
-                // counts by range are ordered as the input is, by digit 1),
-                // first tiles, and output bases (digit 0's start + the
-                // earlier ranges' counts of it)
-                const uint32_t nr = 1u << rangeBits();
-                hRangeMeta.assign(2 * (kRanges + 1) + kRanges * kBins, 0);
-                uint64_t at = 0, tile = 0, longest = 0;
-                for (int x = 0; x <= kRanges; ++x)
-                {
-                    hRangeMeta[x] = at;
-                    hRangeMeta[kRanges + 1 + x] = tile;
-                    if (x == kRanges)
-                        break;
-                    uint64_t cnt = 0;
-                    for (uint32_t d = 0; d < bins; ++d)
-                    {
-                        hRangeMeta[2 * (kRanges + 1) + x * kBins + d] =
-                            hDigitStart[d] + (x ? hRangeMeta[2 * (kRanges + 1) + (x - 1) * kBins + d] -
-                                                      hDigitStart[d] + hRange[(x - 1) * kBins + d]
-                                                : 0);
-                        cnt += hRange[x * kBins + d];
-                    }
-                    if ((uint32_t)x >= nr && cnt)
-                        return setError(HIPIBS_E_HIP, "k-mer counter: a k-mer outside the ranges");
-                    const uint64_t tiles = (cnt + kGroupTile - 1) / kGroupTile;
-                    at += cnt;
-                    tile += tiles;
-                    longest = std::max(longest, tiles);
-                }
-                if (at != nHash)
-                    return setError(HIPIBS_E_HIP, "k-mer counter: slice grouping lost k-mers");
-                HIPIBS_CHECK(hipMemcpy(dRangeMeta, hRangeMeta.data(),
-                                       hRangeMeta.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
-                HIPIBS_CHECK(hipMemsetAsync(dDesc, 0, tile * kBins * wordBytes, stream));
-                HIPIBS_CHECK(hipMemsetAsync(dTileCounter, 0, kRanges * sizeof(unsigned int), stream));
-                if ((rc = timed(1, [&] {
-                         const dim3 grid((uint32_t)(kRanges * longest));
-                         if (narrow)
-                             groupPassKernel<kPassRanged, uint32_t><<<grid, kGroupBlock, 0, stream>>>(
-                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
-                                 dRangeMeta, static_cast<uint32_t *>(dDesc), dTileCounter, failed,
-                                 dst);
-                         else
-                             groupPassKernel<kPassRanged, uint64_t><<<grid, kGroupBlock, 0, stream>>>(
-                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
-                                 dRangeMeta, static_cast<uint64_t *>(dDesc), dTileCounter, failed,
-                                 dst);
-                     })))
-                    return rc;
-            }
-            else
-            {
-                const uint64_t tiles = q == 0 ? tilesFirst : tilesDense;
-                HIPIBS_CHECK(hipMemsetAsync(dDesc, 0, tiles * kBins * wordBytes, stream));
-                HIPIBS_CHECK(hipMemsetAsync(dTileCounter, 0, 4, stream));
-                if ((rc = timed(1, [&] {
-                         const dim3 grid((uint32_t)tiles);
-                         if (q == 0 && narrow)
-                             groupPassKernel<kPassLookBack, uint32_t><<<grid, kGroupBlock, 0, stream>>>(
-                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
-                                 nullptr, static_cast<uint32_t *>(dDesc), dTileCounter, failed, dst);
-                         else if (q == 0)
-                             groupPassKernel<kPassLookBack, uint64_t><<<grid, kGroupBlock, 0, stream>>>(
-                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
-                                 nullptr, static_cast<uint64_t *>(dDesc), dTileCounter, failed, dst);
-                         else if (narrow)
-                             groupPassKernel<kPassStable, uint32_t><<<grid, kGroupBlock, 0, stream>>>(
-                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
-                                 nullptr, static_cast<uint32_t *>(dDesc), dTileCounter, failed, dst);
-                         else
-                             groupPassKernel<kPassStable, uint64_t><<<grid, kGroupBlock, 0, stream>>>(
-                                 src, dRegion, nHash, shift, digitBits, prevShift, starts, nullptr,
-                                 nullptr, static_cast<uint64_t *>(dDesc), dTileCounter, failed, dst);
-                     })))
-                    return rc;
-            }
-            std::swap(src, dst);
-        }
-        if ((rc = timed(1, [&] {
-                 sliceStartKernel<<<(uint32_t)((nSlices + 1 + 255) / 256), 256, 0, stream>>>(
-                     src, nHash, sbits, nSlices, dStart);
-             })))
+        uint64_t *src = nullptr;
+        auto timedStage = [&](int stage, auto &&launch) { return timed(stage, launch); };
+        if ((rc = sorter.sort(dHash[0], dHash[1], nHash, nSlots / kTilePositions, sbits,
+                              SortKey{0, 0, 0}, dCounters + 10, stream, timedStage, &src,
+                              "k-mer counter")))
             return rc;
+        uint64_t *dst = src == dHash[0] ? dHash[1] : dHash[0];
         // the grouped hashes are in src; dst takes the spills
         const uint64_t *sorted = src;
         uint64_t *other = dst;
         if ((rc = timed(2, [&] {
                  countSliceKernel<<<(uint32_t)nSlices, kSliceBlock, 0, stream>>>(
-                     sorted, dStart, dTab, nBuckets, distinct == 0, tableUnfilled, other,
-                     dCounters);
+                     sorted, sorter.dStart, nSlices, dTab, nBuckets, distinct == 0, tableUnfilled,
+                     other, dCounters);
              })))
             return rc;
         HIPIBS_CHECK(hipMemcpyAsync(hCounters, dCounters, 11 * sizeof(unsigned long long),
@@ -1757,10 +2140,8 @@ struct hipibs_counter : ReadSink
         spilled += spills;
         HIPIBS_CHECK(hipMemsetAsync(dCounters + 7, 0, 2 * sizeof(unsigned long long), stream));
         HIPIBS_CHECK(hipMemsetAsync(dHll, 0, 4u << kHllLog, stream));
-        HIPIBS_CHECK(hipMemsetAsync(dDigit, 0, kMaxDigits * kBins * sizeof(unsigned long long),
-                                    stream));
-        HIPIBS_CHECK(hipMemsetAsync(dRange, 0, kRanges * kBins * sizeof(unsigned long long),
-                                    stream));
+        if ((rc = sorter.clearCounts(stream)))
+            return rc;
         HIPIBS_CHECK(hipStreamSynchronize(stream));
         hCounters[7] = hCounters[8] = 0;
         nHash = 0;
@@ -1996,6 +2377,158 @@ static int checkFilter(uint32_t lo, uint32_t hi, uint32_t cmax)
     return 0;
 }
 
+namespace hipibs
+{
+// Whole-k-mer tables of k <= 32 from device keys by sorting, in place of
+// one compare-and-swap walk per key: the keys' hashes are sorted by the
+// table slice (32 KB of buckets) of their home bucket, each slice is filled
+// in LDS and written out whole, and the few keys that overflow their slice
+// are inserted by insertKernel. The table (allocated; its slots need not be
+// cleared: every bucket is written) gets the same keys, each stored once
+// from its home bucket on. Returns 0 with
+// *used = false (nothing done) below HIPIBS_SORTED_BUILD_MIN keys (default
+// 2^20), when the sort's buffers (about 2 x 8 B per key) do not fit, or
+// when the slice index needs more than three digits (the environment's
+// HIPIBS_SORT_DIGIT_BITS / HIPIBS_SORT_LOOKBACK_WIDE apply as to counters).
+int buildTableSorted(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStream_t s,
+                     bool *used)
+{
+    *used = false;
+    uint64_t minKeys = 1ull << 20;
+    if (const char *e = std::getenv("HIPIBS_SORTED_BUILD_MIN"))
+        minKeys = std::strtoull(e, nullptr, 10);
+    if (t->k > 32 || t->mlen || n == 0 || n < minKeys)
+        return 0;
+    uint32_t digitBits = kMaxDigitBits;
+    if (const char *db = std::getenv("HIPIBS_SORT_DIGIT_BITS"))
+        digitBits = (uint32_t)std::min(kMaxDigitBits, std::max(1, std::atoi(db)));
+    const char *lw = std::getenv("HIPIBS_SORT_LOOKBACK_WIDE");
+    const bool wideLookBack = lw && std::atoi(lw) != 0;
+    const int S = (int)t->bucketSlots;
+    const uint32_t sliceLog = S == 16 ? 8 : 9; // 32 KB of keys per slice
+    const uint64_t nB = t->nBuckets;
+    const uint64_t nSlices = (nB + (1ull << sliceLog) - 1) >> sliceLog;
+    const uint32_t sbits = nSlices > 1 ? (uint32_t)(64 - __builtin_clzll(nSlices - 1)) : 0u;
+    if (std::max<uint32_t>(1, (sbits + digitBits - 1) / digitBits) > (uint32_t)kMaxDigits)
+        return 0;
+    const uint64_t tiles = (n + kTilePositions - 1) / kTilePositions;
+    const uint64_t slots = tiles * kTilePositions;
+    // the two hash buffers come from the process's cache when it holds a
+    // pair large enough (the k-mer counter's), else are allocated if they fit
+    uint64_t *buf[2] = {nullptr, nullptr};
+    uint64_t cap = 0;
+    const bool cached = takeCachedHash(t->device, slots, buf, &cap);
+    size_t freeB = 0, totalB = 0;
+    HIPIBS_CHECK(hipMemGetInfo(&freeB, &totalB));
+    const uint64_t need = (cached ? 0 : 2 * slots * 8) +
+                          tiles * (kTileDigits * kBins * 2 + kHashWaves * 4 + kBins * 12) +
+                          (8ull << sbits) + (64ull << 20);
+    if ((double)need > 0.9 * (double)freeB)
+    {
+        if (cached)
+            returnHash(t->device, buf, cap);
+        return 0;
+    }
+    if (!cached)
+    {
+        for (int i = 0; i < 2; ++i)
+            if (hipMalloc(reinterpret_cast<void **>(&buf[i]), slots * 8) != hipSuccess)
+            {
+                (void)hipGetLastError();
+                if (buf[0])
+                    (void)hipFree(buf[0]);
+                return 0; // no room: the caller inserts key by key
+            }
+        cap = slots;
+    }
+    SliceSorter *so = takeBuildSorter(t->device);
+    int rc = 0;
+    if (so)
+        rc = so->clearCounts(s);
+    else
+    {
+        so = new SliceSorter();
+        so->device = t->device;
+        rc = so->init(s);
+    }
+    so->digitBits = digitBits;
+    so->wideLookBack = wideLookBack;
+    unsigned long long *dCnt = nullptr; // [0] hashes, [1] spills, [2] a look-back timed out
+    auto cleanup = [&] {
+        (void)hipStreamSynchronize(s);
+        returnHash(t->device, buf, cap);
+        if (dCnt)
+            (void)hipFree(dCnt);
+        returnBuildSorter(t->device, so);
+    };
+    if (!rc && hipMalloc(reinterpret_cast<void **>(&dCnt), 4 * sizeof(unsigned long long)) !=
+                   hipSuccess)
+        rc = setError(HIPIBS_E_NOMEM, "table build: counter allocation failed");
+    if (!rc && hipMemsetAsync(dCnt, 0, 4 * sizeof(unsigned long long), s) != hipSuccess)
+        rc = setError(HIPIBS_E_HIP, "table build: counter clear failed");
+    if (!rc)
+        rc = so->reserveTiles(tiles);
+    const SortKey sk{nB, sliceLog, sbits};
+    if (!rc)
+    {
+        keyHashKernel<<<gridFor(n, kTilePositions, t->device), kCountBlock, 0, s>>>(
+            dKeys, n, sk, so->digitBits, so->rangeBits(), buf[0], so->dRegion, so->dTileHist,
+            so->dDigit, so->dRange, dCnt);
+        if (hipGetLastError() != hipSuccess)
+            rc = setError(HIPIBS_E_HIP, "table build: hashing launch failed");
+    }
+    unsigned long long hc[4] = {0, 0, 0, 0};
+    if (!rc && (hipMemcpyAsync(hc, dCnt, sizeof(hc), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+        rc = setError(HIPIBS_E_HIP, "table build: hashing failed");
+    uint64_t *sorted = nullptr;
+    if (!rc && !hc[0])
+        rc = clearTableSlots(t, s); // no keys: every bucket EMPTY
+    if (!rc && hc[0])
+    {
+        auto plain = [&](int, auto &&launch) {
+            launch();
+            return hipGetLastError() == hipSuccess
+                       ? 0
+                       : setError(HIPIBS_E_HIP, "table build: a sort launch failed");
+        };
+        rc = so->sort(buf[0], buf[1], hc[0], tiles, sbits, sk, dCnt + 2, s, plain, &sorted,
+                      "table build");
+        uint64_t *spill = sorted == buf[0] ? buf[1] : buf[0];
+        if (!rc)
+        {
+            auto *ctr = reinterpret_cast<unsigned long long *>(t->dCounters);
+            if (S == 16)
+                tableSliceKernel<16, 8><<<(uint32_t)nSlices, 256, 0, s>>>(
+                    sorted, so->dStart, nSlices, t->dSlots, nB, spill, dCnt + 1, ctr);
+            else
+                tableSliceKernel<8, 9><<<(uint32_t)nSlices, 256, 0, s>>>(
+                    sorted, so->dStart, nSlices, t->dSlots, nB, spill, dCnt + 1, ctr);
+            if (hipGetLastError() != hipSuccess)
+                rc = setError(HIPIBS_E_HIP, "table build: slice launch failed");
+        }
+        if (!rc && (hipMemcpyAsync(hc, dCnt, sizeof(hc), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                    hipStreamSynchronize(s) != hipSuccess))
+            rc = setError(HIPIBS_E_HIP, "table build: slice fill failed");
+        if (!rc && hc[2])
+            rc = setError(HIPIBS_E_HIP, "table build: a sort pass's look-back timed out");
+        // keys past their slice's end: inserted from their home bucket on
+        if (!rc && hc[1])
+            rc = launchTableInsert(t, spill, hc[1], s);
+    }
+    cleanup();
+    if (!rc)
+        *used = true;
+    return rc;
+}
+
+// The buffers kept for the next counter or table build, freed.
+void releaseSortCaches()
+{
+    hipibs_counter_release_cache();
+}
+} // namespace hipibs
+
 extern "C" int hipibs_counter_create(uint32_t k, int device, uint64_t expected_keys,
                                      uint64_t chunk_bytes, hipibs_counter **out)
 {
@@ -2024,10 +2557,10 @@ extern "C" int hipibs_counter_create(uint32_t k, int device, uint64_t expected_k
     c->chunkBytes = chunk_bytes;
     if (const char *sc = std::getenv("HIPIBS_COUNT_SKETCH_SCALE"))
         c->sketchScale = std::min(100.0, std::max(0.0, std::atof(sc)));
-    if (const char *db = std::getenv("HIPIBS_COUNT_DIGIT_BITS"))
-        c->digitBits = (uint32_t)std::min(kMaxDigitBits, std::max(1, std::atoi(db)));
-    if (const char *lw = std::getenv("HIPIBS_COUNT_LOOKBACK_WIDE"))
-        c->wideLookBack = std::atoi(lw) != 0;
+    if (const char *db = std::getenv("HIPIBS_SORT_DIGIT_BITS"))
+        c->sorter.digitBits = (uint32_t)std::min(kMaxDigitBits, std::max(1, std::atoi(db)));
+    if (const char *lw = std::getenv("HIPIBS_SORT_LOOKBACK_WIDE"))
+        c->sorter.wideLookBack = std::atoi(lw) != 0;
     auto fail = [c](int rc)
     {
         hipibs_counter_free(c);
@@ -2274,28 +2807,7 @@ extern "C" void hipibs_counter_free(hipibs_counter *c)
         returnHash(c->device, c->dHash, c->hashCap);
     if (c->dHll)
         (void)hipFree(c->dHll);
-    if (c->dStart)
-        (void)hipFree(c->dStart);
-    if (c->dRegion)
-        (void)hipFree(c->dRegion);
-    if (c->dTileHist)
-        (void)hipFree(c->dTileHist);
-    if (c->dGroupSum)
-        (void)hipFree(c->dGroupSum);
-    if (c->dTileOffset)
-        (void)hipFree(c->dTileOffset);
-    if (c->dDigit)
-        (void)hipFree(c->dDigit);
-    if (c->dRange)
-        (void)hipFree(c->dRange);
-    if (c->dRangeMeta)
-        (void)hipFree(c->dRangeMeta);
-    if (c->dDigitStart)
-        (void)hipFree(c->dDigitStart);
-    if (c->dDesc)
-        (void)hipFree(c->dDesc);
-    if (c->dTileCounter)
-        (void)hipFree(c->dTileCounter);
+    c->sorter.release();
     for (char *h : c->hText)
         if (h)
             (void)hipHostFree(h);
@@ -2310,6 +2822,7 @@ extern "C" void hipibs_counter_free(hipibs_counter *c)
 
 extern "C" void hipibs_counter_release_cache(void)
 {
+    releaseBuildSorter();
     std::lock_guard<std::mutex> g(gHashCache.m);
     if (!gHashCache.p[0])
         return;

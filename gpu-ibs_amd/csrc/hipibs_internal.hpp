@@ -314,6 +314,18 @@ namespace hipibs
 {
 // Launch helpers implemented in the .hip files (all asynchronous).
 int launchTableInsert(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStream_t s);
+// A whole-k-mer table (k <= 32) filled from device keys by sorting them by
+// table slice (count.hip); *used = false when it did nothing (small inputs,
+// or no room for its buffers): insert with launchTableInsert then.
+int buildTableSorted(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStream_t s,
+                     bool *used);
+// hipibs_table_build_keys_device; allowSorted = false keeps whole-k-mer
+// builds on the key-by-key insert (its buffers are allocated and freed with
+// calls that wait for the whole device, which a server's builds must not).
+int buildKeysDevice(const uint64_t *d_keys, uint64_t n, uint32_t k, const hipibs_table_opts *opts,
+                    hipibs_table **out, hipStream_t s, bool allowSorted);
+// Frees the buffers the process keeps for its next counter or table build.
+void releaseSortCaches();
 int launchProbe(const hipibs_table *t, const uint64_t *dCodes, const uint8_t *dValid,
                 uint64_t n, uint8_t *dHits, uint64_t *dCount, hipStream_t s);
 int launchScan(const hipibs_table *const *tables, int nTables, const uint64_t *dSeq,
@@ -336,10 +348,12 @@ int kmcReadKeys(const std::string &prefix, uint64_t *keys, uint64_t cap, uint64_
                 unsigned threads, hipibs_kmc_info *infoOut);
 
 // Table construction steps shared by the build paths: slots sized for
-// nKeys and cleared on s; after the inserts, the build counters read back
-// (synchronises s) and checked.
+// nKeys and cleared on s (clearSlots = false leaves them for a builder that
+// writes every bucket; clearTableSlots clears them later); after the
+// inserts, the build counters read back (synchronises s) and checked.
 int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts, hipibs_table **out,
-               hipStream_t s, bool streamOrdered = false);
+               hipStream_t s, bool streamOrdered = false, bool clearSlots = true);
+int clearTableSlots(hipibs_table *t, hipStream_t s);
 int finishTable(hipibs_table *t, hipStream_t s);
 // Frees t; a stream-ordered table's memory goes back to the pool after the
 // work already on s.

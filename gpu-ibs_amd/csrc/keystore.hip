@@ -470,7 +470,7 @@ int buildTableFromStore(const KeyStore &ks, const hipibs_table_opts *opts, Store
     if (!rc && whole)
         rc = finishTable(t, st->compute); // synchronises the compute stream
     else if (!rc)
-        rc = hipibs_table_build_keys_device(dAll, n, ks.k, opts, &t, st->compute);
+        rc = buildKeysDevice(dAll, n, ks.k, opts, &t, st->compute, false);
     (void)hipStreamSynchronize(st->copy);
     (void)hipStreamSynchronize(st->compute);
     if (dAll)

@@ -280,7 +280,7 @@ int ensureResident(hipibs_serve *s, uint32_t id, bool async, bool force,
     o.device = s->device;
     int rc = !sm.kmcPrefix.empty()
                  ? hipibs_table_build_kmc(sm.kmcPrefix.c_str(), &o, &sm.table)
-                 : dKeys ? hipibs_table_build_keys_device(dKeys, sm.n, s->k, &o, &sm.table, nullptr)
+                 : dKeys ? buildKeysDevice(dKeys, sm.n, s->k, &o, &sm.table, nullptr, false)
                          : setError(HIPIBS_E_ARG, "no device keys for a preload");
     if (rc)
     {

@@ -539,7 +539,7 @@ int launchProbe(const hipibs_table *t, const uint64_t *dCodes, const uint8_t *dV
 
 // Allocates and clears a table sized for nKeys keys.
 int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts, hipibs_table **out,
-               hipStream_t s, bool streamOrdered)
+               hipStream_t s, bool streamOrdered, bool clearSlots)
 {
     hipibs_table_opts o{};
     if (opts)
@@ -585,6 +585,13 @@ int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts, hipibs
         return streamOrdered ? hipMallocAsync(p, b, s) : hipMalloc(p, b);
     };
     hipError_t e = alloc(reinterpret_cast<void **>(&t->dSlots), bytes);
+    if (e != hipSuccess && !streamOrdered)
+    {
+        // the buffers kept for the next counter or build may be in the way
+        (void)hipGetLastError();
+        releaseSortCaches();
+        e = alloc(reinterpret_cast<void **>(&t->dSlots), bytes);
+    }
     if (e != hipSuccess)
     {
         delete t;
@@ -598,8 +605,9 @@ int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts, hipibs
         return setError(HIPIBS_E_NOMEM, "allocation of table counters failed");
     }
     const uint64_t nSlots = nBuckets * o.bucket_slots;
-    hipLaunchKernelGGL(fillKernel, dim3(gridFor(nSlots, 256, o.device)), dim3(256), 0, s,
-                       t->dSlots, nSlots, EMPTY);
+    if (clearSlots)
+        hipLaunchKernelGGL(fillKernel, dim3(gridFor(nSlots, 256, o.device)), dim3(256), 0, s,
+                           t->dSlots, nSlots, EMPTY);
     hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, s, t->dCounters, (uint64_t)4, 0ull);
     e = hipGetLastError();
     if (e != hipSuccess)
@@ -608,6 +616,15 @@ int allocTable(uint64_t nKeys, uint32_t k, const hipibs_table_opts *opts, hipibs
         return hipFail(e, "table clear");
     }
     *out = t;
+    return 0;
+}
+
+int clearTableSlots(hipibs_table *t, hipStream_t s)
+{
+    const uint64_t nSlots = t->nBuckets * t->bucketSlots;
+    hipLaunchKernelGGL(fillKernel, dim3(gridFor(nSlots, 256, t->device)), dim3(256), 0, s,
+                       t->dSlots, nSlots, EMPTY);
+    HIPIBS_CHECK(hipGetLastError());
     return 0;
 }
 
@@ -796,23 +813,31 @@ static int buildMinimizerTable(const uint64_t *dKeys, uint64_t n, uint32_t k,
 
 using namespace hipibs;
 
-extern "C" int hipibs_table_build_keys_device(const uint64_t *d_keys, uint64_t n, uint32_t k,
-                                              const hipibs_table_opts *opts, hipibs_table **out,
-                                              void *stream)
+namespace hipibs
+{
+int buildKeysDevice(const uint64_t *d_keys, uint64_t n, uint32_t k, const hipibs_table_opts *opts,
+                    hipibs_table **out, hipStream_t s, bool allowSorted)
 {
     if (!out || (n && !d_keys))
         return setError(HIPIBS_E_ARG, "null argument");
     *out = nullptr;
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (opts && opts->minimizer_len && k > 32)
         return setError(HIPIBS_E_ARG, "minimizer tables need k <= 32");
     if (opts && opts->minimizer_len)
         return buildMinimizerTable(d_keys, n, k, opts, s, out);
     hipibs_table *t = nullptr;
-    int rc = allocTable(n, k, opts, &t, s);
+    // a sorted build writes every bucket: the slots are cleared only when
+    // the keys go in one by one
+    int rc = allocTable(n, k, opts, &t, s, false, !allowSorted);
     if (rc)
         return rc;
-    rc = launchTableInsert(t, d_keys, n, s);
+    bool sorted = false;
+    if (allowSorted)
+        rc = buildTableSorted(t, d_keys, n, s, &sorted);
+    if (!rc && !sorted && allowSorted)
+        rc = clearTableSlots(t, s);
+    if (!rc && !sorted)
+        rc = launchTableInsert(t, d_keys, n, s);
     if (!rc)
         rc = finishTable(t, s);
     if (rc)
@@ -822,6 +847,14 @@ extern "C" int hipibs_table_build_keys_device(const uint64_t *d_keys, uint64_t n
     }
     *out = t;
     return 0;
+}
+} // namespace hipibs
+
+extern "C" int hipibs_table_build_keys_device(const uint64_t *d_keys, uint64_t n, uint32_t k,
+                                              const hipibs_table_opts *opts, hipibs_table **out,
+                                              void *stream)
+{
+    return buildKeysDevice(d_keys, n, k, opts, out, reinterpret_cast<hipStream_t>(stream), true);
 }
 
 // memcpy on up to 4 host threads (staging copies into pinned memory are

@@ -249,11 +249,11 @@ def test_slice_sort_passes_and_lookback_words(small_reads, monkeypatch, bits, wi
     r = reads[2][:12000]
     text = b"\n".join(corpus.to_text(x) for x in r)
     want_k, want_c = corpus.count_kmers(r, 25, ci=1, cs=ALL)
-    monkeypatch.setenv("HIPIBS_COUNT_DIGIT_BITS", str(bits))
-    monkeypatch.setenv("HIPIBS_COUNT_LOOKBACK_WIDE", wide)
+    monkeypatch.setenv("HIPIBS_SORT_DIGIT_BITS", str(bits))
+    monkeypatch.setenv("HIPIBS_SORT_LOOKBACK_WIDE", wide)
     monkeypatch.setenv("HIPIBS_COUNT_SKETCH_SCALE", "40")
     c = hipibs.KmerCounter(25)
-    for v in ("HIPIBS_COUNT_DIGIT_BITS", "HIPIBS_COUNT_LOOKBACK_WIDE", "HIPIBS_COUNT_SKETCH_SCALE"):
+    for v in ("HIPIBS_SORT_DIGIT_BITS", "HIPIBS_SORT_LOOKBACK_WIDE", "HIPIBS_COUNT_SKETCH_SCALE"):
         monkeypatch.delenv(v)
     c.add_text(text)
     st = c.stats()
@@ -276,10 +276,10 @@ def test_repetitive_reads_fill_few_digits(monkeypatch):
     r = np.stack([seqs[int(i)] for i in pick])
     text = b"\n".join(corpus.to_text(x) for x in r)
     want_k, want_c = corpus.count_kmers(r, 21, ci=1, cs=ALL)
-    monkeypatch.setenv("HIPIBS_COUNT_DIGIT_BITS", "3")
+    monkeypatch.setenv("HIPIBS_SORT_DIGIT_BITS", "3")
     monkeypatch.setenv("HIPIBS_COUNT_SKETCH_SCALE", "100")
     c = hipibs.KmerCounter(21)
-    monkeypatch.delenv("HIPIBS_COUNT_DIGIT_BITS")
+    monkeypatch.delenv("HIPIBS_SORT_DIGIT_BITS")
     monkeypatch.delenv("HIPIBS_COUNT_SKETCH_SCALE")
     c.add_text(text)
     st = c.stats()

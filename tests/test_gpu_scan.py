@@ -200,6 +200,51 @@ def test_duplicate_keys_stored_once():
 D = hipibs.DeviceBuffer
 
 
+@pytest.mark.parametrize("slots,lf,n,bits", [(16, 0.6, 3_000_000, 9), (16, 0.95, 2_000_000, 9),
+                                             (8, 0.9, 1_500_000, 9), (16, 0.6, 200_000, 9),
+                                             (8, 0.3, 700_000, 5), (16, 0.6, 3_000_000, 4)])
+def test_sorted_table_build_equals_insert(small_corpus, monkeypatch, slots, lf, n, bits):
+    """Whole-k-mer tables from device keys are built by sorting the keys by
+    table slice (above HIPIBS_SORTED_BUILD_MIN keys): the same key set as
+    the key-by-key insert -- repeated keys stored once, EMPTY rows skipped,
+    keys past their slice's end (load 0.95) inserted afterwards -- over one,
+    two and three sort passes (narrow digits forced) and both bucket widths;
+    every key answers, non-keys answer as set arithmetic says, and a scan of
+    the 1 Mbp reference with the sample's keys added gives the same rows as
+    the oracle for both builds."""
+    ref, samples = small_corpus
+    seq = corpus.to_text(ref)
+    sample = samples[1][0][:n]
+    rnd = corpus.splitmix(n, 17, n) & np.uint64((1 << 50) - 1)
+    keys = np.concatenate([sample, rnd, rnd[::5], sample[::3]])
+    keys[::100003] = np.uint64(0xFFFFFFFFFFFFFFFF)  # EMPTY rows: no key
+    distinct = np.unique(keys[keys != np.uint64(0xFFFFFFFFFFFFFFFF)])
+    other = corpus.splitmix(n, 29, 200_000) & np.uint64((1 << 50) - 1)
+    want_other = int(np.isin(other, distinct).sum())
+    d = D.from_array(keys)
+    note(f"sorted build slots={slots} lf={lf} n={keys.size} digit bits={bits}")
+    monkeypatch.setenv("HIPIBS_SORT_DIGIT_BITS", str(bits))
+    monkeypatch.setenv("HIPIBS_SORTED_BUILD_MIN", "1")
+    t_sort = hipibs.Table.from_device_keys(d.ptr, keys.size, 25, bucket_slots=slots,
+                                           load_factor=lf)
+    monkeypatch.setenv("HIPIBS_SORTED_BUILD_MIN", str(1 << 62))
+    t_ins = hipibs.Table.from_device_keys(d.ptr, keys.size, 25, bucket_slots=slots,
+                                          load_factor=lf)
+    monkeypatch.delenv("HIPIBS_SORTED_BUILD_MIN")
+    monkeypatch.delenv("HIPIBS_SORT_DIGIT_BITS")
+    nb = t_sort.info()["n_buckets"]
+    note(f"  {nb} buckets, {-(-nb // (256 if slots == 16 else 512))} slices")
+    want_rows = oracle_scan_rows([b"chr1"], [seq], 50000, 25, sorted_keys(distinct))
+    for t in (t_sort, t_ins):
+        assert t.info()["n_keys"] == distinct.size
+        assert t.probe_count(distinct) == distinct.size
+        assert t.probe_count(other) == want_other
+        assert hipibs.scan_sequences([t], [b"chr1"], [seq], 50000)[0] == want_rows
+    t_sort.free()
+    t_ins.free()
+    d.free()
+
+
 def test_coverage_matches_golden(small_corpus, expected):
     ref, samples = small_corpus
     seq = corpus.to_text(ref)
