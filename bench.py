@@ -688,6 +688,9 @@ def run_panel_device(args):
              ("minimizer", dict(minimizer_len=21, minimizer_cap=16, load_factor=0.2))]
     results = {}
     for kind, opts in kinds:
+        # the library keeps the last build's sort buffer for the next build
+        # of this kind; the other kind's groups get that memory back
+        hipibs._lib.hipibs_counter_release_cache()
         free = torch.cuda.mem_get_info()[0]
         # beside a group's tables: a pool (~16 GB), a sample's draw and
         # keys (~22 B per key), a minimizer build's group map, slack

@@ -1114,13 +1114,15 @@ __global__ __launch_bounds__(kSliceBlock) void countSliceKernel(
 // key is placed as insertOne places it: from its home bucket on, in the
 // first EMPTY slot (claimed by compare-and-swap; a bucket is read again
 // when another key took it), stored once. A key whose walk passes the
-// slice's end is appended to spill (spillCount) for the global insert.
+// slice's end is appended to spill (at most spillCap of them; spillCount
+// counts them all) for the global insert.
 // counters: [0] += keys stored, [2] = max displacement (as insertKernel).
 template <int S, int SLICE_LOG>
 __global__ __launch_bounds__(256) void tableSliceKernel(const uint64_t *__restrict__ hs,
                                                         const uint64_t *__restrict__ start,
                                                         uint64_t nSlices, uint64_t *slots,
                                                         uint64_t nBuckets, uint64_t *spill,
+                                                        uint64_t spillCap,
                                                         unsigned long long *spillCount,
                                                         unsigned long long *counters)
 {
@@ -1213,7 +1215,11 @@ __global__ __launch_bounds__(256) void tableSliceKernel(const uint64_t *__restri
                         ++b;
                 }
                 if (!done)
-                    spill[atomicAdd(spillCount, 1ull)] = key;
+                {
+                    const unsigned long long at = atomicAdd(spillCount, 1ull);
+                    if (at < spillCap)
+                        spill[at] = key;
+                }
             }
         }
         __syncthreads();
@@ -1421,6 +1427,7 @@ struct SliceSorter
     void *dDesc = nullptr; // look-back words
     uint64_t descBytes = 0;
     unsigned int *dTileCounter = nullptr;
+    unsigned long long *dBuild = nullptr; // a table build's counters (see buildTableSorted)
     uint64_t *dStart = nullptr; // the slices' starts
     uint64_t startCap = 0;
     std::vector<unsigned long long> hDigit, hRange;
@@ -1442,6 +1449,7 @@ struct SliceSorter
                                (2 * (kRanges + 1) + kRanges * kBins) * sizeof(uint64_t)));
         HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dTileCounter),
                                kRanges * sizeof(unsigned int)));
+        HIPIBS_CHECK(hipMalloc(reinterpret_cast<void **>(&dBuild), 4 * sizeof(unsigned long long)));
         return clearCounts(s);
     }
 
@@ -1449,6 +1457,7 @@ struct SliceSorter
     {
         HIPIBS_CHECK(hipMemsetAsync(dDigit, 0, kMaxDigits * kBins * sizeof(unsigned long long), s));
         HIPIBS_CHECK(hipMemsetAsync(dRange, 0, kRanges * kBins * sizeof(unsigned long long), s));
+        HIPIBS_CHECK(hipMemsetAsync(dBuild, 0, 4 * sizeof(unsigned long long), s));
         return 0;
     }
 
@@ -1476,7 +1485,8 @@ struct SliceSorter
     {
         for (void *p : {(void *)dDigit, (void *)dRange, (void *)dRegion, (void *)dTileHist,
                         (void *)dDigitStart, (void *)dRangeMeta, (void *)dGroupSum,
-                        (void *)dTileOffset, dDesc, (void *)dTileCounter, (void *)dStart})
+                        (void *)dTileOffset, dDesc, (void *)dTileCounter, (void *)dBuild,
+                        (void *)dStart})
             if (p)
                 (void)hipFree(p);
         *this = SliceSorter();
@@ -1777,8 +1787,64 @@ void returnBuildSorter(int device, SliceSorter *so)
     gBuildSorter.so = so;
 }
 
+// The sort buffer of table builds (8 B per key plus room for spills) is
+// kept for the process's next build: the largest one so far.
+struct ScratchCache
+{
+    std::mutex m;
+    int device = -1;
+    uint64_t *p = nullptr;
+    uint64_t cap = 0; // entries
+} gScratch;
+
+bool takeScratch(int device, uint64_t need, uint64_t **out, uint64_t *cap)
+{
+    std::lock_guard<std::mutex> g(gScratch.m);
+    if (!gScratch.p || gScratch.device != device || gScratch.cap < need)
+        return false;
+    *out = gScratch.p;
+    *cap = gScratch.cap;
+    gScratch.p = nullptr;
+    gScratch.cap = 0;
+    return true;
+}
+
+void returnScratch(int device, uint64_t *p, uint64_t cap)
+{
+    std::lock_guard<std::mutex> g(gScratch.m);
+    if (gScratch.p && gScratch.cap >= cap)
+    {
+        (void)hipFree(p);
+        return;
+    }
+    if (gScratch.p)
+    {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(gScratch.device);
+        (void)hipFree(gScratch.p);
+        (void)hipSetDevice(cur);
+    }
+    gScratch.device = device;
+    gScratch.p = p;
+    gScratch.cap = cap;
+}
+
 void releaseBuildSorter()
 {
+    {
+        std::lock_guard<std::mutex> g(gScratch.m);
+        if (gScratch.p)
+        {
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(gScratch.device);
+            (void)hipFree(gScratch.p);
+            (void)hipSetDevice(cur);
+            gScratch.p = nullptr;
+            gScratch.cap = 0;
+        }
+    }
     std::lock_guard<std::mutex> g(gBuildSorter.m);
     if (!gBuildSorter.so)
         return;
@@ -2383,11 +2449,12 @@ namespace hipibs
 // one compare-and-swap walk per key: the keys' hashes are sorted by the
 // table slice (32 KB of buckets) of their home bucket, each slice is filled
 // in LDS and written out whole, and the few keys that overflow their slice
-// are inserted by insertKernel. The table (allocated; its slots need not be
+// are inserted by insertKernel. The sort needs one buffer of about 9 B per
+// key (kept for the next build); the table's own slots are its other. The table (allocated; its slots need not be
 // cleared: every bucket is written) gets the same keys, each stored once
 // from its home bucket on. Returns 0 with
 // *used = false (nothing done) below HIPIBS_SORTED_BUILD_MIN keys (default
-// 2^20), when the sort's buffers (about 2 x 8 B per key) do not fit, or
+// 2^20), when the sort's buffer does not fit, or
 // when the slice index needs more than three digits (the environment's
 // HIPIBS_SORT_DIGIT_BITS / HIPIBS_SORT_LOOKBACK_WIDE apply as to counters).
 int buildTableSorted(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStream_t s,
@@ -2413,34 +2480,44 @@ int buildTableSorted(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStre
         return 0;
     const uint64_t tiles = (n + kTilePositions - 1) / kTilePositions;
     const uint64_t slots = tiles * kTilePositions;
-    // the two hash buffers come from the process's cache when it holds a
-    // pair large enough (the k-mer counter's), else are allocated if they fit
-    uint64_t *buf[2] = {nullptr, nullptr};
+    // one sort buffer of its own: the sorted hashes, then room for the keys
+    // that overflow their slice; the table's slots (written whole at the end)
+    // serve as the other -- the hashing writes there when the passes are
+    // odd, so that the last pass ends in the buffer
+    const uint32_t passes = std::max<uint32_t>(1, (sbits + digitBits - 1) / digitBits);
+    if (nB * (uint64_t)S < slots)
+        return 0;
+    const uint64_t spillRoom = slots / 8 + 65536;
+    uint64_t *buf = nullptr;
     uint64_t cap = 0;
-    const bool cached = takeCachedHash(t->device, slots, buf, &cap);
+    const bool cached = takeScratch(t->device, slots + spillRoom, &buf, &cap);
     size_t freeB = 0, totalB = 0;
     HIPIBS_CHECK(hipMemGetInfo(&freeB, &totalB));
-    const uint64_t need = (cached ? 0 : 2 * slots * 8) +
+    const uint64_t need = (cached ? 0 : (slots + spillRoom) * 8) +
                           tiles * (kTileDigits * kBins * 2 + kHashWaves * 4 + kBins * 12) +
                           (8ull << sbits) + (64ull << 20);
     if ((double)need > 0.9 * (double)freeB)
     {
         if (cached)
-            returnHash(t->device, buf, cap);
+            returnScratch(t->device, buf, cap);
         return 0;
     }
     if (!cached)
     {
-        for (int i = 0; i < 2; ++i)
-            if (hipMalloc(reinterpret_cast<void **>(&buf[i]), slots * 8) != hipSuccess)
-            {
-                (void)hipGetLastError();
-                if (buf[0])
-                    (void)hipFree(buf[0]);
-                return 0; // no room: the caller inserts key by key
-            }
-        cap = slots;
+        // room for larger builds to come (the buffer is kept for them): half
+        // again as many entries, when that fits
+        cap = slots + spillRoom;
+        const uint64_t roomy = cap / 2 * 3;
+        if ((double)(need + (roomy - cap) * 8) < 0.8 * (double)freeB)
+            cap = roomy;
+        if (hipMalloc(reinterpret_cast<void **>(&buf), cap * 8) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return 0; // no room: the caller inserts key by key
+        }
     }
+    uint64_t *first = passes % 2 == 0 ? buf : t->dSlots; // the hashing's output
+    uint64_t *second = first == buf ? t->dSlots : buf;
     SliceSorter *so = takeBuildSorter(t->device);
     int rc = 0;
     if (so)
@@ -2453,26 +2530,20 @@ int buildTableSorted(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStre
     }
     so->digitBits = digitBits;
     so->wideLookBack = wideLookBack;
-    unsigned long long *dCnt = nullptr; // [0] hashes, [1] spills, [2] a look-back timed out
+    // [0] hashes, [1] spills, [2] a look-back timed out (cleared with the counts)
+    unsigned long long *dCnt = so->dBuild;
     auto cleanup = [&] {
         (void)hipStreamSynchronize(s);
-        returnHash(t->device, buf, cap);
-        if (dCnt)
-            (void)hipFree(dCnt);
+        returnScratch(t->device, buf, cap);
         returnBuildSorter(t->device, so);
     };
-    if (!rc && hipMalloc(reinterpret_cast<void **>(&dCnt), 4 * sizeof(unsigned long long)) !=
-                   hipSuccess)
-        rc = setError(HIPIBS_E_NOMEM, "table build: counter allocation failed");
-    if (!rc && hipMemsetAsync(dCnt, 0, 4 * sizeof(unsigned long long), s) != hipSuccess)
-        rc = setError(HIPIBS_E_HIP, "table build: counter clear failed");
     if (!rc)
         rc = so->reserveTiles(tiles);
     const SortKey sk{nB, sliceLog, sbits};
     if (!rc)
     {
         keyHashKernel<<<gridFor(n, kTilePositions, t->device), kCountBlock, 0, s>>>(
-            dKeys, n, sk, so->digitBits, so->rangeBits(), buf[0], so->dRegion, so->dTileHist,
+            dKeys, n, sk, so->digitBits, so->rangeBits(), first, so->dRegion, so->dTileHist,
             so->dDigit, so->dRange, dCnt);
         if (hipGetLastError() != hipSuccess)
             rc = setError(HIPIBS_E_HIP, "table build: hashing launch failed");
@@ -2492,18 +2563,21 @@ int buildTableSorted(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStre
                        ? 0
                        : setError(HIPIBS_E_HIP, "table build: a sort launch failed");
         };
-        rc = so->sort(buf[0], buf[1], hc[0], tiles, sbits, sk, dCnt + 2, s, plain, &sorted,
+        rc = so->sort(first, second, hc[0], tiles, sbits, sk, dCnt + 2, s, plain, &sorted,
                       "table build");
-        uint64_t *spill = sorted == buf[0] ? buf[1] : buf[0];
+        if (!rc && sorted != buf)
+            rc = setError(HIPIBS_E_HIP, "table build: the sort ended in the table's slots");
+        uint64_t *spill = buf + slots;
+        const uint64_t spillCap = cap - slots;
         if (!rc)
         {
             auto *ctr = reinterpret_cast<unsigned long long *>(t->dCounters);
             if (S == 16)
                 tableSliceKernel<16, 8><<<(uint32_t)nSlices, 256, 0, s>>>(
-                    sorted, so->dStart, nSlices, t->dSlots, nB, spill, dCnt + 1, ctr);
+                    sorted, so->dStart, nSlices, t->dSlots, nB, spill, spillCap, dCnt + 1, ctr);
             else
                 tableSliceKernel<8, 9><<<(uint32_t)nSlices, 256, 0, s>>>(
-                    sorted, so->dStart, nSlices, t->dSlots, nB, spill, dCnt + 1, ctr);
+                    sorted, so->dStart, nSlices, t->dSlots, nB, spill, spillCap, dCnt + 1, ctr);
             if (hipGetLastError() != hipSuccess)
                 rc = setError(HIPIBS_E_HIP, "table build: slice launch failed");
         }
@@ -2513,8 +2587,10 @@ int buildTableSorted(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStre
         if (!rc && hc[2])
             rc = setError(HIPIBS_E_HIP, "table build: a sort pass's look-back timed out");
         // keys past their slice's end: inserted from their home bucket on
+        // (more than the room holds: every key, the ones stored are found)
         if (!rc && hc[1])
-            rc = launchTableInsert(t, spill, hc[1], s);
+            rc = hc[1] <= spillCap ? launchTableInsert(t, spill, hc[1], s)
+                                   : launchTableInsert(t, dKeys, n, s);
     }
     cleanup();
     if (!rc)

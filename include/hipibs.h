@@ -175,9 +175,9 @@ int hipibs_table_build_keys(const uint64_t *h_keys, uint64_t n, uint32_t k,
 /* Same from device keys on `stream` (synchronises before returning). A
  * whole-k-mer table of k <= 32 from 2^20 keys or more is built by sorting
  * the keys' hashes by table slice and filling each 16 KB slice in LDS (the
- * same keys in the same probe order as a key-by-key insert); its two 8 B
- * per key sort buffers are taken from, and then kept in, the process's
- * cache of hashed k-mer buffers (see hipibs_counter_release_cache). */
+ * same keys in the same probe order as a key-by-key insert); the sort uses
+ * the table's own slots and one buffer of about 9 B per key, which the
+ * process keeps for its next build (see hipibs_counter_release_cache). */
 int hipibs_table_build_keys_device(const uint64_t *d_keys, uint64_t n, uint32_t k,
                                    const hipibs_table_opts *opts, hipibs_table **out,
                                    void *stream);
@@ -295,9 +295,9 @@ int hipibs_counter_write_kmc(hipibs_counter *c, const char *prefix, uint32_t min
  * one pair per process. */
 void hipibs_counter_free(hipibs_counter *c);
 /* Frees the buffers kept for the process's next counter or table build:
- * the pair of hashed k-mer buffers and a table build's sort state (no
- * counter needs to exist). A table allocation that fails frees them and
- * tries once more. */
+ * the pair of hashed k-mer buffers, a table build's sort buffer and sort
+ * state (no counter needs to exist). A table allocation that fails frees
+ * them and tries once more. */
 void hipibs_counter_release_cache(void);
 
 /* ---- the fused scan (device-resident inputs) ------------------------------ */
