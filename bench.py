@@ -89,15 +89,18 @@ def parse():
                     help="wide workload: Mbp of the reference's string-path CPU sample")
     ap.add_argument("--workload", default="scan",
                     choices=["scan", "panel", "panel-device", "serve", "large", "count", "wide",
-                             "gups", "copy"],
+                             "build", "gups", "copy"],
                     help="scan: the single-sample device scan (default line); panel: "
                          "end-to-end fastibs runs, N single-sample runs vs one panel run; "
                          "panel-device: the same comparison through the library from device "
                          "keys at full key counts; "
                          "serve: served queries; large: the k=31 4.2 Gbp scan + coverage; "
                          "count: k-mer counting from reads; wide: the scan at k = 47; "
+                         "build: whole-k-mer table builds from device keys, sorted vs inserted; "
                          "gups: the random-bucket read ceiling (--gups-mode); copy: a plain "
                          "2 GiB copy that calibrates rocprofv3's byte counters")
+    ap.add_argument("--build-keys", type=float, default=1e9,
+                    help="build workload: random 50-bit codes to build into a table")
     ap.add_argument("--gups-mode", default="sizes", choices=["sizes", "policies", "alloc"])
     ap.add_argument("--cpu-mapper-mbp", type=float, default=8.0,
                     help="large: bases (Mbp) of the reference mapper's CPU sample")
@@ -1907,6 +1910,69 @@ def run_gups(args):
             buf.free()
 
 
+def run_build(args):
+    """Whole-k-mer table builds from device keys (hipibs_table_build_keys_
+    device): --build-keys random 50-bit codes (k = 25, generated on the GPU)
+    into a table of 16 slots per 128 B bucket at load 0.6, built by sorting
+    the keys by table slice (the library's default from 2^20 keys) and by
+    key-by-key inserts (HIPIBS_SORTED_BUILD_MIN above the key count),
+    alternating, --warmup + --steps times each. Both tables must hold the
+    same keys: the same key count, and every key of a 1 M sample found."""
+    import numpy as np
+    import torch
+
+    import hipibs
+    n = int(args.build_keys)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(args.seed + 90)
+    keys = torch.randint(0, 2 ** 50, (n,), dtype=torch.int64, device="cuda", generator=g)
+    sample = keys[:: max(1, n // (1 << 20))].cpu().numpy().astype(np.uint64)
+    torch.cuda.synchronize()
+    times = {"sorted": [], "insert": []}
+    info = {}
+    ok = True
+    for step in range(args.warmup + args.steps):
+        for mode in ("sorted", "insert"):
+            os.environ["HIPIBS_SORTED_BUILD_MIN"] = "1" if mode == "sorted" else str(1 << 62)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            t = hipibs.Table.from_device_keys(keys.data_ptr(), n, K)
+            t1 = time.perf_counter()
+            info[mode] = t.info()
+            ok &= t.probe_count(sample) == sample.size
+            t.free()
+            if step >= args.warmup:
+                times[mode].append(t1 - t0)
+    os.environ.pop("HIPIBS_SORTED_BUILD_MIN", None)
+    ok &= info["sorted"]["n_keys"] == info["insert"]["n_keys"]
+    s_ms = float(np.median(times["sorted"])) * 1e3
+    i_ms = float(np.median(times["insert"])) * 1e3
+    nb = info["sorted"]["n_buckets"]
+    slices = -(-nb // 256)
+    passes = max(1, -(-max(0, (slices - 1).bit_length()) // 9))
+    # algorithmic bytes per key: the hashing reads and writes 8 B, each pass
+    # reads and writes 8 B, the slice fill reads 8 B and writes the table
+    alg = n * (16 + 16 * passes + 8) + info["sorted"]["bytes"]
+    log(f"build of {n/1e9:.2f} G keys ({info['sorted']['n_keys']} distinct, "
+        f"{info['sorted']['bytes']/1e9:.1f} GB, {passes} sort passes): sorted {s_ms:.1f} ms, "
+        f"insert {i_ms:.1f} ms ({i_ms/s_ms:.2f}x); same keys {ok}")
+    out = {"metric": "table keys built/s (whole-k-mer, device keys)", "value": n / (s_ms / 1e3),
+           "unit": "keys/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": s_ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u64", "data": "synthetic (GPU-generated, seeded)",
+           "config": {"workload": f"table build, k=25: {n} random 50-bit codes in HBM, 16 slots "
+                                  f"per 128 B bucket, load 0.6",
+                      "sorted_ms": s_ms, "insert_ms": i_ms, "sort_passes": passes,
+                      "table_bytes": info["sorted"]["bytes"], "same_keys": ok},
+           "roofline": {"bound": "hbm", "achieved": alg / (s_ms / 1e3) / 1e9,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": alg / (s_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "algorithmic_bytes": alg,
+                        "note": "the whole build (all kernels, wall time of the call)"},
+           "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def run_copy(args):
     """A plain 2 GiB device-to-device copy, three times: the known-size
     reference that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters
@@ -1943,6 +2009,8 @@ def main():
         return run_gups(args)
     if args.workload == "copy":
         return run_copy(args)
+    if args.workload == "build":
+        return run_build(args)
     import numpy as np
     import torch
 
