@@ -174,7 +174,7 @@ int hipibs_table_build_keys(const uint64_t *h_keys, uint64_t n, uint32_t k,
 
 /* Same from device keys on `stream` (synchronises before returning). A
  * whole-k-mer table of k <= 32 from 2^20 keys or more is built by sorting
- * the keys' hashes by table slice and filling each 16 KB slice in LDS (the
+ * the keys' hashes by table slice and filling each 32 KB slice in LDS (the
  * same keys in the same probe order as a key-by-key insert); the sort uses
  * the table's own slots and one buffer of about 9 B per key, which the
  * process keeps for its next build (see hipibs_counter_release_cache). */
