@@ -1110,51 +1110,226 @@ __global__ __launch_bounds__(kSliceBlock) void countSliceKernel(
 
 // A table build's slice: buckets [s << SLICE_LOG, +2^SLICE_LOG) of a
 // whole-k-mer table of S key slots per bucket (32 KB), filled in LDS from
-// the slice's sorted hashes and written out whole, empty buckets too. A
-// key is placed as insertOne places it: from its home bucket on, in the
-// first EMPTY slot (claimed by compare-and-swap; a bucket is read again
-// when another key took it), stored once. A key whose walk passes the
-// slice's end is appended to spill (at most spillCap of them; spillCount
-// counts them all) for the global insert.
-// counters: [0] += keys stored, [2] = max displacement (as insertKernel).
+// the slice's sorted hashes and written out whole, empty buckets too.
+//
+// Counted placement (a slice of at most 256 x kSliceKeys keys, every one
+// distinct): each key takes a rank in its home bucket (an LDS counter),
+// a block scan turns the counts into runs -- bucket b's keys start at
+// S x b plus the keys the buckets before it pushed past their end, i.e.
+// o(b) = max(0, o(b-1) + count(b) - S), o(-1) = 0 -- and a key goes to
+// slot start(home) + rank. Every slot from a key's home bucket up to its
+// own is taken, as in a key-by-key insert (where a key takes the first
+// EMPTY slot from its home on). Repeated keys share a home run: a key
+// that meets its value earlier in its run sends the whole slice to the
+// compare-and-swap placement below.
+//
+// Compare-and-swap placement (larger slices, or keys repeated): a key is
+// placed as insertOne places it: from its home bucket on, in the first
+// EMPTY slot (claimed by compare-and-swap; a bucket is read again when
+// another key took it), stored once.
+//
+// Either way a key whose slot would lie past the slice's end is appended
+// to spill (at most spillCap of them; spillCount counts them all) for the
+// global insert. counters: [0] += keys stored, [2] = max displacement (as
+// insertKernel).
+constexpr int kSliceKeys = 16; // keys per thread of a counted slice
+
+// Blocks loop over slices s = block, block + grid, ...: a slice's hashes
+// are loaded into registers while the slice before it is placed, so that
+// their latency overlaps that work.
+// (128 VGPRs at most: four blocks of 33-35 KB of LDS share a CU)
 template <int S, int SLICE_LOG>
-__global__ __launch_bounds__(256) void tableSliceKernel(const uint64_t *__restrict__ hs,
-                                                        const uint64_t *__restrict__ start,
-                                                        uint64_t nSlices, uint64_t *slots,
-                                                        uint64_t nBuckets, uint64_t *spill,
-                                                        uint64_t spillCap,
-                                                        unsigned long long *spillCount,
-                                                        unsigned long long *counters)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void
+tableSliceKernel(const uint64_t *__restrict__ hs, const uint64_t *__restrict__ start,
+                 uint64_t nSlices, uint64_t *slots, uint64_t nBuckets, uint64_t *spill,
+                 uint64_t spillCap, unsigned long long *spillCount, unsigned long long *counters)
 {
     constexpr uint32_t kB = 1u << SLICE_LOG;
-    constexpr uint32_t W = S / 2; // 16 B words per bucket
+    constexpr uint32_t W = S / 2;       // 16 B words per bucket
+    constexpr uint32_t kPer = kB / 256; // buckets per thread in the run scan
+    constexpr int kSBits = S == 16 ? 4 : 3;
+    constexpr uint64_t kCap = 256ull * kSliceKeys; // most keys of a counted slice
     __shared__ __align__(16) unsigned long long keys[kB * S];
-    __shared__ uint32_t stored, widest;
+    __shared__ uint32_t runStart[kB];
+    __shared__ int32_t waveSum[4], waveMin[4];
+    __shared__ uint32_t stored, widest, repeated;
     if (threadIdx.x == 0)
         stored = widest = 0;
     uint32_t mine = 0, disp = 0;
+    // slot p of the slice in LDS: a bucket's 16 B words are stored swizzled
+    // (word w at w ^ (b mod W)), so that lanes reading the same word of
+    // different buckets meet different LDS banks
+    auto at = [](uint32_t p) {
+        const uint32_t b = p >> kSBits, j = p & (S - 1);
+        return (b * W + ((j >> 1) ^ (b & (W - 1)))) * 2 + (j & 1);
+    };
     constexpr int kAhead = 8;
+    uint64_t kv[kSliceKeys];
+    uint64_t lo = 0, hi = 0;
     if (blockIdx.x < nSlices)
     {
-        const uint64_t s = blockIdx.x;
+        lo = start[blockIdx.x];
+        hi = start[blockIdx.x + 1];
+        if (hi - lo <= kCap)
+#pragma unroll
+            for (int r = 0; r < kSliceKeys; ++r)
+            {
+                const uint64_t i = lo + threadIdx.x + (uint64_t)r * 256;
+                kv[r] = i < hi ? hs[i] : 0;
+            }
+    }
+    for (uint64_t s = blockIdx.x; s < nSlices; s += gridDim.x)
+    {
+        const uint64_t sn = s + gridDim.x;
+        const uint64_t loN = sn < nSlices ? start[sn] : 0, hiN = sn < nSlices ? start[sn + 1] : 0;
         const uint64_t b0 = s << SLICE_LOG;
         const uint32_t nb = (uint32_t)min<uint64_t>(kB, nBuckets - b0);
-        const uint64_t lo = start[s], hi = start[s + 1];
-        uint64_t hv[kAhead];
-        // the first hashes are loaded before the LDS fill, so that the
-        // fill overlaps their latency
-#pragma unroll
-        for (int a = 0; a < kAhead; ++a)
-        {
-            const uint64_t i = lo + threadIdx.x + (uint64_t)a * blockDim.x;
-            hv[a] = i < hi ? hs[i] : 0;
-        }
-        for (uint32_t q = threadIdx.x; q < nb * S; q += blockDim.x)
+        const uint32_t limit = nb * S;
+        bool counted = hi - lo <= kCap;
+        for (uint32_t q = threadIdx.x; q < limit; q += 256)
             keys[q] = EMPTY;
+        for (uint32_t q = threadIdx.x; q < kB; q += 256)
+            runStart[q] = 0;
+        if (threadIdx.x == 0)
+            repeated = 0;
         __syncthreads();
-        for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)blockDim.x * kAhead)
+        uint32_t hr[kSliceKeys]; // home << 16 | rank in the home bucket
+        if (counted)
+#pragma unroll
+            for (int r = 0; r < kSliceKeys; ++r)
+                if (lo + threadIdx.x + (uint64_t)r * 256 < hi)
+                {
+                    const uint64_t h = kv[r];
+                    const uint32_t home = (uint32_t)((((h >> 32) * nBuckets) >> 32) - b0);
+                    hr[r] = home << 16 | atomicAdd(&runStart[home], 1u);
+                    kv[r] = keyUnmix(h);
+                }
+        // the next slice's hashes
+        uint64_t kn[kSliceKeys];
+        if (hiN - loN <= kCap)
+#pragma unroll
+            for (int r = 0; r < kSliceKeys; ++r)
+            {
+                const uint64_t i = loN + threadIdx.x + (uint64_t)r * 256;
+                kn[r] = i < hiN ? hs[i] : 0;
+            }
+        __syncthreads();
+        if (counted)
         {
-            if (i0 != lo + threadIdx.x)
+            // runs: thread t scans buckets [t kPer, +kPer) of (count - S):
+            // the segment's sum and its smallest prefix sum, combined over
+            // the block (A then B: sum A + sum B, min(min A, sum A + min B))
+            int32_t cnt[kPer];
+            int32_t sum = 0, mn = INT32_MAX;
+#pragma unroll
+            for (uint32_t j = 0; j < kPer; ++j)
+            {
+                cnt[j] = (int32_t)runStart[threadIdx.x * kPer + j];
+                sum += cnt[j] - S;
+                mn = min(mn, sum);
+            }
+            const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1)
+            {
+                const int32_t ps = __shfl_up(sum, d, 64), pm = __shfl_up(mn, d, 64);
+                if (lane >= (uint32_t)d)
+                {
+                    mn = min(pm, ps + mn);
+                    sum = ps + sum;
+                }
+            }
+            if (lane == 63)
+            {
+                waveSum[wave] = sum;
+                waveMin[wave] = mn;
+            }
+            // the segment before this thread's: exclusive of the wave's
+            // inclusive scan
+            int32_t esum = __shfl_up(sum, 1, 64), emin = __shfl_up(mn, 1, 64);
+            if (lane == 0)
+            {
+                esum = 0;
+                emin = INT32_MAX;
+            }
+            __syncthreads();
+            int32_t wsum = 0, wmin = INT32_MAX; // the waves before this one
+            for (uint32_t w = 0; w < wave; ++w)
+            {
+                wmin = min(wmin, wsum + waveMin[w]);
+                wsum += waveSum[w];
+            }
+            if (emin != INT32_MAX)
+                wmin = min(wmin, wsum + emin);
+            wsum += esum;
+            // o(b-1) = P(b-1) - min(0, smallest prefix sum through b-1)
+#pragma unroll
+            for (uint32_t j = 0; j < kPer; ++j)
+            {
+                const uint32_t b = threadIdx.x * kPer + j;
+                runStart[b] = S * b + (uint32_t)(wsum - min(0, wmin));
+                wsum += cnt[j] - S;
+                wmin = min(wmin, wsum);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < kSliceKeys; ++r)
+                if (lo + threadIdx.x + (uint64_t)r * 256 < hi)
+                {
+                    const uint32_t home = hr[r] >> 16;
+                    const uint32_t pos = runStart[home] + (hr[r] & 0xffffu);
+                    if (pos < limit)
+                        keys[at(pos)] = kv[r];
+                }
+            __syncthreads();
+            // a repeated key meets its value earlier in its home run
+            bool rep = false;
+#pragma unroll
+            for (int r = 0; r < kSliceKeys; ++r)
+                if (lo + threadIdx.x + (uint64_t)r * 256 < hi)
+                {
+                    const uint32_t first = runStart[hr[r] >> 16];
+                    const uint32_t pos = min(first + (hr[r] & 0xffffu), limit);
+                    for (uint32_t p = first; p < pos; ++p)
+                        rep |= keys[at(p)] == kv[r];
+                }
+            if (rep)
+                repeated = 1;
+            __syncthreads();
+            counted = repeated == 0;
+            if (counted)
+            {
+#pragma unroll
+                for (int r = 0; r < kSliceKeys; ++r)
+                    if (lo + threadIdx.x + (uint64_t)r * 256 < hi)
+                    {
+                        const uint32_t home = hr[r] >> 16;
+                        const uint32_t pos = runStart[home] + (hr[r] & 0xffffu);
+                        if (pos < limit)
+                        {
+                            mine += 1;
+                            disp = max(disp, (pos >> kSBits) - home);
+                        }
+                        else
+                        {
+                            const unsigned long long k = atomicAdd(spillCount, 1ull);
+                            if (k < spillCap)
+                                spill[k] = kv[r];
+                        }
+                    }
+            }
+            else
+            {
+                for (uint32_t q = threadIdx.x; q < limit; q += 256)
+                    keys[q] = EMPTY;
+                __syncthreads();
+            }
+        }
+        if (!counted)
+        {
+            uint64_t hv[kAhead];
+            for (uint64_t i0 = lo + threadIdx.x; i0 < hi; i0 += (uint64_t)blockDim.x * kAhead)
+            {
 #pragma unroll
                 for (int a = 0; a < kAhead; ++a)
                 {
@@ -1162,63 +1337,59 @@ __global__ __launch_bounds__(256) void tableSliceKernel(const uint64_t *__restri
                     hv[a] = i < hi ? hs[i] : 0;
                 }
 #pragma unroll
-            for (int a = 0; a < kAhead; ++a)
-            {
-                if (i0 + (uint64_t)a * blockDim.x >= hi)
-                    break;
-                const uint64_t h = hv[a];
-                const uint64_t key = keyUnmix(h);
-                const uint32_t home = (uint32_t)((((h >> 32) * nBuckets) >> 32) - b0);
-                bool done = false;
-                for (uint32_t b = home; b < nb && !done;)
+                for (int a = 0; a < kAhead; ++a)
                 {
-                    // a bucket's 16 B words are stored swizzled (word w at
-                    // w ^ (b mod W)), so that lanes reading the same word of
-                    // different buckets meet different LDS banks
-                    const ulonglong2 *kb = reinterpret_cast<const ulonglong2 *>(keys) + b * W;
-                    uint64_t kk[S];
+                    if (i0 + (uint64_t)a * blockDim.x >= hi)
+                        break;
+                    const uint64_t h = hv[a];
+                    const uint64_t key = keyUnmix(h);
+                    const uint32_t home = (uint32_t)((((h >> 32) * nBuckets) >> 32) - b0);
+                    bool done = false;
+                    for (uint32_t b = home; b < nb && !done;)
+                    {
+                        const ulonglong2 *kb = reinterpret_cast<const ulonglong2 *>(keys) + b * W;
+                        uint64_t kk[S];
 #pragma unroll
-                    for (int q = 0; q < (int)W; ++q)
-                    {
-                        const ulonglong2 x = kb[q ^ (b & (W - 1))];
-                        kk[2 * q] = x.x;
-                        kk[2 * q + 1] = x.y;
-                    }
-                    // keys take the first EMPTY slot and are never removed: a
-                    // stored key sits before the first EMPTY one
-                    int hit = S, emp = S;
-#pragma unroll
-                    for (int q = S - 1; q >= 0; --q)
-                    {
-                        hit = kk[q] == key ? q : hit;
-                        emp = kk[q] == EMPTY ? q : emp;
-                    }
-                    if (hit < emp)
-                        done = true;
-                    else if (emp < S)
-                    {
-                        const uint32_t at =
-                            (b * W + (((uint32_t)emp >> 1) ^ (b & (W - 1)))) * 2 + (emp & 1);
-                        const unsigned long long prev =
-                            atomicCAS(&keys[at], EMPTY, (unsigned long long)key);
-                        if (prev == EMPTY)
+                        for (int q = 0; q < (int)W; ++q)
                         {
-                            mine += 1;
-                            disp = max(disp, b - home);
-                            done = true;
+                            const ulonglong2 x = kb[q ^ (b & (W - 1))];
+                            kk[2 * q] = x.x;
+                            kk[2 * q + 1] = x.y;
                         }
-                        else if (prev == key)
+                        // keys take the first EMPTY slot and are never
+                        // removed: a stored key sits before the first EMPTY one
+                        int hit = S, emp = S;
+#pragma unroll
+                        for (int q = S - 1; q >= 0; --q)
+                        {
+                            hit = kk[q] == key ? q : hit;
+                            emp = kk[q] == EMPTY ? q : emp;
+                        }
+                        if (hit < emp)
                             done = true;
-                        // else another key took the slot: this bucket is read again
+                        else if (emp < S)
+                        {
+                            const unsigned long long prev = atomicCAS(
+                                &keys[at(b * S + (uint32_t)emp)], EMPTY, (unsigned long long)key);
+                            if (prev == EMPTY)
+                            {
+                                mine += 1;
+                                disp = max(disp, b - home);
+                                done = true;
+                            }
+                            else if (prev == key)
+                                done = true;
+                            // else another key took the slot: this bucket is read again
+                        }
+                        else
+                            ++b;
                     }
-                    else
-                        ++b;
-                }
-                if (!done)
-                {
-                    const unsigned long long at = atomicAdd(spillCount, 1ull);
-                    if (at < spillCap)
-                        spill[at] = key;
+                    if (!done)
+                    {
+                        const unsigned long long k = atomicAdd(spillCount, 1ull);
+                        if (k < spillCap)
+                            spill[k] = key;
+                    }
                 }
             }
         }
@@ -1231,6 +1402,13 @@ __global__ __launch_bounds__(256) void tableSliceKernel(const uint64_t *__restri
             const uint32_t b = q / W;
             dst[q] = src[b * W + ((q % W) ^ (b & (W - 1)))];
         }
+        // the next slice fills LDS once every thread has read it
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSliceKeys; ++r)
+            kv[r] = kn[r];
+        lo = loN;
+        hi = hiN;
     }
     if (mine)
         atomicAdd(&stored, mine);
@@ -2572,11 +2750,21 @@ int buildTableSorted(hipibs_table *t, const uint64_t *dKeys, uint64_t n, hipStre
         if (!rc)
         {
             auto *ctr = reinterpret_cast<unsigned long long *>(t->dCounters);
+            // as many blocks as are resident at once (each loops over slices)
+            int perCu = 0;
             if (S == 16)
-                tableSliceKernel<16, 8><<<(uint32_t)nSlices, 256, 0, s>>>(
+                (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &perCu, tableSliceKernel<16, 8>, 256, 0);
+            else
+                (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &perCu, tableSliceKernel<8, 9>, 256, 0);
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(
+                nSlices, (uint64_t)cuCount(t->device) * (uint64_t)std::max(1, perCu));
+            if (S == 16)
+                tableSliceKernel<16, 8><<<grid, 256, 0, s>>>(
                     sorted, so->dStart, nSlices, t->dSlots, nB, spill, spillCap, dCnt + 1, ctr);
             else
-                tableSliceKernel<8, 9><<<(uint32_t)nSlices, 256, 0, s>>>(
+                tableSliceKernel<8, 9><<<grid, 256, 0, s>>>(
                     sorted, so->dStart, nSlices, t->dSlots, nB, spill, spillCap, dCnt + 1, ctr);
             if (hipGetLastError() != hipSuccess)
                 rc = setError(HIPIBS_E_HIP, "table build: slice launch failed");
