@@ -245,6 +245,63 @@ def test_sorted_table_build_equals_insert(small_corpus, monkeypatch, slots, lf, 
     d.free()
 
 
+def home_buckets(keys, n_buckets):
+    """Home bucket of every key: the top 32 bits of the library's
+    multiply-xorshift mix scaled onto the bucket count (used here only to
+    choose keys that crowd one slice)."""
+    x = keys.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        h = (x ^ (x >> np.uint64(31))) * np.uint64(0x9E3779B97F4A7C15)
+        h ^= h >> np.uint64(29)
+        h = h * np.uint64(0xBF58476D1CE4E5B9)
+        return ((h >> np.uint64(32)) * np.uint64(n_buckets)) >> np.uint64(32)
+
+
+@pytest.mark.parametrize("slots", [16, 8])
+def test_sorted_table_build_distinct_and_crowded_slices(small_corpus, monkeypatch, slots):
+    """Distinct keys: every slice but one is filled by rank and run scan (no
+    repeated key sends it to the compare-and-swap walk); one slice is
+    crowded with 6,000 keys for its 4,096 slots, so it takes the walk and
+    its overflow is inserted afterwards. The sorted build holds the same
+    keys as the key-by-key insert, and a scan with either table gives the
+    oracle's rows."""
+    ref, samples = small_corpus
+    seq = corpus.to_text(ref)
+    mask = np.uint64((1 << 50) - 1)
+    base = np.unique(np.concatenate([samples[1][0], corpus.splitmix(slots, 41, 1_500_000) & mask]))
+    lf = 0.6
+    nb = max(2, -(-int(np.ceil(base.size / lf)) // slots))  # as the table sizes itself
+    per_slice = 256 if slots == 16 else 512
+    cand = np.setdiff1d(corpus.splitmix(slots, 43, 8_000_000) & mask, base)
+    crowd = cand[home_buckets(cand, nb) // per_slice == 3][:6000]
+    assert crowd.size == 6000
+    keys = np.concatenate([base[crowd.size:], crowd])  # same row count, all distinct
+    assert np.unique(keys).size == keys.size == base.size
+    other = corpus.splitmix(slots, 47, 200_000) & mask
+    want_other = int(np.isin(other, keys).sum())
+    d = D.from_array(keys)
+    note(f"sorted build, distinct keys, slots={slots}: {keys.size} keys, {nb} buckets, "
+         f"slice 3 holds {crowd.size}")
+    monkeypatch.setenv("HIPIBS_SORTED_BUILD_MIN", "1")
+    t_sort = hipibs.Table.from_device_keys(d.ptr, keys.size, 25, bucket_slots=slots,
+                                           load_factor=lf)
+    monkeypatch.setenv("HIPIBS_SORTED_BUILD_MIN", str(1 << 62))
+    t_ins = hipibs.Table.from_device_keys(d.ptr, keys.size, 25, bucket_slots=slots,
+                                          load_factor=lf)
+    monkeypatch.delenv("HIPIBS_SORTED_BUILD_MIN")
+    assert t_sort.info()["n_buckets"] == nb
+    want_rows = oracle_scan_rows([b"chr1"], [seq], 50000, 25, sorted_keys(keys))
+    for t in (t_sort, t_ins):
+        assert t.info()["n_keys"] == keys.size
+        assert t.probe_count(keys) == keys.size
+        assert t.probe_count(crowd) == crowd.size
+        assert t.probe_count(other) == want_other
+        assert hipibs.scan_sequences([t], [b"chr1"], [seq], 50000)[0] == want_rows
+    t_sort.free()
+    t_ins.free()
+    d.free()
+
+
 def test_coverage_matches_golden(small_corpus, expected):
     ref, samples = small_corpus
     seq = corpus.to_text(ref)
