@@ -685,56 +685,95 @@ static int allocSlots(uint64_t items, uint32_t slotsPerBucket, double loadFactor
     return 0;
 }
 
-// A minimizer table from n device-resident keys: count the keys of every
-// minimizer group, size both parts from the counts, then place the keys.
-static int buildMinimizerTable(const uint64_t *dKeys, uint64_t n, uint32_t k,
-                               const hipibs_table_opts *opts, hipStream_t s, hipibs_table **out)
+// A minimizer table's build in three steps: begin() sizes and clears the
+// group map for n keys; count() adds keys to their minimizer groups' counts
+// (any number of calls, e.g. one per chunk while the rest of the keys are
+// still being read); finish() sizes both parts from the counts and places
+// all n keys, which must then be resident together.
+struct MinimizerBuild
+{
+    hipibs_table_opts o{};
+    uint32_t k = 0, mlen = 0, cap = 0;
+    uint64_t mapSize = 0;
+    unsigned long long *map = nullptr, *tally = nullptr;
+
+    int begin(uint64_t n, uint32_t kk, const hipibs_table_opts *opts, hipStream_t s)
+    {
+        if (opts)
+            o = *opts;
+        if (o.load_factor <= 0)
+            o.load_factor = 0.6;
+        k = kk;
+        mlen = o.minimizer_len;
+        if (k == 0 || k > 32)
+            return setError(HIPIBS_E_ARG, "k must be in 1..32");
+        if (mlen < 5 || mlen > 31 || mlen > k)
+            return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (off) or 5..min(31, k)");
+        if (o.bucket_slots == 0)
+            o.bucket_slots = 16;
+        if (o.bucket_slots != 8 && o.bucket_slots != 16)
+            return setError(HIPIBS_E_ARG, "bucket_slots must be 8 or 16");
+        if (!(o.load_factor > 0 && o.load_factor < 0.98))
+            return setError(HIPIBS_E_ARG, "load_factor must be in (0, 0.98)");
+        cap = o.minimizer_cap ? o.minimizer_cap : kDefaultMinimizerCap;
+        if (cap > 4096)
+            return setError(HIPIBS_E_ARG, "minimizer_cap must be at most 4096");
+        // a group map entry holds its group's key count in bits 32..62 (bit
+        // 63 is the placement's marker flag): with fewer than 2^32 keys a
+        // count never wraps, and one that reaches 2^31 shows as bit 63 in
+        // the tally
+        if (n >= (1ull << 32))
+            return setError(HIPIBS_E_ARG, "minimizer tables hold fewer than 2^32 keys");
+        HIPIBS_CHECK(hipSetDevice(o.device));
+        // group map: at most one group per key, so a map of >= n entries
+        // always has room
+        mapSize = pow2AtLeast(std::max<uint64_t>(1024, n));
+        if (hipMalloc(reinterpret_cast<void **>(&map), mapSize * 8) != hipSuccess)
+            return setError(HIPIBS_E_NOMEM, "hipMalloc of the minimizer group map failed");
+        if (hipMalloc(reinterpret_cast<void **>(&tally), 5 * 8) != hipSuccess)
+        {
+            release();
+            return setError(HIPIBS_E_NOMEM, "hipMalloc of the group tally failed");
+        }
+        hipLaunchKernelGGL(fillKernel, dim3(gridFor(mapSize, 256, o.device)), dim3(256), 0, s,
+                           reinterpret_cast<uint64_t *>(map), mapSize, EMPTY);
+        hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, s,
+                           reinterpret_cast<uint64_t *>(tally), (uint64_t)5, 0ull);
+        if (hipGetLastError() != hipSuccess)
+        {
+            release();
+            return setError(HIPIBS_E_HIP, "minimizer group map clear failed");
+        }
+        return 0;
+    }
+
+    int count(const uint64_t *dKeys, uint64_t m, hipStream_t s)
+    {
+        if (m)
+            hipLaunchKernelGGL(groupCountKernel, dim3(gridFor(m, 256, o.device)), dim3(256), 0, s,
+                               dKeys, m, k, mlen, map, mapSize - 1, tally + 3);
+        return hipGetLastError() == hipSuccess
+                   ? 0
+                   : setError(HIPIBS_E_HIP, "minimizer group count launch failed");
+    }
+
+    // Frees the map and the tally once the stream is done with them.
+    void release()
+    {
+        if (map)
+            hipFree(map);
+        if (tally)
+            hipFree(tally);
+        map = tally = nullptr;
+    }
+
+    int finish(const uint64_t *dKeys, uint64_t n, hipStream_t s, hipibs_table **out);
+};
+
+int MinimizerBuild::finish(const uint64_t *dKeys, uint64_t n, hipStream_t s, hipibs_table **out)
 {
     *out = nullptr;
-    hipibs_table_opts o{};
-    if (opts)
-        o = *opts;
-    if (o.load_factor <= 0)
-        o.load_factor = 0.6;
-    const uint32_t mlen = o.minimizer_len;
-    if (k == 0 || k > 32)
-        return setError(HIPIBS_E_ARG, "k must be in 1..32");
-    if (mlen < 5 || mlen > 31 || mlen > k)
-        return setError(HIPIBS_E_ARG, "minimizer_len must be 0 (off) or 5..min(31, k)");
-    if (o.bucket_slots == 0)
-        o.bucket_slots = 16;
-    if (o.bucket_slots != 8 && o.bucket_slots != 16)
-        return setError(HIPIBS_E_ARG, "bucket_slots must be 8 or 16");
-    if (!(o.load_factor > 0 && o.load_factor < 0.98))
-        return setError(HIPIBS_E_ARG, "load_factor must be in (0, 0.98)");
-    const uint32_t cap = o.minimizer_cap ? o.minimizer_cap : kDefaultMinimizerCap;
-    if (cap > 4096)
-        return setError(HIPIBS_E_ARG, "minimizer_cap must be at most 4096");
-    // a group map entry holds its group's key count in bits 32..62 (bit 63
-    // is the placement's marker flag): with fewer than 2^32 keys a count
-    // never wraps, and one that reaches 2^31 shows as bit 63 in the tally
-    if (n >= (1ull << 32))
-        return setError(HIPIBS_E_ARG, "minimizer tables hold fewer than 2^32 keys");
-    HIPIBS_CHECK(hipSetDevice(o.device));
-    // group map: at most one group per key, so a map of >= n entries always
-    // has room
-    const uint64_t mapSize = pow2AtLeast(std::max<uint64_t>(1024, n));
-    unsigned long long *map = nullptr, *tally = nullptr;
-    if (hipMalloc(reinterpret_cast<void **>(&map), mapSize * 8) != hipSuccess)
-        return setError(HIPIBS_E_NOMEM, "hipMalloc of the minimizer group map failed");
-    if (hipMalloc(reinterpret_cast<void **>(&tally), 5 * 8) != hipSuccess)
-    {
-        hipFree(map);
-        return setError(HIPIBS_E_NOMEM, "hipMalloc of the group tally failed");
-    }
-    int dev = o.device;
-    hipLaunchKernelGGL(fillKernel, dim3(gridFor(mapSize, 256, dev)), dim3(256), 0, s,
-                       reinterpret_cast<uint64_t *>(map), mapSize, EMPTY);
-    hipLaunchKernelGGL(fillKernel, dim3(1), dim3(64), 0, s, reinterpret_cast<uint64_t *>(tally),
-                       (uint64_t)5, 0ull);
-    if (n)
-        hipLaunchKernelGGL(groupCountKernel, dim3(gridFor(n, 256, dev)), dim3(256), 0, s, dKeys, n,
-                           k, mlen, map, mapSize - 1, tally + 3);
+    const int dev = o.device;
     hipLaunchKernelGGL(groupTallyKernel, dim3(gridFor(mapSize, 256, dev)), dim3(256), 0, s, map,
                        mapSize, cap, tally);
     uint64_t tl[5] = {0, 0, 0, 0, 0};
@@ -793,8 +832,7 @@ static int buildMinimizerTable(const uint64_t *dKeys, uint64_t n, uint32_t k,
     if (!rc && c[1])
         rc = setError(HIPIBS_E_FULL, "table overflow: no free slot for a key");
     (void)hipStreamSynchronize(s);
-    hipFree(map);
-    hipFree(tally);
+    release();
     if (rc)
     {
         hipibs_table_free(t);
@@ -807,6 +845,25 @@ static int buildMinimizerTable(const uint64_t *dKeys, uint64_t n, uint32_t k,
     t->nMarkers = c[5];
     *out = t;
     return 0;
+}
+
+// A minimizer table from n device-resident keys: count the keys of every
+// minimizer group, size both parts from the counts, then place the keys.
+static int buildMinimizerTable(const uint64_t *dKeys, uint64_t n, uint32_t k,
+                               const hipibs_table_opts *opts, hipStream_t s, hipibs_table **out)
+{
+    *out = nullptr;
+    MinimizerBuild mb;
+    int rc = mb.begin(n, k, opts, s);
+    if (!rc)
+        rc = mb.count(dKeys, n, s);
+    if (rc)
+    {
+        (void)hipStreamSynchronize(s);
+        mb.release();
+        return rc;
+    }
+    return mb.finish(dKeys, n, s, out);
 }
 
 } // namespace hipibs
@@ -1308,19 +1365,44 @@ extern "C" int hipibs_table_build_kmc(const char *prefix, const hipibs_table_opt
             hipFree(dAll);
             return setError(HIPIBS_E_HIP, "stream creation failed");
         }
+        // the groups are counted chunk by chunk while the records are read,
+        // on a stream of their own so that the next chunk's upload does not
+        // wait for them; the placement needs them all
+        MinimizerBuild mb;
+        hipStream_t cs = nullptr;
+        hipEvent_t copied = nullptr;
+        rc = mb.begin(n, src.info.kmer_length, opts, s);
+        if (!rc && (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess ||
+                    hipEventCreateWithFlags(&copied, hipEventDisableTiming) != hipSuccess))
+            rc = setError(HIPIBS_E_HIP, "stream creation failed");
         uint64_t off = 0;
-        rc = streamKmcKeys(src, opts->device, opts->load_threads, s,
-                           [&](const uint64_t *dKeys, uint64_t m)
-                           {
-                               if (hipMemcpyAsync(dAll + off, dKeys, m * 8,
-                                                  hipMemcpyDeviceToDevice, s) != hipSuccess)
-                                   return setError(HIPIBS_E_HIP, "key copy failed");
-                               off += m;
-                               return 0;
-                           });
         if (!rc)
-            rc = buildMinimizerTable(dAll, n, src.info.kmer_length, opts, s, &t);
+            rc = streamKmcKeys(src, opts->device, opts->load_threads, s,
+                               [&](const uint64_t *dKeys, uint64_t m)
+                               {
+                                   if (hipMemcpyAsync(dAll + off, dKeys, m * 8,
+                                                      hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                                       hipEventRecord(copied, s) != hipSuccess ||
+                                       hipStreamWaitEvent(cs, copied, 0) != hipSuccess)
+                                       return setError(HIPIBS_E_HIP, "key copy failed");
+                                   off += m;
+                                   return mb.count(dAll + off - m, m, cs);
+                               });
+        // the placement (on s) after every count
+        if (!rc && (hipEventRecord(copied, cs) != hipSuccess ||
+                    hipStreamWaitEvent(s, copied, 0) != hipSuccess))
+            rc = setError(HIPIBS_E_HIP, "stream ordering failed");
+        if (!rc)
+            rc = mb.finish(dAll, n, s, &t);
+        if (cs)
+        {
+            (void)hipStreamSynchronize(cs);
+            (void)hipStreamDestroy(cs);
+        }
+        if (copied)
+            (void)hipEventDestroy(copied);
         (void)hipStreamSynchronize(s);
+        mb.release();
         hipStreamDestroy(s);
         hipFree(dAll);
     }
