@@ -16,8 +16,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <sys/mman.h>
+
 #include <cstdint>
+#include <cstdlib>
 #include <memory>
+#include <new>
 #include <string>
 #include <utility>
 #include <vector>
@@ -414,6 +418,29 @@ struct NoInitAlloc : std::allocator<T>
     template <class U>
     NoInitAlloc(const NoInitAlloc<U> &) noexcept
     {
+    }
+    // Arrays of 64 MiB or more are 2 MiB aligned and asked for huge pages:
+    // the threads that fill them from the page cache then take one fault
+    // per 2 MiB instead of one per 4 KiB (a 1 GB KMC LUT reads about twice
+    // as fast).
+    static constexpr size_t kHuge = size_t(64) << 20;
+    T *allocate(size_t n)
+    {
+        const size_t bytes = n * sizeof(T);
+        if (bytes < kHuge)
+            return std::allocator<T>::allocate(n);
+        void *p = nullptr;
+        if (posix_memalign(&p, size_t(2) << 20, bytes) != 0)
+            throw std::bad_alloc();
+        (void)madvise(p, bytes, MADV_HUGEPAGE);
+        return static_cast<T *>(p);
+    }
+    void deallocate(T *p, size_t n)
+    {
+        if (n * sizeof(T) < kHuge)
+            std::allocator<T>::deallocate(p, n);
+        else
+            std::free(p);
     }
     template <class U>
     void construct(U *p) noexcept
