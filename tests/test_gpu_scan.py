@@ -527,22 +527,33 @@ def test_build_from_kmc_decodes_every_layout(tmp_path, variant):
     t.free()
 
 
-def test_build_from_kmc_across_chunks(tmp_path):
+@pytest.mark.parametrize("kind", ["whole", "minimizer"])
+def test_build_from_kmc_across_chunks(tmp_path, kind):
     """More records than one staging chunk (32M): the decode of every chunk
-    starts at its own record and bucket range."""
+    starts at its own record and bucket range; a minimizer table counts its
+    groups chunk by chunk while the records stream, and must come out the
+    same size as the one built from the same keys on the host (the sizes
+    follow from the group counts)."""
     k = 31
     keys = np.unique(corpus.splitmix(31, 5, 40_000_000) & np.uint64((1 << 62) - 1))
     keys = np.unique(np.minimum(keys, corpus.revcomp(keys, k)))
-    note(f"kmc across chunks: {keys.size} keys")
+    note(f"kmc across chunks ({kind}): {keys.size} keys")
     pre = str(tmp_path / "big")
     hipibs.kmc_write(pre, keys, k, p=7, sig_len=9, counter_size=1, version=0x200,
                      min_count=1, max_count=255)
-    t = hipibs.Table.from_kmc(pre)
+    opts = dict(minimizer_len=27, minimizer_cap=16, load_factor=0.2) if kind == "minimizer" else {}
+    t = hipibs.Table.from_kmc(pre, **opts)
     assert t.info()["n_keys"] == keys.size
     assert t.probe_count(keys) == keys.size
     probe = keys[::97] ^ np.uint64(1)
     assert t.probe_count(probe) == int(np.isin(probe, keys).sum())
     assert t.probe_count_kmc(pre) == keys.size
+    if kind == "minimizer":
+        assert t.info()["minimizer_len"] == 27
+        t_host = hipibs.Table.from_keys(keys, k, **opts)
+        assert t_host.info()["bytes"] == t.info()["bytes"]
+        assert t_host.info()["n_keys"] == keys.size
+        t_host.free()
     t.free()
 
 
