@@ -1933,6 +1933,54 @@ void returnHash(int device, uint64_t *p[2], uint64_t cap)
     gHashCache.cap = cap;
 }
 
+// A freed counter's table is kept for the process's next counter: the
+// largest one so far (hipibs_counter_release_cache frees it). A counter
+// allocates its table once it knows its size, and a new allocation of
+// tens of GB that reuses memory the process freed can cost the runtime a
+// second or more.
+struct CountTableCache
+{
+    std::mutex m;
+    int device = -1;
+    uint64_t *p = nullptr;
+    uint64_t bytes = 0;
+} gCountTable;
+
+bool takeCountTable(int device, uint64_t need, uint64_t **out, uint64_t *bytes)
+{
+    std::lock_guard<std::mutex> g(gCountTable.m);
+    if (!gCountTable.p || gCountTable.device != device || gCountTable.bytes < need)
+        return false;
+    *out = gCountTable.p;
+    *bytes = gCountTable.bytes;
+    gCountTable.p = nullptr;
+    gCountTable.bytes = 0;
+    return true;
+}
+
+// Keeps p if it is larger than the kept table (the smaller is freed);
+// called with p's device current and its stream idle.
+void returnCountTable(int device, uint64_t *p, uint64_t bytes)
+{
+    std::lock_guard<std::mutex> g(gCountTable.m);
+    if (gCountTable.p && gCountTable.bytes >= bytes)
+    {
+        (void)hipFree(p);
+        return;
+    }
+    if (gCountTable.p)
+    {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(gCountTable.device);
+        (void)hipFree(gCountTable.p);
+        (void)hipSetDevice(cur);
+    }
+    gCountTable.device = device;
+    gCountTable.p = p;
+    gCountTable.bytes = bytes;
+}
+
 // The sorter of table builds is kept for the process's next build (its
 // per-tile buffers grow with the largest build), one per process.
 struct BuildSorterCache
@@ -2011,6 +2059,19 @@ void returnScratch(int device, uint64_t *p, uint64_t cap)
 void releaseBuildSorter()
 {
     {
+        std::lock_guard<std::mutex> g(gCountTable.m);
+        if (gCountTable.p)
+        {
+            int cur = 0;
+            (void)hipGetDevice(&cur);
+            (void)hipSetDevice(gCountTable.device);
+            (void)hipFree(gCountTable.p);
+            (void)hipSetDevice(cur);
+            gCountTable.p = nullptr;
+            gCountTable.bytes = 0;
+        }
+    }
+    {
         std::lock_guard<std::mutex> g(gScratch.m);
         if (gScratch.p)
         {
@@ -2044,6 +2105,7 @@ struct hipibs_counter : ReadSink
     uint64_t chunkBytes = 0;
     hipStream_t stream = nullptr;
     uint64_t *dTab = nullptr;
+    uint64_t tabBytes = 0; // dTab's allocation (a kept table may be larger)
     uint64_t nBuckets = 0;
     unsigned long long *dCounters = nullptr; // [0] distinct, [1] counted, [2] full, [3..5] filter,
                                              // [6] count of A^32 T^32 (k = 64), [7] spilled
@@ -2484,13 +2546,18 @@ struct hipibs_counter : ReadSink
         if (want >= (1ull << 32))
             return setError(HIPIBS_E_NOMEM, "k-mer counter: more than 2^32 buckets needed");
         uint64_t *t = nullptr;
-        const hipError_t e = hipMalloc(reinterpret_cast<void **>(&t), want * kCountWords * 8);
-        if (e != hipSuccess)
+        uint64_t tBytes = want * kCountWords * 8;
+        if (!takeCountTable(device, tBytes, &t, &tBytes))
         {
-            (void)hipGetLastError();
-            return setError(HIPIBS_E_NOMEM,
-                            "k-mer counter: no device memory for " + std::to_string(want) +
-                                " buckets (" + std::to_string(distinct) + " distinct k-mers so far)");
+            const hipError_t e = hipMalloc(reinterpret_cast<void **>(&t), tBytes);
+            if (e != hipSuccess)
+            {
+                (void)hipGetLastError();
+                return setError(HIPIBS_E_NOMEM, "k-mer counter: no device memory for " +
+                                                    std::to_string(want) + " buckets (" +
+                                                    std::to_string(distinct) +
+                                                    " distinct k-mers so far)");
+            }
         }
         const uint64_t words = want * kCountWords;
         // a new table with no keys to move in may stay unfilled when the
@@ -2512,9 +2579,10 @@ struct hipibs_counter : ReadSink
                                            stream>>>(dTab, nBuckets, t, want, dCounters);
             HIPIBS_CHECK(hipGetLastError());
             HIPIBS_CHECK(hipStreamSynchronize(stream));
-            HIPIBS_CHECK(hipFree(dTab));
+            returnCountTable(device, dTab, tabBytes);
         }
         dTab = t;
+        tabBytes = tBytes;
         nBuckets = want;
         return 0;
     }
@@ -3062,7 +3130,7 @@ extern "C" void hipibs_counter_free(hipibs_counter *c)
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
     if (c->dTab)
-        (void)hipFree(c->dTab);
+        returnCountTable(c->device, c->dTab, c->tabBytes);
     if (c->dCounters)
         (void)hipFree(c->dCounters);
     if (c->dText)

@@ -289,15 +289,15 @@ int hipibs_counter_build_table(hipibs_counter *c, uint32_t min_count, uint32_t m
  * KmcRaIndex and hipibs_table_build_kmc. filt may be NULL. */
 int hipibs_counter_write_kmc(hipibs_counter *c, const char *prefix, uint32_t min_count,
                              uint32_t max_count, uint32_t counter_max, hipibs_count_filter *filt);
-/* Frees the counter. For k <= 32 its two buffers of hashed k-mers (8 B per
- * k-mer of one counting round, up to 16 chunks' worth) are kept for the
- * process's next counter on the same device: the largest pair freed so far,
- * one pair per process. */
+/* Frees the counter. Its table, and for k <= 32 its two buffers of hashed
+ * k-mers (8 B per k-mer of one counting round, up to 16 chunks' worth), are
+ * kept for the process's next counter on the same device: the largest table
+ * and the largest pair freed so far, one of each per process. */
 void hipibs_counter_free(hipibs_counter *c);
 /* Frees the buffers kept for the process's next counter or table build:
- * the pair of hashed k-mer buffers, a table build's sort buffer and sort
- * state (no counter needs to exist). A table allocation that fails frees
- * them and tries once more. */
+ * a counter's table, the pair of hashed k-mer buffers, a table build's
+ * sort buffer and sort state (no counter needs to exist). A table
+ * allocation that fails frees them and tries once more. */
 void hipibs_counter_release_cache(void);
 
 /* ---- the fused scan (device-resident inputs) ------------------------------ */
