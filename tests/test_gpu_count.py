@@ -294,9 +294,10 @@ def test_repetitive_reads_fill_few_digits(monkeypatch):
 
 def test_hash_buffers_reused_across_counters(small_reads):
     """Counters alive together each keep their own buffers; a freed
-    counter's buffers serve the next counter, larger or smaller, and
-    hipibs_counter_release_cache frees them between counters: every count
-    equals the oracle's."""
+    counter's hash buffers and table serve the next counter, larger or
+    smaller (a kept table still holds the last counter's keys and counts),
+    and hipibs_counter_release_cache frees them between counters: every
+    count equals the oracle's."""
     _, reads = small_reads
     sets = [reads[0][:6000], reads[1][:1500], reads[2][:9000], reads[0][:500]]
     texts = [b"\n".join(corpus.to_text(x) for x in r) for r in sets]
@@ -319,6 +320,10 @@ def test_hash_buffers_reused_across_counters(small_reads):
     check(c, 2)
     b.free()
     c.free()
+    e = hipibs.KmerCounter(25)  # the largest kept table, full of c's counts
+    e.add_text(texts[1])
+    check(e, 1)
+    e.free()
     hipibs._lib.hipibs_counter_release_cache()
     d = hipibs.KmerCounter(25)
     d.add_text(texts[3])
