@@ -73,13 +73,13 @@ template <> struct Layout<true>
 
 // ACGT / acgt -> 0..3, any other byte 4 (the codec's base table,
 // src/KmerCodec.hpp:105-118). The low two bits of (b >> 1) ^ (b >> 2) are
-// 0, 1, 2, 3 for A, C, G, T in either case; kValid marks the eight letters
-// among the bytes 0x40..0x7f.
+// 0, 1, 2, 3 for A, C, G, T in either case; among the bytes 0x40..0x7f,
+// kValid marks their positions b & 31 (1, 3, 7, 20), shared by both cases.
+// No branch: every lane of a wave takes the same path.
 __device__ inline uint32_t baseCode(uint32_t b)
 {
-    constexpr uint64_t kValid = (1ull << 1) | (1ull << 3) | (1ull << 7) | (1ull << 20) |
-                                (1ull << 33) | (1ull << 35) | (1ull << 39) | (1ull << 52);
-    const bool ok = (b >> 6) == 1u && ((kValid >> (b & 63u)) & 1u);
+    constexpr uint32_t kValid = (1u << 1) | (1u << 3) | (1u << 7) | (1u << 20);
+    const uint32_t ok = (uint32_t)((b & 0xC0u) == 0x40u) & ((kValid >> (b & 31u)) & 1u);
     return ok ? (((b >> 1) ^ (b >> 2)) & 3u) : 4u;
 }
 
