@@ -2579,7 +2579,8 @@ struct hipibs_counter : ReadSink
                                            stream>>>(dTab, nBuckets, t, want, dCounters);
             HIPIBS_CHECK(hipGetLastError());
             HIPIBS_CHECK(hipStreamSynchronize(stream));
-            returnCountTable(device, dTab, tabBytes);
+            // freed at once: a growing counter may need the memory next
+            HIPIBS_CHECK(hipFree(dTab));
         }
         dTab = t;
         tabBytes = tBytes;
