@@ -2549,7 +2549,15 @@ struct hipibs_counter : ReadSink
         uint64_t tBytes = want * kCountWords * 8;
         if (!takeCountTable(device, tBytes, &t, &tBytes))
         {
-            const hipError_t e = hipMalloc(reinterpret_cast<void **>(&t), tBytes);
+            hipError_t e = hipMalloc(reinterpret_cast<void **>(&t), tBytes);
+            if (e != hipSuccess)
+            {
+                // the buffers kept for the next counter or build may be in
+                // the way (a kept table too small for this one among them)
+                (void)hipGetLastError();
+                releaseBuildSorter();
+                e = hipMalloc(reinterpret_cast<void **>(&t), tBytes);
+            }
             if (e != hipSuccess)
             {
                 (void)hipGetLastError();
