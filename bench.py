@@ -107,6 +107,9 @@ def parse():
     ap.add_argument("--e2e", type=int, default=1,
                     help="scan, large: also run the tools end to end from files (scan: "
                          "fastibs, and the reference's CPU whole scan of the same files)")
+    ap.add_argument("--keep-e2e", type=int, default=0,
+                    help="scan: keep the end-to-end FASTA and KMC2 files (in "
+                         "<workdir>/hipibs_bench_e2e: refs/, c2_sample/) for a later run")
     ap.add_argument("--panel-samples", type=int, default=48)
     ap.add_argument("--panel-min-keys", type=float, default=0.05e9)
     ap.add_argument("--panel-max-keys", type=float, default=0.625e9)
@@ -422,7 +425,10 @@ def end_to_end_and_cpu(torch, ref_codes, host_keys, wins, st, args, n):
                "whole_scan_s": med("whole_scan_s"), "probe_phase_s": med("probe_stats_s"),
                "index_load_s": med("index_load_s"), "fasta_read_s": med("fasta_read_s"),
                "passes": passes, "tsv_identical_to_gpu": same, "host": host_info()}
-    shutil.rmtree(root, ignore_errors=True)
+    if args.keep_e2e:
+        log(f"end-to-end files kept in {root}")
+    else:
+        shutil.rmtree(root, ignore_errors=True)
     return e2e, cpu, (tsvs.get("default_table") == want_tsv if tsvs else None)
 
 
