@@ -1088,37 +1088,9 @@ static int streamKmcKeys(const KmcRecords &src, int device, unsigned threads, hi
     if (!rc && (hipMalloc(reinterpret_cast<void **>(&dKeys), chunk * (wide ? 16 : 8)) != hipSuccess ||
                 hipMalloc(reinterpret_cast<void **>(&dLut), src.lut.size() * 8) != hipSuccess))
         rc = setError(HIPIBS_E_NOMEM, "device allocation for the KMC decode failed");
-    // the LUT goes up through the two pinned staging buffers before they
-    // take records: pieces copied on host threads, each piece's DMA
-    // overlapping the next piece's copy (a copy from pageable memory would
-    // be staged by one thread)
-    bool used[2] = {false, false};
-    const uint64_t lutBytes = src.lut.size() * 8;
-    if (!rc && rawBytes < (16ull << 20)) // small staging buffers: one copy
-    {
-        if (hipMemcpyAsync(dLut, src.lut.data(), lutBytes, hipMemcpyHostToDevice, s) !=
-            hipSuccess)
-            rc = setError(HIPIBS_E_HIP, "LUT upload failed");
-    }
-    else
-    {
-        for (uint64_t off = 0, it = 0; off < lutBytes && !rc; off += rawBytes, ++it)
-        {
-            const int b = (int)(it & 1);
-            const uint64_t m = std::min(rawBytes, lutBytes - off);
-            if (used[b] && hipEventSynchronize(copied[b]) != hipSuccess)
-            {
-                rc = setError(HIPIBS_E_HIP, "event wait failed");
-                break;
-            }
-            parallelCopy(pinned[b], reinterpret_cast<const uint8_t *>(src.lut.data()) + off, m);
-            if (hipMemcpyAsync(reinterpret_cast<uint8_t *>(dLut) + off, pinned[b], m,
-                               hipMemcpyHostToDevice, s) != hipSuccess ||
-                hipEventRecord(copied[b], s) != hipSuccess)
-                rc = setError(HIPIBS_E_HIP, "LUT upload failed");
-            used[b] = true;
-        }
-    }
+    if (!rc && hipMemcpyAsync(dLut, src.lut.data(), src.lut.size() * 8, hipMemcpyHostToDevice,
+                              s) != hipSuccess)
+        rc = setError(HIPIBS_E_HIP, "LUT upload failed");
     auto bucketOfRecord = [&](uint64_t r)
     {
         const uint64_t u = (uint64_t)(std::upper_bound(src.lut.begin(), src.lut.begin() + nBuckets,
@@ -1126,6 +1098,7 @@ static int streamKmcKeys(const KmcRecords &src, int device, unsigned threads, hi
                                       src.lut.begin());
         return u ? u - 1 : 0;
     };
+    bool used[2] = {false, false};
     for (uint64_t r0 = 0, it = 0; r0 < total && !rc; r0 += chunk, ++it)
     {
         const int b = (int)(it & 1);
